@@ -1,0 +1,111 @@
+"""ORACLE (test infrastructure): ctypes binding of oracle/liboracle.so."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO):
+            raise RuntimeError(f"{SO} not built (make -C oracle)")
+        _lib = C.CDLL(SO)
+        p, u = C.c_void_p, C.c_uint64
+        _lib.xko_resolve.argtypes = [u, p, p, p, p, u, p, p]
+        _lib.xko_resolve.restype = C.c_int64
+        _lib.xko_coherency.argtypes = [u, p, p, p, p, p, u, p, p, u, p, p, p]
+        _lib.xko_coherency.restype = C.c_int64
+        _lib.xko_partition.argtypes = [u, p, p, p, u, p, p]
+        _lib.xko_partition.restype = C.c_int64
+    return _lib
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dt)
+
+
+def resolve(task, mode, rects, nrect, want_frags=False):
+    """(pred, succ) access-index edges, grouped by predecessor (reference order)."""
+    L = lib()
+    n = len(task)
+    task = _c(task, np.uint32); mode = _c(mode, np.uint8)
+    rects = _c(rects, np.uint64); nrect = _c(nrect, np.uint8)
+    frag = np.zeros(max(n, 1), np.uint64) if want_frags else None
+    cap = max(1024, 4 * n)
+    while True:
+        edges = np.empty((cap, 2), np.uint32)
+        ne = L.xko_resolve(n, task.ctypes.data, mode.ctypes.data, nrect.ctypes.data, rects.ctypes.data,
+                           cap, edges.ctypes.data, frag.ctypes.data if frag is not None else None)
+        if ne <= cap:
+            break
+        cap = int(ne)
+    if want_frags:
+        return edges[:ne].copy(), frag[:n]
+    return edges[:ne].copy()
+
+
+def partition(dev, rects, nrect):
+    L = lib()
+    n = len(nrect)
+    rects = _c(rects, np.uint64); nrect = _c(nrect, np.uint8)
+    dev = _c(dev, np.uint8) if dev is not None else None
+    off = np.zeros(n + 1, np.uint64)
+    cap = max(1024, 8 * n)
+    while True:
+        parts = np.empty((cap, 4), np.uint64)
+        np_ = L.xko_partition(n, dev.ctypes.data if dev is not None else None, nrect.ctypes.data,
+                              rects.ctypes.data, cap, off.ctypes.data, parts.ctypes.data)
+        if np_ <= cap:
+            break
+        cap = int(np_)
+    return off.astype(np.int64), parts[:np_].copy()
+
+
+class CoherencyPlan:
+    """Per-access valid-region split and fetch list of one coherency tree."""
+
+    def __init__(self, part_off, parts, fetch_off, fetches, fail_at):
+        self.part_off = part_off
+        self.parts = parts          # (P, 6) x0 x1 y0 y1 coh_before fetch_before
+        self.fetch_off = fetch_off
+        self.fetches = fetches      # (F, 8) x0 x1 y0 y1 src_dev src_chunk dst_dev dst_chunk
+        self.fail_at = fail_at      # index of the access that would abort, or -1
+
+    def parts_of(self, i):
+        return self.parts[self.part_off[i]:self.part_off[i + 1]]
+
+    def fetches_of(self, i):
+        return self.fetches[self.fetch_off[i]:self.fetch_off[i + 1]]
+
+
+def coherency(dev, mode, rects, nrect, chunk_bytes):
+    L = lib()
+    n = len(dev)
+    dev = _c(dev, np.uint8); mode = _c(mode, np.uint8)
+    rects = _c(rects, np.uint64); nrect = _c(nrect, np.uint8)
+    cb = _c(chunk_bytes, np.uint64)
+    po = np.zeros(n + 1, np.int64); fo = np.zeros(n + 1, np.int64)
+    fail = np.zeros(1, np.int64)
+    pc, fc = max(1024, 8 * n), max(1024, 8 * n)
+    while True:
+        parts = np.empty((pc, 6), np.int64); fetches = np.empty((fc, 8), np.int64)
+        rc = L.xko_coherency(n, dev.ctypes.data, mode.ctypes.data, nrect.ctypes.data, rects.ctypes.data,
+                             cb.ctypes.data, pc, po.ctypes.data, parts.ctypes.data, fc, fo.ctypes.data,
+                             fetches.ctypes.data, fail.ctypes.data)
+        last = int(fail[0]) if rc < 0 else n
+        tp, tf = int(po[last]), int(fo[last])
+        if tp <= pc and tf <= fc:
+            break
+        pc, fc = max(pc, tp), max(fc, tf)
+    last = int(fail[0]) if rc < 0 else n
+    if rc < 0:
+        po[last + 1:] = po[last]
+        fo[last + 1:] = fo[last]
+    return CoherencyPlan(po, parts[:po[last]].copy(), fo, fetches[:fo[last]].copy(), int(fail[0]) if rc < 0 else -1)

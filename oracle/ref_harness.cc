@@ -1,0 +1,295 @@
+// Test infrastructure only: a driver around the reference's own KHP-tree
+// engine (include/xkrt/memory/access/common/khp-tree.hpp), compiled from the
+// reference sources by oracle/Makefile into oracle/_ref/libkhpref.so.
+//
+// The engine (insert / intersect / red-black k-subtrees) is the reference's
+// code, included unmodified.  The BLAS dependency-tree hooks and the edge
+// materialisation cannot be included: dependency-tree.hpp pulls task.hpp ->
+// logger.h -> the CMake-generated xkrt/support.h.  They are therefore restated
+// here, line for line in behaviour, from
+//   include/xkrt/memory/access/blas/dependency-tree.hpp:93-186  (node state)
+//   include/xkrt/memory/access/blas/dependency-tree.hpp:229-256 (on_insert)
+//   include/xkrt/memory/access/blas/dependency-tree.hpp:296-355 (stop test, on_intersect)
+//   include/xkrt/memory/access/blas/dependency-tree.hpp:367-414 (link / put)
+//   include/xkrt/task/task.hpp:494-512                         (__access_precedes dedup)
+//   src/task/task.cc:285-299                                   (LINK pass then PUT pass)
+// Tasks never complete in this harness, so every edge is materialised.
+//
+// The memory-tree partition helper inserts access rects and lists the nodes
+// that intersect them (memory-tree.hpp:1738-1775 steps (1)-(2), with the
+// host short path of 1183-1192: nothing is inserted while the tree is empty).
+
+#include <xkrt/memory/access/common/khp-tree.hpp>
+
+#include <cstdint>
+#include <ctime>
+#include <vector>
+
+namespace {
+
+enum : uint8_t { MODE_R = 1, MODE_W = 2 };
+
+struct Acc;
+using Rect = KHyperrect<2>;
+
+struct Acc {
+    uint32_t idx;
+    uint32_t task;
+    uint8_t mode;
+    int nrect;
+    Rect rects[2];
+    std::vector<Acc *> successors;
+};
+
+struct DepSearch {
+    Acc *access;
+};
+
+using DepBase = KHPTree<2, DepSearch>;
+
+class DepNode : public DepBase::Node {
+    using Base = DepBase::Node;
+  public:
+    std::vector<Acc *> last_reads;
+    Acc *last_write;
+    int nwrites;
+
+    DepNode(const Rect &h, int k, Color c) : Base(h, k, c), last_reads(), last_write(nullptr), nwrites(0) {}
+    DepNode(const Rect &h, int k, Color c, const DepNode *inherit)
+        : Base(h, k, c), last_reads(inherit->last_reads), last_write(inherit->last_write), nwrites(0) {}
+
+    void update_includes(void) override
+    {
+        Base::update_includes();
+        nwrites = last_write ? 1 : 0;
+        for (int k = 0; k < 2; ++k)
+            for (int d = 0; d < 2; ++d) {
+                DepNode *c = reinterpret_cast<DepNode *>(this->st[k].children[d]);
+                if (c) nwrites += c->nwrites;
+            }
+    }
+};
+
+static inline void access_precedes(Acc *pred, Acc *succ)
+{
+    if (!pred->successors.empty() && pred->successors.back()->task == succ->task)
+        return;
+    pred->successors.push_back(succ);
+}
+
+class DepTree : public DepBase {
+  public:
+    Node *new_node(DepSearch &, const Rect &h, const int k, const Color c) const override
+    {
+        return new DepNode(h, k, c);
+    }
+    Node *new_node(DepSearch &, const Rect &h, const int k, const Color c, const Node *inherit) const override
+    {
+        return new DepNode(h, k, c, reinterpret_cast<const DepNode *>(inherit));
+    }
+    void on_insert(Node *nb, DepSearch &s) override
+    {
+        DepNode *node = reinterpret_cast<DepNode *>(nb);
+        for (int r = 0; r < s.access->nrect; ++r) {
+            if (s.access->rects[r].intersects(node->hyperrect)) {
+                if (s.access->mode & MODE_W) {
+                    node->last_reads.clear();
+                    node->last_write = s.access;
+                } else if (s.access->mode == MODE_R) {
+                    node->last_reads.push_back(s.access);
+                }
+                break;
+            }
+        }
+    }
+    void on_shrink(Node *, const Interval &, int) override {}
+    bool intersect_stop_test(Node *nb, DepSearch &s, const Rect &) const override
+    {
+        return s.access->mode == MODE_R && reinterpret_cast<DepNode *>(nb)->nwrites == 0;
+    }
+    void on_intersect(Node *nb, DepSearch &s, const Rect &) const override
+    {
+        DepNode *node = reinterpret_cast<DepNode *>(nb);
+        if ((s.access->mode & MODE_W) && !node->last_reads.empty()) {
+            for (Acc *p : node->last_reads) access_precedes(p, s.access);
+        } else if (node->last_write) {
+            access_precedes(node->last_write, s.access);
+        }
+    }
+    void link(Acc *a)
+    {
+        DepSearch s{a};
+        for (int r = 0; r < a->nrect; ++r) intersect(s, a->rects[r]);
+    }
+    void put(Acc *a)
+    {
+        DepSearch s{a};
+        for (int r = 0; r < a->nrect; ++r) insert(s, a->rects[r]);
+    }
+};
+
+// Plain fragment tree used for the memory-tree partition (hooks are no-ops,
+// as memory-tree.hpp's INSERTING_BLOCKS on_insert is).
+struct PartSearch { std::vector<Rect> *out; };
+using PartBase = KHPTree<2, PartSearch>;
+class PartTree : public PartBase {
+  public:
+    Node *new_node(PartSearch &, const Rect &h, const int k, const Color c) const override { return new Node(h, k, c); }
+    Node *new_node(PartSearch &, const Rect &h, const int k, const Color c, const Node *) const override { return new Node(h, k, c); }
+    void on_insert(Node *, PartSearch &) override {}
+    void on_shrink(Node *, const Interval &, int) override {}
+    bool intersect_stop_test(Node *, PartSearch &, const Rect &) const override { return false; }
+    void on_intersect(Node *nb, PartSearch &s, const Rect &) const override { s.out->push_back(nb->hyperrect); }
+};
+
+static inline Rect make_rect(const uint64_t *r)
+{
+    Interval l[2] = {Interval(r[0], r[1]), Interval(r[2], r[3])};
+    return Rect(l);
+}
+
+static inline uint64_t now_ns(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (uint64_t)ts.tv_sec * 1000000000ull + (uint64_t)ts.tv_nsec;
+}
+
+static uint64_t count_frags(DepTree &t, const Acc &a)
+{
+    // Fragments are the tree's nodes; count those intersecting the access.
+    uint64_t n = 0;
+    std::vector<DepBase::Node *> stack;
+    if (t.root) stack.push_back(t.root);
+    while (!stack.empty()) {
+        DepBase::Node *node = stack.back();
+        stack.pop_back();
+        bool hit = false, sub = false;
+        for (int r = 0; r < a.nrect; ++r) {
+            if (a.rects[r].intersects(node->hyperrect)) hit = true;
+            if (a.rects[r].intersects(node->includes.hyperrect)) sub = true;
+        }
+        if (!sub) continue;
+        if (hit) n++;
+        for (int k = 0; k < 2; ++k)
+            for (int d = 0; d < 2; ++d)
+                if (node->st[k].children[d]) stack.push_back(node->st[k].children[d]);
+    }
+    return n;
+}
+
+} // namespace
+
+extern "C" {
+
+// Resolve one dependency domain.  Accesses are in spawn order, grouped by
+// task (task[] non-decreasing); rects holds 8 u64 per access (rect 0 then
+// rect 1), nrect 1 or 2.  Writes up to edge_cap (pred, succ) access-index
+// pairs, grouped by predecessor in access order then successor-list order.
+// Returns the number of edges (which may exceed edge_cap), or -1.
+// frag (nullable) receives per access the number of tree nodes intersecting
+// its rects after its task's PUT pass.  task_ns (nullable, one entry per
+// task) receives the wall time of each task's LINK + PUT.
+int64_t khpref_resolve(uint64_t nacc, const uint32_t *task, const uint8_t *mode, const uint8_t *nrect,
+                       const uint64_t *rects, uint64_t edge_cap, uint32_t *edges, uint64_t *frag,
+                       uint64_t *task_ns)
+{
+    std::vector<Acc> acc(nacc);
+    for (uint64_t i = 0; i < nacc; ++i) {
+        acc[i].idx = (uint32_t)i;
+        acc[i].task = task[i];
+        acc[i].mode = mode[i];
+        acc[i].nrect = nrect[i];
+        acc[i].rects[0] = make_rect(rects + 8 * i);
+        acc[i].rects[1] = make_rect(rects + 8 * i + 4);
+        acc[i].successors.reserve(8);
+    }
+    DepTree tree;
+    uint64_t i = 0, tix = 0;
+    while (i < nacc) {
+        uint64_t j = i;
+        while (j < nacc && task[j] == task[i]) ++j;
+        uint64_t t0 = task_ns ? now_ns() : 0;
+        for (uint64_t a = i; a < j; ++a) tree.link(&acc[a]);
+        for (uint64_t a = i; a < j; ++a) tree.put(&acc[a]);
+        if (task_ns) task_ns[tix] = now_ns() - t0;
+        if (frag)
+            for (uint64_t a = i; a < j; ++a) frag[a] = count_frags(tree, acc[a]);
+        i = j;
+        tix++;
+    }
+    uint64_t ne = 0;
+    for (uint64_t p = 0; p < nacc; ++p)
+        for (Acc *s : acc[p].successors) {
+            if (ne < edge_cap) {
+                edges[2 * ne] = (uint32_t)p;
+                edges[2 * ne + 1] = s->idx;
+            }
+            ne++;
+        }
+    return (int64_t)ne;
+}
+
+// Total wall time (ns) of LINK + PUT over the stream, nothing else recorded.
+uint64_t khpref_time(uint64_t nacc, const uint32_t *task, const uint8_t *mode, const uint8_t *nrect,
+                     const uint64_t *rects, uint64_t *nedges)
+{
+    std::vector<Acc> acc(nacc);
+    for (uint64_t i = 0; i < nacc; ++i) {
+        acc[i].idx = (uint32_t)i;
+        acc[i].task = task[i];
+        acc[i].mode = mode[i];
+        acc[i].nrect = nrect[i];
+        acc[i].rects[0] = make_rect(rects + 8 * i);
+        acc[i].rects[1] = make_rect(rects + 8 * i + 4);
+        acc[i].successors.reserve(8);
+    }
+    DepTree tree;
+    uint64_t t0 = now_ns();
+    uint64_t i = 0;
+    while (i < nacc) {
+        uint64_t j = i;
+        while (j < nacc && task[j] == task[i]) ++j;
+        for (uint64_t a = i; a < j; ++a) tree.link(&acc[a]);
+        for (uint64_t a = i; a < j; ++a) tree.put(&acc[a]);
+        i = j;
+    }
+    uint64_t dt = now_ns() - t0;
+    uint64_t ne = 0;
+    for (uint64_t p = 0; p < nacc; ++p) ne += acc[p].successors.size();
+    if (nedges) *nedges = ne;
+    return dt;
+}
+
+// Memory-tree partition of each access of one coherency domain: host
+// accesses (dev 0) are skipped while the tree is empty; otherwise every rect
+// is inserted, then every node intersecting a rect is listed.  part_off gets
+// nacc + 1 offsets; parts gets 4 u64 per partite (x0, x1, y0, y1).  Returns
+// the total number of partites (may exceed part_cap).
+int64_t khpref_partition(uint64_t nacc, const uint8_t *dev, const uint8_t *nrect, const uint64_t *rects,
+                         uint64_t part_cap, uint64_t *part_off, uint64_t *parts)
+{
+    PartTree tree;
+    std::vector<Rect> out;
+    uint64_t np = 0;
+    for (uint64_t i = 0; i < nacc; ++i) {
+        part_off[i] = np;
+        if (dev[i] == 0 && tree.root == nullptr) continue;
+        PartSearch s{&out};
+        Rect r[2] = {make_rect(rects + 8 * i), make_rect(rects + 8 * i + 4)};
+        for (int k = 0; k < nrect[i]; ++k) tree.insert(s, r[k]);
+        out.clear();
+        for (int k = 0; k < nrect[i]; ++k) tree.intersect(s, r[k]);
+        for (const Rect &p : out) {
+            if (np < part_cap) {
+                parts[4 * np + 0] = p[0].a; parts[4 * np + 1] = p[0].b;
+                parts[4 * np + 2] = p[1].a; parts[4 * np + 3] = p[1].b;
+            }
+            np++;
+        }
+    }
+    part_off[nacc] = np;
+    return (int64_t)np;
+}
+
+} // extern "C"

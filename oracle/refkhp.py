@@ -1,0 +1,88 @@
+"""Test infrastructure: ctypes binding of oracle/_ref/libkhpref.so, the
+reference KHP engine driven by oracle/ref_harness.cc.  Only tests, smoke() and
+bench.py's CPU-baseline leg may use this module."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+REF_SO = os.path.join(_HERE, "_ref", "libkhpref.so")
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(REF_SO)
+        p, u = C.c_void_p, C.c_uint64
+        _lib.khpref_resolve.argtypes = [u, p, p, p, p, u, p, p, p]
+        _lib.khpref_resolve.restype = C.c_int64
+        _lib.khpref_time.argtypes = [u, p, p, p, p, p]
+        _lib.khpref_time.restype = u
+        _lib.khpref_partition.argtypes = [u, p, p, p, u, p, p]
+        _lib.khpref_partition.restype = C.c_int64
+    return _lib
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dt)
+
+
+def resolve(task, mode, rects, nrect, want_frags=False, want_times=False):
+    """Edges (E, 2) uint32 (pred, succ) access indices; optional per-access
+    fragment counts and per-task LINK+PUT nanoseconds."""
+    L = lib()
+    n = len(task)
+    task = _c(task, np.uint32); mode = _c(mode, np.uint8)
+    rects = _c(rects, np.uint64); nrect = _c(nrect, np.uint8)
+    frag = np.zeros(n, np.uint64) if want_frags else None
+    ntask = int(np.count_nonzero(np.diff(task.astype(np.int64))) + 1) if n else 0
+    tns = np.zeros(ntask, np.uint64) if want_times else None
+    cap = max(1024, 4 * n)
+    while True:
+        edges = np.empty((cap, 2), np.uint32)
+        ne = L.khpref_resolve(n, task.ctypes.data, mode.ctypes.data, nrect.ctypes.data, rects.ctypes.data,
+                              cap, edges.ctypes.data, frag.ctypes.data if frag is not None else None,
+                              tns.ctypes.data if tns is not None else None)
+        if ne <= cap:
+            break
+        cap = int(ne)
+    out = [edges[:ne].copy()]
+    if want_frags:
+        out.append(frag)
+    if want_times:
+        out.append(tns)
+    return out[0] if len(out) == 1 else tuple(out)
+
+
+def time_resolve(task, mode, rects, nrect):
+    L = lib()
+    n = len(task)
+    task = _c(task, np.uint32); mode = _c(mode, np.uint8)
+    rects = _c(rects, np.uint64); nrect = _c(nrect, np.uint8)
+    ne = np.zeros(1, np.uint64)
+    ns = L.khpref_time(n, task.ctypes.data, mode.ctypes.data, nrect.ctypes.data, rects.ctypes.data, ne.ctypes.data)
+    return int(ns), int(ne[0])
+
+
+def partition(dev, rects, nrect):
+    L = lib()
+    n = len(dev)
+    dev = _c(dev, np.uint8); rects = _c(rects, np.uint64); nrect = _c(nrect, np.uint8)
+    off = np.zeros(n + 1, np.uint64)
+    cap = max(1024, 8 * n)
+    while True:
+        parts = np.empty((cap, 4), np.uint64)
+        np_ = L.khpref_partition(n, dev.ctypes.data, nrect.ctypes.data, rects.ctypes.data, cap,
+                                 off.ctypes.data, parts.ctypes.data)
+        if np_ <= cap:
+            break
+        cap = int(np_)
+    return off.astype(np.int64), parts[:np_].copy()

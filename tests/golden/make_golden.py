@@ -1,0 +1,76 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own KHP
+engine (oracle/_ref/libkhpref.so, built by `make -C oracle` from
+/root/reference).  Run in the build container:  python tests/golden/make_golden.py
+
+Fixtures (npz, allow_pickle=False):
+  cholesky_t2_edges.npz      T = 2 Cholesky (the closed-form 3-edge DAG)
+  cholesky_4096_nb256.npz    tiled Cholesky 4096^2 fp64, 256^2 tiles (816 tasks)
+  lu_1024_nb128.npz          tiled LU 1024^2 fp64, 128^2 tiles
+  heat_1024_t128_s4.npz      heat halo stream 1024^2 fp32, 128^2 tiles, 4 steps
+  random2d_4000.npz          first 4000 random tasks over 65536^2 (4 B elements)
+  wrap_mixed.npz             wrapped 2-rect tiles, every access mode, seeded
+  partition_coherency.npz    memory-tree partition geometry of 500 coherency accesses (view 0)
+Each resolver fixture holds the stream (task, slot, mode, addr, ld, m, n, s),
+the edges as (pred_task, pred_slot, succ_task, succ_slot) sorted rows, and
+the per-access fragment counts F_i.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import refkhp  # noqa: E402
+from xkaapi_amd import streams as S  # noqa: E402
+
+
+def wrap_stream(n=1500, ld=16, cols=40, s=2, maxm=12, maxn=6, seed=7):
+    rng = np.random.default_rng(seed)
+    task = (np.arange(n) // 2).astype(np.uint32)
+    slot = (np.arange(n) % 2).astype(np.uint16)
+    mode = rng.choice(np.array([1, 2, 3, 4, 5, 6, 8, 9], np.uint8), n)
+    m = rng.integers(1, maxm + 1, n).astype(np.uint64)
+    nn = rng.integers(1, maxn + 1, n).astype(np.uint64)
+    row = rng.integers(0, ld, n).astype(np.uint64)
+    col = rng.integers(0, cols, n).astype(np.uint64)
+    return S.Stream(task, slot, mode, np.ones(n, np.uint8), (col * ld + row) * s, np.full(n, ld, np.uint64),
+                    m, nn, np.full(n, s, np.uint64), name="wrap")
+
+
+def edge_rows(st, e):
+    rows = np.stack([st.task[e[:, 0]].astype(np.int64), st.slot[e[:, 0]].astype(np.int64),
+                     st.task[e[:, 1]].astype(np.int64), st.slot[e[:, 1]].astype(np.int64)], axis=1)
+    return rows[np.lexsort(rows.T[::-1])]
+
+
+def save_resolver(name, st):
+    r, nr = st.rects()
+    e, f = refkhp.resolve(st.task, st.mode, r, nr, want_frags=True)
+    np.savez_compressed(os.path.join(HERE, name), task=st.task, slot=st.slot, mode=st.mode, addr=st.addr,
+                        ld=st.ld, m=st.m, n=st.n, s=st.s, edges=edge_rows(st, e), frag=f.astype(np.int64))
+    print(name, len(st), "accesses", len(e), "edges")
+
+
+def main():
+    if not refkhp.available():
+        raise SystemExit("oracle/_ref/libkhpref.so missing: run make -C oracle")
+    save_resolver("cholesky_t2_edges.npz", S.cholesky(512, 256))
+    save_resolver("cholesky_4096_nb256.npz", S.cholesky(4096, 256))
+    save_resolver("lu_1024_nb128.npz", S.lu(1024, 128))
+    save_resolver("heat_1024_t128_s4.npz", S.heat(1024, 128, 4, 4))
+    save_resolver("random2d_4000.npz", S.random2d(4000))
+    save_resolver("wrap_mixed.npz", wrap_stream())
+    st = S.coherency(3000)
+    sel = np.flatnonzero(st.ld == 8192)[:500]
+    sub = st.take(sel)
+    r, nr = sub.rects()
+    off, parts = refkhp.partition(sub.dev, r, nr)
+    np.savez_compressed(os.path.join(HERE, "partition_coherency.npz"), dev=sub.dev, mode=sub.mode, addr=sub.addr,
+                        ld=sub.ld, m=sub.m, n=sub.n, s=sub.s, part_off=off, parts=parts.astype(np.int64))
+    print("partition_coherency.npz", len(sub), "accesses", len(parts), "partites")
+
+
+if __name__ == "__main__":
+    main()
