@@ -1,0 +1,153 @@
+/*
+ * xkrt_hip.h -- C ABI of the MI355X (gfx950) dependency resolver and
+ * coherency fetch planner.
+ *
+ * Plain C types only: pointers, sizes, POD structs.  Every entry point
+ * returns 0 on success or a negative errno:
+ *   -EINVAL     bad argument (null handle, zero ld, non-contiguous tasks, ...)
+ *   -ENODEV     no HIP device / the HIP runtime failed
+ *   -ENOMEM     device memory budget or allocation exhausted
+ *   -ENOSPC     a per-bin capacity (fragments, readers, records) was exceeded;
+ *               the domain is poisoned and must be destroyed
+ *   -EOVERFLOW  the edge / candidate buffers overflowed; the domain is poisoned
+ *   -ERANGE     a coordinate does not fit the domain's 32-bit frame
+ *   -ENOTSUP    an access the device path does not reproduce bit-exactly
+ *               (a wrapped 2-rect tile that the reference would mark beyond
+ *               its rects, see xkrt_hip_dep_submit)
+ *
+ * Reference interfaces replaced (rpereira-dev/xkaapi):
+ *   DependencyDomain::link / put           include/xkrt/memory/access/dependency-domain.hpp:48-80
+ *   KBLASDependencyTree<2>::link / put      include/xkrt/memory/access/blas/dependency-tree.hpp:367-414
+ *   task_get_dependency_domain_blas_matrix  src/task/task.cc:153-201  (one domain per (ld, sizeof_type))
+ *   __access_precedes (edge dedup)          include/xkrt/task/task.hpp:494-512
+ *   KBLASMemoryTree<2>::fetch               include/xkrt/memory/access/blas/memory-tree.hpp:1806-1874
+ *   task_get_memory_controller              src/task/task.cc:50-150   (one tree per (ld, sizeof_type))
+ */
+#ifndef XKRT_HIP_H
+#define XKRT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* access modes, identical to xkrt_access_mode_t (include/xkrt/memory/access/mode.h:56-62) */
+enum {
+    XKRT_HIP_MODE_R  = 1,
+    XKRT_HIP_MODE_W  = 2,
+    XKRT_HIP_MODE_RW = 3,
+    XKRT_HIP_MODE_V  = 4,
+    XKRT_HIP_MODE_D  = 8
+};
+
+/* One column-major tile access, the region of the reference's BLAS access
+ * constructor (include/xkrt/memory/access/access.hpp:539-574) with the
+ * offsets folded into addr; ld and sizeof_type are the domain's. */
+typedef struct xkrt_hip_access {
+    uint64_t addr;   /* begin address in bytes */
+    uint64_t m;      /* rows (elements) */
+    uint64_t n;      /* columns */
+    uint32_t task;   /* spawn-order task number; a task's accesses are contiguous, slot = order inside it */
+    uint8_t  mode;   /* XKRT_HIP_MODE_* bits */
+    uint8_t  pad[3];
+} xkrt_hip_access_t;
+
+/* Device-resident rect records (structure of arrays, 24 B per rect).
+ * Coordinates are in the domain frame: x = byte offset in the column,
+ * y = column - the domain's y base (see xkrt_hip_dep_frame).  tm packs the
+ * launch-local task index (bits 8..31) and the mode (bits 0..7).  acc is the
+ * domain access id the edges refer to. */
+typedef struct xkrt_hip_rect_soa {
+    const uint32_t *x0, *x1, *y0, *y1, *acc, *tm;
+} xkrt_hip_rect_soa_t;
+
+typedef struct xkrt_hip_dep_config {
+    uint32_t bin_x_bytes;  /* bin extent in dim 0 (bytes); 0 = auto */
+    uint32_t bin_y_cols;   /* bin extent in dim 1 (columns); 0 = auto */
+    uint32_t kernel;       /* LDS capacity class: 0 = small, 1 = medium, 2 = large */
+    uint32_t pad;
+    uint64_t max_rects;    /* max rect records per submit (sizes scratch) */
+    uint64_t max_edges;    /* edge buffer capacity before a drain is required */
+    uint64_t budget_bytes; /* device memory budget of the domain; 0 = unlimited */
+    uint64_t y_hint_lo;    /* expected column range [lo, hi) (pre-sizes the bin grid); lo == hi: none */
+    uint64_t y_hint_hi;
+} xkrt_hip_dep_config_t;
+
+typedef struct xkrt_hip_dep_stats {
+    uint64_t accesses;      /* accesses resolved so far */
+    uint64_t rects;         /* rect records resolved so far */
+    uint64_t edges;         /* edges produced so far */
+    uint64_t candidates;    /* edge candidates before the cross-bin dedup */
+    uint64_t launches;      /* resolve launches */
+    uint64_t bins;          /* bins in the grid */
+    uint64_t max_frags;     /* largest fragment count seen in one bin */
+    uint64_t max_reads;     /* largest reader-pool use seen in one bin */
+    uint64_t frag_visits;   /* fragment records scanned by the resolve kernel */
+    uint64_t device_bytes;  /* device memory held by the domain */
+} xkrt_hip_dep_stats_t;
+
+typedef struct xkrt_hip_dep_domain *xkrt_hip_dep_domain_t;
+
+/* Selects the HIP device used by later calls from this thread. */
+int xkrt_hip_init(int device);
+
+/* Creates one dependency domain keyed by (ld, sizeof_type), the seam of
+ * task_get_dependency_domain_blas_matrix (src/task/task.cc:153-201).  cfg may
+ * be NULL (defaults).  stream is a hipStream_t (NULL = a stream of its own). */
+int xkrt_hip_dep_domain_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_dep_config_t *cfg,
+                               void *stream, xkrt_hip_dep_domain_t *out);
+int xkrt_hip_dep_domain_destroy(xkrt_hip_dep_domain_t d);
+
+/* Enqueues the resolution of n accesses in spawn order: every access of a
+ * task is LINKed (slot order) before any is PUT, as task_dependency_resolve
+ * does (src/task/task.cc:285-299).  Access ids are assigned consecutively
+ * from *first_id (out).  Asynchronous: edges become visible after
+ * xkrt_hip_dep_sync.  A wrapped tile (2 rects) is refused with -ENOTSUP, and
+ * nothing of the batch is enqueued, when an earlier rect of the domain spans
+ * its wrap gap in dim 0: there the reference marks whole split fragments
+ * beyond the tile (dependency-tree.hpp:229-256). */
+int xkrt_hip_dep_submit(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n, uint64_t *first_id);
+
+/* Same, from device-resident rect records (no host conversion, no copy).
+ * ntask = launch-local tasks; [ylo, yhi) bounds every record's y (domain
+ * frame); naccess = accesses the records belong to (for accounting). */
+int xkrt_hip_dep_submit_device(xkrt_hip_dep_domain_t d, const xkrt_hip_rect_soa_t *rec, uint64_t nrect,
+                               uint64_t naccess, uint32_t ylo, uint32_t yhi);
+
+/* Converts accesses to device-frame rect records on the host (the work
+ * xkrt_hip_dep_submit does before its copy).  out arrays hold cap records;
+ * returns the record count in *nrect; task numbers become launch-local;
+ * access ids start at first_id.  *ylo / *yhi receive the y bounds. */
+int xkrt_hip_dep_build_records(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n,
+                               uint32_t first_id, uint64_t cap, uint32_t *x0, uint32_t *x1, uint32_t *y0,
+                               uint32_t *y1, uint32_t *accid, uint32_t *tm, uint64_t *nrect, uint32_t *ylo,
+                               uint32_t *yhi);
+
+/* y base of the domain frame (fixed at the first submit or from the hint). */
+int xkrt_hip_dep_frame(xkrt_hip_dep_domain_t d, uint64_t *ybase);
+
+/* Waits for every enqueued launch; *nedges = edges not yet consumed. */
+int xkrt_hip_dep_sync(xkrt_hip_dep_domain_t d, uint64_t *nedges);
+
+/* Device pointer to the unconsumed edges, 2 x u32 (pred id, succ id) each,
+ * and their count (synchronises first). */
+int xkrt_hip_dep_edges_device(xkrt_hip_dep_domain_t d, const uint32_t **edges, uint64_t *n);
+
+/* Copies up to cap unconsumed edges to host memory (synchronises first),
+ * then consumes them all.  *n = number of edges that were unconsumed. */
+int xkrt_hip_dep_edges_copy(xkrt_hip_dep_domain_t d, uint32_t *host_edges, uint64_t cap, uint64_t *n);
+
+/* Drops every unconsumed edge (asynchronous). */
+int xkrt_hip_dep_edges_consume(xkrt_hip_dep_domain_t d);
+
+int xkrt_hip_dep_stats(xkrt_hip_dep_domain_t d, xkrt_hip_dep_stats_t *out);
+
+/* Stream the domain enqueues on (a hipStream_t). */
+void *xkrt_hip_dep_stream(xkrt_hip_dep_domain_t d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XKRT_HIP_H */

@@ -1,0 +1,1228 @@
+// MI355X (gfx950, wave64) dependency resolver for 2D (ld, sizeof_type)
+// region accesses: RAW / WAR / WAW predecessor edges with the reference's
+// semantics (include/xkrt/memory/access/blas/dependency-tree.hpp:229-355,
+// include/xkrt/task/task.hpp:494-512, src/task/task.cc:285-299).
+//
+// Data layout in HBM, per domain:
+//   * the domain frame is cut into a dense grid of bins, BX bytes (dim 0) by
+//     BY columns (dim 1), both powers of two;
+//   * each bin holds a list of fragments that tile it exactly, every
+//     fragment with a uniform state: last writer (access id) and the
+//     readers since that writer (a slice of the bin's reader pool).
+//     32 B per fragment ({x0,x1,y0,y1}, {lw, rs_off, rs_len, 0}), FCAP per bin,
+//     RCAP u32 reader entries per bin;
+//   * rect records of a launch arrive as structure of arrays
+//     (x0, x1, y0, y1, acc, task<<8|mode: 24 B).
+//
+// One launch resolves one batch of records in six kernels:
+//   k_count   records -> per-bin record counts
+//   k_alloc   per-bin segments in the incidence array, active-bin list
+//   k_fill    incidence array (record ids, per-bin segments)
+//   k_resolve one wave64 per active bin: the bin's fragments and reader pool
+//             move to LDS, its records are sorted (bitonic, LDS) and replayed
+//             in spawn order -- LINK of every access of a task, then PUT --
+//             emitting (pred, succ access, succ task) candidates; W paints
+//             over fragments, R splits them and appends to the reader lists;
+//             the bin is written back compacted
+//   k_dedup   candidates -> open-addressing table keyed (pred, succ task),
+//             keeping the lowest succ access id (= lowest slot, since LINK
+//             runs in slot order)
+//   k_final   table entries -> (pred id, succ id) edges; table entries reset
+//
+// Because every fragment of a bin carries a uniform state, the edges found
+// per bin are exactly the per-point rule; the union over bins followed by
+// the (pred, task) dedup is the reference's edge set.
+
+#include <hip/hip_runtime.h>
+
+#include <errno.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/xkrt_hip.h"
+
+#define NONE 0xFFFFFFFFu
+#define EMPTY64 0xFFFFFFFFFFFFFFFFull
+
+enum : uint32_t {
+    ERR_BINREC = 1u,    // incidence array overflow
+    ERR_LIST = 2u,      // a bin got more records than the kernel's LDS list holds
+    ERR_FRAG = 4u,      // a bin's fragment list overflowed
+    ERR_READS = 8u,     // a bin's reader pool overflowed
+    ERR_CAND = 16u,     // candidate buffer overflow
+    ERR_HASH = 32u,     // dedup table full
+    ERR_EDGES = 64u,    // edge buffer overflow
+};
+
+struct DevCounters {
+    unsigned int total;      // incidences allocated this launch
+    unsigned int nactive;    // active bins this launch
+    unsigned int work;       // resolve work counter
+    unsigned int ncand;      // candidates this launch
+    unsigned int err;        // sticky ERR_* bits
+    unsigned int maxfrag;    // stats: max fragments in a bin
+    unsigned int maxrs;      // stats: max reader-pool use in a bin
+    unsigned int pad;
+    unsigned long long edge_total;   // unconsumed edges
+    unsigned long long edge_base;    // edge_total when the launch started
+    unsigned long long cand_total;   // stats
+    unsigned long long frag_visits;  // stats
+};
+
+struct KArgs {
+    const uint32_t *rx0, *rx1, *ry0, *ry1, *racc, *rtm;
+    uint32_t nrec;
+    uint32_t bxs, bys;        // log2 of the bin extents
+    uint32_t nbx, nby;        // bin grid
+    uint32_t ybin0;           // frame y of bin row 0
+    uint32_t xmax;            // LDs: dim-0 extent of the frame
+    uint32_t *bin_nf, *bin_nrs;
+    uint4 *frag;              // nbins * fcap * 2
+    uint32_t *rs;             // nbins * rcap
+    uint32_t fcap, rcap;
+    uint32_t *bin_cnt, *bin_cur, *active, *binrec;
+    uint32_t binrec_cap;
+    uint32_t *cand;
+    uint32_t cand_cap;
+    unsigned long long *hkeys;
+    uint32_t *hvals;
+    uint32_t hmask;
+    uint32_t *eslot;
+    uint32_t *edges;
+    unsigned long long edge_cap;
+    DevCounters *ctr;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const uint32_t l = lane_id();
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(v, d, 64);
+        if (l >= d) v += y;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_bcast(uint32_t v, int src) { return __shfl(v, src, 64); }
+
+__device__ __forceinline__ uint64_t mix64(uint64_t k)
+{
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// ---------------------------------------------------------------- binning
+
+__device__ __forceinline__ void rec_bins(const KArgs &a, uint32_t r, uint32_t &bx0, uint32_t &bx1, uint32_t &by0,
+                                         uint32_t &by1)
+{
+    const uint32_t x0 = a.rx0[r], x1 = a.rx1[r], y0 = a.ry0[r], y1 = a.ry1[r];
+    bx0 = x0 >> a.bxs;
+    bx1 = (x1 - 1) >> a.bxs;
+    by0 = (y0 - a.ybin0) >> a.bys;
+    by1 = (y1 - 1 - a.ybin0) >> a.bys;
+}
+
+__global__ void k_count(KArgs a)
+{
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) {
+        a.ctr->total = 0;
+        a.ctr->nactive = 0;
+        a.ctr->work = 0;
+        a.ctr->ncand = 0;
+        a.ctr->edge_base = a.ctr->edge_total;
+    }
+    for (uint32_t r = tid; r < a.nrec; r += gridDim.x * blockDim.x) {
+        uint32_t bx0, bx1, by0, by1;
+        rec_bins(a, r, bx0, bx1, by0, by1);
+        for (uint32_t by = by0; by <= by1; ++by)
+            for (uint32_t bx = bx0; bx <= bx1; ++bx) atomicAdd(&a.bin_cnt[by * a.nbx + bx], 1u);
+    }
+}
+
+__global__ void k_alloc(KArgs a, uint32_t nbins)
+{
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = slot < nbins ? a.bin_cnt[slot] : 0u;
+    const uint32_t incl = wave_incl_scan(c);
+    const uint32_t sum = wave_bcast(incl, 63);
+    const uint64_t act = __ballot(c > 0);
+    if (act == 0) return;
+    uint32_t base = 0, abase = 0;
+    if (lane_id() == 0) {
+        base = atomicAdd(&a.ctr->total, sum);
+        abase = atomicAdd(&a.ctr->nactive, (uint32_t)__popcll(act));
+    }
+    base = wave_bcast(base, 0);
+    abase = wave_bcast(abase, 0);
+    if (lane_id() == 0 && (uint64_t)base + sum > a.binrec_cap) atomicOr(&a.ctr->err, ERR_BINREC);
+    if (c > 0) {
+        const uint32_t rank = (uint32_t)__popcll(act & ((1ull << lane_id()) - 1ull));
+        a.bin_cur[slot] = base + incl - c;
+        a.active[abase + rank] = slot;
+    }
+}
+
+__global__ void k_fill(KArgs a)
+{
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t r = tid; r < a.nrec; r += gridDim.x * blockDim.x) {
+        uint32_t bx0, bx1, by0, by1;
+        rec_bins(a, r, bx0, bx1, by0, by1);
+        for (uint32_t by = by0; by <= by1; ++by)
+            for (uint32_t bx = bx0; bx <= bx1; ++bx) {
+                const uint32_t pos = atomicAdd(&a.bin_cur[by * a.nbx + bx], 1u);
+                if (pos < a.binrec_cap) a.binrec[pos] = r;
+            }
+    }
+}
+
+// ---------------------------------------------------------------- resolve
+
+#define DSZ 256u   // per-task dedup set (LDS)
+#define CCAP 512u  // candidate staging (LDS), 3 u32 each
+
+template <int FCAP, int RCAP, int LCAP>
+struct BinLds {
+    uint32_t fx0[2][FCAP], fx1[2][FCAP], fy0[2][FCAP], fy1[2][FCAP];
+    uint32_t flw[2][FCAP], froff[2][FCAP], frlen[2][FCAP];
+    uint32_t rs[2][RCAP];
+    uint32_t L[LCAP];
+    uint32_t D[DSZ];
+    uint32_t C[3 * CCAP];
+    uint32_t ccount;
+};
+
+struct BinState {
+    uint32_t nf;      // fragments in the current bank
+    uint32_t fb;      // current fragment bank
+    uint32_t rb;      // current reader-pool bank
+    uint32_t rtop;    // reader-pool use
+    uint32_t err;     // local error bits
+};
+
+template <int FCAP, int RCAP, int LCAP>
+__device__ void flush_cands(const KArgs &a, BinLds<FCAP, RCAP, LCAP> &s)
+{
+    __syncthreads();
+    const uint32_t n = s.ccount;
+    if (n) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(&a.ctr->ncand, n);
+        base = wave_bcast(base, 0);
+        if ((uint64_t)base + n > a.cand_cap) {
+            if (lane_id() == 0) atomicOr(&a.ctr->err, ERR_CAND);
+        } else {
+            for (uint32_t i = lane_id(); i < 3 * n; i += 64) a.cand[3ull * base + i] = s.C[i];
+        }
+    }
+    __syncthreads();
+    if (lane_id() == 0) s.ccount = 0;
+    __syncthreads();
+}
+
+// insert pred into the task's dedup set; true when first seen (or the set is full)
+template <int FCAP, int RCAP, int LCAP>
+__device__ __forceinline__ bool dedup_insert(BinLds<FCAP, RCAP, LCAP> &s, uint32_t p)
+{
+    uint32_t h = (p * 2654435761u) >> 24;
+    for (uint32_t probe = 0; probe < DSZ; ++probe) {
+        const uint32_t old = atomicCAS(&s.D[h], NONE, p);
+        if (old == NONE) return true;
+        if (old == p) return false;
+        h = (h + 1) & (DSZ - 1);
+    }
+    return true;
+}
+
+template <int FCAP, int RCAP, int LCAP>
+__device__ __forceinline__ void emit(const KArgs &a, BinLds<FCAP, RCAP, LCAP> &s, uint32_t p, uint32_t acc,
+                                     uint32_t task, bool direct)
+{
+    if (!dedup_insert(s, p)) return;
+    if (direct) {
+        const uint32_t pos = atomicAdd(&a.ctr->ncand, 1u);
+        if (pos < a.cand_cap) {
+            a.cand[3ull * pos] = p;
+            a.cand[3ull * pos + 1] = acc;
+            a.cand[3ull * pos + 2] = task;
+        } else {
+            atomicOr(&a.ctr->err, ERR_CAND);
+        }
+    } else {
+        const uint32_t pos = atomicAdd(&s.ccount, 1u);
+        s.C[3 * pos] = p;
+        s.C[3 * pos + 1] = acc;
+        s.C[3 * pos + 2] = task;
+    }
+}
+
+__device__ __forceinline__ bool overlaps(uint32_t ax0, uint32_t ax1, uint32_t ay0, uint32_t ay1, uint32_t bx0,
+                                         uint32_t bx1, uint32_t by0, uint32_t by1)
+{
+    return ax0 < bx1 && bx0 < ax1 && ay0 < by1 && by0 < ay1;
+}
+
+// LINK (dependency-tree.hpp:311-355): per overlapping fragment, the readers
+// if the access writes and readers exist, else the last writer.
+template <int FCAP, int RCAP, int LCAP>
+__device__ void do_link(const KArgs &a, BinLds<FCAP, RCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1,
+                        uint32_t Y0, uint32_t Y1, uint32_t mode, uint32_t acc, uint32_t task)
+{
+    const uint32_t fb = st.fb, rb = st.rb;
+    const bool w = (mode & 2u) != 0;
+    for (uint32_t c = 0; c < st.nf; c += 64) {
+        const uint32_t f = c + lane_id();
+        uint32_t k = 0;
+        bool use_reads = false;
+        if (f < st.nf && overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f], X0, X1, Y0, Y1)) {
+            if (w && s.frlen[fb][f]) {
+                k = s.frlen[fb][f];
+                use_reads = true;
+            } else if (s.flw[fb][f] != NONE) {
+                k = 1;
+            }
+        }
+        const uint32_t U = wave_bcast(wave_incl_scan(k), 63);
+        if (U == 0) continue;
+        __syncthreads();
+        if (s.ccount + U > CCAP) flush_cands(a, s);
+        const bool direct = U > CCAP;
+        if (k) {
+            if (use_reads) {
+                const uint32_t off = s.froff[fb][f];
+                for (uint32_t i = 0; i < k; ++i) emit(a, s, s.rs[rb][off + i], acc, task, direct);
+            } else {
+                emit(a, s, s.flw[fb][f], acc, task, direct);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// guillotine pieces of f \ R (up to 4), inheriting f's state
+__device__ __forceinline__ uint32_t outside_pieces(uint32_t fx0, uint32_t fx1, uint32_t fy0, uint32_t fy1,
+                                                   uint32_t X0, uint32_t X1, uint32_t Y0, uint32_t Y1,
+                                                   uint32_t px0[4], uint32_t px1[4], uint32_t py0[4],
+                                                   uint32_t py1[4])
+{
+    uint32_t n = 0;
+    const uint32_t mx0 = max(fx0, X0), mx1 = min(fx1, X1);
+    if (fx0 < X0) { px0[n] = fx0; px1[n] = X0; py0[n] = fy0; py1[n] = fy1; ++n; }
+    if (X1 < fx1) { px0[n] = X1; px1[n] = fx1; py0[n] = fy0; py1[n] = fy1; ++n; }
+    if (fy0 < Y0) { px0[n] = mx0; px1[n] = mx1; py0[n] = fy0; py1[n] = Y0; ++n; }
+    if (Y1 < fy1) { px0[n] = mx0; px1[n] = mx1; py0[n] = Y1; py1[n] = fy1; ++n; }
+    return n;
+}
+
+// PUT of a writer (dependency-tree.hpp:238-242): the rect's points get
+// last_write = acc and no readers.  Fragments under the rect are cut away and
+// the rect (clipped to the bin) becomes one fragment.
+template <int FCAP, int RCAP, int LCAP>
+__device__ void do_paint(BinLds<FCAP, RCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
+                         uint32_t Y1, uint32_t acc)
+{
+    const uint32_t fb = st.fb, nb = fb ^ 1u;
+    uint32_t base = 0;
+    for (uint32_t c = 0; c < st.nf; c += 64) {
+        const uint32_t f = c + lane_id();
+        uint32_t npc = 0;
+        uint32_t px0[4], px1[4], py0[4], py1[4];
+        uint32_t fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
+        if (f < st.nf) {
+            fx0 = s.fx0[fb][f]; fx1 = s.fx1[fb][f]; fy0 = s.fy0[fb][f]; fy1 = s.fy1[fb][f];
+            if (overlaps(fx0, fx1, fy0, fy1, X0, X1, Y0, Y1)) {
+                npc = outside_pieces(fx0, fx1, fy0, fy1, X0, X1, Y0, Y1, px0, px1, py0, py1);
+            } else {
+                px0[0] = fx0; px1[0] = fx1; py0[0] = fy0; py1[0] = fy1;
+                npc = 1;
+            }
+        }
+        const uint32_t incl = wave_incl_scan(npc);
+        const uint32_t tot = wave_bcast(incl, 63);
+        if (base + tot + 1 > (uint32_t)FCAP) {
+            st.err |= ERR_FRAG;
+            return;
+        }
+        const uint32_t o = base + incl - npc;
+        if (npc) {
+            const uint32_t lw = s.flw[fb][f], ro = s.froff[fb][f], rl = s.frlen[fb][f];
+            for (uint32_t i = 0; i < npc; ++i) {
+                s.fx0[nb][o + i] = px0[i]; s.fx1[nb][o + i] = px1[i];
+                s.fy0[nb][o + i] = py0[i]; s.fy1[nb][o + i] = py1[i];
+                s.flw[nb][o + i] = lw; s.froff[nb][o + i] = ro; s.frlen[nb][o + i] = rl;
+            }
+        }
+        base += tot;
+    }
+    if (lane_id() == 0) {
+        s.fx0[nb][base] = X0; s.fx1[nb][base] = X1; s.fy0[nb][base] = Y0; s.fy1[nb][base] = Y1;
+        s.flw[nb][base] = acc; s.froff[nb][base] = 0; s.frlen[nb][base] = 0;
+    }
+    st.nf = base + 1;
+    st.fb = nb;
+    __syncthreads();
+}
+
+// copy every live reader list to the other pool bank, compacted
+template <int FCAP, int RCAP, int LCAP>
+__device__ void compact_reads(BinLds<FCAP, RCAP, LCAP> &s, BinState &st)
+{
+    const uint32_t fb = st.fb, rb = st.rb, nrb = rb ^ 1u;
+    uint32_t base = 0;
+    for (uint32_t c = 0; c < st.nf; c += 64) {
+        const uint32_t f = c + lane_id();
+        const uint32_t len = f < st.nf ? s.frlen[fb][f] : 0u;
+        const uint32_t incl = wave_incl_scan(len);
+        const uint32_t tot = wave_bcast(incl, 63);
+        if (base + tot > (uint32_t)RCAP) {
+            st.err |= ERR_READS;
+            return;
+        }
+        if (len) {
+            const uint32_t o = base + incl - len, src = s.froff[fb][f];
+            for (uint32_t i = 0; i < len; ++i) s.rs[nrb][o + i] = s.rs[rb][src + i];
+            s.froff[fb][f] = o;
+        }
+        base += tot;
+    }
+    st.rb = nrb;
+    st.rtop = base;
+    __syncthreads();
+}
+
+// PUT of an exact reader (dependency-tree.hpp:243-244): the rect's points get
+// acc appended to their readers.  Overlapping fragments are split into the
+// part inside the rect (new reader list = old + acc) and up to 4 outside
+// parts that keep sharing the old list.
+template <int FCAP, int RCAP, int LCAP>
+__device__ void do_annotate(BinLds<FCAP, RCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
+                            uint32_t Y1, uint32_t acc)
+{
+    // reader-pool space this record needs
+    {
+        const uint32_t fb = st.fb;
+        uint32_t need = 0;
+        for (uint32_t c = 0; c < st.nf; c += 64) {
+            const uint32_t f = c + lane_id();
+            uint32_t k = 0;
+            if (f < st.nf && overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f], X0, X1, Y0, Y1))
+                k = s.frlen[fb][f] + 1;
+            need += wave_bcast(wave_incl_scan(k), 63);
+        }
+        if (st.rtop + need > (uint32_t)RCAP) {
+            compact_reads(s, st);
+            if (st.err) return;
+            if (st.rtop + need > (uint32_t)RCAP) {
+                st.err |= ERR_READS;
+                return;
+            }
+        }
+    }
+    const uint32_t fb = st.fb, nb = fb ^ 1u, rb = st.rb;
+    uint32_t base = 0;
+    for (uint32_t c = 0; c < st.nf; c += 64) {
+        const uint32_t f = c + lane_id();
+        uint32_t npc = 0, rneed = 0;
+        bool ov = false;
+        uint32_t px0[4], px1[4], py0[4], py1[4];
+        uint32_t fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
+        if (f < st.nf) {
+            fx0 = s.fx0[fb][f]; fx1 = s.fx1[fb][f]; fy0 = s.fy0[fb][f]; fy1 = s.fy1[fb][f];
+            ov = overlaps(fx0, fx1, fy0, fy1, X0, X1, Y0, Y1);
+            if (ov) {
+                npc = outside_pieces(fx0, fx1, fy0, fy1, X0, X1, Y0, Y1, px0, px1, py0, py1) + 1;
+                rneed = s.frlen[fb][f] + 1;
+            } else {
+                px0[0] = fx0; px1[0] = fx1; py0[0] = fy0; py1[0] = fy1;
+                npc = 1;
+            }
+        }
+        const uint32_t incl = wave_incl_scan(npc);
+        const uint32_t tot = wave_bcast(incl, 63);
+        const uint32_t rincl = wave_incl_scan(rneed);
+        const uint32_t rtot = wave_bcast(rincl, 63);
+        if (base + tot > (uint32_t)FCAP) {
+            st.err |= ERR_FRAG;
+            return;
+        }
+        const uint32_t o = base + incl - npc;
+        if (npc) {
+            const uint32_t lw = s.flw[fb][f], ro = s.froff[fb][f], rl = s.frlen[fb][f];
+            const uint32_t nout = ov ? npc - 1 : npc;
+            for (uint32_t i = 0; i < nout; ++i) {
+                s.fx0[nb][o + i] = px0[i]; s.fx1[nb][o + i] = px1[i];
+                s.fy0[nb][o + i] = py0[i]; s.fy1[nb][o + i] = py1[i];
+                s.flw[nb][o + i] = lw; s.froff[nb][o + i] = ro; s.frlen[nb][o + i] = rl;
+            }
+            if (ov) {
+                const uint32_t j = o + nout;
+                const uint32_t ro2 = st.rtop + rincl - rneed;
+                for (uint32_t i = 0; i < rl; ++i) s.rs[rb][ro2 + i] = s.rs[rb][ro + i];
+                s.rs[rb][ro2 + rl] = acc;
+                s.fx0[nb][j] = max(fx0, X0); s.fx1[nb][j] = min(fx1, X1);
+                s.fy0[nb][j] = max(fy0, Y0); s.fy1[nb][j] = min(fy1, Y1);
+                s.flw[nb][j] = lw; s.froff[nb][j] = ro2; s.frlen[nb][j] = rl + 1;
+            }
+        }
+        base += tot;
+        st.rtop += rtot;
+    }
+    st.nf = base;
+    st.fb = nb;
+    __syncthreads();
+}
+
+template <int FCAP, int RCAP, int LCAP>
+__global__ void __launch_bounds__(64) k_resolve(KArgs a)
+{
+    extern __shared__ unsigned char lds_raw[];
+    BinLds<FCAP, RCAP, LCAP> &s = *reinterpret_cast<BinLds<FCAP, RCAP, LCAP> *>(lds_raw);
+    const uint32_t lane = lane_id();
+    if (a.ctr->err) return;  // a poisoned domain touches no state
+    const uint32_t nactive = a.ctr->nactive;
+    unsigned long long visits = 0;
+    uint32_t maxf = 0, maxr = 0;
+    while (true) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(&a.ctr->work, 1u);
+        b = wave_bcast(b, 0);
+        if (b >= nactive) break;
+        const uint32_t slot = a.active[b];
+        const uint32_t cnt = a.bin_cnt[slot];
+        const uint32_t start = a.bin_cur[slot] - cnt;
+        if (cnt > (uint32_t)LCAP) {
+            if (lane == 0) {
+                atomicOr(&a.ctr->err, ERR_LIST);
+                a.bin_cnt[slot] = 0;
+            }
+            continue;
+        }
+        // bin geometry
+        const uint32_t bxi = slot % a.nbx, byi = slot / a.nbx;
+        const uint32_t BX0 = bxi << a.bxs, BX1 = min((bxi + 1) << a.bxs, a.xmax);
+        const uint32_t BY0 = a.ybin0 + (byi << a.bys), BY1 = BY0 + (1u << a.bys);
+
+        // record list, sorted ascending (spawn order)
+        uint32_t n2 = 1;
+        while (n2 < cnt) n2 <<= 1;
+        for (uint32_t i = lane; i < n2; i += 64) s.L[i] = i < cnt ? a.binrec[start + i] : NONE;
+        __syncthreads();
+        for (uint32_t k = 2; k <= n2; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = lane; i < n2; i += 64) {
+                    const uint32_t ixj = i ^ j;
+                    if (ixj > i) {
+                        const uint32_t x = s.L[i], y = s.L[ixj];
+                        const bool up = (i & k) == 0;
+                        if ((x > y) == up) { s.L[i] = y; s.L[ixj] = x; }
+                    }
+                }
+                __syncthreads();
+            }
+
+        // bin state
+        BinState st;
+        st.fb = 0; st.rb = 0; st.err = 0;
+        const uint32_t nf0 = a.bin_nf[slot];
+        if (nf0 == 0) {
+            if (lane == 0) {
+                s.fx0[0][0] = BX0; s.fx1[0][0] = BX1; s.fy0[0][0] = BY0; s.fy1[0][0] = BY1;
+                s.flw[0][0] = NONE; s.froff[0][0] = 0; s.frlen[0][0] = 0;
+            }
+            st.nf = 1;
+            st.rtop = 0;
+        } else {
+            const uint4 *fr = a.frag + (size_t)slot * a.fcap * 2;
+            for (uint32_t f = lane; f < nf0; f += 64) {
+                const uint4 g = fr[2 * f], h = fr[2 * f + 1];
+                s.fx0[0][f] = g.x; s.fx1[0][f] = g.y; s.fy0[0][f] = g.z; s.fy1[0][f] = g.w;
+                s.flw[0][f] = h.x; s.froff[0][f] = h.y; s.frlen[0][f] = h.z;
+            }
+            st.nf = nf0;
+            st.rtop = a.bin_nrs[slot];
+            const uint32_t *rr = a.rs + (size_t)slot * a.rcap;
+            for (uint32_t i = lane; i < st.rtop; i += 64) s.rs[0][i] = rr[i];
+        }
+        if (lane == 0) s.ccount = 0;
+        __syncthreads();
+
+        uint32_t i = 0;
+        while (i < cnt && !st.err) {
+            const uint32_t r0 = __builtin_amdgcn_readfirstlane(s.L[i]);
+            const uint32_t task = a.rtm[r0] >> 8;
+            uint32_t j = i + 1;
+            while (j < cnt && (a.rtm[__builtin_amdgcn_readfirstlane(s.L[j])] >> 8) == task) ++j;
+            for (uint32_t q = lane; q < DSZ; q += 64) s.D[q] = NONE;
+            __syncthreads();
+            for (uint32_t q = i; q < j; ++q) {
+                const uint32_t r = __builtin_amdgcn_readfirstlane(s.L[q]);
+                const uint32_t X0 = max(a.rx0[r], BX0), X1 = min(a.rx1[r], BX1);
+                const uint32_t Y0 = max(a.ry0[r], BY0), Y1 = min(a.ry1[r], BY1);
+                do_link(a, s, st, X0, X1, Y0, Y1, a.rtm[r] & 0xffu, a.racc[r], task);
+                visits += st.nf;
+            }
+            for (uint32_t q = i; q < j && !st.err; ++q) {
+                const uint32_t r = __builtin_amdgcn_readfirstlane(s.L[q]);
+                const uint32_t X0 = max(a.rx0[r], BX0), X1 = min(a.rx1[r], BX1);
+                const uint32_t Y0 = max(a.ry0[r], BY0), Y1 = min(a.ry1[r], BY1);
+                const uint32_t mode = a.rtm[r] & 0xffu;
+                visits += st.nf;
+                if (mode & 2u)
+                    do_paint(s, st, X0, X1, Y0, Y1, a.racc[r]);
+                else if (mode == 1u)
+                    do_annotate(s, st, X0, X1, Y0, Y1, a.racc[r]);
+            }
+            i = j;
+        }
+        flush_cands(a, s);
+        if (st.err) {
+            if (lane == 0) {
+                atomicOr(&a.ctr->err, st.err);
+                a.bin_cnt[slot] = 0;
+            }
+            continue;
+        }
+
+        // write back, reader lists compacted
+        {
+            const uint32_t fb = st.fb, rb = st.rb;
+            uint32_t total = 0;
+            for (uint32_t c = 0; c < st.nf; c += 64) {
+                const uint32_t f = c + lane;
+                total += wave_bcast(wave_incl_scan(f < st.nf ? s.frlen[fb][f] : 0u), 63);
+            }
+            if (total > a.rcap) {
+                if (lane == 0) {
+                    atomicOr(&a.ctr->err, ERR_READS);
+                    a.bin_cnt[slot] = 0;
+                }
+                continue;
+            }
+            uint4 *fr = a.frag + (size_t)slot * a.fcap * 2;
+            uint32_t *rr = a.rs + (size_t)slot * a.rcap;
+            uint32_t base = 0;
+            for (uint32_t c = 0; c < st.nf; c += 64) {
+                const uint32_t f = c + lane;
+                const uint32_t len = f < st.nf ? s.frlen[fb][f] : 0u;
+                const uint32_t incl = wave_incl_scan(len);
+                const uint32_t tot = wave_bcast(incl, 63);
+                if (f < st.nf) {
+                    const uint32_t o = base + incl - len, src = s.froff[fb][f];
+                    for (uint32_t q = 0; q < len; ++q) rr[o + q] = s.rs[rb][src + q];
+                    fr[2 * f] = make_uint4(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f]);
+                    fr[2 * f + 1] = make_uint4(s.flw[fb][f], o, len, 0u);
+                }
+                base += tot;
+            }
+            if (lane == 0) {
+                a.bin_nf[slot] = st.nf;
+                a.bin_nrs[slot] = base;
+                a.bin_cnt[slot] = 0;
+            }
+            maxf = max(maxf, st.nf);
+            maxr = max(maxr, base);
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        atomicMax(&a.ctr->maxfrag, maxf);
+        atomicMax(&a.ctr->maxrs, maxr);
+        atomicAdd(&a.ctr->frag_visits, visits);
+    }
+}
+
+// ---------------------------------------------------------------- dedup
+
+__global__ void k_dedup(KArgs a)
+{
+    const uint32_t n = min(a.ctr->ncand, a.cand_cap);
+    const unsigned long long ebase = a.ctr->edge_base;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = a.cand[3ull * i], acc = a.cand[3ull * i + 1], t = a.cand[3ull * i + 2];
+        const unsigned long long key = ((unsigned long long)p << 32) | t;
+        uint32_t h = (uint32_t)mix64(key) & a.hmask;
+        uint32_t probe = 0;
+        while (true) {
+            const unsigned long long old = atomicCAS(&a.hkeys[h], EMPTY64, key);
+            if (old == EMPTY64) {
+                atomicMin(&a.hvals[h], acc);
+                const unsigned long long pos = atomicAdd(&a.ctr->edge_total, 1ull);
+                if (pos >= a.edge_cap || pos - ebase > a.hmask) atomicOr(&a.ctr->err, ERR_EDGES);
+                else a.eslot[pos - ebase] = h;
+                break;
+            }
+            if (old == key) {
+                atomicMin(&a.hvals[h], acc);
+                break;
+            }
+            h = (h + 1) & a.hmask;
+            if (++probe > a.hmask) {
+                atomicOr(&a.ctr->err, ERR_HASH);
+                break;
+            }
+        }
+    }
+}
+
+__global__ void k_final(KArgs a)
+{
+    if (a.ctr->err & (ERR_EDGES | ERR_HASH)) return;  // eslot is incomplete
+    const unsigned long long ebase = a.ctr->edge_base;
+    unsigned long long et = a.ctr->edge_total;
+    if (et > a.edge_cap) et = a.edge_cap;
+    unsigned long long n = et > ebase ? et - ebase : 0ull;
+    if (n > (unsigned long long)a.hmask + 1) n = (unsigned long long)a.hmask + 1;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) a.ctr->cand_total += a.ctr->ncand;
+    for (unsigned long long e = tid; e < n; e += (unsigned long long)gridDim.x * blockDim.x) {
+        const uint32_t h = a.eslot[e];
+        const unsigned long long key = a.hkeys[h];
+        a.edges[2 * (ebase + e)] = (uint32_t)(key >> 32);
+        a.edges[2 * (ebase + e) + 1] = a.hvals[h];
+        a.hkeys[h] = EMPTY64;
+        a.hvals[h] = NONE;
+    }
+}
+
+__global__ void k_consume(DevCounters *c)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->edge_total = 0;
+}
+
+// grow the bin grid: copy every old bin into its new slot
+__global__ void k_regrid(const uint32_t *onf, const uint32_t *onrs, const uint4 *ofr, const uint32_t *ors,
+                         uint32_t onbx, uint32_t onby, uint32_t *nnf, uint32_t *nnrs, uint4 *nfr, uint32_t *nrs,
+                         uint32_t nnbx, uint32_t dy, uint32_t fcap, uint32_t rcap)
+{
+    const uint32_t ob = blockIdx.x;
+    if (ob >= onbx * onby) return;
+    const uint32_t bx = ob % onbx, by = ob / onbx;
+    const uint32_t nbi = (by + dy) * nnbx + bx;
+    const uint32_t nf = onf[ob], nr = onrs[ob];
+    if (threadIdx.x == 0) { nnf[nbi] = nf; nnrs[nbi] = nr; }
+    for (uint32_t i = threadIdx.x; i < 2 * nf; i += blockDim.x) nfr[(size_t)nbi * fcap * 2 + i] = ofr[(size_t)ob * fcap * 2 + i];
+    for (uint32_t i = threadIdx.x; i < nr; i += blockDim.x) nrs[(size_t)nbi * rcap + i] = ors[(size_t)ob * rcap + i];
+}
+
+// ---------------------------------------------------------------- host side
+
+namespace {
+
+struct KClass {
+    int fcap, rcap, lcap;
+};
+const KClass KCLASSES[3] = {{32, 512, 512}, {128, 1024, 1024}, {512, 4096, 2048}};
+
+template <int F, int R, int L>
+size_t lds_bytes() { return sizeof(BinLds<F, R, L>); }
+
+int hip_err(hipError_t e)
+{
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorOutOfMemory) return -ENOMEM;
+    return -ENODEV;
+}
+
+#define HIPCHK(x) do { hipError_t _e = (x); if (_e != hipSuccess) return hip_err(_e); } while (0)
+
+uint32_t ilog2(uint64_t v)
+{
+    uint32_t r = 0;
+    while ((1ull << (r + 1)) <= v) ++r;
+    return r;
+}
+
+bool is_pow2(uint64_t v) { return v && !(v & (v - 1)); }
+
+} // namespace
+
+struct xkrt_hip_dep_domain {
+    uint64_t ld = 0, s = 0, LDs = 0;
+    xkrt_hip_dep_config_t cfg{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+    uint32_t bxs = 0, bys = 0, kclass = 0;
+    uint32_t fcap = 0, rcap = 0;
+    // frame
+    bool framed = false;
+    uint64_t ybase = 0;
+    uint32_t nbx = 0, nby = 0, ybin0 = 0;
+    // device state
+    uint32_t *bin_nf = nullptr, *bin_nrs = nullptr, *rs = nullptr;
+    uint4 *frag = nullptr;
+    uint32_t *bin_cnt = nullptr, *bin_cur = nullptr, *active = nullptr;
+    uint32_t *binrec = nullptr;
+    uint32_t binrec_cap = 0;
+    uint32_t *cand = nullptr;
+    uint32_t cand_cap = 0;
+    unsigned long long *hkeys = nullptr;
+    uint32_t *hvals = nullptr;
+    uint32_t hmask = 0;
+    uint32_t *eslot = nullptr;
+    uint32_t *edges = nullptr;
+    uint64_t edge_cap = 0;
+    DevCounters *ctr = nullptr;
+    DevCounters *ctr_host = nullptr;  // pinned
+    // host staging for xkrt_hip_dep_submit
+    uint32_t *rec_dev[6] = {};
+    uint32_t *rec_host[6] = {};
+    uint64_t rec_cap = 0;
+    hipEvent_t staged = nullptr;
+    bool staged_pending = false;
+    // host-side tracking
+    uint64_t next_id = 0;
+    uint64_t max_xlen = 0;
+    bool poisoned = false;
+    uint64_t n_acc = 0, n_rect = 0, n_launch = 0;
+    uint64_t bytes_state = 0, bytes_scratch = 0;
+};
+
+static uint64_t grid_bytes(uint64_t nbins, uint32_t fcap, uint32_t rcap)
+{
+    return nbins * (8ull + 32ull * fcap + 4ull * rcap + 8ull);
+}
+
+static int alloc_grid(xkrt_hip_dep_domain *d, uint32_t nbx, uint32_t nby, uint32_t **nf, uint32_t **nrs,
+                      uint4 **fr, uint32_t **rs, uint32_t **cnt, uint32_t **cur, uint32_t **act)
+{
+    const uint64_t nb = (uint64_t)nbx * nby;
+    HIPCHK(hipMalloc(nf, nb * 4));
+    HIPCHK(hipMalloc(nrs, nb * 4));
+    HIPCHK(hipMalloc(fr, nb * d->fcap * 32ull));
+    HIPCHK(hipMalloc(rs, nb * d->rcap * 4ull));
+    HIPCHK(hipMalloc(cnt, nb * 4));
+    HIPCHK(hipMalloc(cur, nb * 4));
+    HIPCHK(hipMalloc(act, nb * 4));
+    HIPCHK(hipMemsetAsync(*nf, 0, nb * 4, d->stream));
+    HIPCHK(hipMemsetAsync(*nrs, 0, nb * 4, d->stream));
+    HIPCHK(hipMemsetAsync(*cnt, 0, nb * 4, d->stream));
+    return 0;
+}
+
+static void free_grid(uint32_t *nf, uint32_t *nrs, uint4 *fr, uint32_t *rs, uint32_t *cnt, uint32_t *cur, uint32_t *act)
+{
+    (void)hipFree(nf); (void)hipFree(nrs); (void)hipFree(fr); (void)hipFree(rs);
+    (void)hipFree(cnt); (void)hipFree(cur); (void)hipFree(act);
+}
+
+// make the bin grid cover [ylo, yhi) (frame coordinates); grows by half the
+// current span in the direction of growth so repeated growth stays cheap
+static int ensure_grid(xkrt_hip_dep_domain *d, uint32_t ylo, uint32_t yhi)
+{
+    const uint64_t BY = 1ull << d->bys;
+    const uint64_t lo = (uint64_t)ylo & ~(BY - 1), hi = ((uint64_t)yhi + BY - 1) & ~(BY - 1);
+    const uint64_t olo = d->ybin0, ohi = (uint64_t)d->ybin0 + (uint64_t)d->nby * BY;
+    if (d->nby && lo >= olo && hi <= ohi) return 0;
+    uint64_t nlo = lo, nhi = hi;
+    if (d->nby) {
+        const uint64_t extra = ((ohi - olo) / 2) & ~(BY - 1);
+        nlo = std::min(lo, olo);
+        nhi = std::max(hi, ohi);
+        if (lo < olo) nlo = nlo > extra ? nlo - extra : 0;
+        if (hi > ohi) nhi = std::min<uint64_t>(nhi + extra, (1ull << 32) - BY);
+    }
+    const uint32_t rows = (uint32_t)((nhi - nlo) / BY);
+    const uint64_t nb_new = (uint64_t)d->nbx * rows;
+    const uint64_t new_bytes = grid_bytes(nb_new, d->fcap, d->rcap);
+    if (d->cfg.budget_bytes && new_bytes + d->bytes_scratch > d->cfg.budget_bytes) return -ENOMEM;
+    if (nb_new > 0xFFFFFFFFull) return -ERANGE;
+    uint32_t *nf, *nrs, *rs, *cnt, *cur, *act;
+    uint4 *fr;
+    int rc = alloc_grid(d, d->nbx, rows, &nf, &nrs, &fr, &rs, &cnt, &cur, &act);
+    if (rc) return rc;
+    if (d->nby) {
+        const uint32_t dy = (uint32_t)((olo - nlo) / BY);
+        hipLaunchKernelGGL(k_regrid, dim3(d->nbx * d->nby), dim3(256), 0, d->stream, d->bin_nf, d->bin_nrs,
+                           d->frag, d->rs, d->nbx, d->nby, nf, nrs, fr, rs, d->nbx, dy, d->fcap, d->rcap);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(d->stream));
+        free_grid(d->bin_nf, d->bin_nrs, d->frag, d->rs, d->bin_cnt, d->bin_cur, d->active);
+    }
+    d->bin_nf = nf; d->bin_nrs = nrs; d->frag = fr; d->rs = rs;
+    d->bin_cnt = cnt; d->bin_cur = cur; d->active = act;
+    d->ybin0 = (uint32_t)nlo;
+    d->nby = rows;
+    d->bytes_state = new_bytes;
+    return 0;
+}
+
+static KArgs make_args(xkrt_hip_dep_domain *d, const xkrt_hip_rect_soa_t *r, uint32_t nrec)
+{
+    KArgs a;
+    a.rx0 = r->x0; a.rx1 = r->x1; a.ry0 = r->y0; a.ry1 = r->y1; a.racc = r->acc; a.rtm = r->tm;
+    a.nrec = nrec;
+    a.bxs = d->bxs; a.bys = d->bys; a.nbx = d->nbx; a.nby = d->nby; a.ybin0 = d->ybin0;
+    a.xmax = (uint32_t)d->LDs;
+    a.bin_nf = d->bin_nf; a.bin_nrs = d->bin_nrs; a.frag = d->frag; a.rs = d->rs;
+    a.fcap = d->fcap; a.rcap = d->rcap;
+    a.bin_cnt = d->bin_cnt; a.bin_cur = d->bin_cur; a.active = d->active; a.binrec = d->binrec;
+    a.binrec_cap = d->binrec_cap;
+    a.cand = d->cand; a.cand_cap = d->cand_cap;
+    a.hkeys = d->hkeys; a.hvals = d->hvals; a.hmask = d->hmask;
+    a.eslot = d->eslot; a.edges = d->edges; a.edge_cap = d->edge_cap;
+    a.ctr = d->ctr;
+    return a;
+}
+
+template <int F, int R, int L>
+static int launch_resolve(xkrt_hip_dep_domain *d, const KArgs &a, uint32_t nbins)
+{
+    const size_t lds = lds_bytes<F, R, L>();
+    static bool attr_set = false;
+    if (!attr_set) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_resolve<F, R, L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        attr_set = true;
+    }
+    const uint32_t blocks = std::min<uint32_t>(nbins, 4096u);
+    hipLaunchKernelGGL((k_resolve<F, R, L>), dim3(blocks), dim3(64), lds, d->stream, a);
+    return hip_err(hipGetLastError());
+}
+
+static int enqueue(xkrt_hip_dep_domain *d, const xkrt_hip_rect_soa_t *rec, uint32_t nrec)
+{
+    if (nrec == 0) return 0;
+    const KArgs a = make_args(d, rec, nrec);
+    const uint32_t nbins = d->nbx * d->nby;
+    const uint32_t rblocks = std::min<uint32_t>((nrec + 255) / 256, 8192u);
+    hipLaunchKernelGGL(k_count, dim3(rblocks), dim3(256), 0, d->stream, a);
+    hipLaunchKernelGGL(k_alloc, dim3((nbins + 255) / 256), dim3(256), 0, d->stream, a, nbins);
+    hipLaunchKernelGGL(k_fill, dim3(rblocks), dim3(256), 0, d->stream, a);
+    int rc = 0;
+    switch (d->kclass) {
+    case 0: rc = launch_resolve<32, 512, 512>(d, a, nbins); break;
+    case 1: rc = launch_resolve<128, 1024, 1024>(d, a, nbins); break;
+    default: rc = launch_resolve<512, 4096, 2048>(d, a, nbins); break;
+    }
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dedup, dim3(1024), dim3(256), 0, d->stream, a);
+    hipLaunchKernelGGL(k_final, dim3(1024), dim3(256), 0, d->stream, a);
+    HIPCHK(hipGetLastError());
+    d->n_launch++;
+    return 0;
+}
+
+static int err_to_errno(uint32_t e)
+{
+    if (e & (ERR_LIST | ERR_FRAG | ERR_READS | ERR_BINREC)) return -ENOSPC;
+    if (e & (ERR_CAND | ERR_HASH | ERR_EDGES)) return -EOVERFLOW;
+    return 0;
+}
+
+extern "C" {
+
+int xkrt_hip_init(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device) return -ENODEV;
+    return hip_err(hipSetDevice(device));
+}
+
+int xkrt_hip_dep_domain_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_dep_config_t *cfg, void *stream,
+                               xkrt_hip_dep_domain_t *out)
+{
+    if (!out || ld == 0 || sizeof_type == 0) return -EINVAL;
+    const uint64_t LDs = ld * sizeof_type;
+    if (LDs > 0xFFFFFFFFull) return -ERANGE;
+    xkrt_hip_dep_domain *d = new xkrt_hip_dep_domain();
+    d->ld = ld; d->s = sizeof_type; d->LDs = LDs;
+    if (cfg) d->cfg = *cfg;
+    uint32_t BX = d->cfg.bin_x_bytes ? d->cfg.bin_x_bytes : 1024u;
+    uint32_t BY = d->cfg.bin_y_cols ? d->cfg.bin_y_cols : 128u;
+    if (!is_pow2(BX) || !is_pow2(BY) || d->cfg.kernel > 2) { delete d; return -EINVAL; }
+    d->bxs = ilog2(BX); d->bys = ilog2(BY);
+    d->kclass = d->cfg.kernel;
+    d->fcap = KCLASSES[d->kclass].fcap;
+    d->rcap = KCLASSES[d->kclass].rcap;
+    d->nbx = (uint32_t)((LDs + BX - 1) / BX);
+    if (hipGetDevice(&d->device) != hipSuccess) { delete d; return -ENODEV; }
+    if (stream) {
+        d->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) { delete d; return -ENODEV; }
+        d->own_stream = true;
+    }
+    const uint64_t max_rects = d->cfg.max_rects ? d->cfg.max_rects : (1ull << 20);
+    d->cfg.max_rects = max_rects;
+    d->binrec_cap = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, std::max<uint64_t>(4 * max_rects, 1 << 16));
+    d->cand_cap = (uint32_t)std::min<uint64_t>(0x3FFFFFF0ull, std::max<uint64_t>(8 * max_rects, 1 << 16));
+    uint64_t hsz = 1;
+    while (hsz < 2ull * d->cand_cap) hsz <<= 1;
+    d->hmask = (uint32_t)(hsz - 1);
+    d->edge_cap = d->cfg.max_edges ? d->cfg.max_edges : std::max<uint64_t>(16 * max_rects, 1 << 20);
+    d->bytes_scratch = 4ull * d->binrec_cap + 12ull * d->cand_cap + 12ull * hsz + 4ull * hsz + 8ull * d->edge_cap +
+                       sizeof(DevCounters);
+    int rc = 0;
+    auto fail = [&](int e) { rc = e; };
+    if (d->cfg.budget_bytes && d->bytes_scratch > d->cfg.budget_bytes) fail(-ENOMEM);
+    if (!rc) fail(hip_err(hipMalloc(&d->binrec, 4ull * d->binrec_cap)));
+    if (!rc) fail(hip_err(hipMalloc(&d->cand, 12ull * d->cand_cap)));
+    if (!rc) fail(hip_err(hipMalloc(&d->hkeys, 8ull * hsz)));
+    if (!rc) fail(hip_err(hipMalloc(&d->hvals, 4ull * hsz)));
+    if (!rc) fail(hip_err(hipMalloc(&d->eslot, 4ull * hsz)));
+    if (!rc) fail(hip_err(hipMalloc(&d->edges, 8ull * d->edge_cap)));
+    if (!rc) fail(hip_err(hipMalloc(&d->ctr, sizeof(DevCounters))));
+    if (!rc) fail(hip_err(hipHostMalloc(&d->ctr_host, sizeof(DevCounters), hipHostMallocDefault)));
+    if (!rc) fail(hip_err(hipMemsetAsync(d->hkeys, 0xFF, 8ull * hsz, d->stream)));
+    if (!rc) fail(hip_err(hipMemsetAsync(d->hvals, 0xFF, 4ull * hsz, d->stream)));
+    if (!rc) fail(hip_err(hipMemsetAsync(d->ctr, 0, sizeof(DevCounters), d->stream)));
+    if (!rc) fail(hip_err(hipEventCreateWithFlags(&d->staged, hipEventDisableTiming)));
+    if (!rc && d->cfg.y_hint_lo < d->cfg.y_hint_hi) {
+        d->ybase = d->cfg.y_hint_lo;
+        d->framed = true;
+        const uint64_t span = d->cfg.y_hint_hi - d->cfg.y_hint_lo;
+        if (span >= (1ull << 32)) fail(-ERANGE);
+        else fail(ensure_grid(d, 0, (uint32_t)span));
+    }
+    if (!rc) fail(hip_err(hipStreamSynchronize(d->stream)));
+    if (rc) {
+        xkrt_hip_dep_domain_destroy(d);
+        return rc;
+    }
+    *out = d;
+    return 0;
+}
+
+int xkrt_hip_dep_domain_destroy(xkrt_hip_dep_domain_t d)
+{
+    if (!d) return -EINVAL;
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    if (d->nby) free_grid(d->bin_nf, d->bin_nrs, d->frag, d->rs, d->bin_cnt, d->bin_cur, d->active);
+    (void)hipFree(d->binrec); (void)hipFree(d->cand); (void)hipFree(d->hkeys); (void)hipFree(d->hvals);
+    (void)hipFree(d->eslot); (void)hipFree(d->edges); (void)hipFree(d->ctr);
+    if (d->ctr_host) (void)hipHostFree(d->ctr_host);
+    for (int i = 0; i < 6; ++i) {
+        if (d->rec_dev[i]) (void)hipFree(d->rec_dev[i]);
+        if (d->rec_host[i]) (void)hipHostFree(d->rec_host[i]);
+    }
+    if (d->staged) (void)hipEventDestroy(d->staged);
+    if (d->own_stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+    return 0;
+}
+
+int xkrt_hip_dep_frame(xkrt_hip_dep_domain_t d, uint64_t *ybase)
+{
+    if (!d || !ybase) return -EINVAL;
+    *ybase = d->ybase;
+    return d->framed ? 0 : -EINVAL;
+}
+
+int xkrt_hip_dep_build_records(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n, uint32_t first_id,
+                               uint64_t cap, uint32_t *x0, uint32_t *x1, uint32_t *y0, uint32_t *y1, uint32_t *accid,
+                               uint32_t *tm, uint64_t *nrect, uint32_t *ylo, uint32_t *yhi)
+{
+    if (!d || (!acc && n) || !nrect) return -EINVAL;
+    const uint64_t LDs = d->LDs, s = d->s;
+    // frame: fixed by the first access ever seen
+    if (!d->framed && n) {
+        uint64_t ymin = UINT64_MAX;
+        for (uint64_t i = 0; i < n; ++i) ymin = std::min(ymin, acc[i].addr / LDs);
+        d->ybase = ymin > (1ull << 30) ? ymin - (1ull << 30) : 0;
+        d->framed = true;
+    }
+    uint64_t k = 0, lo = UINT64_MAX, hi = 0;
+    uint32_t tl = 0;
+    uint64_t maxx = d->max_xlen;
+    for (uint64_t i = 0; i < n; ++i) {
+        const xkrt_hip_access_t &q = acc[i];
+        if (i && q.task != acc[i - 1].task) {
+            if (q.task < acc[i - 1].task) return -EINVAL;
+            tl++;
+        }
+        if (q.m == 0 || q.n == 0) return -EINVAL;
+        if (tl > 0xFFFFFFu) return -ERANGE;
+        // access.hpp:205-287 matrix_to_rects
+        const uint64_t X0 = q.addr % LDs, Y0 = q.addr / LDs, ms = q.m * s;
+        uint64_t r[2][4];
+        int nr;
+        if (X0 + ms <= LDs) {
+            r[0][0] = X0; r[0][1] = X0 + ms; r[0][2] = Y0; r[0][3] = Y0 + q.n;
+            nr = 1;
+        } else {
+            if (ms > LDs) return -EINVAL;
+            r[0][0] = X0; r[0][1] = LDs; r[0][2] = Y0; r[0][3] = Y0 + q.n;
+            r[1][0] = 0; r[1][1] = ms - (LDs - X0); r[1][2] = Y0 + 1; r[1][3] = Y0 + q.n + 1;
+            nr = 2;
+            // a prior rect spanning [x3 - 1, x0] in dim 0 makes the reference
+            // mark split fragments beyond the tile
+            if (maxx > X0 - r[1][1]) return -ENOTSUP;
+        }
+        for (int j = 0; j < nr; ++j) {
+            if (r[j][2] < d->ybase || r[j][3] - d->ybase > 0xFFFFFFFFull) return -ERANGE;
+            maxx = std::max(maxx, r[j][1] - r[j][0]);
+            if (k < cap) {
+                x0[k] = (uint32_t)r[j][0]; x1[k] = (uint32_t)r[j][1];
+                y0[k] = (uint32_t)(r[j][2] - d->ybase); y1[k] = (uint32_t)(r[j][3] - d->ybase);
+                accid[k] = first_id + (uint32_t)i;
+                tm[k] = (tl << 8) | q.mode;
+            }
+            lo = std::min(lo, r[j][2] - d->ybase);
+            hi = std::max(hi, r[j][3] - d->ybase);
+            ++k;
+        }
+    }
+    d->max_xlen = maxx;
+    *nrect = k;
+    if (ylo) *ylo = n ? (uint32_t)lo : 0;
+    if (yhi) *yhi = n ? (uint32_t)hi : 0;
+    return 0;
+}
+
+int xkrt_hip_dep_submit_device(xkrt_hip_dep_domain_t d, const xkrt_hip_rect_soa_t *rec, uint64_t nrect,
+                               uint64_t naccess, uint32_t ylo, uint32_t yhi)
+{
+    if (!d || !rec) return -EINVAL;
+    if (d->poisoned) return -EINVAL;
+    if (nrect == 0) return 0;
+    if (nrect > d->cfg.max_rects) return -EINVAL;
+    if (!d->framed) d->framed = true;  // device records are already in the frame (ybase 0 unless hinted)
+    int rc = ensure_grid(d, ylo, yhi);
+    if (rc) return rc;
+    rc = enqueue(d, rec, (uint32_t)nrect);
+    if (rc) { d->poisoned = true; return rc; }
+    d->n_acc += naccess;
+    d->n_rect += nrect;
+    return 0;
+}
+
+int xkrt_hip_dep_submit(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n, uint64_t *first_id)
+{
+    if (!d || (!acc && n)) return -EINVAL;
+    if (d->poisoned) return -EINVAL;
+    if (first_id) *first_id = d->next_id;
+    if (n == 0) return 0;
+    if (n > 0xFFFFFFFFull || d->next_id + n > 0xFFFFFFFFull) return -ERANGE;
+    const uint64_t need = 2 * n;
+    if (need > d->cfg.max_rects) return -EINVAL;
+    if (need > d->rec_cap) {
+        for (int i = 0; i < 6; ++i) {
+            if (d->rec_dev[i]) (void)hipFree(d->rec_dev[i]);
+            if (d->rec_host[i]) (void)hipHostFree(d->rec_host[i]);
+            d->rec_dev[i] = d->rec_host[i] = nullptr;
+        }
+        d->rec_cap = 0;
+        for (int i = 0; i < 6; ++i) {
+            HIPCHK(hipMalloc(&d->rec_dev[i], 4 * need));
+            HIPCHK(hipHostMalloc(&d->rec_host[i], 4 * need, hipHostMallocDefault));
+        }
+        d->rec_cap = need;
+    }
+    // the staging buffers must be free: wait for the previous copy
+    if (d->staged_pending) {
+        HIPCHK(hipEventSynchronize(d->staged));
+        d->staged_pending = false;
+    }
+    uint64_t nrect = 0;
+    uint32_t ylo = 0, yhi = 0;
+    const uint64_t saved_max = d->max_xlen;
+    const bool was_framed = d->framed;
+    int rc = xkrt_hip_dep_build_records(d, acc, n, (uint32_t)d->next_id, d->rec_cap, d->rec_host[0], d->rec_host[1],
+                                        d->rec_host[2], d->rec_host[3], d->rec_host[4], d->rec_host[5], &nrect, &ylo,
+                                        &yhi);
+    if (rc) {
+        d->max_xlen = saved_max;
+        if (!was_framed) d->framed = false;
+        return rc;
+    }
+    rc = ensure_grid(d, ylo, yhi);
+    if (rc) return rc;
+    for (int i = 0; i < 6; ++i)
+        HIPCHK(hipMemcpyAsync(d->rec_dev[i], d->rec_host[i], 4 * nrect, hipMemcpyHostToDevice, d->stream));
+    HIPCHK(hipEventRecord(d->staged, d->stream));
+    d->staged_pending = true;
+    xkrt_hip_rect_soa_t soa = {d->rec_dev[0], d->rec_dev[1], d->rec_dev[2], d->rec_dev[3], d->rec_dev[4], d->rec_dev[5]};
+    rc = enqueue(d, &soa, (uint32_t)nrect);
+    if (rc) { d->poisoned = true; return rc; }
+    d->next_id += n;
+    d->n_acc += n;
+    d->n_rect += nrect;
+    return 0;
+}
+
+static int read_counters(xkrt_hip_dep_domain_t d)
+{
+    HIPCHK(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    if (d->ctr_host->err) {
+        d->poisoned = true;
+        return err_to_errno(d->ctr_host->err);
+    }
+    return 0;
+}
+
+int xkrt_hip_dep_sync(xkrt_hip_dep_domain_t d, uint64_t *nedges)
+{
+    if (!d) return -EINVAL;
+    int rc = read_counters(d);
+    if (nedges) *nedges = d->ctr_host->edge_total;
+    return rc;
+}
+
+int xkrt_hip_dep_edges_device(xkrt_hip_dep_domain_t d, const uint32_t **edges, uint64_t *n)
+{
+    if (!d || !edges || !n) return -EINVAL;
+    int rc = read_counters(d);
+    *edges = d->edges;
+    *n = d->ctr_host->edge_total;
+    return rc;
+}
+
+int xkrt_hip_dep_edges_consume(xkrt_hip_dep_domain_t d)
+{
+    if (!d) return -EINVAL;
+    hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, d->stream, d->ctr);
+    return hip_err(hipGetLastError());
+}
+
+int xkrt_hip_dep_edges_copy(xkrt_hip_dep_domain_t d, uint32_t *host_edges, uint64_t cap, uint64_t *n)
+{
+    if (!d || !n || (cap && !host_edges)) return -EINVAL;
+    int rc = read_counters(d);
+    const uint64_t ne = d->ctr_host->edge_total;
+    *n = ne;
+    if (rc) return rc;
+    const uint64_t m = std::min(ne, cap);
+    if (m) {
+        HIPCHK(hipMemcpyAsync(host_edges, d->edges, 8 * m, hipMemcpyDeviceToHost, d->stream));
+        HIPCHK(hipStreamSynchronize(d->stream));
+    }
+    return xkrt_hip_dep_edges_consume(d);
+}
+
+int xkrt_hip_dep_stats(xkrt_hip_dep_domain_t d, xkrt_hip_dep_stats_t *out)
+{
+    if (!d || !out) return -EINVAL;
+    int rc = read_counters(d);
+    const DevCounters &c = *d->ctr_host;
+    out->accesses = d->n_acc;
+    out->rects = d->n_rect;
+    out->edges = c.edge_total;
+    out->candidates = c.cand_total;
+    out->launches = d->n_launch;
+    out->bins = (uint64_t)d->nbx * d->nby;
+    out->max_frags = c.maxfrag;
+    out->max_reads = c.maxrs;
+    out->frag_visits = c.frag_visits;
+    out->device_bytes = d->bytes_state + d->bytes_scratch;
+    return rc;
+}
+
+void *xkrt_hip_dep_stream(xkrt_hip_dep_domain_t d) { return d ? (void *)d->stream : nullptr; }
+
+} // extern "C"

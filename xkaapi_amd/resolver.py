@@ -1,0 +1,233 @@
+"""Host-side binding of the MI355X dependency resolver (libxkrt_hip.so).
+
+Mirrors the reference's BLAS dependency domain
+(include/xkrt/memory/access/blas/dependency-tree.hpp:188-428): one domain per
+``(ld, sizeof_type)`` (src/task/task.cc:153-201), accesses LINKed then PUT per
+task (src/task/task.cc:285-299), edges ``pred -> succ`` with one entry per
+(pred access, succ task) at the lowest succ slot (include/xkrt/task/task.hpp:494-512).
+
+There is no CPU fallback: if the HIP library is missing the constructor
+raises, and every device error is raised as ``ResolverError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import errno
+import os
+
+import numpy as np
+
+from ._loader import LibraryMissing, lib_path
+
+_u64, _u32, _p = C.c_uint64, C.c_uint32, C.c_void_p
+
+
+class AccessT(C.Structure):
+    _fields_ = [("addr", C.c_uint64), ("m", C.c_uint64), ("n", C.c_uint64), ("task", C.c_uint32),
+                ("mode", C.c_uint8), ("pad", C.c_uint8 * 3)]
+
+
+ACCESS_DTYPE = np.dtype([("addr", "<u8"), ("m", "<u8"), ("n", "<u8"), ("task", "<u4"), ("mode", "u1"),
+                         ("pad", "u1", (3,))])
+assert ACCESS_DTYPE.itemsize == C.sizeof(AccessT) == 32
+
+
+class ConfigT(C.Structure):
+    _fields_ = [("bin_x_bytes", C.c_uint32), ("bin_y_cols", C.c_uint32), ("kernel", C.c_uint32),
+                ("pad", C.c_uint32), ("max_rects", C.c_uint64), ("max_edges", C.c_uint64),
+                ("budget_bytes", C.c_uint64), ("y_hint_lo", C.c_uint64), ("y_hint_hi", C.c_uint64)]
+
+
+class StatsT(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("accesses", "rects", "edges", "candidates", "launches", "bins",
+                                          "max_frags", "max_reads", "frag_visits", "device_bytes")]
+
+
+class RectSoaT(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("x0", "x1", "y0", "y1", "acc", "tm")]
+
+
+class ResolverError(RuntimeError):
+    def __init__(self, rc: int, what: str):
+        name = errno.errorcode.get(-rc, str(rc))
+        super().__init__(f"{what}: {name} ({rc})")
+        self.rc = rc
+
+
+_LIB = None
+
+
+def hip_lib():
+    """Load libxkrt_hip.so (after torch, so both share torch's HIP runtime)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    try:
+        import torch  # noqa: F401  -- one HIP runtime per process
+    except Exception:
+        pass
+    path = lib_path("libxkrt_hip.so")
+    if not os.path.exists(path):
+        raise LibraryMissing(f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(path)
+    L.xkrt_hip_init.argtypes = [C.c_int]
+    L.xkrt_hip_dep_domain_create.argtypes = [_u64, _u64, C.POINTER(ConfigT), _p, C.POINTER(_p)]
+    L.xkrt_hip_dep_domain_destroy.argtypes = [_p]
+    L.xkrt_hip_dep_submit.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_dep_submit_device.argtypes = [_p, C.POINTER(RectSoaT), _u64, _u64, _u32, _u32]
+    L.xkrt_hip_dep_build_records.argtypes = [_p, _p, _u64, _u32, _u64, _p, _p, _p, _p, _p, _p,
+                                             C.POINTER(_u64), C.POINTER(_u32), C.POINTER(_u32)]
+    L.xkrt_hip_dep_frame.argtypes = [_p, C.POINTER(_u64)]
+    L.xkrt_hip_dep_sync.argtypes = [_p, C.POINTER(_u64)]
+    L.xkrt_hip_dep_edges_device.argtypes = [_p, C.POINTER(_p), C.POINTER(_u64)]
+    L.xkrt_hip_dep_edges_copy.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_dep_edges_consume.argtypes = [_p]
+    L.xkrt_hip_dep_stats.argtypes = [_p, C.POINTER(StatsT)]
+    L.xkrt_hip_dep_stream.argtypes = [_p]
+    L.xkrt_hip_dep_stream.restype = _p
+    _LIB = L
+    return L
+
+
+def _chk(rc: int, what: str):
+    if rc != 0:
+        raise ResolverError(rc, what)
+
+
+def to_access_array(task, mode, addr, m, n) -> np.ndarray:
+    a = np.zeros(len(task), ACCESS_DTYPE)
+    a["task"] = task
+    a["mode"] = mode
+    a["addr"] = addr
+    a["m"] = m
+    a["n"] = n
+    return a
+
+
+class DependencyDomain:
+    """A GPU-resident BLAS dependency domain for one (ld, sizeof_type)."""
+
+    def __init__(self, ld: int, sizeof_type: int, bin_x_bytes: int = 1024, bin_y_cols: int = 128,
+                 kernel: int = 0, max_rects: int = 1 << 20, max_edges: int = 0, budget_bytes: int = 0,
+                 y_hint: tuple[int, int] | None = None, stream: int | None = None, device: int = 0):
+        self.L = hip_lib()
+        _chk(self.L.xkrt_hip_init(device), "xkrt_hip_init")
+        cfg = ConfigT(bin_x_bytes, bin_y_cols, kernel, 0, max_rects, max_edges, budget_bytes,
+                      y_hint[0] if y_hint else 0, y_hint[1] if y_hint else 0)
+        h = _p()
+        _chk(self.L.xkrt_hip_dep_domain_create(ld, sizeof_type, C.byref(cfg), stream, C.byref(h)),
+             "xkrt_hip_dep_domain_create")
+        self.h = h
+        self.ld, self.s = ld, sizeof_type
+        self.max_rects = max_rects
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.xkrt_hip_dep_domain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def stream(self) -> int:
+        return int(self.L.xkrt_hip_dep_stream(self.h) or 0)
+
+    def submit(self, acc: np.ndarray) -> int:
+        """Enqueue a batch of accesses (ACCESS_DTYPE); returns the first access id."""
+        acc = np.ascontiguousarray(acc, ACCESS_DTYPE)
+        first = _u64()
+        _chk(self.L.xkrt_hip_dep_submit(self.h, acc.ctypes.data, len(acc), C.byref(first)), "xkrt_hip_dep_submit")
+        return int(first.value)
+
+    def build_records(self, acc: np.ndarray, first_id: int):
+        """Host conversion to device-frame rect records: (x0,x1,y0,y1,acc,tm), ylo, yhi."""
+        acc = np.ascontiguousarray(acc, ACCESS_DTYPE)
+        cap = 2 * len(acc)
+        out = [np.empty(cap, np.uint32) for _ in range(6)]
+        nrect, ylo, yhi = _u64(), _u32(), _u32()
+        _chk(self.L.xkrt_hip_dep_build_records(self.h, acc.ctypes.data, len(acc), first_id, cap,
+                                               *[o.ctypes.data for o in out], C.byref(nrect), C.byref(ylo),
+                                               C.byref(yhi)), "xkrt_hip_dep_build_records")
+        k = int(nrect.value)
+        return [o[:k] for o in out], int(ylo.value), int(yhi.value)
+
+    def submit_device(self, ptrs, nrect: int, naccess: int, ylo: int, yhi: int):
+        """Enqueue device-resident rect records (six device pointers)."""
+        soa = RectSoaT(*[int(p) for p in ptrs])
+        _chk(self.L.xkrt_hip_dep_submit_device(self.h, C.byref(soa), nrect, naccess, ylo, yhi),
+             "xkrt_hip_dep_submit_device")
+
+    def sync(self) -> int:
+        n = _u64()
+        _chk(self.L.xkrt_hip_dep_sync(self.h, C.byref(n)), "xkrt_hip_dep_sync")
+        return int(n.value)
+
+    def edges(self) -> np.ndarray:
+        """Copy and consume every unconsumed edge: (E, 2) uint32 (pred id, succ id)."""
+        n = self.sync()
+        out = np.empty((max(n, 1), 2), np.uint32)
+        got = _u64()
+        _chk(self.L.xkrt_hip_dep_edges_copy(self.h, out.ctypes.data, n, C.byref(got)), "xkrt_hip_dep_edges_copy")
+        return out[:n]
+
+    def edges_device(self):
+        ptr, n = _p(), _u64()
+        _chk(self.L.xkrt_hip_dep_edges_device(self.h, C.byref(ptr), C.byref(n)), "xkrt_hip_dep_edges_device")
+        return int(ptr.value or 0), int(n.value)
+
+    def consume(self):
+        _chk(self.L.xkrt_hip_dep_edges_consume(self.h), "xkrt_hip_dep_edges_consume")
+
+    def stats(self) -> dict:
+        st = StatsT()
+        _chk(self.L.xkrt_hip_dep_stats(self.h, C.byref(st)), "xkrt_hip_dep_stats")
+        return {f: int(getattr(st, f)) for f, _ in StatsT._fields_}
+
+
+def batches_by_tasks(task: np.ndarray, tasks_per_batch) -> list[tuple[int, int]]:
+    """Split access indices into batches of whole tasks.  tasks_per_batch is an
+    int (fixed) or an array of batch sizes in tasks."""
+    starts = np.flatnonzero(np.r_[True, np.diff(task.astype(np.int64)) != 0])
+    ends = np.r_[starts[1:], len(task)]
+    out = []
+    if np.isscalar(tasks_per_batch):
+        sizes = []
+        t = 0
+        while t < len(starts):
+            sizes.append(min(int(tasks_per_batch), len(starts) - t))
+            t += sizes[-1]
+    else:
+        sizes = [int(x) for x in tasks_per_batch]
+    t = 0
+    for b in sizes:
+        if t >= len(starts):
+            break
+        e = min(t + b, len(starts))
+        out.append((int(starts[t]), int(ends[e - 1])))
+        t = e
+    return out
+
+
+def resolve_stream(stream, tasks_per_batch=4096, **domain_kw) -> np.ndarray:
+    """Resolve a whole single-domain stream on the GPU, one launch per batch;
+    returns (pred, succ) access-index edges (stream positions)."""
+    ld, s = int(stream.ld[0]), int(stream.s[0])
+    acc = to_access_array(stream.task, stream.mode, stream.addr, stream.m, stream.n)
+    with DependencyDomain(ld, s, **domain_kw) as dom:
+        first = None
+        for (a, b) in batches_by_tasks(stream.task, tasks_per_batch):
+            f = dom.submit(acc[a:b])
+            if first is None:
+                first = f
+        e = dom.edges()
+    return e.astype(np.int64) - (first or 0)
