@@ -1,0 +1,210 @@
+#!/usr/bin/env python
+"""Resolver benchmark on one MI355X.
+
+Workload: the plain-GPU configuration -- (a) tiled Cholesky on an 8192^2 fp64
+matrix in 128^2 tiles (45,760 tasks, 133,120 accesses) and (b) 1,000,000
+seeded random single-access tasks over a 65536^2 fp32 index space -- both in
+submission batches of 4096 tasks.  Synthetic streams, generated in-repo.
+
+One step = one pass of the resolver over both streams from an empty domain,
+with the rect records already resident in HBM.  Two forms are timed:
+  plain   : one launch (count/alloc/fill/resolve/dedup/final) per 4096-task batch
+  batched : consecutive batches merged into one launch (up to --merge batches)
+`value` is the plain form's accesses/sec (whole job); the batched form, the
+per-batch latency percentiles, the roofline of the dominant kernel and the
+reference CPU resolver's rate are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def weighted_pct(values, weights, q):
+    o = np.argsort(values)
+    v = np.asarray(values)[o]
+    w = np.asarray(weights, np.float64)[o]
+    c = np.cumsum(w) / w.sum()
+    return float(v[min(np.searchsorted(c, q / 100.0), len(v) - 1)])
+
+
+class Prepared:
+    """Device-resident rect records of one stream, per plain batch and per merged group."""
+
+    def __init__(self, name, st, dom_kw, merge, torch):
+        from xkaapi_amd import resolver as R
+        self.name, self.st = name, st
+        ld, s = int(st.ld[0]), int(st.s[0])
+        self.acc = R.to_access_array(st.task, st.mode, st.addr, st.m, st.n)
+        self.batches = R.batches_by_tasks(st.task, 4096)
+        self.groups = [(self.batches[i][0], self.batches[min(i + merge, len(self.batches)) - 1][1])
+                       for i in range(0, len(self.batches), merge)]
+        maxrec = 2 * max(b - a for a, b in self.groups)
+        self.dom = R.DependencyDomain(ld, s, max_rects=maxrec, max_edges=8 * len(st) + (1 << 20),
+                                      stream=torch.cuda.current_stream().cuda_stream, **dom_kw)
+        self.plain = [self._upload(a, b, torch) for a, b in self.batches]
+        self.merged = [self._upload(a, b, torch) for a, b in self.groups]
+        self.ntasks_plain = [int(np.count_nonzero(np.diff(st.task[a:b].astype(np.int64))) + 1) for a, b in self.batches]
+
+    def _upload(self, a, b, torch):
+        recs, ylo, yhi = self.dom.build_records(self.acc[a:b], a)
+        dev = [torch.from_numpy(r.view(np.int32)).cuda() for r in recs]
+        return dev, len(recs[0]), b - a, ylo, yhi
+
+    def run(self, which):
+        self.dom.reset()
+        for dev, nrec, nacc, ylo, yhi in (self.plain if which == "plain" else self.merged):
+            self.dom.submit_device([t.data_ptr() for t in dev], nrec, nacc, ylo, yhi)
+
+
+def cpu_baseline(st, target_s=10.0):
+    """Reference KHP engine (oracle/_ref, built from the reference sources) on
+    the Cholesky stream, repeated to about target_s seconds on one core."""
+    from oracle import refkhp
+    if not refkhp.available():
+        return None
+    r, nr = st.rects()
+    tot_ns, reps = 0, 0
+    while tot_ns < target_s * 1e9 and reps < 1000:
+        ns, ne = refkhp.time_resolve(st.task, st.mode, r, nr)
+        tot_ns += ns
+        reps += 1
+    return {"value": reps * len(st) / (tot_ns / 1e9), "unit": "accesses/s", "cores": 1, "kind": "reference",
+            "sample": f"Cholesky 8192^2 nb128 stream ({len(st)} accesses) x{reps}, LINK+PUT only, "
+                      f"reference khp-tree.hpp engine, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--random-n", type=int, default=1_000_000)
+    ap.add_argument("--merge", type=int, default=16, help="batches per launch in the batched form")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (run --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    from xkaapi_amd import streams as S
+
+    t0 = time.time()
+    chol = S.cholesky(8192, 128)
+    rnd = S.random2d(args.random_n)
+    log(f"streams: cholesky {len(chol)} acc / {chol.ntasks} tasks, random {len(rnd)} acc ({time.time()-t0:.1f}s)")
+    preps = [Prepared("cholesky_8192_nb128", chol, dict(bin_x_bytes=1024, bin_y_cols=128, kernel=0), args.merge, torch),
+             Prepared("random2d_65536", rnd, dict(bin_x_bytes=2048, bin_y_cols=256, kernel=1), args.merge, torch)]
+    torch.cuda.synchronize()
+    log(f"records resident in HBM ({time.time()-t0:.1f}s)")
+
+    # correctness gate on the Cholesky stream: the closed-form edge count
+    p0 = preps[0]
+    p0.run("plain")
+    assert p0.dom.sync() == 131040, "Cholesky 8192/128 must have 131,040 edges"
+    p0.run("batched")
+    assert p0.dom.sync() == 131040
+
+    for _ in range(args.warmup):
+        for p in preps:
+            p.run("plain")
+            p.run("batched")
+    torch.cuda.synchronize()
+    for p in preps:
+        p.dom.sync()
+        p.dom.kernel_times(clear=True)
+        p.dom.launch_times(clear=True)
+        p.dom.set_timing(True)
+
+    nacc = sum(len(p.st) for p in preps)
+    res = {}
+    for form in ("plain", "batched"):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            for p in preps:
+                p.run(form)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t1) / args.steps
+        kt = {p.name: p.dom.kernel_times(clear=True) for p in preps}
+        lt = {p.name: p.dom.launch_times(clear=True) for p in preps}
+        edges = {p.name: p.dom.sync() for p in preps}
+        res[form] = dict(ms=dt * 1e3, rate=nacc / dt, kt=kt, lt=lt, edges=edges)
+        log(f"{form}: {dt*1e3:.2f} ms/step, {nacc/dt:.4g} acc/s, edges {edges}")
+        for name, (k, n) in kt.items():
+            log(f"   {name}: launches {n} " + " ".join(f"{kk}={vv/max(n,1)*1e3:.1f}us" for kk, vv in k.items()))
+
+    # per-batch latency (plain form): launch start -> edges in HBM, task weighted
+    lat_us, wts = [], []
+    for p in preps:
+        l = res["plain"]["lt"][p.name] * 1e3
+        nb = len(p.batches)
+        l = l.reshape(args.steps, nb) if len(l) == args.steps * nb else l[:nb][None, :]
+        for row in l:
+            lat_us.extend(row.tolist())
+            wts.extend(p.ntasks_plain)
+    p50 = weighted_pct(lat_us, wts, 50)
+    p99 = weighted_pct(lat_us, wts, 99)
+
+    # roofline of the dominant kernel (k_resolve) on the Cholesky stream, plain form:
+    # algorithmic bytes 24 N_acc + 64 sum F_i + 12 E; tile-aligned accesses overlap exactly one
+    # fragment of the reference's partition, so sum F_i = N_acc
+    kt, nl = res["plain"]["kt"]["cholesky_8192_nb128"]
+    n_a = len(chol)
+    e = res["plain"]["edges"]["cholesky_8192_nb128"]
+    b_alg = (24 * n_a + 64 * n_a + 12 * e) / nl
+    t_res = kt["k_resolve"] / nl * 1e-3
+    achieved = b_alg / t_res / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+            "kernel": "k_resolve", "avg_us": round(t_res * 1e6, 2), "bytes_per_launch": int(b_alg),
+            "workload": "cholesky 8192^2 nb128, 4096-task launches"}
+
+    cpu = None if args.no_cpu else cpu_baseline(chol)
+    out = {
+        "metric": "resolved task accesses/sec + p99 submit-to-ready µs, batched vs plain GPU",
+        "value": round(res["plain"]["rate"], 1),
+        "unit": "accesses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["plain"]["ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "plain GPU resolver, one 4096-task batch per launch: cholesky 8192^2 fp64 nb128 "
+                               f"(45,760 tasks) + {args.random_n} random single-access tasks over 65536^2 fp32",
+                   "accesses": nacc, "batch_tasks": 4096, "parallelism": "single GPU"},
+        "batched": {"value": round(res["batched"]["rate"], 1), "ms_per_step": round(res["batched"]["ms"], 3),
+                    "batches_per_launch": args.merge,
+                    "speedup_vs_plain": round(res["batched"]["rate"] / res["plain"]["rate"], 3)},
+        "p50_batch_latency_us": round(p50, 2),
+        "p99_batch_latency_us": round(p99, 2),
+        "edges": res["plain"]["edges"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

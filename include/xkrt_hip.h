@@ -85,6 +85,8 @@ typedef struct xkrt_hip_dep_stats {
     uint64_t max_reads;     /* largest reader-pool use seen in one bin */
     uint64_t frag_visits;   /* fragment records scanned by the resolve kernel */
     uint64_t device_bytes;  /* device memory held by the domain */
+    uint64_t err;           /* sticky device error bits (0 = healthy): 1 incidences, 2 bin record list,
+                               4 bin fragments, 8 bin readers, 16 candidates, 32 dedup table, 64 edges */
 } xkrt_hip_dep_stats_t;
 
 typedef struct xkrt_hip_dep_domain *xkrt_hip_dep_domain_t;
@@ -145,6 +147,24 @@ int xkrt_hip_dep_stats(xkrt_hip_dep_domain_t d, xkrt_hip_dep_stats_t *out);
 
 /* Stream the domain enqueues on (a hipStream_t). */
 void *xkrt_hip_dep_stream(xkrt_hip_dep_domain_t d);
+
+/* Forgets every access of the domain (all points back to "never accessed"),
+ * drops unconsumed edges and clears a poisoned state; the bin grid and the
+ * frame stay.  Asynchronous.  The reference equivalent is deleting the
+ * domain in runtime_t::reset (src/memory/invalidate.cc:68-131). */
+int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d);
+
+/* Per-kernel HIP-event timing of every later launch (count, alloc, fill,
+ * resolve, dedup, final). */
+int xkrt_hip_dep_timing(xkrt_hip_dep_domain_t d, int enable);
+
+/* Accumulated milliseconds per kernel (6 doubles) over *launches timed
+ * launches; synchronises; clear != 0 restarts the accumulation. */
+int xkrt_hip_dep_kernel_times(xkrt_hip_dep_domain_t d, double *ms, uint64_t *launches, int clear);
+
+/* Per timed launch, milliseconds from its first kernel's start to its last
+ * kernel's end (up to cap values; *n = launches recorded). */
+int xkrt_hip_dep_launch_times(xkrt_hip_dep_domain_t d, float *ms, uint64_t cap, uint64_t *n, int clear);
 
 #ifdef __cplusplus
 }

@@ -80,8 +80,8 @@ def tiny_stream(n, ld, cols, s, maxm, maxn, seed, modes=(1, 2, 3, 4, 5, 6, 8, 9)
 def test_random_small_space_all_modes(seed):
     st = tiny_stream(3000, 64, 64, 4, 40, 40, seed)
     want = oracle_edges(st)
-    for bx, by, tpb in ((256, 64, 3000), (16, 4, 97), (64, 16, 1)):
-        got = R.resolve_stream(st, tasks_per_batch=tpb, bin_x_bytes=bx, bin_y_cols=by, kernel=1)
+    for bx, by, tpb, kc in ((256, 64, 400, 2), (64, 16, 300, 2), (16, 4, 97, 1), (64, 16, 1, 1)):
+        got = R.resolve_stream(st, tasks_per_batch=tpb, bin_x_bytes=bx, bin_y_cols=by, kernel=kc)
         assert same(got, want), (bx, by, tpb)
 
 
@@ -132,5 +132,5 @@ def test_wrapped_tile_exact_when_no_overshoot():
                   np.full(n, ld, np.uint64), m, nn, np.full(n, s, np.uint64))
     r, nr = st.rects()
     assert (nr == 2).any()
-    got = R.resolve_stream(st, tasks_per_batch=50, bin_x_bytes=16, bin_y_cols=4)
+    got = R.resolve_stream(st, tasks_per_batch=50, bin_x_bytes=16, bin_y_cols=4, kernel=1)
     assert same(got, oracle_edges(st))

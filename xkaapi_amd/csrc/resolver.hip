@@ -82,8 +82,8 @@ struct KArgs {
     uint32_t ybin0;           // frame y of bin row 0
     uint32_t xmax;            // LDs: dim-0 extent of the frame
     uint32_t *bin_nf, *bin_nrs;
-    uint4 *frag;              // nbins * fcap * 2
-    uint32_t *rs;             // nbins * rcap
+    uint32_t *frag;           // nbins * 5 * fcap (SoA x0, x1, y0, y1, lw)
+    uint32_t *rs;             // nbins * 5 * rcap reader log (SoA x0, x1, y0, y1, acc)
     uint32_t fcap, rcap;
     uint32_t *bin_cnt, *bin_cur, *active, *binrec;
     uint32_t binrec_cap;
@@ -191,31 +191,38 @@ __global__ void k_fill(KArgs a)
 }
 
 // ---------------------------------------------------------------- resolve
+//
+// Bin state: a painter map of the writers -- fragments tiling the bin, each
+// with a uniform last writer `lw` (access id, NONE = never written) -- and a
+// log of the exact readers (rect clipped to the bin, access id).  Access ids
+// grow in spawn order, so a logged reader r is among the readers-since-the-
+// last-writer of point x iff acc(r) > lw(x) and x is in r.
 
 #define DSZ 256u   // per-task dedup set (LDS)
-#define CCAP 512u  // candidate staging (LDS), 3 u32 each
+#define CCAP 256u  // candidate staging (LDS), 3 u32 each
+#define KCAP 128u  // readers of one fragment kept for the coverage test
 
-template <int FCAP, int RCAP, int LCAP>
+template <int FCAP, int GCAP, int LCAP>
 struct BinLds {
-    uint32_t fx0[2][FCAP], fx1[2][FCAP], fy0[2][FCAP], fy1[2][FCAP];
-    uint32_t flw[2][FCAP], froff[2][FCAP], frlen[2][FCAP];
-    uint32_t rs[2][RCAP];
+    uint32_t fx0[2][FCAP], fx1[2][FCAP], fy0[2][FCAP], fy1[2][FCAP], flw[2][FCAP];
+    uint32_t gx0[GCAP], gx1[GCAP], gy0[GCAP], gy1[GCAP], gacc[GCAP];
     uint32_t L[LCAP];
     uint32_t D[DSZ];
     uint32_t C[3 * CCAP];
+    uint32_t kx0[KCAP], kx1[KCAP], ky0[KCAP], ky1[KCAP];
+    uint32_t cx[KCAP + 1], cy[KCAP + 1];
     uint32_t ccount;
 };
 
 struct BinState {
-    uint32_t nf;      // fragments in the current bank
-    uint32_t fb;      // current fragment bank
-    uint32_t rb;      // current reader-pool bank
-    uint32_t rtop;    // reader-pool use
-    uint32_t err;     // local error bits
+    uint32_t nf;   // fragments in the current bank
+    uint32_t fb;   // current fragment bank
+    uint32_t ng;   // reader-log entries
+    uint32_t err;  // local error bits
 };
 
-template <int FCAP, int RCAP, int LCAP>
-__device__ void flush_cands(const KArgs &a, BinLds<FCAP, RCAP, LCAP> &s)
+template <int FCAP, int GCAP, int LCAP>
+__device__ void flush_cands(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s)
 {
     __syncthreads();
     const uint32_t n = s.ccount;
@@ -234,9 +241,17 @@ __device__ void flush_cands(const KArgs &a, BinLds<FCAP, RCAP, LCAP> &s)
     __syncthreads();
 }
 
+// room for one candidate per lane in the staging buffer
+template <int FCAP, int GCAP, int LCAP>
+__device__ __forceinline__ void room(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s)
+{
+    __syncthreads();
+    if (s.ccount + 64 > CCAP) flush_cands(a, s);
+}
+
 // insert pred into the task's dedup set; true when first seen (or the set is full)
-template <int FCAP, int RCAP, int LCAP>
-__device__ __forceinline__ bool dedup_insert(BinLds<FCAP, RCAP, LCAP> &s, uint32_t p)
+template <int FCAP, int GCAP, int LCAP>
+__device__ __forceinline__ bool dedup_insert(BinLds<FCAP, GCAP, LCAP> &s, uint32_t p)
 {
     uint32_t h = (p * 2654435761u) >> 24;
     for (uint32_t probe = 0; probe < DSZ; ++probe) {
@@ -248,26 +263,14 @@ __device__ __forceinline__ bool dedup_insert(BinLds<FCAP, RCAP, LCAP> &s, uint32
     return true;
 }
 
-template <int FCAP, int RCAP, int LCAP>
-__device__ __forceinline__ void emit(const KArgs &a, BinLds<FCAP, RCAP, LCAP> &s, uint32_t p, uint32_t acc,
-                                     uint32_t task, bool direct)
+template <int FCAP, int GCAP, int LCAP>
+__device__ __forceinline__ void emit(BinLds<FCAP, GCAP, LCAP> &s, uint32_t p, uint32_t acc, uint32_t task)
 {
     if (!dedup_insert(s, p)) return;
-    if (direct) {
-        const uint32_t pos = atomicAdd(&a.ctr->ncand, 1u);
-        if (pos < a.cand_cap) {
-            a.cand[3ull * pos] = p;
-            a.cand[3ull * pos + 1] = acc;
-            a.cand[3ull * pos + 2] = task;
-        } else {
-            atomicOr(&a.ctr->err, ERR_CAND);
-        }
-    } else {
-        const uint32_t pos = atomicAdd(&s.ccount, 1u);
-        s.C[3 * pos] = p;
-        s.C[3 * pos + 1] = acc;
-        s.C[3 * pos + 2] = task;
-    }
+    const uint32_t pos = atomicAdd(&s.ccount, 1u);
+    s.C[3 * pos] = p;
+    s.C[3 * pos + 1] = acc;
+    s.C[3 * pos + 2] = task;
 }
 
 __device__ __forceinline__ bool overlaps(uint32_t ax0, uint32_t ax1, uint32_t ay0, uint32_t ay1, uint32_t bx0,
@@ -276,63 +279,104 @@ __device__ __forceinline__ bool overlaps(uint32_t ax0, uint32_t ax1, uint32_t ay
     return ax0 < bx1 && bx0 < ax1 && ay0 < by1 && by0 < ay1;
 }
 
-// LINK (dependency-tree.hpp:311-355): per overlapping fragment, the readers
-// if the access writes and readers exist, else the last writer.
-template <int FCAP, int RCAP, int LCAP>
-__device__ void do_link(const KArgs &a, BinLds<FCAP, RCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1,
+// LINK (dependency-tree.hpp:311-355).  Per point x of the rect: a writer links
+// to the readers since the last writer of x when there are some, else to the
+// last writer; any other access links to the last writer.
+template <int FCAP, int GCAP, int LCAP>
+__device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1,
                         uint32_t Y0, uint32_t Y1, uint32_t mode, uint32_t acc, uint32_t task)
 {
-    const uint32_t fb = st.fb, rb = st.rb;
-    const bool w = (mode & 2u) != 0;
+    const uint32_t fb = st.fb;
+    const uint32_t lane = lane_id();
+    if (!(mode & 2u)) {
+        for (uint32_t c = 0; c < st.nf; c += 64) {
+            const uint32_t f = c + lane;
+            const bool hit = f < st.nf && s.flw[fb][f] != NONE &&
+                             overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f], X0, X1, Y0, Y1);
+            if (__ballot(hit) == 0) continue;
+            room(a, s);
+            if (hit) emit(s, s.flw[fb][f], acc, task);
+            __syncthreads();
+        }
+        return;
+    }
     for (uint32_t c = 0; c < st.nf; c += 64) {
-        const uint32_t f = c + lane_id();
-        uint32_t k = 0;
-        bool use_reads = false;
-        if (f < st.nf && overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f], X0, X1, Y0, Y1)) {
-            if (w && s.frlen[fb][f]) {
-                k = s.frlen[fb][f];
-                use_reads = true;
-            } else if (s.flw[fb][f] != NONE) {
-                k = 1;
+        const uint32_t f0 = c + lane;
+        uint64_t m = __ballot(f0 < st.nf && overlaps(s.fx0[fb][f0], s.fx1[fb][f0], s.fy0[fb][f0], s.fy1[fb][f0],
+                                                      X0, X1, Y0, Y1));
+        while (m) {
+            const uint32_t f = c + (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t qx0 = max(s.fx0[fb][f], X0), qx1 = min(s.fx1[fb][f], X1);
+            const uint32_t qy0 = max(s.fy0[fb][f], Y0), qy1 = min(s.fy1[fb][f], Y1);
+            const uint32_t lw = s.flw[fb][f];
+            // readers logged after lw that touch Q = f n rect: all are predecessors
+            uint32_t kc = 0;
+            for (uint32_t g0 = 0; g0 < st.ng; g0 += 64) {
+                const uint32_t g = g0 + lane;
+                bool in = false;
+                if (g < st.ng)
+                    in = (lw == NONE || s.gacc[g] > lw) &&
+                         overlaps(s.gx0[g], s.gx1[g], s.gy0[g], s.gy1[g], qx0, qx1, qy0, qy1);
+                const uint64_t km = __ballot(in);
+                if (km == 0) continue;
+                room(a, s);
+                if (in) {
+                    emit(s, s.gacc[g], acc, task);
+                    const uint32_t k = kc + (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
+                    if (k < KCAP) {
+                        s.kx0[k] = max(s.gx0[g], qx0); s.kx1[k] = min(s.gx1[g], qx1);
+                        s.ky0[k] = max(s.gy0[g], qy0); s.ky1[k] = min(s.gy1[g], qy1);
+                    }
+                }
+                kc += (uint32_t)__popcll(km);
+                __syncthreads();
+            }
+            if (lw == NONE) continue;
+            // the last writer is a predecessor iff some point of Q has no such reader.
+            // If Q minus the readers is not empty it holds a point (cx, cy) with cx in
+            // {qx0} U {reader x1} and cy in {qy0} U {reader y1}: the lowest uncovered
+            // point of the leftmost uncovered column.
+            bool uncovered = kc == 0;
+            if (kc > KCAP) {
+                st.err |= ERR_READS;
+                return;
+            }
+            if (kc) {
+                const uint32_t nc = kc + 1;
+                if (lane == 0) { s.cx[0] = qx0; s.cy[0] = qy0; }
+                for (uint32_t k = lane; k < kc; k += 64) {
+                    s.cx[k + 1] = s.kx1[k] < qx1 ? s.kx1[k] : qx0;
+                    s.cy[k + 1] = s.ky1[k] < qy1 ? s.ky1[k] : qy0;
+                }
+                __syncthreads();
+                bool unc = false;
+                for (uint32_t p = lane; p < nc * nc && !unc; p += 64) {
+                    const uint32_t px = s.cx[p % nc], py = s.cy[p / nc];
+                    bool cov = false;
+                    for (uint32_t k = 0; k < kc && !cov; ++k)
+                        cov = s.kx0[k] <= px && px < s.kx1[k] && s.ky0[k] <= py && py < s.ky1[k];
+                    unc = !cov;
+                }
+                uncovered = __ballot(unc) != 0;
+                __syncthreads();
+            }
+            if (uncovered) {
+                room(a, s);
+                if (lane == 0) emit(s, lw, acc, task);
+                __syncthreads();
             }
         }
-        const uint32_t U = wave_bcast(wave_incl_scan(k), 63);
-        if (U == 0) continue;
-        __syncthreads();
-        if (s.ccount + U > CCAP) flush_cands(a, s);
-        const bool direct = U > CCAP;
-        if (k) {
-            if (use_reads) {
-                const uint32_t off = s.froff[fb][f];
-                for (uint32_t i = 0; i < k; ++i) emit(a, s, s.rs[rb][off + i], acc, task, direct);
-            } else {
-                emit(a, s, s.flw[fb][f], acc, task, direct);
-            }
-        }
-        __syncthreads();
     }
 }
 
-// guillotine pieces of f \ R (up to 4), inheriting f's state
-__device__ __forceinline__ uint32_t outside_pieces(uint32_t fx0, uint32_t fx1, uint32_t fy0, uint32_t fy1,
-                                                   uint32_t X0, uint32_t X1, uint32_t Y0, uint32_t Y1,
-                                                   uint32_t px0[4], uint32_t px1[4], uint32_t py0[4],
-                                                   uint32_t py1[4])
-{
-    uint32_t n = 0;
-    const uint32_t mx0 = max(fx0, X0), mx1 = min(fx1, X1);
-    if (fx0 < X0) { px0[n] = fx0; px1[n] = X0; py0[n] = fy0; py1[n] = fy1; ++n; }
-    if (X1 < fx1) { px0[n] = X1; px1[n] = fx1; py0[n] = fy0; py1[n] = fy1; ++n; }
-    if (fy0 < Y0) { px0[n] = mx0; px1[n] = mx1; py0[n] = fy0; py1[n] = Y0; ++n; }
-    if (Y1 < fy1) { px0[n] = mx0; px1[n] = mx1; py0[n] = Y1; py1[n] = fy1; ++n; }
-    return n;
-}
-
 // PUT of a writer (dependency-tree.hpp:238-242): the rect's points get
-// last_write = acc and no readers.  Fragments under the rect are cut away and
-// the rect (clipped to the bin) becomes one fragment.
-template <int FCAP, int RCAP, int LCAP>
-__device__ void do_paint(BinLds<FCAP, RCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
+// last_write = acc.  Fragments under the rect are cut away (up to 4 guillotine
+// pieces of f minus the rect survive, keeping f's writer) and the rect,
+// clipped to the bin, becomes one fragment.  Logged readers are not touched:
+// their ids are now older than the rect's writer.
+template <int FCAP, int GCAP, int LCAP>
+__device__ void do_paint(BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
                          uint32_t Y1, uint32_t acc)
 {
     const uint32_t fb = st.fb, nb = fb ^ 1u;
@@ -341,11 +385,16 @@ __device__ void do_paint(BinLds<FCAP, RCAP, LCAP> &s, BinState &st, uint32_t X0,
         const uint32_t f = c + lane_id();
         uint32_t npc = 0;
         uint32_t px0[4], px1[4], py0[4], py1[4];
-        uint32_t fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
+        uint32_t lw = NONE;
         if (f < st.nf) {
-            fx0 = s.fx0[fb][f]; fx1 = s.fx1[fb][f]; fy0 = s.fy0[fb][f]; fy1 = s.fy1[fb][f];
+            const uint32_t fx0 = s.fx0[fb][f], fx1 = s.fx1[fb][f], fy0 = s.fy0[fb][f], fy1 = s.fy1[fb][f];
+            lw = s.flw[fb][f];
             if (overlaps(fx0, fx1, fy0, fy1, X0, X1, Y0, Y1)) {
-                npc = outside_pieces(fx0, fx1, fy0, fy1, X0, X1, Y0, Y1, px0, px1, py0, py1);
+                const uint32_t mx0 = max(fx0, X0), mx1 = min(fx1, X1);
+                if (fx0 < X0) { px0[npc] = fx0; px1[npc] = X0; py0[npc] = fy0; py1[npc] = fy1; ++npc; }
+                if (X1 < fx1) { px0[npc] = X1; px1[npc] = fx1; py0[npc] = fy0; py1[npc] = fy1; ++npc; }
+                if (fy0 < Y0) { px0[npc] = mx0; px1[npc] = mx1; py0[npc] = fy0; py1[npc] = Y0; ++npc; }
+                if (Y1 < fy1) { px0[npc] = mx0; px1[npc] = mx1; py0[npc] = Y1; py1[npc] = fy1; ++npc; }
             } else {
                 px0[0] = fx0; px1[0] = fx1; py0[0] = fy0; py1[0] = fy1;
                 npc = 1;
@@ -358,144 +407,84 @@ __device__ void do_paint(BinLds<FCAP, RCAP, LCAP> &s, BinState &st, uint32_t X0,
             return;
         }
         const uint32_t o = base + incl - npc;
-        if (npc) {
-            const uint32_t lw = s.flw[fb][f], ro = s.froff[fb][f], rl = s.frlen[fb][f];
-            for (uint32_t i = 0; i < npc; ++i) {
-                s.fx0[nb][o + i] = px0[i]; s.fx1[nb][o + i] = px1[i];
-                s.fy0[nb][o + i] = py0[i]; s.fy1[nb][o + i] = py1[i];
-                s.flw[nb][o + i] = lw; s.froff[nb][o + i] = ro; s.frlen[nb][o + i] = rl;
-            }
+        for (uint32_t i = 0; i < npc; ++i) {
+            s.fx0[nb][o + i] = px0[i]; s.fx1[nb][o + i] = px1[i];
+            s.fy0[nb][o + i] = py0[i]; s.fy1[nb][o + i] = py1[i];
+            s.flw[nb][o + i] = lw;
         }
         base += tot;
     }
     if (lane_id() == 0) {
         s.fx0[nb][base] = X0; s.fx1[nb][base] = X1; s.fy0[nb][base] = Y0; s.fy1[nb][base] = Y1;
-        s.flw[nb][base] = acc; s.froff[nb][base] = 0; s.frlen[nb][base] = 0;
+        s.flw[nb][base] = acc;
     }
     st.nf = base + 1;
     st.fb = nb;
     __syncthreads();
 }
 
-// copy every live reader list to the other pool bank, compacted
-template <int FCAP, int RCAP, int LCAP>
-__device__ void compact_reads(BinLds<FCAP, RCAP, LCAP> &s, BinState &st)
+// drop the logged readers that no point still sees: every fragment they
+// touch has a newer writer
+template <int FCAP, int GCAP, int LCAP>
+__device__ void prune_log(BinLds<FCAP, GCAP, LCAP> &s, BinState &st)
 {
-    const uint32_t fb = st.fb, rb = st.rb, nrb = rb ^ 1u;
+    const uint32_t fb = st.fb;
     uint32_t base = 0;
-    for (uint32_t c = 0; c < st.nf; c += 64) {
-        const uint32_t f = c + lane_id();
-        const uint32_t len = f < st.nf ? s.frlen[fb][f] : 0u;
-        const uint32_t incl = wave_incl_scan(len);
+    for (uint32_t g0 = 0; g0 < st.ng; g0 += 64) {
+        const uint32_t g = g0 + lane_id();
+        uint32_t x0 = 0, x1 = 0, y0 = 0, y1 = 0, ga = 0;
+        bool alive = false;
+        if (g < st.ng) {
+            x0 = s.gx0[g]; x1 = s.gx1[g]; y0 = s.gy0[g]; y1 = s.gy1[g]; ga = s.gacc[g];
+            for (uint32_t f = 0; f < st.nf && !alive; ++f) {
+                const uint32_t lw = s.flw[fb][f];
+                alive = (lw == NONE || lw < ga) && overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f],
+                                                            x0, x1, y0, y1);
+            }
+        }
+        __syncthreads();
+        const uint32_t incl = wave_incl_scan(alive ? 1u : 0u);
         const uint32_t tot = wave_bcast(incl, 63);
-        if (base + tot > (uint32_t)RCAP) {
+        if (alive) {
+            const uint32_t o = base + incl - 1;
+            s.gx0[o] = x0; s.gx1[o] = x1; s.gy0[o] = y0; s.gy1[o] = y1; s.gacc[o] = ga;
+        }
+        base += tot;
+        __syncthreads();
+    }
+    st.ng = base;
+}
+
+// PUT of an exact reader (dependency-tree.hpp:243-244): log it
+template <int FCAP, int GCAP, int LCAP>
+__device__ void do_log(BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
+                       uint32_t Y1, uint32_t acc)
+{
+    if (st.ng >= (uint32_t)GCAP) {
+        prune_log(s, st);
+        if (st.ng >= (uint32_t)GCAP) {
             st.err |= ERR_READS;
             return;
         }
-        if (len) {
-            const uint32_t o = base + incl - len, src = s.froff[fb][f];
-            for (uint32_t i = 0; i < len; ++i) s.rs[nrb][o + i] = s.rs[rb][src + i];
-            s.froff[fb][f] = o;
-        }
-        base += tot;
     }
-    st.rb = nrb;
-    st.rtop = base;
+    if (lane_id() == 0) {
+        const uint32_t g = st.ng;
+        s.gx0[g] = X0; s.gx1[g] = X1; s.gy0[g] = Y0; s.gy1[g] = Y1; s.gacc[g] = acc;
+    }
+    st.ng++;
     __syncthreads();
 }
 
-// PUT of an exact reader (dependency-tree.hpp:243-244): the rect's points get
-// acc appended to their readers.  Overlapping fragments are split into the
-// part inside the rect (new reader list = old + acc) and up to 4 outside
-// parts that keep sharing the old list.
-template <int FCAP, int RCAP, int LCAP>
-__device__ void do_annotate(BinLds<FCAP, RCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
-                            uint32_t Y1, uint32_t acc)
-{
-    // reader-pool space this record needs
-    {
-        const uint32_t fb = st.fb;
-        uint32_t need = 0;
-        for (uint32_t c = 0; c < st.nf; c += 64) {
-            const uint32_t f = c + lane_id();
-            uint32_t k = 0;
-            if (f < st.nf && overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f], X0, X1, Y0, Y1))
-                k = s.frlen[fb][f] + 1;
-            need += wave_bcast(wave_incl_scan(k), 63);
-        }
-        if (st.rtop + need > (uint32_t)RCAP) {
-            compact_reads(s, st);
-            if (st.err) return;
-            if (st.rtop + need > (uint32_t)RCAP) {
-                st.err |= ERR_READS;
-                return;
-            }
-        }
-    }
-    const uint32_t fb = st.fb, nb = fb ^ 1u, rb = st.rb;
-    uint32_t base = 0;
-    for (uint32_t c = 0; c < st.nf; c += 64) {
-        const uint32_t f = c + lane_id();
-        uint32_t npc = 0, rneed = 0;
-        bool ov = false;
-        uint32_t px0[4], px1[4], py0[4], py1[4];
-        uint32_t fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0;
-        if (f < st.nf) {
-            fx0 = s.fx0[fb][f]; fx1 = s.fx1[fb][f]; fy0 = s.fy0[fb][f]; fy1 = s.fy1[fb][f];
-            ov = overlaps(fx0, fx1, fy0, fy1, X0, X1, Y0, Y1);
-            if (ov) {
-                npc = outside_pieces(fx0, fx1, fy0, fy1, X0, X1, Y0, Y1, px0, px1, py0, py1) + 1;
-                rneed = s.frlen[fb][f] + 1;
-            } else {
-                px0[0] = fx0; px1[0] = fx1; py0[0] = fy0; py1[0] = fy1;
-                npc = 1;
-            }
-        }
-        const uint32_t incl = wave_incl_scan(npc);
-        const uint32_t tot = wave_bcast(incl, 63);
-        const uint32_t rincl = wave_incl_scan(rneed);
-        const uint32_t rtot = wave_bcast(rincl, 63);
-        if (base + tot > (uint32_t)FCAP) {
-            st.err |= ERR_FRAG;
-            return;
-        }
-        const uint32_t o = base + incl - npc;
-        if (npc) {
-            const uint32_t lw = s.flw[fb][f], ro = s.froff[fb][f], rl = s.frlen[fb][f];
-            const uint32_t nout = ov ? npc - 1 : npc;
-            for (uint32_t i = 0; i < nout; ++i) {
-                s.fx0[nb][o + i] = px0[i]; s.fx1[nb][o + i] = px1[i];
-                s.fy0[nb][o + i] = py0[i]; s.fy1[nb][o + i] = py1[i];
-                s.flw[nb][o + i] = lw; s.froff[nb][o + i] = ro; s.frlen[nb][o + i] = rl;
-            }
-            if (ov) {
-                const uint32_t j = o + nout;
-                const uint32_t ro2 = st.rtop + rincl - rneed;
-                for (uint32_t i = 0; i < rl; ++i) s.rs[rb][ro2 + i] = s.rs[rb][ro + i];
-                s.rs[rb][ro2 + rl] = acc;
-                s.fx0[nb][j] = max(fx0, X0); s.fx1[nb][j] = min(fx1, X1);
-                s.fy0[nb][j] = max(fy0, Y0); s.fy1[nb][j] = min(fy1, Y1);
-                s.flw[nb][j] = lw; s.froff[nb][j] = ro2; s.frlen[nb][j] = rl + 1;
-            }
-        }
-        base += tot;
-        st.rtop += rtot;
-    }
-    st.nf = base;
-    st.fb = nb;
-    __syncthreads();
-}
-
-template <int FCAP, int RCAP, int LCAP>
+template <int FCAP, int GCAP, int LCAP>
 __global__ void __launch_bounds__(64) k_resolve(KArgs a)
 {
     extern __shared__ unsigned char lds_raw[];
-    BinLds<FCAP, RCAP, LCAP> &s = *reinterpret_cast<BinLds<FCAP, RCAP, LCAP> *>(lds_raw);
+    BinLds<FCAP, GCAP, LCAP> &s = *reinterpret_cast<BinLds<FCAP, GCAP, LCAP> *>(lds_raw);
     const uint32_t lane = lane_id();
     if (a.ctr->err) return;  // a poisoned domain touches no state
     const uint32_t nactive = a.ctr->nactive;
     unsigned long long visits = 0;
-    uint32_t maxf = 0, maxr = 0;
+    uint32_t maxf = 0, maxg = 0;
     while (true) {
         uint32_t b = 0;
         if (lane == 0) b = atomicAdd(&a.ctr->work, 1u);
@@ -511,7 +500,6 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a)
             }
             continue;
         }
-        // bin geometry
         const uint32_t bxi = slot % a.nbx, byi = slot / a.nbx;
         const uint32_t BX0 = bxi << a.bxs, BX1 = min((bxi + 1) << a.bxs, a.xmax);
         const uint32_t BY0 = a.ybin0 + (byi << a.bys), BY1 = BY0 + (1u << a.bys);
@@ -534,28 +522,30 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a)
                 __syncthreads();
             }
 
-        // bin state
+        // bin state (SoA in HBM: 5 x fcap fragment words, 5 x gcap log words)
         BinState st;
-        st.fb = 0; st.rb = 0; st.err = 0;
+        st.fb = 0; st.err = 0;
         const uint32_t nf0 = a.bin_nf[slot];
+        const uint32_t *fr = a.frag + (size_t)slot * a.fcap * 5;
+        const uint32_t *gr = a.rs + (size_t)slot * a.rcap * 5;
         if (nf0 == 0) {
             if (lane == 0) {
                 s.fx0[0][0] = BX0; s.fx1[0][0] = BX1; s.fy0[0][0] = BY0; s.fy1[0][0] = BY1;
-                s.flw[0][0] = NONE; s.froff[0][0] = 0; s.frlen[0][0] = 0;
+                s.flw[0][0] = NONE;
             }
             st.nf = 1;
-            st.rtop = 0;
+            st.ng = 0;
         } else {
-            const uint4 *fr = a.frag + (size_t)slot * a.fcap * 2;
             for (uint32_t f = lane; f < nf0; f += 64) {
-                const uint4 g = fr[2 * f], h = fr[2 * f + 1];
-                s.fx0[0][f] = g.x; s.fx1[0][f] = g.y; s.fy0[0][f] = g.z; s.fy1[0][f] = g.w;
-                s.flw[0][f] = h.x; s.froff[0][f] = h.y; s.frlen[0][f] = h.z;
+                s.fx0[0][f] = fr[f]; s.fx1[0][f] = fr[a.fcap + f]; s.fy0[0][f] = fr[2 * a.fcap + f];
+                s.fy1[0][f] = fr[3 * a.fcap + f]; s.flw[0][f] = fr[4 * a.fcap + f];
             }
             st.nf = nf0;
-            st.rtop = a.bin_nrs[slot];
-            const uint32_t *rr = a.rs + (size_t)slot * a.rcap;
-            for (uint32_t i = lane; i < st.rtop; i += 64) s.rs[0][i] = rr[i];
+            st.ng = a.bin_nrs[slot];
+            for (uint32_t g = lane; g < st.ng; g += 64) {
+                s.gx0[g] = gr[g]; s.gx1[g] = gr[a.rcap + g]; s.gy0[g] = gr[2 * a.rcap + g];
+                s.gy1[g] = gr[3 * a.rcap + g]; s.gacc[g] = gr[4 * a.rcap + g];
+            }
         }
         if (lane == 0) s.ccount = 0;
         __syncthreads();
@@ -568,7 +558,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a)
             while (j < cnt && (a.rtm[__builtin_amdgcn_readfirstlane(s.L[j])] >> 8) == task) ++j;
             for (uint32_t q = lane; q < DSZ; q += 64) s.D[q] = NONE;
             __syncthreads();
-            for (uint32_t q = i; q < j; ++q) {
+            for (uint32_t q = i; q < j && !st.err; ++q) {
                 const uint32_t r = __builtin_amdgcn_readfirstlane(s.L[q]);
                 const uint32_t X0 = max(a.rx0[r], BX0), X1 = min(a.rx1[r], BX1);
                 const uint32_t Y0 = max(a.ry0[r], BY0), Y1 = min(a.ry1[r], BY1);
@@ -584,11 +574,12 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a)
                 if (mode & 2u)
                     do_paint(s, st, X0, X1, Y0, Y1, a.racc[r]);
                 else if (mode == 1u)
-                    do_annotate(s, st, X0, X1, Y0, Y1, a.racc[r]);
+                    do_log(s, st, X0, X1, Y0, Y1, a.racc[r]);
             }
             i = j;
         }
         flush_cands(a, s);
+        if (!st.err) prune_log(s, st);
         if (st.err) {
             if (lane == 0) {
                 atomicOr(&a.ctr->err, st.err);
@@ -597,50 +588,32 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a)
             continue;
         }
 
-        // write back, reader lists compacted
+        // write back
         {
-            const uint32_t fb = st.fb, rb = st.rb;
-            uint32_t total = 0;
-            for (uint32_t c = 0; c < st.nf; c += 64) {
-                const uint32_t f = c + lane;
-                total += wave_bcast(wave_incl_scan(f < st.nf ? s.frlen[fb][f] : 0u), 63);
+            const uint32_t fb = st.fb;
+            uint32_t *fw = a.frag + (size_t)slot * a.fcap * 5;
+            uint32_t *gw = a.rs + (size_t)slot * a.rcap * 5;
+            for (uint32_t f = lane; f < st.nf; f += 64) {
+                fw[f] = s.fx0[fb][f]; fw[a.fcap + f] = s.fx1[fb][f]; fw[2 * a.fcap + f] = s.fy0[fb][f];
+                fw[3 * a.fcap + f] = s.fy1[fb][f]; fw[4 * a.fcap + f] = s.flw[fb][f];
             }
-            if (total > a.rcap) {
-                if (lane == 0) {
-                    atomicOr(&a.ctr->err, ERR_READS);
-                    a.bin_cnt[slot] = 0;
-                }
-                continue;
-            }
-            uint4 *fr = a.frag + (size_t)slot * a.fcap * 2;
-            uint32_t *rr = a.rs + (size_t)slot * a.rcap;
-            uint32_t base = 0;
-            for (uint32_t c = 0; c < st.nf; c += 64) {
-                const uint32_t f = c + lane;
-                const uint32_t len = f < st.nf ? s.frlen[fb][f] : 0u;
-                const uint32_t incl = wave_incl_scan(len);
-                const uint32_t tot = wave_bcast(incl, 63);
-                if (f < st.nf) {
-                    const uint32_t o = base + incl - len, src = s.froff[fb][f];
-                    for (uint32_t q = 0; q < len; ++q) rr[o + q] = s.rs[rb][src + q];
-                    fr[2 * f] = make_uint4(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f]);
-                    fr[2 * f + 1] = make_uint4(s.flw[fb][f], o, len, 0u);
-                }
-                base += tot;
+            for (uint32_t g = lane; g < st.ng; g += 64) {
+                gw[g] = s.gx0[g]; gw[a.rcap + g] = s.gx1[g]; gw[2 * a.rcap + g] = s.gy0[g];
+                gw[3 * a.rcap + g] = s.gy1[g]; gw[4 * a.rcap + g] = s.gacc[g];
             }
             if (lane == 0) {
                 a.bin_nf[slot] = st.nf;
-                a.bin_nrs[slot] = base;
+                a.bin_nrs[slot] = st.ng;
                 a.bin_cnt[slot] = 0;
             }
             maxf = max(maxf, st.nf);
-            maxr = max(maxr, base);
+            maxg = max(maxg, st.ng);
         }
         __syncthreads();
     }
     if (lane == 0) {
         atomicMax(&a.ctr->maxfrag, maxf);
-        atomicMax(&a.ctr->maxrs, maxr);
+        atomicMax(&a.ctr->maxrs, maxg);
         atomicAdd(&a.ctr->frag_visits, visits);
     }
 }
@@ -698,14 +671,32 @@ __global__ void k_final(KArgs a)
     }
 }
 
+__global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint32_t nbins, DevCounters *c)
+{
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t i = tid; i < nbins; i += gridDim.x * blockDim.x) {
+        nf[i] = 0;
+        nrs[i] = 0;
+        cnt[i] = 0;
+    }
+    if (tid == 0) {
+        c->err = 0;
+        c->edge_total = 0;
+        c->maxfrag = 0;
+        c->maxrs = 0;
+        c->cand_total = 0;
+        c->frag_visits = 0;
+    }
+}
+
 __global__ void k_consume(DevCounters *c)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) c->edge_total = 0;
 }
 
 // grow the bin grid: copy every old bin into its new slot
-__global__ void k_regrid(const uint32_t *onf, const uint32_t *onrs, const uint4 *ofr, const uint32_t *ors,
-                         uint32_t onbx, uint32_t onby, uint32_t *nnf, uint32_t *nnrs, uint4 *nfr, uint32_t *nrs,
+__global__ void k_regrid(const uint32_t *onf, const uint32_t *onrs, const uint32_t *ofr, const uint32_t *ors,
+                         uint32_t onbx, uint32_t onby, uint32_t *nnf, uint32_t *nnrs, uint32_t *nfr, uint32_t *nrs,
                          uint32_t nnbx, uint32_t dy, uint32_t fcap, uint32_t rcap)
 {
     const uint32_t ob = blockIdx.x;
@@ -714,8 +705,10 @@ __global__ void k_regrid(const uint32_t *onf, const uint32_t *onrs, const uint4 
     const uint32_t nbi = (by + dy) * nnbx + bx;
     const uint32_t nf = onf[ob], nr = onrs[ob];
     if (threadIdx.x == 0) { nnf[nbi] = nf; nnrs[nbi] = nr; }
-    for (uint32_t i = threadIdx.x; i < 2 * nf; i += blockDim.x) nfr[(size_t)nbi * fcap * 2 + i] = ofr[(size_t)ob * fcap * 2 + i];
-    for (uint32_t i = threadIdx.x; i < nr; i += blockDim.x) nrs[(size_t)nbi * rcap + i] = ors[(size_t)ob * rcap + i];
+    for (uint32_t i = threadIdx.x; i < 5 * fcap; i += blockDim.x)
+        if (i % fcap < nf) nfr[(size_t)nbi * fcap * 5 + i] = ofr[(size_t)ob * fcap * 5 + i];
+    for (uint32_t i = threadIdx.x; i < 5 * rcap; i += blockDim.x)
+        if (i % rcap < nr) nrs[(size_t)nbi * rcap * 5 + i] = ors[(size_t)ob * rcap * 5 + i];
 }
 
 // ---------------------------------------------------------------- host side
@@ -725,7 +718,8 @@ namespace {
 struct KClass {
     int fcap, rcap, lcap;
 };
-const KClass KCLASSES[3] = {{32, 512, 512}, {128, 1024, 1024}, {512, 4096, 2048}};
+// fragments, reader-log entries, records per bin and launch
+const KClass KCLASSES[3] = {{64, 128, 512}, {256, 512, 1024}, {1024, 2048, 2048}};
 
 template <int F, int R, int L>
 size_t lds_bytes() { return sizeof(BinLds<F, R, L>); }
@@ -764,7 +758,7 @@ struct xkrt_hip_dep_domain {
     uint32_t nbx = 0, nby = 0, ybin0 = 0;
     // device state
     uint32_t *bin_nf = nullptr, *bin_nrs = nullptr, *rs = nullptr;
-    uint4 *frag = nullptr;
+    uint32_t *frag = nullptr;
     uint32_t *bin_cnt = nullptr, *bin_cur = nullptr, *active = nullptr;
     uint32_t *binrec = nullptr;
     uint32_t binrec_cap = 0;
@@ -790,21 +784,59 @@ struct xkrt_hip_dep_domain {
     bool poisoned = false;
     uint64_t n_acc = 0, n_rect = 0, n_launch = 0;
     uint64_t bytes_state = 0, bytes_scratch = 0;
+    // optional per-kernel HIP-event timing of every launch
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<hipEvent_t> ev_pending;  // 7 events per launch, in launch order
+    double kernel_ms[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t timed_launches = 0;
+    std::vector<float> launch_ms;        // per timed launch: first kernel start -> last kernel end
 };
+
+static hipEvent_t take_event(xkrt_hip_dep_domain *d)
+{
+    hipEvent_t e = nullptr;
+    if (!d->ev_pool.empty()) {
+        e = d->ev_pool.back();
+        d->ev_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+    }
+    return e;
+}
+
+// accumulate the elapsed times of completed launches (stream synchronised)
+static void harvest_events(xkrt_hip_dep_domain *d)
+{
+    for (size_t i = 0; i + 7 <= d->ev_pending.size(); i += 7) {
+        float tot = 0.f;
+        for (int k = 0; k < 6; ++k) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, d->ev_pending[i + k], d->ev_pending[i + k + 1]) == hipSuccess) {
+                d->kernel_ms[k] += ms;
+                tot += ms;
+            }
+        }
+        d->launch_ms.push_back(tot);
+        d->timed_launches++;
+    }
+    for (hipEvent_t e : d->ev_pending) d->ev_pool.push_back(e);
+    d->ev_pending.clear();
+}
 
 static uint64_t grid_bytes(uint64_t nbins, uint32_t fcap, uint32_t rcap)
 {
-    return nbins * (8ull + 32ull * fcap + 4ull * rcap + 8ull);
+    return nbins * (8ull + 20ull * fcap + 20ull * rcap + 12ull);
 }
 
 static int alloc_grid(xkrt_hip_dep_domain *d, uint32_t nbx, uint32_t nby, uint32_t **nf, uint32_t **nrs,
-                      uint4 **fr, uint32_t **rs, uint32_t **cnt, uint32_t **cur, uint32_t **act)
+                      uint32_t **fr, uint32_t **rs, uint32_t **cnt, uint32_t **cur, uint32_t **act)
 {
     const uint64_t nb = (uint64_t)nbx * nby;
     HIPCHK(hipMalloc(nf, nb * 4));
     HIPCHK(hipMalloc(nrs, nb * 4));
-    HIPCHK(hipMalloc(fr, nb * d->fcap * 32ull));
-    HIPCHK(hipMalloc(rs, nb * d->rcap * 4ull));
+    HIPCHK(hipMalloc(fr, nb * d->fcap * 20ull));
+    HIPCHK(hipMalloc(rs, nb * d->rcap * 20ull));
     HIPCHK(hipMalloc(cnt, nb * 4));
     HIPCHK(hipMalloc(cur, nb * 4));
     HIPCHK(hipMalloc(act, nb * 4));
@@ -814,7 +846,7 @@ static int alloc_grid(xkrt_hip_dep_domain *d, uint32_t nbx, uint32_t nby, uint32
     return 0;
 }
 
-static void free_grid(uint32_t *nf, uint32_t *nrs, uint4 *fr, uint32_t *rs, uint32_t *cnt, uint32_t *cur, uint32_t *act)
+static void free_grid(uint32_t *nf, uint32_t *nrs, uint32_t *fr, uint32_t *rs, uint32_t *cnt, uint32_t *cur, uint32_t *act)
 {
     (void)hipFree(nf); (void)hipFree(nrs); (void)hipFree(fr); (void)hipFree(rs);
     (void)hipFree(cnt); (void)hipFree(cur); (void)hipFree(act);
@@ -842,7 +874,7 @@ static int ensure_grid(xkrt_hip_dep_domain *d, uint32_t ylo, uint32_t yhi)
     if (d->cfg.budget_bytes && new_bytes + d->bytes_scratch > d->cfg.budget_bytes) return -ENOMEM;
     if (nb_new > 0xFFFFFFFFull) return -ERANGE;
     uint32_t *nf, *nrs, *rs, *cnt, *cur, *act;
-    uint4 *fr;
+    uint32_t *fr;
     int rc = alloc_grid(d, d->nbx, rows, &nf, &nrs, &fr, &rs, &cnt, &cur, &act);
     if (rc) return rc;
     if (d->nby) {
@@ -900,19 +932,36 @@ static int enqueue(xkrt_hip_dep_domain *d, const xkrt_hip_rect_soa_t *rec, uint3
     const KArgs a = make_args(d, rec, nrec);
     const uint32_t nbins = d->nbx * d->nby;
     const uint32_t rblocks = std::min<uint32_t>((nrec + 255) / 256, 8192u);
+    hipEvent_t ev[7] = {};
+    if (d->timing)
+        for (int k = 0; k < 7; ++k) ev[k] = take_event(d);
+    auto mark = [&](int k) { if (d->timing && ev[k]) (void)hipEventRecord(ev[k], d->stream); };
+    mark(0);
     hipLaunchKernelGGL(k_count, dim3(rblocks), dim3(256), 0, d->stream, a);
+    mark(1);
     hipLaunchKernelGGL(k_alloc, dim3((nbins + 255) / 256), dim3(256), 0, d->stream, a, nbins);
+    mark(2);
     hipLaunchKernelGGL(k_fill, dim3(rblocks), dim3(256), 0, d->stream, a);
+    mark(3);
     int rc = 0;
     switch (d->kclass) {
-    case 0: rc = launch_resolve<32, 512, 512>(d, a, nbins); break;
-    case 1: rc = launch_resolve<128, 1024, 1024>(d, a, nbins); break;
-    default: rc = launch_resolve<512, 4096, 2048>(d, a, nbins); break;
+    case 0: rc = launch_resolve<64, 128, 512>(d, a, nbins); break;
+    case 1: rc = launch_resolve<256, 512, 1024>(d, a, nbins); break;
+    default: rc = launch_resolve<1024, 2048, 2048>(d, a, nbins); break;
     }
     if (rc) return rc;
+    mark(4);
     hipLaunchKernelGGL(k_dedup, dim3(1024), dim3(256), 0, d->stream, a);
+    mark(5);
     hipLaunchKernelGGL(k_final, dim3(1024), dim3(256), 0, d->stream, a);
+    mark(6);
     HIPCHK(hipGetLastError());
+    if (d->timing) {
+        bool ok = true;
+        for (int k = 0; k < 7; ++k) ok = ok && ev[k];
+        if (ok) d->ev_pending.insert(d->ev_pending.end(), ev, ev + 7);
+        else for (int k = 0; k < 7; ++k) if (ev[k]) d->ev_pool.push_back(ev[k]);
+    }
     d->n_launch++;
     return 0;
 }
@@ -1002,6 +1051,8 @@ int xkrt_hip_dep_domain_destroy(xkrt_hip_dep_domain_t d)
 {
     if (!d) return -EINVAL;
     if (d->stream) (void)hipStreamSynchronize(d->stream);
+    for (hipEvent_t e : d->ev_pending) (void)hipEventDestroy(e);
+    for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
     if (d->nby) free_grid(d->bin_nf, d->bin_nrs, d->frag, d->rs, d->bin_cnt, d->bin_cur, d->active);
     (void)hipFree(d->binrec); (void)hipFree(d->cand); (void)hipFree(d->hkeys); (void)hipFree(d->hvals);
     (void)hipFree(d->eslot); (void)hipFree(d->edges); (void)hipFree(d->ctr);
@@ -1159,6 +1210,7 @@ static int read_counters(xkrt_hip_dep_domain_t d)
 {
     HIPCHK(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, d->stream));
     HIPCHK(hipStreamSynchronize(d->stream));
+    harvest_events(d);
     if (d->ctr_host->err) {
         d->poisoned = true;
         return err_to_errno(d->ctr_host->err);
@@ -1220,9 +1272,60 @@ int xkrt_hip_dep_stats(xkrt_hip_dep_domain_t d, xkrt_hip_dep_stats_t *out)
     out->max_reads = c.maxrs;
     out->frag_visits = c.frag_visits;
     out->device_bytes = d->bytes_state + d->bytes_scratch;
-    return rc;
+    out->err = c.err;
+    (void)rc;  // the error bits are reported in out->err
+    return 0;
 }
 
 void *xkrt_hip_dep_stream(xkrt_hip_dep_domain_t d) { return d ? (void *)d->stream : nullptr; }
+
+int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d)
+{
+    if (!d) return -EINVAL;
+    const uint32_t nbins = d->nbx * d->nby;
+    if (nbins) {
+        hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>((nbins + 255) / 256, 4096u)), dim3(256), 0, d->stream,
+                           d->bin_nf, d->bin_nrs, d->bin_cnt, nbins, d->ctr);
+    } else {
+        hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, d->stream, d->ctr);
+    }
+    HIPCHK(hipGetLastError());
+    d->poisoned = false;
+    d->next_id = 0;
+    d->max_xlen = 0;
+    return 0;
+}
+
+int xkrt_hip_dep_timing(xkrt_hip_dep_domain_t d, int enable)
+{
+    if (!d) return -EINVAL;
+    d->timing = enable != 0;
+    return 0;
+}
+
+int xkrt_hip_dep_kernel_times(xkrt_hip_dep_domain_t d, double *ms, uint64_t *launches, int clear)
+{
+    if (!d || !ms || !launches) return -EINVAL;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    harvest_events(d);
+    for (int k = 0; k < 6; ++k) ms[k] = d->kernel_ms[k];
+    *launches = d->timed_launches;
+    if (clear) {
+        for (int k = 0; k < 6; ++k) d->kernel_ms[k] = 0;
+        d->timed_launches = 0;
+    }
+    return 0;
+}
+
+int xkrt_hip_dep_launch_times(xkrt_hip_dep_domain_t d, float *ms, uint64_t cap, uint64_t *n, int clear)
+{
+    if (!d || !n || (cap && !ms)) return -EINVAL;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    harvest_events(d);
+    *n = d->launch_ms.size();
+    for (uint64_t i = 0; i < std::min<uint64_t>(cap, d->launch_ms.size()); ++i) ms[i] = d->launch_ms[i];
+    if (clear) d->launch_ms.clear();
+    return 0;
+}
 
 } // extern "C"

@@ -40,7 +40,7 @@ class ConfigT(C.Structure):
 
 class StatsT(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("accesses", "rects", "edges", "candidates", "launches", "bins",
-                                          "max_frags", "max_reads", "frag_visits", "device_bytes")]
+                                          "max_frags", "max_reads", "frag_visits", "device_bytes", "err")]
 
 
 class RectSoaT(C.Structure):
@@ -85,6 +85,10 @@ def hip_lib():
     L.xkrt_hip_dep_stats.argtypes = [_p, C.POINTER(StatsT)]
     L.xkrt_hip_dep_stream.argtypes = [_p]
     L.xkrt_hip_dep_stream.restype = _p
+    L.xkrt_hip_dep_reset.argtypes = [_p]
+    L.xkrt_hip_dep_timing.argtypes = [_p, C.c_int]
+    L.xkrt_hip_dep_kernel_times.argtypes = [_p, C.POINTER(C.c_double), C.POINTER(_u64), C.c_int]
+    L.xkrt_hip_dep_launch_times.argtypes = [_p, _p, _u64, C.POINTER(_u64), C.c_int]
     _LIB = L
     return L
 
@@ -187,6 +191,31 @@ class DependencyDomain:
 
     def consume(self):
         _chk(self.L.xkrt_hip_dep_edges_consume(self.h), "xkrt_hip_dep_edges_consume")
+
+    def reset(self):
+        """Forget every access (asynchronous); the grid and frame stay."""
+        _chk(self.L.xkrt_hip_dep_reset(self.h), "xkrt_hip_dep_reset")
+
+    def set_timing(self, on: bool = True):
+        _chk(self.L.xkrt_hip_dep_timing(self.h, 1 if on else 0), "xkrt_hip_dep_timing")
+
+    KERNELS = ("k_count", "k_alloc", "k_fill", "k_resolve", "k_dedup", "k_final")
+
+    def kernel_times(self, clear: bool = True):
+        """({kernel: total ms}, timed launches) from HIP events on the domain's stream."""
+        ms = (C.c_double * 6)()
+        n = _u64()
+        _chk(self.L.xkrt_hip_dep_kernel_times(self.h, ms, C.byref(n), 1 if clear else 0), "xkrt_hip_dep_kernel_times")
+        return {k: float(ms[i]) for i, k in enumerate(self.KERNELS)}, int(n.value)
+
+    def launch_times(self, clear: bool = True) -> np.ndarray:
+        """Per timed launch: ms from its first kernel start to its last kernel end."""
+        n = _u64()
+        _chk(self.L.xkrt_hip_dep_launch_times(self.h, None, 0, C.byref(n), 0), "xkrt_hip_dep_launch_times")
+        out = np.zeros(max(int(n.value), 1), np.float32)
+        _chk(self.L.xkrt_hip_dep_launch_times(self.h, out.ctypes.data, len(out), C.byref(n), 1 if clear else 0),
+             "xkrt_hip_dep_launch_times")
+        return out[:int(n.value)]
 
     def stats(self) -> dict:
         st = StatsT()
