@@ -145,6 +145,7 @@ __global__ void k_count(KArgs a)
         a.ctr->ncand = 0;
         a.ctr->edge_base = a.ctr->edge_total;
     }
+    if (a.ctr->err) return;  // a poisoned domain touches no state
     for (uint32_t r = tid; r < a.nrec; r += gridDim.x * blockDim.x) {
         uint32_t bx0, bx1, by0, by1;
         rec_bins(a, r, bx0, bx1, by0, by1);
@@ -155,6 +156,7 @@ __global__ void k_count(KArgs a)
 
 __global__ void k_alloc(KArgs a, uint32_t nbins)
 {
+    if (a.ctr->err) return;
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t c = slot < nbins ? a.bin_cnt[slot] : 0u;
     const uint32_t incl = wave_incl_scan(c);
@@ -178,6 +180,7 @@ __global__ void k_alloc(KArgs a, uint32_t nbins)
 
 __global__ void k_fill(KArgs a)
 {
+    if (a.ctr->err) return;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t r = tid; r < a.nrec; r += gridDim.x * blockDim.x) {
         uint32_t bx0, bx1, by0, by1;
@@ -622,6 +625,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a)
 
 __global__ void k_dedup(KArgs a)
 {
+    if (a.ctr->err & (ERR_CAND | ERR_BINREC)) return;  // candidates are incomplete
     const uint32_t n = min(a.ctr->ncand, a.cand_cap);
     const unsigned long long ebase = a.ctr->edge_base;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1009,7 +1013,7 @@ int xkrt_hip_dep_domain_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip
     const uint64_t max_rects = d->cfg.max_rects ? d->cfg.max_rects : (1ull << 20);
     d->cfg.max_rects = max_rects;
     d->binrec_cap = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, std::max<uint64_t>(4 * max_rects, 1 << 16));
-    d->cand_cap = (uint32_t)std::min<uint64_t>(0x3FFFFFF0ull, std::max<uint64_t>(8 * max_rects, 1 << 16));
+    d->cand_cap = (uint32_t)std::min<uint64_t>(0x3FFFFFF0ull, std::max<uint64_t>(32 * max_rects, 1 << 20));
     uint64_t hsz = 1;
     while (hsz < 2ull * d->cand_cap) hsz <<= 1;
     d->hmask = (uint32_t)(hsz - 1);

@@ -47,11 +47,19 @@ class RectSoaT(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("x0", "x1", "y0", "y1", "acc", "tm")]
 
 
+ERR_BITS = {1: "incidence array", 2: "bin record list", 4: "bin fragments", 8: "bin reader log",
+            16: "edge candidates", 32: "dedup table", 64: "edge buffer"}
+
+
 class ResolverError(RuntimeError):
-    def __init__(self, rc: int, what: str):
+    def __init__(self, rc: int, what: str, err_bits: int = 0):
         name = errno.errorcode.get(-rc, str(rc))
-        super().__init__(f"{what}: {name} ({rc})")
+        extra = ""
+        if err_bits:
+            extra = " [overflow: " + ", ".join(v for k, v in ERR_BITS.items() if err_bits & k) + "]"
+        super().__init__(f"{what}: {name} ({rc}){extra}")
         self.rc = rc
+        self.err_bits = err_bits
 
 
 _LIB = None
@@ -173,7 +181,11 @@ class DependencyDomain:
 
     def sync(self) -> int:
         n = _u64()
-        _chk(self.L.xkrt_hip_dep_sync(self.h, C.byref(n)), "xkrt_hip_dep_sync")
+        rc = self.L.xkrt_hip_dep_sync(self.h, C.byref(n))
+        if rc != 0:
+            st = StatsT()
+            self.L.xkrt_hip_dep_stats(self.h, C.byref(st))
+            raise ResolverError(rc, "xkrt_hip_dep_sync", int(st.err))
         return int(n.value)
 
     def edges(self) -> np.ndarray:
