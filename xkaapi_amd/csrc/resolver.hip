@@ -61,8 +61,8 @@ enum : uint32_t {
 
 struct DevCounters {
     unsigned int total;      // incidences allocated this launch
-    unsigned int nactive;    // active bins this launch
-    unsigned int work;       // resolve work counter
+    unsigned int ntier[3];   // active bins per LDS tier this launch (heavy ones, from the list front)
+    unsigned int nlight[3];  // light active bins per tier (from the list back)
     unsigned int ncand;      // candidates this launch
     unsigned int err;        // sticky ERR_* bits
     unsigned int maxfrag;    // stats: max fragments in a bin
@@ -85,8 +85,12 @@ struct KArgs {
     uint32_t *frag;           // nbins * 5 * fcap (SoA x0, x1, y0, y1, lw)
     uint32_t *rs;             // nbins * 5 * rcap reader log (SoA x0, x1, y0, y1, acc)
     uint32_t fcap, rcap;
-    uint32_t *bin_cnt, *bin_cur, *active, *binrec;
+    uint32_t *bin_cnt, *bin_cur, *binrec;
+    uint4 *active;  // per tier, nbins entries {slot, incidence start, record count, nf | ng << 16}
     uint32_t binrec_cap;
+    uint32_t nbins;
+    uint32_t max_tier;                              // tiers 0..max_tier; storage sized for max_tier
+    uint32_t tier_f[3], tier_g[3], tier_l[3];       // per tier: fragments, log entries, records
     uint32_t *cand;
     uint32_t cand_cap;
     unsigned long long *hkeys;
@@ -140,8 +144,10 @@ __global__ void k_count(KArgs a)
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
         a.ctr->total = 0;
-        a.ctr->nactive = 0;
-        a.ctr->work = 0;
+        for (int t = 0; t < 3; ++t) {
+            a.ctr->ntier[t] = 0;
+            a.ctr->nlight[t] = 0;
+        }
         a.ctr->ncand = 0;
         a.ctr->edge_base = a.ctr->edge_total;
     }
@@ -154,6 +160,8 @@ __global__ void k_count(KArgs a)
     }
 }
 
+// per-bin segments of the incidence array; each active bin goes to the
+// smallest LDS tier whose capacities fit it with headroom
 __global__ void k_alloc(KArgs a, uint32_t nbins)
 {
     if (a.ctr->err) return;
@@ -161,20 +169,34 @@ __global__ void k_alloc(KArgs a, uint32_t nbins)
     const uint32_t c = slot < nbins ? a.bin_cnt[slot] : 0u;
     const uint32_t incl = wave_incl_scan(c);
     const uint32_t sum = wave_bcast(incl, 63);
-    const uint64_t act = __ballot(c > 0);
-    if (act == 0) return;
-    uint32_t base = 0, abase = 0;
-    if (lane_id() == 0) {
-        base = atomicAdd(&a.ctr->total, sum);
-        abase = atomicAdd(&a.ctr->nactive, (uint32_t)__popcll(act));
-    }
+    if (__ballot(c > 0) == 0) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(&a.ctr->total, sum);
     base = wave_bcast(base, 0);
-    abase = wave_bcast(abase, 0);
     if (lane_id() == 0 && (uint64_t)base + sum > a.binrec_cap) atomicOr(&a.ctr->err, ERR_BINREC);
+    uint32_t tier = 0, nfl = 0, ngl = 0;
     if (c > 0) {
-        const uint32_t rank = (uint32_t)__popcll(act & ((1ull << lane_id()) - 1ull));
         a.bin_cur[slot] = base + incl - c;
-        a.active[abase + rank] = slot;
+        nfl = a.bin_nf[slot];
+        ngl = a.bin_nrs[slot];
+        while (tier < a.max_tier &&
+               (c > a.tier_l[tier] || 2 * nfl > a.tier_f[tier] || 2 * ngl > a.tier_g[tier]))
+            ++tier;
+    }
+    // heavy bins (long record lists) at the front of the tier list, so they start first
+    for (uint32_t t = 0; t <= a.max_tier; ++t) {
+        for (int heavy = 1; heavy >= 0; --heavy) {
+            const bool mine = c > 0 && tier == t && ((c >= 16) == (heavy != 0));
+            const uint64_t m = __ballot(mine);
+            if (m == 0) continue;
+            uint32_t tb = 0;
+            if (lane_id() == 0) tb = atomicAdd(heavy ? &a.ctr->ntier[t] : &a.ctr->nlight[t], (uint32_t)__popcll(m));
+            tb = wave_bcast(tb, 0);
+            const uint32_t r = tb + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
+            if (mine)
+                a.active[(size_t)t * a.nbins + (heavy ? r : a.nbins - 1 - r)] =
+                    make_uint4(slot, base + incl - c, c, nfl | (ngl << 16));
+        }
     }
 }
 
@@ -201,20 +223,32 @@ __global__ void k_fill(KArgs a)
 // grow in spawn order, so a logged reader r is among the readers-since-the-
 // last-writer of point x iff acc(r) > lw(x) and x is in r.
 
-#define DSZ 256u   // per-task dedup set (LDS)
 #define CCAP 256u  // candidate staging (LDS), 3 u32 each
 #define KCAP 128u  // readers of one fragment kept for the coverage test
 
+// per-bin (pred, task) dedup set size: scales with the tier's record capacity
+template <int LCAP>
+struct Dsz {
+    static constexpr uint32_t value = LCAP >= 1024 ? 1024u : (LCAP >= 512 ? 512u : 256u);
+};
+
 template <int FCAP, int GCAP, int LCAP>
 struct BinLds {
+    static constexpr uint32_t DSZ = Dsz<LCAP>::value;
     uint32_t fx0[2][FCAP], fx1[2][FCAP], fy0[2][FCAP], fy1[2][FCAP], flw[2][FCAP];
     uint32_t gx0[GCAP], gx1[GCAP], gy0[GCAP], gy1[GCAP], gacc[GCAP];
     uint32_t L[LCAP];
-    uint32_t D[DSZ];
+    // the bin's records in spawn order, clipped to the bin
+    uint32_t rx0[LCAP], rx1[LCAP], ry0[LCAP], ry1[LCAP], racc[LCAP], rtm[LCAP];
+    unsigned long long D[DSZ];  // (pred, task) keys of the current run / group
+    uint32_t Dv[DSZ];           // lowest successor access id found for the key
+    uint32_t Du[DSZ];           // slots of D in use
     uint32_t C[3 * CCAP];
     uint32_t kx0[KCAP], kx1[KCAP], ky0[KCAP], ky1[KCAP];
     uint32_t cx[KCAP + 1], cy[KCAP + 1];
+    uint32_t slow[64];          // fragments needing the exact coverage test
     uint32_t ccount;
+    uint32_t dcount;            // entries of Du
 };
 
 struct BinState {
@@ -252,28 +286,62 @@ __device__ __forceinline__ void room(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s
     if (s.ccount + 64 > CCAP) flush_cands(a, s);
 }
 
-// insert pred into the task's dedup set; true when first seen (or the set is full)
+// Record predecessor p of access acc (task `task`): the run's dedup set keeps
+// the lowest successor access per (pred, task) -- LINK runs in slot order in
+// the reference, so the lowest access id is the slot that owns the edge.  When
+// the set is full the candidate goes straight to the global candidate buffer
+// (the cross-bin dedup keeps the minimum too).
 template <int FCAP, int GCAP, int LCAP>
-__device__ __forceinline__ bool dedup_insert(BinLds<FCAP, GCAP, LCAP> &s, uint32_t p)
+__device__ __forceinline__ void emit(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, uint32_t p, uint32_t acc,
+                                     uint32_t task)
 {
-    uint32_t h = (p * 2654435761u) >> 24;
-    for (uint32_t probe = 0; probe < DSZ; ++probe) {
-        const uint32_t old = atomicCAS(&s.D[h], NONE, p);
-        if (old == NONE) return true;
-        if (old == p) return false;
+    const unsigned long long key = ((unsigned long long)p << 32) | task;
+    constexpr uint32_t DSZ = BinLds<FCAP, GCAP, LCAP>::DSZ;
+    uint32_t h = (uint32_t)(mix64(key) & (DSZ - 1));
+    for (uint32_t probe = 0; probe < 32; ++probe) {
+        const unsigned long long old = atomicCAS(&s.D[h], EMPTY64, key);
+        if (old == EMPTY64 || old == key) {
+            atomicMin(&s.Dv[h], acc);
+            if (old == EMPTY64) s.Du[atomicAdd(&s.dcount, 1u)] = h;
+            return;
+        }
         h = (h + 1) & (DSZ - 1);
     }
-    return true;
+    const uint32_t pos = atomicAdd(&a.ctr->ncand, 1u);
+    if (pos < a.cand_cap) {
+        a.cand[3ull * pos] = p;
+        a.cand[3ull * pos + 1] = acc;
+        a.cand[3ull * pos + 2] = task;
+    } else {
+        atomicOr(&a.ctr->err, ERR_CAND);
+    }
 }
 
+// move the run's (pred, task, lowest successor) entries to the staging buffer
 template <int FCAP, int GCAP, int LCAP>
-__device__ __forceinline__ void emit(BinLds<FCAP, GCAP, LCAP> &s, uint32_t p, uint32_t acc, uint32_t task)
+__device__ void drain_dedup(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s)
 {
-    if (!dedup_insert(s, p)) return;
-    const uint32_t pos = atomicAdd(&s.ccount, 1u);
-    s.C[3 * pos] = p;
-    s.C[3 * pos + 1] = acc;
-    s.C[3 * pos + 2] = task;
+    __syncthreads();
+    const uint32_t n = s.dcount;
+    for (uint32_t c = 0; c < n; c += 64) {
+        room(a, s);
+        const uint32_t u = c + lane_id();
+        if (u < n) {
+            const uint32_t h = s.Du[u];
+            const unsigned long long key = s.D[h];
+            const uint32_t pos = s.ccount + u - c;
+            s.C[3 * pos] = (uint32_t)(key >> 32);
+            s.C[3 * pos + 1] = s.Dv[h];
+            s.C[3 * pos + 2] = (uint32_t)key;
+            s.D[h] = EMPTY64;
+            s.Dv[h] = NONE;
+        }
+        __syncthreads();
+        if (lane_id() == 0) s.ccount += min(64u, n - c);
+        __syncthreads();
+    }
+    if (lane_id() == 0) s.dcount = 0;
+    __syncthreads();
 }
 
 __device__ __forceinline__ bool overlaps(uint32_t ax0, uint32_t ax1, uint32_t ay0, uint32_t ay1, uint32_t bx0,
@@ -294,82 +362,139 @@ __device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &s
     if (!(mode & 2u)) {
         for (uint32_t c = 0; c < st.nf; c += 64) {
             const uint32_t f = c + lane;
-            const bool hit = f < st.nf && s.flw[fb][f] != NONE &&
-                             overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f], X0, X1, Y0, Y1);
-            if (__ballot(hit) == 0) continue;
-            room(a, s);
-            if (hit) emit(s, s.flw[fb][f], acc, task);
-            __syncthreads();
+            if (f < st.nf && s.flw[fb][f] != NONE &&
+                overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f], X0, X1, Y0, Y1))
+                emit(a, s, s.flw[fb][f], acc, task);
         }
+        __syncthreads();
         return;
     }
+    // writer: one lane per overlapping fragment f, Q = f n rect.  The readers
+    // logged after lw(f) that touch Q are all predecessors; lw(f) is one iff
+    // some point of Q has none of them.
     for (uint32_t c = 0; c < st.nf; c += 64) {
-        const uint32_t f0 = c + lane;
-        uint64_t m = __ballot(f0 < st.nf && overlaps(s.fx0[fb][f0], s.fx1[fb][f0], s.fy0[fb][f0], s.fy1[fb][f0],
-                                                      X0, X1, Y0, Y1));
+        const uint32_t f = c + lane;
+        bool ov = false;
+        uint32_t qx0 = 0, qx1 = 0, qy0 = 0, qy1 = 0, lw = NONE;
+        if (f < st.nf) {
+            qx0 = max(s.fx0[fb][f], X0); qx1 = min(s.fx1[fb][f], X1);
+            qy0 = max(s.fy0[fb][f], Y0); qy1 = min(s.fy1[fb][f], Y1);
+            ov = qx0 < qx1 && qy0 < qy1;
+            lw = s.flw[fb][f];
+        }
+        if (__ballot(ov) == 0) continue;
+        uint32_t kc = 0;
+        bool cov1 = false;
+        for (uint32_t g = 0; g < st.ng; ++g) {
+            const uint32_t gx0 = s.gx0[g], gx1 = s.gx1[g], gy0 = s.gy0[g], gy1 = s.gy1[g], ga = s.gacc[g];
+            if (ov && (lw == NONE || ga > lw) && overlaps(gx0, gx1, gy0, gy1, qx0, qx1, qy0, qy1)) {
+                emit(a, s, ga, acc, task);
+                ++kc;
+                cov1 = cov1 || (gx0 <= qx0 && qx1 <= gx1 && gy0 <= qy0 && qy1 <= gy1);
+            }
+        }
+        if (ov && lw != NONE && kc == 0) emit(a, s, lw, acc, task);
+        // partially covered fragments: exact test, one fragment at a time
+        uint64_t m = __ballot(ov && lw != NONE && kc > 0 && !cov1);
         while (m) {
-            const uint32_t f = c + (uint32_t)__builtin_ctzll(m);
+            const uint32_t ff = c + (uint32_t)__builtin_ctzll(m);
             m &= m - 1;
-            const uint32_t qx0 = max(s.fx0[fb][f], X0), qx1 = min(s.fx1[fb][f], X1);
-            const uint32_t qy0 = max(s.fy0[fb][f], Y0), qy1 = min(s.fy1[fb][f], Y1);
-            const uint32_t lw = s.flw[fb][f];
-            // readers logged after lw that touch Q = f n rect: all are predecessors
-            uint32_t kc = 0;
+            const uint32_t rx0 = max(s.fx0[fb][ff], X0), rx1 = min(s.fx1[fb][ff], X1);
+            const uint32_t ry0 = max(s.fy0[fb][ff], Y0), ry1 = min(s.fy1[fb][ff], Y1);
+            const uint32_t rlw = s.flw[fb][ff];
+            uint32_t nk = 0;
             for (uint32_t g0 = 0; g0 < st.ng; g0 += 64) {
                 const uint32_t g = g0 + lane;
-                bool in = false;
-                if (g < st.ng)
-                    in = (lw == NONE || s.gacc[g] > lw) &&
-                         overlaps(s.gx0[g], s.gx1[g], s.gy0[g], s.gy1[g], qx0, qx1, qy0, qy1);
+                const bool in = g < st.ng && s.gacc[g] > rlw &&
+                                overlaps(s.gx0[g], s.gx1[g], s.gy0[g], s.gy1[g], rx0, rx1, ry0, ry1);
                 const uint64_t km = __ballot(in);
-                if (km == 0) continue;
-                room(a, s);
                 if (in) {
-                    emit(s, s.gacc[g], acc, task);
-                    const uint32_t k = kc + (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
+                    const uint32_t k = nk + (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
                     if (k < KCAP) {
-                        s.kx0[k] = max(s.gx0[g], qx0); s.kx1[k] = min(s.gx1[g], qx1);
-                        s.ky0[k] = max(s.gy0[g], qy0); s.ky1[k] = min(s.gy1[g], qy1);
+                        s.kx0[k] = max(s.gx0[g], rx0); s.kx1[k] = min(s.gx1[g], rx1);
+                        s.ky0[k] = max(s.gy0[g], ry0); s.ky1[k] = min(s.gy1[g], ry1);
                     }
                 }
-                kc += (uint32_t)__popcll(km);
-                __syncthreads();
+                nk += (uint32_t)__popcll(km);
             }
-            if (lw == NONE) continue;
-            // the last writer is a predecessor iff some point of Q has no such reader.
-            // If Q minus the readers is not empty it holds a point (cx, cy) with cx in
-            // {qx0} U {reader x1} and cy in {qy0} U {reader y1}: the lowest uncovered
-            // point of the leftmost uncovered column.
-            bool uncovered = kc == 0;
-            if (kc > KCAP) {
+            if (nk > KCAP) {
                 st.err |= ERR_READS;
                 return;
             }
-            if (kc) {
-                const uint32_t nc = kc + 1;
-                if (lane == 0) { s.cx[0] = qx0; s.cy[0] = qy0; }
-                for (uint32_t k = lane; k < kc; k += 64) {
-                    s.cx[k + 1] = s.kx1[k] < qx1 ? s.kx1[k] : qx0;
-                    s.cy[k + 1] = s.ky1[k] < qy1 ? s.ky1[k] : qy0;
-                }
-                __syncthreads();
-                bool unc = false;
-                for (uint32_t p = lane; p < nc * nc && !unc; p += 64) {
-                    const uint32_t px = s.cx[p % nc], py = s.cy[p / nc];
-                    bool cov = false;
-                    for (uint32_t k = 0; k < kc && !cov; ++k)
-                        cov = s.kx0[k] <= px && px < s.kx1[k] && s.ky0[k] <= py && py < s.ky1[k];
-                    unc = !cov;
-                }
-                uncovered = __ballot(unc) != 0;
-                __syncthreads();
+            // If Q minus the readers is not empty it holds a point (cx, cy) with
+            // cx in {qx0} U {reader x1} and cy in {qy0} U {reader y1}: the lowest
+            // uncovered point of the leftmost uncovered column.
+            const uint32_t nc = nk + 1;
+            if (lane == 0) { s.cx[0] = rx0; s.cy[0] = ry0; }
+            for (uint32_t k = lane; k < nk; k += 64) {
+                s.cx[k + 1] = s.kx1[k] < rx1 ? s.kx1[k] : rx0;
+                s.cy[k + 1] = s.ky1[k] < ry1 ? s.ky1[k] : ry0;
             }
-            if (uncovered) {
-                room(a, s);
-                if (lane == 0) emit(s, lw, acc, task);
-                __syncthreads();
+            __syncthreads();
+            bool unc = false;
+            for (uint32_t p = lane; p < nc * nc && !unc; p += 64) {
+                const uint32_t px = s.cx[p % nc], py = s.cy[p / nc];
+                bool cov = false;
+                for (uint32_t k = 0; k < nk && !cov; ++k)
+                    cov = s.kx0[k] <= px && px < s.kx1[k] && s.ky0[k] <= py && py < s.ky1[k];
+                unc = !cov;
+            }
+            if (__ballot(unc) != 0 && lane == 0) emit(a, s, rlw, acc, task);
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+}
+
+template <int FCAP, int GCAP, int LCAP>
+__device__ void prune_log(BinLds<FCAP, GCAP, LCAP> &s, BinState &st);
+
+// A run of whole task groups none of whose records writes: LINK of a
+// non-writer reads only the writer map, and PUT of a reader only appends to
+// the log, so every record of the run links in parallel (one per lane) and
+// the exact readers are then logged in spawn order.
+template <int FCAP, int GCAP, int LCAP>
+__device__ void do_run(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t i0, uint32_t i1)
+{
+    const uint32_t fb = st.fb;
+    const uint32_t lane = lane_id();
+    for (uint32_t c = i0; c < i1; c += 64) {
+        const uint32_t q = c + lane;
+        const bool live = q < i1;
+        uint32_t X0 = 0, X1 = 0, Y0 = 0, Y1 = 0, acc = 0, task = 0;
+        if (live) {
+            X0 = s.rx0[q]; X1 = s.rx1[q]; Y0 = s.ry0[q]; Y1 = s.ry1[q];
+            acc = s.racc[q]; task = s.rtm[q] >> 8;
+        }
+        for (uint32_t f = 0; f < st.nf; ++f) {
+            const uint32_t lw = s.flw[fb][f];
+            if (live && lw != NONE &&
+                overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f], X0, X1, Y0, Y1))
+                emit(a, s, lw, acc, task);
+        }
+    }
+    __syncthreads();
+    // log the exact readers
+    for (uint32_t c = i0; c < i1; c += 64) {
+        const uint32_t q = c + lane;
+        const bool rd = q < i1 && (s.rtm[q] & 0xffu) == 1u;
+        const uint64_t m = __ballot(rd);
+        const uint32_t n = (uint32_t)__popcll(m);
+        if (!n) continue;
+        if (st.ng + n > (uint32_t)GCAP) {
+            prune_log(s, st);
+            if (st.ng + n > (uint32_t)GCAP) {
+                st.err |= ERR_READS;
+                return;
             }
         }
+        if (rd) {
+            const uint32_t g = st.ng + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            s.gx0[g] = s.rx0[q]; s.gx1[g] = s.rx1[q]; s.gy0[g] = s.ry0[q]; s.gy1[g] = s.ry1[q];
+            s.gacc[g] = s.racc[q];
+        }
+        st.ng += n;
+        __syncthreads();
     }
 }
 
@@ -386,34 +511,31 @@ __device__ void do_paint(BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0,
     uint32_t base = 0;
     for (uint32_t c = 0; c < st.nf; c += 64) {
         const uint32_t f = c + lane_id();
-        uint32_t npc = 0;
-        uint32_t px0[4], px1[4], py0[4], py1[4];
-        uint32_t lw = NONE;
-        if (f < st.nf) {
-            const uint32_t fx0 = s.fx0[fb][f], fx1 = s.fx1[fb][f], fy0 = s.fy0[fb][f], fy1 = s.fy1[fb][f];
+        uint32_t fx0 = 0, fx1 = 0, fy0 = 0, fy1 = 0, lw = NONE;
+        bool live = f < st.nf, ov = false;
+        if (live) {
+            fx0 = s.fx0[fb][f]; fx1 = s.fx1[fb][f]; fy0 = s.fy0[fb][f]; fy1 = s.fy1[fb][f];
             lw = s.flw[fb][f];
-            if (overlaps(fx0, fx1, fy0, fy1, X0, X1, Y0, Y1)) {
-                const uint32_t mx0 = max(fx0, X0), mx1 = min(fx1, X1);
-                if (fx0 < X0) { px0[npc] = fx0; px1[npc] = X0; py0[npc] = fy0; py1[npc] = fy1; ++npc; }
-                if (X1 < fx1) { px0[npc] = X1; px1[npc] = fx1; py0[npc] = fy0; py1[npc] = fy1; ++npc; }
-                if (fy0 < Y0) { px0[npc] = mx0; px1[npc] = mx1; py0[npc] = fy0; py1[npc] = Y0; ++npc; }
-                if (Y1 < fy1) { px0[npc] = mx0; px1[npc] = mx1; py0[npc] = Y1; py1[npc] = fy1; ++npc; }
-            } else {
-                px0[0] = fx0; px1[0] = fx1; py0[0] = fy0; py1[0] = fy1;
-                npc = 1;
-            }
+            ov = overlaps(fx0, fx1, fy0, fy1, X0, X1, Y0, Y1);
         }
+        // pieces of f minus the rect: top, bottom, then left and right of the middle band
+        const bool pt = ov && fx0 < X0, pbm = ov && X1 < fx1, pl = ov && fy0 < Y0, pr = ov && Y1 < fy1;
+        const uint32_t npc = live ? (ov ? (uint32_t)pt + pbm + pl + pr : 1u) : 0u;
         const uint32_t incl = wave_incl_scan(npc);
         const uint32_t tot = wave_bcast(incl, 63);
         if (base + tot + 1 > (uint32_t)FCAP) {
             st.err |= ERR_FRAG;
             return;
         }
-        const uint32_t o = base + incl - npc;
-        for (uint32_t i = 0; i < npc; ++i) {
-            s.fx0[nb][o + i] = px0[i]; s.fx1[nb][o + i] = px1[i];
-            s.fy0[nb][o + i] = py0[i]; s.fy1[nb][o + i] = py1[i];
-            s.flw[nb][o + i] = lw;
+        uint32_t o = base + incl - npc;
+        if (live && !ov) {
+            s.fx0[nb][o] = fx0; s.fx1[nb][o] = fx1; s.fy0[nb][o] = fy0; s.fy1[nb][o] = fy1; s.flw[nb][o] = lw;
+        } else if (ov) {
+            const uint32_t mx0 = max(fx0, X0), mx1 = min(fx1, X1);
+            if (pt) { s.fx0[nb][o] = fx0; s.fx1[nb][o] = X0; s.fy0[nb][o] = fy0; s.fy1[nb][o] = fy1; s.flw[nb][o] = lw; ++o; }
+            if (pbm) { s.fx0[nb][o] = X1; s.fx1[nb][o] = fx1; s.fy0[nb][o] = fy0; s.fy1[nb][o] = fy1; s.flw[nb][o] = lw; ++o; }
+            if (pl) { s.fx0[nb][o] = mx0; s.fx1[nb][o] = mx1; s.fy0[nb][o] = fy0; s.fy1[nb][o] = Y0; s.flw[nb][o] = lw; ++o; }
+            if (pr) { s.fx0[nb][o] = mx0; s.fx1[nb][o] = mx1; s.fy0[nb][o] = Y1; s.fy1[nb][o] = fy1; s.flw[nb][o] = lw; ++o; }
         }
         base += tot;
     }
@@ -478,116 +600,204 @@ __device__ void do_log(BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0, u
     __syncthreads();
 }
 
+// A bin that outgrows this tier's LDS capacities is abandoned -- nothing is
+// written back, its record list stays -- and queued for the next tier; the
+// candidates it staged before are a subset of what the retry emits, and the
+// cross-bin dedup absorbs the repeats.  The last tier reports the overflow.
 template <int FCAP, int GCAP, int LCAP>
-__global__ void __launch_bounds__(64) k_resolve(KArgs a)
+__device__ __forceinline__ void escalate(const KArgs &a, uint32_t tier, uint4 e, uint32_t err)
+{
+    if (lane_id() != 0) return;
+    if (tier < a.max_tier) {
+        const uint32_t pos = atomicAdd(&a.ctr->ntier[tier + 1], 1u);
+        a.active[(size_t)(tier + 1) * a.nbins + pos] = e;
+    } else {
+        atomicOr(&a.ctr->err, err);
+        a.bin_cnt[e.x] = 0;
+    }
+}
+
+template <int FCAP, int GCAP, int LCAP>
+__global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
 {
     extern __shared__ unsigned char lds_raw[];
     BinLds<FCAP, GCAP, LCAP> &s = *reinterpret_cast<BinLds<FCAP, GCAP, LCAP> *>(lds_raw);
     const uint32_t lane = lane_id();
     if (a.ctr->err) return;  // a poisoned domain touches no state
-    const uint32_t nactive = a.ctr->nactive;
+    const uint32_t nheavy = a.ctr->ntier[tier];
+    const uint32_t nlist = nheavy + a.ctr->nlight[tier];
+    const uint4 *list = a.active + (size_t)tier * a.nbins;
     unsigned long long visits = 0;
     uint32_t maxf = 0, maxg = 0;
-    while (true) {
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(&a.ctr->work, 1u);
-        b = wave_bcast(b, 0);
-        if (b >= nactive) break;
-        const uint32_t slot = a.active[b];
-        const uint32_t cnt = a.bin_cnt[slot];
-        const uint32_t start = a.bin_cur[slot] - cnt;
-        if (cnt > (uint32_t)LCAP) {
-            if (lane == 0) {
-                atomicOr(&a.ctr->err, ERR_LIST);
-                a.bin_cnt[slot] = 0;
-            }
+    if (lane == 0) s.ccount = 0;
+    __syncthreads();
+    for (uint32_t b = blockIdx.x; b < nlist; b += gridDim.x) {
+        const uint4 e = b < nheavy ? list[b] : list[a.nbins - 1 - (b - nheavy)];
+        const uint32_t slot = e.x, start = e.y, cnt = e.z, nf0 = e.w & 0xffffu, ng0 = e.w >> 16;
+        if (cnt > (uint32_t)LCAP || nf0 > (uint32_t)FCAP || ng0 > (uint32_t)GCAP) {
+            escalate<FCAP, GCAP, LCAP>(a, tier, e, ERR_LIST);
             continue;
         }
         const uint32_t bxi = slot % a.nbx, byi = slot / a.nbx;
         const uint32_t BX0 = bxi << a.bxs, BX1 = min((bxi + 1) << a.bxs, a.xmax);
         const uint32_t BY0 = a.ybin0 + (byi << a.bys), BY1 = BY0 + (1u << a.bys);
 
-        // record list, sorted ascending (spawn order)
-        uint32_t n2 = 1;
-        while (n2 < cnt) n2 <<= 1;
-        for (uint32_t i = lane; i < n2; i += 64) s.L[i] = i < cnt ? a.binrec[start + i] : NONE;
-        __syncthreads();
-        for (uint32_t k = 2; k <= n2; k <<= 1)
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = lane; i < n2; i += 64) {
-                    const uint32_t ixj = i ^ j;
-                    if (ixj > i) {
-                        const uint32_t x = s.L[i], y = s.L[ixj];
-                        const bool up = (i & k) == 0;
-                        if ((x > y) == up) { s.L[i] = y; s.L[ixj] = x; }
-                    }
-                }
-                __syncthreads();
-            }
-
-        // bin state (SoA in HBM: 5 x fcap fragment words, 5 x gcap log words)
+        // record ids, fragments and reader log of the bin, loads issued together
+        // (SoA in HBM: 5 x fcap fragment words, 5 x gcap log words)
         BinState st;
         st.fb = 0; st.err = 0;
-        const uint32_t nf0 = a.bin_nf[slot];
-        const uint32_t *fr = a.frag + (size_t)slot * a.fcap * 5;
-        const uint32_t *gr = a.rs + (size_t)slot * a.rcap * 5;
+        uint32_t n2 = 1;
+        while (n2 < cnt) n2 <<= 1;
+        {
+            const uint32_t *fr = a.frag + (size_t)slot * a.fcap * 5;
+            const uint32_t *gr = a.rs + (size_t)slot * a.rcap * 5;
+            const uint32_t nmax = max(n2, max(nf0, ng0));
+            for (uint32_t i = lane; i < nmax; i += 64) {
+                const uint32_t rid = i < cnt ? a.binrec[start + i] : NONE;
+                uint32_t f0 = 0, f1 = 0, f2 = 0, f3 = 0, f4 = 0, g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
+                if (i < nf0) {
+                    f0 = fr[i]; f1 = fr[a.fcap + i]; f2 = fr[2 * a.fcap + i]; f3 = fr[3 * a.fcap + i];
+                    f4 = fr[4 * a.fcap + i];
+                }
+                if (i < ng0) {
+                    g0 = gr[i]; g1 = gr[a.rcap + i]; g2 = gr[2 * a.rcap + i]; g3 = gr[3 * a.rcap + i];
+                    g4 = gr[4 * a.rcap + i];
+                }
+                if (i < n2) s.L[i] = rid;
+                if (i < nf0) { s.fx0[0][i] = f0; s.fx1[0][i] = f1; s.fy0[0][i] = f2; s.fy1[0][i] = f3; s.flw[0][i] = f4; }
+                if (i < ng0) { s.gx0[i] = g0; s.gx1[i] = g1; s.gy0[i] = g2; s.gy1[i] = g3; s.gacc[i] = g4; }
+            }
+        }
         if (nf0 == 0) {
             if (lane == 0) {
                 s.fx0[0][0] = BX0; s.fx1[0][0] = BX1; s.fy0[0][0] = BY0; s.fy1[0][0] = BY1;
                 s.flw[0][0] = NONE;
             }
             st.nf = 1;
-            st.ng = 0;
         } else {
-            for (uint32_t f = lane; f < nf0; f += 64) {
-                s.fx0[0][f] = fr[f]; s.fx1[0][f] = fr[a.fcap + f]; s.fy0[0][f] = fr[2 * a.fcap + f];
-                s.fy1[0][f] = fr[3 * a.fcap + f]; s.flw[0][f] = fr[4 * a.fcap + f];
-            }
             st.nf = nf0;
-            st.ng = a.bin_nrs[slot];
-            for (uint32_t g = lane; g < st.ng; g += 64) {
-                s.gx0[g] = gr[g]; s.gx1[g] = gr[a.rcap + g]; s.gy0[g] = gr[2 * a.rcap + g];
-                s.gy1[g] = gr[3 * a.rcap + g]; s.gacc[g] = gr[4 * a.rcap + g];
-            }
         }
-        if (lane == 0) s.ccount = 0;
+        st.ng = ng0;
+        __syncthreads();
+
+        // record list in spawn order: rank sort for short lists, bitonic otherwise
+        if (n2 <= 64) {
+            const uint32_t key = lane < n2 ? s.L[lane] : NONE;
+            uint32_t rank = 0;
+            for (uint32_t j = 0; j < n2; ++j) {
+                const uint32_t o = s.L[j];
+                rank += (o < key) || (o == key && j < lane);
+            }
+            __syncthreads();
+            if (lane < n2) s.L[rank] = key;
+            __syncthreads();
+        } else {
+            for (uint32_t k = 2; k <= n2; k <<= 1)
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t i = lane; i < n2; i += 64) {
+                        const uint32_t ixj = i ^ j;
+                        if (ixj > i) {
+                            const uint32_t x = s.L[i], y = s.L[ixj];
+                            const bool up = (i & k) == 0;
+                            if ((x > y) == up) { s.L[i] = y; s.L[ixj] = x; }
+                        }
+                    }
+                    __syncthreads();
+                }
+        }
+        if (lane == 0) s.dcount = 0;
+        for (uint32_t q = lane; q < BinLds<FCAP, GCAP, LCAP>::DSZ; q += 64) {
+            s.D[q] = EMPTY64;
+            s.Dv[q] = NONE;
+        }
+        // records of the bin, clipped to it, in spawn order
+        for (uint32_t q = lane; q < cnt; q += 64) {
+            const uint32_t r = s.L[q];
+            s.rx0[q] = max(a.rx0[r], BX0); s.rx1[q] = min(a.rx1[r], BX1);
+            s.ry0[q] = max(a.ry0[r], BY0); s.ry1[q] = min(a.ry1[r], BY1);
+            s.racc[q] = a.racc[r]; s.rtm[q] = a.rtm[r];
+        }
         __syncthreads();
 
         uint32_t i = 0;
         while (i < cnt && !st.err) {
-            const uint32_t r0 = __builtin_amdgcn_readfirstlane(s.L[i]);
-            const uint32_t task = a.rtm[r0] >> 8;
+            // first writer record at or after i
+            uint32_t w = cnt;
+            for (uint32_t c = i; c < cnt && w == cnt; c += 64) {
+                const uint32_t q = c + lane;
+                const uint64_t m = __ballot(q < cnt && (s.rtm[q] & 2u));
+                if (m) w = c + (uint32_t)__builtin_ctzll(m);
+            }
+            // start of the writer's task group
+            uint32_t k = w;
+            if (w < cnt)
+                while (k > i && (s.rtm[k - 1] >> 8) == (s.rtm[w] >> 8)) --k;
+            if (k > i) {
+                do_run(a, s, st, i, k);
+                if (s.dcount > BinLds<FCAP, GCAP, LCAP>::DSZ / 2) drain_dedup(a, s);
+                visits += (unsigned long long)st.nf * (k - i);
+                i = k;
+                continue;
+            }
+            // the task group at i writes: LINK each record, then PUT each, in slot order
+            const uint32_t task = s.rtm[i] >> 8;
             uint32_t j = i + 1;
-            while (j < cnt && (a.rtm[__builtin_amdgcn_readfirstlane(s.L[j])] >> 8) == task) ++j;
-            for (uint32_t q = lane; q < DSZ; q += 64) s.D[q] = NONE;
-            __syncthreads();
-            for (uint32_t q = i; q < j && !st.err; ++q) {
-                const uint32_t r = __builtin_amdgcn_readfirstlane(s.L[q]);
-                const uint32_t X0 = max(a.rx0[r], BX0), X1 = min(a.rx1[r], BX1);
-                const uint32_t Y0 = max(a.ry0[r], BY0), Y1 = min(a.ry1[r], BY1);
-                do_link(a, s, st, X0, X1, Y0, Y1, a.rtm[r] & 0xffu, a.racc[r], task);
-                visits += st.nf;
+            while (j < cnt && (s.rtm[j] >> 8) == task) ++j;
+            // a chain of one-record writer groups on the same rect: each links to
+            // the previous one (it wrote the whole rect and nobody read since), so
+            // only the first needs the full LINK and the last one's PUT stands
+            if (j == i + 1) {
+                uint32_t m = 1;
+                for (uint32_t c = i + 1; c < cnt; c += 64) {
+                    const uint32_t q = c + lane;
+                    bool ok = false;
+                    if (q < cnt) {
+                        const uint32_t tq = s.rtm[q] >> 8;
+                        ok = (s.rtm[q] & 2u) && s.rx0[q] == s.rx0[i] && s.rx1[q] == s.rx1[i] && s.ry0[q] == s.ry0[i] &&
+                             s.ry1[q] == s.ry1[i] && tq != (s.rtm[q - 1] >> 8) && (q + 1 == cnt || (s.rtm[q + 1] >> 8) != tq);
+                    }
+                    const uint64_t bad = __ballot(!ok);
+                    if (bad) {
+                        m = c + (uint32_t)__builtin_ctzll(bad) - i;
+                        break;
+                    }
+                    m = c + 64 - i;
+                }
+                if (m >= 2) {
+                    do_link(a, s, st, s.rx0[i], s.rx1[i], s.ry0[i], s.ry1[i], s.rtm[i] & 0xffu, s.racc[i], task);
+                    for (uint32_t q = i + 1 + lane; q < i + m; q += 64) emit(a, s, s.racc[q - 1], s.racc[q], s.rtm[q] >> 8);
+                    __syncthreads();
+                    if (s.dcount > BinLds<FCAP, GCAP, LCAP>::DSZ / 2) drain_dedup(a, s);
+                    const uint32_t last = i + m - 1;
+                    do_paint(s, st, s.rx0[last], s.rx1[last], s.ry0[last], s.ry1[last], s.racc[last]);
+                    visits += (unsigned long long)st.nf * (m + 1);
+                    i += m;
+                    continue;
+                }
             }
             for (uint32_t q = i; q < j && !st.err; ++q) {
-                const uint32_t r = __builtin_amdgcn_readfirstlane(s.L[q]);
-                const uint32_t X0 = max(a.rx0[r], BX0), X1 = min(a.rx1[r], BX1);
-                const uint32_t Y0 = max(a.ry0[r], BY0), Y1 = min(a.ry1[r], BY1);
-                const uint32_t mode = a.rtm[r] & 0xffu;
+                do_link(a, s, st, s.rx0[q], s.rx1[q], s.ry0[q], s.ry1[q], s.rtm[q] & 0xffu, s.racc[q], task);
+                visits += st.nf;
+            }
+            __syncthreads();
+            if (s.dcount > BinLds<FCAP, GCAP, LCAP>::DSZ / 2) drain_dedup(a, s);
+            for (uint32_t q = i; q < j && !st.err; ++q) {
+                const uint32_t mode = s.rtm[q] & 0xffu;
                 visits += st.nf;
                 if (mode & 2u)
-                    do_paint(s, st, X0, X1, Y0, Y1, a.racc[r]);
+                    do_paint(s, st, s.rx0[q], s.rx1[q], s.ry0[q], s.ry1[q], s.racc[q]);
                 else if (mode == 1u)
-                    do_log(s, st, X0, X1, Y0, Y1, a.racc[r]);
+                    do_log(s, st, s.rx0[q], s.rx1[q], s.ry0[q], s.ry1[q], s.racc[q]);
             }
             i = j;
         }
-        flush_cands(a, s);
-        if (!st.err) prune_log(s, st);
+        if (!st.err) {
+            drain_dedup(a, s);
+            prune_log(s, st);
+        }
         if (st.err) {
-            if (lane == 0) {
-                atomicOr(&a.ctr->err, st.err);
-                a.bin_cnt[slot] = 0;
-            }
+            escalate<FCAP, GCAP, LCAP>(a, tier, e, st.err);
+            __syncthreads();
             continue;
         }
 
@@ -614,7 +824,8 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a)
         }
         __syncthreads();
     }
-    if (lane == 0) {
+    flush_cands(a, s);
+    if (lane == 0 && visits) {
         atomicMax(&a.ctr->maxfrag, maxf);
         atomicMax(&a.ctr->maxrs, maxg);
         atomicAdd(&a.ctr->frag_visits, visits);
@@ -623,15 +834,25 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a)
 
 // ---------------------------------------------------------------- dedup
 
+// probe window of the dedup table: a power of two >= 2 x this launch's
+// candidates, so the part of the table a launch touches stays cache resident
+__device__ __forceinline__ uint32_t dedup_mask(const KArgs &a, uint32_t n)
+{
+    uint32_t m = 1024u;
+    while (m < 2u * n && m <= a.hmask) m <<= 1;
+    return min(m - 1u, a.hmask);
+}
+
 __global__ void k_dedup(KArgs a)
 {
     if (a.ctr->err & (ERR_CAND | ERR_BINREC)) return;  // candidates are incomplete
     const uint32_t n = min(a.ctr->ncand, a.cand_cap);
+    const uint32_t mask = dedup_mask(a, n);
     const unsigned long long ebase = a.ctr->edge_base;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t p = a.cand[3ull * i], acc = a.cand[3ull * i + 1], t = a.cand[3ull * i + 2];
         const unsigned long long key = ((unsigned long long)p << 32) | t;
-        uint32_t h = (uint32_t)mix64(key) & a.hmask;
+        uint32_t h = (uint32_t)mix64(key) & mask;
         uint32_t probe = 0;
         while (true) {
             const unsigned long long old = atomicCAS(&a.hkeys[h], EMPTY64, key);
@@ -646,8 +867,8 @@ __global__ void k_dedup(KArgs a)
                 atomicMin(&a.hvals[h], acc);
                 break;
             }
-            h = (h + 1) & a.hmask;
-            if (++probe > a.hmask) {
+            h = (h + 1) & mask;
+            if (++probe > mask) {
                 atomicOr(&a.ctr->err, ERR_HASH);
                 break;
             }
@@ -723,7 +944,7 @@ struct KClass {
     int fcap, rcap, lcap;
 };
 // fragments, reader-log entries, records per bin and launch
-const KClass KCLASSES[3] = {{64, 128, 512}, {256, 512, 1024}, {1024, 2048, 2048}};
+const KClass KCLASSES[3] = {{64, 128, 256}, {256, 512, 512}, {1024, 2048, 1024}};
 
 template <int F, int R, int L>
 size_t lds_bytes() { return sizeof(BinLds<F, R, L>); }
@@ -763,7 +984,8 @@ struct xkrt_hip_dep_domain {
     // device state
     uint32_t *bin_nf = nullptr, *bin_nrs = nullptr, *rs = nullptr;
     uint32_t *frag = nullptr;
-    uint32_t *bin_cnt = nullptr, *bin_cur = nullptr, *active = nullptr;
+    uint32_t *bin_cnt = nullptr, *bin_cur = nullptr;
+    uint4 *active = nullptr;
     uint32_t *binrec = nullptr;
     uint32_t binrec_cap = 0;
     uint32_t *cand = nullptr;
@@ -830,11 +1052,11 @@ static void harvest_events(xkrt_hip_dep_domain *d)
 
 static uint64_t grid_bytes(uint64_t nbins, uint32_t fcap, uint32_t rcap)
 {
-    return nbins * (8ull + 20ull * fcap + 20ull * rcap + 12ull);
+    return nbins * (8ull + 20ull * fcap + 20ull * rcap + 8ull + 48ull);
 }
 
 static int alloc_grid(xkrt_hip_dep_domain *d, uint32_t nbx, uint32_t nby, uint32_t **nf, uint32_t **nrs,
-                      uint32_t **fr, uint32_t **rs, uint32_t **cnt, uint32_t **cur, uint32_t **act)
+                      uint32_t **fr, uint32_t **rs, uint32_t **cnt, uint32_t **cur, uint4 **act)
 {
     const uint64_t nb = (uint64_t)nbx * nby;
     HIPCHK(hipMalloc(nf, nb * 4));
@@ -843,14 +1065,14 @@ static int alloc_grid(xkrt_hip_dep_domain *d, uint32_t nbx, uint32_t nby, uint32
     HIPCHK(hipMalloc(rs, nb * d->rcap * 20ull));
     HIPCHK(hipMalloc(cnt, nb * 4));
     HIPCHK(hipMalloc(cur, nb * 4));
-    HIPCHK(hipMalloc(act, nb * 4));
+    HIPCHK(hipMalloc(act, nb * 16 * 3));
     HIPCHK(hipMemsetAsync(*nf, 0, nb * 4, d->stream));
     HIPCHK(hipMemsetAsync(*nrs, 0, nb * 4, d->stream));
     HIPCHK(hipMemsetAsync(*cnt, 0, nb * 4, d->stream));
     return 0;
 }
 
-static void free_grid(uint32_t *nf, uint32_t *nrs, uint32_t *fr, uint32_t *rs, uint32_t *cnt, uint32_t *cur, uint32_t *act)
+static void free_grid(uint32_t *nf, uint32_t *nrs, uint32_t *fr, uint32_t *rs, uint32_t *cnt, uint32_t *cur, uint4 *act)
 {
     (void)hipFree(nf); (void)hipFree(nrs); (void)hipFree(fr); (void)hipFree(rs);
     (void)hipFree(cnt); (void)hipFree(cur); (void)hipFree(act);
@@ -877,7 +1099,8 @@ static int ensure_grid(xkrt_hip_dep_domain *d, uint32_t ylo, uint32_t yhi)
     const uint64_t new_bytes = grid_bytes(nb_new, d->fcap, d->rcap);
     if (d->cfg.budget_bytes && new_bytes + d->bytes_scratch > d->cfg.budget_bytes) return -ENOMEM;
     if (nb_new > 0xFFFFFFFFull) return -ERANGE;
-    uint32_t *nf, *nrs, *rs, *cnt, *cur, *act;
+    uint32_t *nf, *nrs, *rs, *cnt, *cur;
+    uint4 *act;
     uint32_t *fr;
     int rc = alloc_grid(d, d->nbx, rows, &nf, &nrs, &fr, &rs, &cnt, &cur, &act);
     if (rc) return rc;
@@ -908,6 +1131,13 @@ static KArgs make_args(xkrt_hip_dep_domain *d, const xkrt_hip_rect_soa_t *r, uin
     a.fcap = d->fcap; a.rcap = d->rcap;
     a.bin_cnt = d->bin_cnt; a.bin_cur = d->bin_cur; a.active = d->active; a.binrec = d->binrec;
     a.binrec_cap = d->binrec_cap;
+    a.nbins = d->nbx * d->nby;
+    a.max_tier = d->kclass;
+    for (int t = 0; t < 3; ++t) {
+        a.tier_f[t] = KCLASSES[t].fcap;
+        a.tier_g[t] = KCLASSES[t].rcap;
+        a.tier_l[t] = KCLASSES[t].lcap;
+    }
     a.cand = d->cand; a.cand_cap = d->cand_cap;
     a.hkeys = d->hkeys; a.hvals = d->hvals; a.hmask = d->hmask;
     a.eslot = d->eslot; a.edges = d->edges; a.edge_cap = d->edge_cap;
@@ -916,7 +1146,7 @@ static KArgs make_args(xkrt_hip_dep_domain *d, const xkrt_hip_rect_soa_t *r, uin
 }
 
 template <int F, int R, int L>
-static int launch_resolve(xkrt_hip_dep_domain *d, const KArgs &a, uint32_t nbins)
+static int launch_resolve(xkrt_hip_dep_domain *d, const KArgs &a, uint32_t nbins, uint32_t tier)
 {
     const size_t lds = lds_bytes<F, R, L>();
     static bool attr_set = false;
@@ -925,8 +1155,10 @@ static int launch_resolve(xkrt_hip_dep_domain *d, const KArgs &a, uint32_t nbins
                                    (int)lds));
         attr_set = true;
     }
-    const uint32_t blocks = std::min<uint32_t>(nbins, 4096u);
-    hipLaunchKernelGGL((k_resolve<F, R, L>), dim3(blocks), dim3(64), lds, d->stream, a);
+    // enough blocks to fill the chip at this tier's LDS footprint; each walks its share of the list
+    const uint32_t per_cu = std::max<uint32_t>(1u, (uint32_t)(160u * 1024u / lds));
+    const uint32_t blocks = std::min<uint32_t>(nbins, 256u * std::min<uint32_t>(per_cu, 8u));
+    hipLaunchKernelGGL((k_resolve<F, R, L>), dim3(blocks), dim3(64), lds, d->stream, a, tier);
     return hip_err(hipGetLastError());
 }
 
@@ -948,11 +1180,10 @@ static int enqueue(xkrt_hip_dep_domain *d, const xkrt_hip_rect_soa_t *rec, uint3
     hipLaunchKernelGGL(k_fill, dim3(rblocks), dim3(256), 0, d->stream, a);
     mark(3);
     int rc = 0;
-    switch (d->kclass) {
-    case 0: rc = launch_resolve<64, 128, 512>(d, a, nbins); break;
-    case 1: rc = launch_resolve<256, 512, 1024>(d, a, nbins); break;
-    default: rc = launch_resolve<1024, 2048, 2048>(d, a, nbins); break;
-    }
+    // tier 0 first; bins that outgrow a tier are retried by the next one
+    rc = launch_resolve<64, 128, 256>(d, a, nbins, 0);
+    if (!rc && d->kclass >= 1) rc = launch_resolve<256, 512, 512>(d, a, nbins, 1);
+    if (!rc && d->kclass >= 2) rc = launch_resolve<1024, 2048, 1024>(d, a, nbins, 2);
     if (rc) return rc;
     mark(4);
     hipLaunchKernelGGL(k_dedup, dim3(1024), dim3(256), 0, d->stream, a);
