@@ -96,6 +96,11 @@ def main():
     ap.add_argument("--random-n", type=int, default=1_000_000)
     ap.add_argument("--merge", type=int, default=16, help="batches per launch in the batched form")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("all", "cholesky", "random"), default="all",
+                    help="restrict to one stream (profiling runs)")
+    ap.add_argument("--forms", choices=("both", "plain", "batched"), default="both")
+    ap.add_argument("--calibrate-mib", type=int, default=0,
+                    help="also run one streaming copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (run --gpus 1)",
@@ -108,24 +113,30 @@ def main():
 
     t0 = time.time()
     chol = S.cholesky(8192, 128)
-    rnd = S.random2d(args.random_n)
-    log(f"streams: cholesky {len(chol)} acc / {chol.ntasks} tasks, random {len(rnd)} acc ({time.time()-t0:.1f}s)")
-    preps = [Prepared("cholesky_8192_nb128", chol, dict(bin_x_bytes=1024, bin_y_cols=128, kernel=0), args.merge, torch),
-             Prepared("random2d_65536", rnd, dict(bin_x_bytes=2048, bin_y_cols=256, kernel=1), args.merge, torch)]
+    rnd = S.random2d(args.random_n) if args.workload != "cholesky" else None
+    log(f"streams: cholesky {len(chol)} acc / {chol.ntasks} tasks ({time.time()-t0:.1f}s)")
+    preps = []
+    if args.workload in ("all", "cholesky"):
+        preps.append(Prepared("cholesky_8192_nb128", chol, dict(bin_x_bytes=1024, bin_y_cols=128, kernel=0),
+                              args.merge, torch))
+    if rnd is not None:
+        preps.append(Prepared("random2d_65536", rnd, dict(bin_x_bytes=2048, bin_y_cols=256, kernel=1), args.merge,
+                              torch))
     torch.cuda.synchronize()
     log(f"records resident in HBM ({time.time()-t0:.1f}s)")
 
+    forms = ("plain", "batched") if args.forms == "both" else (args.forms,)
     # correctness gate on the Cholesky stream: the closed-form edge count
-    p0 = preps[0]
-    p0.run("plain")
-    assert p0.dom.sync() == 131040, "Cholesky 8192/128 must have 131,040 edges"
-    p0.run("batched")
-    assert p0.dom.sync() == 131040
+    if preps[0].name.startswith("cholesky"):
+        p0 = preps[0]
+        for form in forms:
+            p0.run(form)
+            assert p0.dom.sync() == 131040, "Cholesky 8192/128 must have 131,040 edges"
 
     for _ in range(args.warmup):
         for p in preps:
-            p.run("plain")
-            p.run("batched")
+            for form in forms:
+                p.run(form)
     torch.cuda.synchronize()
     for p in preps:
         p.dom.sync()
@@ -133,9 +144,19 @@ def main():
         p.dom.launch_times(clear=True)
         p.dom.set_timing(True)
 
+    if args.calibrate_mib:
+        from xkaapi_amd import resolver as R
+        nb = args.calibrate_mib << 20
+        src = torch.empty(nb // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        R.device_copy(dst.data_ptr(), src.data_ptr(), nb, torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_mib} MiB")
+
     nacc = sum(len(p.st) for p in preps)
     res = {}
-    for form in ("plain", "batched"):
+    for form in forms:
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for _ in range(args.steps):
@@ -151,6 +172,10 @@ def main():
         for name, (k, n) in kt.items():
             log(f"   {name}: launches {n} " + " ".join(f"{kk}={vv/max(n,1)*1e3:.1f}us" for kk, vv in k.items()))
 
+    if "plain" not in res:
+        res["plain"] = res["batched"]
+    if "batched" not in res:
+        res["batched"] = res["plain"]
     # per-batch latency (plain form): launch start -> edges in HBM, task weighted
     lat_us, wts = [], []
     for p in preps:
@@ -166,16 +191,18 @@ def main():
     # roofline of the dominant kernel (k_resolve) on the Cholesky stream, plain form:
     # algorithmic bytes 24 N_acc + 64 sum F_i + 12 E; tile-aligned accesses overlap exactly one
     # fragment of the reference's partition, so sum F_i = N_acc
-    kt, nl = res["plain"]["kt"]["cholesky_8192_nb128"]
-    n_a = len(chol)
-    e = res["plain"]["edges"]["cholesky_8192_nb128"]
-    b_alg = (24 * n_a + 64 * n_a + 12 * e) / nl
-    t_res = kt["k_resolve"] / nl * 1e-3
-    achieved = b_alg / t_res / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-            "kernel": "k_resolve", "avg_us": round(t_res * 1e6, 2), "bytes_per_launch": int(b_alg),
-            "workload": "cholesky 8192^2 nb128, 4096-task launches"}
+    roof = None
+    if "cholesky_8192_nb128" in res["plain"]["kt"]:
+        kt, nl = res["plain"]["kt"]["cholesky_8192_nb128"]
+        n_a = len(chol)
+        e = res["plain"]["edges"]["cholesky_8192_nb128"]
+        b_alg = (24 * n_a + 64 * n_a + 12 * e) / nl
+        t_res = kt["k_resolve"] / nl * 1e-3
+        achieved = b_alg / t_res / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "kernel": "k_resolve", "avg_us": round(t_res * 1e6, 2), "bytes_per_launch": int(b_alg),
+                "workload": "cholesky 8192^2 nb128, 4096-task launches (plain form)"}
 
     cpu = None if args.no_cpu else cpu_baseline(chol)
     out = {
@@ -191,8 +218,8 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": "plain GPU resolver, one 4096-task batch per launch: cholesky 8192^2 fp64 nb128 "
-                               f"(45,760 tasks) + {args.random_n} random single-access tasks over 65536^2 fp32",
+        "config": {"workload": "plain GPU resolver, one 4096-task batch per launch: "
+                               + " + ".join(p.name for p in preps),
                    "accesses": nacc, "batch_tasks": 4096, "parallelism": "single GPU"},
         "batched": {"value": round(res["batched"]["rate"], 1), "ms_per_step": round(res["batched"]["ms"], 3),
                     "batches_per_launch": args.merge,

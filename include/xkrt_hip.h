@@ -162,6 +162,10 @@ int xkrt_hip_dep_timing(xkrt_hip_dep_domain_t d, int enable);
  * launches; synchronises; clear != 0 restarts the accumulation. */
 int xkrt_hip_dep_kernel_times(xkrt_hip_dep_domain_t d, double *ms, uint64_t *launches, int clear);
 
+/* Device-to-device streaming copy of `bytes` (a multiple of 16) on `stream`:
+ * the reference kernel whose HBM byte counters calibrate the resolver's. */
+int xkrt_hip_copy(void *dst, const void *src, uint64_t bytes, void *stream);
+
 /* Per timed launch, milliseconds from its first kernel's start to its last
  * kernel's end (up to cap values; *n = launches recorded). */
 int xkrt_hip_dep_launch_times(xkrt_hip_dep_domain_t d, float *ms, uint64_t cap, uint64_t *n, int clear);

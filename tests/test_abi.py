@@ -1,0 +1,52 @@
+"""The C-ABI library loads without a GPU and exports every entry point that
+include/*.h declares; the ctypes structs match the header's layout."""
+import ctypes as C
+import glob
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DECL = re.compile(r"^\s*(?:int|void\s*\*|void|uint64_t|uint32_t)\s+\**\s*(xkrt_hip_\w+)\s*\(", re.M)
+
+
+def declared():
+    names = []
+    for h in sorted(glob.glob(os.path.join(ROOT, "include", "*.h"))):
+        names += DECL.findall(open(h).read())
+    return names
+
+
+def hip_lib():
+    import torch  # noqa: F401  -- one HIP runtime per process: torch's libamdhip64 is loaded first
+    from xkaapi_amd import _loader
+    return _loader.load_host_lib("libxkrt_hip.so")
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "xkrt_hip_dep_domain_create" in names and "xkrt_hip_dep_submit" in names
+    assert len(names) == len(set(names))
+
+
+@pytest.mark.parametrize("name", declared())
+def test_library_exports(name):
+    lib = hip_lib()
+    assert hasattr(lib, name), f"{name} declared in include/ but not exported"
+
+
+def test_ctypes_layouts_match_header():
+    from xkaapi_amd import resolver as R
+    assert C.sizeof(R.ConfigT) == 4 * 4 + 5 * 8
+    assert C.sizeof(R.StatsT) == 11 * 8
+    assert C.sizeof(R.RectSoaT) == 6 * 8
+    assert R.ACCESS_DTYPE.itemsize == C.sizeof(R.AccessT) == 32
+
+
+def test_product_fails_loudly_without_library(monkeypatch, tmp_path):
+    from xkaapi_amd import _loader
+    monkeypatch.setattr(_loader, "LIB_DIR", str(tmp_path))
+    monkeypatch.setattr(_loader, "_cache", {})
+    with pytest.raises(_loader.LibraryMissing):
+        _loader.load_host_lib("libxkrt_hip.so")

@@ -6,24 +6,25 @@
 // Data layout in HBM, per domain:
 //   * the domain frame is cut into a dense grid of bins, BX bytes (dim 0) by
 //     BY columns (dim 1), both powers of two;
-//   * each bin holds a list of fragments that tile it exactly, every
-//     fragment with a uniform state: last writer (access id) and the
-//     readers since that writer (a slice of the bin's reader pool).
-//     32 B per fragment ({x0,x1,y0,y1}, {lw, rs_off, rs_len, 0}), FCAP per bin,
-//     RCAP u32 reader entries per bin;
+//   * each bin holds a writer painter map -- fragments tiling the bin's
+//     written area, each with its last writer (access id) -- and a log of the
+//     exact readers clipped to the bin (AoS, 5 u32 per fragment / entry:
+//     x0, x1, y0, y1, lw or acc; fcap fragments and rcap entries per bin);
 //   * rect records of a launch arrive as structure of arrays
-//     (x0, x1, y0, y1, acc, task<<8|mode: 24 B).
+//     (x0, x1, y0, y1, acc, task<<8|mode: 24 B);
+//   * incidences (record x bin) are 32 B: {x0, x1, y0, y1}, {acc, tm, record, 0}.
 //
 // One launch resolves one batch of records in six kernels:
 //   k_count   records -> per-bin record counts
-//   k_alloc   per-bin segments in the incidence array, active-bin list
-//   k_fill    incidence array (record ids, per-bin segments)
-//   k_resolve one wave64 per active bin: the bin's fragments and reader pool
-//             move to LDS, its records are sorted (bitonic, LDS) and replayed
-//             in spawn order -- LINK of every access of a task, then PUT --
-//             emitting (pred, succ access, succ task) candidates; W paints
-//             over fragments, R splits them and appends to the reader lists;
-//             the bin is written back compacted
+//   k_alloc   per-bin segments in the incidence array, per-tier active-bin lists
+//   k_fill    incidence array (per-bin segments)
+//   k_resolve one wave64 per active bin: the bin's fragments, reader log and
+//             incidences move to LDS, its records are put in spawn order
+//             (rank sort up to 64, bitonic on 64-bit keys above) and replayed
+//             -- LINK of every access of a task, then PUT -- emitting
+//             (pred, succ access, succ task) candidates; W paints over the
+//             writer map, an exact R appends to the reader log; the bin is
+//             written back compacted
 //   k_dedup   candidates -> open-addressing table keyed (pred, succ task),
 //             keeping the lowest succ access id (= lowest slot, since LINK
 //             runs in slot order)
@@ -85,7 +86,8 @@ struct KArgs {
     uint32_t *frag;           // nbins * 5 * fcap (SoA x0, x1, y0, y1, lw)
     uint32_t *rs;             // nbins * 5 * rcap reader log (SoA x0, x1, y0, y1, acc)
     uint32_t fcap, rcap;
-    uint32_t *bin_cnt, *bin_cur, *binrec;
+    uint32_t *bin_cnt, *bin_cur;
+    uint4 *binrec;  // 2 x uint4 per incidence: {x0, x1, y0, y1}, {acc, tm, record index, 0}
     uint4 *active;  // per tier, nbins entries {slot, incidence start, record count, nf | ng << 16}
     uint32_t binrec_cap;
     uint32_t nbins;
@@ -210,7 +212,10 @@ __global__ void k_fill(KArgs a)
         for (uint32_t by = by0; by <= by1; ++by)
             for (uint32_t bx = bx0; bx <= bx1; ++bx) {
                 const uint32_t pos = atomicAdd(&a.bin_cur[by * a.nbx + bx], 1u);
-                if (pos < a.binrec_cap) a.binrec[pos] = r;
+                if (pos < a.binrec_cap) {
+                    a.binrec[2ull * pos] = make_uint4(a.rx0[r], a.rx1[r], a.ry0[r], a.ry1[r]);
+                    a.binrec[2ull * pos + 1] = make_uint4(a.racc[r], a.rtm[r], r, 0u);
+                }
             }
     }
 }
@@ -237,7 +242,7 @@ struct BinLds {
     static constexpr uint32_t DSZ = Dsz<LCAP>::value;
     uint32_t fx0[2][FCAP], fx1[2][FCAP], fy0[2][FCAP], fy1[2][FCAP], flw[2][FCAP];
     uint32_t gx0[GCAP], gx1[GCAP], gy0[GCAP], gy1[GCAP], gacc[GCAP];
-    uint32_t L[LCAP];
+    unsigned long long L[LCAP];  // sort keys: record index << 32 | staging position
     // the bin's records in spawn order, clipped to the bin
     uint32_t rx0[LCAP], rx1[LCAP], ry0[LCAP], ry1[LCAP], racc[LCAP], rtm[LCAP];
     unsigned long long D[DSZ];  // (pred, task) keys of the current run / group
@@ -417,6 +422,7 @@ __device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &s
                 }
                 nk += (uint32_t)__popcll(km);
             }
+            __syncthreads();  // kx/ky rows are read by other lanes below
             if (nk > KCAP) {
                 st.err |= ERR_READS;
                 return;
@@ -649,21 +655,29 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
         uint32_t n2 = 1;
         while (n2 < cnt) n2 <<= 1;
         {
-            const uint32_t *fr = a.frag + (size_t)slot * a.fcap * 5;
-            const uint32_t *gr = a.rs + (size_t)slot * a.rcap * 5;
+            const uint32_t *fr = a.frag + (size_t)slot * a.fcap * 5;  // AoS, 5 words per fragment
+            const uint32_t *gr = a.rs + (size_t)slot * a.rcap * 5;    // AoS, 5 words per log entry
+            const uint4 *ir = a.binrec + 2ull * start;
             const uint32_t nmax = max(n2, max(nf0, ng0));
             for (uint32_t i = lane; i < nmax; i += 64) {
-                const uint32_t rid = i < cnt ? a.binrec[start + i] : NONE;
+                uint4 q0 = make_uint4(0, 0, 0, 0), q1 = make_uint4(NONE, 0, NONE, 0);
                 uint32_t f0 = 0, f1 = 0, f2 = 0, f3 = 0, f4 = 0, g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
+                if (i < cnt) { q0 = ir[2 * i]; q1 = ir[2 * i + 1]; }
                 if (i < nf0) {
-                    f0 = fr[i]; f1 = fr[a.fcap + i]; f2 = fr[2 * a.fcap + i]; f3 = fr[3 * a.fcap + i];
-                    f4 = fr[4 * a.fcap + i];
+                    const uint32_t *q = fr + 5 * i;
+                    f0 = q[0]; f1 = q[1]; f2 = q[2]; f3 = q[3]; f4 = q[4];
                 }
                 if (i < ng0) {
-                    g0 = gr[i]; g1 = gr[a.rcap + i]; g2 = gr[2 * a.rcap + i]; g3 = gr[3 * a.rcap + i];
-                    g4 = gr[4 * a.rcap + i];
+                    const uint32_t *q = gr + 5 * i;
+                    g0 = q[0]; g1 = q[1]; g2 = q[2]; g3 = q[3]; g4 = q[4];
                 }
-                if (i < n2) s.L[i] = rid;
+                if (i < n2) {
+                    // unsorted staging: key = record index, payload in the record arrays
+                    s.L[i] = i < cnt ? (unsigned long long)q1.z : EMPTY64;
+                    s.rx0[i] = max(q0.x, BX0); s.rx1[i] = min(q0.y, BX1);
+                    s.ry0[i] = max(q0.z, BY0); s.ry1[i] = min(q0.w, BY1);
+                    s.racc[i] = q1.x; s.rtm[i] = q1.y;
+                }
                 if (i < nf0) { s.fx0[0][i] = f0; s.fx1[0][i] = f1; s.fy0[0][i] = f2; s.fy1[0][i] = f3; s.flw[0][i] = f4; }
                 if (i < ng0) { s.gx0[i] = g0; s.gx1[i] = g1; s.gy0[i] = g2; s.gy1[i] = g3; s.gacc[i] = g4; }
             }
@@ -680,42 +694,56 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
         st.ng = ng0;
         __syncthreads();
 
-        // record list in spawn order: rank sort for short lists, bitonic otherwise
+        // records in spawn order (record index): short lists move each lane's
+        // record straight to its rank; longer ones sort (key, position) pairs and
+        // re-read the incidences (L2-resident) in sorted order
         if (n2 <= 64) {
-            const uint32_t key = lane < n2 ? s.L[lane] : NONE;
+            const unsigned long long key = lane < n2 ? s.L[lane] : EMPTY64;
             uint32_t rank = 0;
             for (uint32_t j = 0; j < n2; ++j) {
-                const uint32_t o = s.L[j];
+                const unsigned long long o = s.L[j];
                 rank += (o < key) || (o == key && j < lane);
             }
+            uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+            if (lane < cnt) {
+                v0 = s.rx0[lane]; v1 = s.rx1[lane]; v2 = s.ry0[lane]; v3 = s.ry1[lane]; v4 = s.racc[lane]; v5 = s.rtm[lane];
+            }
             __syncthreads();
-            if (lane < n2) s.L[rank] = key;
+            if (lane < cnt) {
+                s.rx0[rank] = v0; s.rx1[rank] = v1; s.ry0[rank] = v2; s.ry1[rank] = v3; s.racc[rank] = v4;
+                s.rtm[rank] = v5;
+            }
             __syncthreads();
         } else {
+            // keys (record index << 32 | staging position)
+            for (uint32_t i = lane; i < n2; i += 64) s.L[i] = i < cnt ? (s.L[i] << 32) | i : EMPTY64;
+            __syncthreads();
             for (uint32_t k = 2; k <= n2; k <<= 1)
                 for (uint32_t j = k >> 1; j > 0; j >>= 1) {
                     for (uint32_t i = lane; i < n2; i += 64) {
                         const uint32_t ixj = i ^ j;
                         if (ixj > i) {
-                            const uint32_t x = s.L[i], y = s.L[ixj];
+                            const unsigned long long x = s.L[i], y = s.L[ixj];
                             const bool up = (i & k) == 0;
                             if ((x > y) == up) { s.L[i] = y; s.L[ixj] = x; }
                         }
                     }
                     __syncthreads();
                 }
+            const uint4 *ir = a.binrec + 2ull * start;
+            for (uint32_t i = lane; i < cnt; i += 64) {
+                const uint32_t pos = (uint32_t)s.L[i];
+                const uint4 q0 = ir[2 * pos], q1 = ir[2 * pos + 1];
+                s.rx0[i] = max(q0.x, BX0); s.rx1[i] = min(q0.y, BX1);
+                s.ry0[i] = max(q0.z, BY0); s.ry1[i] = min(q0.w, BY1);
+                s.racc[i] = q1.x; s.rtm[i] = q1.y;
+            }
+            __syncthreads();
         }
         if (lane == 0) s.dcount = 0;
         for (uint32_t q = lane; q < BinLds<FCAP, GCAP, LCAP>::DSZ; q += 64) {
             s.D[q] = EMPTY64;
             s.Dv[q] = NONE;
-        }
-        // records of the bin, clipped to it, in spawn order
-        for (uint32_t q = lane; q < cnt; q += 64) {
-            const uint32_t r = s.L[q];
-            s.rx0[q] = max(a.rx0[r], BX0); s.rx1[q] = min(a.rx1[r], BX1);
-            s.ry0[q] = max(a.ry0[r], BY0); s.ry1[q] = min(a.ry1[r], BY1);
-            s.racc[q] = a.racc[r]; s.rtm[q] = a.rtm[r];
         }
         __syncthreads();
 
@@ -807,12 +835,12 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
             uint32_t *fw = a.frag + (size_t)slot * a.fcap * 5;
             uint32_t *gw = a.rs + (size_t)slot * a.rcap * 5;
             for (uint32_t f = lane; f < st.nf; f += 64) {
-                fw[f] = s.fx0[fb][f]; fw[a.fcap + f] = s.fx1[fb][f]; fw[2 * a.fcap + f] = s.fy0[fb][f];
-                fw[3 * a.fcap + f] = s.fy1[fb][f]; fw[4 * a.fcap + f] = s.flw[fb][f];
+                uint32_t *q = fw + 5 * f;
+                q[0] = s.fx0[fb][f]; q[1] = s.fx1[fb][f]; q[2] = s.fy0[fb][f]; q[3] = s.fy1[fb][f]; q[4] = s.flw[fb][f];
             }
             for (uint32_t g = lane; g < st.ng; g += 64) {
-                gw[g] = s.gx0[g]; gw[a.rcap + g] = s.gx1[g]; gw[2 * a.rcap + g] = s.gy0[g];
-                gw[3 * a.rcap + g] = s.gy1[g]; gw[4 * a.rcap + g] = s.gacc[g];
+                uint32_t *q = gw + 5 * g;
+                q[0] = s.gx0[g]; q[1] = s.gx1[g]; q[2] = s.gy0[g]; q[3] = s.gy1[g]; q[4] = s.gacc[g];
             }
             if (lane == 0) {
                 a.bin_nf[slot] = st.nf;
@@ -834,20 +862,11 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
 
 // ---------------------------------------------------------------- dedup
 
-// probe window of the dedup table: a power of two >= 2 x this launch's
-// candidates, so the part of the table a launch touches stays cache resident
-__device__ __forceinline__ uint32_t dedup_mask(const KArgs &a, uint32_t n)
-{
-    uint32_t m = 1024u;
-    while (m < 2u * n && m <= a.hmask) m <<= 1;
-    return min(m - 1u, a.hmask);
-}
-
 __global__ void k_dedup(KArgs a)
 {
     if (a.ctr->err & (ERR_CAND | ERR_BINREC)) return;  // candidates are incomplete
     const uint32_t n = min(a.ctr->ncand, a.cand_cap);
-    const uint32_t mask = dedup_mask(a, n);
+    const uint32_t mask = a.hmask;  // the whole table: spread keys keep concurrent atomics on distinct lines
     const unsigned long long ebase = a.ctr->edge_base;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t p = a.cand[3ull * i], acc = a.cand[3ull * i + 1], t = a.cand[3ull * i + 2];
@@ -914,6 +933,13 @@ __global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint32_t nbi
     }
 }
 
+// plain streaming copy, the reference point for HBM byte counters
+__global__ void k_copy(uint4 *dst, const uint4 *src, uint64_t n16)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_consume(DevCounters *c)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) c->edge_total = 0;
@@ -930,10 +956,8 @@ __global__ void k_regrid(const uint32_t *onf, const uint32_t *onrs, const uint32
     const uint32_t nbi = (by + dy) * nnbx + bx;
     const uint32_t nf = onf[ob], nr = onrs[ob];
     if (threadIdx.x == 0) { nnf[nbi] = nf; nnrs[nbi] = nr; }
-    for (uint32_t i = threadIdx.x; i < 5 * fcap; i += blockDim.x)
-        if (i % fcap < nf) nfr[(size_t)nbi * fcap * 5 + i] = ofr[(size_t)ob * fcap * 5 + i];
-    for (uint32_t i = threadIdx.x; i < 5 * rcap; i += blockDim.x)
-        if (i % rcap < nr) nrs[(size_t)nbi * rcap * 5 + i] = ors[(size_t)ob * rcap * 5 + i];
+    for (uint32_t i = threadIdx.x; i < 5 * nf; i += blockDim.x) nfr[(size_t)nbi * fcap * 5 + i] = ofr[(size_t)ob * fcap * 5 + i];
+    for (uint32_t i = threadIdx.x; i < 5 * nr; i += blockDim.x) nrs[(size_t)nbi * rcap * 5 + i] = ors[(size_t)ob * rcap * 5 + i];
 }
 
 // ---------------------------------------------------------------- host side
@@ -948,6 +972,7 @@ const KClass KCLASSES[3] = {{64, 128, 256}, {256, 512, 512}, {1024, 2048, 1024}}
 
 template <int F, int R, int L>
 size_t lds_bytes() { return sizeof(BinLds<F, R, L>); }
+static_assert(sizeof(BinLds<1024, 2048, 1024>) <= 160u * 1024u, "largest tier exceeds the 160 KiB LDS of a CU");
 
 int hip_err(hipError_t e)
 {
@@ -986,7 +1011,7 @@ struct xkrt_hip_dep_domain {
     uint32_t *frag = nullptr;
     uint32_t *bin_cnt = nullptr, *bin_cur = nullptr;
     uint4 *active = nullptr;
-    uint32_t *binrec = nullptr;
+    uint4 *binrec = nullptr;
     uint32_t binrec_cap = 0;
     uint32_t *cand = nullptr;
     uint32_t cand_cap = 0;
@@ -1249,12 +1274,12 @@ int xkrt_hip_dep_domain_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip
     while (hsz < 2ull * d->cand_cap) hsz <<= 1;
     d->hmask = (uint32_t)(hsz - 1);
     d->edge_cap = d->cfg.max_edges ? d->cfg.max_edges : std::max<uint64_t>(16 * max_rects, 1 << 20);
-    d->bytes_scratch = 4ull * d->binrec_cap + 12ull * d->cand_cap + 12ull * hsz + 4ull * hsz + 8ull * d->edge_cap +
+    d->bytes_scratch = 32ull * d->binrec_cap + 12ull * d->cand_cap + 12ull * hsz + 4ull * hsz + 8ull * d->edge_cap +
                        sizeof(DevCounters);
     int rc = 0;
     auto fail = [&](int e) { rc = e; };
     if (d->cfg.budget_bytes && d->bytes_scratch > d->cfg.budget_bytes) fail(-ENOMEM);
-    if (!rc) fail(hip_err(hipMalloc(&d->binrec, 4ull * d->binrec_cap)));
+    if (!rc) fail(hip_err(hipMalloc(&d->binrec, 32ull * d->binrec_cap)));
     if (!rc) fail(hip_err(hipMalloc(&d->cand, 12ull * d->cand_cap)));
     if (!rc) fail(hip_err(hipMalloc(&d->hkeys, 8ull * hsz)));
     if (!rc) fail(hip_err(hipMalloc(&d->hvals, 4ull * hsz)));
@@ -1529,6 +1554,15 @@ int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d)
     d->next_id = 0;
     d->max_xlen = 0;
     return 0;
+}
+
+int xkrt_hip_copy(void *dst, const void *src, uint64_t bytes, void *stream)
+{
+    if (!dst || !src || (bytes & 15)) return -EINVAL;
+    const uint64_t n16 = bytes / 16;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n16 + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (uint4 *)dst, (const uint4 *)src, n16);
+    return hip_err(hipGetLastError());
 }
 
 int xkrt_hip_dep_timing(xkrt_hip_dep_domain_t d, int enable)

@@ -97,6 +97,7 @@ def hip_lib():
     L.xkrt_hip_dep_timing.argtypes = [_p, C.c_int]
     L.xkrt_hip_dep_kernel_times.argtypes = [_p, C.POINTER(C.c_double), C.POINTER(_u64), C.c_int]
     L.xkrt_hip_dep_launch_times.argtypes = [_p, _p, _u64, C.POINTER(_u64), C.c_int]
+    L.xkrt_hip_copy.argtypes = [_p, _p, _u64, _p]
     _LIB = L
     return L
 
@@ -104,6 +105,11 @@ def hip_lib():
 def _chk(rc: int, what: str):
     if rc != 0:
         raise ResolverError(rc, what)
+
+
+def device_copy(dst_ptr: int, src_ptr: int, nbytes: int, stream: int = 0):
+    """Streaming device copy (the calibration kernel for HBM byte counters)."""
+    _chk(hip_lib().xkrt_hip_copy(dst_ptr, src_ptr, nbytes, stream or None), "xkrt_hip_copy")
 
 
 def to_access_array(task, mode, addr, m, n) -> np.ndarray:
