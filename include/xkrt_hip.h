@@ -170,6 +170,76 @@ int xkrt_hip_copy(void *dst, const void *src, uint64_t bytes, void *stream);
  * kernel's end (up to cap values; *n = launches recorded). */
 int xkrt_hip_dep_launch_times(xkrt_hip_dep_domain_t d, float *ms, uint64_t cap, uint64_t *n, int clear);
 
+/* ------------------------------------------------------------------------
+ * Coherency fetch planner: one memory tree per (ld, sizeof_type), a host
+ * (device 0) and one GPU (device 1), synchronous completion (the fetches of
+ * an access complete before the next access), no eviction.
+ *
+ * Replaces KBLASMemoryTree<2>::fetch (memory-tree.hpp:1806-1874) with its
+ * host path (fetch_list_to_host, 1110-1215), allocation-view reuse and
+ * creation (1293-1450), copy setup (1471-1631), set coherent (1633-1667),
+ * fetch list (1067-1108) and completion (955-1061, 2067-2127); and
+ * who_owns (1903-1933).  One tree per (ld, sizeof_type) as
+ * task_get_memory_controller creates them (src/task/task.cc:50-150).
+ * ------------------------------------------------------------------------ */
+
+typedef struct xkrt_hip_mcc_config {
+    uint64_t y_lo, y_hi;   /* column range of the tree's frame: columns addr / (ld * sizeof_type) in [y_lo, y_hi) */
+    uint64_t budget_bytes; /* device memory budget of the tree state; 0 = unlimited */
+    uint64_t pad;
+} xkrt_hip_mcc_config_t;
+
+typedef struct xkrt_hip_mcc_access {
+    uint64_t addr;   /* begin address in bytes (a multiple of sizeof_type) */
+    uint64_t m;      /* rows (elements); the tile must not wrap a column */
+    uint64_t n;      /* columns */
+    uint8_t  dev;    /* 0 = host, 1 = the GPU */
+    uint8_t  mode;   /* XKRT_HIP_MODE_* bits */
+    uint8_t  pad[6];
+} xkrt_hip_mcc_access_t;
+
+typedef struct xkrt_hip_mcc_stats {
+    uint64_t accesses;      /* accesses planned so far */
+    uint64_t gpu_accesses;  /* of which on the GPU */
+    uint64_t chunks;        /* GPU chunks allocated */
+    uint64_t last_parts;    /* valid-region rows of the last plan call */
+    uint64_t last_fetches;  /* fetch rows of the last plan call */
+    uint64_t device_bytes;  /* device memory of the tree state */
+    uint64_t poisoned;      /* 1 after an abort (ninth view) or a capacity error */
+    float    last_ms;       /* GPU time of the last plan call, first kernel to last */
+    uint32_t pad;
+} xkrt_hip_mcc_stats_t;
+
+typedef struct xkrt_hip_mcc *xkrt_hip_mcc_t;
+
+int xkrt_hip_mcc_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_mcc_config_t *cfg, void *stream,
+                        xkrt_hip_mcc_t *h);
+int xkrt_hip_mcc_destroy(xkrt_hip_mcc_t h);
+
+/* Plans n accesses in order (synchronous).  *nparts / *nfetches = rows of
+ * this call.  *fail_at = index in this call of the access at which the
+ * reference aborts with a ninth allocation view (memory-tree.hpp:1362-1365),
+ * or -1; the rows stop before it and the tree refuses later calls
+ * (-ECANCELED).  -ENOTSUP: a tile wrapping a column (two rects). */
+int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64_t n, uint64_t *nparts,
+                      uint64_t *nfetches, int64_t *fail_at);
+
+/* Valid-region split of the last plan call: part_off[n + 1] row offsets per
+ * access; parts = 6 int64 per row (x0, x1 bytes; y0, y1 columns;
+ * coherency bits before the access, 1 = host, 2 = GPU; fetching bits). */
+int xkrt_hip_mcc_parts_copy(xkrt_hip_mcc_t h, uint64_t *part_off, int64_t *parts, uint64_t cap);
+
+/* Fetch list of the last plan call: fetch_off[n + 1]; 8 int64 per row
+ * (x0, x1, y0, y1, src device, src chunk, dst device, dst chunk; chunk -1 =
+ * host memory; chunks are numbered in allocation order from 0). */
+int xkrt_hip_mcc_fetches_copy(xkrt_hip_mcc_t h, uint64_t *fetch_off, int64_t *fetches, uint64_t cap);
+
+/* who_owns for n rects on the current state: device bitfield owning the
+ * most bytes (ties: several bits; 0 when nobody owns a byte). */
+int xkrt_hip_mcc_who_owns(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64_t n, uint32_t *owners);
+
+int xkrt_hip_mcc_stats(xkrt_hip_mcc_t h, xkrt_hip_mcc_stats_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,7 +20,7 @@ def lib():
         p, u = C.c_void_p, C.c_uint64
         _lib.xko_resolve.argtypes = [u, p, p, p, p, u, p, p]
         _lib.xko_resolve.restype = C.c_int64
-        _lib.xko_coherency.argtypes = [u, p, p, p, p, p, u, p, p, u, p, p, p]
+        _lib.xko_coherency.argtypes = [u, p, p, p, p, p, u, p, p, u, p, p, p, u, p, p, p]
         _lib.xko_coherency.restype = C.c_int64
         _lib.xko_partition.argtypes = [u, p, p, p, u, p, p]
         _lib.xko_partition.restype = C.c_int64
@@ -71,6 +71,8 @@ def partition(dev, rects, nrect):
 class CoherencyPlan:
     """Per-access valid-region split and fetch list of one coherency tree."""
 
+    owners = None  # who_owns answers for the queries, on the final state
+
     def __init__(self, part_off, parts, fetch_off, fetches, fail_at):
         self.part_off = part_off
         self.parts = parts          # (P, 6) x0 x1 y0 y1 coh_before fetch_before
@@ -85,8 +87,15 @@ class CoherencyPlan:
         return self.fetches[self.fetch_off[i]:self.fetch_off[i + 1]]
 
 
-def coherency(dev, mode, rects, nrect, chunk_bytes):
+def coherency(dev, mode, rects, nrect, chunk_bytes, queries=None):
+    """queries: optional (rects, nrect) answered with who_owns after the stream."""
     L = lib()
+    if queries is not None:
+        qr, qn = _c(queries[0], np.uint64), _c(queries[1], np.uint8)
+    else:
+        qr, qn = np.zeros((1, 8), np.uint64), np.zeros(1, np.uint8)
+    nq = len(qn) if queries is not None else 0
+    owners = np.zeros(max(nq, 1), np.uint32)
     n = len(dev)
     dev = _c(dev, np.uint8); mode = _c(mode, np.uint8)
     rects = _c(rects, np.uint64); nrect = _c(nrect, np.uint8)
@@ -98,7 +107,8 @@ def coherency(dev, mode, rects, nrect, chunk_bytes):
         parts = np.empty((pc, 6), np.int64); fetches = np.empty((fc, 8), np.int64)
         rc = L.xko_coherency(n, dev.ctypes.data, mode.ctypes.data, nrect.ctypes.data, rects.ctypes.data,
                              cb.ctypes.data, pc, po.ctypes.data, parts.ctypes.data, fc, fo.ctypes.data,
-                             fetches.ctypes.data, fail.ctypes.data)
+                             fetches.ctypes.data, fail.ctypes.data, nq, qn.ctypes.data, qr.ctypes.data,
+                             owners.ctypes.data)
         last = int(fail[0]) if rc < 0 else n
         tp, tf = int(po[last]), int(fo[last])
         if tp <= pc and tf <= fc:
@@ -108,4 +118,6 @@ def coherency(dev, mode, rects, nrect, chunk_bytes):
     if rc < 0:
         po[last + 1:] = po[last]
         fo[last + 1:] = fo[last]
-    return CoherencyPlan(po, parts[:po[last]].copy(), fo, fetches[:fo[last]].copy(), int(fail[0]) if rc < 0 else -1)
+    plan = CoherencyPlan(po, parts[:po[last]].copy(), fo, fetches[:fo[last]].copy(), int(fail[0]) if rc < 0 else -1)
+    plan.owners = owners[:nq]
+    return plan

@@ -543,10 +543,52 @@ static void part_cb(void *pc, uint64_t x0, uint64_t x1, void *ctx)
  * (memory-tree.hpp:1362-1365 LOGGER_FATAL); *fail_at = its index and the
  * offsets stop there.
  */
+typedef struct {
+    uint64_t x0, x1, y0, y1;
+    uint64_t bytes[2];
+} octx_t;
+
+/* SEARCH_OWNERS (memory-tree.hpp:2130-2139): bytes of the query inside the
+ * fragment, added to every device whose coherency bit the fragment holds */
+static void own_cb(void *pc, uint64_t x0, uint64_t x1, void *ctx)
+{
+    octx_t *o = ctx;
+    const cpiece_t *p = pc;
+    const uint64_t ax0 = x0 > o->x0 ? x0 : o->x0, ax1 = x1 < o->x1 ? x1 : o->x1;
+    const uint64_t ay0 = p->y0 > o->y0 ? p->y0 : o->y0, ay1 = p->y1 < o->y1 ? p->y1 : o->y1;
+    if (ax0 >= ax1 || ay0 >= ay1) return;
+    for (int d = 0; d < 2; ++d)
+        if (p->coh & (1u << d)) o->bytes[d] += (ax1 - ax0) * (ay1 - ay0);
+}
+
+/* who_owns (memory-tree.hpp:1903-1933): the devices owning the most bytes of
+ * the query rects, ties included, 0 when nobody owns a byte */
+static uint32_t who_owns(flat_t *f, const uint64_t *q, int nr)
+{
+    octx_t o = {0, 0, 0, 0, {0, 0}};
+    for (int r = 0; r < nr; ++r) {
+        o.x0 = q[4 * r]; o.x1 = q[4 * r + 1]; o.y0 = q[4 * r + 2]; o.y1 = q[4 * r + 3];
+        flat_intersect(f, o.x0, o.x1, o.y0, o.y1, own_cb, &o);
+    }
+    uint32_t owners = 0;
+    uint64_t best = 0;
+    for (int d = 0; d < 2; ++d) {
+        if (best < o.bytes[d]) {
+            best = o.bytes[d];
+            owners = 1u << d;
+        } else if (best && best == o.bytes[d]) {
+            owners |= 1u << d;
+        }
+    }
+    return owners;
+}
+
+/* queries answered on the state after the stream (or at the abort):
+ * nq rects of 8 u64 (two rects, 4 u64 each) with qnrect[q] rects each */
 int64_t xko_coherency(uint64_t nacc, const uint8_t *dev, const uint8_t *mode, const uint8_t *nrect,
                       const uint64_t *rects, const uint64_t *chunk_bytes, uint64_t part_cap, int64_t *part_off,
                       int64_t *parts, uint64_t fetch_cap, int64_t *fetch_off, int64_t *fetches,
-                      int64_t *fail_at)
+                      int64_t *fail_at, uint64_t nq, const uint8_t *qnrect, const uint64_t *qrects, uint32_t *owners)
 {
     flat_t f = {&CPT, NULL, 0, 0};
     pvec_t pv = {NULL, 0, 0};
@@ -710,6 +752,7 @@ done:
         part_off[nacc] = (int64_t)np_;
         fetch_off[nacc] = (int64_t)nf;
     }
+    for (uint64_t q = 0; q < nq; ++q) owners[q] = who_owns(&f, qrects + 8 * q, qnrect[q]);
     /* on failure part_off[a] / fetch_off[a] already hold the start of the
      * failing access: its partites are not part of the plan */
     free(pv.v);

@@ -72,10 +72,11 @@ def resolve(task, mode, rects, nrect, xdim, ydim):
     return edges
 
 
-def coherency(dev, mode, rects, nrect, chunk_bytes, xdim, ydim):
+def coherency(dev, mode, rects, nrect, chunk_bytes, xdim, ydim, want_state=False):
     """Per access: (coh_before map {(x,y): bits}, fetch map {(x,y): (src_dev,
     src_chunk, dst_dev, dst_chunk)}), or the index of the access that would
-    exceed 8 allocation views (then the lists stop before it)."""
+    exceed 8 allocation views (then the lists stop before it).  With
+    want_state, the per-point coherency bits at the end come third."""
     n = len(dev)
     coh = np.zeros((xdim, ydim), np.int64)
     views = [[[] for _ in range(ydim)] for _ in range(xdim)]
@@ -118,7 +119,7 @@ def coherency(dev, mode, rects, nrect, chunk_bytes, xdim, ydim):
         else:
             for p in pts:
                 if len(views[p[0]][p[1]]) >= 8:
-                    return out, a
+                    return (out, a, coh) if want_state else (out, a)
             chunk = len(chunk_size)
             chunk_size.append(int(chunk_bytes[a]))
             for p in pts:
@@ -142,4 +143,4 @@ def coherency(dev, mode, rects, nrect, chunk_bytes, xdim, ydim):
             vcoh[p[0]][p[1]].add(chunk)
             coh[p] |= 2
         out.append((before, fetch))
-    return out, -1
+    return (out, -1, coh) if want_state else (out, -1)

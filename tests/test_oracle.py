@@ -188,3 +188,37 @@ def test_coherency_known_answers():
     st2.dev[:] = 0
     plan = khp.coherency(st2.dev, st2.mode, r, nr, st2.m * st2.n * st2.s)
     assert len(plan.fetches) == 0 and len(plan.parts) == 0
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_who_owns_matches_pointwise(seed):
+    # memory-tree.hpp:1903-1933 on the final state: bytes per coherent device, the maxima win
+    rng = np.random.default_rng(100 + seed)
+    n, ld, cols, s = 200, 12, 14, 2
+    mode = rng.choice(np.array([1, 2, 3], np.uint8), n, p=[0.5, 0.25, 0.25])
+    m = rng.integers(1, 11, n).astype(np.uint64)
+    nn = rng.integers(1, 9, n).astype(np.uint64)
+    row = rng.integers(0, ld, n).astype(np.uint64)
+    col = rng.integers(0, cols, n).astype(np.uint64)
+    m = np.minimum(m, ld - row)
+    nn = np.minimum(nn, cols - col)
+    dev = rng.integers(0, 2, n).astype(np.uint8)
+    st = S.Stream(np.arange(n, dtype=np.uint32), np.zeros(n, np.uint16), mode, dev, (col * ld + row) * s,
+                  np.full(n, ld, np.uint64), m, nn, np.full(n, s, np.uint64))
+    r, nr = st.rects()
+    cb = st.m * st.n * st.s
+    q = st.take(slice(0, 60))
+    qr, qn = q.rects()
+    plan = khp.coherency(st.dev, st.mode, r, nr, cb, queries=(qr, qn))
+    _, fail, coh = pointwise.coherency(st.dev, st.mode, r, nr, cb, 24, 16, want_state=True)
+    assert plan.fail_at == fail
+    for k in range(len(qn)):
+        by = [0, 0]
+        for (x0, x1, y0, y1) in [tuple(int(v) for v in qr[k, 4 * j:4 * j + 4]) for j in range(qn[k])]:
+            for x in range(x0, x1):
+                for y in range(y0, y1):
+                    for d in range(2):
+                        by[d] += (int(coh[x, y]) >> d) & 1
+        best = max(by)
+        want = 0 if best == 0 else sum(1 << d for d in range(2) if by[d] == best)
+        assert int(plan.owners[k]) == want, k

@@ -1,0 +1,1102 @@
+// MI355X (gfx950, wave64) coherency planner for one (ld, sizeof_type) memory
+// tree with a host and one GPU: for every access, the valid-region split
+// (the tree's fragments inside the access, with their coherency bits before
+// it) and the fetch list, with the reference's semantics
+// (include/xkrt/memory/access/blas/memory-tree.hpp:1067-1215, 1293-1874,
+// 955-1061; synchronous completion: the fetches of an access complete before
+// the next access; no eviction).
+//
+// The tree is restated per point:
+//   * geometry: the fragments are the global dim-0 breakpoints (every x
+//     endpoint inserted so far) times, per slab, the dim-1 breakpoints of the
+//     rects covering that slab.  TX[u] and TY[u][y] hold the first access
+//     that introduced breakpoint u (x unit = sizeof_type bytes) and y (at
+//     x unit u), so access a sees exactly the breakpoints with a stamp <= a,
+//     and the fragments of every access of a batch are enumerated in
+//     parallel (c_count, c_rows);
+//   * allocation views: a GPU access a reuses the chunk of the earliest
+//     earlier GPU access whose rect contains a's rect (that access allocated
+//     it and appended it to every point of its rect; the reference takes the
+//     first chunk of the first partite's view list present in every partite,
+//     1376-1425); otherwise a allocates the next chunk (c_contain, c_chunks);
+//   * state: per point, coherency bits (1 host, 2 GPU), the view list (chunk
+//     ids in allocation order, at most 8: the reference aborts on a ninth,
+//     1362-1365) and the per-view coherency bits.  c_state replays a tile's
+//     accesses in order on 32 x 32 points, one workgroup per tile, and fills
+//     the payload of every fragment whose first point lies in the tile.
+//
+// HBM layout per tree: TX (nxu + 1 u32), TY (nxu x (ny + 1) u32, y fastest),
+// point state tile-major (u32 word + 8 u32 views per point), the GPU access
+// history (rect, bytes, earliest container, chunk), chunk sizes.  Per plan
+// call: access arrays (SoA), fragment rows (SoA x0, x1, y0, y1, payload,
+// source chunk, access), fetch rows (SoA), tile incidence lists.
+
+#include <hip/hip_runtime.h>
+
+#include <errno.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/xkrt_hip.h"
+
+namespace {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t VMAX = 8;                 // memory-tree.hpp:86 allocation views per replica
+constexpr uint32_t TW = 32, TH = 32;         // state tile: x units by columns
+constexpr uint32_t TP = TW * TH;
+constexpr uint32_t TTHREADS = 256;
+constexpr uint32_t PPT = TP / TTHREADS;      // points per thread
+constexpr uint32_t TLIST = 4096;             // accesses of one tile per plan call (LDS list)
+constexpr uint32_t SCAN_BLOCK = 1024, SCAN_PER = 4, SCAN_CHUNK = SCAN_BLOCK * SCAN_PER;
+
+enum : uint32_t { F_GPU = 1u, F_INS = 2u, F_R = 4u, F_W = 8u };
+enum : uint32_t { CERR_TILE = 1u };
+
+struct CCounters {
+    unsigned int nchunks;   // chunks allocated so far
+    unsigned int fail;      // lowest access id that would add a ninth view (NONE = none)
+    unsigned int err;       // CERR_* bits
+    unsigned int nact;      // active tiles this call
+    unsigned int rows;      // fragment rows this call
+    unsigned int fetches;   // fetch rows this call
+    unsigned int viewerr;   // lowest access id whose chunk was missing from a point (NONE = none)
+    unsigned int pad;
+};
+
+struct CArgs {
+    // batch (index i in [0, n))
+    uint32_t n, gid0;
+    const uint32_t *bx0, *bx1, *by0, *by1, *bflags, *bhist, *bxoff;
+    uint32_t *pcount, *rowbase;
+    // tree
+    uint32_t nxu, ny, ny1, s;
+    uint32_t *TX, *TY;
+    uint32_t ntx, nty;
+    uint32_t *pst, *pviews;
+    // GPU history
+    const uint32_t *hx0, *hx1, *hy0, *hy1;
+    const unsigned long long *hbytes;
+    uint32_t *hbeta, *hchunk;
+    unsigned long long *chunk_size;
+    // per-slab fragment index
+    uint32_t *xcnt, *xrow;
+    // fragment rows
+    uint32_t *rx0, *rx1, *ry0, *ry1, *rpay, *rsrc, *racc;
+    uint32_t *rflag, *rfpos;
+    // fetch rows
+    uint32_t *fx0, *fx1, *fy0, *fy1, *fsd, *fsc, *fdd, *fdc;
+    uint32_t *foff;
+    // tiles
+    uint32_t *tcnt, *tcur, *tstart, *tinc, *tact;
+    CCounters *ctr;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+__device__ __forceinline__ uint32_t wave_bcast(uint32_t v, int src) { return __shfl(v, src, 64); }
+
+// ---------------------------------------------------------------- views
+
+// earliest earlier GPU access of the tree whose rect contains this one's
+__global__ void c_contain(CArgs a, uint32_t h0, uint32_t nh)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t k = wid; k < nh; k += nw) {
+        const uint32_t h = h0 + k;
+        const uint32_t x0 = a.hx0[h], x1 = a.hx1[h], y0 = a.hy0[h], y1 = a.hy1[h];
+        uint32_t found = NONE;
+        for (uint32_t c = 0; c < h && found == NONE; c += 64) {
+            const uint32_t j = c + lane;
+            const bool in = j < h && a.hx0[j] <= x0 && x1 <= a.hx1[j] && a.hy0[j] <= y0 && y1 <= a.hy1[j];
+            const uint64_t m = __ballot(in);
+            if (m) found = c + (uint32_t)__builtin_ctzll(m);
+        }
+        if (lane == 0) a.hbeta[h] = found;
+    }
+}
+
+// chunk ids: allocating accesses in order get the next ids; the others take
+// their earliest container's chunk (that container allocated)
+__global__ void __launch_bounds__(1024) c_chunks(CArgs a, uint32_t h0, uint32_t nh)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t base;
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) base = a.ctr->nchunks;
+    __syncthreads();
+    for (uint32_t c = 0; c < nh; c += 1024) {
+        const uint32_t k = c + threadIdx.x;
+        const bool al = k < nh && a.hbeta[h0 + k] == NONE;
+        const uint64_t m = __ballot(al);
+        if (lane == 0) wsum[w] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t off = 0, tot = 0;
+        for (uint32_t q = 0; q < 16; ++q) {
+            off += q < w ? wsum[q] : 0u;
+            tot += wsum[q];
+        }
+        if (al) {
+            const uint32_t ch = base + off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            a.hchunk[h0 + k] = ch;
+            a.chunk_size[ch] = a.hbytes[h0 + k];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) base += tot;
+        __syncthreads();
+    }
+    __threadfence();
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nh; k += 1024) {
+        const uint32_t b = a.hbeta[h0 + k];
+        if (b != NONE) a.hchunk[h0 + k] = a.hchunk[b];
+    }
+    if (threadIdx.x == 0) a.ctr->nchunks = base;
+}
+
+// ---------------------------------------------------------------- geometry
+
+__global__ void c_breaks(CArgs a)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = wid; i < a.n; i += nw) {
+        if (!(a.bflags[i] & F_INS)) continue;
+        const uint32_t aid = a.gid0 + i;
+        const uint32_t x0 = a.bx0[i], x1 = a.bx1[i], y0 = a.by0[i], y1 = a.by1[i];
+        if (lane == 0) {
+            atomicMin(&a.TX[x0], aid);
+            atomicMin(&a.TX[x1], aid);
+        }
+        for (uint32_t u = x0 + lane; u < x1; u += 64) {
+            atomicMin(&a.TY[(size_t)u * a.ny1 + y0], aid);
+            atomicMin(&a.TY[(size_t)u * a.ny1 + y1], aid);
+        }
+    }
+}
+
+// fragments of each access: slabs x pieces, counted per slab
+__global__ void c_count(CArgs a)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = wid; i < a.n; i += nw) {
+        uint32_t total = 0;
+        if (a.bflags[i] & F_INS) {
+            const uint32_t aid = a.gid0 + i;
+            const uint32_t x0 = a.bx0[i], x1 = a.bx1[i], y0 = a.by0[i], y1 = a.by1[i], xo = a.bxoff[i];
+            for (uint32_t uc = x0; uc < x1; uc += 64) {
+                const uint32_t u = uc + lane;
+                uint64_t m = __ballot(u < x1 && (u == x0 || a.TX[u] <= aid));
+                while (m) {
+                    const uint32_t us = uc + (uint32_t)__builtin_ctzll(m);
+                    m &= m - 1;
+                    const uint32_t *row = a.TY + (size_t)us * a.ny1;
+                    uint32_t np = 0;
+                    for (uint32_t yc = y0; yc < y1; yc += 64) {
+                        const uint32_t y = yc + lane;
+                        np += (uint32_t)__popcll(__ballot(y < y1 && (y == y0 || row[y] <= aid)));
+                    }
+                    if (lane == 0) a.xcnt[xo + (us - x0)] = np;
+                    total += np;
+                }
+            }
+        }
+        if (lane == 0) a.pcount[i] = total;
+    }
+}
+
+// the fragment rows of one slab [us, ue) of access i, from row index r
+__device__ void slab_rows(const CArgs &a, uint32_t i, uint32_t aid, uint32_t us, uint32_t ue, uint32_t y0,
+                          uint32_t y1, uint32_t r)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t *row = a.TY + (size_t)us * a.ny1;
+    uint32_t pend_y = NONE, pend_r = 0;
+    for (uint32_t yc = y0; yc < y1; yc += 64) {
+        const uint32_t y = yc + lane;
+        const bool ps = y < y1 && (y == y0 || row[y] <= aid);
+        const uint64_t m = __ballot(ps);
+        if (!m) continue;
+        if (pend_y != NONE && lane == 0) {
+            a.rx0[pend_r] = us * a.s; a.rx1[pend_r] = ue * a.s;
+            a.ry0[pend_r] = pend_y; a.ry1[pend_r] = yc + (uint32_t)__builtin_ctzll(m);
+            a.racc[pend_r] = i;
+        }
+        const uint64_t above = lane == 63 ? 0ull : (m & (~0ull << (lane + 1)));
+        const uint32_t idx = r + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (ps && above) {
+            a.rx0[idx] = us * a.s; a.rx1[idx] = ue * a.s;
+            a.ry0[idx] = y; a.ry1[idx] = yc + (uint32_t)__builtin_ctzll(above);
+            a.racc[idx] = i;
+        }
+        const uint32_t last = 63u - (uint32_t)__builtin_clzll(m);
+        r += (uint32_t)__popcll(m);
+        pend_y = yc + last;
+        pend_r = r - 1;
+    }
+    if (pend_y != NONE && lane == 0) {
+        a.rx0[pend_r] = us * a.s; a.rx1[pend_r] = ue * a.s;
+        a.ry0[pend_r] = pend_y; a.ry1[pend_r] = y1;
+        a.racc[pend_r] = i;
+    }
+}
+
+__global__ void c_rows(CArgs a)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = wid; i < a.n; i += nw) {
+        if (!(a.bflags[i] & F_INS)) continue;
+        const uint32_t aid = a.gid0 + i;
+        const uint32_t x0 = a.bx0[i], x1 = a.bx1[i], y0 = a.by0[i], y1 = a.by1[i], xo = a.bxoff[i];
+        uint32_t r = a.rowbase[i];
+        uint32_t pend = NONE;
+        for (uint32_t uc = x0; uc < x1; uc += 64) {
+            const uint32_t u = uc + lane;
+            uint64_t m = __ballot(u < x1 && (u == x0 || a.TX[u] <= aid));
+            while (m) {
+                const uint32_t us = uc + (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                if (pend != NONE) {
+                    if (lane == 0) a.xrow[xo + (pend - x0)] = r;
+                    slab_rows(a, i, aid, pend, us, y0, y1, r);
+                    r += a.xcnt[xo + (pend - x0)];
+                }
+                pend = us;
+            }
+        }
+        if (pend != NONE) {
+            if (lane == 0) a.xrow[xo + (pend - x0)] = r;
+            slab_rows(a, i, aid, pend, x1, y0, y1, r);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- tiles
+
+__device__ __forceinline__ void tile_range(const CArgs &a, uint32_t i, uint32_t &tx0, uint32_t &tx1, uint32_t &ty0,
+                                           uint32_t &ty1)
+{
+    tx0 = a.bx0[i] / TW; tx1 = (a.bx1[i] - 1) / TW;
+    ty0 = a.by0[i] / TH; ty1 = (a.by1[i] - 1) / TH;
+}
+
+__global__ void c_tcount(CArgs a)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = wid; i < a.n; i += nw) {
+        if (!(a.bflags[i] & F_INS)) continue;
+        uint32_t tx0, tx1, ty0, ty1;
+        tile_range(a, i, tx0, tx1, ty0, ty1);
+        const uint32_t w = tx1 - tx0 + 1, nt = w * (ty1 - ty0 + 1);
+        for (uint32_t k = lane; k < nt; k += 64) atomicAdd(&a.tcnt[(ty0 + k / w) * a.ntx + tx0 + k % w], 1u);
+    }
+}
+
+__global__ void c_tfill(CArgs a)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t i = wid; i < a.n; i += nw) {
+        if (!(a.bflags[i] & F_INS)) continue;
+        uint32_t tx0, tx1, ty0, ty1;
+        tile_range(a, i, tx0, tx1, ty0, ty1);
+        const uint32_t w = tx1 - tx0 + 1, nt = w * (ty1 - ty0 + 1);
+        for (uint32_t k = lane; k < nt; k += 64) {
+            const uint32_t t = (ty0 + k / w) * a.ntx + tx0 + k % w;
+            a.tinc[a.tstart[t] + atomicAdd(&a.tcur[t], 1u)] = i;
+        }
+    }
+}
+
+__global__ void c_tact(CArgs a, uint32_t ntiles)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = t < ntiles && a.tcnt[t] > 0;
+    const uint64_t m = __ballot(on);
+    if (!m) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(&a.ctr->nact, (uint32_t)__popcll(m));
+    base = wave_bcast(base, 0);
+    if (on) a.tact[base + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull))] = t;
+}
+
+// ---------------------------------------------------------------- state
+
+// Replay the tile's accesses in order on its 32 x 32 points (4 per thread,
+// views in registers through constant-index selects).
+__global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
+{
+    __shared__ uint32_t L[TLIST];
+    const uint32_t nact = a.ctr->nact;
+    for (uint32_t b = blockIdx.x; b < nact; b += gridDim.x) {
+        const uint32_t tile = a.tact[b];
+        const uint32_t cnt = a.tcnt[tile], start = a.tstart[tile];
+        __syncthreads();
+        if (cnt > TLIST) {
+            if (threadIdx.x == 0) atomicOr(&a.ctr->err, CERR_TILE);
+            continue;
+        }
+        uint32_t n2 = 1;
+        while (n2 < cnt) n2 <<= 1;
+        for (uint32_t q = threadIdx.x; q < n2; q += TTHREADS) L[q] = q < cnt ? a.tinc[start + q] : NONE;
+        __syncthreads();
+        for (uint32_t k = 2; k <= n2; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t q = threadIdx.x; q < n2; q += TTHREADS) {
+                    const uint32_t p = q ^ j;
+                    if (p > q) {
+                        const uint32_t x = L[q], y = L[p];
+                        if ((x > y) == ((q & k) == 0)) { L[q] = y; L[p] = x; }
+                    }
+                }
+                __syncthreads();
+            }
+
+        const uint32_t tx0 = (tile % a.ntx) * TW, ty0 = (tile / a.ntx) * TH;
+        uint32_t st[PPT], vw[PPT][VMAX], px[PPT], py[PPT];
+        bool valid[PPT];
+#pragma unroll
+        for (uint32_t k = 0; k < PPT; ++k) {
+            const uint32_t p = threadIdx.x + TTHREADS * k;
+            px[k] = tx0 + p % TW;
+            py[k] = ty0 + p / TW;
+            valid[k] = px[k] < a.nxu && py[k] < a.ny;
+            st[k] = a.pst[(size_t)tile * TP + p];
+#pragma unroll
+            for (uint32_t v = 0; v < VMAX; ++v) vw[k][v] = a.pviews[((size_t)tile * VMAX + v) * TP + p];
+        }
+
+        for (uint32_t q = 0; q < cnt; ++q) {
+            const uint32_t i = L[q];
+            const uint32_t aid = a.gid0 + i;
+            const uint32_t x0 = a.bx0[i], x1 = a.bx1[i], y0 = a.by0[i], y1 = a.by1[i], fl = a.bflags[i];
+            const uint32_t chunk = (fl & F_GPU) ? a.hchunk[a.bhist[i]] : NONE;
+            const bool alloc = (fl & F_GPU) && a.hbeta[a.bhist[i]] == NONE;
+#pragma unroll
+            for (uint32_t k = 0; k < PPT; ++k) {
+                if (!valid[k] || px[k] < x0 || px[k] >= x1 || py[k] < y0 || py[k] >= y1) continue;
+                uint32_t coh = st[k] & 3u, vc = (st[k] >> 2) & 0xffu, nv = (st[k] >> 10) & 0xfu;
+                const uint32_t before = coh;
+                bool need = false;
+                uint32_t sdev = 0, schunk = NONE;
+                if (fl & F_GPU) {
+                    uint32_t v = NONE;
+                    if (alloc) {
+                        if (nv >= VMAX) {
+                            atomicMin(&a.ctr->fail, aid);
+                            continue;
+                        }
+#pragma unroll
+                        for (uint32_t t = 0; t < VMAX; ++t)
+                            if (t == nv) vw[k][t] = chunk;
+                        v = nv++;
+                    } else {
+#pragma unroll
+                        for (uint32_t t = 0; t < VMAX; ++t)
+                            if (t < nv && vw[k][t] == chunk && v == NONE) v = t;
+                        if (v == NONE) {  // only after an abort has left this point's views short
+                            atomicMin(&a.ctr->viewerr, aid);
+                            continue;
+                        }
+                    }
+                    const uint32_t bit = 1u << v;
+                    if ((fl & F_R) && !(vc & bit)) {
+                        // fetch_access_setup_copies (1471-1631): from a coherent view of the GPU, else from host
+                        need = true;
+                        if (coh & 2u) {
+                            const uint32_t sv = (uint32_t)__builtin_ctz(vc);
+                            sdev = 1;
+#pragma unroll
+                            for (uint32_t t = 0; t < VMAX; ++t)
+                                if (t == sv) schunk = vw[k][t];
+                        }
+                    }
+                    if (fl & F_W) {  // fetch_access_set_coherent (1633-1667)
+                        coh = 2u;
+                        vc = bit;
+                    }
+                    if (need) {      // completion (2067-2127)
+                        vc |= bit;
+                        coh |= 2u;
+                    }
+                } else if (coh == 2u) {
+                    // fetch_list_to_host_setup_partition (1110-1181): from the GPU view of the largest chunk
+                    need = true;
+                    sdev = 1;
+                    unsigned long long best = 0;
+#pragma unroll
+                    for (uint32_t t = 0; t < VMAX; ++t) {
+                        if (t < nv) {
+                            const unsigned long long sz = a.chunk_size[vw[k][t]];
+                            if (t == 0 || best < sz) {
+                                best = sz;
+                                schunk = vw[k][t];
+                            }
+                        }
+                    }
+                    coh |= 1u;
+                }
+                st[k] = coh | (vc << 2) | (nv << 10);
+                // the first point of one of the access's fragments: fill that row
+                const uint32_t u = px[k], y = py[k];
+                if ((u == x0 || a.TX[u] <= aid) && (y == y0 || a.TY[(size_t)u * a.ny1 + y] <= aid)) {
+                    const uint32_t xo = a.bxoff[i] + (u - x0);
+                    uint32_t lo = a.xrow[xo], hi = lo + a.xcnt[xo];
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (a.ry0[mid] <= y) lo = mid; else hi = mid;
+                    }
+                    a.rpay[lo] = before | ((need ? 1u : 0u) << 8) | (sdev << 9);
+                    a.rsrc[lo] = schunk;
+                    a.rflag[lo] = need ? 1u : 0u;
+                }
+            }
+        }
+
+#pragma unroll
+        for (uint32_t k = 0; k < PPT; ++k) {
+            const uint32_t p = threadIdx.x + TTHREADS * k;
+            a.pst[(size_t)tile * TP + p] = st[k];
+#pragma unroll
+            for (uint32_t v = 0; v < VMAX; ++v) a.pviews[((size_t)tile * VMAX + v) * TP + p] = vw[k][v];
+        }
+        if (threadIdx.x == 0) {
+            a.tcnt[tile] = 0;
+            a.tcur[tile] = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- fetch rows
+
+__global__ void c_fetch(CArgs a, uint32_t nrows)
+{
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += gridDim.x * blockDim.x) {
+        if (!a.rflag[r]) continue;
+        const uint32_t f = a.rfpos[r], i = a.racc[r];
+        a.fx0[f] = a.rx0[r]; a.fx1[f] = a.rx1[r]; a.fy0[f] = a.ry0[r]; a.fy1[f] = a.ry1[r];
+        a.fsd[f] = (a.rpay[r] >> 9) & 1u;
+        a.fsc[f] = a.rsrc[r];
+        const bool gpu = a.bflags[i] & F_GPU;
+        a.fdd[f] = gpu ? 1u : 0u;
+        a.fdc[f] = gpu ? a.hchunk[a.bhist[i]] : NONE;
+    }
+}
+
+__global__ void c_foff(CArgs a, uint32_t nrows, uint32_t nfetch)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n) a.foff[i] = a.rowbase[i] < nrows ? a.rfpos[a.rowbase[i]] : nfetch;
+}
+
+// ---------------------------------------------------------------- who_owns
+
+// memory-tree.hpp:1903-1933 (SEARCH_OWNERS 2130-2139): bytes per coherent
+// device over the query rect, then the devices owning the most bytes
+__global__ void __launch_bounds__(256) c_owners(CArgs a, const uint32_t *qx0, const uint32_t *qx1,
+                                                const uint32_t *qy0, const uint32_t *qy1, uint32_t nq,
+                                                uint32_t *owners)
+{
+    __shared__ unsigned long long part[2][4];
+    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        const uint32_t x0 = qx0[q], x1 = qx1[q], y0 = qy0[q], y1 = qy1[q];
+        const uint32_t w = x1 - x0;
+        const uint64_t npts = (uint64_t)w * (y1 - y0);
+        unsigned long long h = 0, g = 0;
+        for (uint64_t p = threadIdx.x; p < npts; p += blockDim.x) {
+            const uint32_t u = x0 + (uint32_t)(p % w), y = y0 + (uint32_t)(p / w);
+            const uint32_t tile = (y / TH) * a.ntx + u / TW;
+            const uint32_t c = a.pst[(size_t)tile * TP + (y % TH) * TW + (u % TW)] & 3u;
+            h += c & 1u;
+            g += (c >> 1) & 1u;
+        }
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            h += __shfl_down(h, d, 64);
+            g += __shfl_down(g, d, 64);
+        }
+        if (lane_id() == 0) {
+            part[0][threadIdx.x >> 6] = h;
+            part[1][threadIdx.x >> 6] = g;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long bh = (part[0][0] + part[0][1] + part[0][2] + part[0][3]) * a.s;
+            const unsigned long long bg = (part[1][0] + part[1][1] + part[1][2] + part[1][3]) * a.s;
+            uint32_t o = 0;
+            if (bh > 0 || bg > 0) o = bh > bg ? 1u : (bg > bh ? 2u : 3u);
+            owners[q] = o;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- scan
+
+__global__ void __launch_bounds__(SCAN_BLOCK) s_reduce(const uint32_t *in, uint32_t n, uint32_t *bsum)
+{
+    __shared__ uint32_t ws[SCAN_BLOCK / 64];
+    const uint32_t base = blockIdx.x * SCAN_CHUNK;
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER; ++k) {
+        const uint32_t i = base + k * SCAN_BLOCK + threadIdx.x;
+        v += i < n ? in[i] : 0u;
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    if (lane_id() == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < SCAN_BLOCK / 64; ++w) t += ws[w];
+        bsum[blockIdx.x] = t;
+    }
+}
+
+// exclusive scan of the block sums in place; total to *total
+__global__ void __launch_bounds__(1024) s_top(uint32_t *bsum, uint32_t nb, uint32_t *total)
+{
+    __shared__ uint32_t ws[16];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t c = 0; c < nb; c += 1024) {
+        const uint32_t i = c + threadIdx.x;
+        const uint32_t v = i < nb ? bsum[i] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, 64);
+            if (lane_id() >= d) x += y;
+        }
+        if (lane_id() == 63) ws[threadIdx.x >> 6] = x;
+        __syncthreads();
+        uint32_t off = carry;
+        for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) off += ws[w];
+        if (i < nb) bsum[i] = off + x - v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (uint32_t w = 0; w < 16; ++w) t += ws[w];
+            carry += t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) s_down(const uint32_t *in, uint32_t n, const uint32_t *bsum,
+                                                     uint32_t *out)
+{
+    __shared__ uint32_t ws[SCAN_BLOCK / 64];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = bsum[blockIdx.x];
+    __syncthreads();
+    const uint32_t base = blockIdx.x * SCAN_CHUNK;
+    for (uint32_t k = 0; k < SCAN_PER; ++k) {
+        const uint32_t i = base + k * SCAN_BLOCK + threadIdx.x;
+        const uint32_t v = i < n ? in[i] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, 64);
+            if (lane_id() >= d) x += y;
+        }
+        if (lane_id() == 63) ws[threadIdx.x >> 6] = x;
+        __syncthreads();
+        uint32_t off = carry;
+        for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) off += ws[w];
+        if (i < n) out[i] = off + x - v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (uint32_t w = 0; w < SCAN_BLOCK / 64; ++w) t += ws[w];
+            carry += t;
+        }
+        __syncthreads();
+    }
+}
+
+int cerr(hipError_t e)
+{
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorOutOfMemory) return -ENOMEM;
+    return -ENODEV;
+}
+
+#define CCHK(x) do { hipError_t _e = (x); if (_e != hipSuccess) return cerr(_e); } while (0)
+
+template <typename T>
+int grow(T **p, uint64_t *cap, uint64_t need, bool keep, hipStream_t s)
+{
+    if (need <= *cap) return 0;
+    uint64_t nc = std::max<uint64_t>(need, *cap + *cap / 2);
+    nc = std::max<uint64_t>(nc, 1024);
+    T *q = nullptr;
+    CCHK(hipMalloc(&q, nc * sizeof(T)));
+    if (keep && *p && *cap) CCHK(hipMemcpyAsync(q, *p, *cap * sizeof(T), hipMemcpyDeviceToDevice, s));
+    if (*p) {
+        CCHK(hipStreamSynchronize(s));
+        (void)hipFree(*p);
+    }
+    *p = q;
+    *cap = nc;
+    return 0;
+}
+
+uint32_t waves_grid(uint64_t n) { return (uint32_t)std::min<uint64_t>(std::max<uint64_t>((n + 3) / 4, 1), 65536); }
+
+} // namespace
+
+struct xkrt_hip_mcc {
+    uint64_t ld = 0, s = 0, LDs = 0;
+    xkrt_hip_mcc_config_t cfg{};
+    uint32_t nxu = 0, ny = 0, ntx = 0, nty = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool gpu_seen = false;
+    bool poisoned = false;
+    uint64_t naccess = 0;   // accesses planned so far (global ids)
+    uint32_t nhist = 0;     // GPU accesses so far
+    uint64_t device_bytes = 0;
+    // tree
+    uint32_t *TX = nullptr, *TY = nullptr, *pst = nullptr, *pviews = nullptr;
+    uint32_t *tcnt = nullptr, *tcur = nullptr, *tstart = nullptr, *tact = nullptr;
+    uint32_t *tbsum = nullptr;
+    uint64_t tbsum_cap = 0;
+    CCounters *ctr = nullptr, *ctr_host = nullptr;
+    // GPU history
+    uint32_t *hx0 = nullptr, *hx1 = nullptr, *hy0 = nullptr, *hy1 = nullptr, *hbeta = nullptr, *hchunk = nullptr;
+    unsigned long long *hbytes = nullptr;
+    uint64_t hcap = 0, hcap1 = 0, hcap2 = 0, hcap3 = 0, hcap4 = 0, hcap5 = 0, hcap6 = 0;
+    unsigned long long *chunk_size = nullptr;
+    uint64_t chunk_cap = 0;
+    // batch
+    uint32_t *bx0 = nullptr, *bx1 = nullptr, *by0 = nullptr, *by1 = nullptr, *bflags = nullptr, *bhist = nullptr,
+             *bxoff = nullptr, *pcount = nullptr, *rowbase = nullptr, *foff = nullptr;
+    uint64_t bcap[10] = {};
+    uint32_t *xcnt = nullptr, *xrow = nullptr;
+    uint64_t xcap[2] = {};
+    uint32_t *tinc = nullptr;
+    uint64_t tinc_cap = 0;
+    uint32_t *bsum = nullptr;
+    uint64_t bsum_cap = 0;
+    // rows
+    uint32_t *rx0 = nullptr, *rx1 = nullptr, *ry0 = nullptr, *ry1 = nullptr, *rpay = nullptr, *rsrc = nullptr,
+             *racc = nullptr, *rflag = nullptr, *rfpos = nullptr;
+    uint64_t rcap[9] = {};
+    uint32_t *fx0 = nullptr, *fx1 = nullptr, *fy0 = nullptr, *fy1 = nullptr, *fsd = nullptr, *fsc = nullptr,
+             *fdd = nullptr, *fdc = nullptr;
+    uint64_t fcap[8] = {};
+    // results of the last plan call
+    uint32_t last_n = 0, last_rows = 0, last_fetch = 0;
+    int64_t last_fail = -1;
+    std::vector<uint32_t> h_rowbase, h_foff;
+    // timing of the last plan call (HIP events around its kernels)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+};
+
+namespace {
+
+// exclusive scan of n u32 into out; the total lands in *total (device)
+int scan(xkrt_hip_mcc *h, const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *total)
+{
+    const uint32_t nb = std::max<uint32_t>(1, (n + SCAN_CHUNK - 1) / SCAN_CHUNK);
+    int rc = grow(&h->bsum, &h->bsum_cap, nb, false, h->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(s_reduce, dim3(nb), dim3(SCAN_BLOCK), 0, h->stream, in, n, h->bsum);
+    hipLaunchKernelGGL(s_top, dim3(1), dim3(1024), 0, h->stream, h->bsum, nb, total);
+    hipLaunchKernelGGL(s_down, dim3(nb), dim3(SCAN_BLOCK), 0, h->stream, in, n, h->bsum, out);
+    return cerr(hipGetLastError());
+}
+
+CArgs make_cargs(xkrt_hip_mcc *h, uint32_t n)
+{
+    CArgs a{};
+    a.n = n;
+    a.gid0 = (uint32_t)h->naccess;
+    a.bx0 = h->bx0; a.bx1 = h->bx1; a.by0 = h->by0; a.by1 = h->by1; a.bflags = h->bflags; a.bhist = h->bhist;
+    a.bxoff = h->bxoff; a.pcount = h->pcount; a.rowbase = h->rowbase;
+    a.nxu = h->nxu; a.ny = h->ny; a.ny1 = h->ny + 1; a.s = (uint32_t)h->s;
+    a.TX = h->TX; a.TY = h->TY; a.ntx = h->ntx; a.nty = h->nty; a.pst = h->pst; a.pviews = h->pviews;
+    a.hx0 = h->hx0; a.hx1 = h->hx1; a.hy0 = h->hy0; a.hy1 = h->hy1; a.hbytes = h->hbytes;
+    a.hbeta = h->hbeta; a.hchunk = h->hchunk; a.chunk_size = h->chunk_size;
+    a.xcnt = h->xcnt; a.xrow = h->xrow;
+    a.rx0 = h->rx0; a.rx1 = h->rx1; a.ry0 = h->ry0; a.ry1 = h->ry1; a.rpay = h->rpay; a.rsrc = h->rsrc;
+    a.racc = h->racc; a.rflag = h->rflag; a.rfpos = h->rfpos;
+    a.fx0 = h->fx0; a.fx1 = h->fx1; a.fy0 = h->fy0; a.fy1 = h->fy1; a.fsd = h->fsd; a.fsc = h->fsc;
+    a.fdd = h->fdd; a.fdc = h->fdc; a.foff = h->foff;
+    a.tcnt = h->tcnt; a.tcur = h->tcur; a.tstart = h->tstart; a.tinc = h->tinc; a.tact = h->tact;
+    a.ctr = h->ctr;
+    return a;
+}
+
+void free_all(xkrt_hip_mcc *h)
+{
+    void *ptrs[] = {h->TX, h->TY, h->pst, h->pviews, h->tcnt, h->tcur, h->tstart, h->tact, h->tbsum, h->ctr,
+                    h->hx0, h->hx1, h->hy0, h->hy1, h->hbeta, h->hchunk, h->hbytes, h->chunk_size,
+                    h->bx0, h->bx1, h->by0, h->by1, h->bflags, h->bhist, h->bxoff, h->pcount, h->rowbase, h->foff,
+                    h->xcnt, h->xrow, h->tinc, h->bsum, h->rx0, h->rx1, h->ry0, h->ry1, h->rpay, h->rsrc, h->racc,
+                    h->rflag, h->rfpos, h->fx0, h->fx1, h->fy0, h->fy1, h->fsd, h->fsc, h->fdd, h->fdc};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->ctr_host) (void)hipHostFree(h->ctr_host);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+}
+
+} // namespace
+
+extern "C" {
+
+int xkrt_hip_mcc_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_mcc_config_t *cfg, void *stream,
+                        xkrt_hip_mcc_t *out)
+{
+    if (!out || !cfg || ld == 0 || sizeof_type == 0) return -EINVAL;
+    *out = nullptr;
+    if (cfg->y_hi <= cfg->y_lo || cfg->y_hi - cfg->y_lo > (1ull << 24) || ld > (1ull << 24)) return -EINVAL;
+    if (ld * sizeof_type > 0xFFFFFFFFull) return -ERANGE;
+    xkrt_hip_mcc *h = new xkrt_hip_mcc();
+    h->ld = ld; h->s = sizeof_type; h->LDs = ld * sizeof_type;
+    h->cfg = *cfg;
+    h->nxu = (uint32_t)ld;
+    h->ny = (uint32_t)(cfg->y_hi - cfg->y_lo);
+    h->ntx = (h->nxu + TW - 1) / TW;
+    h->nty = (h->ny + TH - 1) / TH;
+    const uint64_t ntiles = (uint64_t)h->ntx * h->nty;
+    const uint64_t tbytes = 4ull * (h->nxu + 1) + 4ull * h->nxu * (h->ny + 1ull) + ntiles * TP * 4ull * (1 + VMAX) +
+                            ntiles * 16ull;
+    if (cfg->budget_bytes && tbytes > cfg->budget_bytes) {
+        delete h;
+        return -ENOMEM;
+    }
+    int rc = 0;
+    auto chk = [&](hipError_t e) { if (!rc) rc = cerr(e); };
+    if (stream) {
+        h->stream = (hipStream_t)stream;
+    } else {
+        chk(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->own_stream = true;
+    }
+    chk(hipMalloc(&h->TX, 4ull * (h->nxu + 1)));
+    chk(hipMalloc(&h->TY, 4ull * h->nxu * (h->ny + 1ull)));
+    chk(hipMalloc(&h->pst, 4ull * ntiles * TP));
+    chk(hipMalloc(&h->pviews, 4ull * ntiles * TP * VMAX));
+    chk(hipMalloc(&h->tcnt, 4ull * ntiles));
+    chk(hipMalloc(&h->tcur, 4ull * ntiles));
+    chk(hipMalloc(&h->tstart, 4ull * ntiles));
+    chk(hipMalloc(&h->tact, 4ull * ntiles));
+    chk(hipMalloc(&h->ctr, sizeof(CCounters)));
+    chk(hipHostMalloc(&h->ctr_host, sizeof(CCounters), hipHostMallocDefault));
+    chk(hipEventCreate(&h->ev0));
+    chk(hipEventCreate(&h->ev1));
+    if (!rc) {
+        chk(hipMemsetAsync(h->TX, 0xFF, 4ull * (h->nxu + 1), h->stream));
+        chk(hipMemsetAsync(h->TY, 0xFF, 4ull * h->nxu * (h->ny + 1ull), h->stream));
+        chk(hipMemsetAsync(h->pst, 0, 4ull * ntiles * TP, h->stream));
+        chk(hipMemsetAsync(h->pviews, 0xFF, 4ull * ntiles * TP * VMAX, h->stream));
+        chk(hipMemsetAsync(h->tcnt, 0, 4ull * ntiles, h->stream));
+        chk(hipMemsetAsync(h->tcur, 0, 4ull * ntiles, h->stream));
+        CCounters c0{};
+        c0.fail = NONE;
+        c0.viewerr = NONE;
+        chk(hipMemcpyAsync(h->ctr, &c0, sizeof(c0), hipMemcpyHostToDevice, h->stream));
+        chk(hipStreamSynchronize(h->stream));
+    }
+    if (rc) {
+        free_all(h);
+        if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return rc;
+    }
+    h->device_bytes = tbytes;
+    *out = h;
+    return 0;
+}
+
+int xkrt_hip_mcc_destroy(xkrt_hip_mcc_t h)
+{
+    if (!h) return -EINVAL;
+    (void)hipStreamSynchronize(h->stream);
+    free_all(h);
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64_t n, uint64_t *nparts,
+                      uint64_t *nfetches, int64_t *fail_at)
+{
+    if (!h || (!acc && n)) return -EINVAL;
+    if (h->poisoned) return -ECANCELED;
+    if (fail_at) *fail_at = -1;
+    if (nparts) *nparts = 0;
+    if (nfetches) *nfetches = 0;
+    if (n == 0) return 0;
+    if (h->naccess + n >= NONE || n > (1ull << 30)) return -ERANGE;
+
+    // host side: rects in the tree frame, flags, history entries
+    std::vector<uint32_t> x0(n), x1(n), y0(n), y1(n), fl(n), hist(n, NONE), xoff(n);
+    std::vector<uint32_t> gx0, gx1, gy0, gy1;
+    std::vector<unsigned long long> gbytes;
+    uint64_t xtot = 0, ttot = 0;
+    bool seen = h->gpu_seen;
+    for (uint64_t i = 0; i < n; ++i) {
+        const xkrt_hip_mcc_access_t &q = acc[i];
+        if (q.m == 0 || q.n == 0 || q.dev > 1) return -EINVAL;
+        const uint64_t xb = q.addr % h->LDs, col = q.addr / h->LDs;
+        if (xb % h->s) return -EINVAL;
+        if (xb + q.m * h->s > h->LDs) return -ENOTSUP;  // column-wrapping tile: two rects
+        if (col < h->cfg.y_lo || col + q.n > h->cfg.y_hi) return -ERANGE;
+        x0[i] = (uint32_t)(xb / h->s);
+        x1[i] = (uint32_t)(xb / h->s + q.m);
+        y0[i] = (uint32_t)(col - h->cfg.y_lo);
+        y1[i] = (uint32_t)(col - h->cfg.y_lo + q.n);
+        uint32_t f = 0;
+        if (q.dev == 1) {
+            f |= F_GPU | F_INS;
+            seen = true;
+            hist[i] = h->nhist + (uint32_t)gx0.size();
+            gx0.push_back(x0[i]); gx1.push_back(x1[i]); gy0.push_back(y0[i]); gy1.push_back(y1[i]);
+            gbytes.push_back(q.m * q.n * h->s);
+        } else if (seen) {
+            f |= F_INS;  // fetch_list_to_host: nothing while the tree is empty (1191-1192)
+        }
+        if (q.mode & XKRT_HIP_MODE_R) f |= F_R;
+        if (q.mode & XKRT_HIP_MODE_W) f |= F_W;
+        fl[i] = f;
+        xoff[i] = (uint32_t)xtot;
+        if (f & F_INS) {
+            xtot += x1[i] - x0[i];
+            ttot += (uint64_t)((x1[i] - 1) / TW - x0[i] / TW + 1) * ((y1[i] - 1) / TH - y0[i] / TH + 1);
+        }
+    }
+    if (xtot >= NONE || ttot >= NONE) return -ERANGE;
+    const uint32_t ng = (uint32_t)gx0.size();
+    hipStream_t s = h->stream;
+    int rc = 0;
+    // capacities
+    uint64_t hneed = (uint64_t)h->nhist + ng;
+    uint32_t **bufs[] = {&h->bx0, &h->bx1, &h->by0, &h->by1, &h->bflags, &h->bhist, &h->bxoff, &h->pcount,
+                         &h->rowbase, &h->foff};
+    for (int k = 0; k < 10 && !rc; ++k) rc = grow(bufs[k], &h->bcap[k], n, false, s);
+    if (!rc) rc = grow(&h->hx0, &h->hcap, hneed, true, s);
+    if (!rc) rc = grow(&h->hx1, &h->hcap1, hneed, true, s);
+    if (!rc) rc = grow(&h->hy0, &h->hcap2, hneed, true, s);
+    if (!rc) rc = grow(&h->hy1, &h->hcap3, hneed, true, s);
+    if (!rc) rc = grow(&h->hbeta, &h->hcap4, hneed, true, s);
+    if (!rc) rc = grow(&h->hchunk, &h->hcap5, hneed, true, s);
+    if (!rc) rc = grow(&h->hbytes, &h->hcap6, hneed, true, s);
+    if (!rc) rc = grow(&h->chunk_size, &h->chunk_cap, hneed, true, s);
+    if (!rc) rc = grow(&h->xcnt, &h->xcap[0], std::max<uint64_t>(xtot, 1), false, s);
+    if (!rc) rc = grow(&h->xrow, &h->xcap[1], std::max<uint64_t>(xtot, 1), false, s);
+    if (!rc) rc = grow(&h->tinc, &h->tinc_cap, std::max<uint64_t>(ttot, 1), false, s);
+    if (!rc) rc = grow(&h->tbsum, &h->tbsum_cap, 1, false, s);
+    if (rc) return rc;
+    // uploads
+    auto up = [&](uint32_t *dst, const void *src, uint64_t bytes) {
+        if (!rc && bytes) rc = cerr(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    };
+    up(h->bx0, x0.data(), 4 * n); up(h->bx1, x1.data(), 4 * n); up(h->by0, y0.data(), 4 * n);
+    up(h->by1, y1.data(), 4 * n); up(h->bflags, fl.data(), 4 * n); up(h->bhist, hist.data(), 4 * n);
+    up(h->bxoff, xoff.data(), 4 * n);
+    up(h->hx0 + h->nhist, gx0.data(), 4ull * ng); up(h->hx1 + h->nhist, gx1.data(), 4ull * ng);
+    up(h->hy0 + h->nhist, gy0.data(), 4ull * ng); up(h->hy1 + h->nhist, gy1.data(), 4ull * ng);
+    if (!rc && ng) rc = cerr(hipMemcpyAsync(h->hbytes + h->nhist, gbytes.data(), 8ull * ng, hipMemcpyHostToDevice, s));
+    if (rc) return rc;
+
+    CCHK(hipEventRecord(h->ev0, s));
+    CArgs a = make_cargs(h, (uint32_t)n);
+    const uint32_t gw = waves_grid(n);
+    if (ng) {
+        hipLaunchKernelGGL(c_contain, dim3(waves_grid(ng)), dim3(256), 0, s, a, h->nhist, ng);
+        hipLaunchKernelGGL(c_chunks, dim3(1), dim3(1024), 0, s, a, h->nhist, ng);
+    }
+    hipLaunchKernelGGL(c_breaks, dim3(gw), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(c_count, dim3(gw), dim3(256), 0, s, a);
+    rc = scan(h, h->pcount, (uint32_t)n, h->rowbase, &h->ctr->rows);
+    if (rc) return rc;
+    CCHK(hipMemcpyAsync(h->ctr_host, h->ctr, sizeof(CCounters), hipMemcpyDeviceToHost, s));
+    CCHK(hipStreamSynchronize(s));
+    const uint32_t nrows = h->ctr_host->rows;
+    uint32_t **rb[] = {&h->rx0, &h->rx1, &h->ry0, &h->ry1, &h->rpay, &h->rsrc, &h->racc, &h->rflag, &h->rfpos};
+    for (int k = 0; k < 9 && !rc; ++k) rc = grow(rb[k], &h->rcap[k], std::max<uint32_t>(nrows, 1), false, s);
+    if (rc) return rc;
+    a = make_cargs(h, (uint32_t)n);
+    if (nrows) CCHK(hipMemsetAsync(h->rflag, 0, 4ull * nrows, s));
+    hipLaunchKernelGGL(c_rows, dim3(gw), dim3(256), 0, s, a);
+    // tiles
+    const uint32_t ntiles = h->ntx * h->nty;
+    hipLaunchKernelGGL(c_tcount, dim3(gw), dim3(256), 0, s, a);
+    rc = scan(h, h->tcnt, ntiles, h->tstart, h->tbsum);
+    if (rc) return rc;
+    CCHK(hipMemsetAsync(&h->ctr->nact, 0, 4, s));
+    hipLaunchKernelGGL(c_tfill, dim3(gw), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(c_tact, dim3((ntiles + 255) / 256), dim3(256), 0, s, a, ntiles);
+    hipLaunchKernelGGL(c_state, dim3(std::min<uint32_t>(ntiles, 2048)), dim3(TTHREADS), 0, s, a);
+    // fetch rows
+    rc = scan(h, h->rflag, std::max<uint32_t>(nrows, 1), h->rfpos, &h->ctr->fetches);
+    if (rc) return rc;
+    CCHK(hipMemcpyAsync(h->ctr_host, h->ctr, sizeof(CCounters), hipMemcpyDeviceToHost, s));
+    CCHK(hipStreamSynchronize(s));
+    const uint32_t nf = nrows ? h->ctr_host->fetches : 0;
+    uint32_t **fb[] = {&h->fx0, &h->fx1, &h->fy0, &h->fy1, &h->fsd, &h->fsc, &h->fdd, &h->fdc};
+    for (int k = 0; k < 8 && !rc; ++k) rc = grow(fb[k], &h->fcap[k], std::max<uint32_t>(nf, 1), false, s);
+    if (rc) return rc;
+    a = make_cargs(h, (uint32_t)n);
+    if (nrows) hipLaunchKernelGGL(c_fetch, dim3(std::min<uint32_t>((nrows + 255) / 256, 8192)), dim3(256), 0, s, a, nrows);
+    hipLaunchKernelGGL(c_foff, dim3(((uint32_t)n + 255) / 256), dim3(256), 0, s, a, nrows, nf);
+    CCHK(hipEventRecord(h->ev1, s));
+    CCHK(hipGetLastError());
+    CCHK(hipMemcpyAsync(h->ctr_host, h->ctr, sizeof(CCounters), hipMemcpyDeviceToHost, s));
+    CCHK(hipStreamSynchronize(s));
+    (void)hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1);
+    const CCounters c = *h->ctr_host;
+    if (c.viewerr != NONE && c.viewerr < c.fail) {  // a reused chunk missing before any abort: internal error
+        h->poisoned = true;
+        return -EIO;
+    }
+    if (c.err) {
+        h->poisoned = true;
+        return c.err & CERR_TILE ? -ENOSPC : -EIO;
+    }
+    // results: offsets per access, cut at the access the reference would abort on
+    h->h_rowbase.resize(n + 1);
+    h->h_foff.resize(n + 1);
+    CCHK(hipMemcpy(h->h_rowbase.data(), h->rowbase, 4 * n, hipMemcpyDeviceToHost));
+    CCHK(hipMemcpy(h->h_foff.data(), h->foff, 4 * n, hipMemcpyDeviceToHost));
+    h->h_rowbase[n] = nrows;
+    h->h_foff[n] = nf;
+    int64_t fail = -1;
+    uint64_t keep = n;
+    if (c.fail != NONE) {
+        fail = (int64_t)c.fail - (int64_t)h->naccess;
+        keep = (uint64_t)fail;
+        for (uint64_t i = keep; i <= n; ++i) {
+            h->h_rowbase[i] = h->h_rowbase[keep];
+            h->h_foff[i] = h->h_foff[keep];
+        }
+        h->poisoned = true;  // the reference aborts (memory-tree.hpp:1362-1365)
+    }
+    h->last_n = (uint32_t)n;
+    h->last_rows = h->h_rowbase[n];
+    h->last_fetch = h->h_foff[n];
+    h->last_fail = fail;
+    h->naccess += n;
+    h->nhist += ng;
+    h->gpu_seen = seen;
+    if (nparts) *nparts = h->last_rows;
+    if (nfetches) *nfetches = h->last_fetch;
+    if (fail_at) *fail_at = fail;
+    return 0;
+}
+
+int xkrt_hip_mcc_parts_copy(xkrt_hip_mcc_t h, uint64_t *part_off, int64_t *parts, uint64_t cap)
+{
+    if (!h) return -EINVAL;
+    const uint32_t nr = h->last_rows;
+    if (part_off)
+        for (uint32_t i = 0; i <= h->last_n; ++i) part_off[i] = h->h_rowbase.empty() ? 0 : h->h_rowbase[i];
+    if (!parts || nr == 0) return 0;
+    if (cap < nr) return -ENOSPC;
+    std::vector<uint32_t> v[5];
+    uint32_t *src[5] = {h->rx0, h->rx1, h->ry0, h->ry1, h->rpay};
+    for (int k = 0; k < 5; ++k) {
+        v[k].resize(nr);
+        CCHK(hipMemcpy(v[k].data(), src[k], 4ull * nr, hipMemcpyDeviceToHost));
+    }
+    for (uint32_t r = 0; r < nr; ++r) {
+        int64_t *o = parts + 6ull * r;
+        o[0] = v[0][r]; o[1] = v[1][r];
+        o[2] = (int64_t)(v[2][r] + h->cfg.y_lo); o[3] = (int64_t)(v[3][r] + h->cfg.y_lo);
+        o[4] = v[4][r] & 3u;
+        o[5] = 0;  // fetching bits: every fetch completes before the next access
+    }
+    return 0;
+}
+
+int xkrt_hip_mcc_fetches_copy(xkrt_hip_mcc_t h, uint64_t *fetch_off, int64_t *fetches, uint64_t cap)
+{
+    if (!h) return -EINVAL;
+    const uint32_t nf = h->last_fetch;
+    if (fetch_off)
+        for (uint32_t i = 0; i <= h->last_n; ++i) fetch_off[i] = h->h_foff.empty() ? 0 : h->h_foff[i];
+    if (!fetches || nf == 0) return 0;
+    if (cap < nf) return -ENOSPC;
+    std::vector<uint32_t> v[8];
+    uint32_t *src[8] = {h->fx0, h->fx1, h->fy0, h->fy1, h->fsd, h->fsc, h->fdd, h->fdc};
+    for (int k = 0; k < 8; ++k) {
+        v[k].resize(nf);
+        CCHK(hipMemcpy(v[k].data(), src[k], 4ull * nf, hipMemcpyDeviceToHost));
+    }
+    for (uint32_t r = 0; r < nf; ++r) {
+        int64_t *o = fetches + 8ull * r;
+        o[0] = v[0][r]; o[1] = v[1][r];
+        o[2] = (int64_t)(v[2][r] + h->cfg.y_lo); o[3] = (int64_t)(v[3][r] + h->cfg.y_lo);
+        o[4] = v[4][r];
+        o[5] = v[5][r] == NONE ? -1 : (int64_t)v[5][r];
+        o[6] = v[6][r];
+        o[7] = v[7][r] == NONE ? -1 : (int64_t)v[7][r];
+    }
+    return 0;
+}
+
+int xkrt_hip_mcc_who_owns(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64_t n, uint32_t *owners)
+{
+    if (!h || (!acc && n) || (!owners && n)) return -EINVAL;
+    if (h->poisoned) return -ECANCELED;  // the reference has aborted
+    if (n == 0) return 0;
+    std::vector<uint32_t> q(4 * n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t xb = acc[i].addr % h->LDs, col = acc[i].addr / h->LDs;
+        if (acc[i].m == 0 || acc[i].n == 0 || xb % h->s) return -EINVAL;
+        if (xb + acc[i].m * h->s > h->LDs) return -ENOTSUP;
+        if (col < h->cfg.y_lo || col + acc[i].n > h->cfg.y_hi) return -ERANGE;
+        q[i] = (uint32_t)(xb / h->s);
+        q[n + i] = (uint32_t)(xb / h->s + acc[i].m);
+        q[2 * n + i] = (uint32_t)(col - h->cfg.y_lo);
+        q[3 * n + i] = (uint32_t)(col - h->cfg.y_lo + acc[i].n);
+    }
+    uint32_t *dq = nullptr, *dout = nullptr;
+    CCHK(hipMalloc(&dq, 4ull * 4 * n));
+    int rc = cerr(hipMalloc(&dout, 4ull * n));
+    if (!rc) rc = cerr(hipMemcpyAsync(dq, q.data(), 16ull * n, hipMemcpyHostToDevice, h->stream));
+    if (!rc) {
+        CArgs a = make_cargs(h, 0);
+        hipLaunchKernelGGL(c_owners, dim3((uint32_t)std::min<uint64_t>(n, 4096)), dim3(256), 0, h->stream, a, dq,
+                           dq + n, dq + 2 * n, dq + 3 * n, (uint32_t)n, dout);
+        rc = cerr(hipGetLastError());
+    }
+    if (!rc) rc = cerr(hipMemcpyAsync(owners, dout, 4ull * n, hipMemcpyDeviceToHost, h->stream));
+    if (!rc) rc = cerr(hipStreamSynchronize(h->stream));
+    (void)hipFree(dq);
+    (void)hipFree(dout);
+    return rc;
+}
+
+int xkrt_hip_mcc_stats(xkrt_hip_mcc_t h, xkrt_hip_mcc_stats_t *out)
+{
+    if (!h || !out) return -EINVAL;
+    memset(out, 0, sizeof(*out));
+    out->accesses = h->naccess;
+    out->gpu_accesses = h->nhist;
+    CCHK(hipMemcpyAsync(h->ctr_host, h->ctr, sizeof(CCounters), hipMemcpyDeviceToHost, h->stream));
+    CCHK(hipStreamSynchronize(h->stream));
+    out->chunks = h->ctr_host->nchunks;
+    out->last_parts = h->last_rows;
+    out->last_fetches = h->last_fetch;
+    out->last_ms = h->last_ms;
+    out->device_bytes = h->device_bytes;
+    out->poisoned = h->poisoned ? 1 : 0;
+    return 0;
+}
+
+} // extern "C"
