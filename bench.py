@@ -62,13 +62,13 @@ class Prepared:
 
     def _upload(self, a, b, torch):
         recs, ylo, yhi = self.dom.build_records(self.acc[a:b], a)
-        dev = [torch.from_numpy(r.view(np.int32)).cuda() for r in recs]
-        return dev, len(recs[0]), b - a, ylo, yhi
+        dev = torch.from_numpy(recs.view(np.int32).reshape(-1)).cuda()
+        return dev, len(recs), b - a, ylo, yhi
 
     def run(self, which):
         self.dom.reset()
         for dev, nrec, nacc, ylo, yhi in (self.plain if which == "plain" else self.merged):
-            self.dom.submit_device([t.data_ptr() for t in dev], nrec, nacc, ylo, yhi)
+            self.dom.submit_device(dev.data_ptr(), nrec, nacc, ylo, yhi)
 
 
 def cpu_baseline(st, target_s=10.0):
@@ -88,6 +88,67 @@ def cpu_baseline(st, target_s=10.0):
                       f"reference khp-tree.hpp engine, 1 thread"}
 
 
+def coherency_leg(n_stream=6000, reps=3):
+    """Coherency fetch planning on the two-view stream, cut where the reference
+    aborts (the first ninth allocation view): GPU planner per tree, one call
+    each, against the oracle restatement on one host core."""
+    from xkaapi_amd import coherency as K
+    from xkaapi_amd import streams as S
+    st = S.coherency(n_stream)
+    views = [(8192, 8), (16384, 4)]
+    subs = []
+    for ld, s in views:
+        sel = np.nonzero((st.ld == ld) & (st.s == s))[0]
+        subs.append((ld, s, sel, st.take(sel)))
+    # global cut: the first access at which either tree aborts
+    cut = len(st)
+    for ld, s, sel, sub in subs:
+        with K.MemoryTree(ld, s, (0, 8192)) as t:
+            _, _, fail = t.plan_counts(K.to_mcc_array(sub.dev, sub.mode, sub.addr, sub.m, sub.n))
+        if fail >= 0:
+            cut = min(cut, int(sel[fail]))
+    trees = []
+    for ld, s, sel, sub in subs:
+        k = int(np.searchsorted(sel, cut))
+        trees.append((ld, s, K.to_mcc_array(sub.dev[:k], sub.mode[:k], sub.addr[:k], sub.m[:k], sub.n[:k]),
+                      sub.take(slice(0, k))))
+    gpu_ms, wall_ms, parts, fetches = [], [], 0, 0
+    for _ in range(reps):
+        g = w = 0.0
+        parts = fetches = 0
+        for ld, s, acc, _ in trees:
+            with K.MemoryTree(ld, s, (0, 8192)) as t:
+                t0 = time.perf_counter()
+                p, f, fail = t.plan_counts(acc)
+                w += time.perf_counter() - t0
+                g += t.stats()["last_ms"]
+                parts += p
+                fetches += f
+                assert fail < 0
+        gpu_ms.append(g)
+        wall_ms.append(w * 1e3)
+    out = {"accesses": int(cut), "trees": len(trees), "parts": int(parts), "fetches": int(fetches),
+           "gpu_ms": round(min(gpu_ms), 3), "plan_call_ms": round(min(wall_ms), 3),
+           "accesses_per_s_gpu": round(cut / (min(gpu_ms) * 1e-3), 1),
+           "workload": f"two-view coherency stream (8192^2 buffer, LD 8192/8 B and 16384/4 B), first {cut} "
+                       "accesses: up to the reference's ninth-view abort"}
+    return out, [sub for _, _, _, sub in trees]
+
+
+def coherency_cpu_baseline(subs):
+    """The oracle's restatement of the reference memory tree on the same trees, one host core."""
+    from oracle import khp
+    t0 = time.perf_counter()
+    n = 0
+    for sub in subs:
+        r, nr = sub.rects()
+        khp.coherency(sub.dev, sub.mode, r, nr, sub.m * sub.n * sub.s)
+        n += len(sub)
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt, 1), "unit": "accesses/s", "cores": 1, "kind": "port",
+            "sample": f"the same {n} accesses, oracle/khp_oracle.c (restated memory-tree.hpp), 1 thread"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +160,7 @@ def main():
     ap.add_argument("--workload", choices=("all", "cholesky", "random"), default="all",
                     help="restrict to one stream (profiling runs)")
     ap.add_argument("--forms", choices=("both", "plain", "batched"), default="both")
+    ap.add_argument("--no-coherency", action="store_true")
     ap.add_argument("--calibrate-mib", type=int, default=0,
                     help="also run one streaming copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -205,6 +267,12 @@ def main():
                 "workload": "cholesky 8192^2 nb128, 4096-task launches (plain form)"}
 
     cpu = None if args.no_cpu else cpu_baseline(chol)
+    coh = None
+    if not args.no_coherency:
+        coh, coh_subs = coherency_leg()
+        if not args.no_cpu:
+            coh["cpu_baseline"] = coherency_cpu_baseline(coh_subs)
+        log(f"coherency: {coh}")
     out = {
         "metric": "resolved task accesses/sec + p99 submit-to-ready µs, batched vs plain GPU",
         "value": round(res["plain"]["rate"], 1),
@@ -229,6 +297,7 @@ def main():
         "edges": res["plain"]["edges"],
         "roofline": roof,
         "cpu_baseline": cpu,
+        "coherency": coh,
     }
     print(json.dumps(out), flush=True)
 

@@ -53,14 +53,16 @@ typedef struct xkrt_hip_access {
     uint8_t  pad[3];
 } xkrt_hip_access_t;
 
-/* Device-resident rect records (structure of arrays, 24 B per rect).
+/* Device-frame rect record (32 B; arrays of them are what the kernels read).
  * Coordinates are in the domain frame: x = byte offset in the column,
  * y = column - the domain's y base (see xkrt_hip_dep_frame).  tm packs the
- * launch-local task index (bits 8..31) and the mode (bits 0..7).  acc is the
- * domain access id the edges refer to. */
-typedef struct xkrt_hip_rect_soa {
-    const uint32_t *x0, *x1, *y0, *y1, *acc, *tm;
-} xkrt_hip_rect_soa_t;
+ * launch-local task number (bits 8..31) and the mode (bits 0..7).  acc is the
+ * domain access id the edges refer to; dom the domain's index in its pool
+ * (xkrt_hip_dep_index). */
+typedef struct xkrt_hip_rect {
+    uint32_t x0, x1, y0, y1;
+    uint32_t acc, tm, dom, pad;
+} xkrt_hip_rect_t;
 
 typedef struct xkrt_hip_dep_config {
     uint32_t bin_x_bytes;  /* bin extent in dim 0 (bytes); 0 = auto */
@@ -111,20 +113,20 @@ int xkrt_hip_dep_domain_destroy(xkrt_hip_dep_domain_t d);
  * beyond the tile (dependency-tree.hpp:229-256). */
 int xkrt_hip_dep_submit(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n, uint64_t *first_id);
 
-/* Same, from device-resident rect records (no host conversion, no copy).
- * ntask = launch-local tasks; [ylo, yhi) bounds every record's y (domain
- * frame); naccess = accesses the records belong to (for accounting). */
-int xkrt_hip_dep_submit_device(xkrt_hip_dep_domain_t d, const xkrt_hip_rect_soa_t *rec, uint64_t nrect,
+/* Same, from device-resident rect records (no host conversion, no copy):
+ * nrect records of this domain, as xkrt_hip_dep_build_records writes them;
+ * [ylo, yhi) bounds every record's y (domain frame); naccess = accesses the
+ * records belong to (for accounting).  One launch. */
+int xkrt_hip_dep_submit_device(xkrt_hip_dep_domain_t d, const xkrt_hip_rect_t *rec, uint64_t nrect,
                                uint64_t naccess, uint32_t ylo, uint32_t yhi);
 
 /* Converts accesses to device-frame rect records on the host (the work
- * xkrt_hip_dep_submit does before its copy).  out arrays hold cap records;
- * returns the record count in *nrect; task numbers become launch-local;
+ * xkrt_hip_dep_submit does before its copy).  out holds cap records; the
+ * record count goes to *nrect; task numbers become launch-local from 0;
  * access ids start at first_id.  *ylo / *yhi receive the y bounds. */
 int xkrt_hip_dep_build_records(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n,
-                               uint32_t first_id, uint64_t cap, uint32_t *x0, uint32_t *x1, uint32_t *y0,
-                               uint32_t *y1, uint32_t *accid, uint32_t *tm, uint64_t *nrect, uint32_t *ylo,
-                               uint32_t *yhi);
+                               uint32_t first_id, uint64_t cap, xkrt_hip_rect_t *out, uint64_t *nrect,
+                               uint32_t *ylo, uint32_t *yhi);
 
 /* y base of the domain frame (fixed at the first submit or from the hint). */
 int xkrt_hip_dep_frame(xkrt_hip_dep_domain_t d, uint64_t *ybase);
@@ -169,6 +171,71 @@ int xkrt_hip_copy(void *dst, const void *src, uint64_t bytes, void *stream);
 /* Per timed launch, milliseconds from its first kernel's start to its last
  * kernel's end (up to cap values; *n = launches recorded). */
 int xkrt_hip_dep_launch_times(xkrt_hip_dep_domain_t d, float *ms, uint64_t cap, uint64_t *n, int clear);
+
+/* ------------------------------------------------------------------------
+ * Pools: many dependency domains (clients) resident on one device and
+ * resolved together.  A pool holds the bin state of up to 256 domains and
+ * admits a domain only while its device-memory budget allows (the serving
+ * loop's admission control).  Batches of any domains are staged into the
+ * pool's open launch and resolved by one launch -- the same kernels as one
+ * domain, over the union of the batches' bins -- optionally replayed from a
+ * captured hipGraph.  Each domain keeps its own access ids; edges carry
+ * their domain's index.  A domain from xkrt_hip_dep_domain_create is a
+ * private pool with one domain.
+ * No counterpart in the reference, which resolves each task inline in the
+ * spawning thread (src/task/task.cc:285-299).
+ * ------------------------------------------------------------------------ */
+
+typedef struct xkrt_hip_pool *xkrt_hip_pool_t;
+
+typedef struct xkrt_hip_pool_config {
+    uint32_t kernel;        /* LDS capacity class of every bin: 0, 1, 2 (see xkrt_hip_dep_config_t) */
+    uint32_t graph;         /* 1: launches of staged batches replay a captured hipGraph */
+    uint64_t max_rects;     /* rect records per launch (sizes the staging and scratch buffers) */
+    uint64_t max_edges;     /* edge buffer capacity before a drain is required */
+    uint64_t budget_bytes;  /* device memory budget: bins of every domain + scratch; 0 = unlimited */
+} xkrt_hip_pool_config_t;
+
+typedef struct xkrt_hip_pool_stats {
+    uint64_t domains;        /* domains admitted */
+    uint64_t bins_capacity;  /* bins allocated on the device */
+    uint64_t bins_used;      /* bins held by domains */
+    uint64_t bytes_per_bin;
+    uint64_t device_bytes;   /* bins + scratch */
+    uint64_t launches;
+    uint64_t graph;          /* 1 when a captured launch is ready */
+    uint64_t poisoned;
+} xkrt_hip_pool_stats_t;
+
+int xkrt_hip_pool_create(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool_t *p);
+int xkrt_hip_pool_destroy(xkrt_hip_pool_t p);
+
+/* Admits a domain for (ld, sizeof_type) over columns [y_lo, y_hi) with the
+ * given bin extents (powers of two; 0 = 1024 B x 128 columns).  -ENOMEM: the
+ * budget cannot hold its bins now (retry after other domains leave).  Leave
+ * with xkrt_hip_dep_domain_destroy. */
+int xkrt_hip_pool_domain_create(xkrt_hip_pool_t p, uint64_t ld, uint64_t sizeof_type, uint32_t bin_x_bytes,
+                                uint32_t bin_y_cols, uint64_t y_lo, uint64_t y_hi, xkrt_hip_dep_domain_t *d);
+
+/* Appends a batch of d's accesses (whole tasks, spawn order) to the open
+ * launch; ids as xkrt_hip_dep_submit.  -ENOSPC: the open launch is full. */
+int xkrt_hip_pool_stage(xkrt_hip_pool_t p, xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n,
+                        uint64_t *first_id);
+/* Resolves every staged batch in one launch (asynchronous). */
+int xkrt_hip_pool_launch(xkrt_hip_pool_t p);
+/* Staged but not launched: records, accesses, batches. */
+int xkrt_hip_pool_open(xkrt_hip_pool_t p, uint64_t *nrect, uint64_t *naccess, uint64_t *nbatches);
+int xkrt_hip_pool_sync(xkrt_hip_pool_t p, uint64_t *nedges);
+/* Copies and consumes every unconsumed edge: edges = (pred id, succ id)
+ * pairs, doms = the domain index of each. */
+int xkrt_hip_pool_edges_copy(xkrt_hip_pool_t p, uint32_t *edges, uint32_t *doms, uint64_t cap, uint64_t *n);
+int xkrt_hip_pool_stats(xkrt_hip_pool_t p, xkrt_hip_pool_stats_t *out);
+int xkrt_hip_pool_timing(xkrt_hip_pool_t p, int enable);
+/* Per timed launch, milliseconds from its first kernel's start to its last kernel's end. */
+int xkrt_hip_pool_launch_times(xkrt_hip_pool_t p, float *ms, uint64_t cap, uint64_t *n, int clear);
+void *xkrt_hip_pool_stream(xkrt_hip_pool_t p);
+/* The domain's index in its pool (the dom field of records and edges). */
+int xkrt_hip_dep_index(xkrt_hip_dep_domain_t d, uint32_t *idx);
 
 /* ------------------------------------------------------------------------
  * Coherency fetch planner: one memory tree per (ld, sizeof_type), a host

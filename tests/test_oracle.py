@@ -222,3 +222,4 @@ def test_who_owns_matches_pointwise(seed):
         best = max(by)
         want = 0 if best == 0 else sum(1 << d for d in range(2) if by[d] == best)
         assert int(plan.owners[k]) == want, k
+

@@ -61,6 +61,7 @@ enum : uint32_t {
 };
 
 struct DevCounters {
+    unsigned int nrec;       // rect records of this launch (set before the launch)
     unsigned int total;      // incidences allocated this launch
     unsigned int ntier[3];   // active bins per LDS tier this launch (heavy ones, from the list front)
     unsigned int nlight[3];  // light active bins per tier (from the list back)
@@ -68,20 +69,26 @@ struct DevCounters {
     unsigned int err;        // sticky ERR_* bits
     unsigned int maxfrag;    // stats: max fragments in a bin
     unsigned int maxrs;      // stats: max reader-pool use in a bin
-    unsigned int pad;
     unsigned long long edge_total;   // unconsumed edges
     unsigned long long edge_base;    // edge_total when the launch started
     unsigned long long cand_total;   // stats
     unsigned long long frag_visits;  // stats
 };
 
-struct KArgs {
-    const uint32_t *rx0, *rx1, *ry0, *ry1, *racc, *rtm;
-    uint32_t nrec;
-    uint32_t bxs, bys;        // log2 of the bin extents
+// one dependency domain's bin grid inside the pool's bin arrays
+struct DomDesc {
+    uint32_t bin0;            // first pool bin of the grid
     uint32_t nbx, nby;        // bin grid
+    uint32_t bxs, bys;        // log2 of the bin extents
     uint32_t ybin0;           // frame y of bin row 0
     uint32_t xmax;            // LDs: dim-0 extent of the frame
+    uint32_t pad;
+};
+
+struct KArgs {
+    const uint4 *rec;         // 2 x uint4 per rect record: {x0, x1, y0, y1}, {acc, tm, domain, 0}
+    const DomDesc *dom;       // domain table
+    const uint32_t *bin_dom;  // domain of each pool bin
     uint32_t *bin_nf, *bin_nrs;
     uint32_t *frag;           // nbins * 5 * fcap (SoA x0, x1, y0, y1, lw)
     uint32_t *rs;             // nbins * 5 * rcap reader log (SoA x0, x1, y0, y1, acc)
@@ -99,7 +106,8 @@ struct KArgs {
     uint32_t *hvals;
     uint32_t hmask;
     uint32_t *eslot;
-    uint32_t *edges;
+    uint32_t *edges;          // (pred id, succ id) per edge
+    uint32_t *edom;           // domain of each edge
     unsigned long long edge_cap;
     DevCounters *ctr;
 };
@@ -131,14 +139,18 @@ __device__ __forceinline__ uint64_t mix64(uint64_t k)
 
 // ---------------------------------------------------------------- binning
 
-__device__ __forceinline__ void rec_bins(const KArgs &a, uint32_t r, uint32_t &bx0, uint32_t &bx1, uint32_t &by0,
-                                         uint32_t &by1)
+// the bins of record r: [bx0, bx1] x [by0, by1] of its domain's grid
+__device__ __forceinline__ DomDesc rec_bins(const KArgs &a, uint32_t r, uint4 &q0, uint4 &q1, uint32_t &bx0,
+                                            uint32_t &bx1, uint32_t &by0, uint32_t &by1)
 {
-    const uint32_t x0 = a.rx0[r], x1 = a.rx1[r], y0 = a.ry0[r], y1 = a.ry1[r];
-    bx0 = x0 >> a.bxs;
-    bx1 = (x1 - 1) >> a.bxs;
-    by0 = (y0 - a.ybin0) >> a.bys;
-    by1 = (y1 - 1 - a.ybin0) >> a.bys;
+    q0 = a.rec[2ull * r];
+    q1 = a.rec[2ull * r + 1];
+    const DomDesc g = a.dom[q1.z];
+    bx0 = q0.x >> g.bxs;
+    bx1 = (q0.y - 1) >> g.bxs;
+    by0 = (q0.z - g.ybin0) >> g.bys;
+    by1 = (q0.w - 1 - g.ybin0) >> g.bys;
+    return g;
 }
 
 __global__ void k_count(KArgs a)
@@ -153,12 +165,14 @@ __global__ void k_count(KArgs a)
         a.ctr->ncand = 0;
         a.ctr->edge_base = a.ctr->edge_total;
     }
-    if (a.ctr->err) return;  // a poisoned domain touches no state
-    for (uint32_t r = tid; r < a.nrec; r += gridDim.x * blockDim.x) {
+    if (a.ctr->err) return;  // a poisoned pool touches no state
+    const uint32_t nrec = a.ctr->nrec;
+    for (uint32_t r = tid; r < nrec; r += gridDim.x * blockDim.x) {
+        uint4 q0, q1;
         uint32_t bx0, bx1, by0, by1;
-        rec_bins(a, r, bx0, bx1, by0, by1);
+        const DomDesc g = rec_bins(a, r, q0, q1, bx0, bx1, by0, by1);
         for (uint32_t by = by0; by <= by1; ++by)
-            for (uint32_t bx = bx0; bx <= bx1; ++bx) atomicAdd(&a.bin_cnt[by * a.nbx + bx], 1u);
+            for (uint32_t bx = bx0; bx <= bx1; ++bx) atomicAdd(&a.bin_cnt[g.bin0 + by * g.nbx + bx], 1u);
     }
 }
 
@@ -197,7 +211,7 @@ __global__ void k_alloc(KArgs a, uint32_t nbins)
             const uint32_t r = tb + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
             if (mine)
                 a.active[(size_t)t * a.nbins + (heavy ? r : a.nbins - 1 - r)] =
-                    make_uint4(slot, base + incl - c, c, nfl | (ngl << 16));
+                    make_uint4(slot, base + incl - c, c, nfl | (ngl << 11) | (a.bin_dom[slot] << 24));
         }
     }
 }
@@ -206,15 +220,18 @@ __global__ void k_fill(KArgs a)
 {
     if (a.ctr->err) return;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t r = tid; r < a.nrec; r += gridDim.x * blockDim.x) {
+    const uint32_t nrec = a.ctr->nrec;
+    for (uint32_t r = tid; r < nrec; r += gridDim.x * blockDim.x) {
+        uint4 q0, q1;
         uint32_t bx0, bx1, by0, by1;
-        rec_bins(a, r, bx0, bx1, by0, by1);
+        const DomDesc g = rec_bins(a, r, q0, q1, bx0, bx1, by0, by1);
+        const uint4 inc1 = make_uint4(q1.x, q1.y, r, 0u);
         for (uint32_t by = by0; by <= by1; ++by)
             for (uint32_t bx = bx0; bx <= bx1; ++bx) {
-                const uint32_t pos = atomicAdd(&a.bin_cur[by * a.nbx + bx], 1u);
+                const uint32_t pos = atomicAdd(&a.bin_cur[g.bin0 + by * g.nbx + bx], 1u);
                 if (pos < a.binrec_cap) {
-                    a.binrec[2ull * pos] = make_uint4(a.rx0[r], a.rx1[r], a.ry0[r], a.ry1[r]);
-                    a.binrec[2ull * pos + 1] = make_uint4(a.racc[r], a.rtm[r], r, 0u);
+                    a.binrec[2ull * pos] = q0;
+                    a.binrec[2ull * pos + 1] = inc1;
                 }
             }
     }
@@ -254,6 +271,7 @@ struct BinLds {
     uint32_t slow[64];          // fragments needing the exact coverage test
     uint32_t ccount;
     uint32_t dcount;            // entries of Du
+    uint32_t tdom;              // domain << 24: or-ed into task numbers of candidates
 };
 
 struct BinState {
@@ -300,6 +318,7 @@ template <int FCAP, int GCAP, int LCAP>
 __device__ __forceinline__ void emit(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, uint32_t p, uint32_t acc,
                                      uint32_t task)
 {
+    task |= s.tdom;  // (pred, task) keys of different domains never meet
     const unsigned long long key = ((unsigned long long)p << 32) | task;
     constexpr uint32_t DSZ = BinLds<FCAP, GCAP, LCAP>::DSZ;
     uint32_t h = (uint32_t)(mix64(key) & (DSZ - 1));
@@ -639,14 +658,18 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     __syncthreads();
     for (uint32_t b = blockIdx.x; b < nlist; b += gridDim.x) {
         const uint4 e = b < nheavy ? list[b] : list[a.nbins - 1 - (b - nheavy)];
-        const uint32_t slot = e.x, start = e.y, cnt = e.z, nf0 = e.w & 0xffffu, ng0 = e.w >> 16;
+        const uint32_t slot = e.x, start = e.y, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
         if (cnt > (uint32_t)LCAP || nf0 > (uint32_t)FCAP || ng0 > (uint32_t)GCAP) {
             escalate<FCAP, GCAP, LCAP>(a, tier, e, ERR_LIST);
             continue;
         }
-        const uint32_t bxi = slot % a.nbx, byi = slot / a.nbx;
-        const uint32_t BX0 = bxi << a.bxs, BX1 = min((bxi + 1) << a.bxs, a.xmax);
-        const uint32_t BY0 = a.ybin0 + (byi << a.bys), BY1 = BY0 + (1u << a.bys);
+        const uint32_t dm = e.w >> 24;
+        const DomDesc g = a.dom[dm];
+        const uint32_t local = slot - g.bin0;
+        const uint32_t bxi = local % g.nbx, byi = local / g.nbx;
+        const uint32_t BX0 = bxi << g.bxs, BX1 = min((bxi + 1) << g.bxs, g.xmax);
+        const uint32_t BY0 = g.ybin0 + (byi << g.bys), BY1 = BY0 + (1u << g.bys);
+        if (lane == 0) s.tdom = dm << 24;
 
         // record ids, fragments and reader log of the bin, loads issued together
         // (SoA in HBM: 5 x fcap fragment words, 5 x gcap log words)
@@ -910,20 +933,24 @@ __global__ void k_final(KArgs a)
         const unsigned long long key = a.hkeys[h];
         a.edges[2 * (ebase + e)] = (uint32_t)(key >> 32);
         a.edges[2 * (ebase + e) + 1] = a.hvals[h];
+        a.edom[ebase + e] = (uint32_t)(key >> 24) & 0xffu;
         a.hkeys[h] = EMPTY64;
         a.hvals[h] = NONE;
     }
 }
 
-__global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint32_t nbins, DevCounters *c)
+// ---------------------------------------------------------------- maintenance
+
+// forget the state of pool bins [b0, b0 + n); with c, also the pool's counters
+__global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint32_t b0, uint32_t n, DevCounters *c)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t i = tid; i < nbins; i += gridDim.x * blockDim.x) {
-        nf[i] = 0;
-        nrs[i] = 0;
-        cnt[i] = 0;
+    for (uint32_t i = tid; i < n; i += gridDim.x * blockDim.x) {
+        nf[b0 + i] = 0;
+        nrs[b0 + i] = 0;
+        cnt[b0 + i] = 0;
     }
-    if (tid == 0) {
+    if (tid == 0 && c) {
         c->err = 0;
         c->edge_total = 0;
         c->maxfrag = 0;
@@ -931,6 +958,16 @@ __global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint32_t nbi
         c->cand_total = 0;
         c->frag_visits = 0;
     }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_setn(DevCounters *c, uint32_t n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->nrec = n;
 }
 
 // plain streaming copy, the reference point for HBM byte counters
@@ -945,19 +982,22 @@ __global__ void k_consume(DevCounters *c)
     if (threadIdx.x == 0 && blockIdx.x == 0) c->edge_total = 0;
 }
 
-// grow the bin grid: copy every old bin into its new slot
-__global__ void k_regrid(const uint32_t *onf, const uint32_t *onrs, const uint32_t *ofr, const uint32_t *ors,
-                         uint32_t onbx, uint32_t onby, uint32_t *nnf, uint32_t *nnrs, uint32_t *nfr, uint32_t *nrs,
-                         uint32_t nnbx, uint32_t dy, uint32_t fcap, uint32_t rcap)
+// move a domain's grid to another range of the pool: old bin (bx, by) ->
+// new bin (bx, by + dy); the ranges are disjoint
+__global__ void k_regrid(uint32_t *nf, uint32_t *nrs, uint32_t *fr, uint32_t *rs, uint32_t fcap, uint32_t rcap,
+                         uint32_t ob0, uint32_t onbx, uint32_t onby, uint32_t nb0, uint32_t nnbx, uint32_t dy)
 {
     const uint32_t ob = blockIdx.x;
     if (ob >= onbx * onby) return;
     const uint32_t bx = ob % onbx, by = ob / onbx;
-    const uint32_t nbi = (by + dy) * nnbx + bx;
-    const uint32_t nf = onf[ob], nr = onrs[ob];
-    if (threadIdx.x == 0) { nnf[nbi] = nf; nnrs[nbi] = nr; }
-    for (uint32_t i = threadIdx.x; i < 5 * nf; i += blockDim.x) nfr[(size_t)nbi * fcap * 5 + i] = ofr[(size_t)ob * fcap * 5 + i];
-    for (uint32_t i = threadIdx.x; i < 5 * nr; i += blockDim.x) nrs[(size_t)nbi * rcap * 5 + i] = ors[(size_t)ob * rcap * 5 + i];
+    const size_t o = (size_t)ob0 + ob, n = (size_t)nb0 + (size_t)(by + dy) * nnbx + bx;
+    const uint32_t f = nf[o], r = nrs[o];
+    for (uint32_t i = threadIdx.x; i < 5 * f; i += blockDim.x) fr[n * fcap * 5 + i] = fr[o * fcap * 5 + i];
+    for (uint32_t i = threadIdx.x; i < 5 * r; i += blockDim.x) rs[n * rcap * 5 + i] = rs[o * rcap * 5 + i];
+    if (threadIdx.x == 0) {
+        nf[n] = f;
+        nrs[n] = r;
+    }
 }
 
 // ---------------------------------------------------------------- host side
@@ -973,6 +1013,8 @@ const KClass KCLASSES[3] = {{64, 128, 256}, {256, 512, 512}, {1024, 2048, 1024}}
 template <int F, int R, int L>
 size_t lds_bytes() { return sizeof(BinLds<F, R, L>); }
 static_assert(sizeof(BinLds<1024, 2048, 1024>) <= 160u * 1024u, "largest tier exceeds the 160 KiB LDS of a CU");
+
+constexpr uint32_t MAX_DOMS = 256;
 
 int hip_err(hipError_t e)
 {
@@ -994,23 +1036,25 @@ bool is_pow2(uint64_t v) { return v && !(v & (v - 1)); }
 
 } // namespace
 
-struct xkrt_hip_dep_domain {
-    uint64_t ld = 0, s = 0, LDs = 0;
-    xkrt_hip_dep_config_t cfg{};
+struct xkrt_hip_pool {
+    xkrt_hip_pool_config_t cfg{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int device = 0;
-    uint32_t bxs = 0, bys = 0, kclass = 0;
-    uint32_t fcap = 0, rcap = 0;
-    // frame
-    bool framed = false;
-    uint64_t ybase = 0;
-    uint32_t nbx = 0, nby = 0, ybin0 = 0;
-    // device state
-    uint32_t *bin_nf = nullptr, *bin_nrs = nullptr, *rs = nullptr;
-    uint32_t *frag = nullptr;
-    uint32_t *bin_cnt = nullptr, *bin_cur = nullptr;
+    uint32_t kclass = 0, fcap = 0, rcap = 0;
+    // bin arrays (cap_bins bins) and their free ranges [start, len), sorted
+    uint32_t cap_bins = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> free_ranges;
+    uint32_t *bin_nf = nullptr, *bin_nrs = nullptr, *frag = nullptr, *rs = nullptr;
+    uint32_t *bin_cnt = nullptr, *bin_cur = nullptr, *bin_dom = nullptr;
     uint4 *active = nullptr;
+    // domains
+    DomDesc h_dom[MAX_DOMS] = {};
+    xkrt_hip_dep_domain *doms[MAX_DOMS] = {};
+    uint32_t ndoms = 0;
+    DomDesc *d_dom = nullptr;
+    bool dom_dirty = true;
+    // launch scratch
     uint4 *binrec = nullptr;
     uint32_t binrec_cap = 0;
     uint32_t *cand = nullptr;
@@ -1019,37 +1063,61 @@ struct xkrt_hip_dep_domain {
     uint32_t *hvals = nullptr;
     uint32_t hmask = 0;
     uint32_t *eslot = nullptr;
-    uint32_t *edges = nullptr;
+    uint32_t *edges = nullptr, *edom = nullptr;
     uint64_t edge_cap = 0;
     DevCounters *ctr = nullptr;
     DevCounters *ctr_host = nullptr;  // pinned
-    // host staging for xkrt_hip_dep_submit
-    uint32_t *rec_dev[6] = {};
-    uint32_t *rec_host[6] = {};
+    // staging of the open launch: pinned host records + the record count
+    uint4 *rec_dev = nullptr, *rec_host = nullptr;
+    uint32_t *nrec_host = nullptr;    // pinned
     uint64_t rec_cap = 0;
     hipEvent_t staged = nullptr;
     bool staged_pending = false;
-    // host-side tracking
-    uint64_t next_id = 0;
-    uint64_t max_xlen = 0;
+    uint32_t open_nrec = 0, open_tasks = 0;
+    uint64_t open_acc = 0, open_batches = 0;
+    // captured launch
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_valid = false;
+    // accounting
     bool poisoned = false;
-    uint64_t n_acc = 0, n_rect = 0, n_launch = 0;
-    uint64_t bytes_state = 0, bytes_scratch = 0;
-    // optional per-kernel HIP-event timing of every launch
+    uint64_t n_launch = 0;
+    uint64_t bytes_scratch = 0;
+    // optional HIP-event timing of every launch
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
-    std::vector<hipEvent_t> ev_pending;  // 7 events per launch, in launch order
+    std::vector<hipEvent_t> ev_pending;  // 7 events per timed launch (2 for a graph launch), in launch order
+    std::vector<int> ev_kind;            // 7 or 2 per pending launch
     double kernel_ms[6] = {0, 0, 0, 0, 0, 0};
     uint64_t timed_launches = 0;
     std::vector<float> launch_ms;        // per timed launch: first kernel start -> last kernel end
 };
 
-static hipEvent_t take_event(xkrt_hip_dep_domain *d)
+struct xkrt_hip_dep_domain {
+    xkrt_hip_pool *pool = nullptr;
+    bool private_pool = false;
+    uint32_t idx = 0;
+    uint64_t ld = 0, s = 0, LDs = 0;
+    uint32_t bxs = 0, bys = 0;
+    uint32_t nbx = 0, nby = 0, ybin0 = 0, bin0 = 0;
+    bool fixed_frame = false;  // pool domains: the column range is given at creation
+    bool framed = false;
+    uint64_t ybase = 0;
+    uint64_t next_id = 0;
+    uint64_t max_xlen = 0;
+    uint64_t n_acc = 0, n_rect = 0;
+};
+
+namespace {
+
+uint64_t bin_bytes(const xkrt_hip_pool *p) { return 4ull * 7 + 20ull * p->fcap + 20ull * p->rcap + 48ull; }
+
+hipEvent_t take_event(xkrt_hip_pool *p)
 {
     hipEvent_t e = nullptr;
-    if (!d->ev_pool.empty()) {
-        e = d->ev_pool.back();
-        d->ev_pool.pop_back();
+    if (!p->ev_pool.empty()) {
+        e = p->ev_pool.back();
+        p->ev_pool.pop_back();
     } else if (hipEventCreate(&e) != hipSuccess) {
         e = nullptr;
     }
@@ -1057,60 +1125,154 @@ static hipEvent_t take_event(xkrt_hip_dep_domain *d)
 }
 
 // accumulate the elapsed times of completed launches (stream synchronised)
-static void harvest_events(xkrt_hip_dep_domain *d)
+void harvest_events(xkrt_hip_pool *p)
 {
-    for (size_t i = 0; i + 7 <= d->ev_pending.size(); i += 7) {
-        float tot = 0.f;
-        for (int k = 0; k < 6; ++k) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, d->ev_pending[i + k], d->ev_pending[i + k + 1]) == hipSuccess) {
-                d->kernel_ms[k] += ms;
-                tot += ms;
+    size_t i = 0;
+    for (int kind : p->ev_kind) {
+        if (kind == 7) {
+            float tot = 0.f;
+            for (int k = 0; k < 6; ++k) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, p->ev_pending[i + k], p->ev_pending[i + k + 1]) == hipSuccess) {
+                    p->kernel_ms[k] += ms;
+                    tot += ms;
+                }
             }
+            p->launch_ms.push_back(tot);
+        } else {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, p->ev_pending[i], p->ev_pending[i + 1]);
+            p->launch_ms.push_back(ms);
         }
-        d->launch_ms.push_back(tot);
-        d->timed_launches++;
+        p->timed_launches++;
+        i += kind;
     }
-    for (hipEvent_t e : d->ev_pending) d->ev_pool.push_back(e);
-    d->ev_pending.clear();
+    for (hipEvent_t e : p->ev_pending) p->ev_pool.push_back(e);
+    p->ev_pending.clear();
+    p->ev_kind.clear();
 }
 
-static uint64_t grid_bytes(uint64_t nbins, uint32_t fcap, uint32_t rcap)
+void drop_graph(xkrt_hip_pool *p)
 {
-    return nbins * (8ull + 20ull * fcap + 20ull * rcap + 8ull + 48ull);
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+    p->graph_valid = false;
 }
 
-static int alloc_grid(xkrt_hip_dep_domain *d, uint32_t nbx, uint32_t nby, uint32_t **nf, uint32_t **nrs,
-                      uint32_t **fr, uint32_t **rs, uint32_t **cnt, uint32_t **cur, uint4 **act)
+// grow the bin arrays to at least `need` bins (stream synchronised, contents kept)
+int grow_bins(xkrt_hip_pool *p, uint32_t need)
 {
-    const uint64_t nb = (uint64_t)nbx * nby;
-    HIPCHK(hipMalloc(nf, nb * 4));
-    HIPCHK(hipMalloc(nrs, nb * 4));
-    HIPCHK(hipMalloc(fr, nb * d->fcap * 20ull));
-    HIPCHK(hipMalloc(rs, nb * d->rcap * 20ull));
-    HIPCHK(hipMalloc(cnt, nb * 4));
-    HIPCHK(hipMalloc(cur, nb * 4));
-    HIPCHK(hipMalloc(act, nb * 16 * 3));
-    HIPCHK(hipMemsetAsync(*nf, 0, nb * 4, d->stream));
-    HIPCHK(hipMemsetAsync(*nrs, 0, nb * 4, d->stream));
-    HIPCHK(hipMemsetAsync(*cnt, 0, nb * 4, d->stream));
+    uint64_t nc = std::max<uint64_t>(need, (uint64_t)p->cap_bins + p->cap_bins / 2);
+    nc = std::max<uint64_t>(nc, 64);
+    if (nc > 0xFFFFFFull) nc = std::max<uint64_t>(need, 0xFFFFFFull);
+    if (nc > 0xFFFFFFull) return -ERANGE;
+    if (p->cfg.budget_bytes && nc * bin_bytes(p) + p->bytes_scratch > p->cfg.budget_bytes) {
+        nc = need;  // the smallest growth that admits the request
+        if (nc * bin_bytes(p) + p->bytes_scratch > p->cfg.budget_bytes) return -ENOMEM;
+    }
+    HIPCHK(hipStreamSynchronize(p->stream));
+    uint32_t *nf, *nrs, *fr, *rs, *cnt, *cur, *dom;
+    uint4 *act;
+    HIPCHK(hipMalloc(&nf, nc * 4));
+    HIPCHK(hipMalloc(&nrs, nc * 4));
+    HIPCHK(hipMalloc(&fr, nc * p->fcap * 20ull));
+    HIPCHK(hipMalloc(&rs, nc * p->rcap * 20ull));
+    HIPCHK(hipMalloc(&cnt, nc * 4));
+    HIPCHK(hipMalloc(&cur, nc * 4));
+    HIPCHK(hipMalloc(&dom, nc * 4));
+    HIPCHK(hipMalloc(&act, nc * 16 * 3));
+    HIPCHK(hipMemsetAsync(nf, 0, nc * 4, p->stream));
+    HIPCHK(hipMemsetAsync(nrs, 0, nc * 4, p->stream));
+    HIPCHK(hipMemsetAsync(cnt, 0, nc * 4, p->stream));
+    HIPCHK(hipMemsetAsync(dom, 0, nc * 4, p->stream));
+    const uint64_t oc = p->cap_bins;
+    if (oc) {
+        HIPCHK(hipMemcpyAsync(nf, p->bin_nf, oc * 4, hipMemcpyDeviceToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(nrs, p->bin_nrs, oc * 4, hipMemcpyDeviceToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(fr, p->frag, oc * p->fcap * 20ull, hipMemcpyDeviceToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(rs, p->rs, oc * p->rcap * 20ull, hipMemcpyDeviceToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(cnt, p->bin_cnt, oc * 4, hipMemcpyDeviceToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(dom, p->bin_dom, oc * 4, hipMemcpyDeviceToDevice, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+        (void)hipFree(p->bin_nf); (void)hipFree(p->bin_nrs); (void)hipFree(p->frag); (void)hipFree(p->rs);
+        (void)hipFree(p->bin_cnt); (void)hipFree(p->bin_cur); (void)hipFree(p->bin_dom); (void)hipFree(p->active);
+    }
+    p->bin_nf = nf; p->bin_nrs = nrs; p->frag = fr; p->rs = rs;
+    p->bin_cnt = cnt; p->bin_cur = cur; p->bin_dom = dom; p->active = act;
+    // the new tail is free
+    if (!p->free_ranges.empty() && p->free_ranges.back().first + p->free_ranges.back().second == oc)
+        p->free_ranges.back().second += (uint32_t)(nc - oc);
+    else
+        p->free_ranges.push_back({(uint32_t)oc, (uint32_t)(nc - oc)});
+    p->cap_bins = (uint32_t)nc;
+    drop_graph(p);  // the arrays moved
     return 0;
 }
 
-static void free_grid(uint32_t *nf, uint32_t *nrs, uint32_t *fr, uint32_t *rs, uint32_t *cnt, uint32_t *cur, uint4 *act)
+// first-fit range of n bins, cleared and tagged with domain `dom`
+int alloc_bins(xkrt_hip_pool *p, uint32_t n, uint32_t dom, uint32_t *b0)
 {
-    (void)hipFree(nf); (void)hipFree(nrs); (void)hipFree(fr); (void)hipFree(rs);
-    (void)hipFree(cnt); (void)hipFree(cur); (void)hipFree(act);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        for (size_t i = 0; i < p->free_ranges.size(); ++i) {
+            auto &r = p->free_ranges[i];
+            if (r.second < n) continue;
+            *b0 = r.first;
+            r.first += n;
+            r.second -= n;
+            if (r.second == 0) p->free_ranges.erase(p->free_ranges.begin() + (long)i);
+            hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>((n + 255) / 256, 4096u)), dim3(256), 0, p->stream,
+                               p->bin_nf, p->bin_nrs, p->bin_cnt, *b0, n, (DevCounters *)nullptr);
+            hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>((n + 255) / 256, 4096u)), dim3(256), 0,
+                               p->stream, p->bin_dom + *b0, n, dom);
+            return hip_err(hipGetLastError());
+        }
+        // grow so that the trailing free range (if any) can hold n bins
+        uint32_t tail = 0;
+        if (!p->free_ranges.empty() && p->free_ranges.back().first + p->free_ranges.back().second == p->cap_bins)
+            tail = p->free_ranges.back().second;
+        const int rc = grow_bins(p, p->cap_bins + (n - tail));
+        if (rc) return rc;
+    }
+    return -ENOMEM;
 }
 
-// make the bin grid cover [ylo, yhi) (frame coordinates); grows by half the
-// current span in the direction of growth so repeated growth stays cheap
-static int ensure_grid(xkrt_hip_dep_domain *d, uint32_t ylo, uint32_t yhi)
+void free_bins(xkrt_hip_pool *p, uint32_t b0, uint32_t n)
 {
+    if (!n) return;
+    auto &v = p->free_ranges;
+    auto it = std::lower_bound(v.begin(), v.end(), std::make_pair(b0, 0u));
+    it = v.insert(it, {b0, n});
+    size_t i = (size_t)(it - v.begin());
+    if (i + 1 < v.size() && v[i].first + v[i].second == v[i + 1].first) {
+        v[i].second += v[i + 1].second;
+        v.erase(v.begin() + (long)i + 1);
+    }
+    if (i > 0 && v[i - 1].first + v[i - 1].second == v[i].first) {
+        v[i - 1].second += v[i].second;
+        v.erase(v.begin() + (long)i);
+    }
+}
+
+void publish(xkrt_hip_pool *p, xkrt_hip_dep_domain *d)
+{
+    DomDesc &g = p->h_dom[d->idx];
+    g.bin0 = d->bin0; g.nbx = d->nbx; g.nby = d->nby; g.bxs = d->bxs; g.bys = d->bys;
+    g.ybin0 = d->ybin0; g.xmax = (uint32_t)d->LDs; g.pad = 0;
+    p->dom_dirty = true;
+}
+
+// make the domain's grid cover [ylo, yhi) (frame coordinates); grows by half
+// the current span in the direction of growth so repeated growth stays cheap
+int ensure_grid(xkrt_hip_dep_domain *d, uint32_t ylo, uint32_t yhi)
+{
+    xkrt_hip_pool *p = d->pool;
     const uint64_t BY = 1ull << d->bys;
     const uint64_t lo = (uint64_t)ylo & ~(BY - 1), hi = ((uint64_t)yhi + BY - 1) & ~(BY - 1);
     const uint64_t olo = d->ybin0, ohi = (uint64_t)d->ybin0 + (uint64_t)d->nby * BY;
     if (d->nby && lo >= olo && hi <= ohi) return 0;
+    if (d->fixed_frame) return -ERANGE;
     uint64_t nlo = lo, nhi = hi;
     if (d->nby) {
         const uint64_t extra = ((ohi - olo) / 2) & ~(BY - 1);
@@ -1119,228 +1281,288 @@ static int ensure_grid(xkrt_hip_dep_domain *d, uint32_t ylo, uint32_t yhi)
         if (lo < olo) nlo = nlo > extra ? nlo - extra : 0;
         if (hi > ohi) nhi = std::min<uint64_t>(nhi + extra, (1ull << 32) - BY);
     }
-    const uint32_t rows = (uint32_t)((nhi - nlo) / BY);
+    const uint64_t rows = (nhi - nlo) / BY;
     const uint64_t nb_new = (uint64_t)d->nbx * rows;
-    const uint64_t new_bytes = grid_bytes(nb_new, d->fcap, d->rcap);
-    if (d->cfg.budget_bytes && new_bytes + d->bytes_scratch > d->cfg.budget_bytes) return -ENOMEM;
-    if (nb_new > 0xFFFFFFFFull) return -ERANGE;
-    uint32_t *nf, *nrs, *rs, *cnt, *cur;
-    uint4 *act;
-    uint32_t *fr;
-    int rc = alloc_grid(d, d->nbx, rows, &nf, &nrs, &fr, &rs, &cnt, &cur, &act);
+    if (nb_new > 0xFFFFFFull) return -ERANGE;
+    uint32_t nb0 = 0;
+    int rc = alloc_bins(p, (uint32_t)nb_new, d->idx, &nb0);
     if (rc) return rc;
     if (d->nby) {
         const uint32_t dy = (uint32_t)((olo - nlo) / BY);
-        hipLaunchKernelGGL(k_regrid, dim3(d->nbx * d->nby), dim3(256), 0, d->stream, d->bin_nf, d->bin_nrs,
-                           d->frag, d->rs, d->nbx, d->nby, nf, nrs, fr, rs, d->nbx, dy, d->fcap, d->rcap);
+        hipLaunchKernelGGL(k_regrid, dim3(d->nbx * d->nby), dim3(256), 0, p->stream, p->bin_nf, p->bin_nrs, p->frag,
+                           p->rs, p->fcap, p->rcap, d->bin0, d->nbx, d->nby, nb0, d->nbx, dy);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(d->stream));
-        free_grid(d->bin_nf, d->bin_nrs, d->frag, d->rs, d->bin_cnt, d->bin_cur, d->active);
+        free_bins(p, d->bin0, d->nbx * d->nby);
     }
-    d->bin_nf = nf; d->bin_nrs = nrs; d->frag = fr; d->rs = rs;
-    d->bin_cnt = cnt; d->bin_cur = cur; d->active = act;
+    d->bin0 = nb0;
     d->ybin0 = (uint32_t)nlo;
-    d->nby = rows;
-    d->bytes_state = new_bytes;
+    d->nby = (uint32_t)rows;
+    publish(p, d);
     return 0;
 }
 
-static KArgs make_args(xkrt_hip_dep_domain *d, const xkrt_hip_rect_soa_t *r, uint32_t nrec)
+KArgs make_args(xkrt_hip_pool *p, const uint4 *rec)
 {
     KArgs a;
-    a.rx0 = r->x0; a.rx1 = r->x1; a.ry0 = r->y0; a.ry1 = r->y1; a.racc = r->acc; a.rtm = r->tm;
-    a.nrec = nrec;
-    a.bxs = d->bxs; a.bys = d->bys; a.nbx = d->nbx; a.nby = d->nby; a.ybin0 = d->ybin0;
-    a.xmax = (uint32_t)d->LDs;
-    a.bin_nf = d->bin_nf; a.bin_nrs = d->bin_nrs; a.frag = d->frag; a.rs = d->rs;
-    a.fcap = d->fcap; a.rcap = d->rcap;
-    a.bin_cnt = d->bin_cnt; a.bin_cur = d->bin_cur; a.active = d->active; a.binrec = d->binrec;
-    a.binrec_cap = d->binrec_cap;
-    a.nbins = d->nbx * d->nby;
-    a.max_tier = d->kclass;
+    a.rec = rec;
+    a.dom = p->d_dom;
+    a.bin_dom = p->bin_dom;
+    a.bin_nf = p->bin_nf; a.bin_nrs = p->bin_nrs; a.frag = p->frag; a.rs = p->rs;
+    a.fcap = p->fcap; a.rcap = p->rcap;
+    a.bin_cnt = p->bin_cnt; a.bin_cur = p->bin_cur; a.active = p->active; a.binrec = p->binrec;
+    a.binrec_cap = p->binrec_cap;
+    a.nbins = p->cap_bins;
+    a.max_tier = p->kclass;
     for (int t = 0; t < 3; ++t) {
         a.tier_f[t] = KCLASSES[t].fcap;
         a.tier_g[t] = KCLASSES[t].rcap;
         a.tier_l[t] = KCLASSES[t].lcap;
     }
-    a.cand = d->cand; a.cand_cap = d->cand_cap;
-    a.hkeys = d->hkeys; a.hvals = d->hvals; a.hmask = d->hmask;
-    a.eslot = d->eslot; a.edges = d->edges; a.edge_cap = d->edge_cap;
-    a.ctr = d->ctr;
+    a.cand = p->cand; a.cand_cap = p->cand_cap;
+    a.hkeys = p->hkeys; a.hvals = p->hvals; a.hmask = p->hmask;
+    a.eslot = p->eslot; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
+    a.ctr = p->ctr;
     return a;
 }
 
 template <int F, int R, int L>
-static int launch_resolve(xkrt_hip_dep_domain *d, const KArgs &a, uint32_t nbins, uint32_t tier)
+void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier)
 {
-    const size_t lds = lds_bytes<F, R, L>();
-    static bool attr_set = false;
-    if (!attr_set) {
-        HIPCHK(hipFuncSetAttribute((const void *)k_resolve<F, R, L>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-        attr_set = true;
-    }
     // enough blocks to fill the chip at this tier's LDS footprint; each walks its share of the list
+    const size_t lds = lds_bytes<F, R, L>();
     const uint32_t per_cu = std::max<uint32_t>(1u, (uint32_t)(160u * 1024u / lds));
-    const uint32_t blocks = std::min<uint32_t>(nbins, 256u * std::min<uint32_t>(per_cu, 8u));
-    hipLaunchKernelGGL((k_resolve<F, R, L>), dim3(blocks), dim3(64), lds, d->stream, a, tier);
-    return hip_err(hipGetLastError());
+    const uint32_t blocks = std::min<uint32_t>(p->cap_bins, 256u * std::min<uint32_t>(per_cu, 8u));
+    hipLaunchKernelGGL((k_resolve<F, R, L>), dim3(blocks), dim3(64), lds, p->stream, a, tier);
 }
 
-static int enqueue(xkrt_hip_dep_domain *d, const xkrt_hip_rect_soa_t *rec, uint32_t nrec)
+// the kernels of one launch; ev (7 events) brackets them when timing
+void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks, hipEvent_t *ev)
 {
-    if (nrec == 0) return 0;
-    const KArgs a = make_args(d, rec, nrec);
-    const uint32_t nbins = d->nbx * d->nby;
-    const uint32_t rblocks = std::min<uint32_t>((nrec + 255) / 256, 8192u);
-    hipEvent_t ev[7] = {};
-    if (d->timing)
-        for (int k = 0; k < 7; ++k) ev[k] = take_event(d);
-    auto mark = [&](int k) { if (d->timing && ev[k]) (void)hipEventRecord(ev[k], d->stream); };
+    auto mark = [&](int k) { if (ev && ev[k]) (void)hipEventRecord(ev[k], p->stream); };
     mark(0);
-    hipLaunchKernelGGL(k_count, dim3(rblocks), dim3(256), 0, d->stream, a);
+    hipLaunchKernelGGL(k_count, dim3(rblocks), dim3(256), 0, p->stream, a);
     mark(1);
-    hipLaunchKernelGGL(k_alloc, dim3((nbins + 255) / 256), dim3(256), 0, d->stream, a, nbins);
+    hipLaunchKernelGGL(k_alloc, dim3((p->cap_bins + 255) / 256), dim3(256), 0, p->stream, a, p->cap_bins);
     mark(2);
-    hipLaunchKernelGGL(k_fill, dim3(rblocks), dim3(256), 0, d->stream, a);
+    hipLaunchKernelGGL(k_fill, dim3(rblocks), dim3(256), 0, p->stream, a);
     mark(3);
-    int rc = 0;
     // tier 0 first; bins that outgrow a tier are retried by the next one
-    rc = launch_resolve<64, 128, 256>(d, a, nbins, 0);
-    if (!rc && d->kclass >= 1) rc = launch_resolve<256, 512, 512>(d, a, nbins, 1);
-    if (!rc && d->kclass >= 2) rc = launch_resolve<1024, 2048, 1024>(d, a, nbins, 2);
-    if (rc) return rc;
+    launch_resolve<64, 128, 256>(p, a, 0);
+    if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, a, 1);
+    if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, a, 2);
     mark(4);
-    hipLaunchKernelGGL(k_dedup, dim3(1024), dim3(256), 0, d->stream, a);
+    hipLaunchKernelGGL(k_dedup, dim3(1024), dim3(256), 0, p->stream, a);
     mark(5);
-    hipLaunchKernelGGL(k_final, dim3(1024), dim3(256), 0, d->stream, a);
+    hipLaunchKernelGGL(k_final, dim3(1024), dim3(256), 0, p->stream, a);
     mark(6);
-    HIPCHK(hipGetLastError());
-    if (d->timing) {
-        bool ok = true;
-        for (int k = 0; k < 7; ++k) ok = ok && ev[k];
-        if (ok) d->ev_pending.insert(d->ev_pending.end(), ev, ev + 7);
-        else for (int k = 0; k < 7; ++k) if (ev[k]) d->ev_pool.push_back(ev[k]);
-    }
-    d->n_launch++;
+}
+
+int push_domains(xkrt_hip_pool *p)
+{
+    if (!p->dom_dirty) return 0;
+    // stream ordered, so launches in flight still read the table they were enqueued with
+    HIPCHK(hipMemcpyAsync(p->d_dom, p->h_dom, sizeof(p->h_dom), hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));  // h_dom is pageable: keep it until the copy is done
+    p->dom_dirty = false;
     return 0;
 }
 
-static int err_to_errno(uint32_t e)
+// one launch over nrec records at rec (device); the record count is already
+// in the counters when `counted`, else it is set by a kernel
+int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
+{
+    int rc = push_domains(p);
+    if (rc) return rc;
+    if (!counted) hipLaunchKernelGGL(k_setn, dim3(1), dim3(64), 0, p->stream, p->ctr, nrec);
+    const bool graphed = p->cfg.graph && rec == p->rec_dev;
+    if (graphed) {
+        if (!p->graph_valid) {
+            drop_graph(p);
+            const KArgs a = make_args(p, p->rec_dev);
+            const uint32_t rb = (uint32_t)std::min<uint64_t>((p->rec_cap + 255) / 256, 8192u);
+            HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+            launch_kernels(p, a, rb, nullptr);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(p->stream, &g);
+            if (e != hipSuccess) return hip_err(e);
+            p->graph = g;
+            HIPCHK(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
+            p->graph_valid = true;
+        }
+        hipEvent_t ev[2] = {};
+        if (p->timing) {
+            ev[0] = take_event(p);
+            ev[1] = take_event(p);
+        }
+        if (ev[0]) (void)hipEventRecord(ev[0], p->stream);
+        HIPCHK(hipGraphLaunch(p->gexec, p->stream));
+        if (ev[1]) (void)hipEventRecord(ev[1], p->stream);
+        if (ev[0] && ev[1]) {
+            p->ev_pending.insert(p->ev_pending.end(), ev, ev + 2);
+            p->ev_kind.push_back(2);
+        } else {
+            for (hipEvent_t e : ev) if (e) p->ev_pool.push_back(e);
+        }
+    } else {
+        const KArgs a = make_args(p, rec);
+        const uint32_t rb = std::min<uint32_t>((nrec + 255) / 256, 8192u);
+        hipEvent_t ev[7] = {};
+        bool ok = p->timing;
+        if (p->timing)
+            for (int k = 0; k < 7; ++k) ok = (ev[k] = take_event(p)) && ok;
+        launch_kernels(p, a, rb, ok ? ev : nullptr);
+        if (ok) {
+            p->ev_pending.insert(p->ev_pending.end(), ev, ev + 7);
+            p->ev_kind.push_back(7);
+        } else {
+            for (hipEvent_t e : ev) if (e) p->ev_pool.push_back(e);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    p->n_launch++;
+    return 0;
+}
+
+int err_to_errno(uint32_t e)
 {
     if (e & (ERR_LIST | ERR_FRAG | ERR_READS | ERR_BINREC)) return -ENOSPC;
     if (e & (ERR_CAND | ERR_HASH | ERR_EDGES)) return -EOVERFLOW;
     return 0;
 }
 
-extern "C" {
-
-int xkrt_hip_init(int device)
+int read_counters(xkrt_hip_pool *p)
 {
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n <= device) return -ENODEV;
-    return hip_err(hipSetDevice(device));
+    HIPCHK(hipMemcpyAsync(p->ctr_host, p->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    harvest_events(p);
+    if (p->ctr_host->err) {
+        p->poisoned = true;
+        return err_to_errno(p->ctr_host->err);
+    }
+    return 0;
 }
 
-int xkrt_hip_dep_domain_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_dep_config_t *cfg, void *stream,
-                               xkrt_hip_dep_domain_t *out)
+void destroy_pool(xkrt_hip_pool *p)
 {
-    if (!out || ld == 0 || sizeof_type == 0) return -EINVAL;
-    const uint64_t LDs = ld * sizeof_type;
-    if (LDs > 0xFFFFFFFFull) return -ERANGE;
-    xkrt_hip_dep_domain *d = new xkrt_hip_dep_domain();
-    d->ld = ld; d->s = sizeof_type; d->LDs = LDs;
-    if (cfg) d->cfg = *cfg;
-    uint32_t BX = d->cfg.bin_x_bytes ? d->cfg.bin_x_bytes : 1024u;
-    uint32_t BY = d->cfg.bin_y_cols ? d->cfg.bin_y_cols : 128u;
-    if (!is_pow2(BX) || !is_pow2(BY) || d->cfg.kernel > 2) { delete d; return -EINVAL; }
-    d->bxs = ilog2(BX); d->bys = ilog2(BY);
-    d->kclass = d->cfg.kernel;
-    d->fcap = KCLASSES[d->kclass].fcap;
-    d->rcap = KCLASSES[d->kclass].rcap;
-    d->nbx = (uint32_t)((LDs + BX - 1) / BX);
-    if (hipGetDevice(&d->device) != hipSuccess) { delete d; return -ENODEV; }
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    drop_graph(p);
+    for (hipEvent_t e : p->ev_pending) (void)hipEventDestroy(e);
+    for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
+    void *dev[] = {p->bin_nf, p->bin_nrs, p->frag, p->rs, p->bin_cnt, p->bin_cur, p->bin_dom, p->active, p->d_dom,
+                   p->binrec, p->cand, p->hkeys, p->hvals, p->eslot, p->edges, p->edom, p->ctr, p->rec_dev};
+    for (void *q : dev)
+        if (q) (void)hipFree(q);
+    if (p->ctr_host) (void)hipHostFree(p->ctr_host);
+    if (p->rec_host) (void)hipHostFree(p->rec_host);
+    if (p->nrec_host) (void)hipHostFree(p->nrec_host);
+    if (p->staged) (void)hipEventDestroy(p->staged);
+    if (p->own_stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool **out)
+{
+    xkrt_hip_pool *p = new xkrt_hip_pool();
+    if (cfg) p->cfg = *cfg;
+    if (p->cfg.kernel > 2) { delete p; return -EINVAL; }
+    p->kclass = p->cfg.kernel;
+    p->fcap = KCLASSES[p->kclass].fcap;
+    p->rcap = KCLASSES[p->kclass].rcap;
+    if (hipGetDevice(&p->device) != hipSuccess) { delete p; return -ENODEV; }
     if (stream) {
-        d->stream = (hipStream_t)stream;
+        p->stream = (hipStream_t)stream;
     } else {
-        if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) { delete d; return -ENODEV; }
-        d->own_stream = true;
+        if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) { delete p; return -ENODEV; }
+        p->own_stream = true;
     }
-    const uint64_t max_rects = d->cfg.max_rects ? d->cfg.max_rects : (1ull << 20);
-    d->cfg.max_rects = max_rects;
-    d->binrec_cap = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, std::max<uint64_t>(4 * max_rects, 1 << 16));
-    d->cand_cap = (uint32_t)std::min<uint64_t>(0x3FFFFFF0ull, std::max<uint64_t>(32 * max_rects, 1 << 20));
+    const uint64_t max_rects = p->cfg.max_rects ? p->cfg.max_rects : (1ull << 20);
+    p->cfg.max_rects = max_rects;
+    p->rec_cap = max_rects;
+    p->binrec_cap = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, std::max<uint64_t>(4 * max_rects, 1 << 16));
+    p->cand_cap = (uint32_t)std::min<uint64_t>(0x3FFFFFF0ull, std::max<uint64_t>(32 * max_rects, 1 << 20));
     uint64_t hsz = 1;
-    while (hsz < 2ull * d->cand_cap) hsz <<= 1;
-    d->hmask = (uint32_t)(hsz - 1);
-    d->edge_cap = d->cfg.max_edges ? d->cfg.max_edges : std::max<uint64_t>(16 * max_rects, 1 << 20);
-    d->bytes_scratch = 32ull * d->binrec_cap + 12ull * d->cand_cap + 12ull * hsz + 4ull * hsz + 8ull * d->edge_cap +
-                       sizeof(DevCounters);
+    while (hsz < 2ull * p->cand_cap) hsz <<= 1;
+    p->hmask = (uint32_t)(hsz - 1);
+    p->edge_cap = p->cfg.max_edges ? p->cfg.max_edges : std::max<uint64_t>(16 * max_rects, 1 << 20);
+    p->bytes_scratch = 32ull * p->binrec_cap + 12ull * p->cand_cap + 12ull * hsz + 4ull * hsz +
+                       12ull * p->edge_cap + 32ull * p->rec_cap + sizeof(DevCounters) + sizeof(p->h_dom);
     int rc = 0;
-    auto fail = [&](int e) { rc = e; };
-    if (d->cfg.budget_bytes && d->bytes_scratch > d->cfg.budget_bytes) fail(-ENOMEM);
-    if (!rc) fail(hip_err(hipMalloc(&d->binrec, 32ull * d->binrec_cap)));
-    if (!rc) fail(hip_err(hipMalloc(&d->cand, 12ull * d->cand_cap)));
-    if (!rc) fail(hip_err(hipMalloc(&d->hkeys, 8ull * hsz)));
-    if (!rc) fail(hip_err(hipMalloc(&d->hvals, 4ull * hsz)));
-    if (!rc) fail(hip_err(hipMalloc(&d->eslot, 4ull * hsz)));
-    if (!rc) fail(hip_err(hipMalloc(&d->edges, 8ull * d->edge_cap)));
-    if (!rc) fail(hip_err(hipMalloc(&d->ctr, sizeof(DevCounters))));
-    if (!rc) fail(hip_err(hipHostMalloc(&d->ctr_host, sizeof(DevCounters), hipHostMallocDefault)));
-    if (!rc) fail(hip_err(hipMemsetAsync(d->hkeys, 0xFF, 8ull * hsz, d->stream)));
-    if (!rc) fail(hip_err(hipMemsetAsync(d->hvals, 0xFF, 4ull * hsz, d->stream)));
-    if (!rc) fail(hip_err(hipMemsetAsync(d->ctr, 0, sizeof(DevCounters), d->stream)));
-    if (!rc) fail(hip_err(hipEventCreateWithFlags(&d->staged, hipEventDisableTiming)));
-    if (!rc && d->cfg.y_hint_lo < d->cfg.y_hint_hi) {
-        d->ybase = d->cfg.y_hint_lo;
-        d->framed = true;
-        const uint64_t span = d->cfg.y_hint_hi - d->cfg.y_hint_lo;
-        if (span >= (1ull << 32)) fail(-ERANGE);
-        else fail(ensure_grid(d, 0, (uint32_t)span));
-    }
-    if (!rc) fail(hip_err(hipStreamSynchronize(d->stream)));
+    auto chk = [&](hipError_t e) { if (!rc) rc = hip_err(e); };
+    if (p->cfg.budget_bytes && p->bytes_scratch > p->cfg.budget_bytes) rc = -ENOMEM;
+    chk(hipMalloc(&p->binrec, 32ull * p->binrec_cap));
+    chk(hipMalloc(&p->cand, 12ull * p->cand_cap));
+    chk(hipMalloc(&p->hkeys, 8ull * hsz));
+    chk(hipMalloc(&p->hvals, 4ull * hsz));
+    chk(hipMalloc(&p->eslot, 4ull * hsz));
+    chk(hipMalloc(&p->edges, 8ull * p->edge_cap));
+    chk(hipMalloc(&p->edom, 4ull * p->edge_cap));
+    chk(hipMalloc(&p->ctr, sizeof(DevCounters)));
+    chk(hipMalloc(&p->d_dom, sizeof(p->h_dom)));
+    chk(hipMalloc(&p->rec_dev, 32ull * p->rec_cap));
+    chk(hipHostMalloc(&p->ctr_host, sizeof(DevCounters), hipHostMallocDefault));
+    chk(hipHostMalloc(&p->rec_host, 32ull * p->rec_cap, hipHostMallocDefault));
+    chk(hipHostMalloc(&p->nrec_host, 64, hipHostMallocDefault));
+    if (!rc) chk(hipMemsetAsync(p->hkeys, 0xFF, 8ull * hsz, p->stream));
+    if (!rc) chk(hipMemsetAsync(p->hvals, 0xFF, 4ull * hsz, p->stream));
+    if (!rc) chk(hipMemsetAsync(p->ctr, 0, sizeof(DevCounters), p->stream));
+    chk(hipEventCreateWithFlags(&p->staged, hipEventDisableTiming));
+    chk(hipFuncSetAttribute((const void *)k_resolve<64, 128, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes<64, 128, 256>()));
+    chk(hipFuncSetAttribute((const void *)k_resolve<256, 512, 512>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes<256, 512, 512>()));
+    chk(hipFuncSetAttribute((const void *)k_resolve<1024, 2048, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes<1024, 2048, 1024>()));
+    if (!rc) chk(hipStreamSynchronize(p->stream));
     if (rc) {
-        xkrt_hip_dep_domain_destroy(d);
+        destroy_pool(p);
         return rc;
     }
+    *out = p;
+    return 0;
+}
+
+int add_domain(xkrt_hip_pool *p, uint64_t ld, uint64_t s, uint32_t BX, uint32_t BY, xkrt_hip_dep_domain **out)
+{
+    if (ld == 0 || s == 0) return -EINVAL;
+    const uint64_t LDs = ld * s;
+    if (LDs > 0xFFFFFFFFull) return -ERANGE;
+    if (!is_pow2(BX) || !is_pow2(BY)) return -EINVAL;
+    uint32_t idx = MAX_DOMS;
+    for (uint32_t i = 0; i < MAX_DOMS; ++i)
+        if (!p->doms[i]) { idx = i; break; }
+    if (idx == MAX_DOMS) return -ENOSPC;
+    xkrt_hip_dep_domain *d = new xkrt_hip_dep_domain();
+    d->pool = p;
+    d->idx = idx;
+    d->ld = ld; d->s = s; d->LDs = LDs;
+    d->bxs = ilog2(BX); d->bys = ilog2(BY);
+    d->nbx = (uint32_t)((LDs + BX - 1) / BX);
+    p->doms[idx] = d;
+    p->ndoms++;
+    publish(p, d);
     *out = d;
     return 0;
 }
 
-int xkrt_hip_dep_domain_destroy(xkrt_hip_dep_domain_t d)
+void remove_domain(xkrt_hip_dep_domain *d)
 {
-    if (!d) return -EINVAL;
-    if (d->stream) (void)hipStreamSynchronize(d->stream);
-    for (hipEvent_t e : d->ev_pending) (void)hipEventDestroy(e);
-    for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
-    if (d->nby) free_grid(d->bin_nf, d->bin_nrs, d->frag, d->rs, d->bin_cnt, d->bin_cur, d->active);
-    (void)hipFree(d->binrec); (void)hipFree(d->cand); (void)hipFree(d->hkeys); (void)hipFree(d->hvals);
-    (void)hipFree(d->eslot); (void)hipFree(d->edges); (void)hipFree(d->ctr);
-    if (d->ctr_host) (void)hipHostFree(d->ctr_host);
-    for (int i = 0; i < 6; ++i) {
-        if (d->rec_dev[i]) (void)hipFree(d->rec_dev[i]);
-        if (d->rec_host[i]) (void)hipHostFree(d->rec_host[i]);
-    }
-    if (d->staged) (void)hipEventDestroy(d->staged);
-    if (d->own_stream) (void)hipStreamDestroy(d->stream);
+    xkrt_hip_pool *p = d->pool;
+    if (d->nby) free_bins(p, d->bin0, d->nbx * d->nby);
+    p->doms[d->idx] = nullptr;
+    p->h_dom[d->idx] = DomDesc{};
+    p->dom_dirty = true;
+    p->ndoms--;
     delete d;
-    return 0;
 }
 
-int xkrt_hip_dep_frame(xkrt_hip_dep_domain_t d, uint64_t *ybase)
+// host records (AoS) of n accesses into out[0 .. cap): rects in the domain
+// frame, access ids from first_id, launch-local task numbers from task0;
+// *ntask = tasks used
+int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t n, uint32_t first_id, uint32_t task0,
+                  uint64_t cap, xkrt_hip_rect_t *out, uint64_t *nrect, uint32_t *ylo, uint32_t *yhi, uint32_t *ntask)
 {
-    if (!d || !ybase) return -EINVAL;
-    *ybase = d->ybase;
-    return d->framed ? 0 : -EINVAL;
-}
-
-int xkrt_hip_dep_build_records(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n, uint32_t first_id,
-                               uint64_t cap, uint32_t *x0, uint32_t *x1, uint32_t *y0, uint32_t *y1, uint32_t *accid,
-                               uint32_t *tm, uint64_t *nrect, uint32_t *ylo, uint32_t *yhi)
-{
-    if (!d || (!acc && n) || !nrect) return -EINVAL;
     const uint64_t LDs = d->LDs, s = d->s;
-    // frame: fixed by the first access ever seen
+    // frame: fixed by the first access ever seen (or given at creation)
     if (!d->framed && n) {
         uint64_t ymin = UINT64_MAX;
         for (uint64_t i = 0; i < n; ++i) ymin = std::min(ymin, acc[i].addr / LDs);
@@ -1348,7 +1570,7 @@ int xkrt_hip_dep_build_records(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t 
         d->framed = true;
     }
     uint64_t k = 0, lo = UINT64_MAX, hi = 0;
-    uint32_t tl = 0;
+    uint32_t tl = task0;
     uint64_t maxx = d->max_xlen;
     for (uint64_t i = 0; i < n; ++i) {
         const xkrt_hip_access_t &q = acc[i];
@@ -1378,10 +1600,13 @@ int xkrt_hip_dep_build_records(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t 
             if (r[j][2] < d->ybase || r[j][3] - d->ybase > 0xFFFFFFFFull) return -ERANGE;
             maxx = std::max(maxx, r[j][1] - r[j][0]);
             if (k < cap) {
-                x0[k] = (uint32_t)r[j][0]; x1[k] = (uint32_t)r[j][1];
-                y0[k] = (uint32_t)(r[j][2] - d->ybase); y1[k] = (uint32_t)(r[j][3] - d->ybase);
-                accid[k] = first_id + (uint32_t)i;
-                tm[k] = (tl << 8) | q.mode;
+                xkrt_hip_rect_t &o = out[k];
+                o.x0 = (uint32_t)r[j][0]; o.x1 = (uint32_t)r[j][1];
+                o.y0 = (uint32_t)(r[j][2] - d->ybase); o.y1 = (uint32_t)(r[j][3] - d->ybase);
+                o.acc = first_id + (uint32_t)i;
+                o.tm = (tl << 8) | q.mode;
+                o.dom = d->idx;
+                o.pad = 0;
             }
             lo = std::min(lo, r[j][2] - d->ybase);
             hi = std::max(hi, r[j][3] - d->ybase);
@@ -1392,21 +1617,281 @@ int xkrt_hip_dep_build_records(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t 
     *nrect = k;
     if (ylo) *ylo = n ? (uint32_t)lo : 0;
     if (yhi) *yhi = n ? (uint32_t)hi : 0;
+    if (ntask) *ntask = n ? tl - task0 + 1 : 0;
     return 0;
 }
 
-int xkrt_hip_dep_submit_device(xkrt_hip_dep_domain_t d, const xkrt_hip_rect_soa_t *rec, uint64_t nrect,
-                               uint64_t naccess, uint32_t ylo, uint32_t yhi)
+// append a batch to the pool's open launch (host staging)
+int stage(xkrt_hip_pool *p, xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t n, uint64_t *first_id)
+{
+    if (p->poisoned) return -EINVAL;
+    if (first_id) *first_id = d->next_id;
+    if (n == 0) return 0;
+    if (n > 0xFFFFFFFFull || d->next_id + n > 0xFFFFFFFFull) return -ERANGE;
+    if (p->open_nrec + 2 * n > p->rec_cap) return p->open_nrec ? -ENOSPC : -EINVAL;
+    // the staging buffer must be free: wait for the previous upload
+    if (p->staged_pending) {
+        HIPCHK(hipEventSynchronize(p->staged));
+        p->staged_pending = false;
+    }
+    uint64_t nrect = 0;
+    uint32_t ylo = 0, yhi = 0, ntask = 0;
+    const uint64_t saved_max = d->max_xlen;
+    const bool was_framed = d->framed;
+    int rc = build_records(d, acc, n, (uint32_t)d->next_id, p->open_tasks, p->rec_cap - p->open_nrec,
+                           (xkrt_hip_rect_t *)p->rec_host + p->open_nrec, &nrect, &ylo, &yhi, &ntask);
+    if (!rc) rc = ensure_grid(d, ylo, yhi);
+    if (rc) {
+        d->max_xlen = saved_max;
+        if (!was_framed) d->framed = false;
+        return rc;
+    }
+    p->open_nrec += (uint32_t)nrect;
+    p->open_tasks += ntask;
+    p->open_acc += n;
+    p->open_batches++;
+    d->next_id += n;
+    d->n_acc += n;
+    d->n_rect += nrect;
+    return 0;
+}
+
+int launch_open(xkrt_hip_pool *p)
+{
+    if (p->open_nrec == 0) return 0;
+    const uint32_t n = p->open_nrec;
+    *p->nrec_host = n;
+    HIPCHK(hipMemcpyAsync(p->rec_dev, p->rec_host, 32ull * n, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipMemcpyAsync(&p->ctr->nrec, p->nrec_host, 4, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipEventRecord(p->staged, p->stream));
+    p->staged_pending = true;
+    p->open_nrec = 0;
+    p->open_tasks = 0;
+    p->open_acc = 0;
+    p->open_batches = 0;
+    const int rc = enqueue(p, p->rec_dev, n, true);
+    if (rc) p->poisoned = true;
+    return rc;
+}
+
+} // namespace
+
+extern "C" {
+
+int xkrt_hip_init(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device) return -ENODEV;
+    return hip_err(hipSetDevice(device));
+}
+
+// ---- pools
+
+int xkrt_hip_pool_create(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool_t *out)
+{
+    if (!out) return -EINVAL;
+    *out = nullptr;
+    return create_pool(cfg, stream, out);
+}
+
+int xkrt_hip_pool_destroy(xkrt_hip_pool_t p)
+{
+    if (!p) return -EINVAL;
+    for (uint32_t i = 0; i < MAX_DOMS; ++i)
+        if (p->doms[i]) remove_domain(p->doms[i]);
+    destroy_pool(p);
+    return 0;
+}
+
+int xkrt_hip_pool_domain_create(xkrt_hip_pool_t p, uint64_t ld, uint64_t sizeof_type, uint32_t bin_x_bytes,
+                                uint32_t bin_y_cols, uint64_t y_lo, uint64_t y_hi, xkrt_hip_dep_domain_t *out)
+{
+    if (!p || !out || y_hi <= y_lo || y_hi - y_lo >= (1ull << 32)) return -EINVAL;
+    *out = nullptr;
+    xkrt_hip_dep_domain *d = nullptr;
+    int rc = add_domain(p, ld, sizeof_type, bin_x_bytes ? bin_x_bytes : 1024u, bin_y_cols ? bin_y_cols : 128u, &d);
+    if (rc) return rc;
+    d->ybase = y_lo;
+    d->framed = true;
+    rc = ensure_grid(d, 0, (uint32_t)(y_hi - y_lo));
+    if (rc) {
+        remove_domain(d);
+        return rc;  // -ENOMEM: the pool's budget cannot admit the domain now
+    }
+    d->fixed_frame = true;
+    *out = d;
+    return 0;
+}
+
+int xkrt_hip_pool_stage(xkrt_hip_pool_t p, xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n,
+                        uint64_t *first_id)
+{
+    if (!p || !d || d->pool != p || (!acc && n)) return -EINVAL;
+    return stage(p, d, acc, n, first_id);
+}
+
+int xkrt_hip_pool_launch(xkrt_hip_pool_t p)
+{
+    if (!p) return -EINVAL;
+    if (p->poisoned) return -EINVAL;
+    return launch_open(p);
+}
+
+int xkrt_hip_pool_open(xkrt_hip_pool_t p, uint64_t *nrect, uint64_t *naccess, uint64_t *nbatches)
+{
+    if (!p) return -EINVAL;
+    if (nrect) *nrect = p->open_nrec;
+    if (naccess) *naccess = p->open_acc;
+    if (nbatches) *nbatches = p->open_batches;
+    return 0;
+}
+
+int xkrt_hip_pool_sync(xkrt_hip_pool_t p, uint64_t *nedges)
+{
+    if (!p) return -EINVAL;
+    const int rc = read_counters(p);
+    if (nedges) *nedges = p->ctr_host->edge_total;
+    return rc;
+}
+
+int xkrt_hip_pool_edges_copy(xkrt_hip_pool_t p, uint32_t *edges, uint32_t *doms, uint64_t cap, uint64_t *n)
+{
+    if (!p || !n || (cap && (!edges || !doms))) return -EINVAL;
+    int rc = read_counters(p);
+    const uint64_t ne = p->ctr_host->edge_total;
+    *n = ne;
+    if (rc) return rc;
+    const uint64_t m = std::min(ne, cap);
+    if (m) {
+        HIPCHK(hipMemcpyAsync(edges, p->edges, 8 * m, hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipMemcpyAsync(doms, p->edom, 4 * m, hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+    }
+    hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, p->stream, p->ctr);
+    return hip_err(hipGetLastError());
+}
+
+int xkrt_hip_pool_stats(xkrt_hip_pool_t p, xkrt_hip_pool_stats_t *out)
+{
+    if (!p || !out) return -EINVAL;
+    memset(out, 0, sizeof(*out));
+    uint64_t freeb = 0;
+    for (auto &r : p->free_ranges) freeb += r.second;
+    out->domains = p->ndoms;
+    out->bins_capacity = p->cap_bins;
+    out->bins_used = p->cap_bins - freeb;
+    out->device_bytes = (uint64_t)p->cap_bins * bin_bytes(p) + p->bytes_scratch;
+    out->bytes_per_bin = bin_bytes(p);
+    out->launches = p->n_launch;
+    out->graph = p->graph_valid ? 1 : 0;
+    out->poisoned = p->poisoned ? 1 : 0;
+    return 0;
+}
+
+int xkrt_hip_pool_timing(xkrt_hip_pool_t p, int enable)
+{
+    if (!p) return -EINVAL;
+    p->timing = enable != 0;
+    return 0;
+}
+
+int xkrt_hip_pool_launch_times(xkrt_hip_pool_t p, float *ms, uint64_t cap, uint64_t *n, int clear)
+{
+    if (!p || !n || (cap && !ms)) return -EINVAL;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    harvest_events(p);
+    *n = p->launch_ms.size();
+    for (uint64_t i = 0; i < std::min<uint64_t>(cap, p->launch_ms.size()); ++i) ms[i] = p->launch_ms[i];
+    if (clear) p->launch_ms.clear();
+    return 0;
+}
+
+void *xkrt_hip_pool_stream(xkrt_hip_pool_t p) { return p ? (void *)p->stream : nullptr; }
+
+int xkrt_hip_dep_index(xkrt_hip_dep_domain_t d, uint32_t *idx)
+{
+    if (!d || !idx) return -EINVAL;
+    *idx = d->idx;
+    return 0;
+}
+
+// ---- single domains (a private pool each)
+
+int xkrt_hip_dep_domain_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_dep_config_t *cfg, void *stream,
+                               xkrt_hip_dep_domain_t *out)
+{
+    if (!out || ld == 0 || sizeof_type == 0) return -EINVAL;
+    *out = nullptr;
+    if (ld * sizeof_type > 0xFFFFFFFFull) return -ERANGE;
+    xkrt_hip_dep_config_t c{};
+    if (cfg) c = *cfg;
+    const uint32_t BX = c.bin_x_bytes ? c.bin_x_bytes : 1024u, BY = c.bin_y_cols ? c.bin_y_cols : 128u;
+    if (!is_pow2(BX) || !is_pow2(BY) || c.kernel > 2) return -EINVAL;
+    xkrt_hip_pool_config_t pc{};
+    pc.kernel = c.kernel;
+    pc.max_rects = c.max_rects;
+    pc.max_edges = c.max_edges;
+    pc.budget_bytes = c.budget_bytes;
+    xkrt_hip_pool *p = nullptr;
+    int rc = create_pool(&pc, stream, &p);
+    if (rc) return rc;
+    xkrt_hip_dep_domain *d = nullptr;
+    rc = add_domain(p, ld, sizeof_type, BX, BY, &d);
+    if (!rc) {
+        d->private_pool = true;
+        if (c.y_hint_lo < c.y_hint_hi) {
+            d->ybase = c.y_hint_lo;
+            d->framed = true;
+            const uint64_t span = c.y_hint_hi - c.y_hint_lo;
+            rc = span >= (1ull << 32) ? -ERANGE : ensure_grid(d, 0, (uint32_t)span);
+        }
+    }
+    if (!rc) rc = hip_err(hipStreamSynchronize(p->stream));
+    if (rc) {
+        xkrt_hip_pool_destroy(p);
+        return rc;
+    }
+    *out = d;
+    return 0;
+}
+
+int xkrt_hip_dep_domain_destroy(xkrt_hip_dep_domain_t d)
+{
+    if (!d) return -EINVAL;
+    xkrt_hip_pool *p = d->pool;
+    if (d->private_pool) return xkrt_hip_pool_destroy(p);
+    (void)hipStreamSynchronize(p->stream);  // no launch may still read the domain's bins
+    remove_domain(d);
+    return 0;
+}
+
+int xkrt_hip_dep_frame(xkrt_hip_dep_domain_t d, uint64_t *ybase)
+{
+    if (!d || !ybase) return -EINVAL;
+    *ybase = d->ybase;
+    return d->framed ? 0 : -EINVAL;
+}
+
+int xkrt_hip_dep_build_records(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n, uint32_t first_id,
+                               uint64_t cap, xkrt_hip_rect_t *out, uint64_t *nrect, uint32_t *ylo, uint32_t *yhi)
+{
+    if (!d || (!acc && n) || !nrect || (cap && !out)) return -EINVAL;
+    return build_records(d, acc, n, first_id, 0, cap, out, nrect, ylo, yhi, nullptr);
+}
+
+int xkrt_hip_dep_submit_device(xkrt_hip_dep_domain_t d, const xkrt_hip_rect_t *rec, uint64_t nrect, uint64_t naccess,
+                               uint32_t ylo, uint32_t yhi)
 {
     if (!d || !rec) return -EINVAL;
-    if (d->poisoned) return -EINVAL;
+    xkrt_hip_pool *p = d->pool;
+    if (p->poisoned) return -EINVAL;
     if (nrect == 0) return 0;
-    if (nrect > d->cfg.max_rects) return -EINVAL;
+    if (nrect > p->cfg.max_rects) return -EINVAL;
     if (!d->framed) d->framed = true;  // device records are already in the frame (ybase 0 unless hinted)
     int rc = ensure_grid(d, ylo, yhi);
     if (rc) return rc;
-    rc = enqueue(d, rec, (uint32_t)nrect);
-    if (rc) { d->poisoned = true; return rc; }
+    rc = enqueue(p, (const uint4 *)rec, (uint32_t)nrect, false);
+    if (rc) { p->poisoned = true; return rc; }
     d->n_acc += naccess;
     d->n_rect += nrect;
     return 0;
@@ -1415,104 +1900,53 @@ int xkrt_hip_dep_submit_device(xkrt_hip_dep_domain_t d, const xkrt_hip_rect_soa_
 int xkrt_hip_dep_submit(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n, uint64_t *first_id)
 {
     if (!d || (!acc && n)) return -EINVAL;
-    if (d->poisoned) return -EINVAL;
-    if (first_id) *first_id = d->next_id;
-    if (n == 0) return 0;
-    if (n > 0xFFFFFFFFull || d->next_id + n > 0xFFFFFFFFull) return -ERANGE;
-    const uint64_t need = 2 * n;
-    if (need > d->cfg.max_rects) return -EINVAL;
-    if (need > d->rec_cap) {
-        for (int i = 0; i < 6; ++i) {
-            if (d->rec_dev[i]) (void)hipFree(d->rec_dev[i]);
-            if (d->rec_host[i]) (void)hipHostFree(d->rec_host[i]);
-            d->rec_dev[i] = d->rec_host[i] = nullptr;
-        }
-        d->rec_cap = 0;
-        for (int i = 0; i < 6; ++i) {
-            HIPCHK(hipMalloc(&d->rec_dev[i], 4 * need));
-            HIPCHK(hipHostMalloc(&d->rec_host[i], 4 * need, hipHostMallocDefault));
-        }
-        d->rec_cap = need;
+    xkrt_hip_pool *p = d->pool;
+    if (p->poisoned) return -EINVAL;
+    int rc = stage(p, d, acc, n, first_id);
+    if (rc == -ENOSPC) {  // the open launch is full: launch it, then stage again
+        rc = launch_open(p);
+        if (!rc) rc = stage(p, d, acc, n, first_id);
     }
-    // the staging buffers must be free: wait for the previous copy
-    if (d->staged_pending) {
-        HIPCHK(hipEventSynchronize(d->staged));
-        d->staged_pending = false;
-    }
-    uint64_t nrect = 0;
-    uint32_t ylo = 0, yhi = 0;
-    const uint64_t saved_max = d->max_xlen;
-    const bool was_framed = d->framed;
-    int rc = xkrt_hip_dep_build_records(d, acc, n, (uint32_t)d->next_id, d->rec_cap, d->rec_host[0], d->rec_host[1],
-                                        d->rec_host[2], d->rec_host[3], d->rec_host[4], d->rec_host[5], &nrect, &ylo,
-                                        &yhi);
-    if (rc) {
-        d->max_xlen = saved_max;
-        if (!was_framed) d->framed = false;
-        return rc;
-    }
-    rc = ensure_grid(d, ylo, yhi);
     if (rc) return rc;
-    for (int i = 0; i < 6; ++i)
-        HIPCHK(hipMemcpyAsync(d->rec_dev[i], d->rec_host[i], 4 * nrect, hipMemcpyHostToDevice, d->stream));
-    HIPCHK(hipEventRecord(d->staged, d->stream));
-    d->staged_pending = true;
-    xkrt_hip_rect_soa_t soa = {d->rec_dev[0], d->rec_dev[1], d->rec_dev[2], d->rec_dev[3], d->rec_dev[4], d->rec_dev[5]};
-    rc = enqueue(d, &soa, (uint32_t)nrect);
-    if (rc) { d->poisoned = true; return rc; }
-    d->next_id += n;
-    d->n_acc += n;
-    d->n_rect += nrect;
-    return 0;
-}
-
-static int read_counters(xkrt_hip_dep_domain_t d)
-{
-    HIPCHK(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(hipStreamSynchronize(d->stream));
-    harvest_events(d);
-    if (d->ctr_host->err) {
-        d->poisoned = true;
-        return err_to_errno(d->ctr_host->err);
-    }
-    return 0;
+    return launch_open(p);
 }
 
 int xkrt_hip_dep_sync(xkrt_hip_dep_domain_t d, uint64_t *nedges)
 {
     if (!d) return -EINVAL;
-    int rc = read_counters(d);
-    if (nedges) *nedges = d->ctr_host->edge_total;
-    return rc;
+    return xkrt_hip_pool_sync(d->pool, nedges);
 }
 
 int xkrt_hip_dep_edges_device(xkrt_hip_dep_domain_t d, const uint32_t **edges, uint64_t *n)
 {
     if (!d || !edges || !n) return -EINVAL;
-    int rc = read_counters(d);
-    *edges = d->edges;
-    *n = d->ctr_host->edge_total;
+    xkrt_hip_pool *p = d->pool;
+    int rc = read_counters(p);
+    *edges = p->edges;
+    *n = p->ctr_host->edge_total;
     return rc;
 }
 
 int xkrt_hip_dep_edges_consume(xkrt_hip_dep_domain_t d)
 {
     if (!d) return -EINVAL;
-    hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, d->stream, d->ctr);
+    hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, d->pool->stream, d->pool->ctr);
     return hip_err(hipGetLastError());
 }
 
 int xkrt_hip_dep_edges_copy(xkrt_hip_dep_domain_t d, uint32_t *host_edges, uint64_t cap, uint64_t *n)
 {
     if (!d || !n || (cap && !host_edges)) return -EINVAL;
-    int rc = read_counters(d);
-    const uint64_t ne = d->ctr_host->edge_total;
+    xkrt_hip_pool *p = d->pool;
+    if (!d->private_pool) return -EINVAL;  // a shared pool mixes domains: xkrt_hip_pool_edges_copy
+    int rc = read_counters(p);
+    const uint64_t ne = p->ctr_host->edge_total;
     *n = ne;
     if (rc) return rc;
     const uint64_t m = std::min(ne, cap);
     if (m) {
-        HIPCHK(hipMemcpyAsync(host_edges, d->edges, 8 * m, hipMemcpyDeviceToHost, d->stream));
-        HIPCHK(hipStreamSynchronize(d->stream));
+        HIPCHK(hipMemcpyAsync(host_edges, p->edges, 8 * m, hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
     }
     return xkrt_hip_dep_edges_consume(d);
 }
@@ -1520,37 +1954,39 @@ int xkrt_hip_dep_edges_copy(xkrt_hip_dep_domain_t d, uint32_t *host_edges, uint6
 int xkrt_hip_dep_stats(xkrt_hip_dep_domain_t d, xkrt_hip_dep_stats_t *out)
 {
     if (!d || !out) return -EINVAL;
-    int rc = read_counters(d);
-    const DevCounters &c = *d->ctr_host;
+    xkrt_hip_pool *p = d->pool;
+    (void)read_counters(p);  // the error bits are reported in out->err
+    const DevCounters &c = *p->ctr_host;
     out->accesses = d->n_acc;
     out->rects = d->n_rect;
     out->edges = c.edge_total;
     out->candidates = c.cand_total;
-    out->launches = d->n_launch;
+    out->launches = p->n_launch;
     out->bins = (uint64_t)d->nbx * d->nby;
     out->max_frags = c.maxfrag;
     out->max_reads = c.maxrs;
     out->frag_visits = c.frag_visits;
-    out->device_bytes = d->bytes_state + d->bytes_scratch;
+    out->device_bytes = (uint64_t)d->nbx * d->nby * bin_bytes(p) + (d->private_pool ? p->bytes_scratch : 0);
     out->err = c.err;
-    (void)rc;  // the error bits are reported in out->err
     return 0;
 }
 
-void *xkrt_hip_dep_stream(xkrt_hip_dep_domain_t d) { return d ? (void *)d->stream : nullptr; }
+void *xkrt_hip_dep_stream(xkrt_hip_dep_domain_t d) { return d ? (void *)d->pool->stream : nullptr; }
 
 int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d)
 {
     if (!d) return -EINVAL;
-    const uint32_t nbins = d->nbx * d->nby;
-    if (nbins) {
-        hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>((nbins + 255) / 256, 4096u)), dim3(256), 0, d->stream,
-                           d->bin_nf, d->bin_nrs, d->bin_cnt, nbins, d->ctr);
-    } else {
-        hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, d->stream, d->ctr);
+    xkrt_hip_pool *p = d->pool;
+    const uint32_t nb = d->nbx * d->nby;
+    DevCounters *c = d->private_pool ? p->ctr : nullptr;
+    if (nb) {
+        hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>((nb + 255) / 256, 4096u)), dim3(256), 0, p->stream,
+                           p->bin_nf, p->bin_nrs, p->bin_cnt, d->bin0, nb, c);
+    } else if (c) {
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, p->stream, p->bin_nf, p->bin_nrs, p->bin_cnt, 0u, 0u, c);
     }
     HIPCHK(hipGetLastError());
-    d->poisoned = false;
+    if (d->private_pool) p->poisoned = false;
     d->next_id = 0;
     d->max_xlen = 0;
     return 0;
@@ -1568,33 +2004,28 @@ int xkrt_hip_copy(void *dst, const void *src, uint64_t bytes, void *stream)
 int xkrt_hip_dep_timing(xkrt_hip_dep_domain_t d, int enable)
 {
     if (!d) return -EINVAL;
-    d->timing = enable != 0;
-    return 0;
+    return xkrt_hip_pool_timing(d->pool, enable);
 }
 
 int xkrt_hip_dep_kernel_times(xkrt_hip_dep_domain_t d, double *ms, uint64_t *launches, int clear)
 {
     if (!d || !ms || !launches) return -EINVAL;
-    HIPCHK(hipStreamSynchronize(d->stream));
-    harvest_events(d);
-    for (int k = 0; k < 6; ++k) ms[k] = d->kernel_ms[k];
-    *launches = d->timed_launches;
+    xkrt_hip_pool *p = d->pool;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    harvest_events(p);
+    for (int k = 0; k < 6; ++k) ms[k] = p->kernel_ms[k];
+    *launches = p->timed_launches;
     if (clear) {
-        for (int k = 0; k < 6; ++k) d->kernel_ms[k] = 0;
-        d->timed_launches = 0;
+        for (int k = 0; k < 6; ++k) p->kernel_ms[k] = 0;
+        p->timed_launches = 0;
     }
     return 0;
 }
 
 int xkrt_hip_dep_launch_times(xkrt_hip_dep_domain_t d, float *ms, uint64_t cap, uint64_t *n, int clear)
 {
-    if (!d || !n || (cap && !ms)) return -EINVAL;
-    HIPCHK(hipStreamSynchronize(d->stream));
-    harvest_events(d);
-    *n = d->launch_ms.size();
-    for (uint64_t i = 0; i < std::min<uint64_t>(cap, d->launch_ms.size()); ++i) ms[i] = d->launch_ms[i];
-    if (clear) d->launch_ms.clear();
-    return 0;
+    if (!d) return -EINVAL;
+    return xkrt_hip_pool_launch_times(d->pool, ms, cap, n, clear);
 }
 
 } // extern "C"

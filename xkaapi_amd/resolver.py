@@ -43,8 +43,18 @@ class StatsT(C.Structure):
                                           "max_frags", "max_reads", "frag_visits", "device_bytes", "err")]
 
 
-class RectSoaT(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in ("x0", "x1", "y0", "y1", "acc", "tm")]
+RECT_DTYPE = np.dtype([(n, "<u4") for n in ("x0", "x1", "y0", "y1", "acc", "tm", "dom", "pad")])
+assert RECT_DTYPE.itemsize == 32
+
+
+class PoolConfigT(C.Structure):
+    _fields_ = [("kernel", C.c_uint32), ("graph", C.c_uint32), ("max_rects", C.c_uint64),
+                ("max_edges", C.c_uint64), ("budget_bytes", C.c_uint64)]
+
+
+class PoolStatsT(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("domains", "bins_capacity", "bins_used", "bytes_per_bin", "device_bytes",
+                                          "launches", "graph", "poisoned")]
 
 
 ERR_BITS = {1: "incidence array", 2: "bin record list", 4: "bin fragments", 8: "bin reader log",
@@ -82,9 +92,23 @@ def hip_lib():
     L.xkrt_hip_dep_domain_create.argtypes = [_u64, _u64, C.POINTER(ConfigT), _p, C.POINTER(_p)]
     L.xkrt_hip_dep_domain_destroy.argtypes = [_p]
     L.xkrt_hip_dep_submit.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
-    L.xkrt_hip_dep_submit_device.argtypes = [_p, C.POINTER(RectSoaT), _u64, _u64, _u32, _u32]
-    L.xkrt_hip_dep_build_records.argtypes = [_p, _p, _u64, _u32, _u64, _p, _p, _p, _p, _p, _p,
-                                             C.POINTER(_u64), C.POINTER(_u32), C.POINTER(_u32)]
+    L.xkrt_hip_dep_submit_device.argtypes = [_p, _p, _u64, _u64, _u32, _u32]
+    L.xkrt_hip_dep_build_records.argtypes = [_p, _p, _u64, _u32, _u64, _p, C.POINTER(_u64), C.POINTER(_u32),
+                                             C.POINTER(_u32)]
+    L.xkrt_hip_pool_create.argtypes = [C.POINTER(PoolConfigT), _p, C.POINTER(_p)]
+    L.xkrt_hip_pool_destroy.argtypes = [_p]
+    L.xkrt_hip_pool_domain_create.argtypes = [_p, _u64, _u64, _u32, _u32, _u64, _u64, C.POINTER(_p)]
+    L.xkrt_hip_pool_stage.argtypes = [_p, _p, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_pool_launch.argtypes = [_p]
+    L.xkrt_hip_pool_open.argtypes = [_p, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]
+    L.xkrt_hip_pool_sync.argtypes = [_p, C.POINTER(_u64)]
+    L.xkrt_hip_pool_edges_copy.argtypes = [_p, _p, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_pool_stats.argtypes = [_p, C.POINTER(PoolStatsT)]
+    L.xkrt_hip_pool_timing.argtypes = [_p, C.c_int]
+    L.xkrt_hip_pool_launch_times.argtypes = [_p, _p, _u64, C.POINTER(_u64), C.c_int]
+    L.xkrt_hip_pool_stream.argtypes = [_p]
+    L.xkrt_hip_pool_stream.restype = _p
+    L.xkrt_hip_dep_index.argtypes = [_p, C.POINTER(_u32)]
     L.xkrt_hip_dep_frame.argtypes = [_p, C.POINTER(_u64)]
     L.xkrt_hip_dep_sync.argtypes = [_p, C.POINTER(_u64)]
     L.xkrt_hip_dep_edges_device.argtypes = [_p, C.POINTER(_p), C.POINTER(_u64)]
@@ -168,22 +192,25 @@ class DependencyDomain:
         return int(first.value)
 
     def build_records(self, acc: np.ndarray, first_id: int):
-        """Host conversion to device-frame rect records: (x0,x1,y0,y1,acc,tm), ylo, yhi."""
+        """Host conversion to device-frame rect records (RECT_DTYPE), ylo, yhi."""
         acc = np.ascontiguousarray(acc, ACCESS_DTYPE)
         cap = 2 * len(acc)
-        out = [np.empty(cap, np.uint32) for _ in range(6)]
+        out = np.zeros(max(cap, 1), RECT_DTYPE)
         nrect, ylo, yhi = _u64(), _u32(), _u32()
-        _chk(self.L.xkrt_hip_dep_build_records(self.h, acc.ctypes.data, len(acc), first_id, cap,
-                                               *[o.ctypes.data for o in out], C.byref(nrect), C.byref(ylo),
-                                               C.byref(yhi)), "xkrt_hip_dep_build_records")
-        k = int(nrect.value)
-        return [o[:k] for o in out], int(ylo.value), int(yhi.value)
+        _chk(self.L.xkrt_hip_dep_build_records(self.h, acc.ctypes.data, len(acc), first_id, cap, out.ctypes.data,
+                                               C.byref(nrect), C.byref(ylo), C.byref(yhi)),
+             "xkrt_hip_dep_build_records")
+        return out[:int(nrect.value)], int(ylo.value), int(yhi.value)
 
-    def submit_device(self, ptrs, nrect: int, naccess: int, ylo: int, yhi: int):
-        """Enqueue device-resident rect records (six device pointers)."""
-        soa = RectSoaT(*[int(p) for p in ptrs])
-        _chk(self.L.xkrt_hip_dep_submit_device(self.h, C.byref(soa), nrect, naccess, ylo, yhi),
-             "xkrt_hip_dep_submit_device")
+    def submit_device(self, ptr: int, nrect: int, naccess: int, ylo: int, yhi: int):
+        """Enqueue device-resident rect records (a device pointer to RECT_DTYPE records)."""
+        _chk(self.L.xkrt_hip_dep_submit_device(self.h, ptr, nrect, naccess, ylo, yhi), "xkrt_hip_dep_submit_device")
+
+    @property
+    def index(self) -> int:
+        i = _u32()
+        _chk(self.L.xkrt_hip_dep_index(self.h, C.byref(i)), "xkrt_hip_dep_index")
+        return int(i.value)
 
     def sync(self) -> int:
         n = _u64()
@@ -239,6 +266,119 @@ class DependencyDomain:
         st = StatsT()
         _chk(self.L.xkrt_hip_dep_stats(self.h, C.byref(st)), "xkrt_hip_dep_stats")
         return {f: int(getattr(st, f)) for f, _ in StatsT._fields_}
+
+
+class PoolDomain(DependencyDomain):
+    """A domain admitted into a Pool (its bins live in the pool's arrays)."""
+
+    def __init__(self, pool: "Pool", ld: int, sizeof_type: int, y_range: tuple[int, int], bin_x_bytes: int = 1024,
+                 bin_y_cols: int = 128):
+        self.L = hip_lib()
+        h = _p()
+        _chk(self.L.xkrt_hip_pool_domain_create(pool.h, ld, sizeof_type, bin_x_bytes, bin_y_cols, int(y_range[0]),
+                                                int(y_range[1]), C.byref(h)), "xkrt_hip_pool_domain_create")
+        self.h = h
+        self.pool = pool
+        self.ld, self.s = ld, sizeof_type
+
+    def close(self):
+        # a destroyed pool has already released its domains
+        if getattr(self, "h", None) and getattr(self.pool, "h", None):
+            self.L.xkrt_hip_dep_domain_destroy(self.h)
+        self.h = None
+
+
+class Pool:
+    """Many dependency domains resident on the device, resolved together: batches
+    of any domains are staged and resolved by one launch (optionally a replayed
+    hipGraph); the device-memory budget decides which domains are admitted."""
+
+    def __init__(self, kernel: int = 1, graph: bool = False, max_rects: int = 1 << 20, max_edges: int = 0,
+                 budget_bytes: int = 0, stream: int | None = None, device: int = 0):
+        self.L = hip_lib()
+        _chk(self.L.xkrt_hip_init(device), "xkrt_hip_init")
+        cfg = PoolConfigT(kernel, 1 if graph else 0, max_rects, max_edges, budget_bytes)
+        h = _p()
+        _chk(self.L.xkrt_hip_pool_create(C.byref(cfg), stream, C.byref(h)), "xkrt_hip_pool_create")
+        self.h = h
+        self.max_rects = max_rects
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.xkrt_hip_pool_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def domain(self, ld: int, sizeof_type: int, y_range, bin_x_bytes: int = 1024, bin_y_cols: int = 128):
+        """Admit a domain; raises ResolverError(-ENOMEM) when the budget cannot hold it now."""
+        return PoolDomain(self, ld, sizeof_type, y_range, bin_x_bytes, bin_y_cols)
+
+    def leave(self, dom: PoolDomain):
+        """The domain's client is done: its bins return to the pool."""
+        dom.close()
+
+    def stage(self, dom: PoolDomain, acc: np.ndarray) -> int:
+        acc = np.ascontiguousarray(acc, ACCESS_DTYPE)
+        first = _u64()
+        _chk(self.L.xkrt_hip_pool_stage(self.h, dom.h, acc.ctypes.data, len(acc), C.byref(first)),
+             "xkrt_hip_pool_stage")
+        return int(first.value)
+
+    def launch(self):
+        _chk(self.L.xkrt_hip_pool_launch(self.h), "xkrt_hip_pool_launch")
+
+    def open(self):
+        r, a, b = _u64(), _u64(), _u64()
+        _chk(self.L.xkrt_hip_pool_open(self.h, C.byref(r), C.byref(a), C.byref(b)), "xkrt_hip_pool_open")
+        return int(r.value), int(a.value), int(b.value)
+
+    def sync(self) -> int:
+        n = _u64()
+        rc = self.L.xkrt_hip_pool_sync(self.h, C.byref(n))
+        if rc != 0:
+            raise ResolverError(rc, "xkrt_hip_pool_sync")
+        return int(n.value)
+
+    def edges(self):
+        """Copy and consume every unconsumed edge: ((E, 2) pred/succ ids, (E,) domain index)."""
+        n = self.sync()
+        e = np.empty((max(n, 1), 2), np.uint32)
+        d = np.empty(max(n, 1), np.uint32)
+        got = _u64()
+        _chk(self.L.xkrt_hip_pool_edges_copy(self.h, e.ctypes.data, d.ctypes.data, n, C.byref(got)),
+             "xkrt_hip_pool_edges_copy")
+        return e[:n], d[:n]
+
+    def stats(self) -> dict:
+        st = PoolStatsT()
+        _chk(self.L.xkrt_hip_pool_stats(self.h, C.byref(st)), "xkrt_hip_pool_stats")
+        return {f: int(getattr(st, f)) for f, _ in PoolStatsT._fields_}
+
+    def set_timing(self, on: bool = True):
+        _chk(self.L.xkrt_hip_pool_timing(self.h, 1 if on else 0), "xkrt_hip_pool_timing")
+
+    def launch_times(self, clear: bool = True) -> np.ndarray:
+        n = _u64()
+        _chk(self.L.xkrt_hip_pool_launch_times(self.h, None, 0, C.byref(n), 0), "xkrt_hip_pool_launch_times")
+        out = np.zeros(max(int(n.value), 1), np.float32)
+        _chk(self.L.xkrt_hip_pool_launch_times(self.h, out.ctypes.data, len(out), C.byref(n), 1 if clear else 0),
+             "xkrt_hip_pool_launch_times")
+        return out[:int(n.value)]
+
+    @property
+    def stream(self) -> int:
+        return int(self.L.xkrt_hip_pool_stream(self.h) or 0)
 
 
 def batches_by_tasks(task: np.ndarray, tasks_per_batch) -> list[tuple[int, int]]:
