@@ -149,6 +149,94 @@ def coherency_cpu_baseline(subs):
             "sample": f"the same {n} accesses, oracle/khp_oracle.c (restated memory-tree.hpp), 1 thread"}
 
 
+def heat_leg(torch, steps=200, reps=2):
+    """Heat-diffusion halo stream, 16384^2 fp32 in 128^2 tiles (16,384 tiles),
+    `steps` timesteps; uneven batches (geometric, mean 2048 tasks, max 65536).
+    Bins and capacity class from the autotuner on a prefix.  Records resident
+    in HBM; plain = one launch per batch, batched = consecutive batches merged
+    into launches of up to 2^20 records."""
+    from xkaapi_amd import autotune as A
+    from xkaapi_amd import resolver as R
+    from xkaapi_amd import streams as S
+    st = S.heat(16384, 128, 4, steps)
+    sizes = S.geometric_batches(st.ntasks, 2048.0, 65536, 0x3B)
+    batches = R.batches_by_tasks(st.task, sizes)
+    t0 = time.time()
+    tune = A.tune(st.take(slice(0, 300_000)), tasks_per_batch=2048, max_cands=4)
+    log(f"heat: {len(st)} acc / {st.ntasks} tasks / {len(batches)} batches; autotune {tune.bin_x_bytes}x"
+        f"{tune.bin_y_cols} class {tune.kernel} ({time.time()-t0:.1f}s) tried {tune.tried}")
+    acc = R.to_access_array(st.task, st.mode, st.addr, st.m, st.n)
+    cap = 1 << 20
+    groups, cur = [], None
+    for a, b in batches:  # merged launches of up to cap records (2 per access at most)
+        if cur is not None and 2 * (b - cur[0]) <= cap:
+            cur = (cur[0], b)
+        else:
+            if cur is not None:
+                groups.append(cur)
+            cur = (a, b)
+    groups.append(cur)
+    out = {}
+    for form, spans in (("plain", batches), ("batched", groups)):
+        maxrec = 2 * max(b - a for a, b in spans)
+        dom = R.DependencyDomain(int(st.ld[0]), int(st.s[0]), bin_x_bytes=tune.bin_x_bytes,
+                                 bin_y_cols=tune.bin_y_cols, kernel=tune.kernel, max_rects=maxrec,
+                                 max_edges=4 * len(st) + (1 << 20), stream=torch.cuda.current_stream().cuda_stream)
+        dev = []
+        for a, b in spans:
+            recs, ylo, yhi = dom.build_records(acc[a:b], a)
+            dev.append((torch.from_numpy(recs.view(np.int32).reshape(-1)).cuda(), len(recs), b - a, ylo, yhi))
+        torch.cuda.synchronize()
+        best, edges = None, 0
+        for _ in range(reps):
+            dom.reset()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for d, nrec, nacc, ylo, yhi in dev:
+                dom.submit_device(d.data_ptr(), nrec, nacc, ylo, yhi)
+            edges = dom.sync()
+            dt = time.perf_counter() - t1
+            best = dt if best is None else min(best, dt)
+        out[form] = {"launches": len(spans), "s": round(best, 4), "accesses_per_s": round(len(st) / best, 1),
+                     "edges": int(edges)}
+        dom.close()
+        del dev
+        torch.cuda.empty_cache()
+    out["speedup_batched_vs_plain"] = round(out["batched"]["accesses_per_s"] / out["plain"]["accesses_per_s"], 3)
+    out["workload"] = (f"heat 16384^2 fp32, 128^2 tiles, {steps} steps: {len(st)} accesses, {st.ntasks} tasks, "
+                       f"{len(batches)} geometric batches (mean 2048, max 65536)")
+    out["autotune"] = {"bin_x_bytes": tune.bin_x_bytes, "bin_y_cols": tune.bin_y_cols, "kernel": tune.kernel}
+    return out
+
+
+def serving_leg(fracs=(0.1, 0.3, 0.5, 0.7, 0.9)):
+    """64 clients (Cholesky / LU 50/50, n ~ U[4, 48] tiles of 128^2 fp64), one
+    pool with a 16 GB budget.  Saturation (closed loop) of the batched form
+    (continuous batching, hipGraph launches) and of the plain form (one launch
+    per batch); then open-loop Poisson arrivals at fractions of the batched
+    saturation rate, both forms on the same schedule.  Latency = batch arrival
+    -> its launch completing, task-weighted; host staging included."""
+    from xkaapi_amd import serving as V
+    cs = V.make_clients(64, 4, 48, 128, 256)
+    nb = sum(len(c.batches) for c in cs)
+    sat = {m: V.saturation(cs, m) for m in ("batched", "plain")}
+    rate = nb / sat["batched"].makespan_s
+    out = {"clients": len(cs), "accesses": sat["batched"].accesses, "batches": nb, "batch_tasks": 256,
+           "saturation": {m: {"accesses_per_s": round(r.rate, 1), "launches": r.launches,
+                              "makespan_ms": round(r.makespan_s * 1e3, 2)} for m, r in sat.items()},
+           "sweep": []}
+    for f in fracs:
+        sched = V.poisson_schedule(cs, f * rate, seed=11)
+        row = {"load": f, "arrival_batches_per_s": round(f * rate, 1)}
+        for m in ("batched", "plain"):
+            r = V.serve(cs, sched, m)
+            row[m] = {"p50_us": round(r.pct(50), 1), "p99_us": round(r.pct(99), 1),
+                      "accesses_per_s": round(r.rate, 1), "launches": r.launches}
+        out["sweep"].append(row)
+        log(f"serving load {f}: {row}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +249,9 @@ def main():
                     help="restrict to one stream (profiling runs)")
     ap.add_argument("--forms", choices=("both", "plain", "batched"), default="both")
     ap.add_argument("--no-coherency", action="store_true")
+    ap.add_argument("--no-heat", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--heat-steps", type=int, default=200)
     ap.add_argument("--calibrate-mib", type=int, default=0,
                     help="also run one streaming copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -273,6 +364,12 @@ def main():
         if not args.no_cpu:
             coh["cpu_baseline"] = coherency_cpu_baseline(coh_subs)
         log(f"coherency: {coh}")
+    heat = None if args.no_heat else heat_leg(torch, args.heat_steps)
+    if heat:
+        log(f"heat: {heat}")
+    serving = None if args.no_serving else serving_leg()
+    if serving:
+        log(f"serving: {serving}")
     out = {
         "metric": "resolved task accesses/sec + p99 submit-to-ready µs, batched vs plain GPU",
         "value": round(res["plain"]["rate"], 1),
@@ -298,6 +395,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "coherency": coh,
+        "heat": heat,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -221,8 +221,13 @@ int xkrt_hip_pool_domain_create(xkrt_hip_pool_t p, uint64_t ld, uint64_t sizeof_
  * launch; ids as xkrt_hip_dep_submit.  -ENOSPC: the open launch is full. */
 int xkrt_hip_pool_stage(xkrt_hip_pool_t p, xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n,
                         uint64_t *first_id);
-/* Resolves every staged batch in one launch (asynchronous). */
-int xkrt_hip_pool_launch(xkrt_hip_pool_t p);
+/* Resolves every staged batch in one launch (asynchronous).  *ticket (may
+ * be NULL) = the launch's sequence number, 1 for the pool's first launch. */
+int xkrt_hip_pool_launch(xkrt_hip_pool_t p, uint64_t *ticket);
+/* *done = every launch up to this ticket has completed (non-blocking). */
+int xkrt_hip_pool_poll(xkrt_hip_pool_t p, uint64_t *done);
+/* Blocks until the launch with this ticket has completed. */
+int xkrt_hip_pool_wait(xkrt_hip_pool_t p, uint64_t ticket);
 /* Staged but not launched: records, accesses, batches. */
 int xkrt_hip_pool_open(xkrt_hip_pool_t p, uint64_t *nrect, uint64_t *naccess, uint64_t *nbatches);
 int xkrt_hip_pool_sync(xkrt_hip_pool_t p, uint64_t *nedges);

@@ -43,6 +43,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <deque>
 #include <vector>
 
 #include "../../include/xkrt_hip.h"
@@ -1075,6 +1076,10 @@ struct xkrt_hip_pool {
     bool staged_pending = false;
     uint32_t open_nrec = 0, open_tasks = 0;
     uint64_t open_acc = 0, open_batches = 0;
+    // completion tickets: one event per launch, oldest first
+    std::deque<std::pair<uint64_t, hipEvent_t>> tickets;
+    std::vector<hipEvent_t> ticket_pool;
+    uint64_t done = 0;
     // captured launch
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1420,6 +1425,35 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
     }
     HIPCHK(hipGetLastError());
     p->n_launch++;
+    // completion ticket of this launch
+    hipEvent_t t = nullptr;
+    if (!p->ticket_pool.empty()) {
+        t = p->ticket_pool.back();
+        p->ticket_pool.pop_back();
+    } else {
+        HIPCHK(hipEventCreateWithFlags(&t, hipEventDisableTiming));
+    }
+    HIPCHK(hipEventRecord(t, p->stream));
+    p->tickets.push_back({p->n_launch, t});
+    return 0;
+}
+
+// retire the tickets of completed launches (non-blocking unless `until`)
+int retire(xkrt_hip_pool *p, uint64_t until)
+{
+    while (!p->tickets.empty()) {
+        auto &f = p->tickets.front();
+        if (f.first <= until) {
+            HIPCHK(hipEventSynchronize(f.second));
+        } else {
+            const hipError_t q = hipEventQuery(f.second);
+            if (q == hipErrorNotReady) break;
+            if (q != hipSuccess) return hip_err(q);
+        }
+        p->done = f.first;
+        p->ticket_pool.push_back(f.second);
+        p->tickets.pop_front();
+    }
     return 0;
 }
 
@@ -1446,6 +1480,8 @@ void destroy_pool(xkrt_hip_pool *p)
 {
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     drop_graph(p);
+    for (auto &t : p->tickets) (void)hipEventDestroy(t.second);
+    for (hipEvent_t e : p->ticket_pool) (void)hipEventDestroy(e);
     for (hipEvent_t e : p->ev_pending) (void)hipEventDestroy(e);
     for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
     void *dev[] = {p->bin_nf, p->bin_nrs, p->frag, p->rs, p->bin_cnt, p->bin_cur, p->bin_dom, p->active, p->d_dom,
@@ -1730,11 +1766,27 @@ int xkrt_hip_pool_stage(xkrt_hip_pool_t p, xkrt_hip_dep_domain_t d, const xkrt_h
     return stage(p, d, acc, n, first_id);
 }
 
-int xkrt_hip_pool_launch(xkrt_hip_pool_t p)
+int xkrt_hip_pool_launch(xkrt_hip_pool_t p, uint64_t *ticket)
 {
     if (!p) return -EINVAL;
     if (p->poisoned) return -EINVAL;
-    return launch_open(p);
+    const int rc = launch_open(p);
+    if (ticket) *ticket = p->n_launch;
+    return rc;
+}
+
+int xkrt_hip_pool_poll(xkrt_hip_pool_t p, uint64_t *done)
+{
+    if (!p || !done) return -EINVAL;
+    const int rc = retire(p, 0);
+    *done = p->done;
+    return rc;
+}
+
+int xkrt_hip_pool_wait(xkrt_hip_pool_t p, uint64_t ticket)
+{
+    if (!p) return -EINVAL;
+    return retire(p, ticket);
 }
 
 int xkrt_hip_pool_open(xkrt_hip_pool_t p, uint64_t *nrect, uint64_t *naccess, uint64_t *nbatches)

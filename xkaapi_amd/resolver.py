@@ -99,7 +99,9 @@ def hip_lib():
     L.xkrt_hip_pool_destroy.argtypes = [_p]
     L.xkrt_hip_pool_domain_create.argtypes = [_p, _u64, _u64, _u32, _u32, _u64, _u64, C.POINTER(_p)]
     L.xkrt_hip_pool_stage.argtypes = [_p, _p, _p, _u64, C.POINTER(_u64)]
-    L.xkrt_hip_pool_launch.argtypes = [_p]
+    L.xkrt_hip_pool_launch.argtypes = [_p, C.POINTER(_u64)]
+    L.xkrt_hip_pool_poll.argtypes = [_p, C.POINTER(_u64)]
+    L.xkrt_hip_pool_wait.argtypes = [_p, _u64]
     L.xkrt_hip_pool_open.argtypes = [_p, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]
     L.xkrt_hip_pool_sync.argtypes = [_p, C.POINTER(_u64)]
     L.xkrt_hip_pool_edges_copy.argtypes = [_p, _p, _p, _u64, C.POINTER(_u64)]
@@ -335,8 +337,20 @@ class Pool:
              "xkrt_hip_pool_stage")
         return int(first.value)
 
-    def launch(self):
-        _chk(self.L.xkrt_hip_pool_launch(self.h), "xkrt_hip_pool_launch")
+    def launch(self) -> int:
+        """Resolve every staged batch in one launch; returns its completion ticket."""
+        t = _u64()
+        _chk(self.L.xkrt_hip_pool_launch(self.h, C.byref(t)), "xkrt_hip_pool_launch")
+        return int(t.value)
+
+    def poll(self) -> int:
+        """Highest ticket whose launch (and every earlier one) has completed."""
+        d = _u64()
+        _chk(self.L.xkrt_hip_pool_poll(self.h, C.byref(d)), "xkrt_hip_pool_poll")
+        return int(d.value)
+
+    def wait(self, ticket: int):
+        _chk(self.L.xkrt_hip_pool_wait(self.h, ticket), "xkrt_hip_pool_wait")
 
     def open(self):
         r, a, b = _u64(), _u64(), _u64()
