@@ -35,6 +35,7 @@
 // the (pred, task) dedup is the reference's edge set.
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <errno.h>
 #include <stdint.h>
@@ -44,11 +45,14 @@
 
 #include <algorithm>
 #include <deque>
+#include <tuple>
 #include <vector>
 
 #include "../../include/xkrt_hip.h"
 
 #define NONE 0xFFFFFFFFu
+#define CSTRIPES 16u   // candidate buffer stripes, each with its own counter (64 B apart)
+#define BLKSLOTS 4096u // per-block statistics slots per tier (4 tiers)
 #define EMPTY64 0xFFFFFFFFFFFFFFFFull
 
 enum : uint32_t {
@@ -64,8 +68,8 @@ enum : uint32_t {
 struct DevCounters {
     unsigned int nrec;       // rect records of this launch (set before the launch)
     unsigned int total;      // incidences allocated this launch
-    unsigned int ntier[3];   // active bins per LDS tier this launch (heavy ones, from the list front)
-    unsigned int nlight[3];  // light active bins per tier (from the list back)
+    unsigned int ntier[4];   // active bins per LDS tier this launch (heavy ones, from the list front)
+    unsigned int nlight[4];  // light active bins per tier (from the list back)
     unsigned int ncand;      // candidates this launch
     unsigned int err;        // sticky ERR_* bits
     unsigned int maxfrag;    // stats: max fragments in a bin
@@ -88,6 +92,7 @@ struct DomDesc {
 
 struct KArgs {
     const uint4 *rec;         // 2 x uint4 per rect record: {x0, x1, y0, y1}, {acc, tm, domain, 0}
+    uint32_t nrec_arg;        // records of this launch, or NONE: already in the counters
     const DomDesc *dom;       // domain table
     const uint32_t *bin_dom;  // domain of each pool bin
     uint32_t *bin_nf, *bin_nrs;
@@ -100,9 +105,11 @@ struct KArgs {
     uint32_t binrec_cap;
     uint32_t nbins;
     uint32_t max_tier;                              // tiers 0..max_tier; storage sized for max_tier
-    uint32_t tier_f[3], tier_g[3], tier_l[3];       // per tier: fragments, log entries, records
+    uint32_t tier_f[4], tier_g[4], tier_l[4];       // per tier: fragments, log entries, records
     uint32_t *cand;
-    uint32_t cand_cap;
+    uint32_t cand_cap;        // per stripe
+    uint32_t *cstr;           // CSTRIPES counters, 16 u32 apart
+    uint4 *blk;               // per (tier, block): max fragments, max log entries, fragment visits (lo, hi)
     unsigned long long *hkeys;
     uint32_t *hvals;
     uint32_t hmask;
@@ -159,15 +166,17 @@ __global__ void k_count(KArgs a)
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
         a.ctr->total = 0;
-        for (int t = 0; t < 3; ++t) {
+        for (int t = 0; t < 4; ++t) {
             a.ctr->ntier[t] = 0;
             a.ctr->nlight[t] = 0;
         }
         a.ctr->ncand = 0;
         a.ctr->edge_base = a.ctr->edge_total;
+        for (uint32_t q = 0; q < CSTRIPES; ++q) a.cstr[16 * q] = 0;
     }
+    if (tid == 0 && a.nrec_arg != NONE) a.ctr->nrec = a.nrec_arg;  // for the later kernels
     if (a.ctr->err) return;  // a poisoned pool touches no state
-    const uint32_t nrec = a.ctr->nrec;
+    const uint32_t nrec = a.nrec_arg != NONE ? a.nrec_arg : a.ctr->nrec;
     for (uint32_t r = tid; r < nrec; r += gridDim.x * blockDim.x) {
         uint4 q0, q1;
         uint32_t bx0, bx1, by0, by1;
@@ -182,37 +191,41 @@ __global__ void k_count(KArgs a)
 __global__ void k_alloc(KArgs a, uint32_t nbins)
 {
     if (a.ctr->err) return;
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t c = slot < nbins ? a.bin_cnt[slot] : 0u;
-    const uint32_t incl = wave_incl_scan(c);
-    const uint32_t sum = wave_bcast(incl, 63);
-    if (__ballot(c > 0) == 0) return;
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(&a.ctr->total, sum);
-    base = wave_bcast(base, 0);
-    if (lane_id() == 0 && (uint64_t)base + sum > a.binrec_cap) atomicOr(&a.ctr->err, ERR_BINREC);
-    uint32_t tier = 0, nfl = 0, ngl = 0;
-    if (c > 0) {
-        a.bin_cur[slot] = base + incl - c;
-        nfl = a.bin_nf[slot];
-        ngl = a.bin_nrs[slot];
-        while (tier < a.max_tier &&
-               (c > a.tier_l[tier] || 2 * nfl > a.tier_f[tier] || 2 * ngl > a.tier_g[tier]))
-            ++tier;
-    }
-    // heavy bins (long record lists) at the front of the tier list, so they start first
-    for (uint32_t t = 0; t <= a.max_tier; ++t) {
-        for (int heavy = 1; heavy >= 0; --heavy) {
-            const bool mine = c > 0 && tier == t && ((c >= 16) == (heavy != 0));
-            const uint64_t m = __ballot(mine);
-            if (m == 0) continue;
-            uint32_t tb = 0;
-            if (lane_id() == 0) tb = atomicAdd(heavy ? &a.ctr->ntier[t] : &a.ctr->nlight[t], (uint32_t)__popcll(m));
-            tb = wave_bcast(tb, 0);
-            const uint32_t r = tb + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
-            if (mine)
-                a.active[(size_t)t * a.nbins + (heavy ? r : a.nbins - 1 - r)] =
-                    make_uint4(slot, base + incl - c, c, nfl | (ngl << 11) | (a.bin_dom[slot] << 24));
+    // grid-stride over 64-bin chunks, one wave per chunk
+    for (uint32_t s0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); s0 < nbins; s0 += gridDim.x * blockDim.x) {
+        const uint32_t slot = s0 + lane_id();
+        const uint32_t c = slot < nbins ? a.bin_cnt[slot] : 0u;
+        const uint32_t incl = wave_incl_scan(c);
+        const uint32_t sum = wave_bcast(incl, 63);
+        if (__ballot(c > 0) == 0) continue;
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(&a.ctr->total, sum);
+        base = wave_bcast(base, 0);
+        if (lane_id() == 0 && (uint64_t)base + sum > a.binrec_cap) atomicOr(&a.ctr->err, ERR_BINREC);
+        uint32_t tier = 0, nfl = 0, ngl = 0;
+        if (c > 0) {
+            a.bin_cur[slot] = base + incl - c;
+            nfl = a.bin_nf[slot];
+            ngl = a.bin_nrs[slot];
+            while (tier < a.max_tier &&
+                   (c > a.tier_l[tier] || 2 * nfl > a.tier_f[tier] || 2 * ngl > a.tier_g[tier]))
+                ++tier;
+        }
+        // heavy bins (long record lists) at the front of the tier list, so they start first
+        for (uint32_t t = 0; t <= a.max_tier; ++t) {
+            for (int heavy = 1; heavy >= 0; --heavy) {
+                const bool mine = c > 0 && tier == t && ((c >= 16) == (heavy != 0));
+                const uint64_t m = __ballot(mine);
+                if (m == 0) continue;
+                uint32_t tb = 0;
+                if (lane_id() == 0)
+                    tb = atomicAdd(heavy ? &a.ctr->ntier[t] : &a.ctr->nlight[t], (uint32_t)__popcll(m));
+                tb = wave_bcast(tb, 0);
+                const uint32_t r = tb + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
+                if (mine)
+                    a.active[(size_t)t * a.nbins + (heavy ? r : a.nbins - 1 - r)] =
+                        make_uint4(slot, base + incl - c, c, nfl | (ngl << 11) | (a.bin_dom[slot] << 24));
+            }
         }
     }
 }
@@ -246,18 +259,15 @@ __global__ void k_fill(KArgs a)
 // grow in spawn order, so a logged reader r is among the readers-since-the-
 // last-writer of point x iff acc(r) > lw(x) and x is in r.
 
-#define CCAP 256u  // candidate staging (LDS), 3 u32 each
-#define KCAP 128u  // readers of one fragment kept for the coverage test
-
-// per-bin (pred, task) dedup set size: scales with the tier's record capacity
-template <int LCAP>
-struct Dsz {
-    static constexpr uint32_t value = LCAP >= 1024 ? 1024u : (LCAP >= 512 ? 512u : 256u);
-};
-
+// Per-tier LDS sizes besides (fragments, log entries, records): the (pred,
+// task) dedup set, the candidate staging buffer (3 u32 each) and the readers
+// of one fragment kept for the exact coverage test; the small tier keeps
+// them small so that more bins are in flight per CU.
 template <int FCAP, int GCAP, int LCAP>
 struct BinLds {
-    static constexpr uint32_t DSZ = Dsz<LCAP>::value;
+    static constexpr uint32_t DSZ = LCAP >= 1024 ? 1024u : (LCAP >= 512 ? 512u : (LCAP >= 256 ? 256u : 64u));
+    static constexpr uint32_t CCAP = LCAP >= 256 ? 256u : 64u;
+    static constexpr uint32_t KCAP = LCAP >= 256 ? 128u : 64u;
     uint32_t fx0[2][FCAP], fx1[2][FCAP], fy0[2][FCAP], fy1[2][FCAP], flw[2][FCAP];
     uint32_t gx0[GCAP], gx1[GCAP], gy0[GCAP], gy1[GCAP], gacc[GCAP];
     unsigned long long L[LCAP];  // sort keys: record index << 32 | staging position
@@ -269,7 +279,9 @@ struct BinLds {
     uint32_t C[3 * CCAP];
     uint32_t kx0[KCAP], kx1[KCAP], ky0[KCAP], ky1[KCAP];
     uint32_t cx[KCAP + 1], cy[KCAP + 1];
-    uint32_t slow[64];          // fragments needing the exact coverage test
+    uint32_t ovl[FCAP];         // a writer's overlapping fragments
+    uint32_t okc[FCAP];         // per overlapping fragment: qualifying readers | 1 << 31 if one covers f n rect
+    uint32_t novl;
     uint32_t ccount;
     uint32_t dcount;            // entries of Du
     uint32_t tdom;              // domain << 24: or-ed into task numbers of candidates
@@ -288,13 +300,15 @@ __device__ void flush_cands(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s)
     __syncthreads();
     const uint32_t n = s.ccount;
     if (n) {
+        const uint32_t stripe = blockIdx.x % CSTRIPES;
         uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&a.ctr->ncand, n);
+        if (lane_id() == 0) base = atomicAdd(&a.cstr[16 * stripe], n);
         base = wave_bcast(base, 0);
         if ((uint64_t)base + n > a.cand_cap) {
             if (lane_id() == 0) atomicOr(&a.ctr->err, ERR_CAND);
         } else {
-            for (uint32_t i = lane_id(); i < 3 * n; i += 64) a.cand[3ull * base + i] = s.C[i];
+            uint32_t *dst = a.cand + 3ull * ((size_t)stripe * a.cand_cap + base);
+            for (uint32_t i = lane_id(); i < 3 * n; i += 64) dst[i] = s.C[i];
         }
     }
     __syncthreads();
@@ -307,7 +321,7 @@ template <int FCAP, int GCAP, int LCAP>
 __device__ __forceinline__ void room(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s)
 {
     __syncthreads();
-    if (s.ccount + 64 > CCAP) flush_cands(a, s);
+    if (s.ccount + 64 > BinLds<FCAP, GCAP, LCAP>::CCAP) flush_cands(a, s);
 }
 
 // Record predecessor p of access acc (task `task`): the run's dedup set keeps
@@ -332,11 +346,13 @@ __device__ __forceinline__ void emit(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s
         }
         h = (h + 1) & (DSZ - 1);
     }
-    const uint32_t pos = atomicAdd(&a.ctr->ncand, 1u);
+    const uint32_t stripe = blockIdx.x % CSTRIPES;
+    const uint32_t pos = atomicAdd(&a.cstr[16 * stripe], 1u);
     if (pos < a.cand_cap) {
-        a.cand[3ull * pos] = p;
-        a.cand[3ull * pos + 1] = acc;
-        a.cand[3ull * pos + 2] = task;
+        uint32_t *dst = a.cand + 3ull * ((size_t)stripe * a.cand_cap + pos);
+        dst[0] = p;
+        dst[1] = acc;
+        dst[2] = task;
     } else {
         atomicOr(&a.ctr->err, ERR_CAND);
     }
@@ -394,35 +410,61 @@ __device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &s
         __syncthreads();
         return;
     }
-    // writer: one lane per overlapping fragment f, Q = f n rect.  The readers
-    // logged after lw(f) that touch Q are all predecessors; lw(f) is one iff
-    // some point of Q has none of them.
+    // writer.  (1) the fragments f overlapping the rect, compacted; (2) one lane
+    // per logged reader g: g is a predecessor iff it touches some Q = f n rect
+    // with acc(g) > lw(f) -- emitted once -- and it counts towards Q; (3) lw(f)
+    // is a predecessor iff some point of Q has no such reader: none touches Q,
+    // or (not one covers Q and) the exact coverage test finds an uncovered point.
+    if (lane == 0) s.novl = 0;
+    __syncthreads();
     for (uint32_t c = 0; c < st.nf; c += 64) {
         const uint32_t f = c + lane;
-        bool ov = false;
-        uint32_t qx0 = 0, qx1 = 0, qy0 = 0, qy1 = 0, lw = NONE;
-        if (f < st.nf) {
-            qx0 = max(s.fx0[fb][f], X0); qx1 = min(s.fx1[fb][f], X1);
-            qy0 = max(s.fy0[fb][f], Y0); qy1 = min(s.fy1[fb][f], Y1);
-            ov = qx0 < qx1 && qy0 < qy1;
-            lw = s.flw[fb][f];
+        const bool ov = f < st.nf && overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f], X0, X1, Y0, Y1);
+        const uint64_t m = __ballot(ov);
+        if (ov) {
+            const uint32_t k = s.novl + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            s.ovl[k] = f;
+            s.okc[k] = 0;
         }
-        if (__ballot(ov) == 0) continue;
-        uint32_t kc = 0;
-        bool cov1 = false;
-        for (uint32_t g = 0; g < st.ng; ++g) {
+        __syncthreads();
+        if (lane == 0) s.novl += (uint32_t)__popcll(m);
+        __syncthreads();
+    }
+    const uint32_t no = s.novl;
+    for (uint32_t g0 = 0; g0 < st.ng; g0 += 64) {
+        const uint32_t g = g0 + lane;
+        if (g < st.ng) {
             const uint32_t gx0 = s.gx0[g], gx1 = s.gx1[g], gy0 = s.gy0[g], gy1 = s.gy1[g], ga = s.gacc[g];
-            if (ov && (lw == NONE || ga > lw) && overlaps(gx0, gx1, gy0, gy1, qx0, qx1, qy0, qy1)) {
-                emit(a, s, ga, acc, task);
-                ++kc;
-                cov1 = cov1 || (gx0 <= qx0 && qx1 <= gx1 && gy0 <= qy0 && qy1 <= gy1);
+            if (overlaps(gx0, gx1, gy0, gy1, X0, X1, Y0, Y1)) {
+                bool pred = false;
+                for (uint32_t k = 0; k < no; ++k) {
+                    const uint32_t f = s.ovl[k], lw = s.flw[fb][f];
+                    if (lw != NONE && ga <= lw) continue;
+                    const uint32_t qx0 = max(s.fx0[fb][f], X0), qx1 = min(s.fx1[fb][f], X1);
+                    const uint32_t qy0 = max(s.fy0[fb][f], Y0), qy1 = min(s.fy1[fb][f], Y1);
+                    if (!overlaps(gx0, gx1, gy0, gy1, qx0, qx1, qy0, qy1)) continue;
+                    pred = true;
+                    const bool full = gx0 <= qx0 && qx1 <= gx1 && gy0 <= qy0 && qy1 <= gy1;
+                    atomicAdd(&s.okc[k], 1u);
+                    if (full) atomicOr(&s.okc[k], 1u << 31);
+                }
+                if (pred) emit(a, s, ga, acc, task);
             }
         }
-        if (ov && lw != NONE && kc == 0) emit(a, s, lw, acc, task);
+    }
+    __syncthreads();
+    for (uint32_t c = 0; c < no; c += 64) {
+        const uint32_t k = c + lane;
+        uint32_t lw = NONE, kc = 0;
+        if (k < no) {
+            lw = s.flw[fb][s.ovl[k]];
+            kc = s.okc[k];
+        }
+        if (k < no && lw != NONE && kc == 0) emit(a, s, lw, acc, task);
         // partially covered fragments: exact test, one fragment at a time
-        uint64_t m = __ballot(ov && lw != NONE && kc > 0 && !cov1);
+        uint64_t m = __ballot(k < no && lw != NONE && kc != 0 && !(kc >> 31));
         while (m) {
-            const uint32_t ff = c + (uint32_t)__builtin_ctzll(m);
+            const uint32_t ff = s.ovl[c + (uint32_t)__builtin_ctzll(m)];
             m &= m - 1;
             const uint32_t rx0 = max(s.fx0[fb][ff], X0), rx1 = min(s.fx1[fb][ff], X1);
             const uint32_t ry0 = max(s.fy0[fb][ff], Y0), ry1 = min(s.fy1[fb][ff], Y1);
@@ -434,16 +476,16 @@ __device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &s
                                 overlaps(s.gx0[g], s.gx1[g], s.gy0[g], s.gy1[g], rx0, rx1, ry0, ry1);
                 const uint64_t km = __ballot(in);
                 if (in) {
-                    const uint32_t k = nk + (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
-                    if (k < KCAP) {
-                        s.kx0[k] = max(s.gx0[g], rx0); s.kx1[k] = min(s.gx1[g], rx1);
-                        s.ky0[k] = max(s.gy0[g], ry0); s.ky1[k] = min(s.gy1[g], ry1);
+                    const uint32_t kk = nk + (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
+                    if (kk < BinLds<FCAP, GCAP, LCAP>::KCAP) {
+                        s.kx0[kk] = max(s.gx0[g], rx0); s.kx1[kk] = min(s.gx1[g], rx1);
+                        s.ky0[kk] = max(s.gy0[g], ry0); s.ky1[kk] = min(s.gy1[g], ry1);
                     }
                 }
                 nk += (uint32_t)__popcll(km);
             }
             __syncthreads();  // kx/ky rows are read by other lanes below
-            if (nk > KCAP) {
+            if (nk > BinLds<FCAP, GCAP, LCAP>::KCAP) {
                 st.err |= ERR_READS;
                 return;
             }
@@ -452,17 +494,17 @@ __device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &s
             // uncovered point of the leftmost uncovered column.
             const uint32_t nc = nk + 1;
             if (lane == 0) { s.cx[0] = rx0; s.cy[0] = ry0; }
-            for (uint32_t k = lane; k < nk; k += 64) {
-                s.cx[k + 1] = s.kx1[k] < rx1 ? s.kx1[k] : rx0;
-                s.cy[k + 1] = s.ky1[k] < ry1 ? s.ky1[k] : ry0;
+            for (uint32_t q = lane; q < nk; q += 64) {
+                s.cx[q + 1] = s.kx1[q] < rx1 ? s.kx1[q] : rx0;
+                s.cy[q + 1] = s.ky1[q] < ry1 ? s.ky1[q] : ry0;
             }
             __syncthreads();
             bool unc = false;
             for (uint32_t p = lane; p < nc * nc && !unc; p += 64) {
                 const uint32_t px = s.cx[p % nc], py = s.cy[p / nc];
                 bool cov = false;
-                for (uint32_t k = 0; k < nk && !cov; ++k)
-                    cov = s.kx0[k] <= px && px < s.kx1[k] && s.ky0[k] <= py && py < s.ky1[k];
+                for (uint32_t q = 0; q < nk && !cov; ++q)
+                    cov = s.kx0[q] <= px && px < s.kx1[q] && s.ky0[q] <= py && py < s.ky1[q];
                 unc = !cov;
             }
             if (__ballot(unc) != 0 && lane == 0) emit(a, s, rlw, acc, task);
@@ -643,6 +685,35 @@ __device__ __forceinline__ void escalate(const KArgs &a, uint32_t tier, uint4 e,
     }
 }
 
+// the first 64 records, fragments and log entries of a bin (one per lane),
+// loaded ahead of the bin
+struct BinPre {
+    uint4 q0, q1;
+    uint32_t f0, f1, f2, f3, f4, g0, g1, g2, g3, g4;
+};
+
+__device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p)
+{
+    const uint32_t i = lane_id();
+    const uint32_t slot = e.x, start = e.y, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
+    p.q0 = make_uint4(0, 0, 0, 0);
+    p.q1 = make_uint4(NONE, 0, NONE, 0);
+    p.f0 = p.f1 = p.f2 = p.f3 = p.f4 = p.g0 = p.g1 = p.g2 = p.g3 = p.g4 = 0;
+    if (i < cnt && (size_t)start + cnt <= a.binrec_cap) {
+        const uint4 *ir = a.binrec + 2ull * (start + i);
+        p.q0 = ir[0];
+        p.q1 = ir[1];
+    }
+    if (i < nf0 && nf0 <= a.fcap) {
+        const uint32_t *q = a.frag + (size_t)slot * a.fcap * 5 + 5 * i;
+        p.f0 = q[0]; p.f1 = q[1]; p.f2 = q[2]; p.f3 = q[3]; p.f4 = q[4];
+    }
+    if (i < ng0 && ng0 <= a.rcap) {
+        const uint32_t *q = a.rs + (size_t)slot * a.rcap * 5 + 5 * i;
+        p.g0 = q[0]; p.g1 = q[1]; p.g2 = q[2]; p.g3 = q[3]; p.g4 = q[4];
+    }
+}
+
 template <int FCAP, int GCAP, int LCAP>
 __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
 {
@@ -657,10 +728,28 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     uint32_t maxf = 0, maxg = 0;
     if (lane == 0) s.ccount = 0;
     __syncthreads();
-    for (uint32_t b = blockIdx.x; b < nlist; b += gridDim.x) {
-        const uint4 e = b < nheavy ? list[b] : list[a.nbins - 1 - (b - nheavy)];
+    // Software pipeline over this block's bins b, b + G, b + 2G, ...: the
+    // list entry of the bin after next and the first chunk of the next bin's
+    // state are loaded before this bin's write-back, so the next bin's loads
+    // never wait behind this bin's stores.
+    const uint32_t G = gridDim.x;
+    auto entry = [&](uint32_t x) {
+        return x < nlist ? (x < nheavy ? list[x] : list[a.nbins - 1 - (x - nheavy)]) : make_uint4(0, 0, 0, 0);
+    };
+    uint4 e_cur = entry(blockIdx.x), e_nxt = entry(blockIdx.x + G);
+    BinPre pre;
+    bin_pre_load(a, e_cur, pre);
+    for (uint32_t b = blockIdx.x; b < nlist; b += G) {
+        const uint4 e = e_cur;
+        const BinPre cur = pre;
+        auto advance = [&]() {
+            e_cur = e_nxt;
+            bin_pre_load(a, e_cur, pre);
+            e_nxt = entry(b + 2 * G);
+        };
         const uint32_t slot = e.x, start = e.y, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
         if (cnt > (uint32_t)LCAP || nf0 > (uint32_t)FCAP || ng0 > (uint32_t)GCAP) {
+            advance();
             escalate<FCAP, GCAP, LCAP>(a, tier, e, ERR_LIST);
             continue;
         }
@@ -684,16 +773,21 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
             const uint4 *ir = a.binrec + 2ull * start;
             const uint32_t nmax = max(n2, max(nf0, ng0));
             for (uint32_t i = lane; i < nmax; i += 64) {
-                uint4 q0 = make_uint4(0, 0, 0, 0), q1 = make_uint4(NONE, 0, NONE, 0);
-                uint32_t f0 = 0, f1 = 0, f2 = 0, f3 = 0, f4 = 0, g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
-                if (i < cnt) { q0 = ir[2 * i]; q1 = ir[2 * i + 1]; }
-                if (i < nf0) {
-                    const uint32_t *q = fr + 5 * i;
-                    f0 = q[0]; f1 = q[1]; f2 = q[2]; f3 = q[3]; f4 = q[4];
-                }
-                if (i < ng0) {
-                    const uint32_t *q = gr + 5 * i;
-                    g0 = q[0]; g1 = q[1]; g2 = q[2]; g3 = q[3]; g4 = q[4];
+                uint4 q0 = cur.q0, q1 = cur.q1;  // the first chunk was loaded ahead
+                uint32_t f0 = cur.f0, f1 = cur.f1, f2 = cur.f2, f3 = cur.f3, f4 = cur.f4;
+                uint32_t g0 = cur.g0, g1 = cur.g1, g2 = cur.g2, g3 = cur.g3, g4 = cur.g4;
+                if (i >= 64) {
+                    q0 = make_uint4(0, 0, 0, 0); q1 = make_uint4(NONE, 0, NONE, 0);
+                    f0 = f1 = f2 = f3 = f4 = g0 = g1 = g2 = g3 = g4 = 0;
+                    if (i < cnt) { q0 = ir[2 * i]; q1 = ir[2 * i + 1]; }
+                    if (i < nf0) {
+                        const uint32_t *q = fr + 5 * i;
+                        f0 = q[0]; f1 = q[1]; f2 = q[2]; f3 = q[3]; f4 = q[4];
+                    }
+                    if (i < ng0) {
+                        const uint32_t *q = gr + 5 * i;
+                        g0 = q[0]; g1 = q[1]; g2 = q[2]; g3 = q[3]; g4 = q[4];
+                    }
                 }
                 if (i < n2) {
                     // unsorted staging: key = record index, payload in the record arrays
@@ -847,6 +941,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
             drain_dedup(a, s);
             prune_log(s, st);
         }
+        advance();  // before the write-back stores
         if (st.err) {
             escalate<FCAP, GCAP, LCAP>(a, tier, e, st.err);
             __syncthreads();
@@ -877,10 +972,10 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
         __syncthreads();
     }
     flush_cands(a, s);
-    if (lane == 0 && visits) {
-        atomicMax(&a.ctr->maxfrag, maxf);
-        atomicMax(&a.ctr->maxrs, maxg);
-        atomicAdd(&a.ctr->frag_visits, visits);
+    if (lane == 0 && visits && blockIdx.x < BLKSLOTS) {  // statistics: one slot per block, no shared counter
+        uint4 &o = a.blk[tier * BLKSLOTS + blockIdx.x];
+        const unsigned long long v = ((unsigned long long)o.w << 32 | o.z) + visits;
+        o = make_uint4(max(o.x, maxf), max(o.y, maxg), (uint32_t)v, (uint32_t)(v >> 32));
     }
 }
 
@@ -889,25 +984,29 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
 __global__ void k_dedup(KArgs a)
 {
     if (a.ctr->err & (ERR_CAND | ERR_BINREC)) return;  // candidates are incomplete
-    const uint32_t n = min(a.ctr->ncand, a.cand_cap);
+    // candidates: stripe q holds cstr[q] of them from index q * cand_cap
+    uint32_t n = 0;
+    for (uint32_t q = 0; q < CSTRIPES; ++q) n += min(a.cstr[16 * q], a.cand_cap);
     const uint32_t mask = a.hmask;  // the whole table: spread keys keep concurrent atomics on distinct lines
     const unsigned long long ebase = a.ctr->edge_base;
+    const uint32_t lane = lane_id();
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t p = a.cand[3ull * i], acc = a.cand[3ull * i + 1], t = a.cand[3ull * i + 2];
+        uint32_t q = 0, j = i;
+        while (j >= min(a.cstr[16 * q], a.cand_cap)) {
+            j -= min(a.cstr[16 * q], a.cand_cap);
+            ++q;
+        }
+        const uint32_t *c = a.cand + 3ull * ((size_t)q * a.cand_cap + j);
+        const uint32_t p = c[0], acc = c[1], t = c[2];
         const unsigned long long key = ((unsigned long long)p << 32) | t;
         uint32_t h = (uint32_t)mix64(key) & mask;
         uint32_t probe = 0;
+        bool isnew = false;
         while (true) {
             const unsigned long long old = atomicCAS(&a.hkeys[h], EMPTY64, key);
-            if (old == EMPTY64) {
+            if (old == EMPTY64 || old == key) {
                 atomicMin(&a.hvals[h], acc);
-                const unsigned long long pos = atomicAdd(&a.ctr->edge_total, 1ull);
-                if (pos >= a.edge_cap || pos - ebase > a.hmask) atomicOr(&a.ctr->err, ERR_EDGES);
-                else a.eslot[pos - ebase] = h;
-                break;
-            }
-            if (old == key) {
-                atomicMin(&a.hvals[h], acc);
+                isnew = old == EMPTY64;
                 break;
             }
             h = (h + 1) & mask;
@@ -915,6 +1014,24 @@ __global__ void k_dedup(KArgs a)
                 atomicOr(&a.ctr->err, ERR_HASH);
                 break;
             }
+        }
+        // one edge counter update per wave
+        const uint64_t act = __ballot(1);
+        const uint64_t nm = __ballot(isnew);
+        const uint32_t leader = (uint32_t)__builtin_ctzll(act);
+        uint32_t blo = 0, bhi = 0;
+        if (lane == leader && nm) {
+            const unsigned long long b0 = atomicAdd(&a.ctr->edge_total, (unsigned long long)__popcll(nm));
+            blo = (uint32_t)b0;
+            bhi = (uint32_t)(b0 >> 32);
+        }
+        blo = __shfl(blo, (int)leader, 64);
+        bhi = __shfl(bhi, (int)leader, 64);
+        if (isnew) {
+            const unsigned long long pos =
+                (((unsigned long long)bhi << 32) | blo) + (unsigned long long)__popcll(nm & ((1ull << lane) - 1ull));
+            if (pos >= a.edge_cap || pos - ebase > a.hmask) atomicOr(&a.ctr->err, ERR_EDGES);
+            else a.eslot[pos - ebase] = h;
         }
     }
 }
@@ -928,7 +1045,11 @@ __global__ void k_final(KArgs a)
     unsigned long long n = et > ebase ? et - ebase : 0ull;
     if (n > (unsigned long long)a.hmask + 1) n = (unsigned long long)a.hmask + 1;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid == 0) a.ctr->cand_total += a.ctr->ncand;
+    if (tid == 0) {
+        unsigned long long nc = 0;
+        for (uint32_t q = 0; q < CSTRIPES; ++q) nc += a.cstr[16 * q];
+        a.ctr->cand_total += nc;
+    }
     for (unsigned long long e = tid; e < n; e += (unsigned long long)gridDim.x * blockDim.x) {
         const uint32_t h = a.eslot[e];
         const unsigned long long key = a.hkeys[h];
@@ -943,7 +1064,8 @@ __global__ void k_final(KArgs a)
 // ---------------------------------------------------------------- maintenance
 
 // forget the state of pool bins [b0, b0 + n); with c, also the pool's counters
-__global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint32_t b0, uint32_t n, DevCounters *c)
+__global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint32_t b0, uint32_t n, DevCounters *c,
+                        uint4 *blk)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t i = tid; i < n; i += gridDim.x * blockDim.x) {
@@ -951,6 +1073,8 @@ __global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint32_t b0,
         nrs[b0 + i] = 0;
         cnt[b0 + i] = 0;
     }
+    if (c && blk)
+        for (uint32_t i = tid; i < 4 * BLKSLOTS; i += gridDim.x * blockDim.x) blk[i] = make_uint4(0, 0, 0, 0);
     if (tid == 0 && c) {
         c->err = 0;
         c->edge_total = 0;
@@ -964,11 +1088,6 @@ __global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint32_t b0,
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
-}
-
-__global__ void k_setn(DevCounters *c, uint32_t n)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) c->nrec = n;
 }
 
 // plain streaming copy, the reference point for HBM byte counters
@@ -1010,6 +1129,8 @@ struct KClass {
 };
 // fragments, reader-log entries, records per bin and launch
 const KClass KCLASSES[3] = {{64, 128, 256}, {256, 512, 512}, {1024, 2048, 1024}};
+// LDS tiers: a small tier for the many bins with few records, then the three classes
+const KClass TIERS[4] = {{64, 128, 64}, {64, 128, 256}, {256, 512, 512}, {1024, 2048, 1024}};
 
 template <int F, int R, int L>
 size_t lds_bytes() { return sizeof(BinLds<F, R, L>); }
@@ -1059,7 +1180,9 @@ struct xkrt_hip_pool {
     uint4 *binrec = nullptr;
     uint32_t binrec_cap = 0;
     uint32_t *cand = nullptr;
-    uint32_t cand_cap = 0;
+    uint32_t cand_cap = 0;  // per stripe
+    uint32_t *cstr = nullptr;
+    uint4 *blk = nullptr;
     unsigned long long *hkeys = nullptr;
     uint32_t *hvals = nullptr;
     uint32_t hmask = 0;
@@ -1091,8 +1214,9 @@ struct xkrt_hip_pool {
     // optional HIP-event timing of every launch
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
-    std::vector<hipEvent_t> ev_pending;  // 7 events per timed launch (2 for a graph launch), in launch order
-    std::vector<int> ev_kind;            // 7 or 2 per pending launch
+    // per timed launch, in launch order: (kernel slot, start, stop) of every kernel;
+    // a graph launch has one span with slot -1
+    std::vector<std::vector<std::tuple<int, hipEvent_t, hipEvent_t>>> ev_pending;
     double kernel_ms[6] = {0, 0, 0, 0, 0, 0};
     uint64_t timed_launches = 0;
     std::vector<float> launch_ms;        // per timed launch: first kernel start -> last kernel end
@@ -1129,32 +1253,27 @@ hipEvent_t take_event(xkrt_hip_pool *p)
     return e;
 }
 
-// accumulate the elapsed times of completed launches (stream synchronised)
+// accumulate the elapsed times of completed launches (stream synchronised):
+// per kernel, the span of its own start / stop events; per launch, the
+// first kernel's start to the last kernel's end
 void harvest_events(xkrt_hip_pool *p)
 {
-    size_t i = 0;
-    for (int kind : p->ev_kind) {
-        if (kind == 7) {
-            float tot = 0.f;
-            for (int k = 0; k < 6; ++k) {
-                float ms = 0.f;
-                if (hipEventElapsedTime(&ms, p->ev_pending[i + k], p->ev_pending[i + k + 1]) == hipSuccess) {
-                    p->kernel_ms[k] += ms;
-                    tot += ms;
-                }
-            }
-            p->launch_ms.push_back(tot);
-        } else {
+    for (auto &spans : p->ev_pending) {
+        if (spans.empty()) continue;
+        for (auto &[slot, a, b] : spans) {
             float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, p->ev_pending[i], p->ev_pending[i + 1]);
-            p->launch_ms.push_back(ms);
+            if (slot >= 0 && hipEventElapsedTime(&ms, a, b) == hipSuccess) p->kernel_ms[slot] += ms;
         }
+        float tot = 0.f;
+        (void)hipEventElapsedTime(&tot, std::get<1>(spans.front()), std::get<2>(spans.back()));
+        p->launch_ms.push_back(tot);
         p->timed_launches++;
-        i += kind;
+        for (auto &[slot, a, b] : spans) {
+            p->ev_pool.push_back(a);
+            p->ev_pool.push_back(b);
+        }
     }
-    for (hipEvent_t e : p->ev_pending) p->ev_pool.push_back(e);
     p->ev_pending.clear();
-    p->ev_kind.clear();
 }
 
 void drop_graph(xkrt_hip_pool *p)
@@ -1187,7 +1306,7 @@ int grow_bins(xkrt_hip_pool *p, uint32_t need)
     HIPCHK(hipMalloc(&cnt, nc * 4));
     HIPCHK(hipMalloc(&cur, nc * 4));
     HIPCHK(hipMalloc(&dom, nc * 4));
-    HIPCHK(hipMalloc(&act, nc * 16 * 3));
+    HIPCHK(hipMalloc(&act, nc * 16 * 4));
     HIPCHK(hipMemsetAsync(nf, 0, nc * 4, p->stream));
     HIPCHK(hipMemsetAsync(nrs, 0, nc * 4, p->stream));
     HIPCHK(hipMemsetAsync(cnt, 0, nc * 4, p->stream));
@@ -1228,7 +1347,7 @@ int alloc_bins(xkrt_hip_pool *p, uint32_t n, uint32_t dom, uint32_t *b0)
             r.second -= n;
             if (r.second == 0) p->free_ranges.erase(p->free_ranges.begin() + (long)i);
             hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>((n + 255) / 256, 4096u)), dim3(256), 0, p->stream,
-                               p->bin_nf, p->bin_nrs, p->bin_cnt, *b0, n, (DevCounters *)nullptr);
+                               p->bin_nf, p->bin_nrs, p->bin_cnt, *b0, n, (DevCounters *)nullptr, (uint4 *)nullptr);
             hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>((n + 255) / 256, 4096u)), dim3(256), 0,
                                p->stream, p->bin_dom + *b0, n, dom);
             return hip_err(hipGetLastError());
@@ -1306,10 +1425,11 @@ int ensure_grid(xkrt_hip_dep_domain *d, uint32_t ylo, uint32_t yhi)
     return 0;
 }
 
-KArgs make_args(xkrt_hip_pool *p, const uint4 *rec)
+KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
 {
     KArgs a;
     a.rec = rec;
+    a.nrec_arg = nrec_arg;
     a.dom = p->d_dom;
     a.bin_dom = p->bin_dom;
     a.bin_nf = p->bin_nf; a.bin_nrs = p->bin_nrs; a.frag = p->frag; a.rs = p->rs;
@@ -1317,49 +1437,64 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec)
     a.bin_cnt = p->bin_cnt; a.bin_cur = p->bin_cur; a.active = p->active; a.binrec = p->binrec;
     a.binrec_cap = p->binrec_cap;
     a.nbins = p->cap_bins;
-    a.max_tier = p->kclass;
-    for (int t = 0; t < 3; ++t) {
-        a.tier_f[t] = KCLASSES[t].fcap;
-        a.tier_g[t] = KCLASSES[t].rcap;
-        a.tier_l[t] = KCLASSES[t].lcap;
+    a.max_tier = p->kclass + 1;  // tier t + 1 is class t
+    for (int t = 0; t < 4; ++t) {
+        a.tier_f[t] = TIERS[t].fcap;
+        a.tier_g[t] = TIERS[t].rcap;
+        a.tier_l[t] = TIERS[t].lcap;
     }
-    a.cand = p->cand; a.cand_cap = p->cand_cap;
+    a.cand = p->cand; a.cand_cap = p->cand_cap; a.cstr = p->cstr; a.blk = p->blk;
     a.hkeys = p->hkeys; a.hvals = p->hvals; a.hmask = p->hmask;
     a.eslot = p->eslot; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
     a.ctr = p->ctr;
     return a;
 }
 
+// launches a kernel; when `spans` is given, with start / stop events around
+// that kernel alone (hipExtLaunchKernelGGL), filed under kernel slot `slot`
+template <typename K, typename... Args>
+void launch(xkrt_hip_pool *p, std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans, int slot, K kernel,
+            dim3 grid, dim3 block, uint32_t lds, Args... args)
+{
+    if (spans) {
+        hipEvent_t a = take_event(p), b = take_event(p);
+        if (a && b) {
+            hipExtLaunchKernelGGL(kernel, grid, block, lds, p->stream, a, b, 0, args...);
+            spans->push_back({slot, a, b});
+            return;
+        }
+        if (a) p->ev_pool.push_back(a);
+        if (b) p->ev_pool.push_back(b);
+    }
+    hipLaunchKernelGGL(kernel, grid, block, lds, p->stream, args...);
+}
+
 template <int F, int R, int L>
-void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier)
+void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
+                    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
     // enough blocks to fill the chip at this tier's LDS footprint; each walks its share of the list
     const size_t lds = lds_bytes<F, R, L>();
     const uint32_t per_cu = std::max<uint32_t>(1u, (uint32_t)(160u * 1024u / lds));
     const uint32_t blocks = std::min<uint32_t>(p->cap_bins, 256u * std::min<uint32_t>(per_cu, 8u));
-    hipLaunchKernelGGL((k_resolve<F, R, L>), dim3(blocks), dim3(64), lds, p->stream, a, tier);
+    launch(p, spans, 3, k_resolve<F, R, L>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
 }
 
-// the kernels of one launch; ev (7 events) brackets them when timing
-void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks, hipEvent_t *ev)
+// the kernels of one launch (kernel slots: count, alloc, fill, resolve, dedup, final)
+void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
+                    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
-    auto mark = [&](int k) { if (ev && ev[k]) (void)hipEventRecord(ev[k], p->stream); };
-    mark(0);
-    hipLaunchKernelGGL(k_count, dim3(rblocks), dim3(256), 0, p->stream, a);
-    mark(1);
-    hipLaunchKernelGGL(k_alloc, dim3((p->cap_bins + 255) / 256), dim3(256), 0, p->stream, a, p->cap_bins);
-    mark(2);
-    hipLaunchKernelGGL(k_fill, dim3(rblocks), dim3(256), 0, p->stream, a);
-    mark(3);
+    launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), 0, a);
+    launch(p, spans, 1, k_alloc, dim3(std::min<uint32_t>((p->cap_bins + 255) / 256, 4096u)), dim3(256), 0, a,
+           p->cap_bins);
+    launch(p, spans, 2, k_fill, dim3(rblocks), dim3(256), 0, a);
     // tier 0 first; bins that outgrow a tier are retried by the next one
-    launch_resolve<64, 128, 256>(p, a, 0);
-    if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, a, 1);
-    if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, a, 2);
-    mark(4);
-    hipLaunchKernelGGL(k_dedup, dim3(1024), dim3(256), 0, p->stream, a);
-    mark(5);
-    hipLaunchKernelGGL(k_final, dim3(1024), dim3(256), 0, p->stream, a);
-    mark(6);
+    launch_resolve<64, 128, 64>(p, a, 0, spans);
+    launch_resolve<64, 128, 256>(p, a, 1, spans);
+    if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, a, 2, spans);
+    if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, a, 3, spans);
+    launch(p, spans, 4, k_dedup, dim3(1024), dim3(256), 0, a);
+    launch(p, spans, 5, k_final, dim3(1024), dim3(256), 0, a);
 }
 
 int push_domains(xkrt_hip_pool *p)
@@ -1378,12 +1513,11 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
 {
     int rc = push_domains(p);
     if (rc) return rc;
-    if (!counted) hipLaunchKernelGGL(k_setn, dim3(1), dim3(64), 0, p->stream, p->ctr, nrec);
-    const bool graphed = p->cfg.graph && rec == p->rec_dev;
+    const bool graphed = p->cfg.graph && rec == p->rec_dev && counted;
     if (graphed) {
         if (!p->graph_valid) {
             drop_graph(p);
-            const KArgs a = make_args(p, p->rec_dev);
+            const KArgs a = make_args(p, p->rec_dev, NONE);
             const uint32_t rb = (uint32_t)std::min<uint64_t>((p->rec_cap + 255) / 256, 8192u);
             HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
             launch_kernels(p, a, rb, nullptr);
@@ -1403,25 +1537,16 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
         HIPCHK(hipGraphLaunch(p->gexec, p->stream));
         if (ev[1]) (void)hipEventRecord(ev[1], p->stream);
         if (ev[0] && ev[1]) {
-            p->ev_pending.insert(p->ev_pending.end(), ev, ev + 2);
-            p->ev_kind.push_back(2);
+            p->ev_pending.push_back({{-1, ev[0], ev[1]}});
         } else {
             for (hipEvent_t e : ev) if (e) p->ev_pool.push_back(e);
         }
     } else {
-        const KArgs a = make_args(p, rec);
+        const KArgs a = make_args(p, rec, counted ? NONE : nrec);
         const uint32_t rb = std::min<uint32_t>((nrec + 255) / 256, 8192u);
-        hipEvent_t ev[7] = {};
-        bool ok = p->timing;
-        if (p->timing)
-            for (int k = 0; k < 7; ++k) ok = (ev[k] = take_event(p)) && ok;
-        launch_kernels(p, a, rb, ok ? ev : nullptr);
-        if (ok) {
-            p->ev_pending.insert(p->ev_pending.end(), ev, ev + 7);
-            p->ev_kind.push_back(7);
-        } else {
-            for (hipEvent_t e : ev) if (e) p->ev_pool.push_back(e);
-        }
+        std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
+        launch_kernels(p, a, rb, p->timing ? &spans : nullptr);
+        if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
     }
     HIPCHK(hipGetLastError());
     p->n_launch++;
@@ -1482,10 +1607,14 @@ void destroy_pool(xkrt_hip_pool *p)
     drop_graph(p);
     for (auto &t : p->tickets) (void)hipEventDestroy(t.second);
     for (hipEvent_t e : p->ticket_pool) (void)hipEventDestroy(e);
-    for (hipEvent_t e : p->ev_pending) (void)hipEventDestroy(e);
+    for (auto &spans : p->ev_pending)
+        for (auto &[slot, a, b] : spans) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
     for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
     void *dev[] = {p->bin_nf, p->bin_nrs, p->frag, p->rs, p->bin_cnt, p->bin_cur, p->bin_dom, p->active, p->d_dom,
-                   p->binrec, p->cand, p->hkeys, p->hvals, p->eslot, p->edges, p->edom, p->ctr, p->rec_dev};
+                   p->binrec, p->cand, p->cstr, p->blk, p->hkeys, p->hvals, p->eslot, p->edges, p->edom, p->ctr, p->rec_dev};
     for (void *q : dev)
         if (q) (void)hipFree(q);
     if (p->ctr_host) (void)hipHostFree(p->ctr_host);
@@ -1515,18 +1644,21 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     p->cfg.max_rects = max_rects;
     p->rec_cap = max_rects;
     p->binrec_cap = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, std::max<uint64_t>(4 * max_rects, 1 << 16));
-    p->cand_cap = (uint32_t)std::min<uint64_t>(0x3FFFFFF0ull, std::max<uint64_t>(32 * max_rects, 1 << 20));
+    const uint64_t cand_total = std::min<uint64_t>(0x3FFFFFF0ull, std::max<uint64_t>(32 * max_rects, 1 << 20));
+    p->cand_cap = (uint32_t)(cand_total / CSTRIPES);
     uint64_t hsz = 1;
-    while (hsz < 2ull * p->cand_cap) hsz <<= 1;
+    while (hsz < 2ull * cand_total) hsz <<= 1;
     p->hmask = (uint32_t)(hsz - 1);
     p->edge_cap = p->cfg.max_edges ? p->cfg.max_edges : std::max<uint64_t>(16 * max_rects, 1 << 20);
-    p->bytes_scratch = 32ull * p->binrec_cap + 12ull * p->cand_cap + 12ull * hsz + 4ull * hsz +
+    p->bytes_scratch = 32ull * p->binrec_cap + 12ull * CSTRIPES * p->cand_cap + 12ull * hsz + 4ull * hsz +
                        12ull * p->edge_cap + 32ull * p->rec_cap + sizeof(DevCounters) + sizeof(p->h_dom);
     int rc = 0;
     auto chk = [&](hipError_t e) { if (!rc) rc = hip_err(e); };
     if (p->cfg.budget_bytes && p->bytes_scratch > p->cfg.budget_bytes) rc = -ENOMEM;
     chk(hipMalloc(&p->binrec, 32ull * p->binrec_cap));
-    chk(hipMalloc(&p->cand, 12ull * p->cand_cap));
+    chk(hipMalloc(&p->cand, 12ull * CSTRIPES * p->cand_cap));
+    chk(hipMalloc(&p->cstr, 64ull * CSTRIPES));
+    chk(hipMalloc(&p->blk, 16ull * 4 * BLKSLOTS));
     chk(hipMalloc(&p->hkeys, 8ull * hsz));
     chk(hipMalloc(&p->hvals, 4ull * hsz));
     chk(hipMalloc(&p->eslot, 4ull * hsz));
@@ -1541,7 +1673,11 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     if (!rc) chk(hipMemsetAsync(p->hkeys, 0xFF, 8ull * hsz, p->stream));
     if (!rc) chk(hipMemsetAsync(p->hvals, 0xFF, 4ull * hsz, p->stream));
     if (!rc) chk(hipMemsetAsync(p->ctr, 0, sizeof(DevCounters), p->stream));
+    if (!rc) chk(hipMemsetAsync(p->cstr, 0, 64ull * CSTRIPES, p->stream));
+    if (!rc) chk(hipMemsetAsync(p->blk, 0, 16ull * 4 * BLKSLOTS, p->stream));
     chk(hipEventCreateWithFlags(&p->staged, hipEventDisableTiming));
+    chk(hipFuncSetAttribute((const void *)k_resolve<64, 128, 64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes<64, 128, 64>()));
     chk(hipFuncSetAttribute((const void *)k_resolve<64, 128, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds_bytes<64, 128, 256>()));
     chk(hipFuncSetAttribute((const void *)k_resolve<256, 512, 512>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2009,15 +2145,24 @@ int xkrt_hip_dep_stats(xkrt_hip_dep_domain_t d, xkrt_hip_dep_stats_t *out)
     xkrt_hip_pool *p = d->pool;
     (void)read_counters(p);  // the error bits are reported in out->err
     const DevCounters &c = *p->ctr_host;
+    std::vector<uint4> blk(4 * BLKSLOTS);
+    HIPCHK(hipMemcpy(blk.data(), p->blk, 16ull * 4 * BLKSLOTS, hipMemcpyDeviceToHost));
+    uint32_t maxf = 0, maxg = 0;
+    uint64_t visits = 0;
+    for (const uint4 &b : blk) {
+        maxf = std::max(maxf, b.x);
+        maxg = std::max(maxg, b.y);
+        visits += (uint64_t)b.w << 32 | b.z;
+    }
     out->accesses = d->n_acc;
     out->rects = d->n_rect;
     out->edges = c.edge_total;
     out->candidates = c.cand_total;
     out->launches = p->n_launch;
     out->bins = (uint64_t)d->nbx * d->nby;
-    out->max_frags = c.maxfrag;
-    out->max_reads = c.maxrs;
-    out->frag_visits = c.frag_visits;
+    out->max_frags = maxf;
+    out->max_reads = maxg;
+    out->frag_visits = visits;
     out->device_bytes = (uint64_t)d->nbx * d->nby * bin_bytes(p) + (d->private_pool ? p->bytes_scratch : 0);
     out->err = c.err;
     return 0;
@@ -2033,9 +2178,10 @@ int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d)
     DevCounters *c = d->private_pool ? p->ctr : nullptr;
     if (nb) {
         hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>((nb + 255) / 256, 4096u)), dim3(256), 0, p->stream,
-                           p->bin_nf, p->bin_nrs, p->bin_cnt, d->bin0, nb, c);
+                           p->bin_nf, p->bin_nrs, p->bin_cnt, d->bin0, nb, c, p->blk);
     } else if (c) {
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, p->stream, p->bin_nf, p->bin_nrs, p->bin_cnt, 0u, 0u, c);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, p->stream, p->bin_nf, p->bin_nrs, p->bin_cnt, 0u, 0u, c,
+                           p->blk);
     }
     HIPCHK(hipGetLastError());
     if (d->private_pool) p->poisoned = false;
