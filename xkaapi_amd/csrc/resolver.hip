@@ -1476,7 +1476,7 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
     // enough blocks to fill the chip at this tier's LDS footprint; each walks its share of the list
     const size_t lds = lds_bytes<F, R, L>();
     const uint32_t per_cu = std::max<uint32_t>(1u, (uint32_t)(160u * 1024u / lds));
-    const uint32_t blocks = std::min<uint32_t>(p->cap_bins, 256u * std::min<uint32_t>(per_cu, 8u));
+    const uint32_t blocks = std::min<uint32_t>(p->cap_bins, 256u * std::min<uint32_t>(per_cu, 16u));
     launch(p, spans, 3, k_resolve<F, R, L>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
 }
 
