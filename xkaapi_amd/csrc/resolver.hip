@@ -288,11 +288,32 @@ struct BinLds {
 };
 
 struct BinState {
-    uint32_t nf;   // fragments in the current bank
-    uint32_t fb;   // current fragment bank
-    uint32_t ng;   // reader-log entries
-    uint32_t err;  // local error bits
+    uint32_t nf;      // fragments in the current bank
+    uint32_t fb;      // current fragment bank
+    uint32_t ng;      // reader-log entries
+    uint32_t err;     // local error bits
+    uint32_t slot;    // the bin
+    uint32_t ng0;     // log entries in HBM when the bin started
+    uint32_t gload;   // 1 once entries [0, ng0) are in LDS (they are loaded on first use)
+    uint32_t gdirty;  // LDS log entries from this index on differ from HBM
+    uint32_t fdirty;  // 1 when the fragments must be written back
 };
+
+// The reader log is read only by a writer's LINK and by pruning; bins whose
+// records only read append to it without loading it.
+template <int FCAP, int GCAP, int LCAP>
+__device__ void ensure_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st)
+{
+    if (st.gload) return;
+    const uint32_t *gr = a.rs + (size_t)st.slot * a.rcap * 5;
+    for (uint32_t i = lane_id(); i < st.ng0; i += 64) {
+        const uint32_t *q = gr + 5 * i;
+        const uint32_t g0 = q[0], g1 = q[1], g2 = q[2], g3 = q[3], g4 = q[4];
+        s.gx0[i] = g0; s.gx1[i] = g1; s.gy0[i] = g2; s.gy1[i] = g3; s.gacc[i] = g4;
+    }
+    st.gload = 1;
+    __syncthreads();
+}
 
 template <int FCAP, int GCAP, int LCAP>
 __device__ void flush_cands(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s)
@@ -415,6 +436,7 @@ __device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &s
     // with acc(g) > lw(f) -- emitted once -- and it counts towards Q; (3) lw(f)
     // is a predecessor iff some point of Q has no such reader: none touches Q,
     // or (not one covers Q and) the exact coverage test finds an uncovered point.
+    ensure_log(a, s, st);
     if (lane == 0) s.novl = 0;
     __syncthreads();
     for (uint32_t c = 0; c < st.nf; c += 64) {
@@ -515,7 +537,7 @@ __device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &s
 }
 
 template <int FCAP, int GCAP, int LCAP>
-__device__ void prune_log(BinLds<FCAP, GCAP, LCAP> &s, BinState &st);
+__device__ void prune_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st);
 
 // A run of whole task groups none of whose records writes: LINK of a
 // non-writer reads only the writer map, and PUT of a reader only appends to
@@ -550,7 +572,7 @@ __device__ void do_run(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st
         const uint32_t n = (uint32_t)__popcll(m);
         if (!n) continue;
         if (st.ng + n > (uint32_t)GCAP) {
-            prune_log(s, st);
+            prune_log(a, s, st);
             if (st.ng + n > (uint32_t)GCAP) {
                 st.err |= ERR_READS;
                 return;
@@ -613,14 +635,17 @@ __device__ void do_paint(BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0,
     }
     st.nf = base + 1;
     st.fb = nb;
+    st.fdirty = 1;
     __syncthreads();
 }
 
 // drop the logged readers that no point still sees: every fragment they
 // touch has a newer writer
 template <int FCAP, int GCAP, int LCAP>
-__device__ void prune_log(BinLds<FCAP, GCAP, LCAP> &s, BinState &st)
+__device__ void prune_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st)
 {
+    ensure_log(a, s, st);
+    st.gdirty = 0;  // compaction moves entries
     const uint32_t fb = st.fb;
     uint32_t base = 0;
     for (uint32_t g0 = 0; g0 < st.ng; g0 += 64) {
@@ -650,11 +675,11 @@ __device__ void prune_log(BinLds<FCAP, GCAP, LCAP> &s, BinState &st)
 
 // PUT of an exact reader (dependency-tree.hpp:243-244): log it
 template <int FCAP, int GCAP, int LCAP>
-__device__ void do_log(BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
+__device__ void do_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
                        uint32_t Y1, uint32_t acc)
 {
     if (st.ng >= (uint32_t)GCAP) {
-        prune_log(s, st);
+        prune_log(a, s, st);
         if (st.ng >= (uint32_t)GCAP) {
             st.err |= ERR_READS;
             return;
@@ -689,7 +714,7 @@ __device__ __forceinline__ void escalate(const KArgs &a, uint32_t tier, uint4 e,
 // loaded ahead of the bin
 struct BinPre {
     uint4 q0, q1;
-    uint32_t f0, f1, f2, f3, f4, g0, g1, g2, g3, g4;
+    uint32_t f0, f1, f2, f3, f4;
 };
 
 __device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p)
@@ -698,7 +723,7 @@ __device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p)
     const uint32_t slot = e.x, start = e.y, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
     p.q0 = make_uint4(0, 0, 0, 0);
     p.q1 = make_uint4(NONE, 0, NONE, 0);
-    p.f0 = p.f1 = p.f2 = p.f3 = p.f4 = p.g0 = p.g1 = p.g2 = p.g3 = p.g4 = 0;
+    p.f0 = p.f1 = p.f2 = p.f3 = p.f4 = 0;
     if (i < cnt && (size_t)start + cnt <= a.binrec_cap) {
         const uint4 *ir = a.binrec + 2ull * (start + i);
         p.q0 = ir[0];
@@ -708,10 +733,7 @@ __device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p)
         const uint32_t *q = a.frag + (size_t)slot * a.fcap * 5 + 5 * i;
         p.f0 = q[0]; p.f1 = q[1]; p.f2 = q[2]; p.f3 = q[3]; p.f4 = q[4];
     }
-    if (i < ng0 && ng0 <= a.rcap) {
-        const uint32_t *q = a.rs + (size_t)slot * a.rcap * 5 + 5 * i;
-        p.g0 = q[0]; p.g1 = q[1]; p.g2 = q[2]; p.g3 = q[3]; p.g4 = q[4];
-    }
+    (void)ng0;  // the reader log is loaded on first use (ensure_log)
 }
 
 template <int FCAP, int GCAP, int LCAP>
@@ -765,28 +787,23 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
         // (SoA in HBM: 5 x fcap fragment words, 5 x gcap log words)
         BinState st;
         st.fb = 0; st.err = 0;
+        st.slot = slot; st.ng0 = ng0; st.gload = 0; st.gdirty = ng0; st.fdirty = nf0 == 0 ? 1u : 0u;
         uint32_t n2 = 1;
         while (n2 < cnt) n2 <<= 1;
         {
             const uint32_t *fr = a.frag + (size_t)slot * a.fcap * 5;  // AoS, 5 words per fragment
-            const uint32_t *gr = a.rs + (size_t)slot * a.rcap * 5;    // AoS, 5 words per log entry
             const uint4 *ir = a.binrec + 2ull * start;
-            const uint32_t nmax = max(n2, max(nf0, ng0));
+            const uint32_t nmax = max(n2, nf0);
             for (uint32_t i = lane; i < nmax; i += 64) {
                 uint4 q0 = cur.q0, q1 = cur.q1;  // the first chunk was loaded ahead
                 uint32_t f0 = cur.f0, f1 = cur.f1, f2 = cur.f2, f3 = cur.f3, f4 = cur.f4;
-                uint32_t g0 = cur.g0, g1 = cur.g1, g2 = cur.g2, g3 = cur.g3, g4 = cur.g4;
                 if (i >= 64) {
                     q0 = make_uint4(0, 0, 0, 0); q1 = make_uint4(NONE, 0, NONE, 0);
-                    f0 = f1 = f2 = f3 = f4 = g0 = g1 = g2 = g3 = g4 = 0;
+                    f0 = f1 = f2 = f3 = f4 = 0;
                     if (i < cnt) { q0 = ir[2 * i]; q1 = ir[2 * i + 1]; }
                     if (i < nf0) {
                         const uint32_t *q = fr + 5 * i;
                         f0 = q[0]; f1 = q[1]; f2 = q[2]; f3 = q[3]; f4 = q[4];
-                    }
-                    if (i < ng0) {
-                        const uint32_t *q = gr + 5 * i;
-                        g0 = q[0]; g1 = q[1]; g2 = q[2]; g3 = q[3]; g4 = q[4];
                     }
                 }
                 if (i < n2) {
@@ -797,7 +814,6 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
                     s.racc[i] = q1.x; s.rtm[i] = q1.y;
                 }
                 if (i < nf0) { s.fx0[0][i] = f0; s.fx1[0][i] = f1; s.fy0[0][i] = f2; s.fy1[0][i] = f3; s.flw[0][i] = f4; }
-                if (i < ng0) { s.gx0[i] = g0; s.gx1[i] = g1; s.gy0[i] = g2; s.gy1[i] = g3; s.gacc[i] = g4; }
             }
         }
         if (nf0 == 0) {
@@ -933,13 +949,13 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
                 if (mode & 2u)
                     do_paint(s, st, s.rx0[q], s.rx1[q], s.ry0[q], s.ry1[q], s.racc[q]);
                 else if (mode == 1u)
-                    do_log(s, st, s.rx0[q], s.rx1[q], s.ry0[q], s.ry1[q], s.racc[q]);
+                    do_log(a, s, st, s.rx0[q], s.rx1[q], s.ry0[q], s.ry1[q], s.racc[q]);
             }
             i = j;
         }
         if (!st.err) {
             drain_dedup(a, s);
-            prune_log(s, st);
+            if (st.gload) prune_log(a, s, st);  // a log never loaded only grew by appends
         }
         advance();  // before the write-back stores
         if (st.err) {
@@ -953,11 +969,11 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
             const uint32_t fb = st.fb;
             uint32_t *fw = a.frag + (size_t)slot * a.fcap * 5;
             uint32_t *gw = a.rs + (size_t)slot * a.rcap * 5;
-            for (uint32_t f = lane; f < st.nf; f += 64) {
+            for (uint32_t f = lane; f < (st.fdirty ? st.nf : 0u); f += 64) {
                 uint32_t *q = fw + 5 * f;
                 q[0] = s.fx0[fb][f]; q[1] = s.fx1[fb][f]; q[2] = s.fy0[fb][f]; q[3] = s.fy1[fb][f]; q[4] = s.flw[fb][f];
             }
-            for (uint32_t g = lane; g < st.ng; g += 64) {
+            for (uint32_t g = st.gdirty + lane; g < st.ng; g += 64) {
                 uint32_t *q = gw + 5 * g;
                 q[0] = s.gx0[g]; q[1] = s.gx1[g]; q[2] = s.gy0[g]; q[3] = s.gy1[g]; q[4] = s.gacc[g];
             }
