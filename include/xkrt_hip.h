@@ -41,16 +41,26 @@ enum {
     XKRT_HIP_MODE_D  = 8
 };
 
-/* One column-major tile access, the region of the reference's BLAS access
+/* access kinds */
+enum {
+    XKRT_HIP_KIND_TILE    = 0,  /* column-major tile: addr, m rows, n columns */
+    XKRT_HIP_KIND_SEGMENT = 1   /* byte segment [addr, addr + m) seen by a BLAS domain */
+};
+
+/* One access.  A tile is the region of the reference's BLAS access
  * constructor (include/xkrt/memory/access/access.hpp:539-574) with the
- * offsets folded into addr; ld and sizeof_type are the domain's. */
+ * offsets folded into addr; ld and sizeof_type are the domain's.  A segment
+ * is a 1-D access (access.hpp:479-514) that the reference also links against
+ * every BLAS domain (src/task/task.cc:240-257) through the three rects of
+ * interval_to_rects (access.hpp:77-139). */
 typedef struct xkrt_hip_access {
     uint64_t addr;   /* begin address in bytes */
-    uint64_t m;      /* rows (elements) */
-    uint64_t n;      /* columns */
+    uint64_t m;      /* tile: rows (elements); segment: size in bytes */
+    uint64_t n;      /* tile: columns; segment: ignored */
     uint32_t task;   /* spawn-order task number; a task's accesses are contiguous, slot = order inside it */
     uint8_t  mode;   /* XKRT_HIP_MODE_* bits */
-    uint8_t  pad[3];
+    uint8_t  kind;   /* XKRT_HIP_KIND_* */
+    uint8_t  pad[2];
 } xkrt_hip_access_t;
 
 /* Device-frame rect record (32 B; arrays of them are what the kernels read).
@@ -110,7 +120,15 @@ int xkrt_hip_dep_domain_destroy(xkrt_hip_dep_domain_t d);
  * xkrt_hip_dep_sync.  A wrapped tile (2 rects) is refused with -ENOTSUP, and
  * nothing of the batch is enqueued, when an earlier rect of the domain spans
  * its wrap gap in dim 0: there the reference marks whole split fragments
- * beyond the tile (dependency-tree.hpp:229-256). */
+ * beyond the tile (dependency-tree.hpp:229-256).
+ * A segment links like any access but its PUT changes no state: the
+ * reference tests the tree's nodes against the segment's own rects, built
+ * with ld = SIZE_MAX (access.hpp:495-513), and those meet only nodes that
+ * start at column 0 (interval overlap is strict, so the empty rect
+ * [0, SIZE_MAX) x [1, 1) meets a node spanning columns 0 and 1).  Such nodes
+ * exist once an access touched the first ld * sizeof_type bytes of memory;
+ * from then on, and for a segment in those bytes, the batch is refused with
+ * -ENOTSUP (the reference would mark whole nodes there). */
 int xkrt_hip_dep_submit(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n, uint64_t *first_id);
 
 /* Same, from device-resident rect records (no host conversion, no copy):

@@ -22,6 +22,8 @@ def lib():
         _lib.xko_resolve.restype = C.c_int64
         _lib.xko_coherency.argtypes = [u, p, p, p, p, p, u, p, p, u, p, p, p, u, p, p, p]
         _lib.xko_coherency.restype = C.c_int64
+        _lib.xko_resolve_marks.argtypes = [u, p, p, p, p, p, p, u, p]
+        _lib.xko_resolve_marks.restype = C.c_int64
         _lib.xko_partition.argtypes = [u, p, p, p, u, p, p]
         _lib.xko_partition.restype = C.c_int64
     return _lib
@@ -48,6 +50,24 @@ def resolve(task, mode, rects, nrect, want_frags=False):
         cap = int(ne)
     if want_frags:
         return edges[:ne].copy(), frag[:n]
+    return edges[:ne].copy()
+
+
+def resolve_marks(task, mode, rects, nrect, marks, nmark):
+    """Edges with up to 3 inserted rects per access (12 u64) and separate marking rects (segments)."""
+    L = lib()
+    n = len(task)
+    task = _c(task, np.uint32); mode = _c(mode, np.uint8)
+    rects = _c(rects, np.uint64); nrect = _c(nrect, np.uint8)
+    marks = _c(marks, np.uint64); nmark = _c(nmark, np.uint8)
+    cap = max(1024, 4 * n)
+    while True:
+        edges = np.empty((cap, 2), np.uint32)
+        ne = L.xko_resolve_marks(n, task.ctypes.data, mode.ctypes.data, nrect.ctypes.data, rects.ctypes.data,
+                                 nmark.ctypes.data, marks.ctypes.data, cap, edges.ctypes.data)
+        if ne <= cap:
+            break
+        cap = int(ne)
     return edges[:ne].copy()
 
 

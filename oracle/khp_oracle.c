@@ -323,8 +323,9 @@ static void vpush(u32vec_t *v, uint32_t x)
 typedef struct {
     const uint32_t *task;
     const uint8_t *mode;
-    const uint8_t *nrect;
-    const uint64_t *rects;
+    const uint8_t *nmark;
+    const uint64_t *marks;  /* the rects on_insert tests a node against, mstride u64 per access */
+    uint32_t mstride;
     u32vec_t *succ;  /* per predecessor access: successor access indices */
     uint32_t cur;    /* access being linked / put */
     uint64_t cnt;
@@ -364,8 +365,8 @@ static void d_on_insert(void *pc, uint64_t x0, uint64_t x1, void *ctx)
     dpiece_t *p = pc;
     const uint32_t a = c->cur;
     const uint8_t m = c->mode[a];
-    for (int r = 0; r < c->nrect[a]; ++r) {
-        if (rect_hits(c->rects + 8 * a + 4 * r, x0, x1, p->y0, p->y1)) {
+    for (int r = 0; r < c->nmark[a]; ++r) {
+        if (rect_hits(c->marks + (uint64_t)c->mstride * a + 4 * r, x0, x1, p->y0, p->y1)) {
             if (m & M_W) {
                 p->nr = 0;
                 p->lw = a;
@@ -395,11 +396,12 @@ static void d_count(void *pc, uint64_t x0, uint64_t x1, void *ctx)
  * frag (nullable): per access, the fragments intersecting its rects after its
  * task's PUT pass.  Returns the edge count (may exceed edge_cap).
  */
-int64_t xko_resolve(uint64_t nacc, const uint32_t *task, const uint8_t *mode, const uint8_t *nrect,
-                    const uint64_t *rects, uint64_t edge_cap, uint32_t *edges, uint64_t *frag)
+static int64_t resolve_impl(uint64_t nacc, const uint32_t *task, const uint8_t *mode, const uint8_t *nrect,
+                            const uint64_t *rects, uint32_t rstride, const uint8_t *nmark, const uint64_t *marks,
+                            uint32_t mstride, uint64_t edge_cap, uint32_t *edges, uint64_t *frag)
 {
     flat_t f = {&DPT, NULL, 0, 0};
-    dctx_t c = {task, mode, nrect, rects, calloc(nacc ? nacc : 1, sizeof(u32vec_t)), 0, 0};
+    dctx_t c = {task, mode, nmark, marks, mstride, calloc(nacc ? nacc : 1, sizeof(u32vec_t)), 0, 0};
     uint64_t i = 0;
     while (i < nacc) {
         uint64_t j = i;
@@ -407,14 +409,14 @@ int64_t xko_resolve(uint64_t nacc, const uint32_t *task, const uint8_t *mode, co
         for (uint64_t a = i; a < j; ++a) {
             c.cur = (uint32_t)a;
             for (int r = 0; r < nrect[a]; ++r) {
-                const uint64_t *q = rects + 8 * a + 4 * r;
+                const uint64_t *q = rects + (uint64_t)rstride * a + 4 * r;
                 flat_intersect(&f, q[0], q[1], q[2], q[3], d_on_intersect, &c);
             }
         }
         for (uint64_t a = i; a < j; ++a) {
             c.cur = (uint32_t)a;
             for (int r = 0; r < nrect[a]; ++r) {
-                const uint64_t *q = rects + 8 * a + 4 * r;
+                const uint64_t *q = rects + (uint64_t)rstride * a + 4 * r;
                 flat_insert(&f, q[0], q[1], q[2], q[3], d_on_insert, &c);
             }
         }
@@ -457,6 +459,28 @@ int64_t xko_resolve(uint64_t nacc, const uint32_t *task, const uint8_t *mode, co
     free(c.succ);
     flat_free(&f);
     return (int64_t)ne;
+}
+
+int64_t xko_resolve(uint64_t nacc, const uint32_t *task, const uint8_t *mode, const uint8_t *nrect,
+                    const uint64_t *rects, uint64_t edge_cap, uint32_t *edges, uint64_t *frag)
+{
+    return resolve_impl(nacc, task, mode, nrect, rects, 8, nrect, rects, 8, edge_cap, edges, frag);
+}
+
+/*
+ * Same, for streams with up to 3 rects per access (12 u64 each) that the tree
+ * inserts and intersects, and separately the rects (marks, 12 u64 each) that
+ * on_insert tests a node against (dependency-tree.hpp:241-256 iterates
+ * access->rects()).  For a tile both are its rects; for a segment seen by a
+ * BLAS domain the inserted rects are interval_to_rects in the domain's frame
+ * (dependency-tree.hpp:358-364) and the marks its own rects built with
+ * ld = SIZE_MAX (access.hpp:495-513).
+ */
+int64_t xko_resolve_marks(uint64_t nacc, const uint32_t *task, const uint8_t *mode, const uint8_t *nrect,
+                          const uint64_t *rects, const uint8_t *nmark, const uint64_t *marks, uint64_t edge_cap,
+                          uint32_t *edges)
+{
+    return resolve_impl(nacc, task, mode, nrect, rects, 12, nmark, marks, 12, edge_cap, edges, NULL);
 }
 
 /* Fragment list of a rect stream under plain inserts (no state): for tests. */

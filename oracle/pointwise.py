@@ -31,8 +31,10 @@ def _points(rect_row, nrect):
     return out
 
 
-def resolve(task, mode, rects, nrect, xdim, ydim):
-    """Edges as a set of (pred, succ) access indices."""
+def resolve(task, mode, rects, nrect, xdim, ydim, put=None):
+    """Edges as a set of (pred, succ) access indices.  put[a] False: the
+    access links but its PUT changes nothing (a segment seen by a BLAS domain
+    whose own rects meet no node of it)."""
     n = len(task)
     LW = np.full((xdim, ydim), -1, np.int64)
     RS = [[[] for _ in range(ydim)] for _ in range(xdim)]
@@ -60,6 +62,8 @@ def resolve(task, mode, rects, nrect, xdim, ydim):
                     linked[p] = T
                     edges.add((p, a))
         for a in range(i, j):
+            if put is not None and not put[a]:
+                continue
             for (x0, x1, y0, y1) in _points(rects[a], nrect[a]):
                 for x in range(x0, x1):
                     for y in range(y0, y1):

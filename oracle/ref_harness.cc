@@ -37,7 +37,9 @@ struct Acc {
     uint32_t task;
     uint8_t mode;
     int nrect;
-    Rect rects[2];
+    Rect rects[3];  // inserted and intersected (dependency-tree.hpp:366-426)
+    int nmark;
+    Rect marks[3];  // what on_insert tests a node against: access->rects() (dependency-tree.hpp:241-256)
     std::vector<Acc *> successors;
 };
 
@@ -90,8 +92,8 @@ class DepTree : public DepBase {
     void on_insert(Node *nb, DepSearch &s) override
     {
         DepNode *node = reinterpret_cast<DepNode *>(nb);
-        for (int r = 0; r < s.access->nrect; ++r) {
-            if (s.access->rects[r].intersects(node->hyperrect)) {
+        for (int r = 0; r < s.access->nmark; ++r) {
+            if (s.access->marks[r].intersects(node->hyperrect)) {
                 if (s.access->mode & MODE_W) {
                     node->last_reads.clear();
                     node->last_write = s.access;
@@ -202,6 +204,9 @@ int64_t khpref_resolve(uint64_t nacc, const uint32_t *task, const uint8_t *mode,
         acc[i].nrect = nrect[i];
         acc[i].rects[0] = make_rect(rects + 8 * i);
         acc[i].rects[1] = make_rect(rects + 8 * i + 4);
+        acc[i].nmark = acc[i].nrect;
+        acc[i].marks[0] = acc[i].rects[0];
+        acc[i].marks[1] = acc[i].rects[1];
         acc[i].successors.reserve(8);
     }
     DepTree tree;
@@ -230,6 +235,48 @@ int64_t khpref_resolve(uint64_t nacc, const uint32_t *task, const uint8_t *mode,
     return (int64_t)ne;
 }
 
+// Same, up to 3 rects per access (12 u64 each) inserted / intersected, and the
+// rects on_insert tests against given separately (marks, 12 u64 each): for a
+// segment seen by a BLAS domain, interval_to_rects in the domain frame and
+// the segment's own rects (built with ld = SIZE_MAX).
+int64_t khpref_resolve_marks(uint64_t nacc, const uint32_t *task, const uint8_t *mode, const uint8_t *nrect,
+                             const uint64_t *rects, const uint8_t *nmark, const uint64_t *marks, uint64_t edge_cap,
+                             uint32_t *edges)
+{
+    std::vector<Acc> acc(nacc);
+    for (uint64_t i = 0; i < nacc; ++i) {
+        acc[i].idx = (uint32_t)i;
+        acc[i].task = task[i];
+        acc[i].mode = mode[i];
+        acc[i].nrect = nrect[i];
+        acc[i].nmark = nmark[i];
+        for (int r = 0; r < 3; ++r) {
+            acc[i].rects[r] = make_rect(rects + 12 * i + 4 * r);
+            acc[i].marks[r] = make_rect(marks + 12 * i + 4 * r);
+        }
+        acc[i].successors.reserve(8);
+    }
+    DepTree tree;
+    uint64_t i = 0;
+    while (i < nacc) {
+        uint64_t j = i;
+        while (j < nacc && task[j] == task[i]) ++j;
+        for (uint64_t a = i; a < j; ++a) tree.link(&acc[a]);
+        for (uint64_t a = i; a < j; ++a) tree.put(&acc[a]);
+        i = j;
+    }
+    uint64_t ne = 0;
+    for (uint64_t p = 0; p < nacc; ++p)
+        for (Acc *sc : acc[p].successors) {
+            if (ne < edge_cap) {
+                edges[2 * ne] = (uint32_t)p;
+                edges[2 * ne + 1] = sc->idx;
+            }
+            ne++;
+        }
+    return (int64_t)ne;
+}
+
 // Total wall time (ns) of LINK + PUT over the stream, nothing else recorded.
 uint64_t khpref_time(uint64_t nacc, const uint32_t *task, const uint8_t *mode, const uint8_t *nrect,
                      const uint64_t *rects, uint64_t *nedges)
@@ -242,6 +289,9 @@ uint64_t khpref_time(uint64_t nacc, const uint32_t *task, const uint8_t *mode, c
         acc[i].nrect = nrect[i];
         acc[i].rects[0] = make_rect(rects + 8 * i);
         acc[i].rects[1] = make_rect(rects + 8 * i + 4);
+        acc[i].nmark = acc[i].nrect;
+        acc[i].marks[0] = acc[i].rects[0];
+        acc[i].marks[1] = acc[i].rects[1];
         acc[i].successors.reserve(8);
     }
     DepTree tree;

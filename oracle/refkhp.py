@@ -27,6 +27,8 @@ def lib():
         _lib.khpref_time.argtypes = [u, p, p, p, p, p]
         _lib.khpref_time.restype = u
         _lib.khpref_partition.argtypes = [u, p, p, p, u, p, p]
+        _lib.khpref_resolve_marks.argtypes = [u, p, p, p, p, p, p, u, p]
+        _lib.khpref_resolve_marks.restype = C.c_int64
         _lib.khpref_partition.restype = C.c_int64
     return _lib
 
@@ -60,6 +62,24 @@ def resolve(task, mode, rects, nrect, want_frags=False, want_times=False):
     if want_times:
         out.append(tns)
     return out[0] if len(out) == 1 else tuple(out)
+
+
+def resolve_marks(task, mode, rects, nrect, marks, nmark):
+    """The reference engine with up to 3 inserted rects per access and separate marking rects."""
+    L = lib()
+    n = len(task)
+    task = _c(task, np.uint32); mode = _c(mode, np.uint8)
+    rects = _c(rects, np.uint64); nrect = _c(nrect, np.uint8)
+    marks = _c(marks, np.uint64); nmark = _c(nmark, np.uint8)
+    cap = max(1024, 4 * n)
+    while True:
+        edges = np.empty((cap, 2), np.uint32)
+        ne = L.khpref_resolve_marks(n, task.ctypes.data, mode.ctypes.data, nrect.ctypes.data, rects.ctypes.data,
+                                    nmark.ctypes.data, marks.ctypes.data, cap, edges.ctypes.data)
+        if ne <= cap:
+            break
+        cap = int(ne)
+    return edges[:ne].copy()
 
 
 def time_resolve(task, mode, rects, nrect):

@@ -10,6 +10,8 @@ Fixtures (npz, allow_pickle=False):
   random2d_4000.npz          first 4000 random tasks over 65536^2 (4 B elements)
   wrap_mixed.npz             wrapped 2-rect tiles, every access mode, seeded
   partition_coherency.npz    memory-tree partition geometry of 500 coherency accesses (view 0)
+  segments_mixed.npz         tiles and byte segments (at or above the first column) in one BLAS
+                             domain: (pred, succ) access-index edges, unique sorted rows
 Each resolver fixture holds the stream (task, slot, mode, addr, ld, m, n, s),
 the edges as (pred_task, pred_slot, succ_task, succ_slot) sorted rows, and
 the per-access fragment counts F_i.
@@ -23,6 +25,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from oracle import refkhp  # noqa: E402
+from oracle import segments as G  # noqa: E402
 from xkaapi_amd import streams as S  # noqa: E402
 
 
@@ -70,6 +73,13 @@ def main():
     np.savez_compressed(os.path.join(HERE, "partition_coherency.npz"), dev=sub.dev, mode=sub.mode, addr=sub.addr,
                         ld=sub.ld, m=sub.m, n=sub.n, s=sub.s, part_off=off, parts=parts.astype(np.int64))
     print("partition_coherency.npz", len(sub), "accesses", len(parts), "partites")
+    ms = G.mixed_stream(2000, 32, 48, 4, seed=17, pseg=0.35, maxm=24, maxn=8)
+    rects, nrect, marks, nmark = G.stream_rects(ms["kind"], ms["addr"], ms["m"], ms["n"], 32, 4)
+    e = refkhp.resolve_marks(ms["task"], ms["mode"], rects, nrect, marks, nmark)
+    np.savez_compressed(os.path.join(HERE, "segments_mixed.npz"), task=ms["task"], slot=ms["slot"], mode=ms["mode"],
+                        kind=ms["kind"], addr=ms["addr"], m=ms["m"], n=ms["n"], ld=np.int64(32), s=np.int64(4),
+                        edges=np.unique(e.astype(np.int64), axis=0))
+    print("segments_mixed.npz", len(ms["task"]), "accesses", len(e), "edges")
 
 
 if __name__ == "__main__":

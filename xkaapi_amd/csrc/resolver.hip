@@ -908,14 +908,14 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
             // a chain of one-record writer groups on the same rect: each links to
             // the previous one (it wrote the whole rect and nobody read since), so
             // only the first needs the full LINK and the last one's PUT stands
-            if (j == i + 1) {
+            if (j == i + 1 && !(s.rtm[i] & 0x80u)) {
                 uint32_t m = 1;
                 for (uint32_t c = i + 1; c < cnt; c += 64) {
                     const uint32_t q = c + lane;
                     bool ok = false;
                     if (q < cnt) {
                         const uint32_t tq = s.rtm[q] >> 8;
-                        ok = (s.rtm[q] & 2u) && s.rx0[q] == s.rx0[i] && s.rx1[q] == s.rx1[i] && s.ry0[q] == s.ry0[i] &&
+                        ok = (s.rtm[q] & 2u) && !(s.rtm[q] & 0x80u) && s.rx0[q] == s.rx0[i] && s.rx1[q] == s.rx1[i] && s.ry0[q] == s.ry0[i] &&
                              s.ry1[q] == s.ry1[i] && tq != (s.rtm[q - 1] >> 8) && (q + 1 == cnt || (s.rtm[q + 1] >> 8) != tq);
                     }
                     const uint64_t bad = __ballot(!ok);
@@ -946,6 +946,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
             for (uint32_t q = i; q < j && !st.err; ++q) {
                 const uint32_t mode = s.rtm[q] & 0xffu;
                 visits += st.nf;
+                if (mode & 0x80u) continue;  // a segment seen by this domain: LINK only
                 if (mode & 2u)
                     do_paint(s, st, s.rx0[q], s.rx1[q], s.ry0[q], s.ry1[q], s.racc[q]);
                 else if (mode == 1u)
@@ -1250,6 +1251,7 @@ struct xkrt_hip_dep_domain {
     uint64_t ybase = 0;
     uint64_t next_id = 0;
     uint64_t max_xlen = 0;
+    bool col0 = false;  // some rect started at absolute column 0 (addresses below ld * sizeof_type)
     uint64_t n_acc = 0, n_rect = 0;
 };
 
@@ -1760,19 +1762,42 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
     uint64_t k = 0, lo = UINT64_MAX, hi = 0;
     uint32_t tl = task0;
     uint64_t maxx = d->max_xlen;
+    bool col0 = d->col0;
     for (uint64_t i = 0; i < n; ++i) {
         const xkrt_hip_access_t &q = acc[i];
         if (i && q.task != acc[i - 1].task) {
             if (q.task < acc[i - 1].task) return -EINVAL;
             tl++;
         }
-        if (q.m == 0 || q.n == 0) return -EINVAL;
+        if (q.m == 0 || (q.n == 0 && q.kind == XKRT_HIP_KIND_TILE) || q.kind > XKRT_HIP_KIND_SEGMENT) return -EINVAL;
         if (tl > 0xFFFFFFu) return -ERANGE;
-        // access.hpp:205-287 matrix_to_rects
-        const uint64_t X0 = q.addr % LDs, Y0 = q.addr / LDs, ms = q.m * s;
-        uint64_t r[2][4];
+        uint64_t r[3][4];
         int nr;
-        if (X0 + ms <= LDs) {
+        uint32_t flag = 0;
+        const uint64_t X0 = q.addr % LDs, Y0 = q.addr / LDs, ms = q.m * s;
+        if (q.kind == XKRT_HIP_KIND_SEGMENT) {
+            // access.hpp:77-139 interval_to_rects in the domain frame; empty rects
+            // are skipped, as the tree's insert and intersect skip them.  PUT tests
+            // the tree's nodes against the segment's own rects, built with
+            // ld = SIZE_MAX (access.hpp:495-513): [addr, addr + m) x [0, 1) and two
+            // empty rects, one of them [0, SIZE_MAX) x [1, 1).  Interval overlap is
+            // strict (interval.hpp:110-119), so these meet exactly the nodes that
+            // start at column 0 -- which exist only once some access touched the
+            // first ld * sizeof_type bytes of memory.  Before that the PUT marks
+            // nothing and the segment only links; after it, it is refused.
+            if (q.addr < LDs || col0) return -ENOTSUP;
+            const uint64_t size = q.m, x1 = std::min(X0 + size, LDs), dx10 = x1 - X0;
+            const uint64_t x3 = (size - dx10) % LDs;
+            const uint64_t y1 = Y0 + 1, y3 = Y0 + (size - dx10 - x3) / LDs + 2, y2 = y3 - 1;
+            const uint64_t c[3][4] = {{X0, x1, Y0, y1}, {0, LDs, y1, y2}, {0, x3, y2, y3}};
+            nr = 0;
+            for (int j = 0; j < 3; ++j)
+                if (c[j][0] < c[j][1] && c[j][2] < c[j][3]) {
+                    for (int w = 0; w < 4; ++w) r[nr][w] = c[j][w];
+                    ++nr;
+                }
+            flag = 0x80u;  // LINK only: its PUT marks no node of this domain
+        } else if (X0 + ms <= LDs) {
             r[0][0] = X0; r[0][1] = X0 + ms; r[0][2] = Y0; r[0][3] = Y0 + q.n;
             nr = 1;
         } else {
@@ -1787,12 +1812,13 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
         for (int j = 0; j < nr; ++j) {
             if (r[j][2] < d->ybase || r[j][3] - d->ybase > 0xFFFFFFFFull) return -ERANGE;
             maxx = std::max(maxx, r[j][1] - r[j][0]);
+            col0 = col0 || r[j][2] == 0;
             if (k < cap) {
                 xkrt_hip_rect_t &o = out[k];
                 o.x0 = (uint32_t)r[j][0]; o.x1 = (uint32_t)r[j][1];
                 o.y0 = (uint32_t)(r[j][2] - d->ybase); o.y1 = (uint32_t)(r[j][3] - d->ybase);
                 o.acc = first_id + (uint32_t)i;
-                o.tm = (tl << 8) | q.mode;
+                o.tm = (tl << 8) | q.mode | flag;
                 o.dom = d->idx;
                 o.pad = 0;
             }
@@ -1802,6 +1828,7 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
         }
     }
     d->max_xlen = maxx;
+    d->col0 = col0;
     *nrect = k;
     if (ylo) *ylo = n ? (uint32_t)lo : 0;
     if (yhi) *yhi = n ? (uint32_t)hi : 0;
@@ -1816,7 +1843,7 @@ int stage(xkrt_hip_pool *p, xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc
     if (first_id) *first_id = d->next_id;
     if (n == 0) return 0;
     if (n > 0xFFFFFFFFull || d->next_id + n > 0xFFFFFFFFull) return -ERANGE;
-    if (p->open_nrec + 2 * n > p->rec_cap) return p->open_nrec ? -ENOSPC : -EINVAL;
+    if (p->open_nrec + 3 * n > p->rec_cap) return p->open_nrec ? -ENOSPC : -EINVAL;
     // the staging buffer must be free: wait for the previous upload
     if (p->staged_pending) {
         HIPCHK(hipEventSynchronize(p->staged));
@@ -2203,6 +2230,7 @@ int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d)
     if (d->private_pool) p->poisoned = false;
     d->next_id = 0;
     d->max_xlen = 0;
+    d->col0 = false;
     return 0;
 }
 

@@ -24,11 +24,12 @@ _u64, _u32, _p = C.c_uint64, C.c_uint32, C.c_void_p
 
 class AccessT(C.Structure):
     _fields_ = [("addr", C.c_uint64), ("m", C.c_uint64), ("n", C.c_uint64), ("task", C.c_uint32),
-                ("mode", C.c_uint8), ("pad", C.c_uint8 * 3)]
+                ("mode", C.c_uint8), ("kind", C.c_uint8), ("pad", C.c_uint8 * 2)]
 
 
+KIND_TILE, KIND_SEGMENT = 0, 1
 ACCESS_DTYPE = np.dtype([("addr", "<u8"), ("m", "<u8"), ("n", "<u8"), ("task", "<u4"), ("mode", "u1"),
-                         ("pad", "u1", (3,))])
+                         ("kind", "u1"), ("pad", "u1", (2,))])
 assert ACCESS_DTYPE.itemsize == C.sizeof(AccessT) == 32
 
 
@@ -138,13 +139,17 @@ def device_copy(dst_ptr: int, src_ptr: int, nbytes: int, stream: int = 0):
     _chk(hip_lib().xkrt_hip_copy(dst_ptr, src_ptr, nbytes, stream or None), "xkrt_hip_copy")
 
 
-def to_access_array(task, mode, addr, m, n) -> np.ndarray:
+def to_access_array(task, mode, addr, m, n, kind=None) -> np.ndarray:
+    """Accesses for submit: tiles (addr, m rows, n columns) or, where kind is
+    KIND_SEGMENT, byte segments [addr, addr + m)."""
     a = np.zeros(len(task), ACCESS_DTYPE)
     a["task"] = task
     a["mode"] = mode
     a["addr"] = addr
     a["m"] = m
     a["n"] = n
+    if kind is not None:
+        a["kind"] = kind
     return a
 
 
@@ -196,7 +201,7 @@ class DependencyDomain:
     def build_records(self, acc: np.ndarray, first_id: int):
         """Host conversion to device-frame rect records (RECT_DTYPE), ylo, yhi."""
         acc = np.ascontiguousarray(acc, ACCESS_DTYPE)
-        cap = 2 * len(acc)
+        cap = 3 * len(acc)  # a segment gives up to 3 rects, a tile up to 2
         out = np.zeros(max(cap, 1), RECT_DTYPE)
         nrect, ylo, yhi = _u64(), _u32(), _u32()
         _chk(self.L.xkrt_hip_dep_build_records(self.h, acc.ctypes.data, len(acc), first_id, cap, out.ctypes.data,
