@@ -119,3 +119,28 @@ def test_pool_domain_reuse_after_leave():
             e, dd = pool.edges()
             assert np.array_equal(unique_rows(e[dd == d.index].astype(np.int64) - f), oracle_edges(st))
             pool.leave(d)
+
+
+def test_pool_stage_room_counts_rects():
+    # an open launch holds max_rects rects; a tile takes one, a column-wrapped tile two
+    ld, s = 64, 8
+    ntile = 6
+    tiles = R.to_access_array(np.arange(ntile), np.full(ntile, 2), (2 + 4 * np.arange(ntile)) * ld * s,
+                              np.full(ntile, 8), np.full(ntile, 2))
+    # a tile of 8 rows starting at row 60 wraps into the next column: two rects
+    wrapped = R.to_access_array(np.arange(4), np.full(4, 1), (40 + 4 * np.arange(4)) * ld * s + 60 * s,
+                                np.full(4, 8), np.full(4, 1))
+    with R.Pool(kernel=0, max_rects=8) as pool:
+        d = pool.domain(ld, s, (0, 64))
+        pool.stage(d, tiles)             # 6 rects
+        with pytest.raises(R.ResolverError) as ei:
+            pool.stage(d, wrapped[:2])   # 4 more: the open launch is full
+        assert ei.value.rc == -28
+        pool.launch()
+        with pytest.raises(R.ResolverError) as ei:
+            pool.stage(d, np.concatenate([wrapped, wrapped[:1]]))
+        assert ei.value.rc == -22        # 10 rects never fit one launch
+        pool.stage(d, wrapped)           # 8 rects: exactly full
+        pool.launch()
+        e, dd = pool.edges()
+        assert len(e) == 0               # disjoint accesses

@@ -67,6 +67,7 @@ enum : uint32_t {
 
 struct DevCounters {
     unsigned int nrec;       // rect records of this launch (set before the launch)
+    unsigned int ntouched;   // bins that got records this launch (listed by k_count; zeroed by k_final)
     unsigned int total;      // incidences allocated this launch
     unsigned int ntier[4];   // active bins per LDS tier this launch (heavy ones, from the list front)
     unsigned int nlight[4];  // light active bins per tier (from the list back)
@@ -100,6 +101,7 @@ struct KArgs {
     uint32_t *rs;             // nbins * 5 * rcap reader log (SoA x0, x1, y0, y1, acc)
     uint32_t fcap, rcap;
     uint32_t *bin_cnt, *bin_cur;
+    uint32_t *touched;        // bins with records this launch, in first-touch order
     uint4 *binrec;  // 2 x uint4 per incidence: {x0, x1, y0, y1}, {acc, tm, record index, 0}
     uint4 *active;  // per tier, nbins entries {slot, incidence start, record count, nf | ng << 16}
     uint32_t binrec_cap;
@@ -161,6 +163,50 @@ __device__ __forceinline__ DomDesc rec_bins(const KArgs &a, uint32_t r, uint4 &q
     return g;
 }
 
+// The incidences (record x bin) of 64 consecutive records, spread over the
+// wave's lanes: record j (lane j) has n_j = w_j * h_j bins, inclusive prefix
+// incl_j; incidence k belongs to the lane j with incl_{j-1} <= k < incl_j and
+// is bin (bx0_j + off % w_j, by0_j + off / w_j), off = k - incl_{j-1}.  So a
+// record covering many bins does not stall its wave.
+struct WaveRecs {
+    uint32_t r, valid, bx0, w, by0, n, incl, bin0, nbx;
+};
+
+__device__ __forceinline__ WaveRecs wave_recs(const KArgs &a, uint32_t r, uint32_t nrec)
+{
+    WaveRecs q;
+    q.r = r;
+    q.valid = r < nrec;
+    q.bx0 = q.w = q.by0 = q.n = q.bin0 = q.nbx = 0;
+    if (q.valid) {
+        uint4 q0, q1;
+        uint32_t bx0, bx1, by0, by1;
+        const DomDesc g = rec_bins(a, r, q0, q1, bx0, bx1, by0, by1);
+        q.bx0 = bx0; q.w = bx1 - bx0 + 1; q.by0 = by0; q.n = q.w * (by1 - by0 + 1);
+        q.bin0 = g.bin0; q.nbx = g.nbx;
+    }
+    q.incl = wave_incl_scan(q.n);
+    return q;
+}
+
+// owner lane of incidence k and the bin it names (all lanes call it together)
+__device__ __forceinline__ uint32_t wave_rec_bin(const WaveRecs &q, uint32_t k, uint32_t &owner)
+{
+    uint32_t lo = 0;
+#pragma unroll
+    for (uint32_t step = 32; step >= 1; step >>= 1) {
+        const uint32_t v = __shfl(q.incl, (int)(lo + step - 1), 64);
+        if (v <= k) lo += step;
+    }
+    owner = lo > 63 ? 63 : lo;
+    const uint32_t incl = __shfl(q.incl, (int)owner, 64), n = __shfl(q.n, (int)owner, 64);
+    const uint32_t bx0 = __shfl(q.bx0, (int)owner, 64), w = __shfl(q.w, (int)owner, 64);
+    const uint32_t by0 = __shfl(q.by0, (int)owner, 64), bin0 = __shfl(q.bin0, (int)owner, 64);
+    const uint32_t nbx = __shfl(q.nbx, (int)owner, 64);
+    const uint32_t off = k - (incl - n);
+    return bin0 + (by0 + off / max(w, 1u)) * nbx + bx0 + off % max(w, 1u);
+}
+
 __global__ void k_count(KArgs a)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -177,24 +223,38 @@ __global__ void k_count(KArgs a)
     if (tid == 0 && a.nrec_arg != NONE) a.ctr->nrec = a.nrec_arg;  // for the later kernels
     if (a.ctr->err) return;  // a poisoned pool touches no state
     const uint32_t nrec = a.nrec_arg != NONE ? a.nrec_arg : a.ctr->nrec;
-    for (uint32_t r = tid; r < nrec; r += gridDim.x * blockDim.x) {
-        uint4 q0, q1;
-        uint32_t bx0, bx1, by0, by1;
-        const DomDesc g = rec_bins(a, r, q0, q1, bx0, bx1, by0, by1);
-        for (uint32_t by = by0; by <= by1; ++by)
-            for (uint32_t bx = bx0; bx <= bx1; ++bx) atomicAdd(&a.bin_cnt[g.bin0 + by * g.nbx + bx], 1u);
+    const uint32_t lane = lane_id();
+    for (uint32_t r0 = tid & ~63u; r0 < nrec; r0 += gridDim.x * blockDim.x) {
+        const WaveRecs q = wave_recs(a, r0 + lane, nrec);
+        const uint32_t total = __shfl(q.incl, 63, 64);
+        for (uint32_t k0 = 0; k0 < total; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            uint32_t owner;
+            const uint32_t bin = wave_rec_bin(q, min(k, total - 1), owner);
+            const bool first = k < total && atomicAdd(&a.bin_cnt[bin], 1u) == 0;
+            // the bins this launch touches, listed once each
+            const uint64_t m = __ballot(first);
+            if (m) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&a.ctr->ntouched, (uint32_t)__popcll(m));
+                base = wave_bcast(base, 0);
+                if (first) a.touched[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = bin;
+            }
+        }
     }
 }
 
-// per-bin segments of the incidence array; each active bin goes to the
-// smallest LDS tier whose capacities fit it with headroom
+// per-bin segments of the incidence array over the touched bins; each active
+// bin goes to the smallest LDS tier whose capacities fit it with headroom
 __global__ void k_alloc(KArgs a, uint32_t nbins)
 {
     if (a.ctr->err) return;
-    // grid-stride over 64-bin chunks, one wave per chunk
-    for (uint32_t s0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); s0 < nbins; s0 += gridDim.x * blockDim.x) {
-        const uint32_t slot = s0 + lane_id();
-        const uint32_t c = slot < nbins ? a.bin_cnt[slot] : 0u;
+    const uint32_t nt = min(a.ctr->ntouched, nbins);
+    // grid-stride over 64-entry chunks of the touched list, one wave per chunk
+    for (uint32_t s0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); s0 < nt; s0 += gridDim.x * blockDim.x) {
+        const uint32_t ti = s0 + lane_id();
+        const uint32_t slot = ti < nt ? a.touched[ti] : 0u;
+        const uint32_t c = ti < nt ? a.bin_cnt[slot] : 0u;
         const uint32_t incl = wave_incl_scan(c);
         const uint32_t sum = wave_bcast(incl, 63);
         if (__ballot(c > 0) == 0) continue;
@@ -235,19 +295,24 @@ __global__ void k_fill(KArgs a)
     if (a.ctr->err) return;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nrec = a.ctr->nrec;
-    for (uint32_t r = tid; r < nrec; r += gridDim.x * blockDim.x) {
-        uint4 q0, q1;
-        uint32_t bx0, bx1, by0, by1;
-        const DomDesc g = rec_bins(a, r, q0, q1, bx0, bx1, by0, by1);
-        const uint4 inc1 = make_uint4(q1.x, q1.y, r, 0u);
-        for (uint32_t by = by0; by <= by1; ++by)
-            for (uint32_t bx = bx0; bx <= bx1; ++bx) {
-                const uint32_t pos = atomicAdd(&a.bin_cur[g.bin0 + by * g.nbx + bx], 1u);
+    const uint32_t lane = lane_id();
+    for (uint32_t r0 = tid & ~63u; r0 < nrec; r0 += gridDim.x * blockDim.x) {
+        const WaveRecs q = wave_recs(a, r0 + lane, nrec);
+        const uint32_t total = __shfl(q.incl, 63, 64);
+        for (uint32_t k0 = 0; k0 < total; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            uint32_t owner;
+            const uint32_t bin = wave_rec_bin(q, min(k, total - 1), owner);
+            if (k < total) {
+                const uint32_t r = r0 + owner;
+                const uint32_t pos = atomicAdd(&a.bin_cur[bin], 1u);
                 if (pos < a.binrec_cap) {
-                    a.binrec[2ull * pos] = q0;
-                    a.binrec[2ull * pos + 1] = inc1;
+                    const uint4 q1 = a.rec[2ull * r + 1];
+                    a.binrec[2ull * pos] = a.rec[2ull * r];
+                    a.binrec[2ull * pos + 1] = make_uint4(q1.x, q1.y, r, 0u);
                 }
             }
+        }
     }
 }
 
@@ -1007,43 +1072,46 @@ __global__ void k_dedup(KArgs a)
     const uint32_t mask = a.hmask;  // the whole table: spread keys keep concurrent atomics on distinct lines
     const unsigned long long ebase = a.ctr->edge_base;
     const uint32_t lane = lane_id();
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t q = 0, j = i;
-        while (j >= min(a.cstr[16 * q], a.cand_cap)) {
-            j -= min(a.cstr[16 * q], a.cand_cap);
-            ++q;
-        }
-        const uint32_t *c = a.cand + 3ull * ((size_t)q * a.cand_cap + j);
-        const uint32_t p = c[0], acc = c[1], t = c[2];
-        const unsigned long long key = ((unsigned long long)p << 32) | t;
-        uint32_t h = (uint32_t)mix64(key) & mask;
-        uint32_t probe = 0;
+    // wave-uniform trip count: the edge counter update below is one per wave
+    for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + lane;
         bool isnew = false;
-        while (true) {
-            const unsigned long long old = atomicCAS(&a.hkeys[h], EMPTY64, key);
-            if (old == EMPTY64 || old == key) {
-                atomicMin(&a.hvals[h], acc);
-                isnew = old == EMPTY64;
-                break;
+        uint32_t h = 0;
+        if (i < n) {
+            uint32_t q = 0, j = i;
+            while (j >= min(a.cstr[16 * q], a.cand_cap)) {
+                j -= min(a.cstr[16 * q], a.cand_cap);
+                ++q;
             }
-            h = (h + 1) & mask;
-            if (++probe > mask) {
-                atomicOr(&a.ctr->err, ERR_HASH);
-                break;
+            const uint32_t *c = a.cand + 3ull * ((size_t)q * a.cand_cap + j);
+            const uint32_t p = c[0], acc = c[1], t = c[2];
+            const unsigned long long key = ((unsigned long long)p << 32) | t;
+            h = (uint32_t)mix64(key) & mask;
+            uint32_t probe = 0;
+            while (true) {
+                const unsigned long long old = atomicCAS(&a.hkeys[h], EMPTY64, key);
+                if (old == EMPTY64 || old == key) {
+                    atomicMin(&a.hvals[h], acc);
+                    isnew = old == EMPTY64;
+                    break;
+                }
+                h = (h + 1) & mask;
+                if (++probe > mask) {
+                    atomicOr(&a.ctr->err, ERR_HASH);
+                    break;
+                }
             }
         }
-        // one edge counter update per wave
-        const uint64_t act = __ballot(1);
         const uint64_t nm = __ballot(isnew);
-        const uint32_t leader = (uint32_t)__builtin_ctzll(act);
+        if (nm == 0) continue;
         uint32_t blo = 0, bhi = 0;
-        if (lane == leader && nm) {
+        if (lane == 0) {
             const unsigned long long b0 = atomicAdd(&a.ctr->edge_total, (unsigned long long)__popcll(nm));
             blo = (uint32_t)b0;
             bhi = (uint32_t)(b0 >> 32);
         }
-        blo = __shfl(blo, (int)leader, 64);
-        bhi = __shfl(bhi, (int)leader, 64);
+        blo = __shfl(blo, 0, 64);
+        bhi = __shfl(bhi, 0, 64);
         if (isnew) {
             const unsigned long long pos =
                 (((unsigned long long)bhi << 32) | blo) + (unsigned long long)__popcll(nm & ((1ull << lane) - 1ull));
@@ -1055,13 +1123,14 @@ __global__ void k_dedup(KArgs a)
 
 __global__ void k_final(KArgs a)
 {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) a.ctr->ntouched = 0;  // the touched list of the next launch starts empty
     if (a.ctr->err & (ERR_EDGES | ERR_HASH)) return;  // eslot is incomplete
     const unsigned long long ebase = a.ctr->edge_base;
     unsigned long long et = a.ctr->edge_total;
     if (et > a.edge_cap) et = a.edge_cap;
     unsigned long long n = et > ebase ? et - ebase : 0ull;
     if (n > (unsigned long long)a.hmask + 1) n = (unsigned long long)a.hmask + 1;
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
         unsigned long long nc = 0;
         for (uint32_t q = 0; q < CSTRIPES; ++q) nc += a.cstr[16 * q];
@@ -1094,6 +1163,7 @@ __global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint32_t b0,
         for (uint32_t i = tid; i < 4 * BLKSLOTS; i += gridDim.x * blockDim.x) blk[i] = make_uint4(0, 0, 0, 0);
     if (tid == 0 && c) {
         c->err = 0;
+        c->ntouched = 0;
         c->edge_total = 0;
         c->maxfrag = 0;
         c->maxrs = 0;
@@ -1185,7 +1255,7 @@ struct xkrt_hip_pool {
     uint32_t cap_bins = 0;
     std::vector<std::pair<uint32_t, uint32_t>> free_ranges;
     uint32_t *bin_nf = nullptr, *bin_nrs = nullptr, *frag = nullptr, *rs = nullptr;
-    uint32_t *bin_cnt = nullptr, *bin_cur = nullptr, *bin_dom = nullptr;
+    uint32_t *bin_cnt = nullptr, *bin_cur = nullptr, *bin_dom = nullptr, *touched = nullptr;
     uint4 *active = nullptr;
     // domains
     DomDesc h_dom[MAX_DOMS] = {};
@@ -1257,7 +1327,7 @@ struct xkrt_hip_dep_domain {
 
 namespace {
 
-uint64_t bin_bytes(const xkrt_hip_pool *p) { return 4ull * 7 + 20ull * p->fcap + 20ull * p->rcap + 48ull; }
+uint64_t bin_bytes(const xkrt_hip_pool *p) { return 4ull * 6 + 64ull + 20ull * p->fcap + 20ull * p->rcap; }
 
 hipEvent_t take_event(xkrt_hip_pool *p)
 {
@@ -1315,7 +1385,7 @@ int grow_bins(xkrt_hip_pool *p, uint32_t need)
         if (nc * bin_bytes(p) + p->bytes_scratch > p->cfg.budget_bytes) return -ENOMEM;
     }
     HIPCHK(hipStreamSynchronize(p->stream));
-    uint32_t *nf, *nrs, *fr, *rs, *cnt, *cur, *dom;
+    uint32_t *nf, *nrs, *fr, *rs, *cnt, *cur, *dom, *tch;
     uint4 *act;
     HIPCHK(hipMalloc(&nf, nc * 4));
     HIPCHK(hipMalloc(&nrs, nc * 4));
@@ -1324,6 +1394,7 @@ int grow_bins(xkrt_hip_pool *p, uint32_t need)
     HIPCHK(hipMalloc(&cnt, nc * 4));
     HIPCHK(hipMalloc(&cur, nc * 4));
     HIPCHK(hipMalloc(&dom, nc * 4));
+    HIPCHK(hipMalloc(&tch, nc * 4));
     HIPCHK(hipMalloc(&act, nc * 16 * 4));
     HIPCHK(hipMemsetAsync(nf, 0, nc * 4, p->stream));
     HIPCHK(hipMemsetAsync(nrs, 0, nc * 4, p->stream));
@@ -1340,9 +1411,10 @@ int grow_bins(xkrt_hip_pool *p, uint32_t need)
         HIPCHK(hipStreamSynchronize(p->stream));
         (void)hipFree(p->bin_nf); (void)hipFree(p->bin_nrs); (void)hipFree(p->frag); (void)hipFree(p->rs);
         (void)hipFree(p->bin_cnt); (void)hipFree(p->bin_cur); (void)hipFree(p->bin_dom); (void)hipFree(p->active);
+        (void)hipFree(p->touched);
     }
     p->bin_nf = nf; p->bin_nrs = nrs; p->frag = fr; p->rs = rs;
-    p->bin_cnt = cnt; p->bin_cur = cur; p->bin_dom = dom; p->active = act;
+    p->bin_cnt = cnt; p->bin_cur = cur; p->bin_dom = dom; p->active = act; p->touched = tch;
     // the new tail is free
     if (!p->free_ranges.empty() && p->free_ranges.back().first + p->free_ranges.back().second == oc)
         p->free_ranges.back().second += (uint32_t)(nc - oc);
@@ -1453,6 +1525,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.bin_nf = p->bin_nf; a.bin_nrs = p->bin_nrs; a.frag = p->frag; a.rs = p->rs;
     a.fcap = p->fcap; a.rcap = p->rcap;
     a.bin_cnt = p->bin_cnt; a.bin_cur = p->bin_cur; a.active = p->active; a.binrec = p->binrec;
+    a.touched = p->touched;
     a.binrec_cap = p->binrec_cap;
     a.nbins = p->cap_bins;
     a.max_tier = p->kclass + 1;  // tier t + 1 is class t
@@ -1503,7 +1576,7 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
                     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
     launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), 0, a);
-    launch(p, spans, 1, k_alloc, dim3(std::min<uint32_t>((p->cap_bins + 255) / 256, 4096u)), dim3(256), 0, a,
+    launch(p, spans, 1, k_alloc, dim3(std::min<uint32_t>((p->cap_bins + 255) / 256, 1024u)), dim3(256), 0, a,
            p->cap_bins);
     launch(p, spans, 2, k_fill, dim3(rblocks), dim3(256), 0, a);
     // tier 0 first; bins that outgrow a tier are retried by the next one
@@ -1631,7 +1704,8 @@ void destroy_pool(xkrt_hip_pool *p)
             (void)hipEventDestroy(b);
         }
     for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
-    void *dev[] = {p->bin_nf, p->bin_nrs, p->frag, p->rs, p->bin_cnt, p->bin_cur, p->bin_dom, p->active, p->d_dom,
+    void *dev[] = {p->bin_nf, p->bin_nrs, p->frag, p->rs, p->bin_cnt, p->bin_cur, p->bin_dom, p->active, p->touched,
+                   p->d_dom,
                    p->binrec, p->cand, p->cstr, p->blk, p->hkeys, p->hvals, p->eslot, p->edges, p->edom, p->ctr, p->rec_dev};
     for (void *q : dev)
         if (q) (void)hipFree(q);
@@ -1843,7 +1917,7 @@ int stage(xkrt_hip_pool *p, xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc
     if (first_id) *first_id = d->next_id;
     if (n == 0) return 0;
     if (n > 0xFFFFFFFFull || d->next_id + n > 0xFFFFFFFFull) return -ERANGE;
-    if (p->open_nrec + 3 * n > p->rec_cap) return p->open_nrec ? -ENOSPC : -EINVAL;
+    if (p->open_nrec + n > p->rec_cap) return p->open_nrec ? -ENOSPC : -EINVAL;
     // the staging buffer must be free: wait for the previous upload
     if (p->staged_pending) {
         HIPCHK(hipEventSynchronize(p->staged));
@@ -1851,13 +1925,16 @@ int stage(xkrt_hip_pool *p, xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc
     }
     uint64_t nrect = 0;
     uint32_t ylo = 0, yhi = 0, ntask = 0;
-    const uint64_t saved_max = d->max_xlen;
-    const bool was_framed = d->framed;
-    int rc = build_records(d, acc, n, (uint32_t)d->next_id, p->open_tasks, p->rec_cap - p->open_nrec,
+    const uint64_t saved_max = d->max_xlen, room = p->rec_cap - p->open_nrec;
+    const bool was_framed = d->framed, saved_col0 = d->col0;
+    int rc = build_records(d, acc, n, (uint32_t)d->next_id, p->open_tasks, room,
                            (xkrt_hip_rect_t *)p->rec_host + p->open_nrec, &nrect, &ylo, &yhi, &ntask);
+    // the batch's rects (1 to 3 per access) must fit the open launch whole
+    if (!rc && nrect > room) rc = p->open_nrec ? -ENOSPC : -EINVAL;
     if (!rc) rc = ensure_grid(d, ylo, yhi);
     if (rc) {
         d->max_xlen = saved_max;
+        d->col0 = saved_col0;
         if (!was_framed) d->framed = false;
         return rc;
     }
