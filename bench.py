@@ -214,8 +214,10 @@ def serving_leg(fracs=(0.1, 0.3, 0.5, 0.7, 0.9)):
     pool with a 16 GB budget.  Saturation (closed loop) of the batched form
     (continuous batching, hipGraph launches) and of the plain form (one launch
     per batch); then open-loop Poisson arrivals at fractions of the batched
-    saturation rate, both forms on the same schedule.  Latency = batch arrival
-    -> its launch completing, task-weighted; host staging included."""
+    saturation rate, both forms on the same schedule.  The serving engine is
+    the library's native host thread; batches are submitted at their arrival
+    times.  Latency = scheduled arrival -> the engine seeing the launch that
+    resolved it complete, task-weighted; host staging and upload included."""
     from xkaapi_amd import serving as V
     cs = V.make_clients(64, 4, 48, 128, 256)
     nb = sum(len(c.batches) for c in cs)
@@ -231,7 +233,8 @@ def serving_leg(fracs=(0.1, 0.3, 0.5, 0.7, 0.9)):
         for m in ("batched", "plain"):
             r = V.serve(cs, sched, m)
             row[m] = {"p50_us": round(r.pct(50), 1), "p99_us": round(r.pct(99), 1),
-                      "accesses_per_s": round(r.rate, 1), "launches": r.launches}
+                      "accesses_per_s": round(r.rate, 1), "launches": r.launches,
+                      "driver_lag_p99_us": round(float(np.percentile(r.lag_us, 99)), 1)}
         out["sweep"].append(row)
         log(f"serving load {f}: {row}")
     return out

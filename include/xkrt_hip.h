@@ -212,6 +212,10 @@ typedef struct xkrt_hip_pool_config {
     uint64_t max_rects;     /* rect records per launch (sizes the staging and scratch buffers) */
     uint64_t max_edges;     /* edge buffer capacity before a drain is required */
     uint64_t budget_bytes;  /* device memory budget: bins of every domain + scratch; 0 = unlimited */
+    uint32_t direct_dedup;  /* 1: each bin's candidates go straight into the launch's dedup table (one
+                               kernel less per launch: for small, latency-bound launches); 0: they are
+                               staged and deduplicated by a kernel of their own (large launches) */
+    uint32_t pad;
 } xkrt_hip_pool_config_t;
 
 typedef struct xkrt_hip_pool_stats {
@@ -259,6 +263,70 @@ int xkrt_hip_pool_launch_times(xkrt_hip_pool_t p, float *ms, uint64_t cap, uint6
 void *xkrt_hip_pool_stream(xkrt_hip_pool_t p);
 /* The domain's index in its pool (the dom field of records and edges). */
 int xkrt_hip_dep_index(xkrt_hip_dep_domain_t d, uint32_t *idx);
+
+/* ------------------------------------------------------------------------
+ * Serving engine: a host thread that drives one pool for many clients.
+ * Clients submit batches (whole tasks, spawn order) from any thread; the
+ * engine admits a client's domain against the pool's budget at its first
+ * batch (clients wait while the budget is full), stages every arrived batch
+ * of admitted clients into one launch whenever fewer than `depth` launches
+ * are in flight (continuous batching; max_batches = 1 gives the plain
+ * per-batch form), retires launches as they complete, copies each launch's
+ * edges out per client, and releases a closed client's bins once its last
+ * batch is resolved.  Submit-to-ready times are stamped by the engine.
+ * While an engine runs, the pool's own stage / launch / edge calls return
+ * -EBUSY.  No counterpart in the reference (its resolver runs inline in the
+ * spawning thread, src/task/task.cc:285-299).
+ * ------------------------------------------------------------------------ */
+
+typedef struct xkrt_hip_server *xkrt_hip_server_t;
+
+typedef struct xkrt_hip_server_config {
+    uint32_t depth;          /* launches in flight, 1..16 (0 = 2) */
+    uint32_t max_batches;    /* batches per launch; 0 = every arrived batch that fits */
+    uint64_t max_rects;      /* soft cap of rects per launch; 0 = the pool's max_rects */
+    uint64_t reserve_bytes;  /* bin memory allocated up front (within the budget), so
+                                admission does not allocate while serving */
+} xkrt_hip_server_config_t;
+
+typedef struct xkrt_hip_server_stats {
+    uint64_t launches;
+    uint64_t batches;        /* batches resolved */
+    uint64_t admitted;       /* client domains admitted */
+    uint64_t left;           /* clients that left */
+    uint64_t waited;         /* admission attempts refused for lack of budget */
+    uint64_t harvests;       /* edge copies */
+    uint64_t consumes;       /* edge buffer resets */
+    int64_t error;           /* first pool-level error (negative errno), 0 if none */
+} xkrt_hip_server_stats_t;
+
+int xkrt_hip_server_create(xkrt_hip_pool_t p, const xkrt_hip_server_config_t *cfg, xkrt_hip_server_t *s);
+/* Stops the engine (after the launches in flight), releases every client's
+ * domain; the pool stays usable. */
+int xkrt_hip_server_destroy(xkrt_hip_server_t s);
+/* Registers a client: a domain for (ld, sizeof_type) over columns
+ * [y_lo, y_hi) with the given bin extents, admitted at its first batch. */
+int xkrt_hip_server_client(xkrt_hip_server_t s, uint64_t ld, uint64_t sizeof_type, uint32_t bin_x_bytes,
+                           uint32_t bin_y_cols, uint64_t y_lo, uint64_t y_hi, uint32_t *client);
+/* Queues a copy of n accesses of the client; *request = its number (0, 1, ...
+ * over all clients).  The client's access ids continue from its previous
+ * batch, from 0. */
+int xkrt_hip_server_submit(xkrt_hip_server_t s, uint32_t client, const xkrt_hip_access_t *acc, uint64_t n,
+                           uint64_t *request);
+/* The client submits no more batches; it leaves after the last one. */
+int xkrt_hip_server_close(xkrt_hip_server_t s, uint32_t client);
+/* Waits until every submitted batch is resolved (or refused) and every
+ * closed client has left; -ETIMEDOUT after timeout_us (0 = no limit). */
+int xkrt_hip_server_drain(xkrt_hip_server_t s, uint64_t timeout_us);
+/* Per request first .. first + n - 1: microseconds since the engine started
+ * at submit and at ready (the engine saw its launch complete), and status
+ * (1 pending, 0 resolved, negative errno: refused). */
+int xkrt_hip_server_times(xkrt_hip_server_t s, uint64_t first, uint64_t n, double *submit_us, double *ready_us,
+                          int32_t *status);
+/* The client's edges so far: (pred id, succ id) pairs in its own access ids;
+ * *n = pairs available (up to cap copied). */
+int xkrt_hip_server_edges(xkrt_hip_server_t s, uint32_t client, uint32_t *edges, uint64_t cap, uint64_t *n);
+int xkrt_hip_server_stats(xkrt_hip_server_t s, xkrt_hip_server_stats_t *out);
 
 /* ------------------------------------------------------------------------
  * Coherency fetch planner: one memory tree per (ld, sizeof_type), a host

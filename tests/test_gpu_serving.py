@@ -62,3 +62,38 @@ def test_admission_limited_mix(clients):
     bins = (big.ntiles * 128 * 8 // 1024) * ((hi - lo + 127) // 128 + 1)
     res = V.saturation(clients, "batched", max_rects=1 << 14, budget_bytes=base + int(2.5 * bins * bpb))
     check(res, clients)
+
+
+def test_edge_buffer_reset_under_load(clients):
+    # a small edge buffer: the engine copies edges out and resets the buffer several times
+    res = V.saturation(clients, "plain", max_rects=1 << 12, max_edges=2048)
+    check(res, clients)
+    assert res.stats["consumes"] >= 1 and res.stats["error"] == 0
+
+
+def test_refused_client_fails_alone(clients):
+    # client 0's frame is one column wide: its first batch is refused (-ERANGE), and so are its
+    # later ones; the other clients are served as alone
+    pool = R.Pool(kernel=1, max_rects=1 << 14)
+    try:
+        with R.Server(pool) as srv:
+            ids, reqs = {}, {}
+            for c in clients:
+                lo, hi = V.col_range(c.stream)
+                rng = (lo, lo + 1) if c.cid == clients[0].cid else (lo, hi)
+                ids[c.cid] = srv.client(int(c.stream.ld[0]), int(c.stream.s[0]), rng)
+            for c in clients:
+                reqs[c.cid] = [srv.submit(ids[c.cid], c.acc[a:e]) for a, e in c.batches]
+                srv.close_client(ids[c.cid])
+            empty = srv.client(64, 8, (0, 64))  # registered and closed without a batch
+            srv.close_client(empty)
+            srv.drain(timeout_s=60)
+            _, _, st = srv.times()
+            assert set(st[reqs[clients[0].cid]].tolist()) == {-34}
+            for c in clients[1:]:
+                assert (st[reqs[c.cid]] == 0).all()
+                assert np.array_equal(unique_rows(srv.edges(ids[c.cid])), c.want)
+            s = srv.stats()
+            assert s["left"] == len(clients) + 1 and s["error"] == 0
+    finally:
+        pool.close()

@@ -13,7 +13,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("XKRT_HIP_ARCH", "gfx950")
 
 # hipcc flags for every HIP source: gfx950 only, no packed-FP32 instruction forms
-HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
+HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 
 HIP_SOURCES = ["resolver.hip", "coherency.hip"]

@@ -24,11 +24,15 @@
 //             -- LINK of every access of a task, then PUT -- emitting
 //             (pred, succ access, succ task) candidates; W paints over the
 //             writer map, an exact R appends to the reader log; the bin is
-//             written back compacted
-//   k_dedup   candidates -> open-addressing table keyed (pred, succ task),
-//             keeping the lowest succ access id (= lowest slot, since LINK
-//             runs in slot order)
-//   k_final   table entries -> (pred id, succ id) edges; table entries reset
+//             written back compacted.  Candidates are deduplicated per bin
+//             in LDS, then either staged in a candidate buffer (throughput
+//             form) or inserted into the launch's table at once (direct
+//             form: one kernel less per launch, for small launches)
+//   k_dedup   (staged form) candidates -> open-addressing table keyed
+//             (pred, succ task), keeping the lowest succ access id
+//             (= lowest slot, since LINK runs in slot order)
+//   k_final   table entries -> (pred id, succ id) edges; table entries reset;
+//             the launch's edge count and error bits to pinned host memory
 //
 // Because every fragment of a bin carries a uniform state, the edges found
 // per bin are exactly the per-point rule; the union over bins followed by
@@ -44,8 +48,14 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <tuple>
+#include <vector>
 #include <vector>
 
 #include "../../include/xkrt_hip.h"
@@ -60,13 +70,13 @@ enum : uint32_t {
     ERR_LIST = 2u,      // a bin got more records than the kernel's LDS list holds
     ERR_FRAG = 4u,      // a bin's fragment list overflowed
     ERR_READS = 8u,     // a bin's reader pool overflowed
-    ERR_CAND = 16u,     // candidate buffer overflow
+    ERR_CAND = 16u,     // candidate buffer overflow (staged form)
     ERR_HASH = 32u,     // dedup table full
     ERR_EDGES = 64u,    // edge buffer overflow
 };
 
 struct DevCounters {
-    unsigned int nrec;       // rect records of this launch (set before the launch)
+    unsigned int nrec;       // rect records of this launch (written by k_count)
     unsigned int ntouched;   // bins that got records this launch (listed by k_count; zeroed by k_final)
     unsigned int total;      // incidences allocated this launch
     unsigned int ntier[4];   // active bins per LDS tier this launch (heavy ones, from the list front)
@@ -79,7 +89,18 @@ struct DevCounters {
     unsigned long long edge_base;    // edge_total when the launch started
     unsigned long long cand_total;   // stats
     unsigned long long frag_visits;  // stats
+    unsigned long long nlaunch;      // launches completed (k_final counts them)
 };
+
+// What the host learns of a launch without a stream sync: k_final writes it to
+// pinned host memory, slot = launch number % NSNAP.
+struct LaunchSnap {
+    unsigned long long launch;      // launch number (1, 2, ...)
+    unsigned long long edge_total;  // unconsumed edges after it
+    unsigned int err;               // sticky error bits after it
+    unsigned int pad[3];
+};
+constexpr uint32_t NSNAP = 64;
 
 // one dependency domain's bin grid inside the pool's bin arrays
 struct DomDesc {
@@ -93,7 +114,8 @@ struct DomDesc {
 
 struct KArgs {
     const uint4 *rec;         // 2 x uint4 per rect record: {x0, x1, y0, y1}, {acc, tm, domain, 0}
-    uint32_t nrec_arg;        // records of this launch, or NONE: already in the counters
+    uint32_t nrec_arg;        // records of this launch, or NONE: read from *nrec_ptr
+    const uint32_t *nrec_ptr; // record count uploaded with the records (a header 32 B before them)
     const DomDesc *dom;       // domain table
     const uint32_t *bin_dom;  // domain of each pool bin
     uint32_t *bin_nf, *bin_nrs;
@@ -108,9 +130,9 @@ struct KArgs {
     uint32_t nbins;
     uint32_t max_tier;                              // tiers 0..max_tier; storage sized for max_tier
     uint32_t tier_f[4], tier_g[4], tier_l[4];       // per tier: fragments, log entries, records
-    uint32_t *cand;
+    uint32_t *cand;           // staged form: CSTRIPES stripes of cand_cap candidates (3 u32 each)
     uint32_t cand_cap;        // per stripe
-    uint32_t *cstr;           // CSTRIPES counters, 16 u32 apart
+    uint32_t *cstr;           // CSTRIPES candidate counters, 16 u32 apart
     uint4 *blk;               // per (tier, block): max fragments, max log entries, fragment visits (lo, hi)
     unsigned long long *hkeys;
     uint32_t *hvals;
@@ -120,6 +142,7 @@ struct KArgs {
     uint32_t *edom;           // domain of each edge
     unsigned long long edge_cap;
     DevCounters *ctr;
+    LaunchSnap *snap;         // pinned host ring, NSNAP slots
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
@@ -220,9 +243,9 @@ __global__ void k_count(KArgs a)
         a.ctr->edge_base = a.ctr->edge_total;
         for (uint32_t q = 0; q < CSTRIPES; ++q) a.cstr[16 * q] = 0;
     }
-    if (tid == 0 && a.nrec_arg != NONE) a.ctr->nrec = a.nrec_arg;  // for the later kernels
+    const uint32_t nrec = a.nrec_arg != NONE ? a.nrec_arg : *a.nrec_ptr;
+    if (tid == 0) a.ctr->nrec = nrec;  // for the later kernels
     if (a.ctr->err) return;  // a poisoned pool touches no state
-    const uint32_t nrec = a.nrec_arg != NONE ? a.nrec_arg : a.ctr->nrec;
     const uint32_t lane = lane_id();
     for (uint32_t r0 = tid & ~63u; r0 < nrec; r0 += gridDim.x * blockDim.x) {
         const WaveRecs q = wave_recs(a, r0 + lane, nrec);
@@ -328,7 +351,7 @@ __global__ void k_fill(KArgs a)
 // task) dedup set, the candidate staging buffer (3 u32 each) and the readers
 // of one fragment kept for the exact coverage test; the small tier keeps
 // them small so that more bins are in flight per CU.
-template <int FCAP, int GCAP, int LCAP>
+template <int FCAP, int GCAP, int LCAP, bool DIR>
 struct BinLds {
     static constexpr uint32_t DSZ = LCAP >= 1024 ? 1024u : (LCAP >= 512 ? 512u : (LCAP >= 256 ? 256u : 64u));
     static constexpr uint32_t CCAP = LCAP >= 256 ? 256u : 64u;
@@ -366,8 +389,8 @@ struct BinState {
 
 // The reader log is read only by a writer's LINK and by pruning; bins whose
 // records only read append to it without loading it.
-template <int FCAP, int GCAP, int LCAP>
-__device__ void ensure_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st)
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ void ensure_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s, BinState &st)
 {
     if (st.gload) return;
     const uint32_t *gr = a.rs + (size_t)st.slot * a.rcap * 5;
@@ -380,34 +403,82 @@ __device__ void ensure_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState
     __syncthreads();
 }
 
-template <int FCAP, int GCAP, int LCAP>
-__device__ void flush_cands(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s)
+// Inserts candidate (pred p, successor access acc, task with domain bits)
+// into the launch's dedup table, keeping the lowest acc per key.  Returns the
+// table slot when the key is new, NONE otherwise.
+__device__ __forceinline__ uint32_t hash_insert(const KArgs &a, uint32_t p, uint32_t acc, uint32_t task)
+{
+    const unsigned long long key = ((unsigned long long)p << 32) | task;
+    const uint32_t mask = a.hmask;
+    uint32_t h = (uint32_t)mix64(key) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const unsigned long long old = atomicCAS(&a.hkeys[h], EMPTY64, key);
+        if (old == EMPTY64 || old == key) {
+            atomicMin(&a.hvals[h], acc);
+            return old == EMPTY64 ? h : NONE;
+        }
+        h = (h + 1) & mask;
+    }
+    atomicOr(&a.ctr->err, ERR_HASH);
+    return NONE;
+}
+
+// the new key in table slot h takes edge position pos of this launch
+__device__ __forceinline__ void edge_at(const KArgs &a, unsigned long long pos, uint32_t h)
+{
+    const unsigned long long ebase = a.ctr->edge_base;
+    if (pos >= a.edge_cap || pos - ebase > a.hmask) atomicOr(&a.ctr->err, ERR_EDGES);
+    else a.eslot[pos - ebase] = h;
+}
+
+// the staged candidates of this block into the candidate buffer (staged
+// form) or the dedup table (direct form: one edge counter update per 64)
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ void flush_cands(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s)
 {
     __syncthreads();
     const uint32_t n = s.ccount;
-    if (n) {
-        const uint32_t stripe = blockIdx.x % CSTRIPES;
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&a.cstr[16 * stripe], n);
+    const uint32_t lane = lane_id();
+    const uint32_t stripe = blockIdx.x % CSTRIPES;
+    uint32_t base = 0;
+    if (n && lane == 0) base = atomicAdd(&a.cstr[16 * stripe], n);
+    if (!DIR && n) {  // staged form: the candidates go to the buffer for k_dedup
         base = wave_bcast(base, 0);
         if ((uint64_t)base + n > a.cand_cap) {
-            if (lane_id() == 0) atomicOr(&a.ctr->err, ERR_CAND);
+            if (lane == 0) atomicOr(&a.ctr->err, ERR_CAND);
         } else {
             uint32_t *dst = a.cand + 3ull * ((size_t)stripe * a.cand_cap + base);
-            for (uint32_t i = lane_id(); i < 3 * n; i += 64) dst[i] = s.C[i];
+            for (uint32_t i = lane; i < 3 * n; i += 64) dst[i] = s.C[i];
         }
     }
+    for (uint32_t i0 = 0; DIR && i0 < n; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const uint32_t h = i < n ? hash_insert(a, s.C[3 * i], s.C[3 * i + 1], s.C[3 * i + 2]) : NONE;
+        const uint64_t nm = __ballot(h != NONE);
+        if (nm == 0) continue;
+        uint32_t blo = 0, bhi = 0;
+        if (lane == 0) {
+            const unsigned long long b0 = atomicAdd(&a.ctr->edge_total, (unsigned long long)__popcll(nm));
+            blo = (uint32_t)b0;
+            bhi = (uint32_t)(b0 >> 32);
+        }
+        blo = __shfl(blo, 0, 64);
+        bhi = __shfl(bhi, 0, 64);
+        if (h != NONE)
+            edge_at(a, (((unsigned long long)bhi << 32) | blo) + (unsigned long long)__popcll(nm & ((1ull << lane) - 1ull)),
+                    h);
+    }
     __syncthreads();
-    if (lane_id() == 0) s.ccount = 0;
+    if (lane == 0) s.ccount = 0;
     __syncthreads();
 }
 
 // room for one candidate per lane in the staging buffer
-template <int FCAP, int GCAP, int LCAP>
-__device__ __forceinline__ void room(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s)
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ __forceinline__ void room(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s)
 {
     __syncthreads();
-    if (s.ccount + 64 > BinLds<FCAP, GCAP, LCAP>::CCAP) flush_cands(a, s);
+    if (s.ccount + 64 > BinLds<FCAP, GCAP, LCAP, DIR>::CCAP) flush_cands(a, s);
 }
 
 // Record predecessor p of access acc (task `task`): the run's dedup set keeps
@@ -415,13 +486,13 @@ __device__ __forceinline__ void room(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s
 // the reference, so the lowest access id is the slot that owns the edge.  When
 // the set is full the candidate goes straight to the global candidate buffer
 // (the cross-bin dedup keeps the minimum too).
-template <int FCAP, int GCAP, int LCAP>
-__device__ __forceinline__ void emit(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, uint32_t p, uint32_t acc,
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ __forceinline__ void emit(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s, uint32_t p, uint32_t acc,
                                      uint32_t task)
 {
     task |= s.tdom;  // (pred, task) keys of different domains never meet
     const unsigned long long key = ((unsigned long long)p << 32) | task;
-    constexpr uint32_t DSZ = BinLds<FCAP, GCAP, LCAP>::DSZ;
+    constexpr uint32_t DSZ = BinLds<FCAP, GCAP, LCAP, DIR>::DSZ;
     uint32_t h = (uint32_t)(mix64(key) & (DSZ - 1));
     for (uint32_t probe = 0; probe < 32; ++probe) {
         const unsigned long long old = atomicCAS(&s.D[h], EMPTY64, key);
@@ -432,9 +503,13 @@ __device__ __forceinline__ void emit(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s
         }
         h = (h + 1) & (DSZ - 1);
     }
+    // the bin's set is crowded: this candidate goes on alone
     const uint32_t stripe = blockIdx.x % CSTRIPES;
     const uint32_t pos = atomicAdd(&a.cstr[16 * stripe], 1u);
-    if (pos < a.cand_cap) {
+    if (DIR) {
+        const uint32_t slot = hash_insert(a, p, acc, task);
+        if (slot != NONE) edge_at(a, atomicAdd(&a.ctr->edge_total, 1ull), slot);
+    } else if (pos < a.cand_cap) {
         uint32_t *dst = a.cand + 3ull * ((size_t)stripe * a.cand_cap + pos);
         dst[0] = p;
         dst[1] = acc;
@@ -445,8 +520,8 @@ __device__ __forceinline__ void emit(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s
 }
 
 // move the run's (pred, task, lowest successor) entries to the staging buffer
-template <int FCAP, int GCAP, int LCAP>
-__device__ void drain_dedup(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s)
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ void drain_dedup(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s)
 {
     __syncthreads();
     const uint32_t n = s.dcount;
@@ -480,8 +555,8 @@ __device__ __forceinline__ bool overlaps(uint32_t ax0, uint32_t ax1, uint32_t ay
 // LINK (dependency-tree.hpp:311-355).  Per point x of the rect: a writer links
 // to the readers since the last writer of x when there are some, else to the
 // last writer; any other access links to the last writer.
-template <int FCAP, int GCAP, int LCAP>
-__device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1,
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s, BinState &st, uint32_t X0, uint32_t X1,
                         uint32_t Y0, uint32_t Y1, uint32_t mode, uint32_t acc, uint32_t task)
 {
     const uint32_t fb = st.fb;
@@ -564,7 +639,7 @@ __device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &s
                 const uint64_t km = __ballot(in);
                 if (in) {
                     const uint32_t kk = nk + (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
-                    if (kk < BinLds<FCAP, GCAP, LCAP>::KCAP) {
+                    if (kk < BinLds<FCAP, GCAP, LCAP, DIR>::KCAP) {
                         s.kx0[kk] = max(s.gx0[g], rx0); s.kx1[kk] = min(s.gx1[g], rx1);
                         s.ky0[kk] = max(s.gy0[g], ry0); s.ky1[kk] = min(s.gy1[g], ry1);
                     }
@@ -572,7 +647,7 @@ __device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &s
                 nk += (uint32_t)__popcll(km);
             }
             __syncthreads();  // kx/ky rows are read by other lanes below
-            if (nk > BinLds<FCAP, GCAP, LCAP>::KCAP) {
+            if (nk > BinLds<FCAP, GCAP, LCAP, DIR>::KCAP) {
                 st.err |= ERR_READS;
                 return;
             }
@@ -601,15 +676,15 @@ __device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &s
     __syncthreads();
 }
 
-template <int FCAP, int GCAP, int LCAP>
-__device__ void prune_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st);
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ void prune_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s, BinState &st);
 
 // A run of whole task groups none of whose records writes: LINK of a
 // non-writer reads only the writer map, and PUT of a reader only appends to
 // the log, so every record of the run links in parallel (one per lane) and
 // the exact readers are then logged in spawn order.
-template <int FCAP, int GCAP, int LCAP>
-__device__ void do_run(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t i0, uint32_t i1)
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ void do_run(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s, BinState &st, uint32_t i0, uint32_t i1)
 {
     const uint32_t fb = st.fb;
     const uint32_t lane = lane_id();
@@ -658,8 +733,8 @@ __device__ void do_run(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st
 // pieces of f minus the rect survive, keeping f's writer) and the rect,
 // clipped to the bin, becomes one fragment.  Logged readers are not touched:
 // their ids are now older than the rect's writer.
-template <int FCAP, int GCAP, int LCAP>
-__device__ void do_paint(BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ void do_paint(BinLds<FCAP, GCAP, LCAP, DIR> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
                          uint32_t Y1, uint32_t acc)
 {
     const uint32_t fb = st.fb, nb = fb ^ 1u;
@@ -706,8 +781,8 @@ __device__ void do_paint(BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0,
 
 // drop the logged readers that no point still sees: every fragment they
 // touch has a newer writer
-template <int FCAP, int GCAP, int LCAP>
-__device__ void prune_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st)
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ void prune_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s, BinState &st)
 {
     ensure_log(a, s, st);
     st.gdirty = 0;  // compaction moves entries
@@ -739,8 +814,8 @@ __device__ void prune_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState 
 }
 
 // PUT of an exact reader (dependency-tree.hpp:243-244): log it
-template <int FCAP, int GCAP, int LCAP>
-__device__ void do_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ void do_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s, BinState &st, uint32_t X0, uint32_t X1, uint32_t Y0,
                        uint32_t Y1, uint32_t acc)
 {
     if (st.ng >= (uint32_t)GCAP) {
@@ -762,7 +837,7 @@ __device__ void do_log(const KArgs &a, BinLds<FCAP, GCAP, LCAP> &s, BinState &st
 // written back, its record list stays -- and queued for the next tier; the
 // candidates it staged before are a subset of what the retry emits, and the
 // cross-bin dedup absorbs the repeats.  The last tier reports the overflow.
-template <int FCAP, int GCAP, int LCAP>
+template <int FCAP, int GCAP, int LCAP, bool DIR>
 __device__ __forceinline__ void escalate(const KArgs &a, uint32_t tier, uint4 e, uint32_t err)
 {
     if (lane_id() != 0) return;
@@ -801,11 +876,11 @@ __device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p)
     (void)ng0;  // the reader log is loaded on first use (ensure_log)
 }
 
-template <int FCAP, int GCAP, int LCAP>
+template <int FCAP, int GCAP, int LCAP, bool DIR>
 __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
 {
     extern __shared__ unsigned char lds_raw[];
-    BinLds<FCAP, GCAP, LCAP> &s = *reinterpret_cast<BinLds<FCAP, GCAP, LCAP> *>(lds_raw);
+    BinLds<FCAP, GCAP, LCAP, DIR> &s = *reinterpret_cast<BinLds<FCAP, GCAP, LCAP, DIR> *>(lds_raw);
     const uint32_t lane = lane_id();
     if (a.ctr->err) return;  // a poisoned domain touches no state
     const uint32_t nheavy = a.ctr->ntier[tier];
@@ -837,7 +912,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
         const uint32_t slot = e.x, start = e.y, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
         if (cnt > (uint32_t)LCAP || nf0 > (uint32_t)FCAP || ng0 > (uint32_t)GCAP) {
             advance();
-            escalate<FCAP, GCAP, LCAP>(a, tier, e, ERR_LIST);
+            escalate<FCAP, GCAP, LCAP, DIR>(a, tier, e, ERR_LIST);
             continue;
         }
         const uint32_t dm = e.w >> 24;
@@ -940,7 +1015,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
             __syncthreads();
         }
         if (lane == 0) s.dcount = 0;
-        for (uint32_t q = lane; q < BinLds<FCAP, GCAP, LCAP>::DSZ; q += 64) {
+        for (uint32_t q = lane; q < BinLds<FCAP, GCAP, LCAP, DIR>::DSZ; q += 64) {
             s.D[q] = EMPTY64;
             s.Dv[q] = NONE;
         }
@@ -961,7 +1036,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
                 while (k > i && (s.rtm[k - 1] >> 8) == (s.rtm[w] >> 8)) --k;
             if (k > i) {
                 do_run(a, s, st, i, k);
-                if (s.dcount > BinLds<FCAP, GCAP, LCAP>::DSZ / 2) drain_dedup(a, s);
+                if (s.dcount > BinLds<FCAP, GCAP, LCAP, DIR>::DSZ / 2) drain_dedup(a, s);
                 visits += (unsigned long long)st.nf * (k - i);
                 i = k;
                 continue;
@@ -994,7 +1069,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
                     do_link(a, s, st, s.rx0[i], s.rx1[i], s.ry0[i], s.ry1[i], s.rtm[i] & 0xffu, s.racc[i], task);
                     for (uint32_t q = i + 1 + lane; q < i + m; q += 64) emit(a, s, s.racc[q - 1], s.racc[q], s.rtm[q] >> 8);
                     __syncthreads();
-                    if (s.dcount > BinLds<FCAP, GCAP, LCAP>::DSZ / 2) drain_dedup(a, s);
+                    if (s.dcount > BinLds<FCAP, GCAP, LCAP, DIR>::DSZ / 2) drain_dedup(a, s);
                     const uint32_t last = i + m - 1;
                     do_paint(s, st, s.rx0[last], s.rx1[last], s.ry0[last], s.ry1[last], s.racc[last]);
                     visits += (unsigned long long)st.nf * (m + 1);
@@ -1007,7 +1082,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
                 visits += st.nf;
             }
             __syncthreads();
-            if (s.dcount > BinLds<FCAP, GCAP, LCAP>::DSZ / 2) drain_dedup(a, s);
+            if (s.dcount > BinLds<FCAP, GCAP, LCAP, DIR>::DSZ / 2) drain_dedup(a, s);
             for (uint32_t q = i; q < j && !st.err; ++q) {
                 const uint32_t mode = s.rtm[q] & 0xffu;
                 visits += st.nf;
@@ -1025,7 +1100,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
         }
         advance();  // before the write-back stores
         if (st.err) {
-            escalate<FCAP, GCAP, LCAP>(a, tier, e, st.err);
+            escalate<FCAP, GCAP, LCAP, DIR>(a, tier, e, st.err);
             __syncthreads();
             continue;
         }
@@ -1124,7 +1199,15 @@ __global__ void k_dedup(KArgs a)
 __global__ void k_final(KArgs a)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid == 0) a.ctr->ntouched = 0;  // the touched list of the next launch starts empty
+    if (tid == 0) {
+        a.ctr->ntouched = 0;  // the touched list of the next launch starts empty
+        // the launch's outcome to pinned host memory (seen once the launch's completion is)
+        const unsigned long long seq = ++a.ctr->nlaunch;
+        LaunchSnap *o = a.snap + seq % NSNAP;
+        __hip_atomic_store(&o->edge_total, a.ctr->edge_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&o->err, a.ctr->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&o->launch, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     if (a.ctr->err & (ERR_EDGES | ERR_HASH)) return;  // eslot is incomplete
     const unsigned long long ebase = a.ctr->edge_base;
     unsigned long long et = a.ctr->edge_total;
@@ -1220,10 +1303,11 @@ const KClass KCLASSES[3] = {{64, 128, 256}, {256, 512, 512}, {1024, 2048, 1024}}
 const KClass TIERS[4] = {{64, 128, 64}, {64, 128, 256}, {256, 512, 512}, {1024, 2048, 1024}};
 
 template <int F, int R, int L>
-size_t lds_bytes() { return sizeof(BinLds<F, R, L>); }
-static_assert(sizeof(BinLds<1024, 2048, 1024>) <= 160u * 1024u, "largest tier exceeds the 160 KiB LDS of a CU");
+size_t lds_bytes() { return sizeof(BinLds<F, R, L, false>); }  // the same for both forms
+static_assert(sizeof(BinLds<1024, 2048, 1024, false>) <= 160u * 1024u, "largest tier exceeds the 160 KiB LDS of a CU");
 
 constexpr uint32_t MAX_DOMS = 256;
+constexpr uint32_t NDRING = 4;
 
 int hip_err(hipError_t e)
 {
@@ -1267,7 +1351,7 @@ struct xkrt_hip_pool {
     uint4 *binrec = nullptr;
     uint32_t binrec_cap = 0;
     uint32_t *cand = nullptr;
-    uint32_t cand_cap = 0;  // per stripe
+    uint32_t cand_cap = 0;  // per stripe; 0 in the direct form
     uint32_t *cstr = nullptr;
     uint4 *blk = nullptr;
     unsigned long long *hkeys = nullptr;
@@ -1279,11 +1363,23 @@ struct xkrt_hip_pool {
     DevCounters *ctr = nullptr;
     DevCounters *ctr_host = nullptr;  // pinned
     // staging of the open launch: pinned host records + the record count
-    uint4 *rec_dev = nullptr, *rec_host = nullptr;
-    uint32_t *nrec_host = nullptr;    // pinned
+    // (one upload per launch: a 32 B header holding the record count, then the
+    // records; two pinned host buffers in turn, so staging the next launch does
+    // not wait for the upload of the previous one)
+    uint4 *rec_dev_hdr = nullptr, *rec_dev = nullptr;
+    uint4 *rec_host_hdr[2] = {}, *rec_host = nullptr;  // rec_host: records of buffer cur_host
+    uint32_t cur_host = 0;
     uint64_t rec_cap = 0;
-    hipEvent_t staged = nullptr;
-    bool staged_pending = false;
+    hipEvent_t staged[2] = {};
+    bool staged_pending[2] = {};
+    // domain table uploads from a pinned ring, so a change never waits for the stream
+    DomDesc *dom_ring = nullptr;
+    hipEvent_t dom_ev[NDRING] = {};
+    bool dom_ev_pending[NDRING] = {};
+    uint32_t dom_slot = 0;
+    // each launch's outcome, written by its k_final (slot = ticket % NSNAP)
+    LaunchSnap *snap = nullptr;
+    struct xkrt_hip_server *server = nullptr;
     uint32_t open_nrec = 0, open_tasks = 0;
     uint64_t open_acc = 0, open_batches = 0;
     // completion tickets: one event per launch, oldest first
@@ -1520,6 +1616,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     KArgs a;
     a.rec = rec;
     a.nrec_arg = nrec_arg;
+    a.nrec_ptr = (const uint32_t *)p->rec_dev_hdr;
     a.dom = p->d_dom;
     a.bin_dom = p->bin_dom;
     a.bin_nf = p->bin_nf; a.bin_nrs = p->bin_nrs; a.frag = p->frag; a.rs = p->rs;
@@ -1538,6 +1635,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.hkeys = p->hkeys; a.hvals = p->hvals; a.hmask = p->hmask;
     a.eslot = p->eslot; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
     a.ctr = p->ctr;
+    a.snap = p->snap;
     return a;
 }
 
@@ -1568,7 +1666,10 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
     const size_t lds = lds_bytes<F, R, L>();
     const uint32_t per_cu = std::max<uint32_t>(1u, (uint32_t)(160u * 1024u / lds));
     const uint32_t blocks = std::min<uint32_t>(p->cap_bins, 256u * std::min<uint32_t>(per_cu, 16u));
-    launch(p, spans, 3, k_resolve<F, R, L>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
+    if (p->cfg.direct_dedup)
+        launch(p, spans, 3, k_resolve<F, R, L, true>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
+    else
+        launch(p, spans, 3, k_resolve<F, R, L, false>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
 }
 
 // the kernels of one launch (kernel slots: count, alloc, fill, resolve, dedup, final)
@@ -1584,16 +1685,23 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
     launch_resolve<64, 128, 256>(p, a, 1, spans);
     if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, a, 2, spans);
     if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, a, 3, spans);
-    launch(p, spans, 4, k_dedup, dim3(1024), dim3(256), 0, a);
+    if (!p->cfg.direct_dedup) launch(p, spans, 4, k_dedup, dim3(1024), dim3(256), 0, a);
     launch(p, spans, 5, k_final, dim3(1024), dim3(256), 0, a);
 }
 
 int push_domains(xkrt_hip_pool *p)
 {
     if (!p->dom_dirty) return 0;
-    // stream ordered, so launches in flight still read the table they were enqueued with
-    HIPCHK(hipMemcpyAsync(p->d_dom, p->h_dom, sizeof(p->h_dom), hipMemcpyHostToDevice, p->stream));
-    HIPCHK(hipStreamSynchronize(p->stream));  // h_dom is pageable: keep it until the copy is done
+    // stream ordered, so launches in flight still read the table they were enqueued with;
+    // the pinned ring slot stays untouched until its upload is done
+    const uint32_t k = p->dom_slot;
+    if (p->dom_ev_pending[k]) HIPCHK(hipEventSynchronize(p->dom_ev[k]));
+    DomDesc *src = p->dom_ring + (size_t)k * MAX_DOMS;
+    memcpy(src, p->h_dom, sizeof(p->h_dom));
+    HIPCHK(hipMemcpyAsync(p->d_dom, src, sizeof(p->h_dom), hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipEventRecord(p->dom_ev[k], p->stream));
+    p->dom_ev_pending[k] = true;
+    p->dom_slot = (k + 1) % NDRING;
     p->dom_dirty = false;
     return 0;
 }
@@ -1706,15 +1814,32 @@ void destroy_pool(xkrt_hip_pool *p)
     for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
     void *dev[] = {p->bin_nf, p->bin_nrs, p->frag, p->rs, p->bin_cnt, p->bin_cur, p->bin_dom, p->active, p->touched,
                    p->d_dom,
-                   p->binrec, p->cand, p->cstr, p->blk, p->hkeys, p->hvals, p->eslot, p->edges, p->edom, p->ctr, p->rec_dev};
+                   p->binrec, p->cand, p->cstr, p->blk, p->hkeys, p->hvals, p->eslot, p->edges, p->edom, p->ctr,
+                   p->rec_dev_hdr};
     for (void *q : dev)
         if (q) (void)hipFree(q);
     if (p->ctr_host) (void)hipHostFree(p->ctr_host);
-    if (p->rec_host) (void)hipHostFree(p->rec_host);
-    if (p->nrec_host) (void)hipHostFree(p->nrec_host);
-    if (p->staged) (void)hipEventDestroy(p->staged);
+    for (int b = 0; b < 2; ++b) {
+        if (p->rec_host_hdr[b]) (void)hipHostFree(p->rec_host_hdr[b]);
+        if (p->staged[b]) (void)hipEventDestroy(p->staged[b]);
+    }
+    if (p->dom_ring) (void)hipHostFree(p->dom_ring);
+    for (hipEvent_t e : p->dom_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (p->snap) (void)hipHostFree(p->snap);
     if (p->own_stream) (void)hipStreamDestroy(p->stream);
     delete p;
+}
+
+// dynamic LDS limit of one tier's kernels (both dedup forms)
+template <int F, int R, int L>
+hipError_t set_lds()
+{
+    hipError_t e = hipFuncSetAttribute((const void *)k_resolve<F, R, L, false>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes<F, R, L>());
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)k_resolve<F, R, L, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_bytes<F, R, L>());
 }
 
 int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool **out)
@@ -1737,18 +1862,18 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     p->rec_cap = max_rects;
     p->binrec_cap = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, std::max<uint64_t>(4 * max_rects, 1 << 16));
     const uint64_t cand_total = std::min<uint64_t>(0x3FFFFFF0ull, std::max<uint64_t>(32 * max_rects, 1 << 20));
-    p->cand_cap = (uint32_t)(cand_total / CSTRIPES);
     uint64_t hsz = 1;
     while (hsz < 2ull * cand_total) hsz <<= 1;
     p->hmask = (uint32_t)(hsz - 1);
     p->edge_cap = p->cfg.max_edges ? p->cfg.max_edges : std::max<uint64_t>(16 * max_rects, 1 << 20);
+    p->cand_cap = p->cfg.direct_dedup ? 0u : (uint32_t)(cand_total / CSTRIPES);
     p->bytes_scratch = 32ull * p->binrec_cap + 12ull * CSTRIPES * p->cand_cap + 12ull * hsz + 4ull * hsz +
-                       12ull * p->edge_cap + 32ull * p->rec_cap + sizeof(DevCounters) + sizeof(p->h_dom);
+                       12ull * p->edge_cap + 32ull * (p->rec_cap + 1) + sizeof(DevCounters) + sizeof(p->h_dom);
     int rc = 0;
     auto chk = [&](hipError_t e) { if (!rc) rc = hip_err(e); };
     if (p->cfg.budget_bytes && p->bytes_scratch > p->cfg.budget_bytes) rc = -ENOMEM;
     chk(hipMalloc(&p->binrec, 32ull * p->binrec_cap));
-    chk(hipMalloc(&p->cand, 12ull * CSTRIPES * p->cand_cap));
+    if (p->cand_cap) chk(hipMalloc(&p->cand, 12ull * CSTRIPES * p->cand_cap));
     chk(hipMalloc(&p->cstr, 64ull * CSTRIPES));
     chk(hipMalloc(&p->blk, 16ull * 4 * BLKSLOTS));
     chk(hipMalloc(&p->hkeys, 8ull * hsz));
@@ -1758,24 +1883,27 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     chk(hipMalloc(&p->edom, 4ull * p->edge_cap));
     chk(hipMalloc(&p->ctr, sizeof(DevCounters)));
     chk(hipMalloc(&p->d_dom, sizeof(p->h_dom)));
-    chk(hipMalloc(&p->rec_dev, 32ull * p->rec_cap));
+    chk(hipMalloc(&p->rec_dev_hdr, 32ull * (p->rec_cap + 1)));
+    if (p->rec_dev_hdr) p->rec_dev = p->rec_dev_hdr + 2;
     chk(hipHostMalloc(&p->ctr_host, sizeof(DevCounters), hipHostMallocDefault));
-    chk(hipHostMalloc(&p->rec_host, 32ull * p->rec_cap, hipHostMallocDefault));
-    chk(hipHostMalloc(&p->nrec_host, 64, hipHostMallocDefault));
+    for (int b = 0; b < 2; ++b) {
+        chk(hipHostMalloc(&p->rec_host_hdr[b], 32ull * (p->rec_cap + 1), hipHostMallocDefault));
+        chk(hipEventCreateWithFlags(&p->staged[b], hipEventDisableTiming));
+    }
+    if (p->rec_host_hdr[0]) p->rec_host = p->rec_host_hdr[0] + 2;
+    chk(hipHostMalloc(&p->dom_ring, sizeof(DomDesc) * MAX_DOMS * NDRING, hipHostMallocDefault));
+    chk(hipHostMalloc(&p->snap, sizeof(LaunchSnap) * NSNAP, hipHostMallocCoherent));
+    if (p->snap) memset(p->snap, 0, sizeof(LaunchSnap) * NSNAP);
+    for (hipEvent_t &e : p->dom_ev) chk(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     if (!rc) chk(hipMemsetAsync(p->hkeys, 0xFF, 8ull * hsz, p->stream));
     if (!rc) chk(hipMemsetAsync(p->hvals, 0xFF, 4ull * hsz, p->stream));
     if (!rc) chk(hipMemsetAsync(p->ctr, 0, sizeof(DevCounters), p->stream));
     if (!rc) chk(hipMemsetAsync(p->cstr, 0, 64ull * CSTRIPES, p->stream));
     if (!rc) chk(hipMemsetAsync(p->blk, 0, 16ull * 4 * BLKSLOTS, p->stream));
-    chk(hipEventCreateWithFlags(&p->staged, hipEventDisableTiming));
-    chk(hipFuncSetAttribute((const void *)k_resolve<64, 128, 64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes<64, 128, 64>()));
-    chk(hipFuncSetAttribute((const void *)k_resolve<64, 128, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes<64, 128, 256>()));
-    chk(hipFuncSetAttribute((const void *)k_resolve<256, 512, 512>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes<256, 512, 512>()));
-    chk(hipFuncSetAttribute((const void *)k_resolve<1024, 2048, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes<1024, 2048, 1024>()));
+    chk(set_lds<64, 128, 64>());
+    chk(set_lds<64, 128, 256>());
+    chk(set_lds<256, 512, 512>());
+    chk(set_lds<1024, 2048, 1024>());
     if (!rc) chk(hipStreamSynchronize(p->stream));
     if (rc) {
         destroy_pool(p);
@@ -1919,9 +2047,10 @@ int stage(xkrt_hip_pool *p, xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc
     if (n > 0xFFFFFFFFull || d->next_id + n > 0xFFFFFFFFull) return -ERANGE;
     if (p->open_nrec + n > p->rec_cap) return p->open_nrec ? -ENOSPC : -EINVAL;
     // the staging buffer must be free: wait for the previous upload
-    if (p->staged_pending) {
-        HIPCHK(hipEventSynchronize(p->staged));
-        p->staged_pending = false;
+    const uint32_t hb = p->cur_host;
+    if (p->staged_pending[hb]) {
+        HIPCHK(hipEventSynchronize(p->staged[hb]));
+        p->staged_pending[hb] = false;
     }
     uint64_t nrect = 0;
     uint32_t ylo = 0, yhi = 0, ntask = 0;
@@ -1952,11 +2081,13 @@ int launch_open(xkrt_hip_pool *p)
 {
     if (p->open_nrec == 0) return 0;
     const uint32_t n = p->open_nrec;
-    *p->nrec_host = n;
-    HIPCHK(hipMemcpyAsync(p->rec_dev, p->rec_host, 32ull * n, hipMemcpyHostToDevice, p->stream));
-    HIPCHK(hipMemcpyAsync(&p->ctr->nrec, p->nrec_host, 4, hipMemcpyHostToDevice, p->stream));
-    HIPCHK(hipEventRecord(p->staged, p->stream));
-    p->staged_pending = true;
+    const uint32_t hb = p->cur_host;
+    *(uint32_t *)p->rec_host_hdr[hb] = n;
+    HIPCHK(hipMemcpyAsync(p->rec_dev_hdr, p->rec_host_hdr[hb], 32ull * (n + 1), hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipEventRecord(p->staged[hb], p->stream));
+    p->staged_pending[hb] = true;
+    p->cur_host = hb ^ 1u;
+    p->rec_host = p->rec_host_hdr[p->cur_host] + 2;
     p->open_nrec = 0;
     p->open_tasks = 0;
     p->open_acc = 0;
@@ -1989,6 +2120,7 @@ int xkrt_hip_pool_create(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_h
 int xkrt_hip_pool_destroy(xkrt_hip_pool_t p)
 {
     if (!p) return -EINVAL;
+    if (p->server) return -EBUSY;  // destroy the serving engine first
     for (uint32_t i = 0; i < MAX_DOMS; ++i)
         if (p->doms[i]) remove_domain(p->doms[i]);
     destroy_pool(p);
@@ -2000,6 +2132,7 @@ int xkrt_hip_pool_domain_create(xkrt_hip_pool_t p, uint64_t ld, uint64_t sizeof_
 {
     if (!p || !out || y_hi <= y_lo || y_hi - y_lo >= (1ull << 32)) return -EINVAL;
     *out = nullptr;
+    if (p->server) return -EBUSY;
     xkrt_hip_dep_domain *d = nullptr;
     int rc = add_domain(p, ld, sizeof_type, bin_x_bytes ? bin_x_bytes : 1024u, bin_y_cols ? bin_y_cols : 128u, &d);
     if (rc) return rc;
@@ -2019,12 +2152,14 @@ int xkrt_hip_pool_stage(xkrt_hip_pool_t p, xkrt_hip_dep_domain_t d, const xkrt_h
                         uint64_t *first_id)
 {
     if (!p || !d || d->pool != p || (!acc && n)) return -EINVAL;
+    if (p->server) return -EBUSY;
     return stage(p, d, acc, n, first_id);
 }
 
 int xkrt_hip_pool_launch(xkrt_hip_pool_t p, uint64_t *ticket)
 {
     if (!p) return -EINVAL;
+    if (p->server) return -EBUSY;
     if (p->poisoned) return -EINVAL;
     const int rc = launch_open(p);
     if (ticket) *ticket = p->n_launch;
@@ -2034,6 +2169,7 @@ int xkrt_hip_pool_launch(xkrt_hip_pool_t p, uint64_t *ticket)
 int xkrt_hip_pool_poll(xkrt_hip_pool_t p, uint64_t *done)
 {
     if (!p || !done) return -EINVAL;
+    if (p->server) return -EBUSY;
     const int rc = retire(p, 0);
     *done = p->done;
     return rc;
@@ -2042,6 +2178,7 @@ int xkrt_hip_pool_poll(xkrt_hip_pool_t p, uint64_t *done)
 int xkrt_hip_pool_wait(xkrt_hip_pool_t p, uint64_t ticket)
 {
     if (!p) return -EINVAL;
+    if (p->server) return -EBUSY;
     return retire(p, ticket);
 }
 
@@ -2057,6 +2194,7 @@ int xkrt_hip_pool_open(xkrt_hip_pool_t p, uint64_t *nrect, uint64_t *naccess, ui
 int xkrt_hip_pool_sync(xkrt_hip_pool_t p, uint64_t *nedges)
 {
     if (!p) return -EINVAL;
+    if (p->server) return -EBUSY;
     const int rc = read_counters(p);
     if (nedges) *nedges = p->ctr_host->edge_total;
     return rc;
@@ -2065,6 +2203,7 @@ int xkrt_hip_pool_sync(xkrt_hip_pool_t p, uint64_t *nedges)
 int xkrt_hip_pool_edges_copy(xkrt_hip_pool_t p, uint32_t *edges, uint32_t *doms, uint64_t cap, uint64_t *n)
 {
     if (!p || !n || (cap && (!edges || !doms))) return -EINVAL;
+    if (p->server) return -EBUSY;
     int rc = read_counters(p);
     const uint64_t ne = p->ctr_host->edge_total;
     *n = ne;
@@ -2345,6 +2484,518 @@ int xkrt_hip_dep_launch_times(xkrt_hip_dep_domain_t d, float *ms, uint64_t cap, 
 {
     if (!d) return -EINVAL;
     return xkrt_hip_pool_launch_times(d->pool, ms, cap, n, clear);
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- serving engine
+//
+// One host thread drives a pool for many clients.  The API threads only
+// append to an inbox under a mutex; every pool and HIP call happens on the
+// engine thread, which loops over: take the inbox; retire completed launches
+// (stamp their batches ready; what each launch's k_final wrote to pinned host
+// memory gives its edge count and error bits without a stream sync); stage every
+// arrived batch of admitted clients into one launch while fewer than `depth`
+// launches are in flight, admitting a client's domain at its first batch;
+// copy the edges of retired launches out per client (on a second stream,
+// while the device is busy or the loop is idle); reset the edge buffer when
+// it is half full and the pipeline is empty; release closed clients.
+
+namespace {
+
+struct SrvClient;
+
+struct SrvRequest {
+    SrvClient *c = nullptr;
+    std::vector<xkrt_hip_access_t> acc;
+    double t_submit = 0, t_ready = 0;
+    std::atomic<int> status{1};  // 1 pending, 0 resolved, < 0 refused (negative errno)
+};
+
+struct SrvClient {
+    uint64_t ld = 0, s = 0, ylo = 0, yhi = 0;
+    uint32_t bx = 0, by = 0;
+    xkrt_hip_dep_domain *dom = nullptr;
+    int status = 0;         // < 0: a batch was refused; the client's later batches are refused too
+    bool closing = false;   // its close arrived (after all of its batches)
+    bool closed_api = false;  // close was called (API side, under the engine mutex)
+    bool left = false;
+    uint64_t round = 0;     // staging round in which it may not stage further batches
+    uint64_t first_id = UINT64_MAX;  // domain access id of its first access
+    uint64_t nrecv = 0, ndone = 0;
+    uint64_t need = 0;      // pool edge count to harvest before it may leave
+    std::vector<uint32_t> edges;  // (pred, succ) in its own access ids; guarded by the engine mutex
+};
+
+struct SrvFlight {
+    uint64_t ticket;
+    std::vector<SrvRequest *> reqs;
+};
+
+double srv_us(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+}
+
+} // namespace
+
+struct xkrt_hip_server {
+    xkrt_hip_pool *p = nullptr;
+    xkrt_hip_server_config_t cfg{};
+    std::thread th;
+    std::mutex mu;
+    std::atomic<bool> stop{false};
+    std::chrono::steady_clock::time_point t0;
+    // shared, under mu
+    std::vector<std::unique_ptr<SrvClient>> clients;
+    std::vector<std::unique_ptr<SrvRequest>> reqs;
+    std::vector<std::pair<SrvRequest *, SrvClient *>> inbox;  // a batch, or a close (null batch)
+    std::atomic<int> err{0};
+    std::atomic<uint64_t> n_final{0}, n_closed{0}, n_left{0};
+    // engine thread only
+    std::deque<SrvRequest *> pending;
+    std::deque<SrvFlight> flights;
+    std::vector<SrvClient *> live;   // received a batch or a close, not left
+    SrvClient *owner[MAX_DOMS] = {};
+    uint64_t round = 0;
+    uint64_t edges_seen = 0;  // pool edge count after the last retired launch
+    uint64_t harvested = 0;   // edges copied out
+    bool hold = false;        // no launches until the pipeline drains (edge buffer reset)
+    hipStream_t hs = nullptr;
+    uint32_t *he = nullptr, *hd = nullptr;  // pinned harvest buffers
+    uint64_t hcap = 0;
+    // statistics, in the order of xkrt_hip_server_stats_t
+    enum { S_LAUNCHES, S_BATCHES, S_ADMITTED, S_LEFT, S_WAITED, S_HARVESTS, S_CONSUMES, S_N };
+    std::atomic<uint64_t> st[S_N] = {};
+    void count(int k, uint64_t v = 1) { st[k].fetch_add(v, std::memory_order_relaxed); }
+};
+
+namespace {
+
+void srv_finish(xkrt_hip_server *s, SrvRequest *r, int status, double t)
+{
+    r->t_ready = t;
+    r->c->ndone++;
+    r->status.store(status, std::memory_order_release);
+    s->n_final.fetch_add(1, std::memory_order_release);
+}
+
+int srv_admit(xkrt_hip_server *s, SrvClient *c)
+{
+    xkrt_hip_pool *p = s->p;
+    xkrt_hip_dep_domain *d = nullptr;
+    int rc = add_domain(p, c->ld, c->s, c->bx, c->by, &d);
+    if (rc == -ENOSPC) return -ENOMEM;  // every domain slot is taken: wait as for memory
+    if (rc) return rc;
+    d->ybase = c->ylo;
+    d->framed = true;
+    rc = ensure_grid(d, 0, (uint32_t)(c->yhi - c->ylo));
+    if (rc) {
+        remove_domain(d);
+        return rc;
+    }
+    d->fixed_frame = true;
+    c->dom = d;
+    s->owner[d->idx] = c;
+    s->count(xkrt_hip_server::S_ADMITTED);
+    return 0;
+}
+
+int srv_retire(xkrt_hip_server *s, bool &work)
+{
+    if (s->flights.empty()) return 0;
+    xkrt_hip_pool *p = s->p;
+    int rc = retire(p, 0);
+    if (rc) return rc;
+    while (!s->flights.empty() && s->flights.front().ticket <= p->done) {
+        SrvFlight &f = s->flights.front();
+        // the launch's k_final wrote its outcome before the launch completed
+        LaunchSnap *o = &p->snap[f.ticket % NSNAP];
+        if (__atomic_load_n(&o->launch, __ATOMIC_ACQUIRE) != f.ticket) return -EIO;
+        const uint32_t err = __atomic_load_n(&o->err, __ATOMIC_RELAXED);
+        const uint64_t total = __atomic_load_n(&o->edge_total, __ATOMIC_RELAXED);
+        if (err) {
+            p->poisoned = true;
+            rc = err_to_errno(err);
+            return rc ? rc : -EIO;
+        }
+        s->edges_seen = total;
+        // past half the edge buffer: no new launches until it is copied out and reset
+        if (s->edges_seen > p->edge_cap / 2) s->hold = true;
+        const double t = srv_us(s->t0);
+        for (SrvRequest *r : f.reqs) {
+            r->c->need = total;
+            srv_finish(s, r, 0, t);
+        }
+        s->count(xkrt_hip_server::S_BATCHES, f.reqs.size());
+        s->flights.pop_front();
+        work = true;
+    }
+    return 0;
+}
+
+int srv_launch(xkrt_hip_server *s, bool &work)
+{
+    xkrt_hip_pool *p = s->p;
+    if (s->hold || s->pending.empty() || s->flights.size() >= s->cfg.depth) return 0;
+    const uint64_t round = ++s->round;
+    const uint64_t cap = s->cfg.max_rects ? s->cfg.max_rects : p->rec_cap;
+    std::vector<SrvRequest *> batch;
+    std::deque<SrvRequest *> keep;
+    while (!s->pending.empty()) {
+        SrvRequest *r = s->pending.front();
+        SrvClient *c = r->c;
+        if (c->status) {  // an earlier batch of this client was refused
+            s->pending.pop_front();
+            srv_finish(s, r, c->status, srv_us(s->t0));
+            work = true;
+            continue;
+        }
+        if (c->round == round) {  // keeps the client's batches in order
+            keep.push_back(r);
+            s->pending.pop_front();
+            continue;
+        }
+        if ((s->cfg.max_batches && batch.size() >= s->cfg.max_batches) || p->open_nrec >= cap) break;
+        if (!c->dom) {
+            const int rc = srv_admit(s, c);
+            if (rc == -ENOMEM) {
+                s->count(xkrt_hip_server::S_WAITED);
+                c->round = round;
+                continue;
+            }
+            if (rc) {
+                c->status = rc;
+                continue;
+            }
+            work = true;
+        }
+        uint64_t first = 0;
+        const int rc = stage(p, c->dom, r->acc.data(), r->acc.size(), &first);
+        if (p->poisoned) return -EIO;
+        if (rc == -ENOSPC) break;  // the launch is full
+        if (rc) {
+            c->status = rc;
+            continue;
+        }
+        if (c->first_id == UINT64_MAX) c->first_id = first;
+        batch.push_back(r);
+        s->pending.pop_front();
+    }
+    while (!keep.empty()) {
+        s->pending.push_front(keep.back());
+        keep.pop_back();
+    }
+    if (batch.empty()) return 0;
+    const int rc = launch_open(p);
+    if (rc) return rc;
+    for (SrvRequest *r : batch) std::vector<xkrt_hip_access_t>().swap(r->acc);
+    s->flights.push_back({p->n_launch, std::move(batch)});
+    s->count(xkrt_hip_server::S_LAUNCHES);
+    work = true;
+    return 0;
+}
+
+int srv_harvest(xkrt_hip_server *s)
+{
+    xkrt_hip_pool *p = s->p;
+    const uint64_t a = s->harvested, n = s->edges_seen - a;
+    if (!n) return 0;
+    if (n > s->hcap) {
+        if (s->he) (void)hipHostFree(s->he);
+        if (s->hd) (void)hipHostFree(s->hd);
+        s->he = s->hd = nullptr;
+        s->hcap = std::max<uint64_t>(n, 1u << 16);
+        HIPCHK(hipHostMalloc(&s->he, 8 * s->hcap, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc(&s->hd, 4 * s->hcap, hipHostMallocDefault));
+    }
+    // retired launches never change these edges again; later launches append after them
+    HIPCHK(hipMemcpyAsync(s->he, p->edges + 2 * a, 8 * n, hipMemcpyDeviceToHost, s->hs));
+    HIPCHK(hipMemcpyAsync(s->hd, p->edom + a, 4 * n, hipMemcpyDeviceToHost, s->hs));
+    HIPCHK(hipStreamSynchronize(s->hs));
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (uint64_t i = 0; i < n; ++i) {
+            SrvClient *c = s->owner[s->hd[i] & (MAX_DOMS - 1)];
+            if (!c) return -EIO;
+            const uint32_t f = (uint32_t)c->first_id;
+            c->edges.push_back(s->he[2 * i] - f);
+            c->edges.push_back(s->he[2 * i + 1] - f);
+        }
+    }
+    s->harvested = s->edges_seen;
+    s->count(xkrt_hip_server::S_HARVESTS);
+    return 0;
+}
+
+int srv_housekeep(xkrt_hip_server *s, bool &work, double idle_us)
+{
+    xkrt_hip_pool *p = s->p;
+    bool leaving = false;
+    for (SrvClient *c : s->live)
+        if (c->closing && c->ndone == c->nrecv) leaving = true;
+    const bool busy = s->flights.size() >= s->cfg.depth;
+    // idle for a while: arrivals are sparse, a copy now delays none
+    const bool idle = s->flights.empty() && s->pending.empty() && idle_us > 100.0;
+    if (s->edges_seen > s->harvested &&
+        (busy || idle || leaving || s->hold || s->edges_seen - s->harvested > p->edge_cap / 4)) {
+        const int rc = srv_harvest(s);
+        if (rc) return rc;
+        work = true;
+    }
+    // reset the half-used edge buffer once the pipeline has drained and every edge is out
+    if (s->hold && s->flights.empty() && s->harvested == s->edges_seen) {
+        hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, p->stream, p->ctr);
+        HIPCHK(hipGetLastError());
+        s->edges_seen = s->harvested = 0;
+        for (SrvClient *c : s->live) c->need = 0;
+        s->hold = false;
+        s->count(xkrt_hip_server::S_CONSUMES);
+        work = true;
+    }
+    if (!leaving) return 0;
+    for (size_t i = 0; i < s->live.size();) {
+        SrvClient *c = s->live[i];
+        if (c->closing && c->ndone == c->nrecv && c->need <= s->harvested) {
+            if (c->dom) {
+                s->owner[c->dom->idx] = nullptr;
+                remove_domain(c->dom);  // stream order keeps later launches off its old bins
+                c->dom = nullptr;
+            }
+            c->left = true;
+            s->count(xkrt_hip_server::S_LEFT);
+            s->n_left.fetch_add(1, std::memory_order_release);
+            s->live[i] = s->live.back();
+            s->live.pop_back();
+            work = true;
+            continue;
+        }
+        ++i;
+    }
+    return 0;
+}
+
+void srv_loop(xkrt_hip_server *s)
+{
+    (void)hipSetDevice(s->p->device);
+    std::vector<std::pair<SrvRequest *, SrvClient *>> in;
+    double last_work = srv_us(s->t0);
+    while (true) {
+        bool work = false;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            in.swap(s->inbox);
+        }
+        for (auto &[r, c] : in) {
+            if (c->nrecv == 0 && !c->closing) s->live.push_back(c);
+            if (r) {
+                s->pending.push_back(r);
+                c->nrecv++;
+            } else {
+                c->closing = true;
+            }
+        }
+        work = !in.empty();
+        in.clear();
+        int rc = srv_retire(s, work);
+        if (!rc) rc = srv_launch(s, work);
+        const double now = srv_us(s->t0);
+        if (!rc) rc = srv_housekeep(s, work, now - last_work);
+        if (rc) {
+            s->err.store(rc, std::memory_order_release);
+            // refuse everything not resolved, so that drains end
+            const double t = srv_us(s->t0);
+            for (SrvRequest *r : s->pending) srv_finish(s, r, rc, t);
+            for (auto &f : s->flights)
+                for (SrvRequest *r : f.reqs) srv_finish(s, r, rc, t);
+            s->pending.clear();
+            s->flights.clear();
+            break;
+        }
+        if (work) {
+            last_work = now;
+            continue;
+        }
+        if (s->stop.load(std::memory_order_acquire) && s->flights.empty()) break;
+        // poll: an arrival is seen within microseconds; after 2 ms without work, back off
+        if (s->flights.empty() && now - last_work > 2000.0) std::this_thread::sleep_for(std::chrono::microseconds(50));
+        else std::this_thread::yield();
+    }
+    // late submissions after a failure are refused by the API; fail what is still queued
+    std::lock_guard<std::mutex> g(s->mu);
+    const int e = s->err.load();
+    for (auto &[r, c] : s->inbox)
+        if (r) {
+            r->c->ndone++;
+            r->status.store(e ? e : -ECANCELED, std::memory_order_release);
+            s->n_final.fetch_add(1, std::memory_order_release);
+        }
+    s->inbox.clear();
+}
+
+} // namespace
+
+extern "C" {
+
+int xkrt_hip_server_create(xkrt_hip_pool_t p, const xkrt_hip_server_config_t *cfg, xkrt_hip_server_t *out)
+{
+    if (!p || !out) return -EINVAL;
+    *out = nullptr;
+    if (p->server || p->open_nrec) return -EBUSY;
+    if (p->poisoned) return -EINVAL;
+    xkrt_hip_server_config_t c{};
+    if (cfg) c = *cfg;
+    if (c.depth == 0) c.depth = 2;
+    if (c.depth > 16) return -EINVAL;
+    int rc = read_counters(p);  // edges already in the buffer are not the engine's
+    if (rc) return rc;
+    if (c.reserve_bytes) {
+        uint64_t bins = c.reserve_bytes / bin_bytes(p);
+        if (p->cfg.budget_bytes)
+            bins = std::min<uint64_t>(bins, (p->cfg.budget_bytes - std::min(p->cfg.budget_bytes, p->bytes_scratch)) /
+                                                bin_bytes(p));
+        bins = std::min<uint64_t>(bins, 0xFFFFFFull);
+        if (bins > p->cap_bins && (rc = grow_bins(p, (uint32_t)bins))) return rc;
+    }
+    auto *s = new xkrt_hip_server();
+    s->p = p;
+    s->cfg = c;
+    s->edges_seen = s->harvested = p->ctr_host->edge_total;
+    if (hipStreamCreateWithFlags(&s->hs, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return -ENOMEM;
+    }
+    p->server = s;
+    s->t0 = std::chrono::steady_clock::now();
+    s->th = std::thread(srv_loop, s);
+    *out = s;
+    return 0;
+}
+
+int xkrt_hip_server_destroy(xkrt_hip_server_t s)
+{
+    if (!s) return -EINVAL;
+    s->stop.store(true, std::memory_order_release);
+    if (s->th.joinable()) s->th.join();
+    xkrt_hip_pool *p = s->p;
+    (void)hipStreamSynchronize(p->stream);
+    for (auto &c : s->clients)
+        if (c->dom) {
+            remove_domain(c->dom);
+            c->dom = nullptr;
+        }
+    if (s->he) (void)hipHostFree(s->he);
+    if (s->hd) (void)hipHostFree(s->hd);
+    if (s->hs) (void)hipStreamDestroy(s->hs);
+    p->server = nullptr;
+    const int rc = s->err.load();
+    delete s;
+    return rc;
+}
+
+int xkrt_hip_server_client(xkrt_hip_server_t s, uint64_t ld, uint64_t sizeof_type, uint32_t bin_x_bytes,
+                           uint32_t bin_y_cols, uint64_t y_lo, uint64_t y_hi, uint32_t *client)
+{
+    if (!s || !client || ld == 0 || sizeof_type == 0 || y_hi <= y_lo || y_hi - y_lo >= (1ull << 32)) return -EINVAL;
+    bin_x_bytes = bin_x_bytes ? bin_x_bytes : 1024u;
+    bin_y_cols = bin_y_cols ? bin_y_cols : 128u;
+    if (!is_pow2(bin_x_bytes) || !is_pow2(bin_y_cols) || ld * sizeof_type > 0xFFFFFFFFull) return -EINVAL;
+    auto c = std::make_unique<SrvClient>();
+    c->ld = ld; c->s = sizeof_type; c->ylo = y_lo; c->yhi = y_hi;
+    c->bx = bin_x_bytes; c->by = bin_y_cols;
+    std::lock_guard<std::mutex> g(s->mu);
+    *client = (uint32_t)s->clients.size();
+    s->clients.push_back(std::move(c));
+    return 0;
+}
+
+int xkrt_hip_server_submit(xkrt_hip_server_t s, uint32_t client, const xkrt_hip_access_t *acc, uint64_t n,
+                           uint64_t *request)
+{
+    if (!s || (!acc && n) || n == 0) return -EINVAL;
+    const double t = srv_us(s->t0);
+    if (const int e = s->err.load(std::memory_order_acquire)) return e;
+    auto r = std::make_unique<SrvRequest>();
+    r->acc.assign(acc, acc + n);
+    r->t_submit = t;
+    std::lock_guard<std::mutex> g(s->mu);
+    // checked under the mutex: the engine sets the error before it refuses the inbox under it
+    if (const int e = s->err.load(std::memory_order_acquire)) return e;
+    if (client >= s->clients.size()) return -EINVAL;
+    SrvClient *c = s->clients[client].get();
+    if (c->closed_api) return -EINVAL;
+    r->c = c;
+    if (request) *request = s->reqs.size();
+    s->inbox.push_back({r.get(), c});
+    s->reqs.push_back(std::move(r));
+    return 0;
+}
+
+int xkrt_hip_server_close(xkrt_hip_server_t s, uint32_t client)
+{
+    if (!s) return -EINVAL;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (client >= s->clients.size() || s->clients[client]->closed_api) return -EINVAL;
+    s->clients[client]->closed_api = true;
+    s->inbox.push_back({nullptr, s->clients[client].get()});
+    s->n_closed.fetch_add(1, std::memory_order_release);
+    return 0;
+}
+
+int xkrt_hip_server_drain(xkrt_hip_server_t s, uint64_t timeout_us)
+{
+    if (!s) return -EINVAL;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (true) {
+        uint64_t nreq;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            nreq = s->reqs.size();
+        }
+        const int e = s->err.load(std::memory_order_acquire);
+        if (s->n_final.load(std::memory_order_acquire) == nreq &&
+            (e || s->n_left.load(std::memory_order_acquire) == s->n_closed.load(std::memory_order_acquire)))
+            return e;
+        if (timeout_us && srv_us(t0) > (double)timeout_us) return -ETIMEDOUT;
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+int xkrt_hip_server_times(xkrt_hip_server_t s, uint64_t first, uint64_t n, double *submit_us, double *ready_us,
+                          int32_t *status)
+{
+    if (!s) return -EINVAL;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (first > s->reqs.size() || n > s->reqs.size() - first) return -ERANGE;
+    for (uint64_t i = 0; i < n; ++i) {
+        const SrvRequest *r = s->reqs[first + i].get();
+        const int st = r->status.load(std::memory_order_acquire);
+        if (submit_us) submit_us[i] = r->t_submit;
+        if (ready_us) ready_us[i] = st == 1 ? 0.0 : r->t_ready;
+        if (status) status[i] = st;
+    }
+    return 0;
+}
+
+int xkrt_hip_server_edges(xkrt_hip_server_t s, uint32_t client, uint32_t *edges, uint64_t cap, uint64_t *n)
+{
+    if (!s || !n || (cap && !edges)) return -EINVAL;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (client >= s->clients.size()) return -EINVAL;
+    const auto &v = s->clients[client]->edges;
+    *n = v.size() / 2;
+    const uint64_t m = std::min<uint64_t>(*n, cap);
+    if (m) memcpy(edges, v.data(), 8 * m);
+    return 0;
+}
+
+int xkrt_hip_server_stats(xkrt_hip_server_t s, xkrt_hip_server_stats_t *out)
+{
+    if (!s || !out) return -EINVAL;
+    uint64_t *f = &out->launches;
+    for (int k = 0; k < xkrt_hip_server::S_N; ++k) f[k] = s->st[k].load(std::memory_order_relaxed);
+    out->error = s->err.load();
+    return 0;
 }
 
 } // extern "C"

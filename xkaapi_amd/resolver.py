@@ -50,12 +50,23 @@ assert RECT_DTYPE.itemsize == 32
 
 class PoolConfigT(C.Structure):
     _fields_ = [("kernel", C.c_uint32), ("graph", C.c_uint32), ("max_rects", C.c_uint64),
-                ("max_edges", C.c_uint64), ("budget_bytes", C.c_uint64)]
+                ("max_edges", C.c_uint64), ("budget_bytes", C.c_uint64), ("direct_dedup", C.c_uint32),
+                ("pad", C.c_uint32)]
 
 
 class PoolStatsT(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("domains", "bins_capacity", "bins_used", "bytes_per_bin", "device_bytes",
                                           "launches", "graph", "poisoned")]
+
+
+class ServerConfigT(C.Structure):
+    _fields_ = [("depth", C.c_uint32), ("max_batches", C.c_uint32), ("max_rects", C.c_uint64),
+                ("reserve_bytes", C.c_uint64)]
+
+
+class ServerStatsT(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("launches", "batches", "admitted", "left", "waited", "harvests",
+                                          "consumes")] + [("error", C.c_int64)]
 
 
 ERR_BITS = {1: "incidence array", 2: "bin record list", 4: "bin fragments", 8: "bin reader log",
@@ -125,6 +136,15 @@ def hip_lib():
     L.xkrt_hip_dep_kernel_times.argtypes = [_p, C.POINTER(C.c_double), C.POINTER(_u64), C.c_int]
     L.xkrt_hip_dep_launch_times.argtypes = [_p, _p, _u64, C.POINTER(_u64), C.c_int]
     L.xkrt_hip_copy.argtypes = [_p, _p, _u64, _p]
+    L.xkrt_hip_server_create.argtypes = [_p, C.POINTER(ServerConfigT), C.POINTER(_p)]
+    L.xkrt_hip_server_destroy.argtypes = [_p]
+    L.xkrt_hip_server_client.argtypes = [_p, _u64, _u64, _u32, _u32, _u64, _u64, C.POINTER(_u32)]
+    L.xkrt_hip_server_submit.argtypes = [_p, _u32, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_server_close.argtypes = [_p, _u32]
+    L.xkrt_hip_server_drain.argtypes = [_p, _u64]
+    L.xkrt_hip_server_times.argtypes = [_p, _u64, _u64, _p, _p, _p]
+    L.xkrt_hip_server_edges.argtypes = [_p, _u32, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_server_stats.argtypes = [_p, C.POINTER(ServerStatsT)]
     _LIB = L
     return L
 
@@ -301,10 +321,10 @@ class Pool:
     hipGraph); the device-memory budget decides which domains are admitted."""
 
     def __init__(self, kernel: int = 1, graph: bool = False, max_rects: int = 1 << 20, max_edges: int = 0,
-                 budget_bytes: int = 0, stream: int | None = None, device: int = 0):
+                 budget_bytes: int = 0, stream: int | None = None, device: int = 0, direct_dedup: bool = False):
         self.L = hip_lib()
         _chk(self.L.xkrt_hip_init(device), "xkrt_hip_init")
-        cfg = PoolConfigT(kernel, 1 if graph else 0, max_rects, max_edges, budget_bytes)
+        cfg = PoolConfigT(kernel, 1 if graph else 0, max_rects, max_edges, budget_bytes, 1 if direct_dedup else 0, 0)
         h = _p()
         _chk(self.L.xkrt_hip_pool_create(C.byref(cfg), stream, C.byref(h)), "xkrt_hip_pool_create")
         self.h = h
@@ -398,6 +418,86 @@ class Pool:
     @property
     def stream(self) -> int:
         return int(self.L.xkrt_hip_pool_stream(self.h) or 0)
+
+
+class Server:
+    """Serving engine over a Pool: a native host thread admits clients against
+    the pool's budget, batches every arrived batch into the next launch while
+    fewer than `depth` launches are in flight (max_batches=1: one batch per
+    launch), and stamps submit and ready times.  Clients are small ints."""
+
+    def __init__(self, pool: Pool, depth: int = 2, max_batches: int = 0, max_rects: int = 0,
+                 reserve_bytes: int = 0):
+        self.L = hip_lib()
+        self.pool = pool
+        cfg = ServerConfigT(depth, max_batches, max_rects, reserve_bytes)
+        h = _p()
+        _chk(self.L.xkrt_hip_server_create(pool.h, C.byref(cfg), C.byref(h)), "xkrt_hip_server_create")
+        self.h = h
+        self.nreq = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            rc = self.L.xkrt_hip_server_destroy(self.h)
+            self.h = None
+            if rc:
+                raise ResolverError(rc, "xkrt_hip_server_destroy")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def client(self, ld: int, sizeof_type: int, y_range, bin_x_bytes: int = 1024, bin_y_cols: int = 128) -> int:
+        c = _u32()
+        _chk(self.L.xkrt_hip_server_client(self.h, ld, sizeof_type, bin_x_bytes, bin_y_cols, int(y_range[0]),
+                                           int(y_range[1]), C.byref(c)), "xkrt_hip_server_client")
+        return int(c.value)
+
+    def submit(self, client: int, acc: np.ndarray) -> int:
+        acc = np.ascontiguousarray(acc, ACCESS_DTYPE)
+        r = _u64()
+        _chk(self.L.xkrt_hip_server_submit(self.h, client, acc.ctypes.data, len(acc), C.byref(r)),
+             "xkrt_hip_server_submit")
+        self.nreq = int(r.value) + 1
+        return int(r.value)
+
+    def close_client(self, client: int):
+        _chk(self.L.xkrt_hip_server_close(self.h, client), "xkrt_hip_server_close")
+
+    def drain(self, timeout_s: float = 0.0):
+        _chk(self.L.xkrt_hip_server_drain(self.h, int(timeout_s * 1e6)), "xkrt_hip_server_drain")
+
+    def times(self, first: int = 0, n: int | None = None):
+        """(submit_us, ready_us, status) per request."""
+        n = self.nreq - first if n is None else n
+        t0 = np.zeros(max(n, 1), np.float64)
+        t1 = np.zeros(max(n, 1), np.float64)
+        st = np.zeros(max(n, 1), np.int32)
+        _chk(self.L.xkrt_hip_server_times(self.h, first, n, t0.ctypes.data, t1.ctypes.data, st.ctypes.data),
+             "xkrt_hip_server_times")
+        return t0[:n], t1[:n], st[:n]
+
+    def edges(self, client: int) -> np.ndarray:
+        """The client's edges: (E, 2) uint32 (pred, succ) in its own access ids."""
+        n = _u64()
+        _chk(self.L.xkrt_hip_server_edges(self.h, client, None, 0, C.byref(n)), "xkrt_hip_server_edges")
+        out = np.empty((max(int(n.value), 1), 2), np.uint32)
+        _chk(self.L.xkrt_hip_server_edges(self.h, client, out.ctypes.data, len(out), C.byref(n)),
+             "xkrt_hip_server_edges")
+        return out[:int(n.value)]
+
+    def stats(self) -> dict:
+        st = ServerStatsT()
+        _chk(self.L.xkrt_hip_server_stats(self.h, C.byref(st)), "xkrt_hip_server_stats")
+        return {f: int(getattr(st, f)) for f, _ in ServerStatsT._fields_}
 
 
 def batches_by_tasks(task: np.ndarray, tasks_per_batch) -> list[tuple[int, int]]:

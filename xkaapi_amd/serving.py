@@ -1,24 +1,24 @@
-"""Serving loop: many clients submit task streams; the GPU resolver serves them.
+"""Serving: many clients submit task streams; the GPU resolver serves them.
 
-Each client owns a dependency domain admitted into one device Pool against a
-device-memory budget (a client waits for admission while the budget is
-full).  Client batches arrive by an open-loop schedule; the server batches
-continuously: whenever fewer than `depth` launches are in flight, every batch
-that has arrived is staged into one launch (the batched form), or each batch
-gets a launch of its own (the plain per-batch form).  Launches queue on the
-pool's stream, so staging the next launch overlaps the device work of the
-current ones.  Submit-to-ready latency of a batch = its arrival -> the
-completion of the launch that resolved it, as the host observes it; every
-task of the batch counts once in the percentiles.
+The serving engine is native (xkrt_hip_server_*: a host thread in
+libxkrt_hip.so).  Each client owns a dependency domain admitted into one
+device Pool against a device-memory budget at its first batch (a client waits
+for admission while the budget is full).  Client batches arrive by an
+open-loop schedule and are submitted to the engine at their arrival times;
+the engine batches continuously: whenever fewer than `depth` launches are in
+flight, every batch that has arrived goes into one launch (the batched form,
+replayed from a captured hipGraph), or each batch gets a launch of its own
+(the plain per-batch form).  Launches queue on the pool's stream, so staging
+the next launch overlaps the device work of the current ones.
 
-Edges are copied back whenever nothing is in flight (and at the end) and
-filed per client by the domain index each edge carries; a client leaves --
-its domain's bins return to the pool -- at the first such drain after its
-last batch completed.
+Submit-to-ready latency of a batch = its scheduled arrival -> the engine
+seeing the launch that resolved it complete; every task of the batch counts
+once in the percentiles.  A client closes after its last batch and leaves
+(its bins return to the pool) once that batch is resolved and its edges are
+copied out.
 """
 from __future__ import annotations
 
-import collections
 import time
 from dataclasses import dataclass, field
 
@@ -91,6 +91,8 @@ class RunResult:
     lat_us: np.ndarray
     weights: np.ndarray
     edges: dict
+    lag_us: np.ndarray = None   # how late the driver submitted each batch after its scheduled arrival
+    stats: dict = None
 
     @property
     def rate(self):
@@ -103,112 +105,73 @@ class RunResult:
         return float(v[min(int(np.searchsorted(c, q / 100.0)), len(v) - 1)])
 
 
+def bins_needed(clients, bin_x_bytes: int, bin_y_cols: int) -> int:
+    """Bins of every client's domain grid (its columns in rows of bin_y_cols)."""
+    n = 0
+    for c in clients:
+        ld, s = int(c.stream.ld[0]), int(c.stream.s[0])
+        lo, hi = col_range(c.stream)
+        n += -(-ld * s // bin_x_bytes) * -(-(hi - lo) // bin_y_cols)
+    return n
+
+
 def col_range(st):
     ld, s = int(st.ld[0]), int(st.s[0])
     cols = (st.addr // (ld * s)).astype(np.int64)
     return int(cols.min()), int((cols + st.n.astype(np.int64)).max()) + 1
 
 
-def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30, kernel: int = 1,
-          graph: bool = True, depth: int = 2, max_rects: int = 1 << 20, bin_x_bytes: int = 1024,
-          bin_y_cols: int = 128) -> RunResult:
-    """Run one schedule through a fresh pool; mode "batched" or "plain"."""
-    for c in clients:
-        c.next_batch = c.completed = 0
-        c.left = False
-        c.dom = None
-        c.edges = []
+def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30, kernel: int = 0,
+          graph: bool = True, depth: int = 1, max_rects: int = 1 << 20, bin_x_bytes: int = 1024,
+          bin_y_cols: int = 128, reserve: bool = True, max_edges: int = 0,
+          direct_dedup: bool = False) -> RunResult:
+    """Run one schedule through a fresh pool and serving engine; mode "batched" or "plain".
+    Defaults (measured on the 64-client workload): the small LDS class (0), one
+    launch in flight (a batch that arrives during a launch goes into the next
+    one, which starts as soon as the current one ends), staged dedup.
+    reserve: allocate the bins of every client up front (within the budget), so
+    admissions do not allocate device memory while serving."""
     byid = {c.cid: c for c in clients}
-    pool = R.Pool(kernel=kernel, graph=graph and mode == "batched", max_rects=max_rects,
-                  budget_bytes=budget_bytes)
-    owner = {}              # domain index -> client, for filing edges
-    pending = collections.deque()   # (arrival, cid, batch)
-    waiting = collections.deque()   # arrived batches of clients not admitted yet
-    inflight = collections.deque()  # (ticket, [(arrival, cid, batch)])
-    lat, wts = [], []
     nacc = sum(len(c.acc) for c in clients)
-    launches = 0
-
-    def drain():
-        e, d = pool.edges()
-        for idx in np.unique(d):
-            owner[int(idx)].edges.append(e[d == idx])
-        for c in clients:
-            if c.completed == len(c.batches) and c.dom is not None and not c.left:
-                owner.pop(c.dom.index, None)
-                pool.leave(c.dom)
-                c.left = True
-
-    def admit(cid):
-        c = byid[cid]
-        if c.dom is not None:
-            return True
-        try:
-            c.dom = pool.domain(int(c.stream.ld[0]), int(c.stream.s[0]), col_range(c.stream), bin_x_bytes,
-                                bin_y_cols)
-        except R.ResolverError as ex:
-            if ex.rc == -12:  # -ENOMEM: not admitted now
-                return False
-            raise
-        owner[c.dom.index] = c
-        return True
-
-    t0 = time.perf_counter()
-    i = 0
+    pool = R.Pool(kernel=kernel, graph=graph and mode == "batched", max_rects=max_rects,
+                  budget_bytes=budget_bytes, max_edges=max_edges, direct_dedup=direct_dedup)
+    srv = None
     try:
-        while i < len(schedule) or pending or waiting or inflight:
-            now = time.perf_counter() - t0
-            while i < len(schedule) and schedule[i][0] <= now:
-                t, cid = schedule[i]
-                c = byid[cid]
-                pending.append((t, cid, c.next_batch))
-                c.next_batch += 1
-                i += 1
-            if inflight:
-                done = pool.poll()
-                tnow = time.perf_counter() - t0
-                while inflight and inflight[0][0] <= done:
-                    for (ta, cid, b) in inflight.popleft()[1]:
-                        lat.append((tnow - ta) * 1e6)
-                        wts.append(byid[cid].ntasks[b])
-                        byid[cid].completed += 1
-            if not inflight:
-                drain()
-                # clients that could not be admitted retry once others have left
-                while waiting:
-                    pending.appendleft(waiting.pop())
-            if pending and len(inflight) < depth:
-                batch = []
-                hold = collections.deque()
-                while pending:
-                    ta, cid, b = pending.popleft()
-                    if not admit(cid):
-                        hold.append((ta, cid, b))
-                        continue
-                    c = byid[cid]
-                    a, e = c.batches[b]
-                    try:
-                        pool.stage(c.dom, c.acc[a:e])
-                    except R.ResolverError as ex:
-                        if ex.rc != -28:  # -ENOSPC: the launch is full; the rest waits for the next one
-                            raise
-                        pending.appendleft((ta, cid, b))
-                        break
-                    batch.append((ta, cid, b))
-                    if mode == "plain":
-                        break
-                waiting.extend(hold)
-                if not batch and not inflight and hold and pool.stats()["domains"] == 0:
-                    raise RuntimeError("the device budget cannot hold even one waiting client")
-                if batch:
-                    inflight.append((pool.launch(), batch))
-                    launches += 1
-        drain()
-        makespan = time.perf_counter() - t0
+        rb = bins_needed(clients, bin_x_bytes, bin_y_cols) * pool.stats()["bytes_per_bin"] if reserve else 0
+        srv = R.Server(pool, depth=depth, max_batches=1 if mode == "plain" else 0, reserve_bytes=rb)
+        ids = {c.cid: srv.client(int(c.stream.ld[0]), int(c.stream.s[0]), col_range(c.stream), bin_x_bytes,
+                                 bin_y_cols) for c in clients}
+        nxt = {c.cid: 0 for c in clients}
+        meta, lag = [], np.zeros(len(schedule))
+        clock = time.perf_counter
+        t0 = clock()
+        for i, (t, cid) in enumerate(schedule):
+            while clock() - t0 < t:  # open loop: submit at the arrival time
+                pass
+            lag[i] = clock() - t0 - t
+            c = byid[cid]
+            b = nxt[cid]
+            nxt[cid] = b + 1
+            a, e = c.batches[b]
+            srv.submit(ids[cid], c.acc[a:e])
+            meta.append((cid, b))
+            if b + 1 == len(c.batches):
+                srv.close_client(ids[cid])
+        srv.drain(timeout_s=600)
+        ts, tr, st = srv.times()
+        if (st != 0).any():
+            bad = int(np.flatnonzero(st != 0)[0])
+            raise R.ResolverError(int(st[bad]), f"serving: batch {meta[bad]} refused")
+        edges = {c.cid: srv.edges(ids[c.cid]) for c in clients}
+        stats = srv.stats()
     finally:
+        if srv is not None:
+            srv.close()
         pool.close()
-    edges = {c.cid: (np.concatenate(c.edges) if c.edges else np.zeros((0, 2), np.uint32)) for c in clients}
-    return RunResult(mode, nacc, makespan, launches, np.asarray(lat), np.asarray(wts), edges)
+    lat = (tr - ts) + lag * 1e6
+    wts = np.array([byid[cid].ntasks[b] for cid, b in meta])
+    makespan = (float(tr.max()) - float(ts.min())) * 1e-6 + float(lag[0])
+    return RunResult(mode, nacc, makespan, stats["launches"], lat, wts, edges, lag * 1e6, stats)
 
 
 def saturation(clients, mode: str = "batched", **kw) -> RunResult:
