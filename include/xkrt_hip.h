@@ -215,7 +215,9 @@ typedef struct xkrt_hip_pool_config {
     uint32_t direct_dedup;  /* 1: each bin's candidates go straight into the launch's dedup table (one
                                kernel less per launch: for small, latency-bound launches); 0: they are
                                staged and deduplicated by a kernel of their own (large launches) */
-    uint32_t pad;
+    uint32_t small_launch_rects; /* launches of at most this many rects read their records in place from
+                               pinned host memory (no upload) and start at the second LDS tier (one
+                               resolve kernel less); 0 = never */
 } xkrt_hip_pool_config_t;
 
 typedef struct xkrt_hip_pool_stats {

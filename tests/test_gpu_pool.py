@@ -32,10 +32,10 @@ def col_range(st):
     return int(cols.min()), int((cols + st.n.astype(np.int64)).max()) + 1
 
 
-def run_pool(streams, tasks_per_batch, graph=False, kernel=1, order="round_robin"):
+def run_pool(streams, tasks_per_batch, graph=False, kernel=1, **pool_kw):
     """Stage the streams' batches interleaved, one launch per round; edges per client."""
     out = {}
-    with R.Pool(kernel=kernel, graph=graph, max_rects=1 << 16) as pool:
+    with R.Pool(kernel=kernel, graph=graph, max_rects=1 << 16, **pool_kw) as pool:
         doms, batches, firsts = {}, {}, {}
         for name, st in streams:
             doms[name] = pool.domain(int(st.ld[0]), int(st.s[0]), col_range(st), bin_x_bytes=1024, bin_y_cols=128)
@@ -78,6 +78,25 @@ def test_pool_graph_replay_matches_oracle():
     streams = clients()[:3]
     mixed, stats = run_pool(streams, tasks_per_batch=25, graph=True)
     assert stats["graph"] == 1
+    for name, st in streams:
+        assert np.array_equal(mixed[name], oracle_edges(st)), name
+
+
+@pytest.mark.parametrize("kernel,direct", [(0, True), (1, True), (1, False)])
+def test_pool_small_launches_match_oracle(kernel, direct):
+    # records read in place from pinned host memory, resolve from tier 1, both dedup forms;
+    # the rounds of large batches exceed the small-launch limit and take the upload path
+    streams = clients()[:4]
+    for tpb in (12, 300):
+        mixed, _ = run_pool(streams, tasks_per_batch=tpb, kernel=kernel, direct_dedup=direct,
+                            small_launch_rects=2048)
+        for name, st in streams:
+            assert np.array_equal(mixed[name], oracle_edges(st)), (name, tpb)
+
+
+def test_pool_direct_dedup_graph_matches_oracle():
+    streams = clients()[:3]
+    mixed, _ = run_pool(streams, tasks_per_batch=25, graph=True, direct_dedup=True)
     for name, st in streams:
         assert np.array_equal(mixed[name], oracle_edges(st)), name
 

@@ -129,6 +129,7 @@ struct KArgs {
     uint32_t binrec_cap;
     uint32_t nbins;
     uint32_t max_tier;                              // tiers 0..max_tier; storage sized for max_tier
+    uint32_t min_tier;                              // first tier a bin may take (1 in small launches)
     uint32_t tier_f[4], tier_g[4], tier_l[4];       // per tier: fragments, log entries, records
     uint32_t *cand;           // staged form: CSTRIPES stripes of cand_cap candidates (3 u32 each)
     uint32_t cand_cap;        // per stripe
@@ -285,7 +286,7 @@ __global__ void k_alloc(KArgs a, uint32_t nbins)
         if (lane_id() == 0) base = atomicAdd(&a.ctr->total, sum);
         base = wave_bcast(base, 0);
         if (lane_id() == 0 && (uint64_t)base + sum > a.binrec_cap) atomicOr(&a.ctr->err, ERR_BINREC);
-        uint32_t tier = 0, nfl = 0, ngl = 0;
+        uint32_t tier = a.min_tier, nfl = 0, ngl = 0;
         if (c > 0) {
             a.bin_cur[slot] = base + incl - c;
             nfl = a.bin_nf[slot];
@@ -1368,6 +1369,8 @@ struct xkrt_hip_pool {
     // not wait for the upload of the previous one)
     uint4 *rec_dev_hdr = nullptr, *rec_dev = nullptr;
     uint4 *rec_host_hdr[2] = {}, *rec_host = nullptr;  // rec_host: records of buffer cur_host
+    uint4 *rec_host_dev[2] = {};   // device addresses of the host buffers (small launches read them in place)
+    uint64_t buf_ticket[2] = {};   // the last launch that read a host buffer in place
     uint32_t cur_host = 0;
     uint64_t rec_cap = 0;
     hipEvent_t staged[2] = {};
@@ -1626,6 +1629,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.binrec_cap = p->binrec_cap;
     a.nbins = p->cap_bins;
     a.max_tier = p->kclass + 1;  // tier t + 1 is class t
+    a.min_tier = 0;
     for (int t = 0; t < 4; ++t) {
         a.tier_f[t] = TIERS[t].fcap;
         a.tier_g[t] = TIERS[t].rcap;
@@ -1676,12 +1680,14 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
 void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
                     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
+    // a small launch starts at tier 1: one resolve kernel less, occupancy does not matter there
+    const bool small = a.min_tier > 0;
     launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), 0, a);
     launch(p, spans, 1, k_alloc, dim3(std::min<uint32_t>((p->cap_bins + 255) / 256, 1024u)), dim3(256), 0, a,
            p->cap_bins);
     launch(p, spans, 2, k_fill, dim3(rblocks), dim3(256), 0, a);
     // tier 0 first; bins that outgrow a tier are retried by the next one
-    launch_resolve<64, 128, 64>(p, a, 0, spans);
+    if (!small) launch_resolve<64, 128, 64>(p, a, 0, spans);
     launch_resolve<64, 128, 256>(p, a, 1, spans);
     if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, a, 2, spans);
     if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, a, 3, spans);
@@ -1708,6 +1714,24 @@ int push_domains(xkrt_hip_pool *p)
 
 // one launch over nrec records at rec (device); the record count is already
 // in the counters when `counted`, else it is set by a kernel
+int ticket(xkrt_hip_pool *p);
+
+// a small launch: records read in place from the pinned host buffer at hdr + 32 B
+// (header = record count), no graph, resolve from tier 1
+int enqueue_small(xkrt_hip_pool *p, const uint4 *hdr, uint32_t nrec)
+{
+    int rc = push_domains(p);
+    if (rc) return rc;
+    KArgs a = make_args(p, hdr + 2, NONE);
+    a.nrec_ptr = (const uint32_t *)hdr;
+    a.min_tier = 1;
+    const uint32_t rb = std::min<uint32_t>((nrec + 255) / 256, 8192u);
+    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
+    launch_kernels(p, a, rb, p->timing ? &spans : nullptr);
+    if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
+    return ticket(p);
+}
+
 int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
 {
     int rc = push_domains(p);
@@ -1747,6 +1771,12 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
         launch_kernels(p, a, rb, p->timing ? &spans : nullptr);
         if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
     }
+    return ticket(p);
+}
+
+// the completion ticket of the launch just enqueued
+int ticket(xkrt_hip_pool *p)
+{
     HIPCHK(hipGetLastError());
     p->n_launch++;
     // completion ticket of this launch
@@ -1887,7 +1917,8 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     if (p->rec_dev_hdr) p->rec_dev = p->rec_dev_hdr + 2;
     chk(hipHostMalloc(&p->ctr_host, sizeof(DevCounters), hipHostMallocDefault));
     for (int b = 0; b < 2; ++b) {
-        chk(hipHostMalloc(&p->rec_host_hdr[b], 32ull * (p->rec_cap + 1), hipHostMallocDefault));
+        chk(hipHostMalloc(&p->rec_host_hdr[b], 32ull * (p->rec_cap + 1), hipHostMallocCoherent));
+        if (p->rec_host_hdr[b]) chk(hipHostGetDevicePointer((void **)&p->rec_host_dev[b], p->rec_host_hdr[b], 0));
         chk(hipEventCreateWithFlags(&p->staged[b], hipEventDisableTiming));
     }
     if (p->rec_host_hdr[0]) p->rec_host = p->rec_host_hdr[0] + 2;
@@ -2052,6 +2083,10 @@ int stage(xkrt_hip_pool *p, xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc
         HIPCHK(hipEventSynchronize(p->staged[hb]));
         p->staged_pending[hb] = false;
     }
+    if (p->buf_ticket[hb] > p->done) {  // a small launch still reads this buffer in place
+        const int rw = retire(p, p->buf_ticket[hb]);
+        if (rw) return rw;
+    }
     uint64_t nrect = 0;
     uint32_t ylo = 0, yhi = 0, ntask = 0;
     const uint64_t saved_max = d->max_xlen, room = p->rec_cap - p->open_nrec;
@@ -2083,16 +2118,23 @@ int launch_open(xkrt_hip_pool *p)
     const uint32_t n = p->open_nrec;
     const uint32_t hb = p->cur_host;
     *(uint32_t *)p->rec_host_hdr[hb] = n;
-    HIPCHK(hipMemcpyAsync(p->rec_dev_hdr, p->rec_host_hdr[hb], 32ull * (n + 1), hipMemcpyHostToDevice, p->stream));
-    HIPCHK(hipEventRecord(p->staged[hb], p->stream));
-    p->staged_pending[hb] = true;
+    // a small launch reads its records in place, from pinned host memory: no upload
+    const bool small = n <= p->cfg.small_launch_rects;
+    if (!small) {
+        HIPCHK(hipMemcpyAsync(p->rec_dev_hdr, p->rec_host_hdr[hb], 32ull * (n + 1), hipMemcpyHostToDevice,
+                              p->stream));
+        HIPCHK(hipEventRecord(p->staged[hb], p->stream));
+        p->staged_pending[hb] = true;
+    } else {
+        p->buf_ticket[hb] = p->n_launch + 1;  // the launch enqueued below
+    }
     p->cur_host = hb ^ 1u;
     p->rec_host = p->rec_host_hdr[p->cur_host] + 2;
     p->open_nrec = 0;
     p->open_tasks = 0;
     p->open_acc = 0;
     p->open_batches = 0;
-    const int rc = enqueue(p, p->rec_dev, n, true);
+    const int rc = small ? enqueue_small(p, p->rec_host_dev[hb], n) : enqueue(p, p->rec_dev, n, true);
     if (rc) p->poisoned = true;
     return rc;
 }

@@ -51,7 +51,7 @@ assert RECT_DTYPE.itemsize == 32
 class PoolConfigT(C.Structure):
     _fields_ = [("kernel", C.c_uint32), ("graph", C.c_uint32), ("max_rects", C.c_uint64),
                 ("max_edges", C.c_uint64), ("budget_bytes", C.c_uint64), ("direct_dedup", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("small_launch_rects", C.c_uint32)]
 
 
 class PoolStatsT(C.Structure):
@@ -321,10 +321,12 @@ class Pool:
     hipGraph); the device-memory budget decides which domains are admitted."""
 
     def __init__(self, kernel: int = 1, graph: bool = False, max_rects: int = 1 << 20, max_edges: int = 0,
-                 budget_bytes: int = 0, stream: int | None = None, device: int = 0, direct_dedup: bool = False):
+                 budget_bytes: int = 0, stream: int | None = None, device: int = 0, direct_dedup: bool = False,
+                 small_launch_rects: int = 0):
         self.L = hip_lib()
         _chk(self.L.xkrt_hip_init(device), "xkrt_hip_init")
-        cfg = PoolConfigT(kernel, 1 if graph else 0, max_rects, max_edges, budget_bytes, 1 if direct_dedup else 0, 0)
+        cfg = PoolConfigT(kernel, 1 if graph else 0, max_rects, max_edges, budget_bytes, 1 if direct_dedup else 0,
+                          small_launch_rects)
         h = _p()
         _chk(self.L.xkrt_hip_pool_create(C.byref(cfg), stream, C.byref(h)), "xkrt_hip_pool_create")
         self.h = h

@@ -124,17 +124,19 @@ def col_range(st):
 def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30, kernel: int = 0,
           graph: bool = True, depth: int = 1, max_rects: int = 1 << 20, bin_x_bytes: int = 1024,
           bin_y_cols: int = 128, reserve: bool = True, max_edges: int = 0,
-          direct_dedup: bool = False) -> RunResult:
+          direct_dedup: bool = False, small_launch_rects: int = 16384) -> RunResult:
     """Run one schedule through a fresh pool and serving engine; mode "batched" or "plain".
     Defaults (measured on the 64-client workload): the small LDS class (0), one
     launch in flight (a batch that arrives during a launch goes into the next
-    one, which starts as soon as the current one ends), staged dedup.
+    one, which starts as soon as the current one ends), staged dedup, and
+    launches of up to 16384 rects read in place from pinned host memory.
     reserve: allocate the bins of every client up front (within the budget), so
     admissions do not allocate device memory while serving."""
     byid = {c.cid: c for c in clients}
     nacc = sum(len(c.acc) for c in clients)
     pool = R.Pool(kernel=kernel, graph=graph and mode == "batched", max_rects=max_rects,
-                  budget_bytes=budget_bytes, max_edges=max_edges, direct_dedup=direct_dedup)
+                  budget_bytes=budget_bytes, max_edges=max_edges, direct_dedup=direct_dedup,
+                  small_launch_rects=small_launch_rects)
     srv = None
     try:
         rb = bins_needed(clients, bin_x_bytes, bin_y_cols) * pool.stats()["bytes_per_bin"] if reserve else 0
