@@ -880,7 +880,7 @@ __device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p)
 template <int FCAP, int GCAP, int LCAP, bool DIR>
 __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
 {
-    extern __shared__ unsigned char lds_raw[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
     BinLds<FCAP, GCAP, LCAP, DIR> &s = *reinterpret_cast<BinLds<FCAP, GCAP, LCAP, DIR> *>(lds_raw);
     const uint32_t lane = lane_id();
     if (a.ctr->err) return;  // a poisoned domain touches no state
