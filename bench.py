@@ -352,8 +352,9 @@ def main():
         kt, nl = res["plain"]["kt"]["cholesky_8192_nb128"]
         n_a = len(chol)
         e = res["plain"]["edges"]["cholesky_8192_nb128"]
-        b_alg = (24 * n_a + 64 * n_a + 12 * e) / nl
-        t_res = kt["k_resolve"] / nl * 1e-3
+        per_pass = len(next(p for p in preps if p.name == "cholesky_8192_nb128").batches)  # launches of one pass
+        b_alg = (24 * n_a + 64 * n_a + 12 * e) / per_pass  # one pass resolves every access and edge once
+        t_res = kt["k_resolve"] / nl * 1e-3  # nl timed launches (all passes)
         achieved = b_alg / t_res / 1e9
         traffic = None
         tp = os.path.join(ROOT, "profiles", "resolver_cholesky8192_plain_traffic.json")
