@@ -289,6 +289,11 @@ typedef struct xkrt_hip_server_config {
     uint64_t max_rects;      /* soft cap of rects per launch; 0 = the pool's max_rects */
     uint64_t reserve_bytes;  /* bin memory allocated up front (within the budget), so
                                 admission does not allocate while serving */
+    uint32_t prearm;         /* 1: the next launch is queued on the device while the current one
+                                runs, held by a gate kernel that the engine opens by writing pinned
+                                memory once the batches are staged (no launch calls on the critical
+                                path); needs depth = 1 and a pool with small_launch_rects > 0 */
+    uint32_t pad;
 } xkrt_hip_server_config_t;
 
 typedef struct xkrt_hip_server_stats {
@@ -299,6 +304,7 @@ typedef struct xkrt_hip_server_stats {
     uint64_t waited;         /* admission attempts refused for lack of budget */
     uint64_t harvests;       /* edge copies */
     uint64_t consumes;       /* edge buffer resets */
+    uint64_t gated;          /* launches started by opening a gate (prearm) */
     int64_t error;           /* first pool-level error (negative errno), 0 if none */
 } xkrt_hip_server_stats_t;
 

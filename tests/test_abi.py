@@ -43,8 +43,8 @@ def test_ctypes_layouts_match_header():
     assert R.RECT_DTYPE.itemsize == 32
     assert C.sizeof(R.PoolConfigT) == 2 * 4 + 3 * 8 + 2 * 4
     assert C.sizeof(R.PoolStatsT) == 8 * 8
-    assert C.sizeof(R.ServerConfigT) == 2 * 4 + 2 * 8
-    assert C.sizeof(R.ServerStatsT) == 8 * 8
+    assert C.sizeof(R.ServerConfigT) == 2 * 4 + 2 * 8 + 2 * 4
+    assert C.sizeof(R.ServerStatsT) == 9 * 8
     assert R.ACCESS_DTYPE.itemsize == C.sizeof(R.AccessT) == 32
 
 

@@ -1268,6 +1268,33 @@ __global__ void k_copy(uint4 *dst, const uint4 *src, uint64_t n16)
         dst[i] = src[i];
 }
 
+// ---------------------------------------------------------------- launch gate
+//
+// A launch enqueued before its records exist.  k_gate (one lane) waits until
+// the host writes the launch's sequence number into the pinned header of the
+// staging buffer -- after the records and their count -- and hands the count
+// to the kernels queued behind it, so the host's part of such a launch is
+// writing memory.  The wait is bounded: after `limit` wall-clock ticks the gate
+// gives up and the launch runs empty.  Header words: [0] record count,
+// [1] sequence number, [2] the gate's result (written here).
+enum : uint32_t { GATE_WAIT = 0, GATE_OPEN = 1, GATE_EXPIRED = 2 };
+
+__global__ void k_gate(uint32_t *hdr, uint32_t seq, unsigned long long limit, uint32_t *count)
+{
+    if (threadIdx.x != 0) return;
+    const unsigned long long t0 = wall_clock64();
+    uint32_t result = GATE_OPEN;
+    while (__hip_atomic_load(&hdr[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != seq) {
+        if (wall_clock64() - t0 > limit) {
+            result = GATE_EXPIRED;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    *count = result == GATE_OPEN ? __hip_atomic_load(&hdr[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : 0u;
+    __hip_atomic_store(&hdr[2], result, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ void k_consume(DevCounters *c)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) c->edge_total = 0;
@@ -1382,6 +1409,12 @@ struct xkrt_hip_pool {
     uint32_t dom_slot = 0;
     // each launch's outcome, written by its k_final (slot = ticket % NSNAP)
     LaunchSnap *snap = nullptr;
+    // launch gate: a small launch enqueued ahead of its records (serving engine)
+    bool gate_armed = false;
+    uint32_t gate_hb = 0, gate_seq = 0;
+    uint64_t gate_ticket = 0;
+    uint32_t *gate_count = nullptr;      // device word: the gated launch's record count
+    unsigned long long gate_limit = 0;   // wall-clock ticks a gate waits (1 s)
     struct xkrt_hip_server *server = nullptr;
     uint32_t open_nrec = 0, open_tasks = 0;
     uint64_t open_acc = 0, open_batches = 0;
@@ -1845,6 +1878,7 @@ void destroy_pool(xkrt_hip_pool *p)
     void *dev[] = {p->bin_nf, p->bin_nrs, p->frag, p->rs, p->bin_cnt, p->bin_cur, p->bin_dom, p->active, p->touched,
                    p->d_dom,
                    p->binrec, p->cand, p->cstr, p->blk, p->hkeys, p->hvals, p->eslot, p->edges, p->edom, p->ctr,
+                   p->gate_count,
                    p->rec_dev_hdr};
     for (void *q : dev)
         if (q) (void)hipFree(q);
@@ -1912,6 +1946,13 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     chk(hipMalloc(&p->edges, 8ull * p->edge_cap));
     chk(hipMalloc(&p->edom, 4ull * p->edge_cap));
     chk(hipMalloc(&p->ctr, sizeof(DevCounters)));
+    chk(hipMalloc(&p->gate_count, 64));
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, p->device) != hipSuccess || khz <= 0)
+            khz = 100000;
+        p->gate_limit = 1000ull * (unsigned long long)khz;  // 1 s
+    }
     chk(hipMalloc(&p->d_dom, sizeof(p->h_dom)));
     chk(hipMalloc(&p->rec_dev_hdr, 32ull * (p->rec_cap + 1)));
     if (p->rec_dev_hdr) p->rec_dev = p->rec_dev_hdr + 2;
@@ -2137,6 +2178,93 @@ int launch_open(xkrt_hip_pool *p)
     const int rc = small ? enqueue_small(p, p->rec_host_dev[hb], n) : enqueue(p, p->rec_dev, n, true);
     if (rc) p->poisoned = true;
     return rc;
+}
+
+// Arms the gate: enqueues a small launch over the staging buffer the next
+// stage writes into, held by k_gate until gate_ring (or gate_cancel).  Nothing
+// else may be enqueued on the stream while it is armed.
+int gate_arm(xkrt_hip_pool *p)
+{
+    if (p->gate_armed) return 0;
+    if (p->open_nrec) return -EBUSY;
+    const uint32_t hb = p->cur_host;
+    if (p->staged_pending[hb]) {
+        HIPCHK(hipEventSynchronize(p->staged[hb]));
+        p->staged_pending[hb] = false;
+    }
+    if (p->buf_ticket[hb] > p->done) {
+        const int rw = retire(p, p->buf_ticket[hb]);
+        if (rw) return rw;
+    }
+    int rc = push_domains(p);
+    if (rc) return rc;
+    volatile uint32_t *h = (volatile uint32_t *)p->rec_host_hdr[hb];
+    const uint32_t seq = ++p->gate_seq;
+    h[0] = 0;
+    h[1] = seq - 1;
+    h[2] = GATE_WAIT;
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, p->stream, (uint32_t *)p->rec_host_dev[hb], seq, p->gate_limit,
+                       p->gate_count);
+    KArgs a = make_args(p, p->rec_host_dev[hb] + 2, NONE);
+    a.nrec_ptr = p->gate_count;
+    a.min_tier = 1;
+    // grids for any count: k_count and k_fill stride over the records
+    launch_kernels(p, a, std::max<uint32_t>(1u, std::min<uint32_t>((p->cfg.small_launch_rects + 255) / 256, 256u)),
+                   nullptr);
+    rc = ticket(p);
+    if (rc) return rc;
+    p->gate_armed = true;
+    p->gate_hb = hb;
+    p->gate_ticket = p->n_launch;
+    return 0;
+}
+
+// opens the armed gate on the staged records (count 0: an empty launch)
+void gate_open(xkrt_hip_pool *p, uint32_t n)
+{
+    const uint32_t hb = p->gate_hb;
+    volatile uint32_t *h = (volatile uint32_t *)p->rec_host_hdr[hb];
+    h[0] = n;
+    std::atomic_thread_fence(std::memory_order_seq_cst);  // records and count before the sequence number
+    h[1] = p->gate_seq;
+    p->buf_ticket[hb] = p->gate_ticket;
+    p->gate_armed = false;
+}
+
+// resolves the staged records through the armed gate; *ticket = the launch
+int gate_ring(xkrt_hip_pool *p, uint64_t *ticket_out)
+{
+    const uint32_t n = p->open_nrec;
+    gate_open(p, n);
+    *ticket_out = p->gate_ticket;
+    p->cur_host = p->gate_hb ^ 1u;
+    p->rec_host = p->rec_host_hdr[p->cur_host] + 2;
+    p->open_nrec = 0;
+    p->open_tasks = 0;
+    p->open_acc = 0;
+    p->open_batches = 0;
+    return 0;
+}
+
+// lets the armed gate's launch run empty and waits until the gate has seen
+// it, so the staging buffer's header is the host's again
+int gate_cancel(xkrt_hip_pool *p)
+{
+    if (!p->gate_armed) return 0;
+    const uint32_t hb = p->gate_hb;
+    gate_open(p, 0);
+    volatile uint32_t *h = (volatile uint32_t *)p->rec_host_hdr[hb];
+    const auto t0 = std::chrono::steady_clock::now();
+    while (h[2] == GATE_WAIT) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(5)) return -EIO;
+        std::this_thread::yield();
+    }
+    if (!p->open_nrec) {  // nothing staged in this buffer: the next stage takes the other one
+        p->cur_host = hb ^ 1u;
+        p->rec_host = p->rec_host_hdr[p->cur_host] + 2;
+    }
+    return 0;
 }
 
 } // namespace
@@ -2572,6 +2700,7 @@ struct SrvClient {
 struct SrvFlight {
     uint64_t ticket;
     std::vector<SrvRequest *> reqs;
+    int gate_hb;  // staging buffer of a gated launch, -1 otherwise
 };
 
 double srv_us(std::chrono::steady_clock::time_point t0)
@@ -2607,7 +2736,7 @@ struct xkrt_hip_server {
     uint32_t *he = nullptr, *hd = nullptr;  // pinned harvest buffers
     uint64_t hcap = 0;
     // statistics, in the order of xkrt_hip_server_stats_t
-    enum { S_LAUNCHES, S_BATCHES, S_ADMITTED, S_LEFT, S_WAITED, S_HARVESTS, S_CONSUMES, S_N };
+    enum { S_LAUNCHES, S_BATCHES, S_ADMITTED, S_LEFT, S_WAITED, S_HARVESTS, S_CONSUMES, S_GATED, S_N };
     std::atomic<uint64_t> st[S_N] = {};
     void count(int k, uint64_t v = 1) { st[k].fetch_add(v, std::memory_order_relaxed); }
 };
@@ -2651,6 +2780,8 @@ int srv_retire(xkrt_hip_server *s, bool &work)
     if (rc) return rc;
     while (!s->flights.empty() && s->flights.front().ticket <= p->done) {
         SrvFlight &f = s->flights.front();
+        // a gate that gave up before the engine opened it: the batches were not resolved
+        if (f.gate_hb >= 0 && ((volatile uint32_t *)p->rec_host_hdr[f.gate_hb])[2] != GATE_OPEN) return -EIO;
         // the launch's k_final wrote its outcome before the launch completed
         LaunchSnap *o = &p->snap[f.ticket % NSNAP];
         if (__atomic_load_n(&o->launch, __ATOMIC_ACQUIRE) != f.ticket) return -EIO;
@@ -2682,6 +2813,24 @@ int srv_launch(xkrt_hip_server *s, bool &work)
     if (s->hold || s->pending.empty() || s->flights.size() >= s->cfg.depth) return 0;
     const uint64_t round = ++s->round;
     const uint64_t cap = s->cfg.max_rects ? s->cfg.max_rects : p->rec_cap;
+    // admissions first: they enqueue the new bins' resets, which must run before
+    // the launch that uses them (so an armed gate is cancelled first)
+    for (SrvRequest *r : s->pending) {
+        SrvClient *c = r->c;
+        if (c->dom || c->status || c->round == round) continue;
+        if (p->gate_armed) {
+            const int rc = gate_cancel(p);
+            if (rc) return rc;
+        }
+        const int rc = srv_admit(s, c);
+        if (rc == -ENOMEM) {
+            s->count(xkrt_hip_server::S_WAITED);
+            c->round = round;
+            continue;
+        }
+        if (rc) c->status = rc;
+        work = true;
+    }
     std::vector<SrvRequest *> batch;
     std::deque<SrvRequest *> keep;
     while (!s->pending.empty()) {
@@ -2693,25 +2842,15 @@ int srv_launch(xkrt_hip_server *s, bool &work)
             work = true;
             continue;
         }
-        if (c->round == round) {  // keeps the client's batches in order
+        if (c->round == round || !c->dom) {  // waits for admission; keeps the client's batches in order
+            c->round = round;
             keep.push_back(r);
             s->pending.pop_front();
             continue;
         }
         if ((s->cfg.max_batches && batch.size() >= s->cfg.max_batches) || p->open_nrec >= cap) break;
-        if (!c->dom) {
-            const int rc = srv_admit(s, c);
-            if (rc == -ENOMEM) {
-                s->count(xkrt_hip_server::S_WAITED);
-                c->round = round;
-                continue;
-            }
-            if (rc) {
-                c->status = rc;
-                continue;
-            }
-            work = true;
-        }
+        // a gated launch reads its records in place: it stays a small launch (at most 3 rects per access)
+        if (p->gate_armed && !batch.empty() && p->open_nrec + 3 * r->acc.size() > p->cfg.small_launch_rects) break;
         uint64_t first = 0;
         const int rc = stage(p, c->dom, r->acc.data(), r->acc.size(), &first);
         if (p->poisoned) return -EIO;
@@ -2729,28 +2868,40 @@ int srv_launch(xkrt_hip_server *s, bool &work)
         keep.pop_back();
     }
     if (batch.empty()) return 0;
-    const int rc = launch_open(p);
-    if (rc) return rc;
+    uint64_t tk = 0;
+    int ghb = -1;
+    if (p->gate_armed && p->open_nrec <= std::max<uint64_t>(p->cfg.small_launch_rects, 1)) {
+        ghb = (int)p->gate_hb;
+        gate_ring(p, &tk);  // the launch is already queued: opening the gate starts it
+        s->count(xkrt_hip_server::S_GATED);
+    } else {
+        if (p->gate_armed) {
+            const int rc = gate_cancel(p);
+            if (rc) return rc;
+        }
+        const int rc = launch_open(p);
+        if (rc) return rc;
+        tk = p->n_launch;
+    }
     for (SrvRequest *r : batch) std::vector<xkrt_hip_access_t>().swap(r->acc);
-    s->flights.push_back({p->n_launch, std::move(batch)});
+    s->flights.push_back({tk, std::move(batch), ghb});
     s->count(xkrt_hip_server::S_LAUNCHES);
     work = true;
+    // queue the next launch behind this one while the device works
+    if (s->cfg.prearm) {
+        const int rc = gate_arm(p);
+        if (rc) return rc;
+    }
     return 0;
 }
 
 int srv_harvest(xkrt_hip_server *s)
 {
     xkrt_hip_pool *p = s->p;
-    const uint64_t a = s->harvested, n = s->edges_seen - a;
+    // at most hcap edges per copy: the pinned buffers are allocated once, at creation
+    // (freeing pinned memory may synchronise the device)
+    const uint64_t a = s->harvested, n = std::min(s->edges_seen - a, s->hcap);
     if (!n) return 0;
-    if (n > s->hcap) {
-        if (s->he) (void)hipHostFree(s->he);
-        if (s->hd) (void)hipHostFree(s->hd);
-        s->he = s->hd = nullptr;
-        s->hcap = std::max<uint64_t>(n, 1u << 16);
-        HIPCHK(hipHostMalloc(&s->he, 8 * s->hcap, hipHostMallocDefault));
-        HIPCHK(hipHostMalloc(&s->hd, 4 * s->hcap, hipHostMallocDefault));
-    }
     // retired launches never change these edges again; later launches append after them
     HIPCHK(hipMemcpyAsync(s->he, p->edges + 2 * a, 8 * n, hipMemcpyDeviceToHost, s->hs));
     HIPCHK(hipMemcpyAsync(s->hd, p->edom + a, 4 * n, hipMemcpyDeviceToHost, s->hs));
@@ -2765,7 +2916,7 @@ int srv_harvest(xkrt_hip_server *s)
             c->edges.push_back(s->he[2 * i + 1] - f);
         }
     }
-    s->harvested = s->edges_seen;
+    s->harvested = a + n;
     s->count(xkrt_hip_server::S_HARVESTS);
     return 0;
 }
@@ -2787,12 +2938,20 @@ int srv_housekeep(xkrt_hip_server *s, bool &work, double idle_us)
     }
     // reset the half-used edge buffer once the pipeline has drained and every edge is out
     if (s->hold && s->flights.empty() && s->harvested == s->edges_seen) {
+        const int rc = gate_cancel(p);  // the reset must not wait behind a gated launch
+        if (rc) return rc;
         hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, p->stream, p->ctr);
         HIPCHK(hipGetLastError());
         s->edges_seen = s->harvested = 0;
         for (SrvClient *c : s->live) c->need = 0;
         s->hold = false;
         s->count(xkrt_hip_server::S_CONSUMES);
+        work = true;
+    }
+    // a gate armed for long without work runs its launch empty (it would give up after 1 s)
+    if (p->gate_armed && s->flights.empty() && s->pending.empty() && idle_us > 200000.0) {
+        const int rc = gate_cancel(p);
+        if (rc) return rc;
         work = true;
     }
     if (!leaving) return 0;
@@ -2844,6 +3003,7 @@ void srv_loop(xkrt_hip_server *s)
         const double now = srv_us(s->t0);
         if (!rc) rc = srv_housekeep(s, work, now - last_work);
         if (rc) {
+            (void)gate_cancel(s->p);
             s->err.store(rc, std::memory_order_release);
             // refuse everything not resolved, so that drains end
             const double t = srv_us(s->t0);
@@ -2858,7 +3018,11 @@ void srv_loop(xkrt_hip_server *s)
             last_work = now;
             continue;
         }
-        if (s->stop.load(std::memory_order_acquire) && s->flights.empty()) break;
+        if (s->stop.load(std::memory_order_acquire) && s->flights.empty()) {
+            const int rc2 = gate_cancel(s->p);
+            if (rc2) s->err.store(rc2, std::memory_order_release);
+            break;
+        }
         // poll: an arrival is seen within microseconds; after 2 ms without work, back off
         if (s->flights.empty() && now - last_work > 2000.0) std::this_thread::sleep_for(std::chrono::microseconds(50));
         else std::this_thread::yield();
@@ -2889,6 +3053,8 @@ int xkrt_hip_server_create(xkrt_hip_pool_t p, const xkrt_hip_server_config_t *cf
     if (cfg) c = *cfg;
     if (c.depth == 0) c.depth = 2;
     if (c.depth > 16) return -EINVAL;
+    // a gated launch waits in the stream: only with one launch in flight, over small launches
+    if (c.prearm && (c.depth != 1 || p->cfg.small_launch_rects == 0)) return -EINVAL;
     int rc = read_counters(p);  // edges already in the buffer are not the engine's
     if (rc) return rc;
     if (c.reserve_bytes) {
@@ -2903,7 +3069,12 @@ int xkrt_hip_server_create(xkrt_hip_pool_t p, const xkrt_hip_server_config_t *cf
     s->p = p;
     s->cfg = c;
     s->edges_seen = s->harvested = p->ctr_host->edge_total;
-    if (hipStreamCreateWithFlags(&s->hs, hipStreamNonBlocking) != hipSuccess) {
+    s->hcap = std::min<uint64_t>(p->edge_cap, 1ull << 20);
+    if (hipStreamCreateWithFlags(&s->hs, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc(&s->he, 8 * s->hcap, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&s->hd, 4 * s->hcap, hipHostMallocDefault) != hipSuccess) {
+        if (s->he) (void)hipHostFree(s->he);
+        if (s->hd) (void)hipHostFree(s->hd);
         delete s;
         return -ENOMEM;
     }

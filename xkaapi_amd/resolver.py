@@ -61,12 +61,12 @@ class PoolStatsT(C.Structure):
 
 class ServerConfigT(C.Structure):
     _fields_ = [("depth", C.c_uint32), ("max_batches", C.c_uint32), ("max_rects", C.c_uint64),
-                ("reserve_bytes", C.c_uint64)]
+                ("reserve_bytes", C.c_uint64), ("prearm", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class ServerStatsT(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("launches", "batches", "admitted", "left", "waited", "harvests",
-                                          "consumes")] + [("error", C.c_int64)]
+                                          "consumes", "gated")] + [("error", C.c_int64)]
 
 
 ERR_BITS = {1: "incidence array", 2: "bin record list", 4: "bin fragments", 8: "bin reader log",
@@ -429,10 +429,10 @@ class Server:
     launch), and stamps submit and ready times.  Clients are small ints."""
 
     def __init__(self, pool: Pool, depth: int = 2, max_batches: int = 0, max_rects: int = 0,
-                 reserve_bytes: int = 0):
+                 reserve_bytes: int = 0, prearm: bool = False):
         self.L = hip_lib()
         self.pool = pool
-        cfg = ServerConfigT(depth, max_batches, max_rects, reserve_bytes)
+        cfg = ServerConfigT(depth, max_batches, max_rects, reserve_bytes, 1 if prearm else 0, 0)
         h = _p()
         _chk(self.L.xkrt_hip_server_create(pool.h, C.byref(cfg), C.byref(h)), "xkrt_hip_server_create")
         self.h = h
