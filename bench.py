@@ -43,32 +43,62 @@ def weighted_pct(values, weights, q):
 
 
 class Prepared:
-    """Device-resident rect records of one stream, per plain batch and per merged group."""
+    """Device-resident rect records of one stream, per plain batch and per merged
+    group, for each candidate number of batches per launch (merges)."""
 
-    def __init__(self, name, st, dom_kw, merge, torch):
+    def __init__(self, name, st, dom_kw, merges, torch):
         from xkaapi_amd import resolver as R
         self.name, self.st = name, st
         ld, s = int(st.ld[0]), int(st.s[0])
         self.acc = R.to_access_array(st.task, st.mode, st.addr, st.m, st.n)
         self.batches = R.batches_by_tasks(st.task, 4096)
-        self.groups = [(self.batches[i][0], self.batches[min(i + merge, len(self.batches)) - 1][1])
-                       for i in range(0, len(self.batches), merge)]
-        maxrec = 2 * max(b - a for a, b in self.groups)
-        self.dom = R.DependencyDomain(ld, s, max_rects=maxrec, max_edges=8 * len(st) + (1 << 20),
-                                      stream=torch.cuda.current_stream().cuda_stream, **dom_kw)
-        self.plain = [self._upload(a, b, torch) for a, b in self.batches]
-        self.merged = [self._upload(a, b, torch) for a, b in self.groups]
+        groups = {m: [(self.batches[i][0], self.batches[min(i + m, len(self.batches)) - 1][1])
+                      for i in range(0, len(self.batches), m)] for m in merges}
+        # one domain per form, each sized for its own launches
+        stream = torch.cuda.current_stream().cuda_stream
+        self.doms = {
+            "plain": R.DependencyDomain(ld, s, max_rects=2 * max(b - a for a, b in self.batches),
+                                        max_edges=8 * len(st) + (1 << 20), stream=stream, **dom_kw),
+            "batched": R.DependencyDomain(ld, s, max_rects=2 * max(b - a for g in groups.values() for a, b in g),
+                                          max_edges=8 * len(st) + (1 << 20), stream=stream, **dom_kw)}
+        self.plain = [self._upload(self.doms["plain"], a, b, torch) for a, b in self.batches]
+        self.merged_by = {m: [self._upload(self.doms["batched"], a, b, torch) for a, b in g]
+                          for m, g in groups.items()}
+        self.merge = merges[0]
+        self.tried = {}
         self.ntasks_plain = [int(np.count_nonzero(np.diff(st.task[a:b].astype(np.int64))) + 1) for a, b in self.batches]
 
-    def _upload(self, a, b, torch):
-        recs, ylo, yhi = self.dom.build_records(self.acc[a:b], a)
+    @property
+    def merged(self):
+        return self.merged_by[self.merge]
+
+    def tune_merge(self, torch):
+        """Batches per launch of the batched form: the fastest candidate on one pass."""
+        best = None
+        for m in self.merged_by:
+            self.merge = m
+            self.run("batched")
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            self.run("batched")
+            self.doms["batched"].sync()
+            dt = time.perf_counter() - t
+            self.tried[m] = round(dt * 1e3, 3)
+            if best is None or dt < best[1]:
+                best = (m, dt)
+        self.merge = best[0]
+        return self.merge
+
+    def _upload(self, dom, a, b, torch):
+        recs, ylo, yhi = dom.build_records(self.acc[a:b], a)
         dev = torch.from_numpy(recs.view(np.int32).reshape(-1)).cuda()
         return dev, len(recs), b - a, ylo, yhi
 
     def run(self, which):
-        self.dom.reset()
+        dom = self.doms[which]
+        dom.reset()
         for dev, nrec, nacc, ylo, yhi in (self.plain if which == "plain" else self.merged):
-            self.dom.submit_device(dev.data_ptr(), nrec, nacc, ylo, yhi)
+            dom.submit_device(dev.data_ptr(), nrec, nacc, ylo, yhi)
 
 
 def cpu_baseline(st, target_s=10.0):
@@ -237,6 +267,17 @@ def serving_leg(fracs=(0.1, 0.3, 0.5, 0.7, 0.9)):
                       "driver_lag_p99_us": round(float(np.percentile(r.lag_us, 99)), 1)}
         out["sweep"].append(row)
         log(f"serving load {f}: {row}")
+    # throughput-oriented engine: launches queued on the device ahead of their batches (prearm),
+    # which also caps a launch at the small-launch size
+    pa = V.saturation(cs, "batched", prearm=True)
+    prate = nb / pa.makespan_s
+    out["prearm"] = {"saturation_accesses_per_s": round(pa.rate, 1), "launches": pa.launches, "sweep": []}
+    for f in (0.5, 0.7):
+        r = V.serve(cs, V.poisson_schedule(cs, f * prate, seed=11), "batched", prearm=True)
+        out["prearm"]["sweep"].append({"load": f, "accesses_per_s": round(r.rate, 1), "p50_us": round(r.pct(50), 1),
+                                       "p99_us": round(r.pct(99), 1), "launches": r.launches,
+                                       "gated": r.stats["gated"]})
+    log(f"serving prearm: {out['prearm']}")
     return out
 
 
@@ -246,7 +287,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--random-n", type=int, default=1_000_000)
-    ap.add_argument("--merge", type=int, default=16, help="batches per launch in the batched form")
+    ap.add_argument("--merge", type=int, default=0, help="batches per launch in the batched form (0: tuned)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("all", "cholesky", "random"), default="all",
                     help="restrict to one stream (profiling runs)")
@@ -272,11 +313,12 @@ def main():
     rnd = S.random2d(args.random_n) if args.workload != "cholesky" else None
     log(f"streams: cholesky {len(chol)} acc / {chol.ntasks} tasks ({time.time()-t0:.1f}s)")
     preps = []
+    merges = [args.merge] if args.merge else [16, 32, 64, 128]  # batches per launch: given, or tuned
     if args.workload in ("all", "cholesky"):
         preps.append(Prepared("cholesky_8192_nb128", chol, dict(bin_x_bytes=1024, bin_y_cols=128, kernel=0),
-                              args.merge, torch))
+                              merges, torch))
     if rnd is not None:
-        preps.append(Prepared("random2d_65536", rnd, dict(bin_x_bytes=2048, bin_y_cols=256, kernel=1), args.merge,
+        preps.append(Prepared("random2d_65536", rnd, dict(bin_x_bytes=2048, bin_y_cols=256, kernel=1), merges,
                               torch))
     torch.cuda.synchronize()
     log(f"records resident in HBM ({time.time()-t0:.1f}s)")
@@ -287,18 +329,22 @@ def main():
         p0 = preps[0]
         for form in forms:
             p0.run(form)
-            assert p0.dom.sync() == 131040, "Cholesky 8192/128 must have 131,040 edges"
+            assert p0.doms[form].sync() == 131040, "Cholesky 8192/128 must have 131,040 edges"
 
     for _ in range(args.warmup):
         for p in preps:
             for form in forms:
                 p.run(form)
     torch.cuda.synchronize()
+    if "batched" in forms and len(merges) > 1:  # the autotuner's pick of batches per launch, per stream
+        for p in preps:
+            log(f"{p.name}: batches per launch {p.tune_merge(torch)} (ms per pass: {p.tried})")
     for p in preps:
-        p.dom.sync()
-        p.dom.kernel_times(clear=True)
-        p.dom.launch_times(clear=True)
-        p.dom.set_timing(True)
+        for d in p.doms.values():
+            d.sync()
+            d.kernel_times(clear=True)
+            d.launch_times(clear=True)
+            d.set_timing(True)
 
     if args.calibrate_mib:
         from xkaapi_amd import resolver as R
@@ -320,9 +366,9 @@ def main():
                 p.run(form)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t1) / args.steps
-        kt = {p.name: p.dom.kernel_times(clear=True) for p in preps}
-        lt = {p.name: p.dom.launch_times(clear=True) for p in preps}
-        edges = {p.name: p.dom.sync() for p in preps}
+        kt = {p.name: p.doms[form].kernel_times(clear=True) for p in preps}
+        lt = {p.name: p.doms[form].launch_times(clear=True) for p in preps}
+        edges = {p.name: p.doms[form].sync() for p in preps}
         res[form] = dict(ms=dt * 1e3, rate=nacc / dt, kt=kt, lt=lt, edges=edges)
         log(f"{form}: {dt*1e3:.2f} ms/step, {nacc/dt:.4g} acc/s, edges {edges}")
         for name, (k, n) in kt.items():
@@ -397,7 +443,8 @@ def main():
                                + " + ".join(p.name for p in preps),
                    "accesses": nacc, "batch_tasks": 4096, "parallelism": "single GPU"},
         "batched": {"value": round(res["batched"]["rate"], 1), "ms_per_step": round(res["batched"]["ms"], 3),
-                    "batches_per_launch": args.merge,
+                    "batches_per_launch": {p.name: p.merge for p in preps},
+                    "batches_per_launch_tried_ms": {p.name: p.tried for p in preps},
                     "speedup_vs_plain": round(res["batched"]["rate"] / res["plain"]["rate"], 3)},
         "p50_batch_latency_us": round(p50, 2),
         "p99_batch_latency_us": round(p99, 2),
