@@ -124,7 +124,8 @@ def col_range(st):
 def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30, kernel: int = 0,
           graph: bool = True, depth: int = 1, max_rects: int = 1 << 20, bin_x_bytes: int = 1024,
           bin_y_cols: int = 128, reserve: bool = True, max_edges: int = 0,
-          direct_dedup: bool = False, small_launch_rects: int = 16384, prearm: bool = False) -> RunResult:
+          direct_dedup: bool = False, small_launch_rects: int = 16384, prearm: bool = False,
+          launch_rects: int = 0) -> RunResult:
     """Run one schedule through a fresh pool and serving engine; mode "batched" or "plain".
     Defaults (measured on the 64-client workload): the small LDS class (0), one
     launch in flight (a batch that arrives during a launch goes into the next
@@ -140,7 +141,8 @@ def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30
     srv = None
     try:
         rb = bins_needed(clients, bin_x_bytes, bin_y_cols) * pool.stats()["bytes_per_bin"] if reserve else 0
-        srv = R.Server(pool, depth=depth, max_batches=1 if mode == "plain" else 0, reserve_bytes=rb,
+        srv = R.Server(pool, depth=depth, max_batches=1 if mode == "plain" else 0, max_rects=launch_rects,
+                       reserve_bytes=rb,
                        prearm=prearm and depth == 1 and small_launch_rects > 0)
         ids = {c.cid: srv.client(int(c.stream.ld[0]), int(c.stream.s[0]), col_range(c.stream), bin_x_bytes,
                                  bin_y_cols) for c in clients}
