@@ -17,7 +17,7 @@
 // One launch resolves one batch of records in six kernels:
 //   k_count   records -> per-bin record counts
 //   k_alloc   per-bin segments in the incidence array, per-tier active-bin lists
-//   k_fill    incidence array (per-bin segments)
+//             (k_count also writes each incidence into its bin's bucket)
 //   k_resolve one wave64 per active bin: the bin's fragments, reader log and
 //             incidences move to LDS, its records are put in spawn order
 //             (rank sort up to 64, bitonic on 64-bit keys above) and replayed
@@ -122,11 +122,11 @@ struct KArgs {
     uint32_t *frag;           // nbins * 5 * fcap (SoA x0, x1, y0, y1, lw)
     uint32_t *rs;             // nbins * 5 * rcap reader log (SoA x0, x1, y0, y1, acc)
     uint32_t fcap, rcap;
-    uint32_t *bin_cnt, *bin_cur;
+    uint32_t *bin_cnt;        // incidences per bin this launch (zeroed by k_resolve)
     uint32_t *touched;        // bins with records this launch, in first-touch order
-    uint4 *binrec;  // 2 x uint4 per incidence: {x0, x1, y0, y1}, {acc, tm, record index, 0}
+    uint4 *binrec;  // per bin a bucket of lmax incidences, 2 x uint4 each: {x0, x1, y0, y1}, {acc, tm, record, 0}
     uint4 *active;  // per tier, nbins entries {slot, incidence start, record count, nf | ng << 16}
-    uint32_t binrec_cap;
+    uint32_t lmax;            // bucket capacity: records per bin and launch of the highest tier
     uint32_t nbins;
     uint32_t max_tier;                              // tiers 0..max_tier; storage sized for max_tier
     uint32_t min_tier;                              // first tier a bin may take (1 in small launches)
@@ -255,7 +255,20 @@ __global__ void k_count(KArgs a)
             const uint32_t k = k0 + lane;
             uint32_t owner;
             const uint32_t bin = wave_rec_bin(q, min(k, total - 1), owner);
-            const bool first = k < total && atomicAdd(&a.bin_cnt[bin], 1u) == 0;
+            bool first = false;
+            if (k < total) {
+                // the incidence goes straight into its bin's bucket; a bin with more than lmax
+                // incidences fits no tier and fails in k_resolve
+                const uint32_t r = r0 + owner;
+                const uint4 q0 = a.rec[2ull * r], q1 = a.rec[2ull * r + 1];
+                const uint32_t pos = atomicAdd(&a.bin_cnt[bin], 1u);
+                first = pos == 0;
+                if (pos < a.lmax) {
+                    uint4 *o = a.binrec + 2ull * ((size_t)bin * a.lmax + pos);
+                    o[0] = q0;
+                    o[1] = make_uint4(q1.x, q1.y, r, 0u);
+                }
+            }
             // the bins this launch touches, listed once each
             const uint64_t m = __ballot(first);
             if (m) {
@@ -268,8 +281,8 @@ __global__ void k_count(KArgs a)
     }
 }
 
-// per-bin segments of the incidence array over the touched bins; each active
-// bin goes to the smallest LDS tier whose capacities fit it with headroom
+// the touched bins' tier lists: each active bin goes to the smallest LDS tier
+// whose capacities fit it with headroom
 __global__ void k_alloc(KArgs a, uint32_t nbins)
 {
     if (a.ctr->err) return;
@@ -279,16 +292,9 @@ __global__ void k_alloc(KArgs a, uint32_t nbins)
         const uint32_t ti = s0 + lane_id();
         const uint32_t slot = ti < nt ? a.touched[ti] : 0u;
         const uint32_t c = ti < nt ? a.bin_cnt[slot] : 0u;
-        const uint32_t incl = wave_incl_scan(c);
-        const uint32_t sum = wave_bcast(incl, 63);
         if (__ballot(c > 0) == 0) continue;
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&a.ctr->total, sum);
-        base = wave_bcast(base, 0);
-        if (lane_id() == 0 && (uint64_t)base + sum > a.binrec_cap) atomicOr(&a.ctr->err, ERR_BINREC);
         uint32_t tier = a.min_tier, nfl = 0, ngl = 0;
         if (c > 0) {
-            a.bin_cur[slot] = base + incl - c;
             nfl = a.bin_nf[slot];
             ngl = a.bin_nrs[slot];
             while (tier < a.max_tier &&
@@ -308,33 +314,7 @@ __global__ void k_alloc(KArgs a, uint32_t nbins)
                 const uint32_t r = tb + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
                 if (mine)
                     a.active[(size_t)t * a.nbins + (heavy ? r : a.nbins - 1 - r)] =
-                        make_uint4(slot, base + incl - c, c, nfl | (ngl << 11) | (a.bin_dom[slot] << 24));
-            }
-        }
-    }
-}
-
-__global__ void k_fill(KArgs a)
-{
-    if (a.ctr->err) return;
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nrec = a.ctr->nrec;
-    const uint32_t lane = lane_id();
-    for (uint32_t r0 = tid & ~63u; r0 < nrec; r0 += gridDim.x * blockDim.x) {
-        const WaveRecs q = wave_recs(a, r0 + lane, nrec);
-        const uint32_t total = __shfl(q.incl, 63, 64);
-        for (uint32_t k0 = 0; k0 < total; k0 += 64) {
-            const uint32_t k = k0 + lane;
-            uint32_t owner;
-            const uint32_t bin = wave_rec_bin(q, min(k, total - 1), owner);
-            if (k < total) {
-                const uint32_t r = r0 + owner;
-                const uint32_t pos = atomicAdd(&a.bin_cur[bin], 1u);
-                if (pos < a.binrec_cap) {
-                    const uint4 q1 = a.rec[2ull * r + 1];
-                    a.binrec[2ull * pos] = a.rec[2ull * r];
-                    a.binrec[2ull * pos + 1] = make_uint4(q1.x, q1.y, r, 0u);
-                }
+                        make_uint4(slot, 0u, c, nfl | (ngl << 11) | (a.bin_dom[slot] << 24));
             }
         }
     }
@@ -861,12 +841,12 @@ struct BinPre {
 __device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p)
 {
     const uint32_t i = lane_id();
-    const uint32_t slot = e.x, start = e.y, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
+    const uint32_t slot = e.x, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
     p.q0 = make_uint4(0, 0, 0, 0);
     p.q1 = make_uint4(NONE, 0, NONE, 0);
     p.f0 = p.f1 = p.f2 = p.f3 = p.f4 = 0;
-    if (i < cnt && (size_t)start + cnt <= a.binrec_cap) {
-        const uint4 *ir = a.binrec + 2ull * (start + i);
+    if (i < cnt && i < a.lmax) {
+        const uint4 *ir = a.binrec + 2ull * ((size_t)slot * a.lmax + i);
         p.q0 = ir[0];
         p.q1 = ir[1];
     }
@@ -910,7 +890,8 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
             bin_pre_load(a, e_cur, pre);
             e_nxt = entry(b + 2 * G);
         };
-        const uint32_t slot = e.x, start = e.y, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
+        const uint32_t slot = e.x, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
+        const size_t start = (size_t)slot * a.lmax;  // the bin's incidence bucket
         if (cnt > (uint32_t)LCAP || nf0 > (uint32_t)FCAP || ng0 > (uint32_t)GCAP) {
             advance();
             escalate<FCAP, GCAP, LCAP, DIR>(a, tier, e, ERR_LIST);
@@ -933,7 +914,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
         while (n2 < cnt) n2 <<= 1;
         {
             const uint32_t *fr = a.frag + (size_t)slot * a.fcap * 5;  // AoS, 5 words per fragment
-            const uint4 *ir = a.binrec + 2ull * start;
+            const uint4 *ir = a.binrec + 2 * start;
             const uint32_t nmax = max(n2, nf0);
             for (uint32_t i = lane; i < nmax; i += 64) {
                 uint4 q0 = cur.q0, q1 = cur.q1;  // the first chunk was loaded ahead
@@ -1005,7 +986,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
                     }
                     __syncthreads();
                 }
-            const uint4 *ir = a.binrec + 2ull * start;
+            const uint4 *ir = a.binrec + 2 * start;
             for (uint32_t i = lane; i < cnt; i += 64) {
                 const uint32_t pos = (uint32_t)s.L[i];
                 const uint4 q0 = ir[2 * pos], q1 = ir[2 * pos + 1];
@@ -1141,7 +1122,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
 
 __global__ void k_dedup(KArgs a)
 {
-    if (a.ctr->err & (ERR_CAND | ERR_BINREC)) return;  // candidates are incomplete
+    if (a.ctr->err & ERR_CAND) return;  // candidates are incomplete
     // candidates: stripe q holds cstr[q] of them from index q * cand_cap
     uint32_t n = 0;
     for (uint32_t q = 0; q < CSTRIPES; ++q) n += min(a.cstr[16 * q], a.cand_cap);
@@ -1367,7 +1348,7 @@ struct xkrt_hip_pool {
     uint32_t cap_bins = 0;
     std::vector<std::pair<uint32_t, uint32_t>> free_ranges;
     uint32_t *bin_nf = nullptr, *bin_nrs = nullptr, *frag = nullptr, *rs = nullptr;
-    uint32_t *bin_cnt = nullptr, *bin_cur = nullptr, *bin_dom = nullptr, *touched = nullptr;
+    uint32_t *bin_cnt = nullptr, *bin_dom = nullptr, *touched = nullptr;
     uint4 *active = nullptr;
     // domains
     DomDesc h_dom[MAX_DOMS] = {};
@@ -1376,8 +1357,8 @@ struct xkrt_hip_pool {
     DomDesc *d_dom = nullptr;
     bool dom_dirty = true;
     // launch scratch
-    uint4 *binrec = nullptr;
-    uint32_t binrec_cap = 0;
+    uint4 *binrec = nullptr;  // per bin, lmax incidences (32 B each) of the current launch
+    uint32_t lmax = 0;
     uint32_t *cand = nullptr;
     uint32_t cand_cap = 0;  // per stripe; 0 in the direct form
     uint32_t *cstr = nullptr;
@@ -1459,7 +1440,10 @@ struct xkrt_hip_dep_domain {
 
 namespace {
 
-uint64_t bin_bytes(const xkrt_hip_pool *p) { return 4ull * 6 + 64ull + 20ull * p->fcap + 20ull * p->rcap; }
+uint64_t bin_bytes(const xkrt_hip_pool *p)
+{
+    return 4ull * 5 + 64ull + 20ull * p->fcap + 20ull * p->rcap + 32ull * p->lmax;
+}
 
 hipEvent_t take_event(xkrt_hip_pool *p)
 {
@@ -1517,15 +1501,15 @@ int grow_bins(xkrt_hip_pool *p, uint32_t need)
         if (nc * bin_bytes(p) + p->bytes_scratch > p->cfg.budget_bytes) return -ENOMEM;
     }
     HIPCHK(hipStreamSynchronize(p->stream));
-    uint32_t *nf, *nrs, *fr, *rs, *cnt, *cur, *dom, *tch;
-    uint4 *act;
+    uint32_t *nf, *nrs, *fr, *rs, *cnt, *dom, *tch;
+    uint4 *act, *brec;
     HIPCHK(hipMalloc(&nf, nc * 4));
     HIPCHK(hipMalloc(&nrs, nc * 4));
     HIPCHK(hipMalloc(&fr, nc * p->fcap * 20ull));
     HIPCHK(hipMalloc(&rs, nc * p->rcap * 20ull));
     HIPCHK(hipMalloc(&cnt, nc * 4));
-    HIPCHK(hipMalloc(&cur, nc * 4));
     HIPCHK(hipMalloc(&dom, nc * 4));
+    HIPCHK(hipMalloc(&brec, nc * p->lmax * 32ull));  // launch scratch: nothing to copy
     HIPCHK(hipMalloc(&tch, nc * 4));
     HIPCHK(hipMalloc(&act, nc * 16 * 4));
     HIPCHK(hipMemsetAsync(nf, 0, nc * 4, p->stream));
@@ -1542,11 +1526,11 @@ int grow_bins(xkrt_hip_pool *p, uint32_t need)
         HIPCHK(hipMemcpyAsync(dom, p->bin_dom, oc * 4, hipMemcpyDeviceToDevice, p->stream));
         HIPCHK(hipStreamSynchronize(p->stream));
         (void)hipFree(p->bin_nf); (void)hipFree(p->bin_nrs); (void)hipFree(p->frag); (void)hipFree(p->rs);
-        (void)hipFree(p->bin_cnt); (void)hipFree(p->bin_cur); (void)hipFree(p->bin_dom); (void)hipFree(p->active);
-        (void)hipFree(p->touched);
+        (void)hipFree(p->bin_cnt); (void)hipFree(p->bin_dom); (void)hipFree(p->active);
+        (void)hipFree(p->touched); (void)hipFree(p->binrec);
     }
     p->bin_nf = nf; p->bin_nrs = nrs; p->frag = fr; p->rs = rs;
-    p->bin_cnt = cnt; p->bin_cur = cur; p->bin_dom = dom; p->active = act; p->touched = tch;
+    p->bin_cnt = cnt; p->bin_dom = dom; p->active = act; p->touched = tch; p->binrec = brec;
     // the new tail is free
     if (!p->free_ranges.empty() && p->free_ranges.back().first + p->free_ranges.back().second == oc)
         p->free_ranges.back().second += (uint32_t)(nc - oc);
@@ -1657,9 +1641,9 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.bin_dom = p->bin_dom;
     a.bin_nf = p->bin_nf; a.bin_nrs = p->bin_nrs; a.frag = p->frag; a.rs = p->rs;
     a.fcap = p->fcap; a.rcap = p->rcap;
-    a.bin_cnt = p->bin_cnt; a.bin_cur = p->bin_cur; a.active = p->active; a.binrec = p->binrec;
+    a.bin_cnt = p->bin_cnt; a.active = p->active; a.binrec = p->binrec;
     a.touched = p->touched;
-    a.binrec_cap = p->binrec_cap;
+    a.lmax = p->lmax;
     a.nbins = p->cap_bins;
     a.max_tier = p->kclass + 1;  // tier t + 1 is class t
     a.min_tier = 0;
@@ -1709,7 +1693,7 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
         launch(p, spans, 3, k_resolve<F, R, L, false>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
 }
 
-// the kernels of one launch (kernel slots: count, alloc, fill, resolve, dedup, final)
+// the kernels of one launch (timing slots: count, alloc, -, resolve, dedup, final)
 void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
                     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
@@ -1718,7 +1702,6 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
     launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), 0, a);
     launch(p, spans, 1, k_alloc, dim3(std::min<uint32_t>((p->cap_bins + 255) / 256, 1024u)), dim3(256), 0, a,
            p->cap_bins);
-    launch(p, spans, 2, k_fill, dim3(rblocks), dim3(256), 0, a);
     // tier 0 first; bins that outgrow a tier are retried by the next one
     if (!small) launch_resolve<64, 128, 64>(p, a, 0, spans);
     launch_resolve<64, 128, 256>(p, a, 1, spans);
@@ -1875,7 +1858,7 @@ void destroy_pool(xkrt_hip_pool *p)
             (void)hipEventDestroy(b);
         }
     for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
-    void *dev[] = {p->bin_nf, p->bin_nrs, p->frag, p->rs, p->bin_cnt, p->bin_cur, p->bin_dom, p->active, p->touched,
+    void *dev[] = {p->bin_nf, p->bin_nrs, p->frag, p->rs, p->bin_cnt, p->bin_dom, p->active, p->touched,
                    p->d_dom,
                    p->binrec, p->cand, p->cstr, p->blk, p->hkeys, p->hvals, p->eslot, p->edges, p->edom, p->ctr,
                    p->gate_count,
@@ -1924,19 +1907,18 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     const uint64_t max_rects = p->cfg.max_rects ? p->cfg.max_rects : (1ull << 20);
     p->cfg.max_rects = max_rects;
     p->rec_cap = max_rects;
-    p->binrec_cap = (uint32_t)std::min<uint64_t>(0xFFFFFFF0ull, std::max<uint64_t>(4 * max_rects, 1 << 16));
+    p->lmax = TIERS[p->kclass + 1].lcap;  // records per bin and launch of the highest tier
     const uint64_t cand_total = std::min<uint64_t>(0x3FFFFFF0ull, std::max<uint64_t>(32 * max_rects, 1 << 20));
     uint64_t hsz = 1;
     while (hsz < 2ull * cand_total) hsz <<= 1;
     p->hmask = (uint32_t)(hsz - 1);
     p->edge_cap = p->cfg.max_edges ? p->cfg.max_edges : std::max<uint64_t>(16 * max_rects, 1 << 20);
     p->cand_cap = p->cfg.direct_dedup ? 0u : (uint32_t)(cand_total / CSTRIPES);
-    p->bytes_scratch = 32ull * p->binrec_cap + 12ull * CSTRIPES * p->cand_cap + 12ull * hsz + 4ull * hsz +
+    p->bytes_scratch = 12ull * CSTRIPES * p->cand_cap + 12ull * hsz + 4ull * hsz +
                        12ull * p->edge_cap + 32ull * (p->rec_cap + 1) + sizeof(DevCounters) + sizeof(p->h_dom);
     int rc = 0;
     auto chk = [&](hipError_t e) { if (!rc) rc = hip_err(e); };
     if (p->cfg.budget_bytes && p->bytes_scratch > p->cfg.budget_bytes) rc = -ENOMEM;
-    chk(hipMalloc(&p->binrec, 32ull * p->binrec_cap));
     if (p->cand_cap) chk(hipMalloc(&p->cand, 12ull * CSTRIPES * p->cand_cap));
     chk(hipMalloc(&p->cstr, 64ull * CSTRIPES));
     chk(hipMalloc(&p->blk, 16ull * 4 * BLKSLOTS));
@@ -2209,7 +2191,7 @@ int gate_arm(xkrt_hip_pool *p)
     KArgs a = make_args(p, p->rec_host_dev[hb] + 2, NONE);
     a.nrec_ptr = p->gate_count;
     a.min_tier = 1;
-    // grids for any count: k_count and k_fill stride over the records
+    // grids for any count: k_count strides over the records
     launch_kernels(p, a, std::max<uint32_t>(1u, std::min<uint32_t>((p->cfg.small_launch_rects + 255) / 256, 256u)),
                    nullptr);
     rc = ticket(p);
