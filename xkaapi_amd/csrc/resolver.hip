@@ -864,8 +864,12 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     BinLds<FCAP, GCAP, LCAP, DIR> &s = *reinterpret_cast<BinLds<FCAP, GCAP, LCAP, DIR> *>(lds_raw);
     const uint32_t lane = lane_id();
     if (a.ctr->err) return;  // a poisoned domain touches no state
-    const uint32_t nheavy = a.ctr->ntier[tier];
-    const uint32_t nlist = nheavy + a.ctr->nlight[tier];
+    // a small launch's first tier takes every touched bin straight from the touched
+    // list (no k_alloc); other tiers take their active list (bins k_alloc chose, or
+    // escalated from the tier below)
+    const bool touched = a.min_tier > 0 && tier == a.min_tier;
+    const uint32_t nheavy = touched ? min(a.ctr->ntouched, a.nbins) : a.ctr->ntier[tier];
+    const uint32_t nlist = nheavy + (touched ? 0u : a.ctr->nlight[tier]);
     const uint4 *list = a.active + (size_t)tier * a.nbins;
     unsigned long long visits = 0;
     uint32_t maxf = 0, maxg = 0;
@@ -877,7 +881,13 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     // never wait behind this bin's stores.
     const uint32_t G = gridDim.x;
     auto entry = [&](uint32_t x) {
-        return x < nlist ? (x < nheavy ? list[x] : list[a.nbins - 1 - (x - nheavy)]) : make_uint4(0, 0, 0, 0);
+        if (x >= nlist) return make_uint4(0, 0, 0, 0);
+        if (touched) {
+            const uint32_t slot = a.touched[x];
+            return make_uint4(slot, 0u, a.bin_cnt[slot],
+                              a.bin_nf[slot] | (a.bin_nrs[slot] << 11) | (a.bin_dom[slot] << 24));
+        }
+        return x < nheavy ? list[x] : list[a.nbins - 1 - (x - nheavy)];
     };
     uint4 e_cur = entry(blockIdx.x), e_nxt = entry(blockIdx.x + G);
     BinPre pre;
@@ -1697,11 +1707,13 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
 void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
                     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
-    // a small launch starts at tier 1: one resolve kernel less, occupancy does not matter there
+    // a small launch starts at tier 1 (one resolve kernel less: occupancy does not matter
+    // there), whose kernel takes the touched bins itself (no k_alloc)
     const bool small = a.min_tier > 0;
     launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), 0, a);
-    launch(p, spans, 1, k_alloc, dim3(std::min<uint32_t>((p->cap_bins + 255) / 256, 1024u)), dim3(256), 0, a,
-           p->cap_bins);
+    if (!small)
+        launch(p, spans, 1, k_alloc, dim3(std::min<uint32_t>((p->cap_bins + 255) / 256, 1024u)), dim3(256), 0, a,
+               p->cap_bins);
     // tier 0 first; bins that outgrow a tier are retried by the next one
     if (!small) launch_resolve<64, 128, 64>(p, a, 0, spans);
     launch_resolve<64, 128, 256>(p, a, 1, spans);
