@@ -192,6 +192,8 @@ __device__ __forceinline__ DomDesc rec_bins(const KArgs &a, uint32_t r, uint4 &q
 // incl_j; incidence k belongs to the lane j with incl_{j-1} <= k < incl_j and
 // is bin (bx0_j + off % w_j, by0_j + off / w_j), off = k - incl_{j-1}.  So a
 // record covering many bins does not stall its wave.
+constexpr uint32_t KRPW = 16;  // records per wave in k_count
+
 struct WaveRecs {
     uint32_t r, valid, bx0, w, by0, n, incl, bin0, nbx;
 };
@@ -248,8 +250,11 @@ __global__ void k_count(KArgs a)
     if (tid == 0) a.ctr->nrec = nrec;  // for the later kernels
     if (a.ctr->err) return;  // a poisoned pool touches no state
     const uint32_t lane = lane_id();
-    for (uint32_t r0 = tid & ~63u; r0 < nrec; r0 += gridDim.x * blockDim.x) {
-        const WaveRecs q = wave_recs(a, r0 + lane, nrec);
+    // KRPW records per wave: their incidences (several per record in general) are
+    // spread over the wave's 64 lanes, so more waves share a launch's records
+    const uint32_t wave = tid >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t r0 = wave * KRPW; r0 < nrec; r0 += nwaves * KRPW) {
+        const WaveRecs q = wave_recs(a, r0 + lane, lane < KRPW ? min(nrec, r0 + KRPW) : 0u);
         const uint32_t total = __shfl(q.incl, 63, 64);
         for (uint32_t k0 = 0; k0 < total; k0 += 64) {
             const uint32_t k = k0 + lane;
@@ -1704,6 +1709,12 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
 }
 
 // the kernels of one launch (timing slots: count, alloc, -, resolve, dedup, final)
+// k_count blocks (4 waves of KRPW records each) for n records; it strides over more
+uint32_t count_blocks(uint64_t n)
+{
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 4 * KRPW - 1) / (4 * KRPW), 8192));
+}
+
 void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
                     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
@@ -1753,7 +1764,7 @@ int enqueue_small(xkrt_hip_pool *p, const uint4 *hdr, uint32_t nrec)
     KArgs a = make_args(p, hdr + 2, NONE);
     a.nrec_ptr = (const uint32_t *)hdr;
     a.min_tier = 1;
-    const uint32_t rb = std::min<uint32_t>((nrec + 255) / 256, 8192u);
+    const uint32_t rb = count_blocks(nrec);
     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
     launch_kernels(p, a, rb, p->timing ? &spans : nullptr);
     if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
@@ -1769,7 +1780,7 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
         if (!p->graph_valid) {
             drop_graph(p);
             const KArgs a = make_args(p, p->rec_dev, NONE);
-            const uint32_t rb = (uint32_t)std::min<uint64_t>((p->rec_cap + 255) / 256, 8192u);
+            const uint32_t rb = count_blocks(p->rec_cap);
             HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
             launch_kernels(p, a, rb, nullptr);
             hipGraph_t g = nullptr;
@@ -1794,7 +1805,7 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
         }
     } else {
         const KArgs a = make_args(p, rec, counted ? NONE : nrec);
-        const uint32_t rb = std::min<uint32_t>((nrec + 255) / 256, 8192u);
+        const uint32_t rb = count_blocks(nrec);
         std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
         launch_kernels(p, a, rb, p->timing ? &spans : nullptr);
         if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
@@ -2204,8 +2215,7 @@ int gate_arm(xkrt_hip_pool *p)
     a.nrec_ptr = p->gate_count;
     a.min_tier = 1;
     // grids for any count: k_count strides over the records
-    launch_kernels(p, a, std::max<uint32_t>(1u, std::min<uint32_t>((p->cfg.small_launch_rects + 255) / 256, 256u)),
-                   nullptr);
+    launch_kernels(p, a, count_blocks(p->cfg.small_launch_rects), nullptr);
     rc = ticket(p);
     if (rc) return rc;
     p->gate_armed = true;
