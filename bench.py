@@ -401,15 +401,19 @@ def main():
         b_alg = (24 * n_a + 64 * n_a + 12 * e) / per_pass  # one pass resolves every access and edge once
         t_res = kt["k_resolve"] / nl * 1e-3  # nl timed launches (all passes)
         achieved = b_alg / t_res / 1e9
-        traffic = None
+        traffic = prof_us = None
         tp = os.path.join(ROOT, "profiles", "resolver_cholesky8192_plain_traffic.json")
         if os.path.exists(tp):  # HBM bytes per launch from the committed rocprofv3 counter runs
             t = json.load(open(tp))
             traffic = int(1024 * (t["fetch_kib_per_launch"] * t["fetch_factor"]
                                   + t["write_kib_per_launch"] * t["write_factor"]))
+            # the same kernels' duration in that profile's kernel trace; the start / stop events
+            # above add the dispatch of each kernel (~2.3 us per kernel on this part)
+            prof_us = t.get("rocprof_avg_us_per_launch")
         roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-                "kernel": "k_resolve", "avg_us": round(t_res * 1e6, 2), "bytes_per_launch": int(b_alg),
+                "kernel": "k_resolve", "avg_us": round(t_res * 1e6, 2), "rocprof_avg_us": prof_us,
+                "bytes_per_launch": int(b_alg),
                 "workload": "cholesky 8192^2 nb128, 4096-task launches (plain form)"}
 
     cpu = None if args.no_cpu else cpu_baseline(chol)
