@@ -1135,61 +1135,114 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
 
 // ---------------------------------------------------------------- dedup
 
-__global__ void k_dedup(KArgs a)
+// k_dedup's LDS: where each candidate stripe starts in the launch's candidate
+// numbering, and the block's new-edge counts per wave for one edge-counter update
+constexpr uint32_t DEDUP_PER_THREAD = 4;  // candidates per thread and pass
+struct DedupLds {
+    uint32_t pre[CSTRIPES + 1];
+    uint32_t wave[4];
+    unsigned long long base;
+};
+
+// Inserts the staged candidates into the dedup table.  New keys take edge
+// positions from the launch's edge counter, one update per block and pass of
+// 256 x DEDUP_PER_THREAD candidates: that counter is a single address, and one
+// update per wave serialised the kernel on it.
+__global__ void __launch_bounds__(256) k_dedup(KArgs a)
 {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    DedupLds &s = *reinterpret_cast<DedupLds *>(lds_raw);
     if (a.ctr->err & ERR_CAND) return;  // candidates are incomplete
     // candidates: stripe q holds cstr[q] of them from index q * cand_cap
-    uint32_t n = 0;
-    for (uint32_t q = 0; q < CSTRIPES; ++q) n += min(a.cstr[16 * q], a.cand_cap);
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t q = 0; q < CSTRIPES; ++q) {
+            s.pre[q] = t;
+            t += min(a.cstr[16 * q], a.cand_cap);
+        }
+        s.pre[CSTRIPES] = t;
+    }
+    __syncthreads();
+    const uint32_t n = s.pre[CSTRIPES];
     const uint32_t mask = a.hmask;  // the whole table: spread keys keep concurrent atomics on distinct lines
     const unsigned long long ebase = a.ctr->edge_base;
-    const uint32_t lane = lane_id();
-    // wave-uniform trip count: the edge counter update below is one per wave
-    for (uint32_t i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += gridDim.x * blockDim.x) {
-        const uint32_t i = i0 + lane;
-        bool isnew = false;
-        uint32_t h = 0;
-        if (i < n) {
-            uint32_t q = 0, j = i;
-            while (j >= min(a.cstr[16 * q], a.cand_cap)) {
-                j -= min(a.cstr[16 * q], a.cand_cap);
-                ++q;
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    // several candidates per thread only when the grid has more than one each: a
+    // thread's inserts run one after the other
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    const uint32_t dpt = min(DEDUP_PER_THREAD, max(1u, (uint32_t)(((uint64_t)n + nthreads - 1) / nthreads)));
+    const uint32_t per_pass = blockDim.x * dpt;
+    // block-uniform trip count: the barriers below are reached by every thread
+    for (uint32_t i0 = blockIdx.x * per_pass; i0 < n; i0 += gridDim.x * per_pass) {
+        uint32_t h[DEDUP_PER_THREAD], acc[DEDUP_PER_THREAD];
+        unsigned long long key[DEDUP_PER_THREAD], old[DEDUP_PER_THREAD];
+        bool live[DEDUP_PER_THREAD];
+        // the first probe of each of the thread's candidates is issued before any
+        // result is needed, so their atomics are in flight together
+#pragma unroll
+        for (uint32_t k = 0; k < DEDUP_PER_THREAD; ++k) {
+            const uint32_t i = i0 + k * blockDim.x + threadIdx.x;
+            live[k] = k < dpt && i < n;
+            h[k] = 0;
+            acc[k] = 0;
+            key[k] = EMPTY64;
+            old[k] = EMPTY64;
+            if (live[k]) {
+                uint32_t q = 0;
+                while (i >= s.pre[q + 1]) ++q;
+                const uint32_t *c = a.cand + 3ull * ((size_t)q * a.cand_cap + (i - s.pre[q]));
+                key[k] = ((unsigned long long)c[0] << 32) | c[2];
+                acc[k] = c[1];
+                h[k] = (uint32_t)mix64(key[k]) & mask;
+                old[k] = atomicCAS(&a.hkeys[h[k]], EMPTY64, key[k]);
             }
-            const uint32_t *c = a.cand + 3ull * ((size_t)q * a.cand_cap + j);
-            const uint32_t p = c[0], acc = c[1], t = c[2];
-            const unsigned long long key = ((unsigned long long)p << 32) | t;
-            h = (uint32_t)mix64(key) & mask;
-            uint32_t probe = 0;
-            while (true) {
-                const unsigned long long old = atomicCAS(&a.hkeys[h], EMPTY64, key);
-                if (old == EMPTY64 || old == key) {
-                    atomicMin(&a.hvals[h], acc);
-                    isnew = old == EMPTY64;
-                    break;
+        }
+        uint64_t nm[DEDUP_PER_THREAD];
+        uint32_t wc = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < DEDUP_PER_THREAD; ++k) {
+            bool isnew = false;
+            if (live[k]) {
+                // linear probing from the first slot on
+                uint32_t hh = h[k], probe = 0;
+                unsigned long long o = old[k];
+                while (o != EMPTY64 && o != key[k]) {
+                    hh = (hh + 1) & mask;
+                    if (++probe > mask) {
+                        atomicOr(&a.ctr->err, ERR_HASH);
+                        break;
+                    }
+                    o = atomicCAS(&a.hkeys[hh], EMPTY64, key[k]);
                 }
-                h = (h + 1) & mask;
-                if (++probe > mask) {
-                    atomicOr(&a.ctr->err, ERR_HASH);
-                    break;
+                if (o == EMPTY64 || o == key[k]) {
+                    atomicMin(&a.hvals[hh], acc[k]);
+                    isnew = o == EMPTY64;
                 }
+                h[k] = hh;
             }
+            nm[k] = __ballot(isnew);
+            wc += (uint32_t)__popcll(nm[k]);
         }
-        const uint64_t nm = __ballot(isnew);
-        if (nm == 0) continue;
-        uint32_t blo = 0, bhi = 0;
-        if (lane == 0) {
-            const unsigned long long b0 = atomicAdd(&a.ctr->edge_total, (unsigned long long)__popcll(nm));
-            blo = (uint32_t)b0;
-            bhi = (uint32_t)(b0 >> 32);
+        if (lane == 0) s.wave[w] = wc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t total = 0;
+            for (uint32_t u = 0; u < (blockDim.x >> 6); ++u) total += s.wave[u];
+            s.base = total ? atomicAdd(&a.ctr->edge_total, (unsigned long long)total) : 0ull;
         }
-        blo = __shfl(blo, 0, 64);
-        bhi = __shfl(bhi, 0, 64);
-        if (isnew) {
-            const unsigned long long pos =
-                (((unsigned long long)bhi << 32) | blo) + (unsigned long long)__popcll(nm & ((1ull << lane) - 1ull));
-            if (pos >= a.edge_cap || pos - ebase > a.hmask) atomicOr(&a.ctr->err, ERR_EDGES);
-            else a.eslot[pos - ebase] = h;
+        __syncthreads();
+        unsigned long long pos = s.base;
+        for (uint32_t u = 0; u < w; ++u) pos += s.wave[u];
+#pragma unroll
+        for (uint32_t k = 0; k < DEDUP_PER_THREAD; ++k) {
+            if ((nm[k] >> lane) & 1ull) {
+                const unsigned long long e = pos + (unsigned long long)__popcll(nm[k] & ((1ull << lane) - 1ull));
+                if (e >= a.edge_cap || e - ebase > a.hmask) atomicOr(&a.ctr->err, ERR_EDGES);
+                else a.eslot[e - ebase] = h[k];
+            }
+            pos += (unsigned long long)__popcll(nm[k]);
         }
+        __syncthreads();  // s.wave and s.base are rewritten by the next pass
     }
 }
 
@@ -1730,7 +1783,7 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
     launch_resolve<64, 128, 256>(p, a, 1, spans);
     if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, a, 2, spans);
     if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, a, 3, spans);
-    if (!p->cfg.direct_dedup) launch(p, spans, 4, k_dedup, dim3(1024), dim3(256), 0, a);
+    if (!p->cfg.direct_dedup) launch(p, spans, 4, k_dedup, dim3(1024), dim3(256), (uint32_t)sizeof(DedupLds), a);
     launch(p, spans, 5, k_final, dim3(1024), dim3(256), 0, a);
 }
 
