@@ -459,12 +459,14 @@ __device__ void flush_cands(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s)
     __syncthreads();
 }
 
-// room for one candidate per lane in the staging buffer
+// room for m more candidates in the staging buffer.  It is flushed only when
+// full: every flush takes a position from a stripe counter that many blocks
+// share, and a flush per bin put that round trip on every bin's path.
 template <int FCAP, int GCAP, int LCAP, bool DIR>
-__device__ __forceinline__ void room(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s)
+__device__ __forceinline__ void room(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s, uint32_t m)
 {
     __syncthreads();
-    if (s.ccount + 64 > BinLds<FCAP, GCAP, LCAP, DIR>::CCAP) flush_cands(a, s);
+    if (s.ccount + m > BinLds<FCAP, GCAP, LCAP, DIR>::CCAP) flush_cands(a, s);
 }
 
 // Record predecessor p of access acc (task `task`): the run's dedup set keeps
@@ -512,7 +514,7 @@ __device__ void drain_dedup(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s)
     __syncthreads();
     const uint32_t n = s.dcount;
     for (uint32_t c = 0; c < n; c += 64) {
-        room(a, s);
+        room(a, s, min(64u, n - c));
         const uint32_t u = c + lane_id();
         if (u < n) {
             const uint32_t h = s.Du[u];
