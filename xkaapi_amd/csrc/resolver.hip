@@ -233,8 +233,28 @@ __device__ __forceinline__ uint32_t wave_rec_bin(const WaveRecs &q, uint32_t k, 
     return bin0 + (by0 + off / max(w, 1u)) * nbx + bx0 + off % max(w, 1u);
 }
 
-__global__ void k_count(KArgs a)
+// k_count's LDS: each wave's first-touched bins, listed with one update of the
+// touched-list counter per block (or per TBUF bins of a wave), then the four
+// waves' counts and the block's list position.  One update per 64 incidences
+// kept the waves of a large launch queued on that one address.
+constexpr uint32_t TBUF = 256;
+constexpr uint32_t COUNT_LDS = 4 * TBUF * 4 + 8 * 4;  // 256 threads = 4 waves
+
+// this wave's buffered first-touched bins to the touched list
+__device__ __forceinline__ void flush_touched(const KArgs &a, uint32_t *tb, uint32_t n)
 {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(&a.ctr->ntouched, n);
+    base = wave_bcast(base, 0);
+    for (uint32_t i = lane_id(); i < n; i += 64) a.touched[base + i] = tb[i];
+    __builtin_amdgcn_wave_barrier();  // the buffer is read before any lane refills it
+}
+
+__global__ void __launch_bounds__(256) k_count(KArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    uint32_t *tb = reinterpret_cast<uint32_t *>(lds_raw) + (threadIdx.x >> 6) * TBUF;
+    uint32_t ntb = 0;  // wave-uniform: entries of tb in use
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
         a.ctr->total = 0;
@@ -277,13 +297,30 @@ __global__ void k_count(KArgs a)
             // the bins this launch touches, listed once each
             const uint64_t m = __ballot(first);
             if (m) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&a.ctr->ntouched, (uint32_t)__popcll(m));
-                base = wave_bcast(base, 0);
-                if (first) a.touched[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = bin;
+                const uint32_t nm = (uint32_t)__popcll(m);
+                if (ntb + nm > TBUF) {
+                    flush_touched(a, tb, ntb);
+                    ntb = 0;
+                }
+                if (first) tb[ntb + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = bin;
+                ntb += nm;
+                __builtin_amdgcn_wave_barrier();
             }
         }
     }
+    // the block's buffered bins: one counter update, each wave copies its own
+    uint32_t *bc = reinterpret_cast<uint32_t *>(lds_raw) + 4 * TBUF;
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane == 0) bc[w] = ntb;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = bc[0] + bc[1] + bc[2] + bc[3];
+        bc[4] = tot ? atomicAdd(&a.ctr->ntouched, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = bc[4];
+    for (uint32_t u = 0; u < w; ++u) base += bc[u];
+    for (uint32_t i = lane; i < ntb; i += 64) a.touched[base + i] = tb[i];
 }
 
 // the touched bins' tier lists: each active bin goes to the smallest LDS tier
@@ -1271,14 +1308,34 @@ __global__ void k_final(KArgs a)
         for (uint32_t q = 0; q < CSTRIPES; ++q) nc += a.cstr[16 * q];
         a.ctr->cand_total += nc;
     }
-    for (unsigned long long e = tid; e < n; e += (unsigned long long)gridDim.x * blockDim.x) {
-        const uint32_t h = a.eslot[e];
-        const unsigned long long key = a.hkeys[h];
-        a.edges[2 * (ebase + e)] = (uint32_t)(key >> 32);
-        a.edges[2 * (ebase + e) + 1] = a.hvals[h];
-        a.edom[ebase + e] = (uint32_t)(key >> 24) & 0xffu;
-        a.hkeys[h] = EMPTY64;
-        a.hvals[h] = NONE;
+    // four edges per thread and pass, their table reads issued before any store (each
+    // edge is a chain: position -> table slot -> key and successor)
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long e0 = tid; e0 < n; e0 += 4 * stride) {
+        uint32_t h[4], v[4];
+        unsigned long long key[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const unsigned long long e = e0 + k * stride;
+            h[k] = e < n ? a.eslot[e] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const unsigned long long e = e0 + k * stride;
+            key[k] = e < n ? a.hkeys[h[k]] : 0ull;
+            v[k] = e < n ? a.hvals[h[k]] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const unsigned long long e = e0 + k * stride;
+            if (e < n) {
+                a.edges[2 * (ebase + e)] = (uint32_t)(key[k] >> 32);
+                a.edges[2 * (ebase + e) + 1] = v[k];
+                a.edom[ebase + e] = (uint32_t)(key[k] >> 24) & 0xffu;
+                a.hkeys[h[k]] = EMPTY64;
+                a.hvals[h[k]] = NONE;
+            }
+        }
     }
 }
 
@@ -1776,7 +1833,7 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
     // a small launch starts at tier 1 (one resolve kernel less: occupancy does not matter
     // there), whose kernel takes the touched bins itself (no k_alloc)
     const bool small = a.min_tier > 0;
-    launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), 0, a);
+    launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), COUNT_LDS, a);
     if (!small)
         launch(p, spans, 1, k_alloc, dim3(std::min<uint32_t>((p->cap_bins + 255) / 256, 1024u)), dim3(256), 0, a,
                p->cap_bins);
