@@ -379,7 +379,8 @@ def main():
         res[form] = dict(ms=dt * 1e3, rate=nacc / dt, kt=kt, lt=lt, edges=edges)
         log(f"{form}: {dt*1e3:.2f} ms/step, {nacc/dt:.4g} acc/s, edges {edges}")
         for name, (k, n) in kt.items():
-            log(f"   {name}: launches {n} " + " ".join(f"{kk}={vv/max(n,1)*1e3:.1f}us" for kk, vv in k.items()))
+            log(f"   {name}: launches {n} " + " ".join(f"{kk}={vv/max(n,1)*1e3:.1f}us" for kk, vv in k.items()
+                                                    if kk.startswith("k_")))
 
     if "plain" not in res:
         res["plain"] = res["batched"]

@@ -174,12 +174,13 @@ void *xkrt_hip_dep_stream(xkrt_hip_dep_domain_t d);
  * domain in runtime_t::reset (src/memory/invalidate.cc:68-131). */
 int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d);
 
-/* Per-kernel HIP-event timing of every later launch (count, alloc, fill,
- * resolve, dedup, final). */
+/* Per-kernel HIP-event timing of every later launch (count, resolve over all
+ * tiers, dedup, final). */
 int xkrt_hip_dep_timing(xkrt_hip_dep_domain_t d, int enable);
 
-/* Accumulated milliseconds per kernel (6 doubles) over *launches timed
- * launches; synchronises; clear != 0 restarts the accumulation. */
+/* Accumulated milliseconds per kernel over *launches timed launches, as 6
+ * doubles: count, 0, 0, resolve, dedup, final (slots 1 and 2 are kept for the
+ * layout and read 0); synchronises; clear != 0 restarts the accumulation. */
 int xkrt_hip_dep_kernel_times(xkrt_hip_dep_domain_t d, double *ms, uint64_t *launches, int clear);
 
 /* Device-to-device streaming copy of `bytes` (a multiple of 16) on `stream`:

@@ -14,11 +14,12 @@
 //     (x0, x1, y0, y1, acc, task<<8|mode: 24 B);
 //   * incidences (record x bin) are 32 B: {x0, x1, y0, y1}, {acc, tm, record, 0}.
 //
-// One launch resolves one batch of records in six kernels:
-//   k_count   records -> per-bin record counts
-//   k_alloc   per-bin segments in the incidence array, per-tier active-bin lists
-//             (k_count also writes each incidence into its bin's bucket)
-//   k_resolve one wave64 per active bin: the bin's fragments, reader log and
+// One launch resolves one batch of records in these kernels:
+//   k_count   records -> per-bin record counts; each incidence goes into its
+//             bin's bucket, each bin the launch touches into the touched list
+//   k_resolve one kernel per LDS tier, one wave64 per bin: the first tier takes
+//             the touched list, a bin that does not fit a tier (with headroom)
+//             goes on to the next one's list.  The bin's fragments, reader log and
 //             incidences move to LDS, its records are put in spawn order
 //             (rank sort up to 64, bitonic on 64-bit keys above) and replayed
 //             -- LINK of every access of a task, then PUT -- emitting
@@ -79,8 +80,7 @@ struct DevCounters {
     unsigned int nrec;       // rect records of this launch (written by k_count)
     unsigned int ntouched;   // bins that got records this launch (listed by k_count; zeroed by k_final)
     unsigned int total;      // incidences allocated this launch
-    unsigned int ntier[4];   // active bins per LDS tier this launch (heavy ones, from the list front)
-    unsigned int nlight[4];  // light active bins per tier (from the list back)
+    unsigned int ntier[4];   // bins on each LDS tier's list this launch (escalated from the tier below)
     unsigned int ncand;      // candidates this launch
     unsigned int err;        // sticky ERR_* bits
     unsigned int maxfrag;    // stats: max fragments in a bin
@@ -130,7 +130,6 @@ struct KArgs {
     uint32_t nbins;
     uint32_t max_tier;                              // tiers 0..max_tier; storage sized for max_tier
     uint32_t min_tier;                              // first tier a bin may take (1 in small launches)
-    uint32_t tier_f[4], tier_g[4], tier_l[4];       // per tier: fragments, log entries, records
     uint32_t *cand;           // staged form: CSTRIPES stripes of cand_cap candidates (3 u32 each)
     uint32_t cand_cap;        // per stripe
     uint32_t *cstr;           // CSTRIPES candidate counters, 16 u32 apart
@@ -260,7 +259,6 @@ __global__ void __launch_bounds__(256) k_count(KArgs a)
         a.ctr->total = 0;
         for (int t = 0; t < 4; ++t) {
             a.ctr->ntier[t] = 0;
-            a.ctr->nlight[t] = 0;
         }
         a.ctr->ncand = 0;
         a.ctr->edge_base = a.ctr->edge_total;
@@ -321,45 +319,6 @@ __global__ void __launch_bounds__(256) k_count(KArgs a)
     uint32_t base = bc[4];
     for (uint32_t u = 0; u < w; ++u) base += bc[u];
     for (uint32_t i = lane; i < ntb; i += 64) a.touched[base + i] = tb[i];
-}
-
-// the touched bins' tier lists: each active bin goes to the smallest LDS tier
-// whose capacities fit it with headroom
-__global__ void k_alloc(KArgs a, uint32_t nbins)
-{
-    if (a.ctr->err) return;
-    const uint32_t nt = min(a.ctr->ntouched, nbins);
-    // grid-stride over 64-entry chunks of the touched list, one wave per chunk
-    for (uint32_t s0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); s0 < nt; s0 += gridDim.x * blockDim.x) {
-        const uint32_t ti = s0 + lane_id();
-        const uint32_t slot = ti < nt ? a.touched[ti] : 0u;
-        const uint32_t c = ti < nt ? a.bin_cnt[slot] : 0u;
-        if (__ballot(c > 0) == 0) continue;
-        uint32_t tier = a.min_tier, nfl = 0, ngl = 0;
-        if (c > 0) {
-            nfl = a.bin_nf[slot];
-            ngl = a.bin_nrs[slot];
-            while (tier < a.max_tier &&
-                   (c > a.tier_l[tier] || 2 * nfl > a.tier_f[tier] || 2 * ngl > a.tier_g[tier]))
-                ++tier;
-        }
-        // heavy bins (long record lists) at the front of the tier list, so they start first
-        for (uint32_t t = 0; t <= a.max_tier; ++t) {
-            for (int heavy = 1; heavy >= 0; --heavy) {
-                const bool mine = c > 0 && tier == t && ((c >= 16) == (heavy != 0));
-                const uint64_t m = __ballot(mine);
-                if (m == 0) continue;
-                uint32_t tb = 0;
-                if (lane_id() == 0)
-                    tb = atomicAdd(heavy ? &a.ctr->ntier[t] : &a.ctr->nlight[t], (uint32_t)__popcll(m));
-                tb = wave_bcast(tb, 0);
-                const uint32_t r = tb + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
-                if (mine)
-                    a.active[(size_t)t * a.nbins + (heavy ? r : a.nbins - 1 - r)] =
-                        make_uint4(slot, 0u, c, nfl | (ngl << 11) | (a.bin_dom[slot] << 24));
-            }
-        }
-    }
 }
 
 // ---------------------------------------------------------------- resolve
@@ -908,21 +867,21 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     BinLds<FCAP, GCAP, LCAP, DIR> &s = *reinterpret_cast<BinLds<FCAP, GCAP, LCAP, DIR> *>(lds_raw);
     const uint32_t lane = lane_id();
     if (a.ctr->err) return;  // a poisoned domain touches no state
-    // a small launch's first tier takes every touched bin straight from the touched
-    // list (no k_alloc); other tiers take their active list (bins k_alloc chose, or
-    // escalated from the tier below)
-    const bool touched = a.min_tier > 0 && tier == a.min_tier;
-    const uint32_t nheavy = touched ? min(a.ctr->ntouched, a.nbins) : a.ctr->ntier[tier];
-    const uint32_t nlist = nheavy + (touched ? 0u : a.ctr->nlight[tier]);
+    // a launch's first tier takes every touched bin straight from the touched list
+    // (no k_alloc); later tiers take their active list (bins escalated from below)
+    const bool touched = tier == a.min_tier;
+    const uint32_t nlist = touched ? min(a.ctr->ntouched, a.nbins) : min(a.ctr->ntier[tier], a.nbins);
     const uint4 *list = a.active + (size_t)tier * a.nbins;
     unsigned long long visits = 0;
     uint32_t maxf = 0, maxg = 0;
     if (lane == 0) s.ccount = 0;
     __syncthreads();
     // Software pipeline over this block's bins b, b + G, b + 2G, ...: the
-    // list entry of the bin after next and the first chunk of the next bin's
-    // state are loaded before this bin's write-back, so the next bin's loads
-    // never wait behind this bin's stores.
+    // first chunk of the next bin's state is loaded before this bin's
+    // write-back.  The list entries of the block's bins are read 64 at a time,
+    // one per lane, and taken from their lane when a bin comes up: loaded per
+    // bin, each entry sat on that bin's path behind the block's earlier loads
+    // and stores.
     const uint32_t G = gridDim.x;
     auto entry = [&](uint32_t x) {
         if (x >= nlist) return make_uint4(0, 0, 0, 0);
@@ -931,22 +890,37 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
             return make_uint4(slot, 0u, a.bin_cnt[slot],
                               a.bin_nf[slot] | (a.bin_nrs[slot] << 11) | (a.bin_dom[slot] << 24));
         }
-        return x < nheavy ? list[x] : list[a.nbins - 1 - (x - nheavy)];
+        return list[x];
     };
-    uint4 e_cur = entry(blockIdx.x), e_nxt = entry(blockIdx.x + G);
+    uint32_t wbase = 0;  // the window holds the entries of bins k in [wbase, wbase + 64)
+    uint4 ew = entry(blockIdx.x + lane * G);
+    // the entry of the block's k-th bin; k grows by one from call to call
+    auto entry_k = [&](uint32_t k) {
+        if (k >= wbase + 64) {
+            wbase += 64;
+            ew = entry(blockIdx.x + (wbase + lane) * G);
+        }
+        const uint32_t l = k - wbase;
+        return make_uint4(__builtin_amdgcn_readlane(ew.x, l), __builtin_amdgcn_readlane(ew.y, l),
+                          __builtin_amdgcn_readlane(ew.z, l), __builtin_amdgcn_readlane(ew.w, l));
+    };
+    uint4 e_cur = entry_k(0), e_nxt = entry_k(1);
     BinPre pre;
     bin_pre_load(a, e_cur, pre);
-    for (uint32_t b = blockIdx.x; b < nlist; b += G) {
+    for (uint32_t b = blockIdx.x, k = 0; b < nlist; b += G, ++k) {
         const uint4 e = e_cur;
         const BinPre cur = pre;
         auto advance = [&]() {
             e_cur = e_nxt;
             bin_pre_load(a, e_cur, pre);
-            e_nxt = entry(b + 2 * G);
+            e_nxt = entry_k(k + 2);
         };
         const uint32_t slot = e.x, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
         const size_t start = (size_t)slot * a.lmax;  // the bin's incidence bucket
-        if (cnt > (uint32_t)LCAP || nf0 > (uint32_t)FCAP || ng0 > (uint32_t)GCAP) {
+        // a bin goes on to the next tier when it does not fit this one, or when its map or
+        // log fill more than half of this tier's room (they grow during the replay)
+        if (cnt > (uint32_t)LCAP || nf0 > (uint32_t)FCAP || ng0 > (uint32_t)GCAP ||
+            (tier < a.max_tier && (2 * nf0 > (uint32_t)FCAP || 2 * ng0 > (uint32_t)GCAP))) {
             advance();
             escalate<FCAP, GCAP, LCAP, DIR>(a, tier, e, ERR_LIST);
             continue;
@@ -1774,11 +1748,6 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.nbins = p->cap_bins;
     a.max_tier = p->kclass + 1;  // tier t + 1 is class t
     a.min_tier = 0;
-    for (int t = 0; t < 4; ++t) {
-        a.tier_f[t] = TIERS[t].fcap;
-        a.tier_g[t] = TIERS[t].rcap;
-        a.tier_l[t] = TIERS[t].lcap;
-    }
     a.cand = p->cand; a.cand_cap = p->cand_cap; a.cstr = p->cstr; a.blk = p->blk;
     a.hkeys = p->hkeys; a.hvals = p->hvals; a.hmask = p->hmask;
     a.eslot = p->eslot; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
@@ -1820,7 +1789,7 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
         launch(p, spans, 3, k_resolve<F, R, L, false>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
 }
 
-// the kernels of one launch (timing slots: count, alloc, -, resolve, dedup, final)
+// the kernels of one launch (timing slots: count, -, -, resolve, dedup, final)
 // k_count blocks (4 waves of KRPW records each) for n records; it strides over more
 uint32_t count_blocks(uint64_t n)
 {
@@ -1831,12 +1800,9 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
                     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
     // a small launch starts at tier 1 (one resolve kernel less: occupancy does not matter
-    // there), whose kernel takes the touched bins itself (no k_alloc)
+    // there); every launch's first tier takes the touched bins itself (no k_alloc)
     const bool small = a.min_tier > 0;
     launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), COUNT_LDS, a);
-    if (!small)
-        launch(p, spans, 1, k_alloc, dim3(std::min<uint32_t>((p->cap_bins + 255) / 256, 1024u)), dim3(256), 0, a,
-               p->cap_bins);
     // tier 0 first; bins that outgrow a tier are retried by the next one
     if (!small) launch_resolve<64, 128, 64>(p, a, 0, spans);
     launch_resolve<64, 128, 256>(p, a, 1, spans);

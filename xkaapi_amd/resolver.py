@@ -271,7 +271,8 @@ class DependencyDomain:
     def set_timing(self, on: bool = True):
         _chk(self.L.xkrt_hip_dep_timing(self.h, 1 if on else 0), "xkrt_hip_dep_timing")
 
-    KERNELS = ("k_count", "k_alloc", "k_fill", "k_resolve", "k_dedup", "k_final")
+    # the library's six timing slots; slots 1 and 2 read 0 (kept for the layout)
+    KERNELS = ("k_count", "-", "--", "k_resolve", "k_dedup", "k_final")
 
     def kernel_times(self, clear: bool = True):
         """({kernel: total ms}, timed launches) from HIP events on the domain's stream."""
