@@ -1139,10 +1139,11 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
         __syncthreads();
     }
     flush_cands(a, s);
-    if (lane == 0 && visits && blockIdx.x < BLKSLOTS) {  // statistics: one slot per block, no shared counter
-        uint4 &o = a.blk[tier * BLKSLOTS + blockIdx.x];
-        const unsigned long long v = ((unsigned long long)o.w << 32 | o.z) + visits;
-        o = make_uint4(max(o.x, maxf), max(o.y, maxg), (uint32_t)v, (uint32_t)(v >> 32));
+    if (lane == 0 && visits) {  // statistics: per-block slots, shared past BLKSLOTS blocks
+        uint4 *o = &a.blk[tier * BLKSLOTS + blockIdx.x % BLKSLOTS];
+        atomicMax(&o->x, maxf);
+        atomicMax(&o->y, maxg);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&o->z), visits);  // (z, w) = 64-bit count
     }
 }
 
@@ -1779,10 +1780,14 @@ template <int F, int R, int L>
 void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
                     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
-    // enough blocks to fill the chip at this tier's LDS footprint; each walks its share of the list
+    // enough blocks to fill the chip at this tier's LDS footprint; each walks its share of the
+    // list.  Tier 0 (large launches only) has four blocks per resident slot: each walks fewer
+    // bins, and a waiting block starts wherever one finished, so the few bins that take several
+    // times the mean no longer pile up on some blocks while the others idle.
     const size_t lds = lds_bytes<F, R, L>();
     const uint32_t per_cu = std::max<uint32_t>(1u, (uint32_t)(160u * 1024u / lds));
-    const uint32_t blocks = std::min<uint32_t>(p->cap_bins, 256u * std::min<uint32_t>(per_cu, 16u));
+    const uint32_t over = tier == 0 ? 4u : 1u;
+    const uint32_t blocks = std::min<uint32_t>(p->cap_bins, 256u * std::min<uint32_t>(per_cu, 16u) * over);
     if (p->cfg.direct_dedup)
         launch(p, spans, 3, k_resolve<F, R, L, true>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
     else
