@@ -90,6 +90,9 @@ struct DevCounters {
     unsigned long long cand_total;   // stats
     unsigned long long frag_visits;  // stats
     unsigned long long nlaunch;      // launches completed (k_final counts them)
+    unsigned int nearly;     // bins listed for the early top-tier pass, from the touched list's back
+                             // (listed by k_count; zeroed by k_final)
+    unsigned int pad_;
 };
 
 // What the host learns of a launch without a stream sync: k_final writes it to
@@ -130,6 +133,9 @@ struct KArgs {
     uint32_t nbins;
     uint32_t max_tier;                              // tiers 0..max_tier; storage sized for max_tier
     uint32_t min_tier;                              // first tier a bin may take (1 in small launches)
+    // k_count: bins whose map or log already exceed twice room_f / room_g (the first two tiers'
+    // room) go on the early list; k_resolve: 1 in the early pass of tier 2 over that list
+    uint32_t split, room_f, room_g, early;
     uint32_t *cand;           // staged form: CSTRIPES stripes of cand_cap candidates (3 u32 each)
     uint32_t cand_cap;        // per stripe
     uint32_t *cstr;           // CSTRIPES candidate counters, 16 u32 apart
@@ -290,6 +296,12 @@ __global__ void __launch_bounds__(256) k_count(KArgs a)
                     uint4 *o = a.binrec + 2ull * ((size_t)bin * a.lmax + pos);
                     o[0] = q0;
                     o[1] = make_uint4(q1.x, q1.y, r, 0u);
+                }
+                // a bin whose state already fits neither of the first two tiers goes on the
+                // early list, resolved by tier 2 beside them
+                if (first && a.split && (2 * a.bin_nf[bin] > a.room_f || 2 * a.bin_nrs[bin] > a.room_g)) {
+                    a.touched[a.nbins - 1 - atomicAdd(&a.ctr->nearly, 1u)] = bin;
+                    first = false;
                 }
             }
             // the bins this launch touches, listed once each
@@ -868,9 +880,13 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     const uint32_t lane = lane_id();
     if (a.ctr->err) return;  // a poisoned domain touches no state
     // a launch's first tier takes every touched bin straight from the touched list
-    // (no k_alloc); later tiers take their active list (bins escalated from below)
-    const bool touched = tier == a.min_tier;
-    const uint32_t nlist = touched ? min(a.ctr->ntouched, a.nbins) : min(a.ctr->ntier[tier], a.nbins);
+    // (no k_alloc); later tiers take their active list (bins escalated from below); the
+    // early pass of tier 2 takes the early list at the touched list's back
+    const bool early = a.early != 0;
+    const bool touched = !early && tier == a.min_tier;
+    const uint32_t nlist = early     ? min(a.ctr->nearly, a.nbins)
+                           : touched ? min(a.ctr->ntouched, a.nbins)
+                                     : min(a.ctr->ntier[tier], a.nbins);
     const uint4 *list = a.active + (size_t)tier * a.nbins;
     unsigned long long visits = 0;
     uint32_t maxf = 0, maxg = 0;
@@ -885,8 +901,8 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     const uint32_t G = gridDim.x;
     auto entry = [&](uint32_t x) {
         if (x >= nlist) return make_uint4(0, 0, 0, 0);
-        if (touched) {
-            const uint32_t slot = a.touched[x];
+        if (touched || early) {
+            const uint32_t slot = early ? a.touched[a.nbins - 1 - x] : a.touched[x];
             return make_uint4(slot, 0u, a.bin_cnt[slot],
                               a.bin_nf[slot] | (a.bin_nrs[slot] << 11) | (a.bin_dom[slot] << 24));
         }
@@ -1265,6 +1281,7 @@ __global__ void k_final(KArgs a)
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
         a.ctr->ntouched = 0;  // the touched list of the next launch starts empty
+        a.ctr->nearly = 0;
         // the launch's outcome to pinned host memory (seen once the launch's completion is)
         const unsigned long long seq = ++a.ctr->nlaunch;
         LaunchSnap *o = a.snap + seq % NSNAP;
@@ -1444,6 +1461,9 @@ struct xkrt_hip_pool {
     xkrt_hip_pool_config_t cfg{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // classes with a tier 2: its early pass runs on `side` between the fork and join events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int device = 0;
     uint32_t kclass = 0, fcap = 0, rcap = 0;
     // bin arrays (cap_bins bins) and their free ranges [start, len), sorted
@@ -1749,6 +1769,10 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.nbins = p->cap_bins;
     a.max_tier = p->kclass + 1;  // tier t + 1 is class t
     a.min_tier = 0;
+    a.split = 0;
+    a.early = 0;
+    a.room_f = TIERS[1].fcap;
+    a.room_g = TIERS[1].rcap;
     a.cand = p->cand; a.cand_cap = p->cand_cap; a.cstr = p->cstr; a.blk = p->blk;
     a.hkeys = p->hkeys; a.hvals = p->hvals; a.hmask = p->hmask;
     a.eslot = p->eslot; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
@@ -1757,29 +1781,38 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     return a;
 }
 
-// launches a kernel; when `spans` is given, with start / stop events around
-// that kernel alone (hipExtLaunchKernelGGL), filed under kernel slot `slot`
+// launches a kernel on stream st; when `spans` is given, with start / stop
+// events around that kernel alone (hipExtLaunchKernelGGL), filed under kernel
+// slot `slot`
 template <typename K, typename... Args>
-void launch(xkrt_hip_pool *p, std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans, int slot, K kernel,
-            dim3 grid, dim3 block, uint32_t lds, Args... args)
+void launch_on(xkrt_hip_pool *p, hipStream_t st, std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans,
+               int slot, K kernel, dim3 grid, dim3 block, uint32_t lds, Args... args)
 {
     if (spans) {
         hipEvent_t a = take_event(p), b = take_event(p);
         if (a && b) {
-            hipExtLaunchKernelGGL(kernel, grid, block, lds, p->stream, a, b, 0, args...);
+            hipExtLaunchKernelGGL(kernel, grid, block, lds, st, a, b, 0, args...);
             spans->push_back({slot, a, b});
             return;
         }
         if (a) p->ev_pool.push_back(a);
         if (b) p->ev_pool.push_back(b);
     }
-    hipLaunchKernelGGL(kernel, grid, block, lds, p->stream, args...);
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+}
+
+template <typename K, typename... Args>
+void launch(xkrt_hip_pool *p, std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans, int slot, K kernel,
+            dim3 grid, dim3 block, uint32_t lds, Args... args)
+{
+    launch_on(p, p->stream, spans, slot, kernel, grid, block, lds, args...);
 }
 
 template <int F, int R, int L>
 void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
-                    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
+                    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans, hipStream_t st = nullptr)
 {
+    if (!st) st = p->stream;
     // enough blocks to fill the chip at this tier's LDS footprint; each walks its share of the
     // list.  Tier 0 (large launches only) has four blocks per resident slot: each walks fewer
     // bins, and a waiting block starts wherever one finished, so the few bins that take several
@@ -1789,9 +1822,9 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
     const uint32_t over = tier == 0 ? 4u : 1u;
     const uint32_t blocks = std::min<uint32_t>(p->cap_bins, 256u * std::min<uint32_t>(per_cu, 16u) * over);
     if (p->cfg.direct_dedup)
-        launch(p, spans, 3, k_resolve<F, R, L, true>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
+        launch_on(p, st, spans, 3, k_resolve<F, R, L, true>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
     else
-        launch(p, spans, 3, k_resolve<F, R, L, false>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
+        launch_on(p, st, spans, 3, k_resolve<F, R, L, false>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
 }
 
 // the kernels of one launch (timing slots: count, -, -, resolve, dedup, final)
@@ -1802,15 +1835,31 @@ uint32_t count_blocks(uint64_t n)
 }
 
 void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
-                    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
+                    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans, bool allow_split)
 {
     // a small launch starts at tier 1 (one resolve kernel less: occupancy does not matter
     // there); every launch's first tier takes the touched bins itself (no k_alloc)
     const bool small = a.min_tier > 0;
-    launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), COUNT_LDS, a);
+    // A large launch of a class with a tier 2 lists the bins whose state already exceeds the
+    // first two tiers' headroom on an early list, and tier 2 resolves them on the side stream
+    // while tiers 0 and 1 run (tier 2's time is its heaviest bin, which otherwise started
+    // only after tier 1).  Captured graphs keep the single stream.
+    const bool split = allow_split && !small && p->kclass >= 1 && p->side;
+    KArgs ac = a;
+    ac.split = split ? 1u : 0u;
+    launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), COUNT_LDS, ac);
+    if (split) {
+        (void)hipEventRecord(p->ev_fork, p->stream);
+        (void)hipStreamWaitEvent(p->side, p->ev_fork, 0);
+        KArgs ae = a;
+        ae.early = 1;
+        launch_resolve<256, 512, 512>(p, ae, 2, spans, p->side);
+        (void)hipEventRecord(p->ev_join, p->side);
+    }
     // tier 0 first; bins that outgrow a tier are retried by the next one
     if (!small) launch_resolve<64, 128, 64>(p, a, 0, spans);
     launch_resolve<64, 128, 256>(p, a, 1, spans);
+    if (split) (void)hipStreamWaitEvent(p->stream, p->ev_join, 0);
     if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, a, 2, spans);
     if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, a, 3, spans);
     if (!p->cfg.direct_dedup) launch(p, spans, 4, k_dedup, dim3(1024), dim3(256), (uint32_t)sizeof(DedupLds), a);
@@ -1849,7 +1898,7 @@ int enqueue_small(xkrt_hip_pool *p, const uint4 *hdr, uint32_t nrec)
     a.min_tier = 1;
     const uint32_t rb = count_blocks(nrec);
     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
-    launch_kernels(p, a, rb, p->timing ? &spans : nullptr);
+    launch_kernels(p, a, rb, p->timing ? &spans : nullptr, false);
     if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
     return ticket(p);
 }
@@ -1865,7 +1914,7 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
             const KArgs a = make_args(p, p->rec_dev, NONE);
             const uint32_t rb = count_blocks(p->rec_cap);
             HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-            launch_kernels(p, a, rb, nullptr);
+            launch_kernels(p, a, rb, nullptr, false);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(p->stream, &g);
             if (e != hipSuccess) return hip_err(e);
@@ -1890,7 +1939,7 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
         const KArgs a = make_args(p, rec, counted ? NONE : nrec);
         const uint32_t rb = count_blocks(nrec);
         std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
-        launch_kernels(p, a, rb, p->timing ? &spans : nullptr);
+        launch_kernels(p, a, rb, p->timing ? &spans : nullptr, true);
         if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
     }
     return ticket(p);
@@ -1980,6 +2029,9 @@ void destroy_pool(xkrt_hip_pool *p)
     for (hipEvent_t e : p->dom_ev)
         if (e) (void)hipEventDestroy(e);
     if (p->snap) (void)hipHostFree(p->snap);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+    if (p->side) (void)hipStreamDestroy(p->side);
     if (p->own_stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -2009,6 +2061,16 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     } else {
         if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) { delete p; return -ENODEV; }
         p->own_stream = true;
+    }
+    if (p->kclass >= 1 &&
+        (hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess)) {
+        if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+        if (p->side) (void)hipStreamDestroy(p->side);
+        if (p->own_stream) (void)hipStreamDestroy(p->stream);
+        delete p;
+        return -ENODEV;
     }
     const uint64_t max_rects = p->cfg.max_rects ? p->cfg.max_rects : (1ull << 20);
     p->cfg.max_rects = max_rects;
@@ -2298,7 +2360,7 @@ int gate_arm(xkrt_hip_pool *p)
     a.nrec_ptr = p->gate_count;
     a.min_tier = 1;
     // grids for any count: k_count strides over the records
-    launch_kernels(p, a, count_blocks(p->cfg.small_launch_rects), nullptr);
+    launch_kernels(p, a, count_blocks(p->cfg.small_launch_rects), nullptr, false);
     rc = ticket(p);
     if (rc) return rc;
     p->gate_armed = true;
