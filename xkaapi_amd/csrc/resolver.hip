@@ -136,6 +136,7 @@ struct KArgs {
     // k_count: bins whose map or log already exceed twice room_f / room_g (the first two tiers'
     // room) go on the early list; k_resolve: 1 in the early pass of tier 2 over that list
     uint32_t split, room_f, room_g, early;
+    uint32_t skip1;           // 1: tier 0 passes bins to tier 2 (no tier-1 kernel in this launch)
     uint32_t *cand;           // staged form: CSTRIPES stripes of cand_cap candidates (3 u32 each)
     uint32_t cand_cap;        // per stripe
     uint32_t *cstr;           // CSTRIPES candidate counters, 16 u32 apart
@@ -838,8 +839,11 @@ __device__ __forceinline__ void escalate(const KArgs &a, uint32_t tier, uint4 e,
 {
     if (lane_id() != 0) return;
     if (tier < a.max_tier) {
-        const uint32_t pos = atomicAdd(&a.ctr->ntier[tier + 1], 1u);
-        a.active[(size_t)(tier + 1) * a.nbins + pos] = e;
+        // tier 1 has tier 0's fragment and log room and only more records; a large launch
+        // of a class with a tier 2 passes tier 0's bins straight to tier 2
+        const uint32_t next = tier == 0 && a.skip1 ? 2u : tier + 1;
+        const uint32_t pos = atomicAdd(&a.ctr->ntier[next], 1u);
+        a.active[(size_t)next * a.nbins + pos] = e;
     } else {
         atomicOr(&a.ctr->err, err);
         a.bin_cnt[e.x] = 0;
@@ -1771,6 +1775,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.min_tier = 0;
     a.split = 0;
     a.early = 0;
+    a.skip1 = 0;
     a.room_f = TIERS[1].fcap;
     a.room_g = TIERS[1].rcap;
     a.cand = p->cand; a.cand_cap = p->cand_cap; a.cstr = p->cstr; a.blk = p->blk;
@@ -1845,23 +1850,29 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
     // while tiers 0 and 1 run (tier 2's time is its heaviest bin, which otherwise started
     // only after tier 1).  Captured graphs keep the single stream.
     const bool split = allow_split && !small && p->kclass >= 1 && p->side;
-    KArgs ac = a;
+    // a large launch of a class with a tier 2 has no tier-1 kernel: tier 1 adds only record
+    // room to tier 0, and the few bins with more than 64 records go on to tier 2 (the kernel
+    // costs a launch slot even when empty, as it nearly always is on such launches)
+    const bool skip1 = !small && p->kclass >= 1;
+    KArgs ak = a;
+    ak.skip1 = skip1 ? 1u : 0u;
+    KArgs ac = ak;
     ac.split = split ? 1u : 0u;
     launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), COUNT_LDS, ac);
     if (split) {
         (void)hipEventRecord(p->ev_fork, p->stream);
         (void)hipStreamWaitEvent(p->side, p->ev_fork, 0);
-        KArgs ae = a;
+        KArgs ae = ak;
         ae.early = 1;
         launch_resolve<256, 512, 512>(p, ae, 2, spans, p->side);
         (void)hipEventRecord(p->ev_join, p->side);
     }
     // tier 0 first; bins that outgrow a tier are retried by the next one
-    if (!small) launch_resolve<64, 128, 64>(p, a, 0, spans);
-    launch_resolve<64, 128, 256>(p, a, 1, spans);
+    if (!small) launch_resolve<64, 128, 64>(p, ak, 0, spans);
+    if (!skip1) launch_resolve<64, 128, 256>(p, ak, 1, spans);
     if (split) (void)hipStreamWaitEvent(p->stream, p->ev_join, 0);
-    if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, a, 2, spans);
-    if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, a, 3, spans);
+    if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, ak, 2, spans);
+    if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, ak, 3, spans);
     if (!p->cfg.direct_dedup) launch(p, spans, 4, k_dedup, dim3(1024), dim3(256), (uint32_t)sizeof(DedupLds), a);
     launch(p, spans, 5, k_final, dim3(1024), dim3(256), 0, a);
 }
