@@ -1815,16 +1815,14 @@ void launch(xkrt_hip_pool *p, std::vector<std::tuple<int, hipEvent_t, hipEvent_t
 
 template <int F, int R, int L>
 void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
-                    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans, hipStream_t st = nullptr)
+                    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans, hipStream_t st = nullptr,
+                    uint32_t over = 1)
 {
     if (!st) st = p->stream;
-    // enough blocks to fill the chip at this tier's LDS footprint; each walks its share of the
-    // list.  Tier 0 (large launches only) has four blocks per resident slot: each walks fewer
-    // bins, and a waiting block starts wherever one finished, so the few bins that take several
-    // times the mean no longer pile up on some blocks while the others idle.
+    // enough blocks to fill the chip at this tier's LDS footprint (times `over`); each walks
+    // its share of the list
     const size_t lds = lds_bytes<F, R, L>();
     const uint32_t per_cu = std::max<uint32_t>(1u, (uint32_t)(160u * 1024u / lds));
-    const uint32_t over = tier == 0 ? 4u : 1u;
     const uint32_t blocks = std::min<uint32_t>(p->cap_bins, 256u * std::min<uint32_t>(per_cu, 16u) * over);
     if (p->cfg.direct_dedup)
         launch_on(p, st, spans, 3, k_resolve<F, R, L, true>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
@@ -1839,9 +1837,10 @@ uint32_t count_blocks(uint64_t n)
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 4 * KRPW - 1) / (4 * KRPW), 8192));
 }
 
-void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
+void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint64_t nrec,
                     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans, bool allow_split)
 {
+    const uint32_t rblocks = count_blocks(nrec);
     // a small launch starts at tier 1 (one resolve kernel less: occupancy does not matter
     // there); every launch's first tier takes the touched bins itself (no k_alloc)
     const bool small = a.min_tier > 0;
@@ -1859,6 +1858,11 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
     KArgs ac = ak;
     ac.split = split ? 1u : 0u;
     launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), COUNT_LDS, ac);
+    // Tier 0 has several blocks per resident slot: each walks fewer bins, and a waiting block
+    // starts wherever one finished, so the bins that take several times the mean no longer pile
+    // up on some blocks while the others idle.  Launches of many records (tens of records per
+    // bin) gain up to 8 blocks per slot; short ones lose to the extra block starts beyond 2.
+    const uint32_t over0 = nrec >= (1u << 16) ? 8u : 2u;
     if (split) {
         (void)hipEventRecord(p->ev_fork, p->stream);
         (void)hipStreamWaitEvent(p->side, p->ev_fork, 0);
@@ -1868,7 +1872,7 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint32_t rblocks,
         (void)hipEventRecord(p->ev_join, p->side);
     }
     // tier 0 first; bins that outgrow a tier are retried by the next one
-    if (!small) launch_resolve<64, 128, 64>(p, ak, 0, spans);
+    if (!small) launch_resolve<64, 128, 64>(p, ak, 0, spans, nullptr, over0);
     if (!skip1) launch_resolve<64, 128, 256>(p, ak, 1, spans);
     if (split) (void)hipStreamWaitEvent(p->stream, p->ev_join, 0);
     if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, ak, 2, spans);
@@ -1907,9 +1911,8 @@ int enqueue_small(xkrt_hip_pool *p, const uint4 *hdr, uint32_t nrec)
     KArgs a = make_args(p, hdr + 2, NONE);
     a.nrec_ptr = (const uint32_t *)hdr;
     a.min_tier = 1;
-    const uint32_t rb = count_blocks(nrec);
     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
-    launch_kernels(p, a, rb, p->timing ? &spans : nullptr, false);
+    launch_kernels(p, a, nrec, p->timing ? &spans : nullptr, false);
     if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
     return ticket(p);
 }
@@ -1923,9 +1926,8 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
         if (!p->graph_valid) {
             drop_graph(p);
             const KArgs a = make_args(p, p->rec_dev, NONE);
-            const uint32_t rb = count_blocks(p->rec_cap);
             HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-            launch_kernels(p, a, rb, nullptr, false);
+            launch_kernels(p, a, p->rec_cap, nullptr, false);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(p->stream, &g);
             if (e != hipSuccess) return hip_err(e);
@@ -1948,9 +1950,8 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
         }
     } else {
         const KArgs a = make_args(p, rec, counted ? NONE : nrec);
-        const uint32_t rb = count_blocks(nrec);
         std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
-        launch_kernels(p, a, rb, p->timing ? &spans : nullptr, true);
+        launch_kernels(p, a, nrec, p->timing ? &spans : nullptr, true);
         if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
     }
     return ticket(p);
@@ -2371,7 +2372,7 @@ int gate_arm(xkrt_hip_pool *p)
     a.nrec_ptr = p->gate_count;
     a.min_tier = 1;
     // grids for any count: k_count strides over the records
-    launch_kernels(p, a, count_blocks(p->cfg.small_launch_rects), nullptr, false);
+    launch_kernels(p, a, p->cfg.small_launch_rects, nullptr, false);
     rc = ticket(p);
     if (rc) return rc;
     p->gate_armed = true;
