@@ -90,9 +90,6 @@ struct DevCounters {
     unsigned long long cand_total;   // stats
     unsigned long long frag_visits;  // stats
     unsigned long long nlaunch;      // launches completed (k_final counts them)
-    unsigned int nearly;     // bins listed for the early top-tier pass, from the touched list's back
-                             // (listed by k_count; zeroed by k_final)
-    unsigned int pad_;
 };
 
 // What the host learns of a launch without a stream sync: k_final writes it to
@@ -133,10 +130,7 @@ struct KArgs {
     uint32_t nbins;
     uint32_t max_tier;                              // tiers 0..max_tier; storage sized for max_tier
     uint32_t min_tier;                              // first tier a bin may take (1 in small launches)
-    // k_count: bins whose map or log already exceed twice room_f / room_g (the first two tiers'
-    // room) go on the early list; k_resolve: 1 in the early pass of tier 2 over that list
-    uint32_t split, room_f, room_g, early;
-    uint32_t skip1;           // 1: tier 0 passes bins to tier 2 (no tier-1 kernel in this launch)
+    uint32_t skip1;                                 // 1: tier 0 passes bins to tier 2 (no tier-1 kernel)
     uint32_t *cand;           // staged form: CSTRIPES stripes of cand_cap candidates (3 u32 each)
     uint32_t cand_cap;        // per stripe
     uint32_t *cstr;           // CSTRIPES candidate counters, 16 u32 apart
@@ -297,12 +291,6 @@ __global__ void __launch_bounds__(256) k_count(KArgs a)
                     uint4 *o = a.binrec + 2ull * ((size_t)bin * a.lmax + pos);
                     o[0] = q0;
                     o[1] = make_uint4(q1.x, q1.y, r, 0u);
-                }
-                // a bin whose state already fits neither of the first two tiers goes on the
-                // early list, resolved by tier 2 beside them
-                if (first && a.split && (2 * a.bin_nf[bin] > a.room_f || 2 * a.bin_nrs[bin] > a.room_g)) {
-                    a.touched[a.nbins - 1 - atomicAdd(&a.ctr->nearly, 1u)] = bin;
-                    first = false;
                 }
             }
             // the bins this launch touches, listed once each
@@ -839,8 +827,8 @@ __device__ __forceinline__ void escalate(const KArgs &a, uint32_t tier, uint4 e,
 {
     if (lane_id() != 0) return;
     if (tier < a.max_tier) {
-        // tier 1 has tier 0's fragment and log room and only more records; a large launch
-        // of a class with a tier 2 passes tier 0's bins straight to tier 2
+        // tier 1 has tier 0's fragment and log room and only more record room; in a launch
+        // without a tier-1 kernel tier 0 passes its bins straight to tier 2
         const uint32_t next = tier == 0 && a.skip1 ? 2u : tier + 1;
         const uint32_t pos = atomicAdd(&a.ctr->ntier[next], 1u);
         a.active[(size_t)next * a.nbins + pos] = e;
@@ -884,13 +872,9 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     const uint32_t lane = lane_id();
     if (a.ctr->err) return;  // a poisoned domain touches no state
     // a launch's first tier takes every touched bin straight from the touched list
-    // (no k_alloc); later tiers take their active list (bins escalated from below); the
-    // early pass of tier 2 takes the early list at the touched list's back
-    const bool early = a.early != 0;
-    const bool touched = !early && tier == a.min_tier;
-    const uint32_t nlist = early     ? min(a.ctr->nearly, a.nbins)
-                           : touched ? min(a.ctr->ntouched, a.nbins)
-                                     : min(a.ctr->ntier[tier], a.nbins);
+    // (no k_alloc); later tiers take their active list (bins escalated from below)
+    const bool touched = tier == a.min_tier;
+    const uint32_t nlist = touched ? min(a.ctr->ntouched, a.nbins) : min(a.ctr->ntier[tier], a.nbins);
     const uint4 *list = a.active + (size_t)tier * a.nbins;
     unsigned long long visits = 0;
     uint32_t maxf = 0, maxg = 0;
@@ -905,8 +889,8 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     const uint32_t G = gridDim.x;
     auto entry = [&](uint32_t x) {
         if (x >= nlist) return make_uint4(0, 0, 0, 0);
-        if (touched || early) {
-            const uint32_t slot = early ? a.touched[a.nbins - 1 - x] : a.touched[x];
+        if (touched) {
+            const uint32_t slot = a.touched[x];
             return make_uint4(slot, 0u, a.bin_cnt[slot],
                               a.bin_nf[slot] | (a.bin_nrs[slot] << 11) | (a.bin_dom[slot] << 24));
         }
@@ -1285,7 +1269,6 @@ __global__ void k_final(KArgs a)
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
         a.ctr->ntouched = 0;  // the touched list of the next launch starts empty
-        a.ctr->nearly = 0;
         // the launch's outcome to pinned host memory (seen once the launch's completion is)
         const unsigned long long seq = ++a.ctr->nlaunch;
         LaunchSnap *o = a.snap + seq % NSNAP;
@@ -1465,9 +1448,6 @@ struct xkrt_hip_pool {
     xkrt_hip_pool_config_t cfg{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    // classes with a tier 2: its early pass runs on `side` between the fork and join events
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int device = 0;
     uint32_t kclass = 0, fcap = 0, rcap = 0;
     // bin arrays (cap_bins bins) and their free ranges [start, len), sorted
@@ -1773,11 +1753,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.nbins = p->cap_bins;
     a.max_tier = p->kclass + 1;  // tier t + 1 is class t
     a.min_tier = 0;
-    a.split = 0;
-    a.early = 0;
     a.skip1 = 0;
-    a.room_f = TIERS[1].fcap;
-    a.room_g = TIERS[1].rcap;
     a.cand = p->cand; a.cand_cap = p->cand_cap; a.cstr = p->cstr; a.blk = p->blk;
     a.hkeys = p->hkeys; a.hvals = p->hvals; a.hmask = p->hmask;
     a.eslot = p->eslot; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
@@ -1838,43 +1814,28 @@ uint32_t count_blocks(uint64_t n)
 }
 
 void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint64_t nrec,
-                    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans, bool allow_split)
+                    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
     const uint32_t rblocks = count_blocks(nrec);
     // a small launch starts at tier 1 (one resolve kernel less: occupancy does not matter
     // there); every launch's first tier takes the touched bins itself (no k_alloc)
     const bool small = a.min_tier > 0;
-    // A large launch of a class with a tier 2 lists the bins whose state already exceeds the
-    // first two tiers' headroom on an early list, and tier 2 resolves them on the side stream
-    // while tiers 0 and 1 run (tier 2's time is its heaviest bin, which otherwise started
-    // only after tier 1).  Captured graphs keep the single stream.
-    const bool split = allow_split && !small && p->kclass >= 1 && p->side;
-    // a large launch of a class with a tier 2 has no tier-1 kernel: tier 1 adds only record
-    // room to tier 0, and the few bins with more than 64 records go on to tier 2 (the kernel
-    // costs a launch slot even when empty, as it nearly always is on such launches)
-    const bool skip1 = !small && p->kclass >= 1;
+    // A large launch of under 128K records in a class with a tier 2 has no tier-1 kernel:
+    // tier 1 only adds record room (bins of more than 64 records), such launches hardly ever
+    // fill a bin that far, and the empty kernel still costs a launch slot.  Launches of more
+    // records keep it: many of their bins exceed 64 records, and tier 2 fits a third of
+    // tier 1's blocks per CU.
     KArgs ak = a;
-    ak.skip1 = skip1 ? 1u : 0u;
-    KArgs ac = ak;
-    ac.split = split ? 1u : 0u;
-    launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), COUNT_LDS, ac);
+    ak.skip1 = !small && p->kclass >= 1 && nrec < (1u << 17) ? 1u : 0u;
+    launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), COUNT_LDS, ak);
     // Tier 0 has several blocks per resident slot: each walks fewer bins, and a waiting block
     // starts wherever one finished, so the bins that take several times the mean no longer pile
     // up on some blocks while the others idle.  Launches of many records (tens of records per
     // bin) gain up to 8 blocks per slot; short ones lose to the extra block starts beyond 2.
     const uint32_t over0 = nrec >= (1u << 16) ? 8u : 2u;
-    if (split) {
-        (void)hipEventRecord(p->ev_fork, p->stream);
-        (void)hipStreamWaitEvent(p->side, p->ev_fork, 0);
-        KArgs ae = ak;
-        ae.early = 1;
-        launch_resolve<256, 512, 512>(p, ae, 2, spans, p->side);
-        (void)hipEventRecord(p->ev_join, p->side);
-    }
     // tier 0 first; bins that outgrow a tier are retried by the next one
     if (!small) launch_resolve<64, 128, 64>(p, ak, 0, spans, nullptr, over0);
-    if (!skip1) launch_resolve<64, 128, 256>(p, ak, 1, spans);
-    if (split) (void)hipStreamWaitEvent(p->stream, p->ev_join, 0);
+    if (!ak.skip1) launch_resolve<64, 128, 256>(p, ak, 1, spans);
     if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, ak, 2, spans);
     if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, ak, 3, spans);
     if (!p->cfg.direct_dedup) launch(p, spans, 4, k_dedup, dim3(1024), dim3(256), (uint32_t)sizeof(DedupLds), a);
@@ -1912,7 +1873,7 @@ int enqueue_small(xkrt_hip_pool *p, const uint4 *hdr, uint32_t nrec)
     a.nrec_ptr = (const uint32_t *)hdr;
     a.min_tier = 1;
     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
-    launch_kernels(p, a, nrec, p->timing ? &spans : nullptr, false);
+    launch_kernels(p, a, nrec, p->timing ? &spans : nullptr);
     if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
     return ticket(p);
 }
@@ -1927,7 +1888,7 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
             drop_graph(p);
             const KArgs a = make_args(p, p->rec_dev, NONE);
             HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-            launch_kernels(p, a, p->rec_cap, nullptr, false);
+            launch_kernels(p, a, p->rec_cap, nullptr);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(p->stream, &g);
             if (e != hipSuccess) return hip_err(e);
@@ -1951,7 +1912,7 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
     } else {
         const KArgs a = make_args(p, rec, counted ? NONE : nrec);
         std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
-        launch_kernels(p, a, nrec, p->timing ? &spans : nullptr, true);
+        launch_kernels(p, a, nrec, p->timing ? &spans : nullptr);
         if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
     }
     return ticket(p);
@@ -2041,9 +2002,6 @@ void destroy_pool(xkrt_hip_pool *p)
     for (hipEvent_t e : p->dom_ev)
         if (e) (void)hipEventDestroy(e);
     if (p->snap) (void)hipHostFree(p->snap);
-    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
-    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
-    if (p->side) (void)hipStreamDestroy(p->side);
     if (p->own_stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -2073,16 +2031,6 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     } else {
         if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) { delete p; return -ENODEV; }
         p->own_stream = true;
-    }
-    if (p->kclass >= 1 &&
-        (hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
-         hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
-         hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess)) {
-        if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
-        if (p->side) (void)hipStreamDestroy(p->side);
-        if (p->own_stream) (void)hipStreamDestroy(p->stream);
-        delete p;
-        return -ENODEV;
     }
     const uint64_t max_rects = p->cfg.max_rects ? p->cfg.max_rects : (1ull << 20);
     p->cfg.max_rects = max_rects;
@@ -2372,7 +2320,7 @@ int gate_arm(xkrt_hip_pool *p)
     a.nrec_ptr = p->gate_count;
     a.min_tier = 1;
     // grids for any count: k_count strides over the records
-    launch_kernels(p, a, p->cfg.small_launch_rects, nullptr, false);
+    launch_kernels(p, a, p->cfg.small_launch_rects, nullptr);
     rc = ticket(p);
     if (rc) return rc;
     p->gate_armed = true;
