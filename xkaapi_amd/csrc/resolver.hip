@@ -578,26 +578,35 @@ __device__ void do_link(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s, BinSta
         __syncthreads();
     }
     const uint32_t no = s.novl;
-    for (uint32_t g0 = 0; g0 < st.ng; g0 += 64) {
-        const uint32_t g = g0 + lane;
-        if (g < st.ng) {
-            const uint32_t gx0 = s.gx0[g], gx1 = s.gx1[g], gy0 = s.gy0[g], gy1 = s.gy1[g], ga = s.gacc[g];
-            if (overlaps(gx0, gx1, gy0, gy1, X0, X1, Y0, Y1)) {
-                bool pred = false;
-                for (uint32_t k = 0; k < no; ++k) {
-                    const uint32_t f = s.ovl[k], lw = s.flw[fb][f];
-                    if (lw != NONE && ga <= lw) continue;
-                    const uint32_t qx0 = max(s.fx0[fb][f], X0), qx1 = min(s.fx1[fb][f], X1);
-                    const uint32_t qy0 = max(s.fy0[fb][f], Y0), qy1 = min(s.fy1[fb][f], Y1);
-                    if (!overlaps(gx0, gx1, gy0, gy1, qx0, qx1, qy0, qy1)) continue;
+    // one lane per (logged reader, overlapping fragment) pair, reader-major: a lane per reader
+    // walking every overlapping fragment left most lanes idle on bins with few readers and
+    // many fragments, each step a dependent LDS read
+    const uint32_t npairs = st.ng * no;
+    for (uint32_t t0 = 0; t0 < npairs; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        bool pred = false;
+        uint32_t ga = 0;
+        if (t < npairs) {
+            const uint32_t g = t / no, k = t - g * no;
+            const uint32_t gx0 = s.gx0[g], gx1 = s.gx1[g], gy0 = s.gy0[g], gy1 = s.gy1[g];
+            ga = s.gacc[g];
+            const uint32_t f = s.ovl[k], lw = s.flw[fb][f];
+            if (lw == NONE || ga > lw) {
+                const uint32_t qx0 = max(s.fx0[fb][f], X0), qx1 = min(s.fx1[fb][f], X1);
+                const uint32_t qy0 = max(s.fy0[fb][f], Y0), qy1 = min(s.fy1[fb][f], Y1);
+                if (overlaps(gx0, gx1, gy0, gy1, qx0, qx1, qy0, qy1)) {
                     pred = true;
                     const bool full = gx0 <= qx0 && qx1 <= gx1 && gy0 <= qy0 && qy1 <= gy1;
                     atomicAdd(&s.okc[k], 1u);
                     if (full) atomicOr(&s.okc[k], 1u << 31);
                 }
-                if (pred) emit(a, s, ga, acc, task);
             }
         }
+        // a reader is emitted once per pass over its pairs; its repeats across lanes and
+        // passes collapse in the (pred, task) set
+        const uint64_t pm = __ballot(pred);
+        const uint32_t prev = __shfl(ga, (int)(lane + 63u) & 63, 64);  // the lane below (all lanes take part)
+        if (pred && (lane == 0 || !((pm >> (lane - 1)) & 1ull) || prev != ga)) emit(a, s, ga, acc, task);
     }
     __syncthreads();
     for (uint32_t c = 0; c < no; c += 64) {
