@@ -682,19 +682,17 @@ __device__ void do_run(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s, BinStat
 {
     const uint32_t fb = st.fb;
     const uint32_t lane = lane_id();
-    for (uint32_t c = i0; c < i1; c += 64) {
-        const uint32_t q = c + lane;
-        const bool live = q < i1;
-        uint32_t X0 = 0, X1 = 0, Y0 = 0, Y1 = 0, acc = 0, task = 0;
-        if (live) {
-            X0 = s.rx0[q]; X1 = s.rx1[q]; Y0 = s.ry0[q]; Y1 = s.ry1[q];
-            acc = s.racc[q]; task = s.rtm[q] >> 8;
-        }
-        for (uint32_t f = 0; f < st.nf; ++f) {
+    // one lane per (record, fragment) pair, record-major: a lane per record walking every
+    // fragment left most lanes idle on short runs
+    const uint32_t nf = st.nf, npairs = (i1 - i0) * nf;
+    for (uint32_t t0 = 0; t0 < npairs; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        if (t < npairs) {
+            const uint32_t r = t / nf, f = t - r * nf, q = i0 + r;
             const uint32_t lw = s.flw[fb][f];
-            if (live && lw != NONE &&
-                overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f], X0, X1, Y0, Y1))
-                emit(a, s, lw, acc, task);
+            if (lw != NONE &&
+                overlaps(s.fx0[fb][f], s.fx1[fb][f], s.fy0[fb][f], s.fy1[fb][f], s.rx0[q], s.rx1[q], s.ry0[q], s.ry1[q]))
+                emit(a, s, lw, s.racc[q], s.rtm[q] >> 8);
         }
     }
     __syncthreads();
