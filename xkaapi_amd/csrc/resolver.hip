@@ -2947,6 +2947,21 @@ int srv_launch(xkrt_hip_server *s, bool &work)
         if (rc) c->status = rc;
         work = true;
     }
+    // A gated launch reads its records in place and so stays a small launch.  When more is
+    // queued than one small launch takes, the gate is cancelled and the queue goes out in one
+    // large launch: small gated launches one after another let a backlog under load grow.
+    if (p->gate_armed) {
+        uint64_t est = 0;
+        for (SrvRequest *r : s->pending) {
+            if (!r->c->dom || r->c->status) continue;
+            est += 3 * (uint64_t)r->acc.size();  // at most 3 rects per access
+            if (est > p->cfg.small_launch_rects) break;
+        }
+        if (est > p->cfg.small_launch_rects) {
+            const int rc = gate_cancel(p);
+            if (rc) return rc;
+        }
+    }
     std::vector<SrvRequest *> batch;
     std::deque<SrvRequest *> keep;
     while (!s->pending.empty()) {

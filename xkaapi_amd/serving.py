@@ -125,13 +125,15 @@ def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30
           graph: bool = True, depth: int = 1, max_rects: int = 1 << 20, bin_x_bytes: int = 1024,
           bin_y_cols: int = 128, reserve: bool = True, max_edges: int = 0,
           direct_dedup: bool = False, small_launch_rects: int = 16384, prearm: bool = True,
-          launch_rects: int = 0) -> RunResult:
+          launch_rects: int = 32768) -> RunResult:
     """Run one schedule through a fresh pool and serving engine; mode "batched" or "plain".
     Defaults (measured on the 64-client workload): the small LDS class (0), one
     launch in flight (a batch that arrives during a launch goes into the next
     one, which starts as soon as the current one ends), staged dedup, launches
-    of up to 16384 rects read in place from pinned host memory, and the next
-    launch queued on the device behind a gate (prearm).
+    of up to 16384 rects read in place from pinned host memory, the next
+    launch queued on the device behind a gate (prearm), and at most 32768 rects
+    per launch (launch_rects; 0 = no cap).  With no cap a backlog went out in a
+    few huge launches, which resolve more slowly per access.
     reserve: allocate the bins of every client up front (within the budget), so
     admissions do not allocate device memory while serving."""
     byid = {c.cid: c for c in clients}
