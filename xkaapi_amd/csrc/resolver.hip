@@ -2951,9 +2951,10 @@ int srv_launch(xkrt_hip_server *s, bool &work)
     // queued than one small launch takes, the gate is cancelled and the queue goes out in one
     // large launch: small gated launches one after another let a backlog under load grow.
     if (p->gate_armed) {
-        uint64_t est = 0;
+        uint64_t est = 0, nb = 0;
         for (SrvRequest *r : s->pending) {
             if (!r->c->dom || r->c->status) continue;
+            if (s->cfg.max_batches && nb++ >= s->cfg.max_batches) break;  // what this launch may take
             est += 3 * (uint64_t)r->acc.size();  // at most 3 rects per access
             if (est > p->cfg.small_launch_rects) break;
         }
