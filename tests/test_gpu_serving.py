@@ -44,6 +44,15 @@ def test_saturation_mix(clients, mode):
         assert res.launches < sum(len(c.batches) for c in clients)
 
 
+def test_launch_cap_mix(clients):
+    # a cap on rects per launch splits the closed loop's backlog over more launches; same edges
+    capped = V.saturation(clients, "batched", max_rects=1 << 16, launch_rects=1024)
+    check(capped, clients)
+    free = V.saturation(clients, "batched", max_rects=1 << 16, launch_rects=0)
+    check(free, clients)
+    assert capped.launches > free.launches
+
+
 def test_open_loop_mix(clients):
     sat = V.saturation(clients, "batched", max_rects=1 << 16)
     rate = 0.5 * sum(len(c.batches) for c in clients) / sat.makespan_s
