@@ -45,12 +45,19 @@ def test_saturation_mix(clients, mode):
 
 
 def test_launch_cap_mix(clients):
-    # a cap on rects per launch splits the closed loop's backlog over more launches; same edges
-    capped = V.saturation(clients, "batched", max_rects=1 << 16, launch_rects=1024)
+    # a cap on rects per launch splits the closed loop's backlog over more launches; same edges.
+    # A launch closes once it holds `cap` rects and takes at most one batch more (at most 3
+    # rects per access), so the capped run needs at least total / (cap + 3 * largest batch)
+    # launches.  How the uncapped run splits the flood depends on the engine's timing.
+    cap = 128
+    capped = V.saturation(clients, "batched", max_rects=1 << 16, launch_rects=cap)
     check(capped, clients)
     free = V.saturation(clients, "batched", max_rects=1 << 16, launch_rects=0)
     check(free, clients)
-    assert capped.launches > free.launches
+    total = sum(len(c.stream) for c in clients)
+    largest = max(e - a for c in clients for a, e in c.batches)
+    assert capped.launches >= -(-total // (cap + 3 * largest))
+    assert capped.launches > 1
 
 
 def test_open_loop_mix(clients):
