@@ -101,21 +101,184 @@ class Prepared:
             dom.submit_device(dev.data_ptr(), nrec, nacc, ylo, yhi)
 
 
-def cpu_baseline(st, target_s=10.0):
-    """Reference KHP engine (oracle/_ref, built from the reference sources) on
-    the Cholesky stream, repeated to about target_s seconds on one core."""
+def host_info():
+    """CPU model and the cores this process may use (the GPU box's host)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu": model, "nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count()}
+
+
+def load_digests():
+    with open(os.path.join(ROOT, "tests", "golden", "digests.json")) as fh:
+        return json.load(fh)
+
+
+def cpu_baseline(chol, rnd_prefix=10_000, chol_target_s=2.0):
+    """The reference's own KHP engine (oracle/_ref, built from the reference
+    sources) on a bounded sample of the headline workload, one core: the whole
+    Cholesky stream (repeated to about chol_target_s) and the first rnd_prefix
+    accesses of the random stream (its canonical fragmentation grows too fast
+    for more: ~2,500 fragments per access at 10^5 accesses).  value = sample
+    accesses / sample seconds, the streams weighted as one Cholesky pass plus
+    the random prefix."""
+    from oracle import refkhp
+    from xkaapi_amd import streams as S
+    if not refkhp.available():
+        return None
+    r, nr = chol.rects()
+    tot_ns, reps = 0, 0
+    while tot_ns < chol_target_s * 1e9 and reps < 1000:
+        ns, _ = refkhp.time_resolve(chol.task, chol.mode, r, nr)
+        tot_ns += ns
+        reps += 1
+    chol_rate = reps * len(chol) / (tot_ns / 1e9)
+    rs = S.random2d(rnd_prefix)
+    r, nr = rs.rects()
+    rns, _ = refkhp.time_resolve(rs.task, rs.mode, r, nr)
+    rnd_rate = len(rs) / (rns / 1e9)
+    t_sample = len(chol) / chol_rate + rns / 1e9
+    return {"value": round((len(chol) + len(rs)) / t_sample, 1), "unit": "accesses/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"Cholesky 8192^2 nb128 ({len(chol)} accesses, timed x{reps}) + the first {len(rs)} random "
+                      f"accesses; LINK+PUT only, reference khp-tree.hpp engine, 1 thread",
+            "per_stream": {"cholesky_8192_nb128": round(chol_rate, 1), f"random2d_prefix_{len(rs)}": round(rnd_rate, 1)},
+            "host": host_info()}
+
+
+def cpu_baseline_heat(steps=20):
+    """Reference engine on the first `steps` steps of the heat halo stream, one core."""
+    from oracle import refkhp
+    from xkaapi_amd import streams as S
+    if not refkhp.available():
+        return None
+    st = S.heat(16384, 128, 4, steps)
+    r, nr = st.rects()
+    ns, ne = refkhp.time_resolve(st.task, st.mode, r, nr)
+    return {"value": round(len(st) / (ns / 1e9), 1), "unit": "accesses/s", "cores": 1, "kind": "reference",
+            "sample": f"heat 16384^2, first {steps} of 200 steps ({len(st)} accesses, {ne} edges), 1 thread"}
+
+
+def cpu_baseline_serving(clients):
+    """Reference engine, one client domain per core (threads over min(64, cores)
+    cores), every client's whole stream: accesses / wall seconds."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import refkhp
     if not refkhp.available():
         return None
-    r, nr = st.rects()
-    tot_ns, reps = 0, 0
-    while tot_ns < target_s * 1e9 and reps < 1000:
-        ns, ne = refkhp.time_resolve(st.task, st.mode, r, nr)
-        tot_ns += ns
-        reps += 1
-    return {"value": reps * len(st) / (tot_ns / 1e9), "unit": "accesses/s", "cores": 1, "kind": "reference",
-            "sample": f"Cholesky 8192^2 nb128 stream ({len(st)} accesses) x{reps}, LINK+PUT only, "
-                      f"reference khp-tree.hpp engine, 1 thread"}
+    work = [(c.stream.task, c.stream.mode) + c.stream.rects() for c in clients]
+    cores = min(64, len(os.sched_getaffinity(0)))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        list(ex.map(lambda w: refkhp.time_resolve(*w), work))
+    dt = time.perf_counter() - t0
+    n = sum(len(c.stream) for c in clients)
+    return {"value": round(n / dt, 1), "unit": "accesses/s", "cores": cores, "kind": "reference",
+            "sample": f"{len(clients)} client streams ({n} accesses), one domain per thread, closed loop"}
+
+
+def roofline_leg(torch, prefix_name="random2d_102400"):
+    """Roofline of the dominant kernel, k_resolve, on the random stream: the
+    plain form's 4096-task launches over the stream's first 102,400 accesses
+    (the oracle resolves no more in reasonable time), with the benchmark's bins
+    (2048 B x 256 columns, class 1).  Algorithmic bytes per launch from the
+    oracle's counts (tests/golden/digests.json): B_alg = (24 N_acc + 64 sum F_i
+    + 12 E) / launches.  Duration: HIP events around each k_resolve tier on the
+    launch stream."""
+    from xkaapi_amd import resolver as R
+    from xkaapi_amd import streams as S
+    d = load_digests()[prefix_name]
+    st = S.random2d(d["accesses"])
+    acc = R.to_access_array(st.task, st.mode, st.addr, st.m, st.n)
+    batches = R.batches_by_tasks(st.task, d["batch_tasks"])
+    stream = torch.cuda.current_stream().cuda_stream
+    dom = R.DependencyDomain(int(st.ld[0]), int(st.s[0]), bin_x_bytes=2048, bin_y_cols=256, kernel=1,
+                             max_rects=2 * d["batch_tasks"], max_edges=8 * len(st) + (1 << 20), stream=stream)
+    dev = []
+    for a, b in batches:
+        recs, ylo, yhi = dom.build_records(acc[a:b], a)
+        dev.append((torch.from_numpy(recs.view(np.int32).reshape(-1)).cuda(), len(recs), b - a, ylo, yhi))
+    kts = []
+    for rep in range(3):
+        dom.reset()
+        dom.set_timing(rep > 0)
+        for t, nrec, nacc, ylo, yhi in dev:
+            dom.submit_device(t.data_ptr(), nrec, nacc, ylo, yhi)
+        edges = dom.sync()
+        assert edges == d["edges"], "the roofline pass must find the oracle's edge count"
+        if rep > 0:
+            kt, nl = dom.kernel_times(clear=True)
+            kts.append(kt["k_resolve"] / nl)
+        dom.set_timing(False)
+    dom.close()
+    nl = len(batches)
+    b_alg = (24 * d["accesses"] + 64 * sum(d["sum_frags_per_batch"]) + 12 * d["edges"]) / nl
+    t_res = min(kts) * 1e-3
+    achieved = b_alg / t_res / 1e9
+    traffic = prof_us = None
+    tp = os.path.join(ROOT, "profiles", "resolver_random2d_prefix_plain_traffic.json")
+    if os.path.exists(tp):  # HBM bytes per launch from the committed rocprofv3 counter runs
+        t = json.load(open(tp))
+        traffic = int(1024 * (t["fetch_kib_per_launch"] * t["fetch_factor"]
+                              + t["write_kib_per_launch"] * t["write_factor"]))
+        prof_us = t.get("rocprof_avg_us_per_launch")
+    out = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+           "kernel": "k_resolve", "avg_us": round(t_res * 1e6, 2), "rocprof_avg_us": prof_us,
+           "bytes_per_launch": int(b_alg),
+           "units_per_launch": {"accesses": d["accesses"] / nl, "sum_F": sum(d["sum_frags_per_batch"]) / nl,
+                                "edges": d["edges"] / nl},
+           "workload": f"random2d, first {d['accesses']} accesses in {nl} plain 4096-task launches, "
+                       "bins 2048 B x 256 cols, class 1; sum F_i and E from the oracle"}
+    if traffic:
+        out["traffic_achieved"] = round(traffic / t_res / 1e9, 3)
+        out["traffic_frac"] = round(traffic / t_res / 1e9 / HBM_PEAK_GBS, 6)
+    return out
+
+
+def e2e_leg(chol, rnd, merge, reps=3):
+    """End to end, as a runtime would drive it: per batch the host builds the
+    rect records from the accesses and uploads them (xkrt_hip_dep_submit),
+    the device resolves, and at the end every edge is copied to host memory.
+    Timed from the first submit to the last edge on the host; both forms."""
+    from xkaapi_amd import resolver as R
+    out = {}
+    for form in ("plain", "batched"):
+        total_s = 0.0
+        nacc = 0
+        for name, st, kw in (("cholesky_8192_nb128", chol, dict(bin_x_bytes=1024, bin_y_cols=128, kernel=0)),
+                             ("random2d_65536", rnd, dict(bin_x_bytes=2048, bin_y_cols=256, kernel=1))):
+            if st is None:
+                continue
+            acc = R.to_access_array(st.task, st.mode, st.addr, st.m, st.n)
+            tpb = 4096 if form == "plain" else 4096 * merge[name]
+            batches = R.batches_by_tasks(st.task, tpb)
+            best = None
+            with R.DependencyDomain(int(st.ld[0]), int(st.s[0]), max_rects=2 * max(b - a for a, b in batches),
+                                    max_edges=8 * len(st) + (1 << 20), **kw) as dom:
+                for _ in range(reps):
+                    dom.reset()
+                    dom.sync()
+                    t0 = time.perf_counter()
+                    for a, b in batches:
+                        dom.submit(acc[a:b])
+                    e = dom.edges()
+                    dt = time.perf_counter() - t0
+                    best = dt if best is None else min(best, dt)
+            total_s += best
+            nacc += len(st)
+            out.setdefault(form, {})[name] = {"s": round(best, 4), "edges": int(len(e))}
+        out[form]["accesses_per_s"] = round(nacc / total_s, 1)
+        out[form]["ms_per_step"] = round(total_s * 1e3, 3)
+    out["workload"] = ("accesses handed over in host memory; records built and uploaded per launch, edges copied "
+                       "to host at the end; plain: 4096-task launches, batched: the tuned batches per launch")
+    return out
 
 
 def coherency_leg(n_stream=6000, reps=3):
@@ -179,22 +342,21 @@ def coherency_cpu_baseline(subs):
             "sample": f"the same {n} accesses, oracle/khp_oracle.c (restated memory-tree.hpp), 1 thread"}
 
 
-def heat_leg(torch, steps=200, reps=2):
+def heat_leg(torch, steps=200, reps=3, finalists=2):
     """Heat-diffusion halo stream, 16384^2 fp32 in 128^2 tiles (16,384 tiles),
     `steps` timesteps; uneven batches (geometric, mean 2048 tasks, max 65536).
-    Bins and capacity class from the autotuner on a prefix.  Records resident
-    in HBM; plain = one launch per batch, batched = consecutive batches merged
-    into launches of up to 2^20 records."""
+    Records resident in HBM; plain = one launch per batch, batched =
+    consecutive batches merged into launches of up to 2^20 records.
+    Bins and capacity class: the autotuner ranks the candidates on four
+    windows spread across the stream, resolved in launches of the batched
+    form's size, and the best `finalists` are then timed on the whole stream
+    (batched form, median of reps passes); the fastest is used for both forms."""
     from xkaapi_amd import autotune as A
     from xkaapi_amd import resolver as R
     from xkaapi_amd import streams as S
     st = S.heat(16384, 128, 4, steps)
     sizes = S.geometric_batches(st.ntasks, 2048.0, 65536, 0x3B)
     batches = R.batches_by_tasks(st.task, sizes)
-    t0 = time.time()
-    tune = A.tune(st.take(slice(0, 300_000)), tasks_per_batch=2048, max_cands=4)
-    log(f"heat: {len(st)} acc / {st.ntasks} tasks / {len(batches)} batches; autotune {tune.bin_x_bytes}x"
-        f"{tune.bin_y_cols} class {tune.kernel} ({time.time()-t0:.1f}s) tried {tune.tried}")
     acc = R.to_access_array(st.task, st.mode, st.addr, st.m, st.n)
     cap = 1 << 20
     groups, cur = [], None
@@ -206,36 +368,57 @@ def heat_leg(torch, steps=200, reps=2):
                 groups.append(cur)
             cur = (a, b)
     groups.append(cur)
-    out = {}
-    for form, spans in (("plain", batches), ("batched", groups)):
+    t0 = time.time()
+    launch_tasks = int(np.median([st.task[b - 1] - st.task[a] + 1 for a, b in groups]))
+    tune = A.tune(A.spread_windows(st, nwin=4, win_tasks=launch_tasks), tasks_per_batch=2048, max_cands=4,
+                  launch_tasks=launch_tasks)
+    log(f"heat: {len(st)} acc / {st.ntasks} tasks / {len(batches)} batches; windows ranked {tune.ranked} "
+        f"({time.time()-t0:.1f}s)")
+
+    def run_form(spans, cfg, nreps):
+        bx, by, kc = cfg
         maxrec = 2 * max(b - a for a, b in spans)
-        dom = R.DependencyDomain(int(st.ld[0]), int(st.s[0]), bin_x_bytes=tune.bin_x_bytes,
-                                 bin_y_cols=tune.bin_y_cols, kernel=tune.kernel, max_rects=maxrec,
-                                 max_edges=4 * len(st) + (1 << 20), stream=torch.cuda.current_stream().cuda_stream)
+        dom = R.DependencyDomain(int(st.ld[0]), int(st.s[0]), bin_x_bytes=bx, bin_y_cols=by, kernel=kc,
+                                 max_rects=maxrec, max_edges=4 * len(st) + (1 << 20),
+                                 stream=torch.cuda.current_stream().cuda_stream)
         dev = []
         for a, b in spans:
             recs, ylo, yhi = dom.build_records(acc[a:b], a)
             dev.append((torch.from_numpy(recs.view(np.int32).reshape(-1)).cuda(), len(recs), b - a, ylo, yhi))
         torch.cuda.synchronize()
-        best, edges = None, 0
-        for _ in range(reps):
+        ts, edges = [], 0
+        for _ in range(nreps):
             dom.reset()
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             for d, nrec, nacc, ylo, yhi in dev:
                 dom.submit_device(d.data_ptr(), nrec, nacc, ylo, yhi)
             edges = dom.sync()
-            dt = time.perf_counter() - t1
-            best = dt if best is None else min(best, dt)
-        out[form] = {"launches": len(spans), "s": round(best, 4), "accesses_per_s": round(len(st) / best, 1),
-                     "edges": int(edges)}
+            ts.append(time.perf_counter() - t1)
         dom.close()
         del dev
         torch.cuda.empty_cache()
+        return float(np.median(ts)), min(ts), int(edges)
+
+    # the finalists on the whole stream, batched form
+    final = {}
+    for bx, by, kc, _ in tune.ranked[:finalists]:
+        med, _, _ = run_form(groups, (bx, by, kc), reps)
+        final[(bx, by, kc)] = med
+    pick = min(final, key=final.get)
+    log(f"heat: finalists on the whole stream (median s) {final} -> {pick}")
+    out = {}
+    for form, spans in (("plain", batches), ("batched", groups)):
+        med, best, edges = run_form(spans, pick, reps)
+        out[form] = {"launches": len(spans), "s": round(best, 4), "median_s": round(med, 4),
+                     "accesses_per_s": round(len(st) / best, 1), "edges": edges}
     out["speedup_batched_vs_plain"] = round(out["batched"]["accesses_per_s"] / out["plain"]["accesses_per_s"], 3)
     out["workload"] = (f"heat 16384^2 fp32, 128^2 tiles, {steps} steps: {len(st)} accesses, {st.ntasks} tasks, "
                        f"{len(batches)} geometric batches (mean 2048, max 65536)")
-    out["autotune"] = {"bin_x_bytes": tune.bin_x_bytes, "bin_y_cols": tune.bin_y_cols, "kernel": tune.kernel}
+    out["autotune"] = {"bin_x_bytes": pick[0], "bin_y_cols": pick[1], "kernel": pick[2],
+                       "windows_ranked": [list(r[:3]) + [round(r[3], 5)] for r in tune.ranked],
+                       "finalists_whole_stream_s": {f"{k[0]}x{k[1]}c{k[2]}": round(v, 4) for k, v in final.items()},
+                       "window_tasks": launch_tasks}
     return out
 
 

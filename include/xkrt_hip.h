@@ -7,10 +7,14 @@
  *   -EINVAL     bad argument (null handle, zero ld, non-contiguous tasks, ...)
  *   -ENODEV     no HIP device / the HIP runtime failed
  *   -ENOMEM     device memory budget or allocation exhausted
- *   -ENOSPC     a per-bin capacity (fragments, readers, records) was exceeded;
- *               the domain is poisoned and must be destroyed
- *   -EOVERFLOW  the edge / candidate buffers overflowed; the domain is poisoned
- *   -ERANGE     a coordinate does not fit the domain's 32-bit frame
+ *   -ENOSPC     a per-bin capacity (fragments, readers, records) was exceeded:
+ *               the domain failed (reset or destroy it); the other domains of
+ *               its pool are not affected
+ *   -EOVERFLOW  the pool's edge / candidate buffers overflowed; every domain of
+ *               the pool is poisoned
+ *   -ERANGE     a coordinate does not fit the domain's 32-bit frame, or a
+ *               device record leaves its domain's grid (that domain fails)
+ *   -ECANCELED  a batch for a domain that failed earlier (reset it first)
  *   -ENOTSUP    an access the device path does not reproduce bit-exactly
  *               (a wrapped 2-rect tile that the reference would mark beyond
  *               its rects, see xkrt_hip_dep_submit)
@@ -97,8 +101,9 @@ typedef struct xkrt_hip_dep_stats {
     uint64_t max_reads;     /* largest reader-pool use seen in one bin */
     uint64_t frag_visits;   /* fragment records scanned by the resolve kernel */
     uint64_t device_bytes;  /* device memory held by the domain */
-    uint64_t err;           /* sticky device error bits (0 = healthy): 1 incidences, 2 bin record list,
-                               4 bin fragments, 8 bin readers, 16 candidates, 32 dedup table, 64 edges */
+    uint64_t err;           /* sticky device error bits (0 = healthy).  The domain's own: 2 bin record
+                               list, 4 bin fragments, 8 bin readers, 128 record outside the grid; its
+                               pool's: 16 candidates, 32 dedup table, 64 edges */
 } xkrt_hip_dep_stats_t;
 
 typedef struct xkrt_hip_dep_domain *xkrt_hip_dep_domain_t;
@@ -132,9 +137,14 @@ int xkrt_hip_dep_domain_destroy(xkrt_hip_dep_domain_t d);
 int xkrt_hip_dep_submit(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n, uint64_t *first_id);
 
 /* Same, from device-resident rect records (no host conversion, no copy):
- * nrect records of this domain, as xkrt_hip_dep_build_records writes them;
- * [ylo, yhi) bounds every record's y (domain frame); naccess = accesses the
- * records belong to (for accounting).  One launch. */
+ * nrect records, as xkrt_hip_dep_build_records writes them; [ylo, yhi)
+ * bounds every record's y (domain frame) and sizes d's grid; naccess =
+ * accesses the records belong to (for accounting).  One launch.  Records
+ * name their domain (dom): in a pool they may belong to several domains.
+ * The device checks every record against its domain's grid: a record that
+ * leaves it (or an empty one) fails its domain with -ERANGE at the next
+ * sync, before any of that domain's state changes in the launch; a record
+ * naming no domain slot poisons the pool. */
 int xkrt_hip_dep_submit_device(xkrt_hip_dep_domain_t d, const xkrt_hip_rect_t *rec, uint64_t nrect,
                                uint64_t naccess, uint32_t ylo, uint32_t yhi);
 
@@ -149,7 +159,9 @@ int xkrt_hip_dep_build_records(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t 
 /* y base of the domain frame (fixed at the first submit or from the hint). */
 int xkrt_hip_dep_frame(xkrt_hip_dep_domain_t d, uint64_t *ybase);
 
-/* Waits for every enqueued launch; *nedges = edges not yet consumed. */
+/* Waits for every enqueued launch; *nedges = edges not yet consumed (of the
+ * whole pool).  Returns the pool's error, else the domain's own (a domain
+ * that failed in a shared pool fails alone). */
 int xkrt_hip_dep_sync(xkrt_hip_dep_domain_t d, uint64_t *nedges);
 
 /* Device pointer to the unconsumed edges, 2 x u32 (pred id, succ id) each,
@@ -169,8 +181,8 @@ int xkrt_hip_dep_stats(xkrt_hip_dep_domain_t d, xkrt_hip_dep_stats_t *out);
 void *xkrt_hip_dep_stream(xkrt_hip_dep_domain_t d);
 
 /* Forgets every access of the domain (all points back to "never accessed"),
- * drops unconsumed edges and clears a poisoned state; the bin grid and the
- * frame stay.  Asynchronous.  The reference equivalent is deleting the
+ * drops unconsumed edges (of a private domain) and clears its failed state
+ * (a private domain: the pool's too); the bin grid and the frame stay.  Asynchronous.  The reference equivalent is deleting the
  * domain in runtime_t::reset (src/memory/invalidate.cc:68-131). */
 int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d);
 
@@ -219,6 +231,9 @@ typedef struct xkrt_hip_pool_config {
     uint32_t small_launch_rects; /* launches of at most this many rects read their records in place from
                                pinned host memory (no upload) and start at the second LDS tier (one
                                resolve kernel less); 0 = never */
+    uint32_t host_edges;    /* 1: the edge buffer lives in pinned host memory that the device writes
+                               (a launch's edges are on the host when it completes, with no copy) */
+    uint32_t pad;
 } xkrt_hip_pool_config_t;
 
 typedef struct xkrt_hip_pool_stats {
@@ -243,7 +258,8 @@ int xkrt_hip_pool_domain_create(xkrt_hip_pool_t p, uint64_t ld, uint64_t sizeof_
                                 uint32_t bin_y_cols, uint64_t y_lo, uint64_t y_hi, xkrt_hip_dep_domain_t *d);
 
 /* Appends a batch of d's accesses (whole tasks, spawn order) to the open
- * launch; ids as xkrt_hip_dep_submit.  -ENOSPC: the open launch is full. */
+ * launch; ids as xkrt_hip_dep_submit.  -ENOSPC: the open launch is full;
+ * -ECANCELED: d failed earlier (xkrt_hip_dep_sync(d) gives its error). */
 int xkrt_hip_pool_stage(xkrt_hip_pool_t p, xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n,
                         uint64_t *first_id);
 /* Resolves every staged batch in one launch (asynchronous).  *ticket (may
@@ -255,9 +271,12 @@ int xkrt_hip_pool_poll(xkrt_hip_pool_t p, uint64_t *done);
 int xkrt_hip_pool_wait(xkrt_hip_pool_t p, uint64_t ticket);
 /* Staged but not launched: records, accesses, batches. */
 int xkrt_hip_pool_open(xkrt_hip_pool_t p, uint64_t *nrect, uint64_t *naccess, uint64_t *nbatches);
+/* Waits for every launch; returns the pool's error only (a domain that
+ * failed alone is reported by xkrt_hip_dep_sync on it). */
 int xkrt_hip_pool_sync(xkrt_hip_pool_t p, uint64_t *nedges);
 /* Copies and consumes every unconsumed edge: edges = (pred id, succ id)
- * pairs, doms = the domain index of each. */
+ * pairs, doms = the domain index of each.  Edges of failed domains are
+ * dropped (*n counts the rest). */
 int xkrt_hip_pool_edges_copy(xkrt_hip_pool_t p, uint32_t *edges, uint32_t *doms, uint64_t cap, uint64_t *n);
 int xkrt_hip_pool_stats(xkrt_hip_pool_t p, xkrt_hip_pool_stats_t *out);
 int xkrt_hip_pool_timing(xkrt_hip_pool_t p, int enable);
@@ -328,8 +347,10 @@ int xkrt_hip_server_close(xkrt_hip_server_t s, uint32_t client);
  * closed client has left; -ETIMEDOUT after timeout_us (0 = no limit). */
 int xkrt_hip_server_drain(xkrt_hip_server_t s, uint64_t timeout_us);
 /* Per request first .. first + n - 1: microseconds since the engine started
- * at submit and at ready (the engine saw its launch complete), and status
- * (1 pending, 0 resolved, negative errno: refused). */
+ * at submit and at ready (the launch that resolved it completed and its
+ * edges are in its client's host buffer), and status (1 pending, 0 resolved,
+ * negative errno: refused; a client whose domain fails on the device gets
+ * its error, e.g. -ENOSPC, on that batch and every later one, alone). */
 int xkrt_hip_server_times(xkrt_hip_server_t s, uint64_t first, uint64_t n, double *submit_us, double *ready_us,
                           int32_t *status);
 /* The client's edges so far: (pred id, succ id) pairs in its own access ids;

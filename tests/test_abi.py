@@ -41,7 +41,7 @@ def test_ctypes_layouts_match_header():
     assert C.sizeof(R.ConfigT) == 4 * 4 + 5 * 8
     assert C.sizeof(R.StatsT) == 11 * 8
     assert R.RECT_DTYPE.itemsize == 32
-    assert C.sizeof(R.PoolConfigT) == 2 * 4 + 3 * 8 + 2 * 4
+    assert C.sizeof(R.PoolConfigT) == 2 * 4 + 3 * 8 + 4 * 4
     assert C.sizeof(R.PoolStatsT) == 8 * 8
     assert C.sizeof(R.ServerConfigT) == 2 * 4 + 2 * 8 + 2 * 4
     assert C.sizeof(R.ServerStatsT) == 9 * 8

@@ -29,3 +29,18 @@ def test_autotune_candidates_and_capacity():
     worst = max(cnt.values())
     want = next(k for k, cap in enumerate(A.KCLASS_RECORDS) if 2 * worst <= cap) if 2 * worst <= 1024 else 2
     assert A.capacity_class(st, 1024, 128, 64) == want
+
+
+def test_spread_windows_cover_the_stream():
+    # windows of whole tasks at the stream's start, end and evenly between, tasks renumbered
+    st = S.heat(1024, 128, 4, 20)
+    starts = st.task_starts()
+    w = A.spread_windows(st, nwin=4, win_tasks=100)
+    assert w.ntasks == 400 and (np.diff(w.task.astype(np.int64)) >= 0).all()
+    ntask = len(starts) - 1
+    for k in range(4):
+        t0 = (ntask - 100) * k // 3
+        a, b = int(starts[t0]), int(starts[t0 + 100])
+        got = w.take(slice(*[int(x) for x in w.task_starts()[[100 * k, 100 * (k + 1)]]]))
+        assert np.array_equal(got.addr, st.addr[a:b]) and np.array_equal(got.mode, st.mode[a:b])
+    assert A.spread_windows(st, nwin=4, win_tasks=ntask) is st  # a short stream is its own sample

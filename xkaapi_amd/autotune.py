@@ -10,8 +10,16 @@ tier, or -ENOSPC).  The right point depends on the stream, so it is measured:
      bytes, BY near the median tile width in columns, and their neighbours
      (halved and doubled in each dimension);
   2. the capacity class from the densest bin of a launch-sized batch;
-  3. each candidate resolves the sample on the device (plain form, HIP-event
-     timed); failures (a capacity overflow) disqualify it; the fastest wins.
+  3. each candidate resolves the sample on the device in launches of the size
+     the stream will use (the plain form's batches, or the batched form's
+     merged launches), timed by HIP events on the device (host record
+     building is not counted: with large launches it would dominate), the
+     median of a few passes; failures (a capacity overflow) disqualify it;
+     the candidates come back ranked.
+
+The sample is a set of windows spread across the stream (spread_windows), not
+its prefix: a prefix can misstate what later launches cost.  The caller can
+confirm the finalists on the whole stream (the benchmark does).
 
 The batch size per launch of the batched form is chosen the same way from a
 list of candidates.
@@ -46,6 +54,7 @@ class TuneResult:
     kernel: int
     us_per_access: float
     tried: list
+    ranked: list = None   # (BX, BY, class, us per access) of every configuration that resolved, best first
 
 
 def candidates(stream, max_cands: int = 7):
@@ -94,23 +103,46 @@ def capacity_class(stream, bx: int, by: int, tasks_per_batch: int) -> int:
     return 2
 
 
-def measure(stream, bx: int, by: int, kernel: int, tasks_per_batch: int, reps: int = 2) -> float:
-    """Seconds per access to resolve the sample, plain form, on the device."""
+def spread_windows(stream, nwin: int = 4, win_tasks: int = 50_000):
+    """nwin runs of win_tasks whole tasks at evenly spaced points of the stream
+    (its start, its end and between), concatenated with tasks renumbered: a
+    sample whose later windows meet the state the earlier ones left, as the
+    stream's later launches do."""
+    st = stream.task_starts()
+    ntask = len(st) - 1
+    if nwin * win_tasks >= ntask:
+        return stream
+    parts = []
+    for k in range(nwin):
+        t0 = (ntask - win_tasks) * k // max(nwin - 1, 1)
+        parts.append(stream.take(slice(int(st[t0]), int(st[t0 + win_tasks]))))
+    from . import streams as S
+    return S.concat(parts)
+
+
+def measure(stream, bx: int, by: int, kernel: int, tasks_per_batch: int, reps: int = 3,
+            launch_tasks: int | None = None) -> float:
+    """Device seconds per access to resolve the sample in launches of
+    launch_tasks tasks (default: tasks_per_batch, the plain form), the median
+    of reps passes; each launch is timed from its first kernel's start to its
+    last kernel's end."""
     acc = R.to_access_array(stream.task, stream.mode, stream.addr, stream.m, stream.n)
-    batches = R.batches_by_tasks(stream.task, tasks_per_batch)
-    best = float("inf")
+    batches = R.batches_by_tasks(stream.task, launch_tasks or tasks_per_batch)
+    times = []
     with R.DependencyDomain(int(stream.ld[0]), int(stream.s[0]), bin_x_bytes=bx, bin_y_cols=by, kernel=kernel,
-                            max_rects=2 * max(b - a for a, b in batches) + 16) as dom:
+                            max_rects=3 * max(b - a for a, b in batches) + 16) as dom:
         for _ in range(reps):
             dom.reset()
             dom.sync()
-            t0 = time.perf_counter()
+            dom.launch_times(clear=True)
+            dom.set_timing(True)
             for a, b in batches:
                 dom.submit(acc[a:b])
             dom.sync()
-            best = min(best, time.perf_counter() - t0)
+            dom.set_timing(False)
+            times.append(float(dom.launch_times(clear=True).sum()) * 1e-3)
             dom.consume()
-    return best / max(len(stream), 1)
+    return float(np.median(times)) / max(len(stream), 1)
 
 
 # A capacity class above the one the sample needs holds more HBM per bin, and the sample
@@ -120,25 +152,32 @@ def measure(stream, bx: int, by: int, kernel: int, tasks_per_batch: int, reps: i
 CLASS_MARGIN = 0.05
 
 
-def tune(sample, tasks_per_batch: int = 4096, max_cands: int = 7) -> TuneResult:
-    """Pick (BX, BY, class) for streams like `sample` (a prefix of the stream)."""
+def tune(sample, tasks_per_batch: int = 4096, max_cands: int = 7, launch_tasks: int | None = None,
+         reps: int = 3) -> TuneResult:
+    """Pick (BX, BY, class) for streams like `sample` (windows of the stream,
+    see spread_windows), timed in launches of launch_tasks tasks (the batched
+    form) or of tasks_per_batch (the plain form).  `ranked` lists every
+    configuration that resolved the sample, best score first."""
     tried = []
-    best = None
+    scored = []
+    lt = launch_tasks or tasks_per_batch
     for bx, by in candidates(sample, max_cands):
-        k = capacity_class(sample, bx, by, tasks_per_batch)
+        k = capacity_class(sample, bx, by, lt)
         for kc in sorted({k, min(k + 1, 2)}):
             try:
-                t = measure(sample, bx, by, kc, tasks_per_batch)
+                t = measure(sample, bx, by, kc, tasks_per_batch, reps=reps, launch_tasks=launch_tasks)
             except R.ResolverError as e:
                 tried.append((bx, by, kc, None, str(e)))
                 continue
             tried.append((bx, by, kc, t * 1e6, ""))
-            score = t * (1.0 + CLASS_MARGIN * (kc - k))
-            if best is None or score < best[4]:
-                best = (bx, by, kc, t, score)
-    if best is None:
+            scored.append((t * (1.0 + CLASS_MARGIN * (kc - k)), bx, by, kc, t))
+    if not scored:
         raise RuntimeError(f"no bin configuration resolves the sample: {tried}")
-    return TuneResult(best[0], best[1], best[2], best[3] * 1e6, tried)
+    scored.sort()
+    _, bx, by, kc, t = scored[0]
+    res = TuneResult(bx, by, kc, t * 1e6, tried)
+    res.ranked = [(b, c, k, tt * 1e6) for _, b, c, k, tt in scored]
+    return res
 
 
 def tune_merge(pool_factory, run, merges=(1, 4, 16, 64)) -> int:

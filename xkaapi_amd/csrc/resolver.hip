@@ -66,15 +66,21 @@
 #define BLKSLOTS 4096u // per-block statistics slots per tier (4 tiers)
 #define EMPTY64 0xFFFFFFFFFFFFFFFFull
 
+// Error bits.  The first group is per domain (a word per domain slot, sticky
+// until the domain is reset or its slot is reused): a domain that fails stops
+// touching state, the other domains of its pool go on.  The second group is
+// per pool: the launch's shared scratch overflowed, so every later kernel of
+// the pool stops.
 enum : uint32_t {
-    ERR_BINREC = 1u,    // incidence array overflow
     ERR_LIST = 2u,      // a bin got more records than the kernel's LDS list holds
     ERR_FRAG = 4u,      // a bin's fragment list overflowed
     ERR_READS = 8u,     // a bin's reader pool overflowed
+    ERR_RANGE = 128u,   // a record outside its domain's grid (or naming no domain)
     ERR_CAND = 16u,     // candidate buffer overflow (staged form)
     ERR_HASH = 32u,     // dedup table full
     ERR_EDGES = 64u,    // edge buffer overflow
 };
+constexpr uint32_t ERR_DOMAIN_BITS = ERR_LIST | ERR_FRAG | ERR_READS | ERR_RANGE;
 
 struct DevCounters {
     unsigned int nrec;       // rect records of this launch (written by k_count)
@@ -92,13 +98,16 @@ struct DevCounters {
     unsigned long long nlaunch;      // launches completed (k_final counts them)
 };
 
+constexpr uint32_t MAX_DOMS = 256;
+
 // What the host learns of a launch without a stream sync: k_final writes it to
 // pinned host memory, slot = launch number % NSNAP.
 struct LaunchSnap {
     unsigned long long launch;      // launch number (1, 2, ...)
     unsigned long long edge_total;  // unconsumed edges after it
-    unsigned int err;               // sticky error bits after it
+    unsigned int err;               // sticky pool error bits after it
     unsigned int pad[3];
+    unsigned int derr[MAX_DOMS / 4];  // per domain slot, 8 error bits (slot d: word d / 4, byte d % 4)
 };
 constexpr uint32_t NSNAP = 64;
 
@@ -143,6 +152,7 @@ struct KArgs {
     uint32_t *edom;           // domain of each edge
     unsigned long long edge_cap;
     DevCounters *ctr;
+    uint32_t *dom_err;        // MAX_DOMS words: per-domain error bits
     LaunchSnap *snap;         // pinned host ring, NSNAP slots
 };
 
@@ -173,18 +183,33 @@ __device__ __forceinline__ uint64_t mix64(uint64_t k)
 
 // ---------------------------------------------------------------- binning
 
-// the bins of record r: [bx0, bx1] x [by0, by1] of its domain's grid
-__device__ __forceinline__ DomDesc rec_bins(const KArgs &a, uint32_t r, uint4 &q0, uint4 &q1, uint32_t &bx0,
-                                            uint32_t &bx1, uint32_t &by0, uint32_t &by1)
+// The bins of record r: [bx0, bx1] x [by0, by1] of its domain's grid.  False
+// when the record names no domain slot, is empty, or leaves its domain's grid
+// (records may come from the caller's device memory): its domain is marked
+// ERR_RANGE and the record is dropped, so its domain touches no state in this
+// launch (k_resolve skips the bins of failed domains).
+__device__ __forceinline__ bool rec_bins(const KArgs &a, uint32_t r, uint4 &q0, uint4 &q1, DomDesc &g,
+                                         uint32_t &bx0, uint32_t &bx1, uint32_t &by0, uint32_t &by1)
 {
     q0 = a.rec[2ull * r];
     q1 = a.rec[2ull * r + 1];
-    const DomDesc g = a.dom[q1.z];
+    if (q1.z >= MAX_DOMS) {
+        atomicOr(&a.ctr->err, ERR_RANGE);  // no domain to charge: the pool's launch is unsound
+        return false;
+    }
+    g = a.dom[q1.z];
+    const bool ok = g.nbx != 0 && q0.x < q0.y && q0.y <= g.xmax && q0.z < q0.w && q0.z >= g.ybin0 &&
+                    ((q0.w - 1 - g.ybin0) >> g.bys) < g.nby;
+    if (!ok) {
+        atomicOr(&a.dom_err[q1.z], ERR_RANGE);
+        return false;
+    }
+    if (a.dom_err[q1.z]) return false;  // a failed domain: its records are not counted
     bx0 = q0.x >> g.bxs;
     bx1 = (q0.y - 1) >> g.bxs;
     by0 = (q0.z - g.ybin0) >> g.bys;
     by1 = (q0.w - 1 - g.ybin0) >> g.bys;
-    return g;
+    return true;
 }
 
 // The incidences (record x bin) of 64 consecutive records, spread over the
@@ -206,10 +231,12 @@ __device__ __forceinline__ WaveRecs wave_recs(const KArgs &a, uint32_t r, uint32
     q.bx0 = q.w = q.by0 = q.n = q.bin0 = q.nbx = 0;
     if (q.valid) {
         uint4 q0, q1;
+        DomDesc g;
         uint32_t bx0, bx1, by0, by1;
-        const DomDesc g = rec_bins(a, r, q0, q1, bx0, bx1, by0, by1);
-        q.bx0 = bx0; q.w = bx1 - bx0 + 1; q.by0 = by0; q.n = q.w * (by1 - by0 + 1);
-        q.bin0 = g.bin0; q.nbx = g.nbx;
+        if (rec_bins(a, r, q0, q1, g, bx0, bx1, by0, by1)) {
+            q.bx0 = bx0; q.w = bx1 - bx0 + 1; q.by0 = by0; q.n = q.w * (by1 - by0 + 1);
+            q.bin0 = g.bin0; q.nbx = g.nbx;
+        }
     }
     q.incl = wave_incl_scan(q.n);
     return q;
@@ -840,7 +867,7 @@ __device__ __forceinline__ void escalate(const KArgs &a, uint32_t tier, uint4 e,
         const uint32_t pos = atomicAdd(&a.ctr->ntier[next], 1u);
         a.active[(size_t)next * a.nbins + pos] = e;
     } else {
-        atomicOr(&a.ctr->err, err);
+        atomicOr(&a.dom_err[e.w >> 24], err);  // the bin's domain fails; the pool's other domains go on
         a.bin_cnt[e.x] = 0;
     }
 }
@@ -877,7 +904,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     extern __shared__ __align__(16) unsigned char lds_raw[];
     BinLds<FCAP, GCAP, LCAP, DIR> &s = *reinterpret_cast<BinLds<FCAP, GCAP, LCAP, DIR> *>(lds_raw);
     const uint32_t lane = lane_id();
-    if (a.ctr->err) return;  // a poisoned domain touches no state
+    if (a.ctr->err) return;  // a poisoned pool touches no state
     // a launch's first tier takes every touched bin straight from the touched list
     // (no k_alloc); later tiers take their active list (bins escalated from below)
     const bool touched = tier == a.min_tier;
@@ -928,6 +955,12 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
         };
         const uint32_t slot = e.x, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
         const size_t start = (size_t)slot * a.lmax;  // the bin's incidence bucket
+        // a failed domain touches no state: its bins are dropped (their counts cleared)
+        if (a.dom_err[e.w >> 24]) {
+            advance();
+            if (lane == 0) a.bin_cnt[slot] = 0;
+            continue;
+        }
         // a bin goes on to the next tier when it does not fit this one, or when its map or
         // log fill more than half of this tier's room (they grow during the replay)
         if (cnt > (uint32_t)LCAP || nf0 > (uint32_t)FCAP || ng0 > (uint32_t)GCAP ||
@@ -1274,14 +1307,24 @@ __global__ void __launch_bounds__(256) k_dedup(KArgs a)
 __global__ void k_final(KArgs a)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid == 0) {
-        a.ctr->ntouched = 0;  // the touched list of the next launch starts empty
-        // the launch's outcome to pinned host memory (seen once the launch's completion is)
-        const unsigned long long seq = ++a.ctr->nlaunch;
+    if (blockIdx.x == 0) {
+        // the launch's outcome to pinned host memory (seen once the launch's completion is):
+        // the per-domain error bits first, four domain slots per word, then the counters
+        const unsigned long long seq = a.ctr->nlaunch + 1;
         LaunchSnap *o = a.snap + seq % NSNAP;
-        __hip_atomic_store(&o->edge_total, a.ctr->edge_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __hip_atomic_store(&o->err, a.ctr->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __hip_atomic_store(&o->launch, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (threadIdx.x < MAX_DOMS / 4) {
+            const uint32_t *d = a.dom_err + 4 * threadIdx.x;
+            const uint32_t w = (d[0] & 0xffu) | (d[1] & 0xffu) << 8 | (d[2] & 0xffu) << 16 | (d[3] & 0xffu) << 24;
+            __hip_atomic_store(&o->derr[threadIdx.x], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            a.ctr->ntouched = 0;  // the touched list of the next launch starts empty
+            a.ctr->nlaunch = seq;
+            __hip_atomic_store(&o->edge_total, a.ctr->edge_total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&o->err, a.ctr->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&o->launch, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
     if (a.ctr->err & (ERR_EDGES | ERR_HASH)) return;  // eslot is incomplete
     const unsigned long long ebase = a.ctr->edge_base;
@@ -1428,7 +1471,6 @@ template <int F, int R, int L>
 size_t lds_bytes() { return sizeof(BinLds<F, R, L, false>); }  // the same for both forms
 static_assert(sizeof(BinLds<1024, 2048, 1024, false>) <= 160u * 1024u, "largest tier exceeds the 160 KiB LDS of a CU");
 
-constexpr uint32_t MAX_DOMS = 256;
 constexpr uint32_t NDRING = 4;
 
 int hip_err(hipError_t e)
@@ -1480,10 +1522,13 @@ struct xkrt_hip_pool {
     uint32_t *hvals = nullptr;
     uint32_t hmask = 0;
     uint32_t *eslot = nullptr;
-    uint32_t *edges = nullptr, *edom = nullptr;
+    uint32_t *edges = nullptr, *edom = nullptr;  // device addresses (of host memory with host_edges)
+    uint32_t *edges_h = nullptr, *edom_h = nullptr;  // host_edges: the same buffers' host addresses
     uint64_t edge_cap = 0;
     DevCounters *ctr = nullptr;
     DevCounters *ctr_host = nullptr;  // pinned
+    uint32_t *dom_err = nullptr;       // device: MAX_DOMS per-domain error words
+    uint32_t *dom_err_host = nullptr;  // pinned copy (read_counters)
     // staging of the open launch: pinned host records + the record count
     // (one upload per launch: a 32 B header holding the record count, then the
     // records; two pinned host buffers in turn, so staging the next launch does
@@ -1516,10 +1561,12 @@ struct xkrt_hip_pool {
     std::deque<std::pair<uint64_t, hipEvent_t>> tickets;
     std::vector<hipEvent_t> ticket_pool;
     uint64_t done = 0;
-    // captured launch
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    bool graph_valid = false;
+    // captured launches, one per launch-size class (the launch settings follow the record
+    // count: tier-0 blocks per slot below / from 64K records, the tier-1 kernel below 128K)
+    static constexpr int NGRAPH = 3;
+    hipGraph_t graph[NGRAPH] = {};
+    hipGraphExec_t gexec[NGRAPH] = {};
+    bool graph_valid[NGRAPH] = {};
     // accounting
     bool poisoned = false;
     uint64_t n_launch = 0;
@@ -1549,6 +1596,7 @@ struct xkrt_hip_dep_domain {
     uint64_t max_xlen = 0;
     bool col0 = false;  // some rect started at absolute column 0 (addresses below ld * sizeof_type)
     uint64_t n_acc = 0, n_rect = 0;
+    uint32_t err = 0;   // device error bits of this domain seen by the host (sticky until reset)
 };
 
 namespace {
@@ -1595,11 +1643,23 @@ void harvest_events(xkrt_hip_pool *p)
 
 void drop_graph(xkrt_hip_pool *p)
 {
-    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
-    if (p->graph) (void)hipGraphDestroy(p->graph);
-    p->gexec = nullptr;
-    p->graph = nullptr;
-    p->graph_valid = false;
+    for (int k = 0; k < xkrt_hip_pool::NGRAPH; ++k) {
+        if (p->gexec[k]) (void)hipGraphExecDestroy(p->gexec[k]);
+        if (p->graph[k]) (void)hipGraphDestroy(p->graph[k]);
+        p->gexec[k] = nullptr;
+        p->graph[k] = nullptr;
+        p->graph_valid[k] = false;
+    }
+}
+
+// launch-size class of a graphed launch of nrec records, and the record count its
+// graph is captured for (the largest of the class: k_count's grid strides over fewer)
+int graph_class(const xkrt_hip_pool *p, uint64_t nrec, uint64_t *capture_nrec)
+{
+    const uint64_t lim[2] = {1ull << 16, 1ull << 17};
+    int k = nrec < lim[0] ? 0 : (nrec < lim[1] ? 1 : 2);
+    *capture_nrec = std::min<uint64_t>(k < 2 ? lim[k] - 1 : p->rec_cap, p->rec_cap);
+    return k;
 }
 
 // grow the bin arrays to at least `need` bins (stream synchronised, contents kept)
@@ -1765,6 +1825,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.hkeys = p->hkeys; a.hvals = p->hvals; a.hmask = p->hmask;
     a.eslot = p->eslot; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
     a.ctr = p->ctr;
+    a.dom_err = p->dom_err;
     a.snap = p->snap;
     return a;
 }
@@ -1892,17 +1953,22 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
     if (rc) return rc;
     const bool graphed = p->cfg.graph && rec == p->rec_dev && counted;
     if (graphed) {
-        if (!p->graph_valid) {
-            drop_graph(p);
+        uint64_t cap_nrec = 0;
+        const int gk = graph_class(p, nrec, &cap_nrec);
+        if (!p->graph_valid[gk]) {
+            if (p->gexec[gk]) (void)hipGraphExecDestroy(p->gexec[gk]);
+            if (p->graph[gk]) (void)hipGraphDestroy(p->graph[gk]);
+            p->gexec[gk] = nullptr;
+            p->graph[gk] = nullptr;
             const KArgs a = make_args(p, p->rec_dev, NONE);
             HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-            launch_kernels(p, a, p->rec_cap, nullptr);
+            launch_kernels(p, a, cap_nrec, nullptr);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(p->stream, &g);
             if (e != hipSuccess) return hip_err(e);
-            p->graph = g;
-            HIPCHK(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
-            p->graph_valid = true;
+            p->graph[gk] = g;
+            HIPCHK(hipGraphInstantiate(&p->gexec[gk], p->graph[gk], nullptr, nullptr, 0));
+            p->graph_valid[gk] = true;
         }
         hipEvent_t ev[2] = {};
         if (p->timing) {
@@ -1910,7 +1976,7 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
             ev[1] = take_event(p);
         }
         if (ev[0]) (void)hipEventRecord(ev[0], p->stream);
-        HIPCHK(hipGraphLaunch(p->gexec, p->stream));
+        HIPCHK(hipGraphLaunch(p->gexec[gk], p->stream));
         if (ev[1]) (void)hipEventRecord(ev[1], p->stream);
         if (ev[0] && ev[1]) {
             p->ev_pending.push_back({{-1, ev[0], ev[1]}});
@@ -1965,21 +2031,44 @@ int retire(xkrt_hip_pool *p, uint64_t until)
 
 int err_to_errno(uint32_t e)
 {
-    if (e & (ERR_LIST | ERR_FRAG | ERR_READS | ERR_BINREC)) return -ENOSPC;
+    if (e & ERR_RANGE) return -ERANGE;
+    if (e & (ERR_LIST | ERR_FRAG | ERR_READS)) return -ENOSPC;
     if (e & (ERR_CAND | ERR_HASH | ERR_EDGES)) return -EOVERFLOW;
     return 0;
 }
 
+// the device error bits of each admitted domain into its host record (sticky)
+void fold_domain_errors(xkrt_hip_pool *p, const uint32_t *derr)
+{
+    for (uint32_t i = 0; i < MAX_DOMS; ++i)
+        if (p->doms[i] && derr[i]) p->doms[i]->err |= derr[i] & ERR_DOMAIN_BITS;
+}
+
+// Synchronises the pool's stream and reads its counters and the per-domain
+// error words.  Returns a pool-level error (the pool is poisoned); a domain's
+// own failure is kept in its record and reported by the calls on that domain.
 int read_counters(xkrt_hip_pool *p)
 {
     HIPCHK(hipMemcpyAsync(p->ctr_host, p->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipMemcpyAsync(p->dom_err_host, p->dom_err, 4ull * MAX_DOMS, hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
     harvest_events(p);
+    fold_domain_errors(p, p->dom_err_host);
     if (p->ctr_host->err) {
         p->poisoned = true;
         return err_to_errno(p->ctr_host->err);
     }
     return 0;
+}
+
+// the domain's own failure as a negative errno (0: healthy)
+int domain_error(const xkrt_hip_dep_domain *d) { return err_to_errno(d->err); }
+
+// clears the device error word of domain slot idx (stream ordered)
+int clear_domain_error(xkrt_hip_pool *p, uint32_t idx)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, p->stream, p->dom_err + idx, 1u, 0u);
+    return hip_err(hipGetLastError());
 }
 
 void destroy_pool(xkrt_hip_pool *p)
@@ -1995,13 +2084,21 @@ void destroy_pool(xkrt_hip_pool *p)
         }
     for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
     void *dev[] = {p->bin_nf, p->bin_nrs, p->frag, p->rs, p->bin_cnt, p->bin_dom, p->active, p->touched,
-                   p->d_dom,
-                   p->binrec, p->cand, p->cstr, p->blk, p->hkeys, p->hvals, p->eslot, p->edges, p->edom, p->ctr,
+                   p->d_dom, p->dom_err,
+                   p->binrec, p->cand, p->cstr, p->blk, p->hkeys, p->hvals, p->eslot, p->ctr,
                    p->gate_count,
                    p->rec_dev_hdr};
     for (void *q : dev)
         if (q) (void)hipFree(q);
+    if (p->edges_h || p->edom_h) {
+        if (p->edges_h) (void)hipHostFree(p->edges_h);
+        if (p->edom_h) (void)hipHostFree(p->edom_h);
+    } else {
+        if (p->edges) (void)hipFree(p->edges);
+        if (p->edom) (void)hipFree(p->edom);
+    }
     if (p->ctr_host) (void)hipHostFree(p->ctr_host);
+    if (p->dom_err_host) (void)hipHostFree(p->dom_err_host);
     for (int b = 0; b < 2; ++b) {
         if (p->rec_host_hdr[b]) (void)hipHostFree(p->rec_host_hdr[b]);
         if (p->staged[b]) (void)hipEventDestroy(p->staged[b]);
@@ -2051,7 +2148,8 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     p->edge_cap = p->cfg.max_edges ? p->cfg.max_edges : std::max<uint64_t>(16 * max_rects, 1 << 20);
     p->cand_cap = p->cfg.direct_dedup ? 0u : (uint32_t)(cand_total / CSTRIPES);
     p->bytes_scratch = 12ull * CSTRIPES * p->cand_cap + 12ull * hsz + 4ull * hsz +
-                       12ull * p->edge_cap + 32ull * (p->rec_cap + 1) + sizeof(DevCounters) + sizeof(p->h_dom);
+                       (p->cfg.host_edges ? 0ull : 12ull * p->edge_cap) + 32ull * (p->rec_cap + 1) +
+                       sizeof(DevCounters) + sizeof(p->h_dom);
     int rc = 0;
     auto chk = [&](hipError_t e) { if (!rc) rc = hip_err(e); };
     if (p->cfg.budget_bytes && p->bytes_scratch > p->cfg.budget_bytes) rc = -ENOMEM;
@@ -2061,9 +2159,20 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     chk(hipMalloc(&p->hkeys, 8ull * hsz));
     chk(hipMalloc(&p->hvals, 4ull * hsz));
     chk(hipMalloc(&p->eslot, 4ull * hsz));
-    chk(hipMalloc(&p->edges, 8ull * p->edge_cap));
-    chk(hipMalloc(&p->edom, 4ull * p->edge_cap));
+    if (p->cfg.host_edges) {
+        // k_final's edge stores land in pinned host memory: a launch's edges are on the host
+        // when the launch completes, with no copy
+        chk(hipHostMalloc(&p->edges_h, 8ull * p->edge_cap, hipHostMallocCoherent));
+        chk(hipHostMalloc(&p->edom_h, 4ull * p->edge_cap, hipHostMallocCoherent));
+        if (p->edges_h) chk(hipHostGetDevicePointer((void **)&p->edges, p->edges_h, 0));
+        if (p->edom_h) chk(hipHostGetDevicePointer((void **)&p->edom, p->edom_h, 0));
+    } else {
+        chk(hipMalloc(&p->edges, 8ull * p->edge_cap));
+        chk(hipMalloc(&p->edom, 4ull * p->edge_cap));
+    }
     chk(hipMalloc(&p->ctr, sizeof(DevCounters)));
+    chk(hipMalloc(&p->dom_err, 4ull * MAX_DOMS));
+    chk(hipHostMalloc(&p->dom_err_host, 4ull * MAX_DOMS, hipHostMallocDefault));
     chk(hipMalloc(&p->gate_count, 64));
     {
         int khz = 0;
@@ -2088,6 +2197,7 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     if (!rc) chk(hipMemsetAsync(p->hkeys, 0xFF, 8ull * hsz, p->stream));
     if (!rc) chk(hipMemsetAsync(p->hvals, 0xFF, 4ull * hsz, p->stream));
     if (!rc) chk(hipMemsetAsync(p->ctr, 0, sizeof(DevCounters), p->stream));
+    if (!rc) chk(hipMemsetAsync(p->dom_err, 0, 4ull * MAX_DOMS, p->stream));
     if (!rc) chk(hipMemsetAsync(p->cstr, 0, 64ull * CSTRIPES, p->stream));
     if (!rc) chk(hipMemsetAsync(p->blk, 0, 16ull * 4 * BLKSLOTS, p->stream));
     chk(set_lds<64, 128, 64>());
@@ -2119,6 +2229,11 @@ int add_domain(xkrt_hip_pool *p, uint64_t ld, uint64_t s, uint32_t BX, uint32_t 
     d->ld = ld; d->s = s; d->LDs = LDs;
     d->bxs = ilog2(BX); d->bys = ilog2(BY);
     d->nbx = (uint32_t)((LDs + BX - 1) / BX);
+    const int rc = clear_domain_error(p, idx);  // the slot may have held a failed domain
+    if (rc) {
+        delete d;
+        return rc;
+    }
     p->doms[idx] = d;
     p->ndoms++;
     publish(p, d);
@@ -2232,6 +2347,7 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
 int stage(xkrt_hip_pool *p, xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t n, uint64_t *first_id)
 {
     if (p->poisoned) return -EINVAL;
+    if (d->err) return -ECANCELED;  // a failed domain takes no batch until it is reset
     if (first_id) *first_id = d->next_id;
     if (n == 0) return 0;
     if (n > 0xFFFFFFFFull || d->next_id + n > 0xFFFFFFFFull) return -ERANGE;
@@ -2497,9 +2613,28 @@ int xkrt_hip_pool_edges_copy(xkrt_hip_pool_t p, uint32_t *edges, uint32_t *doms,
     if (rc) return rc;
     const uint64_t m = std::min(ne, cap);
     if (m) {
-        HIPCHK(hipMemcpyAsync(edges, p->edges, 8 * m, hipMemcpyDeviceToHost, p->stream));
-        HIPCHK(hipMemcpyAsync(doms, p->edom, 4 * m, hipMemcpyDeviceToHost, p->stream));
-        HIPCHK(hipStreamSynchronize(p->stream));
+        if (p->edges_h) {  // already in host memory (the stream is synchronised)
+            memcpy(edges, p->edges_h, 8 * m);
+            memcpy(doms, p->edom_h, 4 * m);
+        } else {
+            HIPCHK(hipMemcpyAsync(edges, p->edges, 8 * m, hipMemcpyDeviceToHost, p->stream));
+            HIPCHK(hipMemcpyAsync(doms, p->edom, 4 * m, hipMemcpyDeviceToHost, p->stream));
+            HIPCHK(hipStreamSynchronize(p->stream));
+        }
+        // a failed domain's edges are not its reference edges: they are dropped
+        bool any = false;
+        for (uint32_t i = 0; i < MAX_DOMS; ++i) any = any || (p->doms[i] && p->doms[i]->err);
+        if (any) {
+            uint64_t k = 0;
+            for (uint64_t i = 0; i < m; ++i) {
+                const xkrt_hip_dep_domain *d = p->doms[doms[i] & (MAX_DOMS - 1)];
+                if (d && d->err) continue;
+                edges[2 * k] = edges[2 * i];
+                edges[2 * k + 1] = edges[2 * i + 1];
+                doms[k++] = doms[i];
+            }
+            *n = k + (ne - m);
+        }
     }
     hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, p->stream, p->ctr);
     return hip_err(hipGetLastError());
@@ -2517,7 +2652,7 @@ int xkrt_hip_pool_stats(xkrt_hip_pool_t p, xkrt_hip_pool_stats_t *out)
     out->device_bytes = (uint64_t)p->cap_bins * bin_bytes(p) + p->bytes_scratch;
     out->bytes_per_bin = bin_bytes(p);
     out->launches = p->n_launch;
-    out->graph = p->graph_valid ? 1 : 0;
+    out->graph = (p->graph_valid[0] || p->graph_valid[1] || p->graph_valid[2]) ? 1 : 0;
     out->poisoned = p->poisoned ? 1 : 0;
     return 0;
 }
@@ -2616,9 +2751,10 @@ int xkrt_hip_dep_build_records(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t 
 int xkrt_hip_dep_submit_device(xkrt_hip_dep_domain_t d, const xkrt_hip_rect_t *rec, uint64_t nrect, uint64_t naccess,
                                uint32_t ylo, uint32_t yhi)
 {
-    if (!d || !rec) return -EINVAL;
+    if (!d || !rec || ylo > yhi) return -EINVAL;
     xkrt_hip_pool *p = d->pool;
     if (p->poisoned) return -EINVAL;
+    if (d->err) return domain_error(d);
     if (nrect == 0) return 0;
     if (nrect > p->cfg.max_rects) return -EINVAL;
     if (!d->framed) d->framed = true;  // device records are already in the frame (ybase 0 unless hinted)
@@ -2636,6 +2772,7 @@ int xkrt_hip_dep_submit(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, u
     if (!d || (!acc && n)) return -EINVAL;
     xkrt_hip_pool *p = d->pool;
     if (p->poisoned) return -EINVAL;
+    if (d->err) return domain_error(d);
     int rc = stage(p, d, acc, n, first_id);
     if (rc == -ENOSPC) {  // the open launch is full: launch it, then stage again
         rc = launch_open(p);
@@ -2648,7 +2785,11 @@ int xkrt_hip_dep_submit(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, u
 int xkrt_hip_dep_sync(xkrt_hip_dep_domain_t d, uint64_t *nedges)
 {
     if (!d) return -EINVAL;
-    return xkrt_hip_pool_sync(d->pool, nedges);
+    xkrt_hip_pool *p = d->pool;
+    if (p->server) return -EBUSY;
+    const int rc = read_counters(p);
+    if (nedges) *nedges = p->ctr_host->edge_total;
+    return rc ? rc : domain_error(d);
 }
 
 int xkrt_hip_dep_edges_device(xkrt_hip_dep_domain_t d, const uint32_t **edges, uint64_t *n)
@@ -2658,7 +2799,7 @@ int xkrt_hip_dep_edges_device(xkrt_hip_dep_domain_t d, const uint32_t **edges, u
     int rc = read_counters(p);
     *edges = p->edges;
     *n = p->ctr_host->edge_total;
-    return rc;
+    return rc ? rc : domain_error(d);
 }
 
 int xkrt_hip_dep_edges_consume(xkrt_hip_dep_domain_t d)
@@ -2676,9 +2817,12 @@ int xkrt_hip_dep_edges_copy(xkrt_hip_dep_domain_t d, uint32_t *host_edges, uint6
     int rc = read_counters(p);
     const uint64_t ne = p->ctr_host->edge_total;
     *n = ne;
+    if (!rc) rc = domain_error(d);
     if (rc) return rc;
     const uint64_t m = std::min(ne, cap);
-    if (m) {
+    if (m && p->edges_h) {
+        memcpy(host_edges, p->edges_h, 8 * m);
+    } else if (m) {
         HIPCHK(hipMemcpyAsync(host_edges, p->edges, 8 * m, hipMemcpyDeviceToHost, p->stream));
         HIPCHK(hipStreamSynchronize(p->stream));
     }
@@ -2710,7 +2854,7 @@ int xkrt_hip_dep_stats(xkrt_hip_dep_domain_t d, xkrt_hip_dep_stats_t *out)
     out->max_reads = maxg;
     out->frag_visits = visits;
     out->device_bytes = (uint64_t)d->nbx * d->nby * bin_bytes(p) + (d->private_pool ? p->bytes_scratch : 0);
-    out->err = c.err;
+    out->err = c.err | d->err;
     return 0;
 }
 
@@ -2731,6 +2875,9 @@ int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d)
     }
     HIPCHK(hipGetLastError());
     if (d->private_pool) p->poisoned = false;
+    const int rc = clear_domain_error(p, d->idx);
+    if (rc) return rc;
+    d->err = 0;
     d->next_id = 0;
     d->max_xlen = 0;
     d->col0 = false;
@@ -2889,6 +3036,8 @@ int srv_admit(xkrt_hip_server *s, SrvClient *c)
     return 0;
 }
 
+int srv_harvest(xkrt_hip_server *s);
+
 int srv_retire(xkrt_hip_server *s, bool &work)
 {
     if (s->flights.empty()) return 0;
@@ -2912,10 +3061,30 @@ int srv_retire(xkrt_hip_server *s, bool &work)
         s->edges_seen = total;
         // past half the edge buffer: no new launches until it is copied out and reset
         if (s->edges_seen > p->edge_cap / 2) s->hold = true;
+        // A client whose domain failed in this launch (a bin outgrew its capacity class, or a
+        // record left its frame) fails alone: its batches of this launch and every later one
+        // are refused with its error, and its edges are dropped; the other clients of the
+        // launch are resolved.
+        auto failed = [&](const SrvClient *c) -> uint32_t {
+            if (!c->dom) return 0u;
+            const uint32_t idx = c->dom->idx;
+            return (__atomic_load_n(&o->derr[idx / 4], __ATOMIC_RELAXED) >> (8 * (idx % 4))) & 0xffu;
+        };
+        for (SrvRequest *r : f.reqs) {
+            SrvClient *c = r->c;
+            const uint32_t bits = failed(c);
+            if (bits && !c->status) {
+                c->dom->err |= bits;
+                c->status = err_to_errno(bits);
+            }
+        }
+        // a batch is ready once its edges are in its client's host buffer
+        rc = srv_harvest(s);
+        if (rc) return rc;
         const double t = srv_us(s->t0);
         for (SrvRequest *r : f.reqs) {
             r->c->need = total;
-            srv_finish(s, r, 0, t);
+            srv_finish(s, r, failed(r->c) ? r->c->status : 0, t);
         }
         s->count(xkrt_hip_server::S_BATCHES, f.reqs.size());
         s->flights.pop_front();
@@ -3028,29 +3197,42 @@ int srv_launch(xkrt_hip_server *s, bool &work)
     return 0;
 }
 
+// The edges of retired launches into their clients' host buffers.  Retired
+// launches never change these edges again; later launches append after them.
 int srv_harvest(xkrt_hip_server *s)
 {
     xkrt_hip_pool *p = s->p;
-    // at most hcap edges per copy: the pinned buffers are allocated once, at creation
-    // (freeing pinned memory may synchronise the device)
-    const uint64_t a = s->harvested, n = std::min(s->edges_seen - a, s->hcap);
-    if (!n) return 0;
-    // retired launches never change these edges again; later launches append after them
-    HIPCHK(hipMemcpyAsync(s->he, p->edges + 2 * a, 8 * n, hipMemcpyDeviceToHost, s->hs));
-    HIPCHK(hipMemcpyAsync(s->hd, p->edom + a, 4 * n, hipMemcpyDeviceToHost, s->hs));
-    HIPCHK(hipStreamSynchronize(s->hs));
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        for (uint64_t i = 0; i < n; ++i) {
-            SrvClient *c = s->owner[s->hd[i] & (MAX_DOMS - 1)];
-            if (!c) return -EIO;
-            const uint32_t f = (uint32_t)c->first_id;
-            c->edges.push_back(s->he[2 * i] - f);
-            c->edges.push_back(s->he[2 * i + 1] - f);
+    while (s->harvested < s->edges_seen) {
+        const uint64_t a = s->harvested;
+        uint64_t n = s->edges_seen - a;
+        const uint32_t *ee, *ed;
+        if (p->edges_h) {  // k_final wrote them to host memory before its launch completed
+            ee = p->edges_h + 2 * a;
+            ed = p->edom_h + a;
+        } else {
+            // at most hcap edges per copy: the pinned buffers are allocated once, at creation
+            // (freeing pinned memory may synchronise the device)
+            n = std::min(n, s->hcap);
+            HIPCHK(hipMemcpyAsync(s->he, p->edges + 2 * a, 8 * n, hipMemcpyDeviceToHost, s->hs));
+            HIPCHK(hipMemcpyAsync(s->hd, p->edom + a, 4 * n, hipMemcpyDeviceToHost, s->hs));
+            HIPCHK(hipStreamSynchronize(s->hs));
+            ee = s->he;
+            ed = s->hd;
         }
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            for (uint64_t i = 0; i < n; ++i) {
+                SrvClient *c = s->owner[ed[i] & (MAX_DOMS - 1)];
+                if (!c) return -EIO;
+                if (c->status) continue;  // a failed client's edges are not its reference edges
+                const uint32_t f = (uint32_t)c->first_id;
+                c->edges.push_back(ee[2 * i] - f);
+                c->edges.push_back(ee[2 * i + 1] - f);
+            }
+        }
+        s->harvested = a + n;
+        s->count(xkrt_hip_server::S_HARVESTS);
     }
-    s->harvested = a + n;
-    s->count(xkrt_hip_server::S_HARVESTS);
     return 0;
 }
 

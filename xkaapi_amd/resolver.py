@@ -51,7 +51,7 @@ assert RECT_DTYPE.itemsize == 32
 class PoolConfigT(C.Structure):
     _fields_ = [("kernel", C.c_uint32), ("graph", C.c_uint32), ("max_rects", C.c_uint64),
                 ("max_edges", C.c_uint64), ("budget_bytes", C.c_uint64), ("direct_dedup", C.c_uint32),
-                ("small_launch_rects", C.c_uint32)]
+                ("small_launch_rects", C.c_uint32), ("host_edges", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class PoolStatsT(C.Structure):
@@ -69,7 +69,9 @@ class ServerStatsT(C.Structure):
                                           "consumes", "gated")] + [("error", C.c_int64)]
 
 
-ERR_BITS = {1: "incidence array", 2: "bin record list", 4: "bin fragments", 8: "bin reader log",
+# device error bits: 2, 4, 8 and 128 belong to one domain (the pool's other domains go on);
+# 16, 32 and 64 to the pool (shared launch scratch overflowed)
+ERR_BITS = {2: "bin record list", 4: "bin fragments", 8: "bin reader log", 128: "record outside its frame",
             16: "edge candidates", 32: "dedup table", 64: "edge buffer"}
 
 
@@ -323,11 +325,11 @@ class Pool:
 
     def __init__(self, kernel: int = 1, graph: bool = False, max_rects: int = 1 << 20, max_edges: int = 0,
                  budget_bytes: int = 0, stream: int | None = None, device: int = 0, direct_dedup: bool = False,
-                 small_launch_rects: int = 0):
+                 small_launch_rects: int = 0, host_edges: bool = False):
         self.L = hip_lib()
         _chk(self.L.xkrt_hip_init(device), "xkrt_hip_init")
         cfg = PoolConfigT(kernel, 1 if graph else 0, max_rects, max_edges, budget_bytes, 1 if direct_dedup else 0,
-                          small_launch_rects)
+                          small_launch_rects, 1 if host_edges else 0, 0)
         h = _p()
         _chk(self.L.xkrt_hip_pool_create(C.byref(cfg), stream, C.byref(h)), "xkrt_hip_pool_create")
         self.h = h

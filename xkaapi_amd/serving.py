@@ -11,9 +11,10 @@ replayed from a captured hipGraph), or each batch gets a launch of its own
 (the plain per-batch form).  Launches queue on the pool's stream, so staging
 the next launch overlaps the device work of the current ones.
 
-Submit-to-ready latency of a batch = its scheduled arrival -> the engine
-seeing the launch that resolved it complete; every task of the batch counts
-once in the percentiles.  A client closes after its last batch and leaves
+Submit-to-ready latency of a batch = its scheduled arrival -> the launch that
+resolved it complete and its edges copied into the client's host buffer (the
+device writes them to pinned host memory, so this is a host copy); every task
+of the batch counts once in the percentiles.  A client closes after its last batch and leaves
 (its bins return to the pool) once that batch is resolved and its edges are
 copied out.
 """
@@ -125,7 +126,7 @@ def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30
           graph: bool = True, depth: int = 1, max_rects: int = 1 << 20, bin_x_bytes: int = 1024,
           bin_y_cols: int = 128, reserve: bool = True, max_edges: int = 0,
           direct_dedup: bool = False, small_launch_rects: int = 16384, prearm: bool = True,
-          launch_rects: int = 32768) -> RunResult:
+          launch_rects: int = 32768, host_edges: bool = True) -> RunResult:
     """Run one schedule through a fresh pool and serving engine; mode "batched" or "plain".
     Defaults (measured on the 64-client workload): the small LDS class (0), one
     launch in flight (a batch that arrives during a launch goes into the next
@@ -135,12 +136,16 @@ def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30
     per launch (launch_rects; 0 = no cap).  With no cap a backlog went out in a
     few huge launches, which resolve more slowly per access.
     reserve: allocate the bins of every client up front (within the budget), so
-    admissions do not allocate device memory while serving."""
+    admissions do not allocate device memory while serving.
+    host_edges: the pool's edge buffer lives in pinned host memory (4M edges unless
+    max_edges says otherwise), so a launch's edges are on the host when it completes."""
     byid = {c.cid: c for c in clients}
     nacc = sum(len(c.acc) for c in clients)
+    if host_edges and not max_edges:
+        max_edges = 1 << 22
     pool = R.Pool(kernel=kernel, graph=graph and mode == "batched", max_rects=max_rects,
                   budget_bytes=budget_bytes, max_edges=max_edges, direct_dedup=direct_dedup,
-                  small_launch_rects=small_launch_rects)
+                  small_launch_rects=small_launch_rects, host_edges=host_edges)
     srv = None
     try:
         rb = bins_needed(clients, bin_x_bytes, bin_y_cols) * pool.stats()["bytes_per_bin"] if reserve else 0
