@@ -426,6 +426,18 @@ int xkrt_hip_mcc_fetches_copy(xkrt_hip_mcc_t h, uint64_t *fetch_off, int64_t *fe
  * most bytes (ties: several bits; 0 when nobody owns a byte). */
 int xkrt_hip_mcc_who_owns(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64_t n, uint32_t *owners);
 
+/* The same answers as xkrt_hip_mcc_who_owns, with no allocation: the queries,
+ * their rects and the answers are staged in a static device array of the
+ * library (1.6 MiB per GPU, held until the process ends) instead of device
+ * memory allocated per call.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight scheduling loops that
+ * place tasks by owner, code paths that forbid hipMalloc).  At most
+ * XKRT_HIP_WHO_OWNS_STATIC_MAX queries per call (-E2BIG beyond).  Every call
+ * on one GPU shares the array: calls that overlap on one GPU (from several
+ * threads) race and must be serialised by the caller.  Synchronous. */
+#define XKRT_HIP_WHO_OWNS_STATIC_MAX 32768
+int xkrt_hip_mcc_who_owns_static(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64_t n, uint32_t *owners);
+
 int xkrt_hip_mcc_stats(xkrt_hip_mcc_t h, xkrt_hip_mcc_stats_t *out);
 
 #ifdef __cplusplus

@@ -54,6 +54,7 @@ def mcc_lib():
     L.xkrt_hip_mcc_parts_copy.argtypes = [_p, _p, _p, _u64]
     L.xkrt_hip_mcc_fetches_copy.argtypes = [_p, _p, _p, _u64]
     L.xkrt_hip_mcc_who_owns.argtypes = [_p, _p, _u64, _p]
+    L.xkrt_hip_mcc_who_owns_static.argtypes = [_p, _p, _u64, _p]
     L.xkrt_hip_mcc_stats.argtypes = [_p, C.POINTER(MccStatsT)]
     _BOUND = L
     return L
@@ -147,6 +148,17 @@ class MemoryTree:
         q = to_mcc_array(np.zeros(len(addr), np.uint8), np.zeros(len(addr), np.uint8), addr, m, n)
         out = np.zeros(max(len(q), 1), np.uint32)
         _chk(self.L.xkrt_hip_mcc_who_owns(self.h, q.ctypes.data, len(q), out.ctypes.data), "xkrt_hip_mcc_who_owns")
+        return out[:len(q)]
+
+    # queries per who_owns_static call (include/xkrt_hip.h)
+    WHO_OWNS_STATIC_MAX = 32768
+
+    def who_owns_static(self, addr, m, n) -> np.ndarray:
+        """who_owns with no device allocation: staging in the library's static device array."""
+        q = to_mcc_array(np.zeros(len(addr), np.uint8), np.zeros(len(addr), np.uint8), addr, m, n)
+        out = np.zeros(max(len(q), 1), np.uint32)
+        _chk(self.L.xkrt_hip_mcc_who_owns_static(self.h, q.ctypes.data, len(q), out.ctypes.data),
+             "xkrt_hip_mcc_who_owns_static")
         return out[:len(q)]
 
     def stats(self) -> dict:

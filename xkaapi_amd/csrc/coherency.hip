@@ -38,6 +38,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/xkrt_hip.h"
@@ -537,6 +538,40 @@ __global__ void __launch_bounds__(256) c_owners(CArgs a, const uint32_t *qx0, co
     }
 }
 
+// ---------------------------------------------------------------- who_owns without allocation
+//
+// xkrt_hip_mcc_who_owns_static stages its queries (the caller's access
+// records), their rects and the answers in one static device array instead of
+// per-call allocations.  Each GPU holds its own copy of the array until the
+// process ends, and every call on a GPU reuses it, so calls that overlap on
+// one GPU race.  In host code the array's name is not a device address: its
+// address on each device comes from hipGetSymbolAddress, once per device, when
+// the first tree is created there (scratch_setup), and only that address is
+// used.  Every word a call reads is written by the same call first (the
+// copied queries, then their rects, then one answer per query), so nothing
+// has to be cleared between calls.
+constexpr uint64_t WHO_OWNS_STATIC_MAX = XKRT_HIP_WHO_OWNS_STATIC_MAX;  // queries per call
+constexpr size_t OWN_ACC_BYTES = WHO_OWNS_STATIC_MAX * sizeof(xkrt_hip_mcc_access_t);
+constexpr size_t OWN_Q_BYTES = WHO_OWNS_STATIC_MAX * 16;  // x0, x1, y0, y1 (u32, SoA)
+constexpr size_t OWN_OUT_BYTES = WHO_OWNS_STATIC_MAX * 4;
+static_assert(OWN_ACC_BYTES % 256 == 0 && OWN_Q_BYTES % 256 == 0, "scratch sections stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[OWN_ACC_BYTES + OWN_Q_BYTES + OWN_OUT_BYTES];
+
+// the queries' rects in the tree frame (x in sizeof_type units, y relative to
+// the frame's first column), from access records already checked on the host
+__global__ void __launch_bounds__(256) c_owners_prep(const xkrt_hip_mcc_access_t *acc, uint32_t n, uint64_t LDs,
+                                                     uint32_t s, uint64_t ylo, uint32_t *q)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const xkrt_hip_mcc_access_t a = acc[i];
+        const uint64_t xb = a.addr % LDs, col = a.addr / LDs;
+        q[i] = (uint32_t)(xb / s);
+        q[n + i] = (uint32_t)(xb / s + a.m);
+        q[2 * n + i] = (uint32_t)(col - ylo);
+        q[3 * n + i] = (uint32_t)(col - ylo + a.n);
+    }
+}
+
 // ---------------------------------------------------------------- scan
 
 __global__ void __launch_bounds__(SCAN_BLOCK) s_reduce(const uint32_t *in, uint32_t n, uint32_t *bsum)
@@ -653,9 +688,38 @@ int grow(T **p, uint64_t *cap, uint64_t need, bool keep, hipStream_t s)
 
 uint32_t waves_grid(uint64_t n) { return (uint32_t)std::min<uint64_t>(std::max<uint64_t>((n + 3) / 4, 1), 65536); }
 
+// per device: the address of `scratch` on that device (0 until looked up)
+constexpr int MAX_DEVICES = 64;
+void *g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// looks up the address of `scratch` on `device`, which must be current; once per device
+int scratch_setup(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return -ENODEV;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch[device]) return 0;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return -ENODEV;
+    g_scratch[device] = p;
+    return 0;
+}
+
+void *scratch_of(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return g_scratch[device];
+}
+
+// the checks of who_owns on one query, and its rect in the tree frame
+int owner_query(const xkrt_hip_mcc *h, const xkrt_hip_mcc_access_t &a);
+
 } // namespace
 
 struct xkrt_hip_mcc {
+    int device = 0;
     uint64_t ld = 0, s = 0, LDs = 0;
     xkrt_hip_mcc_config_t cfg{};
     uint32_t nxu = 0, ny = 0, ntx = 0, nty = 0;
@@ -780,6 +844,8 @@ int xkrt_hip_mcc_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_mcc_co
     }
     int rc = 0;
     auto chk = [&](hipError_t e) { if (!rc) rc = cerr(e); };
+    chk(hipGetDevice(&h->device));
+    if (!rc) rc = scratch_setup(h->device);  // the static staging of who_owns_static on this device
     if (stream) {
         h->stream = (hipStream_t)stream;
     } else {
@@ -1047,6 +1113,47 @@ int xkrt_hip_mcc_fetches_copy(xkrt_hip_mcc_t h, uint64_t *fetch_off, int64_t *fe
         o[7] = v[7][r] == NONE ? -1 : (int64_t)v[7][r];
     }
     return 0;
+}
+
+namespace {
+
+int owner_query(const xkrt_hip_mcc *h, const xkrt_hip_mcc_access_t &a)
+{
+    const uint64_t xb = a.addr % h->LDs, col = a.addr / h->LDs;
+    if (a.m == 0 || a.n == 0 || xb % h->s) return -EINVAL;
+    if (xb + a.m * h->s > h->LDs) return -ENOTSUP;
+    if (col < h->cfg.y_lo || col + a.n > h->cfg.y_hi) return -ERANGE;
+    return 0;
+}
+
+} // namespace
+
+int xkrt_hip_mcc_who_owns_static(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64_t n, uint32_t *owners)
+{
+    if (!h || (!acc && n) || (!owners && n)) return -EINVAL;
+    if (h->poisoned) return -ECANCELED;  // the reference has aborted
+    if (n == 0) return 0;
+    if (n > WHO_OWNS_STATIC_MAX) return -E2BIG;
+    for (uint64_t i = 0; i < n; ++i) {
+        const int rc = owner_query(h, acc[i]);
+        if (rc) return rc;
+    }
+    unsigned char *base = (unsigned char *)scratch_of(h->device);
+    if (!base) return -ENODEV;
+    xkrt_hip_mcc_access_t *dacc = (xkrt_hip_mcc_access_t *)base;
+    uint32_t *dq = (uint32_t *)(base + OWN_ACC_BYTES);
+    uint32_t *dout = (uint32_t *)(base + OWN_ACC_BYTES + OWN_Q_BYTES);
+    const uint32_t nq = (uint32_t)n;
+    CCHK(hipMemcpyAsync(dacc, acc, sizeof(xkrt_hip_mcc_access_t) * n, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(c_owners_prep, dim3((nq + 255) / 256), dim3(256), 0, h->stream, dacc, nq, h->LDs,
+                       (uint32_t)h->s, h->cfg.y_lo, dq);
+    CCHK(hipGetLastError());
+    CArgs a = make_cargs(h, 0);
+    hipLaunchKernelGGL(c_owners, dim3(std::min<uint32_t>(nq, 4096)), dim3(256), 0, h->stream, a, dq, dq + n,
+                       dq + 2 * n, dq + 3 * n, nq, dout);
+    CCHK(hipGetLastError());
+    CCHK(hipMemcpyAsync(owners, dout, 4ull * n, hipMemcpyDeviceToHost, h->stream));
+    return cerr(hipStreamSynchronize(h->stream));
 }
 
 int xkrt_hip_mcc_who_owns(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64_t n, uint32_t *owners)
