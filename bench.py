@@ -237,8 +237,12 @@ def roofline_leg(torch, prefix_name="random2d_102400"):
            "workload": f"random2d, first {d['accesses']} accesses in {nl} plain 4096-task launches, "
                        "bins 2048 B x 256 cols, class 1; sum F_i and E from the oracle"}
     if traffic:
+        # the hardware view: measured HBM bytes per launch over the same duration
         out["traffic_achieved"] = round(traffic / t_res / 1e9, 3)
         out["traffic_frac"] = round(traffic / t_res / 1e9 / HBM_PEAK_GBS, 6)
+        out["note"] = ("B_alg charges 64 B per fragment of the reference's canonical partition (sum F_i); "
+                       "the bin painter maps never hold those fragments, so achieved exceeds what HBM "
+                       "moves: traffic_frac is the kernel's hardware roofline fraction")
     return out
 
 
@@ -422,7 +426,7 @@ def heat_leg(torch, steps=200, reps=3, finalists=2):
     return out
 
 
-def serving_leg(fracs=(0.1, 0.3, 0.5, 0.7, 0.9)):
+def serving_leg(fracs=(0.1, 0.3, 0.5, 0.7, 0.9), cpu=True):
     """64 clients (Cholesky / LU 50/50, n ~ U[4, 48] tiles of 128^2 fp64), one
     pool with a 16 GB budget, engine defaults (prearmed launches).  Saturation
     (closed loop, median of three runs) of the batched form
@@ -430,8 +434,10 @@ def serving_leg(fracs=(0.1, 0.3, 0.5, 0.7, 0.9)):
     per batch); then open-loop Poisson arrivals at fractions of the batched
     saturation rate, both forms on the same schedule.  The serving engine is
     the library's native host thread; batches are submitted at their arrival
-    times.  Latency = scheduled arrival -> the engine seeing the launch that
-    resolved it complete, task-weighted; host staging and upload included."""
+    times.  Latency = scheduled arrival -> the launch that resolved it complete
+    and its edges in the client's host buffer, task-weighted; host staging and
+    upload included.  Beside it, the reference engine with one client domain
+    per core."""
     from xkaapi_amd import serving as V
     cs = V.make_clients(64, 4, 48, 128, 256)
     nb = sum(len(c.batches) for c in cs)
@@ -460,6 +466,8 @@ def serving_leg(fracs=(0.1, 0.3, 0.5, 0.7, 0.9)):
                                           "p50_us": round(r.pct(50), 1), "p99_us": round(r.pct(99), 1),
                                           "launches": r.launches})
     log(f"serving without prearm: {out['no_prearm']}")
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline_serving(cs)
     return out
 
 
@@ -477,6 +485,7 @@ def main():
     ap.add_argument("--no-coherency", action="store_true")
     ap.add_argument("--no-heat", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--heat-steps", type=int, default=200)
     ap.add_argument("--calibrate-mib", type=int, default=0,
                     help="also run one streaming copy of this many MiB (HBM counter calibration)")
@@ -581,10 +590,10 @@ def main():
     p50 = weighted_pct(lat_us, wts, 50)
     p99 = weighted_pct(lat_us, wts, 99)
 
-    # roofline of the dominant kernel (k_resolve) on the Cholesky stream, plain form:
-    # algorithmic bytes 24 N_acc + 64 sum F_i + 12 E; tile-aligned accesses overlap exactly one
-    # fragment of the reference's partition, so sum F_i = N_acc
-    roof = None
+    # k_resolve on the Cholesky stream, plain form: algorithmic bytes 24 N_acc + 64 sum F_i + 12 E;
+    # tile-aligned accesses overlap exactly one fragment of the reference's partition, so
+    # sum F_i = N_acc.  (The headline roofline is the random stream's, below: it dominates the step.)
+    roof_chol = None
     if "cholesky_8192_nb128" in res["plain"]["kt"]:
         kt, nl = res["plain"]["kt"]["cholesky_8192_nb128"]
         n_a = len(chol)
@@ -602,13 +611,22 @@ def main():
             # the same kernels' duration in that profile's kernel trace; the start / stop events
             # above add the dispatch of each kernel (~2.3 us per kernel on this part)
             prof_us = t.get("rocprof_avg_us_per_launch")
-        roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-                "kernel": "k_resolve", "avg_us": round(t_res * 1e6, 2), "rocprof_avg_us": prof_us,
-                "bytes_per_launch": int(b_alg),
-                "workload": "cholesky 8192^2 nb128, 4096-task launches (plain form)"}
+        roof_chol = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                     "kernel": "k_resolve", "avg_us": round(t_res * 1e6, 2), "rocprof_avg_us": prof_us,
+                     "bytes_per_launch": int(b_alg),
+                     "workload": "cholesky 8192^2 nb128, 4096-task launches (plain form)"}
+    # the dominant kernel of the step: k_resolve of the random stream's plain launches
+    roof = roofline_leg(torch) if rnd is not None else roof_chol
+    log(f"roofline: {roof}")
+    e2e = None
+    if not args.no_e2e and rnd is not None:
+        e2e = e2e_leg(chol, rnd, {p.name: p.merge for p in preps})
+        log(f"end to end: {e2e}")
 
     cpu = None if args.no_cpu else cpu_baseline(chol)
+    if cpu:
+        log(f"cpu baseline: {cpu}")
     coh = None
     if not args.no_coherency:
         coh, coh_subs = coherency_leg()
@@ -617,8 +635,10 @@ def main():
         log(f"coherency: {coh}")
     heat = None if args.no_heat else heat_leg(torch, args.heat_steps)
     if heat:
+        if not args.no_cpu:
+            heat["cpu_baseline"] = cpu_baseline_heat()
         log(f"heat: {heat}")
-    serving = None if args.no_serving else serving_leg()
+    serving = None if args.no_serving else serving_leg(cpu=not args.no_cpu)
     if serving:
         log(f"serving: {serving}")
     out = {
@@ -645,6 +665,8 @@ def main():
         "p99_batch_latency_us": round(p99, 2),
         "edges": res["plain"]["edges"],
         "roofline": roof,
+        "roofline_cholesky": roof_chol,
+        "end_to_end": e2e,
         "cpu_baseline": cpu,
         "coherency": coh,
         "heat": heat,

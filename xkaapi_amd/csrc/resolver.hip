@@ -2944,6 +2944,7 @@ struct SrvRequest {
     std::vector<xkrt_hip_access_t> acc;
     double t_submit = 0, t_ready = 0;
     std::atomic<int> status{1};  // 1 pending, 0 resolved, < 0 refused (negative errno)
+    int outcome = 0;             // engine thread: the launch's verdict, published once the edges are out
 };
 
 struct SrvClient {
@@ -2990,6 +2991,7 @@ struct xkrt_hip_server {
     // engine thread only
     std::deque<SrvRequest *> pending;
     std::deque<SrvFlight> flights;
+    std::vector<SrvFlight> retired;  // completed launches whose batches are stamped ready after their edges are out
     std::vector<SrvClient *> live;   // received a batch or a close, not left
     SrvClient *owner[MAX_DOMS] = {};
     uint64_t round = 0;
@@ -3072,24 +3074,35 @@ int srv_retire(xkrt_hip_server *s, bool &work)
         };
         for (SrvRequest *r : f.reqs) {
             SrvClient *c = r->c;
+            c->need = total;
             const uint32_t bits = failed(c);
             if (bits && !c->status) {
                 c->dom->err |= bits;
                 c->status = err_to_errno(bits);
             }
+            r->outcome = bits ? c->status : 0;  // published by srv_complete
         }
-        // a batch is ready once its edges are in its client's host buffer
-        rc = srv_harvest(s);
-        if (rc) return rc;
-        const double t = srv_us(s->t0);
-        for (SrvRequest *r : f.reqs) {
-            r->c->need = total;
-            srv_finish(s, r, failed(r->c) ? r->c->status : 0, t);
-        }
-        s->count(xkrt_hip_server::S_BATCHES, f.reqs.size());
+        s->retired.push_back(std::move(f));
         s->flights.pop_front();
         work = true;
     }
+    return 0;
+}
+
+// A batch is ready once its edges are in its client's host buffer: the edges of
+// the retired launches are copied out (after the next launch was enqueued, so
+// the device works meanwhile), then their batches are stamped.
+int srv_complete(xkrt_hip_server *s)
+{
+    if (s->retired.empty()) return 0;
+    const int rc = srv_harvest(s);
+    if (rc) return rc;
+    const double t = srv_us(s->t0);
+    for (SrvFlight &f : s->retired) {
+        for (SrvRequest *r : f.reqs) srv_finish(s, r, r->outcome, t);
+        s->count(xkrt_hip_server::S_BATCHES, f.reqs.size());
+    }
+    s->retired.clear();
     return 0;
 }
 
@@ -3315,6 +3328,7 @@ void srv_loop(xkrt_hip_server *s)
         in.clear();
         int rc = srv_retire(s, work);
         if (!rc) rc = srv_launch(s, work);
+        if (!rc) rc = srv_complete(s);
         const double now = srv_us(s->t0);
         if (!rc) rc = srv_housekeep(s, work, now - last_work);
         if (rc) {
@@ -3325,6 +3339,9 @@ void srv_loop(xkrt_hip_server *s)
             for (SrvRequest *r : s->pending) srv_finish(s, r, rc, t);
             for (auto &f : s->flights)
                 for (SrvRequest *r : f.reqs) srv_finish(s, r, rc, t);
+            for (auto &f : s->retired)
+                for (SrvRequest *r : f.reqs) srv_finish(s, r, rc, t);
+            s->retired.clear();
             s->pending.clear();
             s->flights.clear();
             break;
