@@ -54,3 +54,17 @@ def test_product_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(_loader, "_cache", {})
     with pytest.raises(_loader.LibraryMissing):
         _loader.load_host_lib("libxkrt_hip.so")
+
+
+def test_compiled_abi_driver_links_against_the_library():
+    # the C++ caller (tests/abi/xkrt_abi_driver.cpp) is built and every xkrt_* symbol it imports is
+    # declared in include/ and exported by the library (it runs in tests/test_gpu_abi_driver.py)
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "abi", "xkrt_abi_driver")
+    assert os.path.exists(exe), "run __graft_entry__.build()"
+    und = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True, text=True, check=True).stdout
+    used = {l.split()[-1] for l in und.splitlines() if l.split() and l.split()[-1].startswith("xkrt_")}
+    assert {"xkrt_hip_dep_submit", "xkrt_hip_pool_stage", "xkrt_hip_mcc_plan"} <= used
+    assert used <= set(declared())
+    lib = hip_lib()
+    assert all(hasattr(lib, n) for n in used)

@@ -44,9 +44,22 @@ def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-B", "all"])
 
 
+ABI_DIR = os.path.join(ROOT, "tests", "abi")
+
+
+def build_abi_driver():
+    """The compiled C++ caller of the C ABI (tests/abi/xkrt_abi_driver.cpp), linked
+    against lib/libxkrt_hip.so (found at run time through its rpath)."""
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "lib")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
+          "-o", os.path.join(ABI_DIR, "xkrt_abi_driver"), os.path.join(ABI_DIR, "xkrt_abi_driver.cpp"),
+          "-L", LIB, "-lxkrt_hip", "-Wl,-rpath,$ORIGIN/../../xkaapi_amd/lib", "-Wl,-rpath-link," + rocm_lib])
+
+
 def build_all():
     build_host()
     build_hip()
+    build_abi_driver()
     build_oracle()
 
 
