@@ -360,8 +360,10 @@ int xkrt_hip_server_stats(xkrt_hip_server_t s, xkrt_hip_server_stats_t *out);
 
 /* ------------------------------------------------------------------------
  * Coherency fetch planner: one memory tree per (ld, sizeof_type), a host
- * (device 0) and one GPU (device 1), synchronous completion (the fetches of
- * an access complete before the next access), no eviction.
+ * (device 0) and one GPU (device 1), no eviction.  By default completion is
+ * synchronous (the fetches of an access complete before the next access);
+ * with async_completion the fetches stay in flight until
+ * xkrt_hip_mcc_complete, and later accesses see them being fetched.
  *
  * Replaces KBLASMemoryTree<2>::fetch (memory-tree.hpp:1806-1874) with its
  * host path (fetch_list_to_host, 1110-1215), allocation-view reuse and
@@ -374,7 +376,9 @@ int xkrt_hip_server_stats(xkrt_hip_server_t s, xkrt_hip_server_stats_t *out);
 typedef struct xkrt_hip_mcc_config {
     uint64_t y_lo, y_hi;   /* column range of the tree's frame: columns addr / (ld * sizeof_type) in [y_lo, y_hi) */
     uint64_t budget_bytes; /* device memory budget of the tree state; 0 = unlimited */
-    uint64_t pad;
+    uint32_t async_completion; /* 1: fetches complete when xkrt_hip_mcc_complete says so (fetch_callback,
+                                  memory-tree.hpp:955-1061); 0: at the end of their access */
+    uint32_t pad;
 } xkrt_hip_mcc_config_t;
 
 typedef struct xkrt_hip_mcc_access {
@@ -396,6 +400,9 @@ typedef struct xkrt_hip_mcc_stats {
     uint64_t poisoned;      /* 1 after an abort (ninth view) or a capacity error */
     float    last_ms;       /* GPU time of the last plan call, first kernel to last */
     uint32_t pad;
+    uint64_t last_fetch_base;  /* id of the first fetch row of the last plan call (async: ids of row r = base + r) */
+    uint64_t in_flight;        /* async: fetches issued and not completed */
+    uint64_t registrations;    /* async: accesses awaiting and forwards registered, not yet fired */
 } xkrt_hip_mcc_stats_t;
 
 typedef struct xkrt_hip_mcc *xkrt_hip_mcc_t;
@@ -414,13 +421,45 @@ int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64
 
 /* Valid-region split of the last plan call: part_off[n + 1] row offsets per
  * access; parts = 6 int64 per row (x0, x1 bytes; y0, y1 columns;
- * coherency bits before the access, 1 = host, 2 = GPU; fetching bits). */
+ * coherency bits before the access, 1 = host, 2 = GPU; fetching bits before
+ * it, always 0 with synchronous completion). */
 int xkrt_hip_mcc_parts_copy(xkrt_hip_mcc_t h, uint64_t *part_off, int64_t *parts, uint64_t cap);
 
 /* Fetch list of the last plan call: fetch_off[n + 1]; 8 int64 per row
  * (x0, x1, y0, y1, src device, src chunk, dst device, dst chunk; chunk -1 =
  * host memory; chunks are numbered in allocation order from 0). */
 int xkrt_hip_mcc_fetches_copy(xkrt_hip_mcc_t h, uint64_t *fetch_off, int64_t *fetches, uint64_t cap);
+
+/* Asynchronous completion (trees created with async_completion).  Every fetch
+ * row has an id (fetch rows of a plan call: last_fetch_base + row; forwards:
+ * see below).  A GPU read of a view already being fetched does not fetch again:
+ * the access awaits that fetch (memory-tree.hpp:1516-1527).  A GPU read of data
+ * the GPU holds nowhere coherent while some view of it is being fetched there
+ * registers a D2D forward from that view instead (1567-1600).  Both are
+ * registrations, listed per access of the last plan call by
+ * xkrt_hip_mcc_awaits_copy: awaits_off[n + 1] row offsets; rows of 8 int64
+ * (x0, x1 bytes; y0, y1 columns; kind 2 = await, 3 = forward; the chunk whose
+ * fetch fires it; the forward's destination chunk or -1; registration id). */
+int xkrt_hip_mcc_awaits_copy(xkrt_hip_mcc_t h, uint64_t *awaits_off, int64_t *rows, uint64_t cap);
+
+/* Completes n fetches in flight, in this order (fetch_callback,
+ * memory-tree.hpp:955-1061; SEARCH_FETCHED 2067-2127): over each fetch's rect
+ * the destination view becomes coherent and stops fetching, the device becomes
+ * coherent, and stops fetching once none of its views is.  The registrations
+ * on the fetched view fire: awaiting accesses are released and forwards are
+ * issued as new fetches (ids continuing the tree's count, in the order of the
+ * completions and, within one, of the registrations).  -EINVAL: an id not in
+ * flight (or twice).  *nreleased / *nforwards = rows of this call. */
+int xkrt_hip_mcc_complete(xkrt_hip_mcc_t h, const uint64_t *fetch_ids, uint64_t n, uint64_t *nreleased,
+                          uint64_t *nforwards);
+
+/* What the last xkrt_hip_mcc_complete call fired, per completion k:
+ * rel_off[n + 1] / fwd_off[n + 1] row offsets.  released: 7 int64 per row
+ * (access index in the tree's order, x0, x1, y0, y1, chunk, registration id);
+ * forwards: 9 int64 per row (fetch id, x0, x1, y0, y1, src device, src chunk,
+ * dst device, dst chunk). */
+int xkrt_hip_mcc_completed_copy(xkrt_hip_mcc_t h, uint64_t *rel_off, int64_t *released, uint64_t rel_cap,
+                                uint64_t *fwd_off, int64_t *forwards, uint64_t fwd_cap);
 
 /* who_owns for n rects on the current state: device bitfield owning the
  * most bytes (ties: several bits; 0 when nobody owns a byte). */

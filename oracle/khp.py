@@ -26,6 +26,8 @@ def lib():
         _lib.xko_resolve_marks.restype = C.c_int64
         _lib.xko_partition.argtypes = [u, p, p, p, u, p, p]
         _lib.xko_partition.restype = C.c_int64
+        _lib.xko_coherency_events.argtypes = [u, p, p, p, p, p, u, p, p, u, p, p]
+        _lib.xko_coherency_events.restype = C.c_int64
     return _lib
 
 
@@ -141,3 +143,25 @@ def coherency(dev, mode, rects, nrect, chunk_bytes, queries=None):
     plan = CoherencyPlan(po, parts[:po[last]].copy(), fo, fetches[:fo[last]].copy(), int(fail[0]) if rc < 0 else -1)
     plan.owners = owners[:nq]
     return plan
+
+
+def coherency_events(dev, mode, rects, nrect, chunk_bytes, ev_type, ev_arg):
+    """Coherency with fetches in flight, driven by an event schedule (ev_type 0:
+    plan the next access; 1: complete the in-flight fetch of rank ev_arg, in id
+    order).  Returns (log rows (N, 12) int64, fail_at): see xko_coherency_events."""
+    L = lib()
+    n = len(dev)
+    dev = _c(dev, np.uint8); mode = _c(mode, np.uint8)
+    rects = _c(rects, np.uint64); nrect = _c(nrect, np.uint8)
+    cb = _c(chunk_bytes, np.uint64)
+    et = _c(ev_type, np.uint8); ea = _c(ev_arg, np.uint64)
+    fail = np.zeros(1, np.int64)
+    cap = max(1024, 16 * n)
+    while True:
+        rows = np.empty((cap, 12), np.int64)
+        rc = L.xko_coherency_events(n, dev.ctypes.data, mode.ctypes.data, nrect.ctypes.data, rects.ctypes.data,
+                                    cb.ctypes.data, len(et), et.ctypes.data, ea.ctypes.data, cap, rows.ctypes.data,
+                                    fail.ctypes.data)
+        if rc <= cap:
+            return rows[:rc].copy(), int(fail[0])
+        cap = int(rc)

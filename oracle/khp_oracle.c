@@ -814,3 +814,378 @@ int64_t xko_partition(uint64_t nacc, const uint8_t *dev, const uint8_t *nrect, c
     flat_free(&f);
     return (int64_t)l.n;
 }
+
+/* ------------------------------------------------------------------ */
+/* coherency with fetches in flight (asynchronous completion)           */
+/* ------------------------------------------------------------------ */
+
+/*
+ * The same tree, but a fetch completes only when an event says so
+ * (fetch_callback, memory-tree.hpp:955-1061), so later accesses see the
+ * fetching bits that setup_copies set (1471-1631):
+ *   - a GPU read of a view being fetched awaits that fetch: no copy, the
+ *     access is registered on the view (1516-1527);
+ *   - a GPU read of data the GPU does not hold coherent, while some view of
+ *     it is being fetched there, registers a D2D forward on the first
+ *     fetching view (1567-1600, USE_D2D_FORWARDING = 1 at line 85); the
+ *     forward copy is issued when that view's fetch completes;
+ *   - a host read skips data the host is already fetching (1126-1130).
+ * Completion of a fetch of chunk c to device d over rect R (SEARCH_FETCHED,
+ * 2067-2127): every piece inside R gets, on the view holding c, coherency set
+ * and fetching cleared, then device coherency set and device fetching cleared
+ * once none of its views is fetching; the accesses awaiting on that view are
+ * released and its forwards issued.  Registrations live on the pieces and
+ * are copied when a piece splits; one fires once, at the first completion
+ * that reaches a piece holding it (the reference keeps it on the shrunk
+ * original node, which the same completion reaches).
+ *
+ * Events: ev_type[k] = 0 plans the next access; ev_type[k] = 1 completes the
+ * in-flight fetch of rank ev_arg[k] mod (fetches in flight), in id order
+ * (nothing when none is in flight).  Fetch ids count every fetch issued, plan
+ * fetches and forwards, from 0.  Registrations fired by one completion are
+ * handled in registration order, and the forwards among them take fetch ids in
+ * that order.
+ *
+ * Output: log rows of 12 int64 [event, kind, access, x0, x1, y0, y1, a, b, c, d, id]:
+ *   kind 0 partite of a planned access: a = coherency before, b = fetching before
+ *   kind 1 fetch issued by a plan:      a = src dev, b = src chunk, c = dst dev, d = dst chunk, id = fetch id
+ *   kind 2 access awaiting a fetch:     a = chunk awaited, id = registration
+ *   kind 3 forward registered:          a = src chunk (fetching view), b = dst chunk, id = registration
+ *   kind 4 awaiting access released:    a = chunk, id = registration
+ *   kind 5 forward issued:              a = 1, b = src chunk, c = 1, d = dst chunk, id = fetch id
+ * Returns the number of log rows (may exceed log_cap).  At a ninth
+ * allocation view *fail_at = that access and the log stops before it.
+ */
+typedef struct {
+    uint64_t y0, y1;
+    uint16_t coh, fetch;
+    replica_t rep[NDEV];
+    uint32_t *reg;       /* registrations held by this piece */
+    uint32_t nreg, creg;
+} apiece_t;
+
+static void ap_copy(void *dst, const void *src)
+{
+    const apiece_t *s = src;
+    apiece_t *d = dst;
+    *d = *s;
+    d->creg = s->nreg;
+    d->reg = s->nreg ? xmalloc(sizeof(uint32_t) * s->nreg) : NULL;
+    if (s->nreg) memcpy(d->reg, s->reg, sizeof(uint32_t) * s->nreg);
+}
+static void ap_init(void *p) { memset(p, 0, sizeof(apiece_t)); }
+static void ap_fini(void *p) { free(((apiece_t *)p)->reg); }
+static const ptype_t APT = {sizeof(apiece_t), ap_copy, ap_init, ap_fini};
+
+static void ap_addreg(apiece_t *b, uint32_t r)
+{
+    if (b->nreg == b->creg) {
+        b->creg = b->creg ? 2 * b->creg : 4;
+        b->reg = xrealloc(b->reg, sizeof(uint32_t) * b->creg);
+    }
+    b->reg[b->nreg++] = r;
+}
+
+typedef struct {
+    apiece_t *p;
+    uint64_t x0, x1;
+    int dst_view, action, src_dev;
+    int64_t src_chunk;
+} apart_t;
+
+typedef struct {
+    apart_t *v;
+    uint32_t n, c;
+} apvec_t;
+
+static void apart_cb(void *pc, uint64_t x0, uint64_t x1, void *ctx)
+{
+    apvec_t *v = ctx;
+    if (v->n == v->c) {
+        v->c = v->c ? 2 * v->c : 16;
+        v->v = xrealloc(v->v, sizeof(apart_t) * v->c);
+    }
+    apart_t *p = v->v + v->n++;
+    memset(p, 0, sizeof(*p));
+    p->p = pc;
+    p->x0 = x0; p->x1 = x1;
+    p->dst_view = -1;
+    p->src_chunk = -1;
+}
+
+typedef struct {
+    uint64_t x0, x1, y0, y1;
+    int dst_dev;
+    int64_t dst_chunk;
+    int done;
+} afetch_t;
+
+typedef struct {
+    uint32_t acc;
+    int kind;  /* 2 await, 3 forward */
+    uint64_t x0, x1, y0, y1;
+    int64_t chunk, dst_chunk;
+    int fired;
+} areg_t;
+
+typedef struct {
+    int64_t *rows;
+    uint64_t cap, n;
+} alog_t;
+
+static void alog(alog_t *l, int64_t ev, int64_t kind, int64_t acc, uint64_t x0, uint64_t x1, uint64_t y0, uint64_t y1,
+                 int64_t a, int64_t b, int64_t c, int64_t d, int64_t id)
+{
+    if (l->n < l->cap) {
+        int64_t *o = l->rows + 12 * l->n;
+        o[0] = ev; o[1] = kind; o[2] = acc;
+        o[3] = (int64_t)x0; o[4] = (int64_t)x1; o[5] = (int64_t)y0; o[6] = (int64_t)y1;
+        o[7] = a; o[8] = b; o[9] = c; o[10] = d; o[11] = id;
+    }
+    l->n++;
+}
+
+typedef struct {
+    const afetch_t *f;
+    areg_t *regs;
+    u32vec_t hit;  /* registrations fired by this completion */
+} acomp_t;
+
+/* SEARCH_FETCHED on one piece inside the completed fetch's rect (memory-tree.hpp:2067-2127) */
+static void acomplete_cb(void *pc, uint64_t x0, uint64_t x1, void *ctx)
+{
+    (void)x0; (void)x1;
+    acomp_t *c = ctx;
+    apiece_t *b = pc;
+    const int d = c->f->dst_dev;
+    const uint16_t devbit = (uint16_t)(1u << d);
+    replica_t *rr = b->rep + d;
+    if (d != 0) {
+        for (int v = 0; v < rr->nalloc; ++v)
+            if (rr->chunk[v] == c->f->dst_chunk) {
+                rr->coh |= (uint8_t)(1u << v);
+                rr->fetch &= (uint8_t)~(1u << v);
+                break;
+            }
+        /* the view's awaiting accesses and forwards move to this completion */
+        uint32_t k = 0;
+        for (uint32_t i = 0; i < b->nreg; ++i) {
+            areg_t *r = c->regs + b->reg[i];
+            if (r->chunk == c->f->dst_chunk) {
+                if (!r->fired) {
+                    r->fired = 1;
+                    vpush(&c->hit, b->reg[i]);
+                }
+            } else {
+                b->reg[k++] = b->reg[i];
+            }
+        }
+        b->nreg = k;
+    }
+    b->coh |= devbit;
+    if (rr->fetch == 0) b->fetch &= (uint16_t)~devbit;
+}
+
+static int u32cmp(const void *a, const void *b)
+{
+    const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+int64_t xko_coherency_events(uint64_t nacc, const uint8_t *dev, const uint8_t *mode, const uint8_t *nrect,
+                             const uint64_t *rects, const uint64_t *chunk_bytes, uint64_t nev, const uint8_t *ev_type,
+                             const uint64_t *ev_arg, uint64_t log_cap, int64_t *log_rows, int64_t *fail_at)
+{
+    flat_t f = {&APT, NULL, 0, 0};
+    apvec_t pv = {NULL, 0, 0};
+    alog_t lg = {log_rows, log_cap, 0};
+    afetch_t *fe = NULL;
+    uint64_t nfe = 0, cfe = 0;
+    areg_t *rg = NULL;
+    uint64_t nrg = 0, crg = 0;
+    int64_t nchunks = 0;
+    uint64_t *chunk_size = NULL, chunk_cap = 0;
+    int64_t rc = 0;
+    uint64_t a = 0;
+    *fail_at = -1;
+#define NEW_FETCH(X0, X1, Y0, Y1, DD, DC)                                                    \
+    do {                                                                                      \
+        if (nfe == cfe) { cfe = cfe ? 2 * cfe : 256; fe = xrealloc(fe, sizeof(afetch_t) * cfe); } \
+        fe[nfe].x0 = (X0); fe[nfe].x1 = (X1); fe[nfe].y0 = (Y0); fe[nfe].y1 = (Y1);            \
+        fe[nfe].dst_dev = (DD); fe[nfe].dst_chunk = (DC); fe[nfe].done = 0; nfe++;            \
+    } while (0)
+#define NEW_REG(ACC, KIND, P, CH, DCH)                                                      \
+    do {                                                                                      \
+        if (nrg == crg) { crg = crg ? 2 * crg : 256; rg = xrealloc(rg, sizeof(areg_t) * crg); } \
+        rg[nrg].acc = (uint32_t)(ACC); rg[nrg].kind = (KIND);                                 \
+        rg[nrg].x0 = (P)->x0; rg[nrg].x1 = (P)->x1; rg[nrg].y0 = (P)->p->y0; rg[nrg].y1 = (P)->p->y1; \
+        rg[nrg].chunk = (CH); rg[nrg].dst_chunk = (DCH); rg[nrg].fired = 0;                   \
+        ap_addreg((P)->p, (uint32_t)nrg); nrg++;                                              \
+    } while (0)
+
+    for (uint64_t ev = 0; ev < nev; ++ev) {
+        if (ev_type[ev] == 1) {
+            /* complete the in-flight fetch of rank ev_arg mod in-flight count */
+            uint64_t nin = 0;
+            for (uint64_t k = 0; k < nfe; ++k) nin += !fe[k].done;
+            if (!nin) continue;
+            uint64_t want = ev_arg[ev] % nin, id = 0;
+            for (id = 0; id < nfe; ++id)
+                if (!fe[id].done && want-- == 0) break;
+            fe[id].done = 1;
+            acomp_t c = {fe + id, rg, {NULL, 0, 0}};
+            const afetch_t cf = fe[id];
+            c.f = &cf;
+            flat_intersect(&f, cf.x0, cf.x1, cf.y0, cf.y1, acomplete_cb, &c);
+            qsort(c.hit.v, c.hit.n, sizeof(uint32_t), u32cmp);
+            for (uint32_t i = 0; i < c.hit.n; ++i) {
+                const areg_t *r = rg + c.hit.v[i];
+                if (r->kind == 2) {
+                    alog(&lg, (int64_t)ev, 4, r->acc, r->x0, r->x1, r->y0, r->y1, r->chunk, 0, 0, 0, c.hit.v[i]);
+                } else {
+                    /* the forward copy: from the view just fetched to the forward's view */
+                    alog(&lg, (int64_t)ev, 5, r->acc, r->x0, r->x1, r->y0, r->y1, 1, r->chunk, 1, r->dst_chunk,
+                         (int64_t)nfe);
+                    NEW_FETCH(r->x0, r->x1, r->y0, r->y1, 1, r->dst_chunk);
+                }
+            }
+            free(c.hit.v);
+            continue;
+        }
+        if (a >= nacc) continue;
+        const int d = dev[a];
+        const uint8_t m = mode[a];
+        const uint16_t devbit = (uint16_t)(1u << d);
+        if (d == 0 && f.ns == 0) { a++; continue; }  /* fetch_list_to_host: empty tree (1191-1192) */
+        for (int r = 0; r < nrect[a]; ++r) {
+            const uint64_t *q = rects + 8 * a + 4 * r;
+            flat_insert(&f, q[0], q[1], q[2], q[3], nop_insert, NULL);
+        }
+        pv.n = 0;
+        for (int r = 0; r < nrect[a]; ++r) {
+            const uint64_t *q = rects + 8 * a + 4 * r;
+            flat_intersect(&f, q[0], q[1], q[2], q[3], apart_cb, &pv);
+        }
+        for (uint32_t i = 0; i < pv.n; ++i) {
+            const apart_t *p = pv.v + i;
+            alog(&lg, (int64_t)ev, 0, (int64_t)a, p->x0, p->x1, p->p->y0, p->p->y1, p->p->coh, p->p->fetch, 0, 0, 0);
+        }
+        int64_t chunk = -1;
+        if (d == 0) {
+            /* fetch_list_to_host_setup_partition (1110-1181) */
+            for (uint32_t i = 0; i < pv.n; ++i) {
+                apart_t *p = pv.v + i;
+                apiece_t *b = p->p;
+                if (b->coh == 0 || (b->coh & devbit) || (b->fetch & devbit)) continue;
+                b->fetch |= devbit;
+                const int src = __builtin_ffs(b->coh) - 1;
+                replica_t *sr = b->rep + src;
+                int sv = 0;
+                for (int v = 1; v < sr->nalloc; ++v)
+                    if (chunk_size[sr->chunk[sv]] < chunk_size[sr->chunk[v]]) sv = v;
+                p->action = 1;
+                p->src_dev = src;
+                p->src_chunk = sr->chunk[sv];
+            }
+        } else {
+            /* fetch_access_find_allocation (1427-1450) */
+            int found = 0;
+            if (pv.n) {
+                replica_t *r0 = pv.v[0].p->rep + d;
+                for (int j = 0; j < r0->nalloc && !found; ++j) {
+                    const int32_t c = r0->chunk[j];
+                    int ok = 1;
+                    for (uint32_t i = 1; i < pv.n && ok; ++i) {
+                        replica_t *ri = pv.v[i].p->rep + d;
+                        int hit = -1;
+                        for (int k = 0; k < ri->nalloc; ++k)
+                            if (ri->chunk[k] == c) { hit = k; break; }
+                        if (hit < 0) ok = 0; else pv.v[i].dst_view = hit;
+                    }
+                    if (ok) { found = 1; chunk = c; pv.v[0].dst_view = j; }
+                }
+            }
+            if (!found) {
+                /* fetch_access_allocate + create_allocation_views (1293-1372) */
+                if ((uint64_t)nchunks == chunk_cap) {
+                    chunk_cap = chunk_cap ? 2 * chunk_cap : 1024;
+                    chunk_size = xrealloc(chunk_size, sizeof(uint64_t) * chunk_cap);
+                }
+                chunk = nchunks++;
+                chunk_size[chunk] = chunk_bytes[a];
+                for (uint32_t i = 0; i < pv.n; ++i) {
+                    replica_t *ri = pv.v[i].p->rep + d;
+                    if (ri->nalloc >= VIEWS_MAX) { rc = -1; *fail_at = (int64_t)a; goto done; }
+                    ri->chunk[ri->nalloc] = (int32_t)chunk;
+                    pv.v[i].dst_view = ri->nalloc++;
+                }
+            }
+            /* fetch_access_setup_copies (1471-1631) */
+            if (m & M_R) {
+                for (uint32_t i = 0; i < pv.n; ++i) {
+                    apart_t *p = pv.v + i;
+                    apiece_t *b = p->p;
+                    replica_t *dr = b->rep + d;
+                    const uint8_t ab = (uint8_t)(1u << p->dst_view);
+                    if (dr->coh & ab) continue;
+                    if (dr->fetch & ab) {  /* awaiting (1516-1527) */
+                        alog(&lg, (int64_t)ev, 2, (int64_t)a, p->x0, p->x1, b->y0, b->y1, chunk, 0, 0, 0, (int64_t)nrg);
+                        NEW_REG(a, 2, p, chunk, -1);
+                        continue;
+                    }
+                    if (b->coh & ~1u) {
+                        /* RouterAffinity::get_source: dst itself when valid there */
+                        const int src = (b->coh & devbit) ? d : __builtin_ffs(b->coh & ~1u) - 1;
+                        replica_t *sr = b->rep + src;
+                        const int sv = __builtin_ffs(sr->coh) - 1;
+                        p->action = 1;
+                        p->src_dev = src;
+                        p->src_chunk = sr->chunk[sv];
+                    } else if (b->fetch & ~1u) {
+                        /* D2D forward from the first fetching view of the fetching device (1567-1600) */
+                        const int fd = (b->fetch & devbit) ? d : __builtin_ffs(b->fetch & ~1u) - 1;
+                        replica_t *fr = b->rep + fd;
+                        const int fv = __builtin_ffs(fr->fetch) - 1;
+                        alog(&lg, (int64_t)ev, 3, (int64_t)a, p->x0, p->x1, b->y0, b->y1, fr->chunk[fv], chunk, 0, 0,
+                             (int64_t)nrg);
+                        NEW_REG(a, 3, p, fr->chunk[fv], chunk);
+                    } else {
+                        p->action = 1;
+                        p->src_dev = 0;
+                        p->src_chunk = -1;
+                    }
+                    dr->fetch |= ab;
+                    b->fetch |= devbit;
+                }
+            }
+            /* fetch_access_set_coherent (1633-1667) */
+            if (m & M_W) {
+                for (uint32_t i = 0; i < pv.n; ++i) {
+                    apiece_t *b = pv.v[i].p;
+                    for (int dd = 0; dd < NDEV; ++dd) b->rep[dd].coh = 0;
+                    b->rep[d].coh = (uint8_t)(1u << pv.v[i].dst_view);
+                    b->coh = devbit;
+                }
+            }
+        }
+        /* fetch_list_from_partition (1067-1108): the copies are issued; they complete by events */
+        for (uint32_t i = 0; i < pv.n; ++i) {
+            const apart_t *p = pv.v + i;
+            if (p->action != 1) continue;
+            alog(&lg, (int64_t)ev, 1, (int64_t)a, p->x0, p->x1, p->p->y0, p->p->y1, p->src_dev, p->src_chunk, d,
+                 d == 0 ? -1 : chunk, (int64_t)nfe);
+            NEW_FETCH(p->x0, p->x1, p->p->y0, p->p->y1, d, d == 0 ? -1 : chunk);
+        }
+        a++;
+    }
+done:
+#undef NEW_FETCH
+#undef NEW_REG
+    free(pv.v);
+    free(fe);
+    free(rg);
+    free(chunk_size);
+    flat_free(&f);
+    (void)rc;  /* an abort is reported through *fail_at; the log holds what came before it */
+    return (int64_t)lg.n;
+}

@@ -22,7 +22,7 @@ _u64, _u32, _p = C.c_uint64, C.c_uint32, C.c_void_p
 
 
 class MccConfigT(C.Structure):
-    _fields_ = [("y_lo", _u64), ("y_hi", _u64), ("budget_bytes", _u64), ("pad", _u64)]
+    _fields_ = [("y_lo", _u64), ("y_hi", _u64), ("budget_bytes", _u64), ("async_completion", _u32), ("pad", _u32)]
 
 
 class MccAccessT(C.Structure):
@@ -37,7 +37,8 @@ assert MCC_ACCESS_DTYPE.itemsize == C.sizeof(MccAccessT) == 32
 
 class MccStatsT(C.Structure):
     _fields_ = [(n, _u64) for n in ("accesses", "gpu_accesses", "chunks", "last_parts", "last_fetches",
-                                    "device_bytes", "poisoned")] + [("last_ms", C.c_float), ("pad", _u32)]
+                                    "device_bytes", "poisoned")] + [("last_ms", C.c_float), ("pad", _u32)] + \
+               [(n, _u64) for n in ("last_fetch_base", "in_flight", "registrations")]
 
 
 _BOUND = None
@@ -56,6 +57,9 @@ def mcc_lib():
     L.xkrt_hip_mcc_who_owns.argtypes = [_p, _p, _u64, _p]
     L.xkrt_hip_mcc_who_owns_static.argtypes = [_p, _p, _u64, _p]
     L.xkrt_hip_mcc_stats.argtypes = [_p, C.POINTER(MccStatsT)]
+    L.xkrt_hip_mcc_awaits_copy.argtypes = [_p, _p, _p, _u64]
+    L.xkrt_hip_mcc_complete.argtypes = [_p, _p, _u64, C.POINTER(_u64), C.POINTER(_u64)]
+    L.xkrt_hip_mcc_completed_copy.argtypes = [_p, _p, _p, _u64, _p, _p, _u64]
     _BOUND = L
     return L
 
@@ -96,10 +100,10 @@ class MemoryTree:
     """A GPU-resident coherency tree for one (ld, sizeof_type) over columns [y_lo, y_hi)."""
 
     def __init__(self, ld: int, sizeof_type: int, y_range: tuple[int, int], budget_bytes: int = 0,
-                 stream: int | None = None, device: int = 0):
+                 stream: int | None = None, device: int = 0, async_completion: bool = False):
         self.L = mcc_lib()
         _chk(self.L.xkrt_hip_init(device), "xkrt_hip_init")
-        cfg = MccConfigT(int(y_range[0]), int(y_range[1]), budget_bytes, 0)
+        cfg = MccConfigT(int(y_range[0]), int(y_range[1]), budget_bytes, 1 if async_completion else 0, 0)
         h = _p()
         _chk(self.L.xkrt_hip_mcc_create(ld, sizeof_type, C.byref(cfg), stream, C.byref(h)), "xkrt_hip_mcc_create")
         self.h = h
@@ -143,6 +147,33 @@ class MemoryTree:
         _chk(self.L.xkrt_hip_mcc_fetches_copy(self.h, fo.ctypes.data, fetches.ctypes.data, len(fetches)),
              "xkrt_hip_mcc_fetches_copy")
         return CoherencyPlan(po.astype(np.int64), parts[:npart], fo.astype(np.int64), fetches[:nfetch], fail)
+
+    def awaits(self, n: int):
+        """Registrations of the last plan call of n accesses (asynchronous trees):
+        (offsets (n + 1), rows (R, 8): x0, x1, y0, y1, kind 2 await / 3 forward, chunk, dst chunk, id)."""
+        off = np.zeros(n + 1, np.uint64)
+        _chk(self.L.xkrt_hip_mcc_awaits_copy(self.h, off.ctypes.data, None, 0), "xkrt_hip_mcc_awaits_copy")
+        rows = np.zeros((max(int(off[n]), 1), 8), np.int64)
+        _chk(self.L.xkrt_hip_mcc_awaits_copy(self.h, off.ctypes.data, rows.ctypes.data, len(rows)),
+             "xkrt_hip_mcc_awaits_copy")
+        return off.astype(np.int64), rows[:int(off[n])]
+
+    def complete(self, ids):
+        """Complete fetches in flight, in order.  Returns per completion k
+        (released offsets, released (R, 7), forward offsets, forwards (F, 9)):
+        released rows = access, x0, x1, y0, y1, chunk, registration id; forward rows =
+        fetch id, x0, x1, y0, y1, src dev, src chunk, dst dev, dst chunk."""
+        ids = np.ascontiguousarray(ids, np.uint64)
+        nr, nf = _u64(), _u64()
+        _chk(self.L.xkrt_hip_mcc_complete(self.h, ids.ctypes.data, len(ids), C.byref(nr), C.byref(nf)),
+             "xkrt_hip_mcc_complete")
+        ro = np.zeros(len(ids) + 1, np.uint64)
+        fo = np.zeros(len(ids) + 1, np.uint64)
+        rel = np.zeros((max(nr.value, 1), 7), np.int64)
+        fwd = np.zeros((max(nf.value, 1), 9), np.int64)
+        _chk(self.L.xkrt_hip_mcc_completed_copy(self.h, ro.ctypes.data, rel.ctypes.data, len(rel), fo.ctypes.data,
+                                                fwd.ctypes.data, len(fwd)), "xkrt_hip_mcc_completed_copy")
+        return ro.astype(np.int64), rel[:nr.value], fo.astype(np.int64), fwd[:nf.value]
 
     def who_owns(self, addr, m, n) -> np.ndarray:
         q = to_mcc_array(np.zeros(len(addr), np.uint8), np.zeros(len(addr), np.uint8), addr, m, n)

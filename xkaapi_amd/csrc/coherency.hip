@@ -21,9 +21,15 @@
 //     1376-1425); otherwise a allocates the next chunk (c_contain, c_chunks);
 //   * state: per point, coherency bits (1 host, 2 GPU), the view list (chunk
 //     ids in allocation order, at most 8: the reference aborts on a ninth,
-//     1362-1365) and the per-view coherency bits.  c_state replays a tile's
-//     accesses in order on 32 x 32 points, one workgroup per tile, and fills
-//     the payload of every fragment whose first point lies in the tile.
+//     1362-1365), the per-view coherency bits and, for fetches in flight, the
+//     device and per-view fetching bits.  c_state replays a tile's accesses in
+//     order on 32 x 32 points, one workgroup per tile, and fills the payload
+//     of every fragment whose first point lies in the tile;
+//   * fetches in flight (async_completion): a read of a view being fetched
+//     awaits it and a read while another view is being fetched registers a
+//     forward (1516-1527, 1567-1600); c_complete applies completions point by
+//     point with atomics (their effects commute), and the host fires the
+//     registrations of each completed fetch.
 //
 // HBM layout per tree: TX (nxu + 1 u32), TY (nxu x (ny + 1) u32, y fastest),
 // point state tile-major (u32 word + 8 u32 views per point), the GPU access
@@ -65,12 +71,16 @@ struct CCounters {
     unsigned int rows;      // fragment rows this call
     unsigned int fetches;   // fetch rows this call
     unsigned int viewerr;   // lowest access id whose chunk was missing from a point (NONE = none)
-    unsigned int pad;
+    unsigned int regs;      // registration rows this call (fetches in flight)
 };
 
 struct CArgs {
     // batch (index i in [0, n))
     uint32_t n, gid0;
+    uint32_t async;          // 1: fetches stay in flight (fetching bits) until c_complete
+    uint32_t *rreg;          // per fragment row: 1 when the access awaits or registers a forward there
+    uint32_t *rrpos;         // exclusive scan of rreg
+    uint32_t *reg;           // registration rows, 8 u32 each (x0, x1, y0, y1, kind, chunk, dst chunk, access)
     const uint32_t *bx0, *bx1, *by0, *by1, *bflags, *bhist, *bxoff;
     uint32_t *pcount, *rowbase;
     // tree
@@ -329,6 +339,10 @@ __global__ void c_tact(CArgs a, uint32_t ntiles)
 
 // ---------------------------------------------------------------- state
 
+// Per-point state word: bits 0-1 coherency (1 host, 2 GPU), 2-9 coherent
+// views, 10-13 number of views, 14-15 fetching devices, 16-23 fetching views.
+constexpr uint32_t ST_DF = 14, ST_VF = 16;
+
 // Replay the tile's accesses in order on its 32 x 32 points (4 per thread,
 // views in registers through constant-index selects).
 __global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
@@ -383,8 +397,10 @@ __global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
             for (uint32_t k = 0; k < PPT; ++k) {
                 if (!valid[k] || px[k] < x0 || px[k] >= x1 || py[k] < y0 || py[k] >= y1) continue;
                 uint32_t coh = st[k] & 3u, vc = (st[k] >> 2) & 0xffu, nv = (st[k] >> 10) & 0xfu;
-                const uint32_t before = coh;
+                uint32_t df = (st[k] >> ST_DF) & 3u, vf = (st[k] >> ST_VF) & 0xffu;
+                const uint32_t before = coh | (df << 2);
                 bool need = false;
+                uint32_t act = 0;  // 2: awaits a fetch in flight, 3: registers a forward
                 uint32_t sdev = 0, schunk = NONE;
                 if (fl & F_GPU) {
                     uint32_t v = NONE;
@@ -408,25 +424,42 @@ __global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
                     }
                     const uint32_t bit = 1u << v;
                     if ((fl & F_R) && !(vc & bit)) {
-                        // fetch_access_setup_copies (1471-1631): from a coherent view of the GPU, else from host
-                        need = true;
-                        if (coh & 2u) {
-                            const uint32_t sv = (uint32_t)__builtin_ctz(vc);
-                            sdev = 1;
+                        // fetch_access_setup_copies (1471-1631)
+                        if (vf & bit) {
+                            act = 2;  // this view is being fetched: await it (1516-1527)
+                            schunk = chunk;
+                        } else {
+                            if (coh & 2u) {  // from the first coherent view of the GPU
+                                need = true;
+                                const uint32_t sv = (uint32_t)__builtin_ctz(vc);
+                                sdev = 1;
 #pragma unroll
-                            for (uint32_t t = 0; t < VMAX; ++t)
-                                if (t == sv) schunk = vw[k][t];
+                                for (uint32_t t = 0; t < VMAX; ++t)
+                                    if (t == sv) schunk = vw[k][t];
+                            } else if (df & 2u) {  // forward from the first view being fetched (1567-1600)
+                                act = 3;
+                                const uint32_t sv = (uint32_t)__builtin_ctz(vf);
+#pragma unroll
+                                for (uint32_t t = 0; t < VMAX; ++t)
+                                    if (t == sv) schunk = vw[k][t];
+                            } else {  // from host memory
+                                need = true;
+                            }
+                            vf |= bit;
+                            df |= 2u;
                         }
                     }
                     if (fl & F_W) {  // fetch_access_set_coherent (1633-1667)
                         coh = 2u;
                         vc = bit;
                     }
-                    if (need) {      // completion (2067-2127)
+                    if (need && !a.async) {  // synchronous completion (2067-2127)
                         vc |= bit;
                         coh |= 2u;
+                        vf &= ~bit;
+                        if (!vf) df &= ~2u;
                     }
-                } else if (coh == 2u) {
+                } else if (coh != 0u && !(coh & 1u) && !(df & 1u)) {
                     // fetch_list_to_host_setup_partition (1110-1181): from the GPU view of the largest chunk
                     need = true;
                     sdev = 1;
@@ -441,9 +474,10 @@ __global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
                             }
                         }
                     }
-                    coh |= 1u;
+                    if (a.async) df |= 1u;  // fetching to the host (1126-1130)
+                    else coh |= 1u;          // synchronous completion
                 }
-                st[k] = coh | (vc << 2) | (nv << 10);
+                st[k] = coh | (vc << 2) | (nv << 10) | (df << ST_DF) | (vf << ST_VF);
                 // the first point of one of the access's fragments: fill that row
                 const uint32_t u = px[k], y = py[k];
                 if ((u == x0 || a.TX[u] <= aid) && (y == y0 || a.TY[(size_t)u * a.ny1 + y] <= aid)) {
@@ -453,9 +487,10 @@ __global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
                         const uint32_t mid = (lo + hi) >> 1;
                         if (a.ry0[mid] <= y) lo = mid; else hi = mid;
                     }
-                    a.rpay[lo] = before | ((need ? 1u : 0u) << 8) | (sdev << 9);
+                    a.rpay[lo] = before | ((need ? 1u : 0u) << 8) | (sdev << 9) | (act << 12);
                     a.rsrc[lo] = schunk;
                     a.rflag[lo] = need ? 1u : 0u;
+                    if (a.async) a.rreg[lo] = act ? 1u : 0u;
                 }
             }
         }
@@ -494,6 +529,56 @@ __global__ void c_foff(CArgs a, uint32_t nrows, uint32_t nfetch)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < a.n) a.foff[i] = a.rowbase[i] < nrows ? a.rfpos[a.rowbase[i]] : nfetch;
+}
+
+// registration rows of this plan call, compacted in fragment-row order
+__global__ void c_regs(CArgs a, uint32_t nrows)
+{
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += gridDim.x * blockDim.x) {
+        if (!a.rreg[r]) continue;
+        const uint32_t i = a.racc[r];
+        uint32_t *o = a.reg + 8ull * a.rrpos[r];
+        o[0] = a.rx0[r]; o[1] = a.rx1[r]; o[2] = a.ry0[r]; o[3] = a.ry1[r];
+        o[4] = (a.rpay[r] >> 12) & 3u;
+        o[5] = a.rsrc[r];
+        o[6] = ((a.rpay[r] >> 12) & 3u) == 3u ? a.hchunk[a.bhist[i]] : NONE;
+        o[7] = i;
+    }
+}
+
+// Completions (fetch_callback, memory-tree.hpp:955-1061, SEARCH_FETCHED
+// 2067-2127), one block per completed fetch: q = 6 u32 per fetch (x0, x1 in
+// sizeof_type units, y0, y1, dst device, dst chunk).  On every point of the
+// rect the view holding the chunk becomes coherent and stops fetching, the
+// device becomes coherent, and stops fetching once none of its views is.
+// Completions of one call may share points: the updates are atomic and
+// commute, and the one that clears a point's last fetching view clears the
+// device bit.
+__global__ void __launch_bounds__(256) c_complete(CArgs a, const uint32_t *q, uint32_t n)
+{
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const uint32_t u0 = q[6 * k], u1 = q[6 * k + 1], y0 = q[6 * k + 2], y1 = q[6 * k + 3];
+        const uint32_t dd = q[6 * k + 4], dc = q[6 * k + 5];
+        const uint32_t w = u1 - u0;
+        const uint64_t npts = (uint64_t)w * (y1 - y0);
+        for (uint64_t p = threadIdx.x; p < npts; p += blockDim.x) {
+            const uint32_t u = u0 + (uint32_t)(p % w), y = y0 + (uint32_t)(p / w);
+            const uint32_t tile = (y / TH) * a.ntx + u / TW, off = (y % TH) * TW + (u % TW);
+            uint32_t *word = a.pst + (size_t)tile * TP + off;
+            if (dd == 1) {
+                const uint32_t nv = (*word >> 10) & 0xfu;  // the views do not change while completing
+                uint32_t bit = 0;
+                for (uint32_t v = 0; v < nv && !bit; ++v)
+                    if (a.pviews[((size_t)tile * VMAX + v) * TP + off] == dc) bit = 1u << v;
+                atomicOr(word, (bit << 2) | 2u);
+                const uint32_t old = atomicAnd(word, ~(bit << ST_VF));
+                if ((((old >> ST_VF) & 0xffu) & ~bit) == 0) atomicAnd(word, ~(2u << ST_DF));
+            } else {
+                atomicOr(word, 1u);
+                atomicAnd(word, ~(1u << ST_DF));
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------- who_owns
@@ -718,8 +803,22 @@ int owner_query(const xkrt_hip_mcc *h, const xkrt_hip_mcc_access_t &a);
 
 } // namespace
 
+// fetches in flight and registrations (trees with asynchronous completion)
+struct MccFetch {
+    uint32_t u0, u1, y0, y1;  // rect: x in sizeof_type units, y in frame columns
+    uint32_t dd, dc;          // destination device and chunk (NONE: host memory)
+    bool inflight;
+};
+struct MccReg {
+    uint64_t acc;             // access index in the tree's order
+    uint32_t x0, x1, y0, y1;  // x in bytes, y in frame columns
+    uint32_t kind;            // 2 await, 3 forward
+    uint32_t chunk, dst;      // the chunk whose fetch fires it; a forward's destination chunk
+};
+
 struct xkrt_hip_mcc {
     int device = 0;
+    bool async = false;
     uint64_t ld = 0, s = 0, LDs = 0;
     xkrt_hip_mcc_config_t cfg{};
     uint32_t nxu = 0, ny = 0, ntx = 0, nty = 0;
@@ -759,6 +858,16 @@ struct xkrt_hip_mcc {
     uint32_t *fx0 = nullptr, *fx1 = nullptr, *fy0 = nullptr, *fy1 = nullptr, *fsd = nullptr, *fsc = nullptr,
              *fdd = nullptr, *fdc = nullptr;
     uint64_t fcap[8] = {};
+    // fetches in flight (async): registration flags and rows, completion queue
+    uint32_t *rreg = nullptr, *rrpos = nullptr, *reg = nullptr, *cq = nullptr;
+    uint64_t rregcap[2] = {}, regcap = 0, cqcap = 0;
+    std::vector<MccFetch> fetches;      // every fetch issued, by id
+    std::vector<MccReg> regs;           // every registration, by id
+    std::vector<uint32_t> live;         // registrations not fired yet, ascending
+    uint64_t inflight = 0;
+    uint64_t last_fetch_base = 0;
+    std::vector<int64_t> last_reg_off, last_regs;                 // per access of the last plan call
+    std::vector<int64_t> c_rel_off, c_rel, c_fwd_off, c_fwd;      // the last completion call
     // results of the last plan call
     uint32_t last_n = 0, last_rows = 0, last_fetch = 0;
     int64_t last_fail = -1;
@@ -800,6 +909,8 @@ CArgs make_cargs(xkrt_hip_mcc *h, uint32_t n)
     a.fdd = h->fdd; a.fdc = h->fdc; a.foff = h->foff;
     a.tcnt = h->tcnt; a.tcur = h->tcur; a.tstart = h->tstart; a.tinc = h->tinc; a.tact = h->tact;
     a.ctr = h->ctr;
+    a.async = h->async ? 1u : 0u;
+    a.rreg = h->rreg; a.rrpos = h->rrpos; a.reg = h->reg;
     return a;
 }
 
@@ -809,7 +920,8 @@ void free_all(xkrt_hip_mcc *h)
                     h->hx0, h->hx1, h->hy0, h->hy1, h->hbeta, h->hchunk, h->hbytes, h->chunk_size,
                     h->bx0, h->bx1, h->by0, h->by1, h->bflags, h->bhist, h->bxoff, h->pcount, h->rowbase, h->foff,
                     h->xcnt, h->xrow, h->tinc, h->bsum, h->rx0, h->rx1, h->ry0, h->ry1, h->rpay, h->rsrc, h->racc,
-                    h->rflag, h->rfpos, h->fx0, h->fx1, h->fy0, h->fy1, h->fsd, h->fsc, h->fdd, h->fdc};
+                    h->rflag, h->rfpos, h->fx0, h->fx1, h->fy0, h->fy1, h->fsd, h->fsc, h->fdd, h->fdc,
+                    h->rreg, h->rrpos, h->reg, h->cq};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h->ctr_host) (void)hipHostFree(h->ctr_host);
@@ -831,6 +943,7 @@ int xkrt_hip_mcc_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_mcc_co
     xkrt_hip_mcc *h = new xkrt_hip_mcc();
     h->ld = ld; h->s = sizeof_type; h->LDs = ld * sizeof_type;
     h->cfg = *cfg;
+    h->async = cfg->async_completion != 0;
     h->nxu = (uint32_t)ld;
     h->ny = (uint32_t)(cfg->y_hi - cfg->y_lo);
     h->ntx = (h->nxu + TW - 1) / TW;
@@ -897,6 +1010,47 @@ int xkrt_hip_mcc_destroy(xkrt_hip_mcc_t h)
     delete h;
     return 0;
 }
+
+namespace {
+
+// after an asynchronous plan call: its fetch rows (up to the abort) are in flight
+// with ids from last_fetch_base, its registrations join the live list
+int record_async(xkrt_hip_mcc *h, uint64_t n, uint64_t keep, uint32_t nreg)
+{
+    const uint32_t nf = h->h_foff[keep];
+    if (nf) {
+        std::vector<uint32_t> v[6];
+        uint32_t *src[6] = {h->fx0, h->fx1, h->fy0, h->fy1, h->fdd, h->fdc};
+        for (int k = 0; k < 6; ++k) {
+            v[k].resize(nf);
+            CCHK(hipMemcpy(v[k].data(), src[k], 4ull * nf, hipMemcpyDeviceToHost));
+        }
+        for (uint32_t r = 0; r < nf; ++r)
+            h->fetches.push_back({v[0][r] / (uint32_t)h->s, v[1][r] / (uint32_t)h->s, v[2][r], v[3][r], v[4][r],
+                                  v[5][r], true});
+        h->inflight += nf;
+    }
+    h->last_reg_off.assign(n + 1, 0);
+    h->last_regs.clear();
+    std::vector<uint32_t> rows(8ull * nreg);
+    if (nreg) CCHK(hipMemcpy(rows.data(), h->reg, 32ull * nreg, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> per(n + 1, 0);
+    for (uint32_t r = 0; r < nreg; ++r) {
+        const uint32_t *o = rows.data() + 8ull * r;
+        if (o[7] >= keep) continue;  // after the abort: not planned
+        const uint32_t id = (uint32_t)h->regs.size();
+        h->regs.push_back({h->naccess + o[7], o[0], o[1], o[2], o[3], o[4], o[5], o[6]});
+        h->live.push_back(id);
+        per[o[7] + 1]++;
+        const int64_t row[8] = {o[0], o[1], (int64_t)(o[2] + h->cfg.y_lo), (int64_t)(o[3] + h->cfg.y_lo), o[4],
+                                o[5] == NONE ? -1 : (int64_t)o[5], o[6] == NONE ? -1 : (int64_t)o[6], id};
+        h->last_regs.insert(h->last_regs.end(), row, row + 8);
+    }
+    for (uint64_t i = 0; i < n; ++i) h->last_reg_off[i + 1] = h->last_reg_off[i] + (int64_t)per[i + 1];
+    return 0;
+}
+
+} // namespace
 
 int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64_t n, uint64_t *nparts,
                       uint64_t *nfetches, int64_t *fail_at)
@@ -996,6 +1150,12 @@ int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64
     uint32_t **rb[] = {&h->rx0, &h->rx1, &h->ry0, &h->ry1, &h->rpay, &h->rsrc, &h->racc, &h->rflag, &h->rfpos};
     for (int k = 0; k < 9 && !rc; ++k) rc = grow(rb[k], &h->rcap[k], std::max<uint32_t>(nrows, 1), false, s);
     if (rc) return rc;
+    if (h->async) {
+        if (!rc) rc = grow(&h->rreg, &h->rregcap[0], std::max<uint32_t>(nrows, 1), false, s);
+        if (!rc) rc = grow(&h->rrpos, &h->rregcap[1], std::max<uint32_t>(nrows, 1), false, s);
+        if (rc) return rc;
+        if (nrows) CCHK(hipMemsetAsync(h->rreg, 0, 4ull * nrows, s));
+    }
     a = make_cargs(h, (uint32_t)n);
     if (nrows) CCHK(hipMemsetAsync(h->rflag, 0, 4ull * nrows, s));
     hipLaunchKernelGGL(c_rows, dim3(gw), dim3(256), 0, s, a);
@@ -1011,14 +1171,20 @@ int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64
     // fetch rows
     rc = scan(h, h->rflag, std::max<uint32_t>(nrows, 1), h->rfpos, &h->ctr->fetches);
     if (rc) return rc;
+    if (h->async) rc = scan(h, h->rreg, std::max<uint32_t>(nrows, 1), h->rrpos, &h->ctr->regs);
+    if (rc) return rc;
     CCHK(hipMemcpyAsync(h->ctr_host, h->ctr, sizeof(CCounters), hipMemcpyDeviceToHost, s));
     CCHK(hipStreamSynchronize(s));
     const uint32_t nf = nrows ? h->ctr_host->fetches : 0;
     uint32_t **fb[] = {&h->fx0, &h->fx1, &h->fy0, &h->fy1, &h->fsd, &h->fsc, &h->fdd, &h->fdc};
     for (int k = 0; k < 8 && !rc; ++k) rc = grow(fb[k], &h->fcap[k], std::max<uint32_t>(nf, 1), false, s);
     if (rc) return rc;
+    const uint32_t nreg = h->async && nrows ? h->ctr_host->regs : 0;
+    if (nreg) rc = grow(&h->reg, &h->regcap, 8ull * nreg, false, s);
+    if (rc) return rc;
     a = make_cargs(h, (uint32_t)n);
     if (nrows) hipLaunchKernelGGL(c_fetch, dim3(std::min<uint32_t>((nrows + 255) / 256, 8192)), dim3(256), 0, s, a, nrows);
+    if (nreg) hipLaunchKernelGGL(c_regs, dim3(std::min<uint32_t>((nrows + 255) / 256, 8192)), dim3(256), 0, s, a, nrows);
     hipLaunchKernelGGL(c_foff, dim3(((uint32_t)n + 255) / 256), dim3(256), 0, s, a, nrows, nf);
     CCHK(hipEventRecord(h->ev1, s));
     CCHK(hipGetLastError());
@@ -1056,6 +1222,12 @@ int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64
     h->last_rows = h->h_rowbase[n];
     h->last_fetch = h->h_foff[n];
     h->last_fail = fail;
+    h->last_fetch_base = h->fetches.size();
+    if (h->async) {
+        // the fetch rows go in flight; the registrations wait for their fetch
+        rc = record_async(h, n, keep, nreg);
+        if (rc) return rc;
+    }
     h->naccess += n;
     h->nhist += ng;
     h->gpu_seen = seen;
@@ -1084,7 +1256,7 @@ int xkrt_hip_mcc_parts_copy(xkrt_hip_mcc_t h, uint64_t *part_off, int64_t *parts
         o[0] = v[0][r]; o[1] = v[1][r];
         o[2] = (int64_t)(v[2][r] + h->cfg.y_lo); o[3] = (int64_t)(v[3][r] + h->cfg.y_lo);
         o[4] = v[4][r] & 3u;
-        o[5] = 0;  // fetching bits: every fetch completes before the next access
+        o[5] = (v[4][r] >> 2) & 3u;  // fetching bits before the access (0 with synchronous completion)
     }
     return 0;
 }
@@ -1127,6 +1299,112 @@ int owner_query(const xkrt_hip_mcc *h, const xkrt_hip_mcc_access_t &a)
 }
 
 } // namespace
+
+int xkrt_hip_mcc_awaits_copy(xkrt_hip_mcc_t h, uint64_t *awaits_off, int64_t *rows, uint64_t cap)
+{
+    if (!h) return -EINVAL;
+    if (!h->async) return -EINVAL;
+    const uint64_t nr = h->last_regs.size() / 8;
+    if (awaits_off)
+        for (uint32_t i = 0; i <= h->last_n; ++i)
+            awaits_off[i] = h->last_reg_off.size() > i ? (uint64_t)h->last_reg_off[i] : 0;
+    if (!rows || nr == 0) return 0;
+    if (cap < nr) return -ENOSPC;
+    memcpy(rows, h->last_regs.data(), 8 * sizeof(int64_t) * nr);
+    return 0;
+}
+
+int xkrt_hip_mcc_complete(xkrt_hip_mcc_t h, const uint64_t *fetch_ids, uint64_t n, uint64_t *nreleased,
+                          uint64_t *nforwards)
+{
+    if (!h || (!fetch_ids && n)) return -EINVAL;
+    if (!h->async) return -EINVAL;
+    if (h->poisoned) return -ECANCELED;
+    if (nreleased) *nreleased = 0;
+    if (nforwards) *nforwards = 0;
+    std::vector<uint32_t> q(6 * n);
+    for (uint64_t k = 0; k < n; ++k) {
+        const uint64_t id = fetch_ids[k];
+        if (id >= h->fetches.size() || !h->fetches[id].inflight) return -EINVAL;
+        for (uint64_t j = 0; j < k; ++j)
+            if (fetch_ids[j] == id) return -EINVAL;
+        const MccFetch &f = h->fetches[id];
+        const uint32_t row[6] = {f.u0, f.u1, f.y0, f.y1, f.dd, f.dc};
+        memcpy(q.data() + 6 * k, row, sizeof(row));
+    }
+    if (n) {
+        int rc = grow(&h->cq, &h->cqcap, 6 * n, false, h->stream);
+        if (rc) return rc;
+        CCHK(hipMemcpyAsync(h->cq, q.data(), 24ull * n, hipMemcpyHostToDevice, h->stream));
+        CArgs a = make_cargs(h, 0);
+        hipLaunchKernelGGL(c_complete, dim3((uint32_t)std::min<uint64_t>(n, 4096)), dim3(256), 0, h->stream, a, h->cq,
+                           (uint32_t)n);
+        CCHK(hipGetLastError());
+        CCHK(hipStreamSynchronize(h->stream));
+    }
+    // the registrations on each fetched view fire, in completion order then registration order
+    h->c_rel_off.assign(n + 1, 0);
+    h->c_fwd_off.assign(n + 1, 0);
+    h->c_rel.clear();
+    h->c_fwd.clear();
+    for (uint64_t k = 0; k < n; ++k) {
+        MccFetch &f = h->fetches[fetch_ids[k]];
+        f.inflight = false;
+        h->inflight--;
+        if (f.dd == 1) {
+            const uint32_t fx0 = f.u0 * (uint32_t)h->s, fx1 = f.u1 * (uint32_t)h->s;
+            size_t w = 0;
+            for (size_t i = 0; i < h->live.size(); ++i) {
+                const uint32_t id = h->live[i];
+                const MccReg &r = h->regs[id];
+                const bool hit = r.chunk == f.dc && r.x0 < fx1 && fx0 < r.x1 && r.y0 < f.y1 && f.y0 < r.y1;
+                if (!hit) {
+                    h->live[w++] = id;
+                    continue;
+                }
+                const int64_t y0 = (int64_t)(r.y0 + h->cfg.y_lo), y1 = (int64_t)(r.y1 + h->cfg.y_lo);
+                if (r.kind == 2) {
+                    const int64_t row[7] = {(int64_t)r.acc, r.x0, r.x1, y0, y1, r.chunk, id};
+                    h->c_rel.insert(h->c_rel.end(), row, row + 7);
+                } else {
+                    // the forward copy: from the view just fetched to the forward's view
+                    const uint64_t fid = h->fetches.size();
+                    const int64_t row[9] = {(int64_t)fid, r.x0, r.x1, y0, y1, 1, r.chunk, 1, r.dst};
+                    h->c_fwd.insert(h->c_fwd.end(), row, row + 9);
+                    h->fetches.push_back({r.x0 / (uint32_t)h->s, r.x1 / (uint32_t)h->s, r.y0, r.y1, 1u, r.dst, true});
+                    h->inflight++;
+                }
+            }
+            h->live.resize(w);
+        }
+        h->c_rel_off[k + 1] = (int64_t)(h->c_rel.size() / 7);
+        h->c_fwd_off[k + 1] = (int64_t)(h->c_fwd.size() / 9);
+    }
+    if (nreleased) *nreleased = h->c_rel.size() / 7;
+    if (nforwards) *nforwards = h->c_fwd.size() / 9;
+    return 0;
+}
+
+int xkrt_hip_mcc_completed_copy(xkrt_hip_mcc_t h, uint64_t *rel_off, int64_t *released, uint64_t rel_cap,
+                                uint64_t *fwd_off, int64_t *forwards, uint64_t fwd_cap)
+{
+    if (!h) return -EINVAL;
+    const uint64_t nc = h->c_rel_off.empty() ? 0 : h->c_rel_off.size() - 1;
+    if (rel_off)
+        for (uint64_t k = 0; k <= nc; ++k) rel_off[k] = nc ? (uint64_t)h->c_rel_off[k] : 0;
+    if (fwd_off)
+        for (uint64_t k = 0; k <= nc; ++k) fwd_off[k] = nc ? (uint64_t)h->c_fwd_off[k] : 0;
+    const uint64_t nr = h->c_rel.size() / 7, nf = h->c_fwd.size() / 9;
+    if (released && nr) {
+        if (rel_cap < nr) return -ENOSPC;
+        memcpy(released, h->c_rel.data(), 7 * sizeof(int64_t) * nr);
+    }
+    if (forwards && nf) {
+        if (fwd_cap < nf) return -ENOSPC;
+        memcpy(forwards, h->c_fwd.data(), 9 * sizeof(int64_t) * nf);
+    }
+    return 0;
+}
 
 int xkrt_hip_mcc_who_owns_static(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64_t n, uint32_t *owners)
 {
@@ -1203,6 +1481,9 @@ int xkrt_hip_mcc_stats(xkrt_hip_mcc_t h, xkrt_hip_mcc_stats_t *out)
     out->last_ms = h->last_ms;
     out->device_bytes = h->device_bytes;
     out->poisoned = h->poisoned ? 1 : 0;
+    out->last_fetch_base = h->last_fetch_base;
+    out->in_flight = h->inflight;
+    out->registrations = h->live.size();
     return 0;
 }
 
