@@ -461,6 +461,32 @@ int xkrt_hip_mcc_complete(xkrt_hip_mcc_t h, const uint64_t *fetch_ids, uint64_t 
 int xkrt_hip_mcc_completed_copy(xkrt_hip_mcc_t h, uint64_t *rel_off, int64_t *released, uint64_t rel_cap,
                                 uint64_t *fwd_off, int64_t *forwards, uint64_t fwd_cap);
 
+/* Fetch execution (trees with async_completion): the copies behind the fetch
+ * rows, as the reference's HIP driver issues them per fetch with
+ * hipMemcpyParam2DAsync (src/driver/driver_hip.cc:661-751, queue.cc:165-284).
+ * xkrt_hip_mcc_exec_bind gives the tree its memory: host_base = host address
+ * of byte 0 of column y_lo of the matrix (ld * sizeof_type bytes per column),
+ * pinned memory the device can map (hipHostMalloc); arena = device memory for
+ * the GPU chunks (arena_bytes), each chunk placed there on first use and
+ * holding its allocating access's tile column-major (pitch = its rows times
+ * sizeof_type). */
+int xkrt_hip_mcc_exec_bind(xkrt_hip_mcc_t h, void *host_base, void *arena, uint64_t arena_bytes);
+
+/* Enqueues the copies of n fetches in flight on `stream` (a hipStream_t), one
+ * kernel for the batch: host to GPU, GPU to GPU (another view, forwards), GPU
+ * to host, each the 2D copy of its fetch's rect.  It completes nothing: once
+ * the stream has run them, pass the same ids to xkrt_hip_mcc_complete.
+ * -ENOMEM: the arena cannot place a chunk; -EINVAL: an id not in flight. */
+int xkrt_hip_mcc_exec(xkrt_hip_mcc_t h, const uint64_t *fetch_ids, uint64_t n, void *stream);
+
+/* The GPU chunk each access of the last plan call reads or writes through
+ * (its allocation view, memory-tree.hpp:1293-1450), -1 for host accesses. */
+int xkrt_hip_mcc_access_chunks(xkrt_hip_mcc_t h, int64_t *chunks);
+
+/* Device address of a chunk (placed in the arena on first use), its pitch in
+ * bytes, and the rect it holds (x0, x1 bytes; y0, y1 columns). */
+int xkrt_hip_mcc_chunk_ptr(xkrt_hip_mcc_t h, uint32_t chunk, void **ptr, uint64_t *pitch, int64_t *rect);
+
 /* who_owns for n rects on the current state: device bitfield owning the
  * most bytes (ties: several bits; 0 when nobody owns a byte). */
 int xkrt_hip_mcc_who_owns(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64_t n, uint32_t *owners);

@@ -60,6 +60,10 @@ def mcc_lib():
     L.xkrt_hip_mcc_awaits_copy.argtypes = [_p, _p, _p, _u64]
     L.xkrt_hip_mcc_complete.argtypes = [_p, _p, _u64, C.POINTER(_u64), C.POINTER(_u64)]
     L.xkrt_hip_mcc_completed_copy.argtypes = [_p, _p, _p, _u64, _p, _p, _u64]
+    L.xkrt_hip_mcc_exec_bind.argtypes = [_p, _p, _p, _u64]
+    L.xkrt_hip_mcc_exec.argtypes = [_p, _p, _u64, _p]
+    L.xkrt_hip_mcc_access_chunks.argtypes = [_p, _p]
+    L.xkrt_hip_mcc_chunk_ptr.argtypes = [_p, _u32, C.POINTER(_p), C.POINTER(_u64), _p]
     _BOUND = L
     return L
 
@@ -174,6 +178,29 @@ class MemoryTree:
         _chk(self.L.xkrt_hip_mcc_completed_copy(self.h, ro.ctypes.data, rel.ctypes.data, len(rel), fo.ctypes.data,
                                                 fwd.ctypes.data, len(fwd)), "xkrt_hip_mcc_completed_copy")
         return ro.astype(np.int64), rel[:nr.value], fo.astype(np.int64), fwd[:nf.value]
+
+    def exec_bind(self, host_ptr: int, arena_ptr: int, arena_bytes: int):
+        """Memory the fetches move: the host matrix (pinned, from column y_lo) and the chunk arena."""
+        _chk(self.L.xkrt_hip_mcc_exec_bind(self.h, host_ptr, arena_ptr, arena_bytes), "xkrt_hip_mcc_exec_bind")
+
+    def exec(self, ids, stream: int):
+        """Enqueue the copies of these fetches in flight on `stream` (one kernel)."""
+        ids = np.ascontiguousarray(ids, np.uint64)
+        _chk(self.L.xkrt_hip_mcc_exec(self.h, ids.ctypes.data, len(ids), stream or None), "xkrt_hip_mcc_exec")
+
+    def access_chunks(self, n: int) -> np.ndarray:
+        """The GPU chunk of each access of the last plan call (-1: host)."""
+        out = np.zeros(max(n, 1), np.int64)
+        _chk(self.L.xkrt_hip_mcc_access_chunks(self.h, out.ctypes.data), "xkrt_hip_mcc_access_chunks")
+        return out[:n]
+
+    def chunk_ptr(self, chunk: int):
+        """(device address, pitch bytes, rect (x0, x1 bytes, y0, y1 columns)) of a chunk."""
+        ptr, pitch = _p(), _u64()
+        rect = np.zeros(4, np.int64)
+        _chk(self.L.xkrt_hip_mcc_chunk_ptr(self.h, chunk, C.byref(ptr), C.byref(pitch), rect.ctypes.data),
+             "xkrt_hip_mcc_chunk_ptr")
+        return int(ptr.value or 0), int(pitch.value), rect
 
     def who_owns(self, addr, m, n) -> np.ndarray:
         q = to_mcc_array(np.zeros(len(addr), np.uint8), np.zeros(len(addr), np.uint8), addr, m, n)

@@ -44,6 +44,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <mutex>
 #include <vector>
 
@@ -581,6 +582,37 @@ __global__ void __launch_bounds__(256) c_complete(CArgs a, const uint32_t *q, ui
     }
 }
 
+// ---------------------------------------------------------------- fetch execution
+
+// One 2D copy per fetch (the reference issues hipMemcpyParam2DAsync per fetch,
+// src/driver/driver_hip.cc:661-751): `height` columns of `width` bytes from
+// src (pitch spitch) to dst (pitch dpitch).  Host memory is reached through
+// its device mapping; every fetch of a batch goes in one launch, a block per
+// fetch at a time, 16-byte moves when all of a fetch's addresses allow them.
+struct CopyJob {
+    unsigned long long src, dst;
+    uint32_t spitch, dpitch, width, height;
+};
+static_assert(sizeof(CopyJob) == 32, "copy jobs are 32 B");
+
+__global__ void __launch_bounds__(256) c_exec(const CopyJob *jobs, uint32_t njobs)
+{
+    for (uint32_t j = blockIdx.x; j < njobs; j += gridDim.x) {
+        const CopyJob c = jobs[j];
+        const bool v16 = ((c.src | c.dst | c.spitch | c.dpitch | c.width) & 15ull) == 0;
+        const uint32_t unit = v16 ? 16u : 1u;
+        const uint32_t per_col = c.width / unit;
+        const uint64_t total = (uint64_t)per_col * c.height;
+        for (uint64_t t = threadIdx.x; t < total; t += blockDim.x) {
+            const uint32_t col = (uint32_t)(t / per_col), k = (uint32_t)(t % per_col);
+            const unsigned long long so = c.src + (unsigned long long)col * c.spitch + (unsigned long long)k * unit;
+            const unsigned long long d = c.dst + (unsigned long long)col * c.dpitch + (unsigned long long)k * unit;
+            if (v16) *reinterpret_cast<uint4 *>(d) = *reinterpret_cast<const uint4 *>(so);
+            else *reinterpret_cast<unsigned char *>(d) = *reinterpret_cast<const unsigned char *>(so);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- who_owns
 
 // memory-tree.hpp:1903-1933 (SEARCH_OWNERS 2130-2139): bytes per coherent
@@ -808,6 +840,7 @@ struct MccFetch {
     uint32_t u0, u1, y0, y1;  // rect: x in sizeof_type units, y in frame columns
     uint32_t dd, dc;          // destination device and chunk (NONE: host memory)
     bool inflight;
+    uint32_t sd = 0, sc = NONE;  // source device and chunk
 };
 struct MccReg {
     uint64_t acc;             // access index in the tree's order
@@ -868,6 +901,19 @@ struct xkrt_hip_mcc {
     uint64_t last_fetch_base = 0;
     std::vector<int64_t> last_reg_off, last_regs;                 // per access of the last plan call
     std::vector<int64_t> c_rel_off, c_rel, c_fwd_off, c_fwd;      // the last completion call
+    // chunk geometry: the allocating access's rect (x in units, y in frame columns), by chunk id
+    std::vector<std::array<uint32_t, 4>> chunk_rect;
+    std::vector<int64_t> last_chunks;  // per access of the last plan call: its GPU chunk, -1 on the host
+    // fetch execution: the host matrix (device mapping), the chunk arena, placements, job buffers
+    unsigned char *host_dev = nullptr;
+    unsigned char *arena = nullptr;
+    uint64_t arena_bytes = 0, arena_used = 0;
+    std::vector<uint64_t> chunk_off;   // NONE64: not placed yet
+    CopyJob *jobs[2] = {};             // pinned, mapped; used in turn
+    uint64_t jobs_cap[2] = {};
+    hipEvent_t jobs_ev[2] = {};
+    bool jobs_pending[2] = {};
+    uint32_t jobs_cur = 0;
     // results of the last plan call
     uint32_t last_n = 0, last_rows = 0, last_fetch = 0;
     int64_t last_fail = -1;
@@ -925,6 +971,13 @@ void free_all(xkrt_hip_mcc *h)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (h->ctr_host) (void)hipHostFree(h->ctr_host);
+    for (int b = 0; b < 2; ++b) {
+        if (h->jobs_ev[b]) {
+            if (h->jobs_pending[b]) (void)hipEventSynchronize(h->jobs_ev[b]);
+            (void)hipEventDestroy(h->jobs_ev[b]);
+        }
+        if (h->jobs[b]) (void)hipHostFree(h->jobs[b]);
+    }
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
 }
@@ -1019,15 +1072,18 @@ int record_async(xkrt_hip_mcc *h, uint64_t n, uint64_t keep, uint32_t nreg)
 {
     const uint32_t nf = h->h_foff[keep];
     if (nf) {
-        std::vector<uint32_t> v[6];
-        uint32_t *src[6] = {h->fx0, h->fx1, h->fy0, h->fy1, h->fdd, h->fdc};
-        for (int k = 0; k < 6; ++k) {
+        std::vector<uint32_t> v[8];
+        uint32_t *src[8] = {h->fx0, h->fx1, h->fy0, h->fy1, h->fdd, h->fdc, h->fsd, h->fsc};
+        for (int k = 0; k < 8; ++k) {
             v[k].resize(nf);
             CCHK(hipMemcpy(v[k].data(), src[k], 4ull * nf, hipMemcpyDeviceToHost));
         }
-        for (uint32_t r = 0; r < nf; ++r)
-            h->fetches.push_back({v[0][r] / (uint32_t)h->s, v[1][r] / (uint32_t)h->s, v[2][r], v[3][r], v[4][r],
-                                  v[5][r], true});
+        for (uint32_t r = 0; r < nf; ++r) {
+            MccFetch f{v[0][r] / (uint32_t)h->s, v[1][r] / (uint32_t)h->s, v[2][r], v[3][r], v[4][r], v[5][r], true};
+            f.sd = v[6][r];
+            f.sc = v[7][r];
+            h->fetches.push_back(f);
+        }
         h->inflight += nf;
     }
     h->last_reg_off.assign(n + 1, 0);
@@ -1223,6 +1279,19 @@ int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64
     h->last_fetch = h->h_foff[n];
     h->last_fail = fail;
     h->last_fetch_base = h->fetches.size();
+    h->last_chunks.assign(n, -1);
+    if (ng) {  // geometry of the chunks this call allocated (its allocating accesses' rects)
+        std::vector<uint32_t> beta(ng), chk(ng);
+        CCHK(hipMemcpy(beta.data(), h->hbeta + h->nhist, 4ull * ng, hipMemcpyDeviceToHost));
+        CCHK(hipMemcpy(chk.data(), h->hchunk + h->nhist, 4ull * ng, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < ng; ++k) {
+            if (beta[k] != NONE) continue;
+            if (h->chunk_rect.size() <= chk[k]) h->chunk_rect.resize(chk[k] + 1, {0, 0, 0, 0});
+            h->chunk_rect[chk[k]] = {gx0[k], gx1[k], gy0[k], gy1[k]};
+        }
+        for (uint64_t i = 0; i < n; ++i)
+            if (hist[i] != NONE) h->last_chunks[i] = chk[hist[i] - h->nhist];
+    }
     if (h->async) {
         // the fetch rows go in flight; the registrations wait for their fetch
         rc = record_async(h, n, keep, nreg);
@@ -1371,7 +1440,10 @@ int xkrt_hip_mcc_complete(xkrt_hip_mcc_t h, const uint64_t *fetch_ids, uint64_t 
                     const uint64_t fid = h->fetches.size();
                     const int64_t row[9] = {(int64_t)fid, r.x0, r.x1, y0, y1, 1, r.chunk, 1, r.dst};
                     h->c_fwd.insert(h->c_fwd.end(), row, row + 9);
-                    h->fetches.push_back({r.x0 / (uint32_t)h->s, r.x1 / (uint32_t)h->s, r.y0, r.y1, 1u, r.dst, true});
+                    MccFetch nf{r.x0 / (uint32_t)h->s, r.x1 / (uint32_t)h->s, r.y0, r.y1, 1u, r.dst, true};
+                    nf.sd = 1;
+                    nf.sc = r.chunk;
+                    h->fetches.push_back(nf);
                     h->inflight++;
                 }
             }
@@ -1402,6 +1474,129 @@ int xkrt_hip_mcc_completed_copy(xkrt_hip_mcc_t h, uint64_t *rel_off, int64_t *re
     if (forwards && nf) {
         if (fwd_cap < nf) return -ENOSPC;
         memcpy(forwards, h->c_fwd.data(), 9 * sizeof(int64_t) * nf);
+    }
+    return 0;
+}
+
+namespace {
+
+constexpr uint64_t NONE64 = ~0ull;
+
+// device address of (x bytes, y frame column) in chunk c, placing the chunk in the arena on first use
+int chunk_addr(xkrt_hip_mcc *h, uint32_t c, uint32_t x, uint32_t y, unsigned long long *addr, uint32_t *pitch)
+{
+    if (c >= h->chunk_rect.size()) return -EIO;
+    const std::array<uint32_t, 4> &r = h->chunk_rect[c];
+    const uint64_t w = (uint64_t)(r[1] - r[0]) * h->s, bytes = w * (r[3] - r[2]);
+    if (h->chunk_off.size() <= c) h->chunk_off.resize(c + 1, NONE64);
+    if (h->chunk_off[c] == NONE64) {
+        const uint64_t off = (h->arena_used + 255) & ~255ull;
+        if (off + bytes > h->arena_bytes) return -ENOMEM;
+        h->chunk_off[c] = off;
+        h->arena_used = off + bytes;
+    }
+    if (x < r[0] * h->s || y < r[2]) return -EIO;
+    *addr = (unsigned long long)(uintptr_t)(h->arena + h->chunk_off[c]) + (unsigned long long)(y - r[2]) * w +
+            (x - r[0] * h->s);
+    *pitch = (uint32_t)w;
+    return 0;
+}
+
+// the end of a fetch's rect must stay inside its chunk (a fetch that does not fit is refused)
+bool rect_in_chunk(const xkrt_hip_mcc *h, uint32_t c, const MccFetch &f)
+{
+    if (c >= h->chunk_rect.size()) return false;
+    const std::array<uint32_t, 4> &r = h->chunk_rect[c];
+    return r[0] <= f.u0 && f.u1 <= r[1] && r[2] <= f.y0 && f.y1 <= r[3];
+}
+
+} // namespace
+
+int xkrt_hip_mcc_exec_bind(xkrt_hip_mcc_t h, void *host_base, void *arena, uint64_t arena_bytes)
+{
+    if (!h || !host_base || (!arena && arena_bytes)) return -EINVAL;
+    void *dev = nullptr;
+    if (hipHostGetDevicePointer(&dev, host_base, 0) != hipSuccess || !dev) return -EINVAL;  // must be pinned
+    h->host_dev = (unsigned char *)dev;
+    h->arena = (unsigned char *)arena;
+    h->arena_bytes = arena_bytes;
+    h->arena_used = 0;
+    h->chunk_off.clear();
+    return 0;
+}
+
+int xkrt_hip_mcc_exec(xkrt_hip_mcc_t h, const uint64_t *fetch_ids, uint64_t n, void *stream)
+{
+    if (!h || (!fetch_ids && n)) return -EINVAL;
+    if (!h->async || !h->host_dev) return -EINVAL;
+    if (n == 0) return 0;
+    // the job buffer used two calls ago must have been read
+    const uint32_t b = h->jobs_cur;
+    if (h->jobs_pending[b]) {
+        CCHK(hipEventSynchronize(h->jobs_ev[b]));
+        h->jobs_pending[b] = false;
+    }
+    if (h->jobs_cap[b] < n) {
+        if (h->jobs[b]) CCHK(hipHostFree(h->jobs[b]));
+        h->jobs[b] = nullptr;
+        h->jobs_cap[b] = 0;
+        CCHK(hipHostMalloc(&h->jobs[b], sizeof(CopyJob) * n, hipHostMallocCoherent));
+        h->jobs_cap[b] = n;
+    }
+    if (!h->jobs_ev[b]) CCHK(hipEventCreateWithFlags(&h->jobs_ev[b], hipEventDisableTiming));
+    for (uint64_t k = 0; k < n; ++k) {
+        const uint64_t id = fetch_ids[k];
+        if (id >= h->fetches.size() || !h->fetches[id].inflight) return -EINVAL;
+        const MccFetch &f = h->fetches[id];
+        const uint32_t x0 = f.u0 * (uint32_t)h->s;
+        CopyJob &j = h->jobs[b][k];
+        j.width = (f.u1 - f.u0) * (uint32_t)h->s;
+        j.height = f.y1 - f.y0;
+        auto end = [&](uint32_t dev, uint32_t chunk, unsigned long long *addr, uint32_t *pitch) -> int {
+            if (dev == 0) {
+                *addr = (unsigned long long)(uintptr_t)(h->host_dev + (uint64_t)f.y0 * h->LDs + x0);
+                *pitch = (uint32_t)h->LDs;
+                return 0;
+            }
+            if (!rect_in_chunk(h, chunk, f)) return -EIO;
+            return chunk_addr(h, chunk, x0, f.y0, addr, pitch);
+        };
+        int rc = end(f.sd, f.sc, &j.src, &j.spitch);
+        if (!rc) rc = end(f.dd, f.dc, &j.dst, &j.dpitch);
+        if (rc) return rc;
+    }
+    CopyJob *dj = nullptr;
+    CCHK(hipHostGetDevicePointer((void **)&dj, h->jobs[b], 0));
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(c_exec, dim3((uint32_t)std::min<uint64_t>(n, 8192)), dim3(256), 0, st, dj, (uint32_t)n);
+    CCHK(hipGetLastError());
+    CCHK(hipEventRecord(h->jobs_ev[b], st));
+    h->jobs_pending[b] = true;
+    h->jobs_cur = b ^ 1u;
+    return 0;
+}
+
+int xkrt_hip_mcc_access_chunks(xkrt_hip_mcc_t h, int64_t *chunks)
+{
+    if (!h || !chunks) return -EINVAL;
+    for (size_t i = 0; i < h->last_chunks.size(); ++i) chunks[i] = h->last_chunks[i];
+    return 0;
+}
+
+int xkrt_hip_mcc_chunk_ptr(xkrt_hip_mcc_t h, uint32_t chunk, void **ptr, uint64_t *pitch, int64_t *rect)
+{
+    if (!h || !ptr || !h->arena) return -EINVAL;
+    if (chunk >= h->chunk_rect.size()) return -EINVAL;
+    const std::array<uint32_t, 4> &r = h->chunk_rect[chunk];
+    unsigned long long a = 0;
+    uint32_t p = 0;
+    const int rc = chunk_addr(h, chunk, r[0] * (uint32_t)h->s, r[2], &a, &p);
+    if (rc) return rc;
+    *ptr = (void *)(uintptr_t)a;
+    if (pitch) *pitch = p;
+    if (rect) {
+        rect[0] = (int64_t)r[0] * (int64_t)h->s; rect[1] = (int64_t)r[1] * (int64_t)h->s;
+        rect[2] = (int64_t)(r[2] + h->cfg.y_lo); rect[3] = (int64_t)(r[3] + h->cfg.y_lo);
     }
     return 0;
 }
