@@ -165,3 +165,15 @@ def coherency_events(dev, mode, rects, nrect, chunk_bytes, ev_type, ev_arg):
         if rc <= cap:
             return rows[:rc].copy(), int(fail[0])
         cap = int(rc)
+
+
+def resolve_intervals(task, mode, seg):
+    """Segment (K = 1) domain: each access a byte segment [a, b) (seg (n, 2)), the
+    same LINK / PUT rule over one dimension -- the 2D restatement with every
+    segment as [a, b) x [0, 1)."""
+    seg = np.asarray(seg, np.uint64).reshape(-1, 2)
+    rects = np.zeros((len(seg), 8), np.uint64)
+    rects[:, 0] = seg[:, 0]
+    rects[:, 1] = seg[:, 1]
+    rects[:, 3] = 1
+    return resolve(task, mode, rects, np.ones(len(seg), np.uint8))

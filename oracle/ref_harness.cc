@@ -130,6 +130,99 @@ class DepTree : public DepBase {
     }
 };
 
+// mode flag (above the access modes) of an access replayed without a LINK
+constexpr uint8_t PUT_ONLY = 0x40;
+
+// The segment (K = 1) tree: the hooks of IntervalDependencyTree
+// (include/xkrt/memory/access/interval/dependency-tree.hpp:229-355) restated
+// over the reference's KHPTree<1>: a node is marked when the segment
+// intersects it, the LINK rule is the BLAS tree's.
+struct IAcc;
+using IRect = KHyperrect<1>;
+struct IAcc {
+    uint32_t idx;
+    uint32_t task;
+    uint8_t mode;
+    IRect seg;
+    std::vector<IAcc *> successors;
+};
+struct ISearch {
+    IAcc *access;
+};
+using IntBase = KHPTree<1, ISearch>;
+
+class IntNode : public IntBase::Node {
+    using Base = IntBase::Node;
+  public:
+    std::vector<IAcc *> last_reads;
+    IAcc *last_write;
+    int nwrites;
+    IntNode(const IRect &h, int k, Color c) : Base(h, k, c), last_reads(), last_write(nullptr), nwrites(0) {}
+    IntNode(const IRect &h, int k, Color c, const IntNode *inherit)
+        : Base(h, k, c), last_reads(inherit->last_reads), last_write(inherit->last_write), nwrites(0) {}
+    void update_includes(void) override
+    {
+        Base::update_includes();
+        nwrites = last_write ? 1 : 0;
+        for (int d = 0; d < 2; ++d) {
+            IntNode *c = reinterpret_cast<IntNode *>(this->st[0].children[d]);
+            if (c) nwrites += c->nwrites;
+        }
+    }
+};
+
+static inline void iaccess_precedes(IAcc *pred, IAcc *succ)
+{
+    if (!pred->successors.empty() && pred->successors.back()->task == succ->task)
+        return;
+    pred->successors.push_back(succ);
+}
+
+class IntTree : public IntBase {
+  public:
+    Node *new_node(ISearch &, const IRect &h, const int k, const Color c) const override { return new IntNode(h, k, c); }
+    Node *new_node(ISearch &, const IRect &h, const int k, const Color c, const Node *inherit) const override
+    {
+        return new IntNode(h, k, c, reinterpret_cast<const IntNode *>(inherit));
+    }
+    void on_insert(Node *nb, ISearch &s) override
+    {
+        IntNode *node = reinterpret_cast<IntNode *>(nb);
+        if (s.access->seg.intersects(node->hyperrect)) {
+            if (s.access->mode & MODE_W) {
+                node->last_reads.clear();
+                node->last_write = s.access;
+            } else if (s.access->mode == MODE_R) {
+                node->last_reads.push_back(s.access);
+            }
+        }
+    }
+    void on_shrink(Node *, const Interval &, int) override {}
+    bool intersect_stop_test(Node *nb, ISearch &s, const IRect &) const override
+    {
+        return s.access->mode == MODE_R && reinterpret_cast<IntNode *>(nb)->nwrites == 0;
+    }
+    void on_intersect(Node *nb, ISearch &s, const IRect &) const override
+    {
+        IntNode *node = reinterpret_cast<IntNode *>(nb);
+        if ((s.access->mode & MODE_W) && !node->last_reads.empty()) {
+            for (IAcc *p : node->last_reads) iaccess_precedes(p, s.access);
+        } else if (node->last_write) {
+            iaccess_precedes(node->last_write, s.access);
+        }
+    }
+    void link(IAcc *a)
+    {
+        ISearch s{a};
+        intersect(s, a->seg);
+    }
+    void put(IAcc *a)
+    {
+        ISearch s{a};
+        insert(s, a->seg);
+    }
+};
+
 // Plain fragment tree used for the memory-tree partition (hooks are no-ops,
 // as memory-tree.hpp's INSERTING_BLOCKS on_insert is).
 struct PartSearch { std::vector<Rect> *out; };
@@ -247,7 +340,7 @@ int64_t khpref_resolve_marks(uint64_t nacc, const uint32_t *task, const uint8_t 
     for (uint64_t i = 0; i < nacc; ++i) {
         acc[i].idx = (uint32_t)i;
         acc[i].task = task[i];
-        acc[i].mode = mode[i];
+        acc[i].mode = mode[i] & (uint8_t)~PUT_ONLY;
         acc[i].nrect = nrect[i];
         acc[i].nmark = nmark[i];
         for (int r = 0; r < 3; ++r) {
@@ -261,13 +354,53 @@ int64_t khpref_resolve_marks(uint64_t nacc, const uint32_t *task, const uint8_t 
     while (i < nacc) {
         uint64_t j = i;
         while (j < nacc && task[j] == task[i]) ++j;
-        for (uint64_t a = i; a < j; ++a) tree.link(&acc[a]);
+        // an access flagged PUT_ONLY is replayed into the tree without a LINK: the segment
+        // accesses put_interval'd into a BLAS tree when it is created (src/task/task.cc:177-198)
+        for (uint64_t a = i; a < j; ++a)
+            if (!(mode[a] & PUT_ONLY)) tree.link(&acc[a]);
         for (uint64_t a = i; a < j; ++a) tree.put(&acc[a]);
         i = j;
     }
     uint64_t ne = 0;
     for (uint64_t p = 0; p < nacc; ++p)
         for (Acc *sc : acc[p].successors) {
+            if (ne < edge_cap) {
+                edges[2 * ne] = (uint32_t)p;
+                edges[2 * ne + 1] = sc->idx;
+            }
+            ne++;
+        }
+    return (int64_t)ne;
+}
+
+// Resolve one segment (K = 1) dependency domain: IntervalDependencyTree
+// (include/xkrt/memory/access/interval/dependency-tree.hpp:187-363) over
+// the reference's KHPTree<1>.  seg holds 2 u64 per access ([a, b) bytes).
+// Edges as khpref_resolve.
+int64_t khpref_resolve_intervals(uint64_t nacc, const uint32_t *task, const uint8_t *mode, const uint64_t *seg,
+                                 uint64_t edge_cap, uint32_t *edges)
+{
+    std::vector<IAcc> acc(nacc);
+    for (uint64_t i = 0; i < nacc; ++i) {
+        acc[i].idx = (uint32_t)i;
+        acc[i].task = task[i];
+        acc[i].mode = mode[i];
+        Interval l[1] = {Interval(seg[2 * i], seg[2 * i + 1])};
+        acc[i].seg = IRect(l);
+        acc[i].successors.reserve(8);
+    }
+    IntTree tree;
+    uint64_t i = 0;
+    while (i < nacc) {
+        uint64_t j = i;
+        while (j < nacc && task[j] == task[i]) ++j;
+        for (uint64_t a = i; a < j; ++a) tree.link(&acc[a]);
+        for (uint64_t a = i; a < j; ++a) tree.put(&acc[a]);
+        i = j;
+    }
+    uint64_t ne = 0;
+    for (uint64_t p = 0; p < nacc; ++p)
+        for (IAcc *sc : acc[p].successors) {
             if (ne < edge_cap) {
                 edges[2 * ne] = (uint32_t)p;
                 edges[2 * ne + 1] = sc->idx;

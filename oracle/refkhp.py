@@ -30,6 +30,8 @@ def lib():
         _lib.khpref_resolve_marks.argtypes = [u, p, p, p, p, p, p, u, p]
         _lib.khpref_resolve_marks.restype = C.c_int64
         _lib.khpref_partition.restype = C.c_int64
+        _lib.khpref_resolve_intervals.argtypes = [u, p, p, p, u, p]
+        _lib.khpref_resolve_intervals.restype = C.c_int64
     return _lib
 
 
@@ -76,6 +78,27 @@ def resolve_marks(task, mode, rects, nrect, marks, nmark):
         edges = np.empty((cap, 2), np.uint32)
         ne = L.khpref_resolve_marks(n, task.ctypes.data, mode.ctypes.data, nrect.ctypes.data, rects.ctypes.data,
                                     nmark.ctypes.data, marks.ctypes.data, cap, edges.ctypes.data)
+        if ne <= cap:
+            break
+        cap = int(ne)
+    return edges[:ne].copy()
+
+
+# mode flag of an access replayed without a LINK (resolve_marks): the segment
+# accesses put into a new BLAS tree (src/task/task.cc:177-198)
+PUT_ONLY = 0x40
+
+
+def resolve_intervals(task, mode, seg):
+    """The reference's segment (K = 1) tree over [a, b) byte segments (seg: (n, 2) u64)."""
+    L = lib()
+    n = len(task)
+    task = _c(task, np.uint32); mode = _c(mode, np.uint8); seg = _c(seg, np.uint64)
+    cap = max(1024, 4 * n)
+    while True:
+        edges = np.empty((cap, 2), np.uint32)
+        ne = L.khpref_resolve_intervals(n, task.ctypes.data, mode.ctypes.data, seg.ctypes.data, cap,
+                                        edges.ctypes.data)
         if ne <= cap:
             break
         cap = int(ne)
