@@ -177,3 +177,56 @@ def resolve_intervals(task, mode, seg):
     rects[:, 1] = seg[:, 1]
     rects[:, 3] = 1
     return resolve(task, mode, rects, np.ones(len(seg), np.uint8))
+
+
+def resolve_program(task, mode, kind, addr, m, n, ld, s, _mistake=None):
+    """A task domain over a mixed program (src/task/task.cc:153-299): segments
+    resolved by the segment domain (resolve_intervals); each (ld, s) pair's
+    BLAS domain, created at its first tile, resolves its tiles and the segments
+    that LINK into it (those after its creation), with their marking rects
+    (resolve_marks).  The segments put into a BLAS domain with no LINK (the
+    replay at creation, and those of the creating task before the tile) mark
+    nothing, since no access of the program touches the first ld * s bytes;
+    ValueError otherwise.  Edges of all domains in program indices, deduplicated
+    per (predecessor, successor task) to the lowest successor (__access_precedes,
+    task.hpp:494-512; one successor list per access across its domains).
+    _mistake plants an error for the tests that show the comparison catches it:
+    "segments_skip_blas" gives no segment to the BLAS domains, "no_segment_domain"
+    drops the segment domain's edges."""
+    from . import segments as G
+    task = np.asarray(task, np.uint32)
+    kind = np.asarray(kind, np.uint8)
+    addr = np.asarray(addr, np.uint64)
+    m = np.asarray(m, np.uint64)
+    n = np.asarray(n, np.uint64)
+    ld = np.asarray(ld, np.uint64)
+    s = np.asarray(s, np.uint64)
+    mode = np.asarray(mode, np.uint8)
+    out = []
+    seg = np.flatnonzero(kind == 1)
+    if len(seg) and _mistake != "no_segment_domain":
+        e = resolve_intervals(task[seg], mode[seg], np.stack([addr[seg], addr[seg] + m[seg]], axis=1))
+        out.append(seg[np.asarray(e, np.int64)])
+    tiles = np.flatnonzero(kind == 0)
+    pairs = []
+    for i in tiles:
+        key = (int(ld[i]), int(s[i]))
+        if key not in [p[0] for p in pairs]:
+            pairs.append((key, int(i)))
+    for (pld, ps), first in pairs:
+        if (addr < np.uint64(pld * ps)).any():
+            raise ValueError("an access in the first ld * s bytes: the replayed segments would mark nodes")
+        late_seg = (kind == 1) & (np.arange(len(kind)) > first) & (_mistake != "segments_skip_blas")
+        mine = np.flatnonzero(((kind == 0) & (ld == pld) & (s == ps)) | late_seg)
+        if not len(mine):
+            continue
+        r, nr, mk, nmk = G.stream_rects(kind[mine], addr[mine], m[mine], n[mine], pld, ps)
+        e = resolve_marks(task[mine], mode[mine], r, nr, mk, nmk)
+        out.append(mine[np.asarray(e, np.int64)])
+    if not out:
+        return np.zeros((0, 2), np.int64)
+    e = np.concatenate(out).reshape(-1, 2)
+    key = np.stack([e[:, 0], task[e[:, 1]].astype(np.int64), e[:, 1]], axis=1)
+    key = key[np.lexsort((key[:, 2], key[:, 1], key[:, 0]))]
+    first = np.r_[True, (np.diff(key[:, 0]) != 0) | (np.diff(key[:, 1]) != 0)]
+    return key[first][:, [0, 2]]

@@ -21,16 +21,25 @@
 
 #include <xkrt/memory/access/common/khp-tree.hpp>
 
+#include <algorithm>
 #include <cstdint>
 #include <ctime>
+#include <memory>
 #include <vector>
 
 namespace {
 
 enum : uint8_t { MODE_R = 1, MODE_W = 2 };
 
-struct Acc;
 using Rect = KHyperrect<2>;
+
+// an edge target: the successor access's task and index.  An access's list
+// may be shared by its views in several trees (one program: the interval tree
+// and the BLAS trees), so __access_precedes sees every edge of the access.
+struct Succ {
+    uint32_t task, idx;
+};
+using SuccList = std::vector<Succ>;
 
 struct Acc {
     uint32_t idx;
@@ -40,7 +49,8 @@ struct Acc {
     Rect rects[3];  // inserted and intersected (dependency-tree.hpp:366-426)
     int nmark;
     Rect marks[3];  // what on_insert tests a node against: access->rects() (dependency-tree.hpp:241-256)
-    std::vector<Acc *> successors;
+    SuccList own;
+    SuccList *succ = nullptr;  // shared list, else own
 };
 
 struct DepSearch {
@@ -72,11 +82,15 @@ class DepNode : public DepBase::Node {
     }
 };
 
+static inline void precedes(SuccList &l, uint32_t task, uint32_t idx)
+{
+    if (!l.empty() && l.back().task == task) return;
+    l.push_back({task, idx});
+}
+
 static inline void access_precedes(Acc *pred, Acc *succ)
 {
-    if (!pred->successors.empty() && pred->successors.back()->task == succ->task)
-        return;
-    pred->successors.push_back(succ);
+    precedes(pred->succ ? *pred->succ : pred->own, succ->task, succ->idx);
 }
 
 class DepTree : public DepBase {
@@ -137,14 +151,14 @@ constexpr uint8_t PUT_ONLY = 0x40;
 // (include/xkrt/memory/access/interval/dependency-tree.hpp:229-355) restated
 // over the reference's KHPTree<1>: a node is marked when the segment
 // intersects it, the LINK rule is the BLAS tree's.
-struct IAcc;
 using IRect = KHyperrect<1>;
 struct IAcc {
     uint32_t idx;
     uint32_t task;
     uint8_t mode;
     IRect seg;
-    std::vector<IAcc *> successors;
+    SuccList own;
+    SuccList *succ = nullptr;
 };
 struct ISearch {
     IAcc *access;
@@ -173,9 +187,7 @@ class IntNode : public IntBase::Node {
 
 static inline void iaccess_precedes(IAcc *pred, IAcc *succ)
 {
-    if (!pred->successors.empty() && pred->successors.back()->task == succ->task)
-        return;
-    pred->successors.push_back(succ);
+    precedes(pred->succ ? *pred->succ : pred->own, succ->task, succ->idx);
 }
 
 class IntTree : public IntBase {
@@ -300,7 +312,7 @@ int64_t khpref_resolve(uint64_t nacc, const uint32_t *task, const uint8_t *mode,
         acc[i].nmark = acc[i].nrect;
         acc[i].marks[0] = acc[i].rects[0];
         acc[i].marks[1] = acc[i].rects[1];
-        acc[i].successors.reserve(8);
+        acc[i].own.reserve(8);
     }
     DepTree tree;
     uint64_t i = 0, tix = 0;
@@ -318,10 +330,10 @@ int64_t khpref_resolve(uint64_t nacc, const uint32_t *task, const uint8_t *mode,
     }
     uint64_t ne = 0;
     for (uint64_t p = 0; p < nacc; ++p)
-        for (Acc *s : acc[p].successors) {
+        for (const Succ &s : acc[p].own) {
             if (ne < edge_cap) {
                 edges[2 * ne] = (uint32_t)p;
-                edges[2 * ne + 1] = s->idx;
+                edges[2 * ne + 1] = s.idx;
             }
             ne++;
         }
@@ -347,7 +359,7 @@ int64_t khpref_resolve_marks(uint64_t nacc, const uint32_t *task, const uint8_t 
             acc[i].rects[r] = make_rect(rects + 12 * i + 4 * r);
             acc[i].marks[r] = make_rect(marks + 12 * i + 4 * r);
         }
-        acc[i].successors.reserve(8);
+        acc[i].own.reserve(8);
     }
     DepTree tree;
     uint64_t i = 0;
@@ -363,10 +375,10 @@ int64_t khpref_resolve_marks(uint64_t nacc, const uint32_t *task, const uint8_t 
     }
     uint64_t ne = 0;
     for (uint64_t p = 0; p < nacc; ++p)
-        for (Acc *sc : acc[p].successors) {
+        for (const Succ &sc : acc[p].own) {
             if (ne < edge_cap) {
                 edges[2 * ne] = (uint32_t)p;
-                edges[2 * ne + 1] = sc->idx;
+                edges[2 * ne + 1] = sc.idx;
             }
             ne++;
         }
@@ -387,7 +399,7 @@ int64_t khpref_resolve_intervals(uint64_t nacc, const uint32_t *task, const uint
         acc[i].mode = mode[i];
         Interval l[1] = {Interval(seg[2 * i], seg[2 * i + 1])};
         acc[i].seg = IRect(l);
-        acc[i].successors.reserve(8);
+        acc[i].own.reserve(8);
     }
     IntTree tree;
     uint64_t i = 0;
@@ -400,10 +412,10 @@ int64_t khpref_resolve_intervals(uint64_t nacc, const uint32_t *task, const uint
     }
     uint64_t ne = 0;
     for (uint64_t p = 0; p < nacc; ++p)
-        for (IAcc *sc : acc[p].successors) {
+        for (const Succ &sc : acc[p].own) {
             if (ne < edge_cap) {
                 edges[2 * ne] = (uint32_t)p;
-                edges[2 * ne + 1] = sc->idx;
+                edges[2 * ne + 1] = sc.idx;
             }
             ne++;
         }
@@ -425,7 +437,7 @@ uint64_t khpref_time(uint64_t nacc, const uint32_t *task, const uint8_t *mode, c
         acc[i].nmark = acc[i].nrect;
         acc[i].marks[0] = acc[i].rects[0];
         acc[i].marks[1] = acc[i].rects[1];
-        acc[i].successors.reserve(8);
+        acc[i].own.reserve(8);
     }
     DepTree tree;
     uint64_t t0 = now_ns();
@@ -439,7 +451,7 @@ uint64_t khpref_time(uint64_t nacc, const uint32_t *task, const uint8_t *mode, c
     }
     uint64_t dt = now_ns() - t0;
     uint64_t ne = 0;
-    for (uint64_t p = 0; p < nacc; ++p) ne += acc[p].successors.size();
+    for (uint64_t p = 0; p < nacc; ++p) ne += acc[p].own.size();
     if (nedges) *nedges = ne;
     return dt;
 }
@@ -473,6 +485,124 @@ int64_t khpref_partition(uint64_t nacc, const uint8_t *dev, const uint8_t *nrect
     }
     part_off[nacc] = np;
     return (int64_t)np;
+}
+
+// One task domain's dependency resolution over a mixed program: the dispatch
+// of task_dependency_resolve_do (src/task/task.cc:204-277) over one interval
+// tree and one BLAS tree per (ld, sizeof_type), created on first use
+// (task_get_dependency_domain_blas_matrix, task.cc:153-201: a new tree gets
+// the interval tree's accesses put, newest first, with no LINK).  Per task:
+// every access LINKed in slot order, then every access PUT (task.cc:285-299).
+// A segment links / puts into the interval tree and then into every BLAS tree
+// existing at that moment (link_interval / put_interval, dependency-tree.hpp:
+// 384-427: rects in the tree's frame, marks in the ld = SIZE_MAX frame).  Each
+// access keeps one successor list across all its trees (__access_precedes).
+// kind 0 = tile (addr, m rows, n columns, ld, s), 1 = segment [addr, addr + m).
+// Edges as khpref_resolve, in program access indices.
+int64_t khpref_resolve_program(uint64_t nacc, const uint32_t *task, const uint8_t *mode, const uint8_t *kind,
+                               const uint64_t *addr, const uint64_t *m, const uint64_t *n, const uint64_t *ld,
+                               const uint64_t *s, uint64_t edge_cap, uint32_t *edges)
+{
+    struct Blas {
+        uint64_t ld, s;
+        DepTree tree;
+        std::vector<std::unique_ptr<Acc>> views;  // per program access, created on first use
+    };
+    std::vector<SuccList> succ(nacc);
+    std::vector<IAcc> iacc(nacc);
+    std::vector<std::unique_ptr<Blas>> blas;
+    IntTree itree;
+    std::vector<uint32_t> iput;  // accesses put into the interval tree, in put order
+    auto interval_rects = [](uint64_t a, uint64_t size, uint64_t LD, uint64_t sz, Rect out[3]) {
+        const uint64_t LDs = LD * sz, x0 = a % LDs, x1 = std::min(x0 + size, LDs), dx10 = x1 - x0;
+        const uint64_t x3 = (size - dx10) % LDs, y0 = a / LDs, y1 = y0 + 1;
+        const uint64_t y3 = y0 + (size - dx10 - x3) / LDs + 2, y2 = y3 - 1;
+        const uint64_t r[3][4] = {{x0, x1, y0, y1}, {0, LDs, y1, y2}, {0, x3, y2, y3}};
+        for (int k = 0; k < 3; ++k) out[k] = make_rect(r[k]);
+    };
+    auto view = [&](Blas &b, uint32_t i) -> Acc * {
+        if (b.views.size() < nacc) b.views.resize(nacc);
+        if (!b.views[i]) {
+            auto a = std::make_unique<Acc>();
+            a->idx = i;
+            a->task = task[i];
+            a->mode = mode[i];
+            a->succ = &succ[i];
+            if (kind[i] == 1) {
+                interval_rects(addr[i], m[i], b.ld, b.s, a->rects);
+                interval_rects(addr[i], m[i], UINT64_MAX, 1, a->marks);
+                a->nrect = a->nmark = 3;
+            } else {
+                // matrix_to_rects (access.hpp:205-287): one rect, or two when the tile wraps
+                const uint64_t LDs = b.ld * b.s, X0 = addr[i] % LDs, Y0 = addr[i] / LDs, ms = m[i] * b.s;
+                if (X0 + ms <= LDs) {
+                    const uint64_t r[4] = {X0, X0 + ms, Y0, Y0 + n[i]};
+                    a->rects[0] = make_rect(r);
+                    a->nrect = 1;
+                } else {
+                    const uint64_t r0[4] = {X0, LDs, Y0, Y0 + n[i]}, r1[4] = {0, ms - (LDs - X0), Y0 + 1, Y0 + n[i] + 1};
+                    a->rects[0] = make_rect(r0);
+                    a->rects[1] = make_rect(r1);
+                    a->nrect = 2;
+                }
+                a->nmark = a->nrect;
+                for (int k = 0; k < a->nrect; ++k) a->marks[k] = a->rects[k];
+            }
+            b.views[i] = std::move(a);
+        }
+        return b.views[i].get();
+    };
+    auto tree_of = [&](uint32_t i) -> Blas & {
+        for (auto &b : blas)
+            if (b->ld == ld[i] && b->s == s[i]) return *b;
+        blas.push_back(std::make_unique<Blas>());
+        Blas &b = *blas.back();
+        b.ld = ld[i];
+        b.s = s[i];
+        for (auto it = iput.rbegin(); it != iput.rend(); ++it) b.tree.put(view(b, *it));
+        return b;
+    };
+    for (uint64_t i = 0; i < nacc; ++i) {
+        iacc[i].idx = (uint32_t)i;
+        iacc[i].task = task[i];
+        iacc[i].mode = mode[i];
+        Interval l[1] = {Interval(addr[i], addr[i] + m[i])};
+        iacc[i].seg = IRect(l);
+        iacc[i].succ = &succ[i];
+    }
+    uint64_t i = 0;
+    while (i < nacc) {
+        uint64_t j = i;
+        while (j < nacc && task[j] == task[i]) ++j;
+        for (uint64_t a = i; a < j; ++a) {
+            if (kind[a] == 1) {
+                itree.link(&iacc[a]);
+                for (auto &b : blas) b->tree.link(view(*b, (uint32_t)a));
+            } else {
+                tree_of((uint32_t)a).tree.link(view(tree_of((uint32_t)a), (uint32_t)a));
+            }
+        }
+        for (uint64_t a = i; a < j; ++a) {
+            if (kind[a] == 1) {
+                itree.put(&iacc[a]);
+                iput.push_back((uint32_t)a);
+                for (auto &b : blas) b->tree.put(view(*b, (uint32_t)a));
+            } else {
+                tree_of((uint32_t)a).tree.put(view(tree_of((uint32_t)a), (uint32_t)a));
+            }
+        }
+        i = j;
+    }
+    uint64_t ne = 0;
+    for (uint64_t p = 0; p < nacc; ++p)
+        for (const Succ &sc : succ[p]) {
+            if (ne < edge_cap) {
+                edges[2 * ne] = (uint32_t)p;
+                edges[2 * ne + 1] = sc.idx;
+            }
+            ne++;
+        }
+    return (int64_t)ne;
 }
 
 } // extern "C"

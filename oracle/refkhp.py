@@ -32,6 +32,8 @@ def lib():
         _lib.khpref_partition.restype = C.c_int64
         _lib.khpref_resolve_intervals.argtypes = [u, p, p, p, u, p]
         _lib.khpref_resolve_intervals.restype = C.c_int64
+        _lib.khpref_resolve_program.argtypes = [u, p, p, p, p, p, p, p, p, u, p]
+        _lib.khpref_resolve_program.restype = C.c_int64
     return _lib
 
 
@@ -99,6 +101,24 @@ def resolve_intervals(task, mode, seg):
         edges = np.empty((cap, 2), np.uint32)
         ne = L.khpref_resolve_intervals(n, task.ctypes.data, mode.ctypes.data, seg.ctypes.data, cap,
                                         edges.ctypes.data)
+        if ne <= cap:
+            break
+        cap = int(ne)
+    return edges[:ne].copy()
+
+
+def resolve_program(task, mode, kind, addr, m, n, ld, s):
+    """The reference's task-domain dispatch over a mixed program: one interval
+    tree, one BLAS tree per (ld, s) created on first use (with the replay), one
+    successor list per access.  Edges in program access indices."""
+    L = lib()
+    cnt = len(task)
+    a = [_c(task, np.uint32), _c(mode, np.uint8), _c(kind, np.uint8), _c(addr, np.uint64), _c(m, np.uint64),
+         _c(n, np.uint64), _c(ld, np.uint64), _c(s, np.uint64)]
+    cap = max(1024, 4 * cnt)
+    while True:
+        edges = np.empty((cap, 2), np.uint32)
+        ne = L.khpref_resolve_program(cnt, *[x.ctypes.data for x in a], cap, edges.ctypes.data)
         if ne <= cap:
             break
         cap = int(ne)

@@ -103,3 +103,38 @@ def mixed_stream(n, ld, cols, s, seed, pseg=0.3, maxm=12, maxn=6, maxseg=None, m
     nn = np.where(kind == 1, 1, nn).astype(np.uint64)
     return dict(task=task, slot=slot, mode=mode, kind=kind, addr=addr.astype(np.uint64), m=m.astype(np.uint64),
                 n=nn, ld=ld, s=s)
+
+
+def program_stream(n, seed, pairs=((32, 4), (16, 8), (24, 4)), base=1 << 20, span=1 << 15, pseg=0.35,
+                   maxm=24, maxn=6, maxseg=1 << 11, modes=(1, 2, 3, 4, 5, 6, 8, 9), late=0.3):
+    """A mixed program of one task domain: tasks of 1..3 accesses, segments and
+    tiles of several (ld, sizeof_type) pairs over the bytes [base, base + span)
+    (base >= every ld * s: nothing in the first column).  A pair's first tile
+    comes after about late * n accesses, so its BLAS domain is created while
+    segments are in flight.  Tiles do not wrap.  Returns a dict of arrays."""
+    rng = np.random.default_rng(seed)
+    per = rng.integers(1, 4, n)
+    task = np.repeat(np.arange(n, dtype=np.uint32), per)[:n]
+    mode = rng.choice(np.array(modes, np.uint8), n)
+    kind = (rng.random(n) < pseg).astype(np.uint8)
+    pair = rng.integers(0, len(pairs), n)
+    # the last pair appears only in the later part of the program
+    if len(pairs) > 1:
+        pair[: int(late * n)] = rng.integers(0, len(pairs) - 1, int(late * n))
+    ld = np.array([pairs[k][0] for k in pair], np.uint64)
+    s = np.array([pairs[k][1] for k in pair], np.uint64)
+    LDs = ld * s
+    c0 = (np.uint64(base) + LDs - np.uint64(1)) // LDs
+    ncol = np.uint64(span) // LDs
+    col = c0 + (rng.random(n) * (ncol - np.uint64(maxn))).astype(np.uint64)
+    row = (rng.random(n) * ld).astype(np.uint64)
+    m = np.minimum(rng.integers(1, maxm + 1, n).astype(np.uint64), ld - row)
+    nn = rng.integers(1, maxn + 1, n).astype(np.uint64)
+    addr = (col * ld + row) * s
+    sa = rng.integers(base, base + span - 1, n).astype(np.uint64)
+    sl = np.minimum(np.exp2(rng.random(n) * np.log2(maxseg)).astype(np.uint64) + np.uint64(1),
+                    np.uint64(base + span) - sa)
+    addr = np.where(kind == 1, sa, addr).astype(np.uint64)
+    m = np.where(kind == 1, sl, m).astype(np.uint64)
+    nn = np.where(kind == 1, 0, nn).astype(np.uint64)
+    return dict(task=task, mode=mode, kind=kind, addr=addr, m=m, n=nn, ld=ld, s=s)

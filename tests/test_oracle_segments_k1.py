@@ -120,3 +120,41 @@ def test_replayed_segments_in_column_zero_can_change_blas_edges():
         got, alone = replay_vs_alone(a, size, np.full(nseg, 2, np.uint8), *tiles(rng, 200, ld, cols, s), ld, s)
         changed += not same(got, alone)
     assert changed > 0
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_program_composition_matches_reference_dispatch(seed):
+    # a task domain over segments and tiles of three (ld, s) pairs, one pair's BLAS tree created
+    # late (the replay): the composition of one segment domain and one BLAS domain per pair
+    # equals the reference's dispatch over its own trees
+    if not refkhp.available():
+        pytest.skip("oracle/_ref not built (needs /root/reference)")
+    p = G.program_stream(4000, seed)
+    args = [p[k] for k in ("task", "mode", "kind", "addr", "m", "n", "ld", "s")]
+    want = refkhp.resolve_program(*args)
+    got = khp.resolve_program(*args)
+    assert len(want) > 1000
+    assert same(got, want)
+    # the segments reach predecessors through the BLAS domains, and the tiles through the segment domain never
+    kind = p["kind"]
+    e = pairs(want)
+    assert ((kind[e[:, 0]] == 0) & (kind[e[:, 1]] == 1)).any()
+    assert ((kind[e[:, 0]] == 1) & (kind[e[:, 1]] == 1)).any()
+    assert not ((kind[e[:, 0]] == 1) & (kind[e[:, 1]] == 0)).any()
+    # every predecessor's edges come from one domain (a segment's from the segment domain, a
+    # tile's from its pair's BLAS domain): the domains never give the same (pred, succ task)
+    k = np.stack([e[:, 0], p["task"][e[:, 1]].astype(np.int64)], axis=1)
+    assert len(np.unique(k, axis=0)) == len(k)
+
+
+@pytest.mark.parametrize("mistake", ["segments_skip_blas", "no_segment_domain"])
+def test_program_composition_planted_mistakes_fail(mistake):
+    # segments that do not LINK into the BLAS domains, or a lost segment domain, break the
+    # equality with the reference's dispatch
+    if not refkhp.available():
+        pytest.skip("oracle/_ref not built (needs /root/reference)")
+    p = G.program_stream(3000, 11)
+    args = [p[k] for k in ("task", "mode", "kind", "addr", "m", "n", "ld", "s")]
+    want = refkhp.resolve_program(*args)
+    assert same(khp.resolve_program(*args), want)
+    assert not same(khp.resolve_program(*args, _mistake=mistake), want)
