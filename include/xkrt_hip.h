@@ -118,6 +118,32 @@ int xkrt_hip_dep_domain_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip
                                void *stream, xkrt_hip_dep_domain_t *out);
 int xkrt_hip_dep_domain_destroy(xkrt_hip_dep_domain_t d);
 
+/* Creates a segment dependency domain, the seam of the task domain's
+ * IntervalDependencyTree (src/task/task.cc:240-259; the K = 1 tree of
+ * interval/dependency-tree.hpp:187-363).  It takes byte segments only
+ * (XKRT_HIP_KIND_SEGMENT; a tile is refused with -EINVAL) and gives the edges
+ * of the reference's interval tree: each segment is cut into the rects of a
+ * frame of col_bytes-byte columns (at most 3) and LINKed and PUT exactly, the
+ * same per-point rule as the one-dimensional tree.  col_bytes and cfg's bin
+ * extents set how the address space is tiled into bins (bin_x_bytes <=
+ * col_bytes); the frame spans 2^32 columns from the first segment seen.
+ * Submit, sync, edges and destroy as for xkrt_hip_dep_domain_create's
+ * domains.  A task whose accesses mix segments and tiles gives its segments
+ * to this domain and, LINK-only, to every BLAS domain (xkrt_hip_dep_submit). */
+int xkrt_hip_seg_domain_create(uint64_t col_bytes, const xkrt_hip_dep_config_t *cfg, void *stream,
+                               xkrt_hip_dep_domain_t *out);
+
+/* The replay of task_get_dependency_domain_blas_matrix (src/task/task.cc:
+ * 177-198): a BLAS domain created while segment accesses are uncompleted gets
+ * them PUT, with no LINK.  That PUT marks no node unless the segment starts in
+ * the first ld * sizeof_type bytes of memory (see xkrt_hip_dep_submit), so
+ * the call changes no edge; it records the dim-0 extent of the inserted rects
+ * for the wrapped-tile check.  -ENOTSUP, and nothing recorded, when a segment
+ * starts below ld * sizeof_type or the domain has nodes at column 0; -EINVAL
+ * for a tile, an empty segment or a segment domain.  Host only: nothing is
+ * launched. */
+int xkrt_hip_dep_replay_segments(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n);
+
 /* Enqueues the resolution of n accesses in spawn order: every access of a
  * task is LINKed (slot order) before any is PUT, as task_dependency_resolve
  * does (src/task/task.cc:285-299).  Access ids are assigned consecutively
@@ -253,15 +279,31 @@ int xkrt_hip_pool_destroy(xkrt_hip_pool_t p);
 /* Admits a domain for (ld, sizeof_type) over columns [y_lo, y_hi) with the
  * given bin extents (powers of two; 0 = 1024 B x 128 columns).  -ENOMEM: the
  * budget cannot hold its bins now (retry after other domains leave).  Leave
- * with xkrt_hip_dep_domain_destroy. */
+ * with xkrt_hip_dep_domain_destroy.  y_lo == y_hi: no range; the first batch
+ * frames the domain and its grid grows with later batches (within the
+ * budget), as a single domain's does. */
 int xkrt_hip_pool_domain_create(xkrt_hip_pool_t p, uint64_t ld, uint64_t sizeof_type, uint32_t bin_x_bytes,
                                 uint32_t bin_y_cols, uint64_t y_lo, uint64_t y_hi, xkrt_hip_dep_domain_t *d);
+
+/* Admits a segment domain (see xkrt_hip_seg_domain_create) over columns
+ * [y_lo, y_hi) of col_bytes bytes, i.e. addresses [y_lo, y_hi) * col_bytes. */
+int xkrt_hip_pool_seg_domain_create(xkrt_hip_pool_t p, uint64_t col_bytes, uint32_t bin_x_bytes, uint32_t bin_y_cols,
+                                    uint64_t y_lo, uint64_t y_hi, xkrt_hip_dep_domain_t *d);
 
 /* Appends a batch of d's accesses (whole tasks, spawn order) to the open
  * launch; ids as xkrt_hip_dep_submit.  -ENOSPC: the open launch is full;
  * -ECANCELED: d failed earlier (xkrt_hip_dep_sync(d) gives its error). */
 int xkrt_hip_pool_stage(xkrt_hip_pool_t p, xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n,
                         uint64_t *first_id);
+
+/* Stages nd batches, batch k of domain d[k] (acc[k], n[k] accesses), as one
+ * unit: either all of them are appended to the open launch (first ids in
+ * first_id[k], nullable) or none is, and the first refusal's code is returned
+ * (-ENOSPC: they do not fit the open launch together; launch it and stage
+ * again).  A task whose accesses go to several domains is staged this way. */
+int xkrt_hip_pool_stage_group(xkrt_hip_pool_t p, uint32_t nd, const xkrt_hip_dep_domain_t *d,
+                              const xkrt_hip_access_t *const *acc, const uint64_t *n, uint64_t *first_id);
+
 /* Resolves every staged batch in one launch (asynchronous).  *ticket (may
  * be NULL) = the launch's sequence number, 1 for the pool's first launch. */
 int xkrt_hip_pool_launch(xkrt_hip_pool_t p, uint64_t *ticket);
@@ -357,6 +399,81 @@ int xkrt_hip_server_times(xkrt_hip_server_t s, uint64_t first, uint64_t n, doubl
  * *n = pairs available (up to cap copied). */
 int xkrt_hip_server_edges(xkrt_hip_server_t s, uint32_t client, uint32_t *edges, uint64_t cap, uint64_t *n);
 int xkrt_hip_server_stats(xkrt_hip_server_t s, xkrt_hip_server_stats_t *out);
+
+/* ------------------------------------------------------------------------
+ * Task domains: the dependency resolution of one task domain over every kind
+ * of access it holds, the seam of task_dependency_resolve (src/task/task.cc:
+ * 204-299).  A task domain owns a pool with one segment domain, created at
+ * the first segment (the IntervalDependencyTree of task.cc:240-259), and one
+ * BLAS domain per (ld, sizeof_type), created at the first tile of that pair
+ * (task_get_dependency_domain_blas_matrix, task.cc:153-201).  A segment goes
+ * to the segment domain and, LINK-only, to every BLAS domain existing when it
+ * is linked; a BLAS domain created while segments are in flight gets them
+ * replayed (xkrt_hip_dep_replay_segments).  Every batch is staged into all
+ * its domains as one unit and resolved by one launch.  Access ids are
+ * program-wide (64-bit, consecutive in submit order); each predecessor's
+ * edges come from one domain (a segment's from the segment domain, a tile's
+ * from its pair's), so the domains' edges, mapped to program ids, are the
+ * reference's edges with no cross-domain duplicate.
+ * Host memory: 8 bytes per access per domain it enters (id maps), 16 bytes
+ * per segment (the replay).  Handle accesses are not taken (-EINVAL).
+ * ------------------------------------------------------------------------ */
+
+typedef struct xkrt_hip_taskdom *xkrt_hip_taskdom_t;
+
+/* One access of a task domain (48 B). */
+typedef struct xkrt_hip_task_access {
+    uint64_t addr;         /* begin address in bytes */
+    uint64_t m;            /* tile: rows (elements); segment: size in bytes */
+    uint64_t n;            /* tile: columns; segment: ignored */
+    uint64_t ld;           /* tile: leading dimension (elements); segment: ignored */
+    uint32_t sizeof_type;  /* tile: element size (bytes); segment: ignored */
+    uint32_t task;         /* spawn-order task number; a task's accesses are contiguous */
+    uint8_t  mode;         /* XKRT_HIP_MODE_* bits */
+    uint8_t  kind;         /* XKRT_HIP_KIND_TILE or XKRT_HIP_KIND_SEGMENT */
+    uint8_t  pad[6];
+} xkrt_hip_task_access_t;
+
+typedef struct xkrt_hip_taskdom_config {
+    uint32_t kernel;          /* LDS capacity class of every domain (see xkrt_hip_dep_config_t) */
+    uint32_t pad;
+    uint64_t max_rects;       /* rect records per launch (0 = 2^20) */
+    uint64_t max_edges;       /* edge buffer capacity before edges must be copied out (0 = 2^22) */
+    uint64_t budget_bytes;    /* device memory budget of all its domains; 0 = unlimited */
+    uint64_t seg_col_bytes;   /* segment domain: column width of its frame (0 = 65536) */
+    uint32_t seg_bin_x_bytes; /* segment domain bins (0 = min(column width, 4096) B x 1 column) */
+    uint32_t seg_bin_y_cols;
+    uint32_t blas_bin_x_bytes;/* BLAS domain bins (0 = 1024 B x 128 columns) */
+    uint32_t blas_bin_y_cols;
+} xkrt_hip_taskdom_config_t;
+
+typedef struct xkrt_hip_taskdom_stats {
+    uint64_t accesses;        /* accesses resolved or enqueued */
+    uint64_t segments;        /* of which segments */
+    uint64_t blas_domains;    /* BLAS domains created */
+    uint64_t launches;
+    uint64_t edges;           /* edges copied out so far */
+} xkrt_hip_taskdom_stats_t;
+
+int xkrt_hip_taskdom_create(const xkrt_hip_taskdom_config_t *cfg, void *stream, xkrt_hip_taskdom_t *out);
+int xkrt_hip_taskdom_destroy(xkrt_hip_taskdom_t t);
+
+/* Enqueues a batch of whole tasks in spawn order (task numbers non-decreasing
+ * across batches); ids from *first_id (out, nullable).  Launches when the
+ * open launch is full.  Either the whole batch is enqueued or nothing of it:
+ * -EINVAL (bad access, task order, more than 255 BLAS domains), -ENOTSUP (a
+ * case a domain refuses, see xkrt_hip_dep_submit and
+ * xkrt_hip_dep_replay_segments), -ENOMEM (budget). */
+int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *acc, uint64_t n,
+                            uint64_t *first_id);
+/* Launches what is open and waits for every launch; *nedges = edges waiting
+ * to be copied out.  A domain's device error is returned here (and later). */
+int xkrt_hip_taskdom_sync(xkrt_hip_taskdom_t t, uint64_t *nedges);
+/* Copies out and consumes every waiting edge: (pred, succ) program access ids,
+ * 2 u64 each.  -ENOSPC, nothing consumed, *n = edges waiting, when cap is
+ * smaller.  Call after xkrt_hip_taskdom_sync. */
+int xkrt_hip_taskdom_edges_copy(xkrt_hip_taskdom_t t, uint64_t *edges, uint64_t cap, uint64_t *n);
+int xkrt_hip_taskdom_stats(xkrt_hip_taskdom_t t, xkrt_hip_taskdom_stats_t *out);
 
 /* ------------------------------------------------------------------------
  * Coherency fetch planner: one memory tree per (ld, sizeof_type), a host

@@ -68,3 +68,36 @@ def test_compiled_abi_driver_links_against_the_library():
     assert used <= set(declared())
     lib = hip_lib()
     assert all(hasattr(lib, n) for n in used)
+
+
+def test_ctypes_layouts_match_compiled_header(tmp_path):
+    # every ctypes struct against the C compiler's sizeof and offsetof of the header's struct
+    import subprocess
+    from xkaapi_amd import coherency as K
+    from xkaapi_amd import resolver as R
+    structs = {
+        "xkrt_hip_access_t": R.AccessT, "xkrt_hip_dep_config_t": R.ConfigT, "xkrt_hip_dep_stats_t": R.StatsT,
+        "xkrt_hip_pool_config_t": R.PoolConfigT, "xkrt_hip_pool_stats_t": R.PoolStatsT,
+        "xkrt_hip_server_config_t": R.ServerConfigT, "xkrt_hip_server_stats_t": R.ServerStatsT,
+        "xkrt_hip_task_access_t": R.TaskAccessT, "xkrt_hip_taskdom_config_t": R.TaskDomConfigT,
+        "xkrt_hip_taskdom_stats_t": R.TaskDomStatsT, "xkrt_hip_mcc_config_t": K.MccConfigT,
+        "xkrt_hip_mcc_access_t": K.MccAccessT, "xkrt_hip_mcc_stats_t": K.MccStatsT,
+    }
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "xkrt_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            if f[0] != "pad":
+                lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (l.split() for l in out if l)}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f in py._fields_:
+            if f[0] != "pad":
+                assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])

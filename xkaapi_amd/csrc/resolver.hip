@@ -1595,6 +1595,9 @@ struct xkrt_hip_dep_domain {
     uint64_t next_id = 0;
     uint64_t max_xlen = 0;
     bool col0 = false;  // some rect started at absolute column 0 (addresses below ld * sizeof_type)
+    // a segment domain (interval/dependency-tree.hpp:187-363): segments only, each cut into
+    // the rects of a frame of ld-byte columns and PUT exactly
+    bool segments = false;
     uint64_t n_acc = 0, n_rect = 0;
     uint32_t err = 0;   // device error bits of this domain seen by the host (sticky until reset)
 };
@@ -2277,6 +2280,7 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
             tl++;
         }
         if (q.m == 0 || (q.n == 0 && q.kind == XKRT_HIP_KIND_TILE) || q.kind > XKRT_HIP_KIND_SEGMENT) return -EINVAL;
+        if (d->segments && q.kind != XKRT_HIP_KIND_SEGMENT) return -EINVAL;
         if (tl > 0xFFFFFFu) return -ERANGE;
         uint64_t r[3][4];
         int nr;
@@ -2284,7 +2288,11 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
         const uint64_t X0 = q.addr % LDs, Y0 = q.addr / LDs, ms = q.m * s;
         if (q.kind == XKRT_HIP_KIND_SEGMENT) {
             // access.hpp:77-139 interval_to_rects in the domain frame; empty rects
-            // are skipped, as the tree's insert and intersect skip them.  PUT tests
+            // are skipped, as the tree's insert and intersect skip them.
+            // In a segment domain the rects are PUT exactly: the interval tree marks
+            // the nodes that meet the segment itself (dependency-tree.hpp:241-256
+            // with K = 1), so each point of [addr, addr + m) is written or read once.
+            // In a BLAS domain, PUT tests
             // the tree's nodes against the segment's own rects, built with
             // ld = SIZE_MAX (access.hpp:495-513): [addr, addr + m) x [0, 1) and two
             // empty rects, one of them [0, SIZE_MAX) x [1, 1).  Interval overlap is
@@ -2292,7 +2300,7 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
             // start at column 0 -- which exist only once some access touched the
             // first ld * sizeof_type bytes of memory.  Before that the PUT marks
             // nothing and the segment only links; after it, it is refused.
-            if (q.addr < LDs || col0) return -ENOTSUP;
+            if (!d->segments && (q.addr < LDs || col0)) return -ENOTSUP;
             const uint64_t size = q.m, x1 = std::min(X0 + size, LDs), dx10 = x1 - X0;
             const uint64_t x3 = (size - dx10) % LDs;
             const uint64_t y1 = Y0 + 1, y3 = Y0 + (size - dx10 - x3) / LDs + 2, y2 = y3 - 1;
@@ -2303,7 +2311,7 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
                     for (int w = 0; w < 4; ++w) r[nr][w] = c[j][w];
                     ++nr;
                 }
-            flag = 0x80u;  // LINK only: its PUT marks no node of this domain
+            if (!d->segments) flag = 0x80u;  // LINK only: its PUT marks no node of this domain
         } else if (X0 + ms <= LDs) {
             r[0][0] = X0; r[0][1] = X0 + ms; r[0][2] = Y0; r[0][3] = Y0 + q.n;
             nr = 1;
@@ -2533,12 +2541,16 @@ int xkrt_hip_pool_destroy(xkrt_hip_pool_t p)
 int xkrt_hip_pool_domain_create(xkrt_hip_pool_t p, uint64_t ld, uint64_t sizeof_type, uint32_t bin_x_bytes,
                                 uint32_t bin_y_cols, uint64_t y_lo, uint64_t y_hi, xkrt_hip_dep_domain_t *out)
 {
-    if (!p || !out || y_hi <= y_lo || y_hi - y_lo >= (1ull << 32)) return -EINVAL;
+    if (!p || !out || y_hi < y_lo || y_hi - y_lo >= (1ull << 32)) return -EINVAL;
     *out = nullptr;
     if (p->server) return -EBUSY;
     xkrt_hip_dep_domain *d = nullptr;
     int rc = add_domain(p, ld, sizeof_type, bin_x_bytes ? bin_x_bytes : 1024u, bin_y_cols ? bin_y_cols : 128u, &d);
     if (rc) return rc;
+    if (y_lo == y_hi) {  // no range given: framed by its first batch, the grid grows with the batches
+        *out = d;
+        return 0;
+    }
     d->ybase = y_lo;
     d->framed = true;
     rc = ensure_grid(d, 0, (uint32_t)(y_hi - y_lo));
@@ -2551,12 +2563,58 @@ int xkrt_hip_pool_domain_create(xkrt_hip_pool_t p, uint64_t ld, uint64_t sizeof_
     return 0;
 }
 
+int xkrt_hip_pool_seg_domain_create(xkrt_hip_pool_t p, uint64_t col_bytes, uint32_t bin_x_bytes, uint32_t bin_y_cols,
+                                    uint64_t y_lo, uint64_t y_hi, xkrt_hip_dep_domain_t *out)
+{
+    const int rc = xkrt_hip_pool_domain_create(p, col_bytes, 1, bin_x_bytes, bin_y_cols, y_lo, y_hi, out);
+    if (!rc) (*out)->segments = true;
+    return rc;
+}
+
 int xkrt_hip_pool_stage(xkrt_hip_pool_t p, xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n,
                         uint64_t *first_id)
 {
     if (!p || !d || d->pool != p || (!acc && n)) return -EINVAL;
     if (p->server) return -EBUSY;
     return stage(p, d, acc, n, first_id);
+}
+
+int xkrt_hip_pool_stage_group(xkrt_hip_pool_t p, uint32_t nd, const xkrt_hip_dep_domain_t *d,
+                              const xkrt_hip_access_t *const *acc, const uint64_t *n, uint64_t *first_id)
+{
+    if (!p || (nd && (!d || !acc || !n))) return -EINVAL;
+    if (p->server) return -EBUSY;
+    for (uint32_t k = 0; k < nd; ++k)
+        if (!d[k] || d[k]->pool != p || (!acc[k] && n[k])) return -EINVAL;
+    // what staging changes, restored when a later batch of the group is refused
+    struct Saved {
+        uint64_t next_id, n_acc, n_rect, max_xlen, ybase;
+        bool col0, framed;
+    };
+    std::vector<Saved> saved(nd);
+    for (uint32_t k = 0; k < nd; ++k) {
+        const xkrt_hip_dep_domain *q = d[k];
+        saved[k] = {q->next_id, q->n_acc, q->n_rect, q->max_xlen, q->ybase, q->col0, q->framed};
+    }
+    const uint32_t o_nrec = p->open_nrec, o_tasks = p->open_tasks;
+    const uint64_t o_acc = p->open_acc, o_batches = p->open_batches;
+    for (uint32_t k = 0; k < nd; ++k) {
+        const int rc = stage(p, d[k], acc[k], n[k], first_id ? first_id + k : nullptr);
+        if (rc) {
+            for (uint32_t j = nd; j-- > 0;) {  // newest first: a domain named twice gets its oldest state back
+                xkrt_hip_dep_domain *q = d[j];
+                const Saved &v = saved[j];
+                q->next_id = v.next_id; q->n_acc = v.n_acc; q->n_rect = v.n_rect;
+                q->max_xlen = v.max_xlen; q->col0 = v.col0;
+                // a grid allocated meanwhile holds no fragment: it stays, with the frame it was sized in
+                if (!q->nby) { q->framed = v.framed; q->ybase = v.ybase; }
+            }
+            p->open_nrec = o_nrec; p->open_tasks = o_tasks;
+            p->open_acc = o_acc; p->open_batches = o_batches;
+            return rc;
+        }
+    }
+    return 0;
 }
 
 int xkrt_hip_pool_launch(xkrt_hip_pool_t p, uint64_t *ticket)
@@ -2686,8 +2744,10 @@ int xkrt_hip_dep_index(xkrt_hip_dep_domain_t d, uint32_t *idx)
 
 // ---- single domains (a private pool each)
 
-int xkrt_hip_dep_domain_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_dep_config_t *cfg, void *stream,
-                               xkrt_hip_dep_domain_t *out)
+namespace {
+
+int create_single(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_dep_config_t *cfg, void *stream, bool segments,
+                  xkrt_hip_dep_domain_t *out)
 {
     if (!out || ld == 0 || sizeof_type == 0) return -EINVAL;
     *out = nullptr;
@@ -2708,6 +2768,7 @@ int xkrt_hip_dep_domain_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip
     rc = add_domain(p, ld, sizeof_type, BX, BY, &d);
     if (!rc) {
         d->private_pool = true;
+        d->segments = segments;
         if (c.y_hint_lo < c.y_hint_hi) {
             d->ybase = c.y_hint_lo;
             d->framed = true;
@@ -2721,6 +2782,43 @@ int xkrt_hip_dep_domain_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip
         return rc;
     }
     *out = d;
+    return 0;
+}
+
+} // namespace
+
+int xkrt_hip_dep_domain_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_dep_config_t *cfg, void *stream,
+                               xkrt_hip_dep_domain_t *out)
+{
+    return create_single(ld, sizeof_type, cfg, stream, false, out);
+}
+
+int xkrt_hip_seg_domain_create(uint64_t col_bytes, const xkrt_hip_dep_config_t *cfg, void *stream,
+                               xkrt_hip_dep_domain_t *out)
+{
+    return create_single(col_bytes, 1, cfg, stream, true, out);
+}
+
+int xkrt_hip_dep_replay_segments(xkrt_hip_dep_domain_t d, const xkrt_hip_access_t *acc, uint64_t n)
+{
+    if (!d || (!acc && n) || d->segments) return -EINVAL;
+    // task.cc:177-198 puts the uncompleted segment accesses into a new BLAS tree, with no
+    // LINK.  The PUT tests nodes against the segment's own rects in the ld = SIZE_MAX
+    // frame (access.hpp:495-513), which meet only nodes starting at column 0: a segment
+    // below ld * sizeof_type is refused, any other marks nothing.  Its rects are still
+    // inserted, splitting nodes, and a later wrapped tile must see their dim-0 extent.
+    const uint64_t LDs = d->LDs;
+    uint64_t maxx = d->max_xlen;
+    for (uint64_t i = 0; i < n; ++i) {
+        const xkrt_hip_access_t &q = acc[i];
+        if (q.kind != XKRT_HIP_KIND_SEGMENT || q.m == 0) return -EINVAL;
+        if (q.addr < LDs || d->col0) return -ENOTSUP;
+        const uint64_t X0 = q.addr % LDs, x1 = std::min(X0 + q.m, LDs), rest = q.m - (x1 - X0);
+        maxx = std::max(maxx, x1 - X0);
+        if (rest >= LDs) maxx = LDs;  // a whole-column middle rect
+        else if (rest) maxx = std::max(maxx, rest);
+    }
+    d->max_xlen = maxx;
     return 0;
 }
 

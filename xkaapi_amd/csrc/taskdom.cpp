@@ -1,0 +1,304 @@
+// Task domains: one task domain's dependency resolution over segments and
+// BLAS tiles of any (ld, sizeof_type), the dispatch of task_dependency_resolve
+// (src/task/task.cc:204-299) on top of a pool of GPU-resident domains.
+//
+//   * the segment domain is created at the first segment (task.cc:240-259);
+//   * a BLAS domain per (ld, sizeof_type) at its first tile, looked up by a
+//     linear scan as task_get_dependency_domain_blas_matrix does (task.cc:
+//     153-201); a new one gets every earlier segment replayed (task.cc:
+//     177-198), including those of the tile's own task that come before it:
+//     their PUT reaches every tree existing at the PUT pass;
+//   * a segment LINKs into the segment domain and into every BLAS domain that
+//     exists when it is linked (task.cc:240-259): in program order, the BLAS
+//     domains created by earlier accesses;
+//   * a batch's per-domain parts are staged as one unit and resolved by one
+//     launch; each domain's access ids are mapped back to program ids.
+//
+// Host code only: everything on the device goes through the pool's C ABI.
+
+#include "../../include/xkrt_hip.h"
+
+#include <algorithm>
+#include <cerrno>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+struct xkrt_hip_taskdom {
+    xkrt_hip_pool_t pool = nullptr;
+    xkrt_hip_taskdom_config_t cfg{};
+    xkrt_hip_dep_domain_t seg = nullptr;
+    struct Blas {
+        uint64_t ld, s;
+        xkrt_hip_dep_domain_t d;
+    };
+    std::vector<Blas> blas;
+    // program id of each domain access, by pool domain index then domain access id
+    std::vector<std::vector<uint64_t>> ids;
+    // (addr, size) of every segment enqueued: what a new BLAS domain gets replayed
+    std::vector<xkrt_hip_access_t> segs;
+    uint64_t next_id = 0, nseg = 0, launches = 0, edges_out = 0;
+    bool any_task = false;
+    uint32_t last_task = 0;
+    int err = 0;  // a domain's device error: sticky
+    // per-submit work lists, by pool domain index
+    std::vector<std::vector<xkrt_hip_access_t>> part;
+    std::vector<std::vector<uint64_t>> part_ids;
+    std::vector<uint32_t> e32, edom;
+};
+
+namespace {
+
+int dom_index(xkrt_hip_dep_domain_t d)
+{
+    uint32_t i = 0;
+    xkrt_hip_dep_index(d, &i);
+    return (int)i;
+}
+
+xkrt_hip_access_t plain(const xkrt_hip_task_access_t &q)
+{
+    xkrt_hip_access_t a{};
+    a.addr = q.addr;
+    a.m = q.m;
+    a.n = q.n;
+    a.task = q.task;
+    a.mode = q.mode;
+    a.kind = q.kind;
+    return a;
+}
+
+// a new domain of the pool: room in the per-index tables
+void admit(xkrt_hip_taskdom *t, xkrt_hip_dep_domain_t d)
+{
+    const size_t i = (size_t)dom_index(d);
+    if (t->ids.size() <= i) {
+        t->ids.resize(i + 1);
+        t->part.resize(i + 1);
+        t->part_ids.resize(i + 1);
+    }
+    t->ids[i].clear();
+}
+
+} // namespace
+
+extern "C" {
+
+int xkrt_hip_taskdom_create(const xkrt_hip_taskdom_config_t *cfg, void *stream, xkrt_hip_taskdom_t *out)
+{
+    if (!out) return -EINVAL;
+    *out = nullptr;
+    xkrt_hip_taskdom_config_t c{};
+    if (cfg) c = *cfg;
+    if (c.kernel > 2) return -EINVAL;
+    if (!c.max_rects) c.max_rects = 1ull << 20;
+    if (!c.max_edges) c.max_edges = 1ull << 22;
+    if (!c.seg_col_bytes) c.seg_col_bytes = 1ull << 16;
+    if (c.seg_col_bytes > 0xFFFFFFFFull) return -ERANGE;
+    if (!c.seg_bin_x_bytes) c.seg_bin_x_bytes = (uint32_t)std::min<uint64_t>(c.seg_col_bytes, 4096);
+    if (!c.seg_bin_y_cols) c.seg_bin_y_cols = 1;
+    if (!c.blas_bin_x_bytes) c.blas_bin_x_bytes = 1024;
+    if (!c.blas_bin_y_cols) c.blas_bin_y_cols = 128;
+    xkrt_hip_pool_config_t pc{};
+    pc.kernel = c.kernel;
+    pc.max_rects = c.max_rects;
+    pc.max_edges = c.max_edges;
+    pc.budget_bytes = c.budget_bytes;
+    xkrt_hip_pool_t p = nullptr;
+    const int rc = xkrt_hip_pool_create(&pc, stream, &p);
+    if (rc) return rc;
+    xkrt_hip_taskdom *t = new xkrt_hip_taskdom();
+    t->pool = p;
+    t->cfg = c;
+    *out = t;
+    return 0;
+}
+
+int xkrt_hip_taskdom_destroy(xkrt_hip_taskdom_t t)
+{
+    if (!t) return -EINVAL;
+    const int rc = xkrt_hip_pool_destroy(t->pool);  // releases its domains
+    delete t;
+    return rc;
+}
+
+int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *acc, uint64_t n, uint64_t *first_id)
+{
+    if (!t || (!acc && n)) return -EINVAL;
+    if (first_id) *first_id = t->next_id;
+    if (t->err) return -ECANCELED;
+    if (n == 0) return 0;
+    // validation first: a refused batch leaves nothing behind
+    for (uint64_t i = 0; i < n; ++i) {
+        const xkrt_hip_task_access_t &q = acc[i];
+        const uint32_t prev = i ? acc[i - 1].task : t->last_task;
+        if ((i || t->any_task) && q.task < prev) return -EINVAL;
+        if (q.kind != XKRT_HIP_KIND_TILE && q.kind != XKRT_HIP_KIND_SEGMENT) return -EINVAL;
+        if (q.m == 0) return -EINVAL;
+        if (q.kind == XKRT_HIP_KIND_TILE && (q.n == 0 || q.ld == 0 || q.sizeof_type == 0)) return -EINVAL;
+    }
+    // domains this batch creates (destroyed again if the batch is refused)
+    std::vector<xkrt_hip_dep_domain_t> created;
+    const size_t nblas0 = t->blas.size();
+    const size_t nsegs0 = t->segs.size();
+    auto undo = [&](int rc) {
+        for (auto &v : t->part) v.clear();
+        for (auto &v : t->part_ids) v.clear();
+        t->blas.resize(nblas0);
+        t->segs.resize(nsegs0);
+        for (xkrt_hip_dep_domain_t d : created) {
+            if (d == t->seg) t->seg = nullptr;
+            xkrt_hip_dep_domain_destroy(d);
+        }
+        return rc;
+    };
+    for (auto &v : t->part) v.clear();
+    for (auto &v : t->part_ids) v.clear();
+    for (uint64_t i = 0; i < n; ++i) {
+        const xkrt_hip_task_access_t &q = acc[i];
+        const uint64_t id = t->next_id + i;
+        const xkrt_hip_access_t a = plain(q);
+        if (q.kind == XKRT_HIP_KIND_SEGMENT) {
+            if (!t->seg) {
+                int rc = xkrt_hip_pool_seg_domain_create(t->pool, t->cfg.seg_col_bytes, t->cfg.seg_bin_x_bytes,
+                                                         t->cfg.seg_bin_y_cols, 0, 0, &t->seg);
+                if (rc) return undo(rc);
+                created.push_back(t->seg);
+                admit(t, t->seg);
+            }
+            const int si = dom_index(t->seg);
+            t->part[si].push_back(a);
+            t->part_ids[si].push_back(id);
+            for (auto &b : t->blas) {  // LINK-only in every BLAS domain that exists now
+                const int bi = dom_index(b.d);
+                t->part[bi].push_back(a);
+                t->part_ids[bi].push_back(id);
+            }
+            t->segs.push_back(a);
+        } else {
+            auto it = std::find_if(t->blas.begin(), t->blas.end(),
+                                   [&](const xkrt_hip_taskdom::Blas &b) { return b.ld == q.ld && b.s == q.sizeof_type; });
+            if (it == t->blas.end()) {
+                if (t->blas.size() >= 255) return undo(-EINVAL);
+                xkrt_hip_dep_domain_t d = nullptr;
+                int rc = xkrt_hip_pool_domain_create(t->pool, q.ld, q.sizeof_type, t->cfg.blas_bin_x_bytes,
+                                                     t->cfg.blas_bin_y_cols, 0, 0, &d);
+                if (rc) return undo(rc);
+                created.push_back(d);
+                admit(t, d);
+                // every earlier segment is PUT into the new tree with no LINK
+                rc = xkrt_hip_dep_replay_segments(d, t->segs.data(), t->segs.size());
+                if (rc) return undo(rc);
+                t->blas.push_back({q.ld, q.sizeof_type, d});
+                it = t->blas.end() - 1;
+            }
+            const int bi = dom_index(it->d);
+            t->part[bi].push_back(a);
+            t->part_ids[bi].push_back(id);
+        }
+    }
+    // stage every domain's part as one unit; launch the open launch first if it is full
+    std::vector<xkrt_hip_dep_domain_t> doms;
+    std::vector<const xkrt_hip_access_t *> ptr;
+    std::vector<uint64_t> cnt, first;
+    auto add = [&](xkrt_hip_dep_domain_t d) {
+        if (!d) return;
+        const int i = dom_index(d);
+        if (t->part[i].empty()) return;
+        doms.push_back(d);
+        ptr.push_back(t->part[i].data());
+        cnt.push_back(t->part[i].size());
+    };
+    add(t->seg);
+    for (auto &b : t->blas) add(b.d);
+    first.resize(doms.size());
+    int rc = xkrt_hip_pool_stage_group(t->pool, (uint32_t)doms.size(), doms.data(), ptr.data(), cnt.data(),
+                                       first.data());
+    if (rc == -ENOSPC) {
+        uint64_t ticket = 0;
+        rc = xkrt_hip_pool_launch(t->pool, &ticket);
+        if (!rc) {
+            t->launches++;
+            rc = xkrt_hip_pool_stage_group(t->pool, (uint32_t)doms.size(), doms.data(), ptr.data(), cnt.data(),
+                                           first.data());
+            if (rc == -ENOSPC) rc = -EINVAL;  // larger than one launch
+        }
+    }
+    if (rc) return undo(rc);
+    for (size_t k = 0; k < doms.size(); ++k) {
+        std::vector<uint64_t> &m = t->ids[dom_index(doms[k])];
+        const std::vector<uint64_t> &src = t->part_ids[dom_index(doms[k])];
+        if (m.size() != first[k]) return -EIO;  // domain ids and the map must agree
+        m.insert(m.end(), src.begin(), src.end());
+    }
+    for (uint64_t i = 0; i < n; ++i) t->nseg += acc[i].kind == XKRT_HIP_KIND_SEGMENT;
+    t->next_id += n;
+    t->any_task = true;
+    t->last_task = acc[n - 1].task;
+    return 0;
+}
+
+int xkrt_hip_taskdom_sync(xkrt_hip_taskdom_t t, uint64_t *nedges)
+{
+    if (!t) return -EINVAL;
+    uint64_t nr = 0, na = 0, nb = 0;
+    int rc = xkrt_hip_pool_open(t->pool, &nr, &na, &nb);
+    if (!rc && nb) {
+        uint64_t ticket = 0;
+        rc = xkrt_hip_pool_launch(t->pool, &ticket);
+        if (!rc) t->launches++;
+    }
+    uint64_t ne = 0;
+    if (!rc) rc = xkrt_hip_pool_sync(t->pool, &ne);
+    if (nedges) *nedges = ne;
+    if (rc) return rc;
+    std::vector<xkrt_hip_dep_domain_t> all;
+    if (t->seg) all.push_back(t->seg);
+    for (auto &b : t->blas) all.push_back(b.d);
+    for (xkrt_hip_dep_domain_t d : all) {
+        const int e = xkrt_hip_dep_sync(d, nullptr);
+        if (e && !t->err) t->err = e;
+    }
+    return t->err;
+}
+
+int xkrt_hip_taskdom_edges_copy(xkrt_hip_taskdom_t t, uint64_t *edges, uint64_t cap, uint64_t *n)
+{
+    if (!t || !n || (cap && !edges)) return -EINVAL;
+    uint64_t ne = 0;
+    int rc = xkrt_hip_pool_sync(t->pool, &ne);
+    *n = ne;
+    if (rc) return rc;
+    if (ne > cap) return -ENOSPC;
+    t->e32.resize(2 * ne + 2);
+    t->edom.resize(ne + 1);
+    uint64_t got = 0;
+    rc = xkrt_hip_pool_edges_copy(t->pool, t->e32.data(), t->edom.data(), ne, &got);
+    if (rc) return rc;
+    for (uint64_t i = 0; i < got; ++i) {
+        const uint32_t di = t->edom[i];
+        if (di >= t->ids.size()) return -EIO;
+        const std::vector<uint64_t> &m = t->ids[di];
+        const uint32_t p = t->e32[2 * i], q = t->e32[2 * i + 1];
+        if (p >= m.size() || q >= m.size()) return -EIO;
+        edges[2 * i] = m[p];
+        edges[2 * i + 1] = m[q];
+    }
+    *n = got;
+    t->edges_out += got;
+    return 0;
+}
+
+int xkrt_hip_taskdom_stats(xkrt_hip_taskdom_t t, xkrt_hip_taskdom_stats_t *out)
+{
+    if (!t || !out) return -EINVAL;
+    memset(out, 0, sizeof(*out));
+    out->accesses = t->next_id;
+    out->segments = t->nseg;
+    out->blas_domains = t->blas.size();
+    out->launches = t->launches;
+    out->edges = t->edges_out;
+    return 0;
+}
+
+} // extern "C"

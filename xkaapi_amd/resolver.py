@@ -14,6 +14,7 @@ from __future__ import annotations
 import ctypes as C
 import errno
 import os
+import weakref
 
 import numpy as np
 
@@ -69,6 +70,27 @@ class ServerStatsT(C.Structure):
                                           "consumes", "gated")] + [("error", C.c_int64)]
 
 
+class TaskAccessT(C.Structure):
+    _fields_ = [("addr", C.c_uint64), ("m", C.c_uint64), ("n", C.c_uint64), ("ld", C.c_uint64),
+                ("sizeof_type", C.c_uint32), ("task", C.c_uint32), ("mode", C.c_uint8), ("kind", C.c_uint8),
+                ("pad", C.c_uint8 * 6)]
+
+
+TASK_ACCESS_DTYPE = np.dtype([("addr", "<u8"), ("m", "<u8"), ("n", "<u8"), ("ld", "<u8"), ("sizeof_type", "<u4"),
+                              ("task", "<u4"), ("mode", "u1"), ("kind", "u1"), ("pad", "u1", (6,))])
+assert TASK_ACCESS_DTYPE.itemsize == C.sizeof(TaskAccessT) == 48
+
+
+class TaskDomConfigT(C.Structure):
+    _fields_ = [("kernel", C.c_uint32), ("pad", C.c_uint32), ("max_rects", C.c_uint64), ("max_edges", C.c_uint64),
+                ("budget_bytes", C.c_uint64), ("seg_col_bytes", C.c_uint64), ("seg_bin_x_bytes", C.c_uint32),
+                ("seg_bin_y_cols", C.c_uint32), ("blas_bin_x_bytes", C.c_uint32), ("blas_bin_y_cols", C.c_uint32)]
+
+
+class TaskDomStatsT(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("accesses", "segments", "blas_domains", "launches", "edges")]
+
+
 # device error bits: 2, 4, 8 and 128 belong to one domain (the pool's other domains go on);
 # 16, 32 and 64 to the pool (shared launch scratch overflowed)
 ERR_BITS = {2: "bin record list", 4: "bin fragments", 8: "bin reader log", 128: "record outside its frame",
@@ -112,6 +134,9 @@ def hip_lib():
     L.xkrt_hip_pool_create.argtypes = [C.POINTER(PoolConfigT), _p, C.POINTER(_p)]
     L.xkrt_hip_pool_destroy.argtypes = [_p]
     L.xkrt_hip_pool_domain_create.argtypes = [_p, _u64, _u64, _u32, _u32, _u64, _u64, C.POINTER(_p)]
+    L.xkrt_hip_seg_domain_create.argtypes = [_u64, C.POINTER(ConfigT), _p, C.POINTER(_p)]
+    L.xkrt_hip_pool_seg_domain_create.argtypes = [_p, _u64, _u32, _u32, _u64, _u64, C.POINTER(_p)]
+    L.xkrt_hip_dep_replay_segments.argtypes = [_p, _p, _u64]
     L.xkrt_hip_pool_stage.argtypes = [_p, _p, _p, _u64, C.POINTER(_u64)]
     L.xkrt_hip_pool_launch.argtypes = [_p, C.POINTER(_u64)]
     L.xkrt_hip_pool_poll.argtypes = [_p, C.POINTER(_u64)]
@@ -147,6 +172,13 @@ def hip_lib():
     L.xkrt_hip_server_times.argtypes = [_p, _u64, _u64, _p, _p, _p]
     L.xkrt_hip_server_edges.argtypes = [_p, _u32, _p, _u64, C.POINTER(_u64)]
     L.xkrt_hip_server_stats.argtypes = [_p, C.POINTER(ServerStatsT)]
+    L.xkrt_hip_pool_stage_group.argtypes = [_p, _u32, _p, _p, _p, _p]
+    L.xkrt_hip_taskdom_create.argtypes = [C.POINTER(TaskDomConfigT), _p, C.POINTER(_p)]
+    L.xkrt_hip_taskdom_destroy.argtypes = [_p]
+    L.xkrt_hip_taskdom_submit.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_taskdom_sync.argtypes = [_p, C.POINTER(_u64)]
+    L.xkrt_hip_taskdom_edges_copy.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_taskdom_stats.argtypes = [_p, C.POINTER(TaskDomStatsT)]
     _LIB = L
     return L
 
@@ -180,17 +212,29 @@ class DependencyDomain:
 
     def __init__(self, ld: int, sizeof_type: int, bin_x_bytes: int = 1024, bin_y_cols: int = 128,
                  kernel: int = 0, max_rects: int = 1 << 20, max_edges: int = 0, budget_bytes: int = 0,
-                 y_hint: tuple[int, int] | None = None, stream: int | None = None, device: int = 0):
+                 y_hint: tuple[int, int] | None = None, stream: int | None = None, device: int = 0,
+                 _segments: bool = False):
         self.L = hip_lib()
         _chk(self.L.xkrt_hip_init(device), "xkrt_hip_init")
         cfg = ConfigT(bin_x_bytes, bin_y_cols, kernel, 0, max_rects, max_edges, budget_bytes,
                       y_hint[0] if y_hint else 0, y_hint[1] if y_hint else 0)
         h = _p()
-        _chk(self.L.xkrt_hip_dep_domain_create(ld, sizeof_type, C.byref(cfg), stream, C.byref(h)),
-             "xkrt_hip_dep_domain_create")
+        if _segments:
+            _chk(self.L.xkrt_hip_seg_domain_create(ld, C.byref(cfg), stream, C.byref(h)),
+                 "xkrt_hip_seg_domain_create")
+        else:
+            _chk(self.L.xkrt_hip_dep_domain_create(ld, sizeof_type, C.byref(cfg), stream, C.byref(h)),
+                 "xkrt_hip_dep_domain_create")
         self.h = h
         self.ld, self.s = ld, sizeof_type
         self.max_rects = max_rects
+
+    def replay_segments(self, acc: np.ndarray):
+        """Put uncompleted segment accesses into this (new) BLAS domain, with no
+        LINK (src/task/task.cc:177-198): no edge changes; the inserted rects'
+        extent is recorded for the wrapped-tile check."""
+        acc = np.ascontiguousarray(acc, ACCESS_DTYPE)
+        _chk(self.L.xkrt_hip_dep_replay_segments(self.h, acc.ctypes.data, len(acc)), "xkrt_hip_dep_replay_segments")
 
     def close(self):
         if getattr(self, "h", None):
@@ -298,18 +342,36 @@ class DependencyDomain:
         return {f: int(getattr(st, f)) for f, _ in StatsT._fields_}
 
 
+class SegmentDomain(DependencyDomain):
+    """A GPU-resident segment dependency domain (the interval tree of
+    interval/dependency-tree.hpp:187-363): byte segments [addr, addr + m) only,
+    cut into a frame of col_bytes-byte columns."""
+
+    def __init__(self, col_bytes: int = 1 << 16, bin_x_bytes: int = 4096, bin_y_cols: int = 1, **kw):
+        super().__init__(col_bytes, 1, bin_x_bytes, bin_y_cols, _segments=True, **kw)
+
+    def replay_segments(self, acc):
+        raise ResolverError(-22, "replay_segments: a segment domain takes no replay")
+
+
 class PoolDomain(DependencyDomain):
     """A domain admitted into a Pool (its bins live in the pool's arrays)."""
 
     def __init__(self, pool: "Pool", ld: int, sizeof_type: int, y_range: tuple[int, int], bin_x_bytes: int = 1024,
-                 bin_y_cols: int = 128):
+                 bin_y_cols: int = 128, segments: bool = False):
         self.L = hip_lib()
         h = _p()
-        _chk(self.L.xkrt_hip_pool_domain_create(pool.h, ld, sizeof_type, bin_x_bytes, bin_y_cols, int(y_range[0]),
-                                                int(y_range[1]), C.byref(h)), "xkrt_hip_pool_domain_create")
+        if segments:
+            _chk(self.L.xkrt_hip_pool_seg_domain_create(pool.h, ld, bin_x_bytes, bin_y_cols, int(y_range[0]),
+                                                        int(y_range[1]), C.byref(h)), "xkrt_hip_pool_seg_domain_create")
+        else:
+            _chk(self.L.xkrt_hip_pool_domain_create(pool.h, ld, sizeof_type, bin_x_bytes, bin_y_cols,
+                                                    int(y_range[0]), int(y_range[1]), C.byref(h)),
+                 "xkrt_hip_pool_domain_create")
         self.h = h
         self.pool = pool
         self.ld, self.s = ld, sizeof_type
+        pool._domains.add(self)
 
     def close(self):
         # a destroyed pool has already released its domains
@@ -334,9 +396,13 @@ class Pool:
         _chk(self.L.xkrt_hip_pool_create(C.byref(cfg), stream, C.byref(h)), "xkrt_hip_pool_create")
         self.h = h
         self.max_rects = max_rects
+        self._domains = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "h", None):
+            # the pool releases its domains: their handles die with it (later calls raise)
+            for d in list(self._domains):
+                d.h = None
             self.L.xkrt_hip_pool_destroy(self.h)
             self.h = None
 
@@ -356,9 +422,24 @@ class Pool:
         """Admit a domain; raises ResolverError(-ENOMEM) when the budget cannot hold it now."""
         return PoolDomain(self, ld, sizeof_type, y_range, bin_x_bytes, bin_y_cols)
 
+    def segment_domain(self, col_bytes: int, y_range, bin_x_bytes: int = 4096, bin_y_cols: int = 1):
+        """Admit a segment domain over addresses [y_range[0], y_range[1]) * col_bytes."""
+        return PoolDomain(self, col_bytes, 1, y_range, bin_x_bytes, bin_y_cols, segments=True)
+
     def leave(self, dom: PoolDomain):
         """The domain's client is done: its bins return to the pool."""
         dom.close()
+
+    def stage_group(self, doms, accs):
+        """Stage one batch per domain as one unit (all or none); returns the first ids."""
+        nd = len(doms)
+        accs = [np.ascontiguousarray(a, ACCESS_DTYPE) for a in accs]
+        dh = (_p * max(nd, 1))(*[d.h for d in doms])
+        ap = (_p * max(nd, 1))(*[a.ctypes.data for a in accs])
+        cn = (_u64 * max(nd, 1))(*[len(a) for a in accs])
+        first = (_u64 * max(nd, 1))()
+        _chk(self.L.xkrt_hip_pool_stage_group(self.h, nd, dh, ap, cn, first), "xkrt_hip_pool_stage_group")
+        return [int(first[k]) for k in range(nd)]
 
     def stage(self, dom: PoolDomain, acc: np.ndarray) -> int:
         acc = np.ascontiguousarray(acc, ACCESS_DTYPE)
@@ -503,6 +584,77 @@ class Server:
         st = ServerStatsT()
         _chk(self.L.xkrt_hip_server_stats(self.h, C.byref(st)), "xkrt_hip_server_stats")
         return {f: int(getattr(st, f)) for f, _ in ServerStatsT._fields_}
+
+
+def to_task_access_array(task, mode, kind, addr, m, n, ld, sizeof_type) -> np.ndarray:
+    """Task-domain accesses: tiles (kind 0: addr, m rows, n columns, ld, sizeof_type)
+    and segments (kind 1: [addr, addr + m))."""
+    a = np.zeros(len(task), TASK_ACCESS_DTYPE)
+    a["task"], a["mode"], a["kind"] = task, mode, kind
+    a["addr"], a["m"], a["n"] = addr, m, n
+    a["ld"], a["sizeof_type"] = ld, sizeof_type
+    return a
+
+
+class TaskDomain:
+    """One task domain's dependency resolution over segments and tiles of any
+    (ld, sizeof_type) -- the dispatch of task_dependency_resolve
+    (src/task/task.cc:204-299) over a segment domain and one BLAS domain per
+    pair, all in one pool.  Access ids are program-wide."""
+
+    def __init__(self, kernel: int = 1, max_rects: int = 1 << 20, max_edges: int = 1 << 22, budget_bytes: int = 0,
+                 seg_col_bytes: int = 1 << 16, seg_bin: tuple[int, int] = (0, 0), blas_bin: tuple[int, int] = (1024, 128),
+                 stream: int | None = None, device: int = 0):
+        self.L = hip_lib()
+        _chk(self.L.xkrt_hip_init(device), "xkrt_hip_init")
+        cfg = TaskDomConfigT(kernel, 0, max_rects, max_edges, budget_bytes, seg_col_bytes, seg_bin[0], seg_bin[1],
+                             blas_bin[0], blas_bin[1])
+        h = _p()
+        _chk(self.L.xkrt_hip_taskdom_create(C.byref(cfg), stream, C.byref(h)), "xkrt_hip_taskdom_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.xkrt_hip_taskdom_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, acc: np.ndarray) -> int:
+        acc = np.ascontiguousarray(acc, TASK_ACCESS_DTYPE)
+        first = _u64()
+        _chk(self.L.xkrt_hip_taskdom_submit(self.h, acc.ctypes.data, len(acc), C.byref(first)),
+             "xkrt_hip_taskdom_submit")
+        return int(first.value)
+
+    def sync(self) -> int:
+        n = _u64()
+        _chk(self.L.xkrt_hip_taskdom_sync(self.h, C.byref(n)), "xkrt_hip_taskdom_sync")
+        return int(n.value)
+
+    def edges(self) -> np.ndarray:
+        """Sync, then copy out and consume the waiting edges: (E, 2) uint64 program ids."""
+        n = self.sync()
+        out = np.empty((max(n, 1), 2), np.uint64)
+        got = _u64()
+        _chk(self.L.xkrt_hip_taskdom_edges_copy(self.h, out.ctypes.data, n, C.byref(got)),
+             "xkrt_hip_taskdom_edges_copy")
+        return out[:int(got.value)]
+
+    def stats(self) -> dict:
+        st = TaskDomStatsT()
+        _chk(self.L.xkrt_hip_taskdom_stats(self.h, C.byref(st)), "xkrt_hip_taskdom_stats")
+        return {f: int(getattr(st, f)) for f, _ in TaskDomStatsT._fields_}
 
 
 def batches_by_tasks(task: np.ndarray, tasks_per_batch) -> list[tuple[int, int]]:
