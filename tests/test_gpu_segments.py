@@ -1,17 +1,7 @@
-"""Segment dependency domains and task domains on the GPU, against the oracle.
-
-* A segment domain (the reference's interval tree, interval/dependency-tree.hpp:
-  187-363) gives the oracle's K = 1 edges (pinned by the reference's
-  KHPTree<1>), for several column widths and bin shapes, in several batchings,
-  at 64-bit addresses.
-* A pool resolves a segment domain and a BLAS domain in one launch.
-* A task domain over a mixed program (segments, tiles of three (ld, s) pairs,
-  one pair's domain created late so that segments are replayed) gives the
-  oracle's composition, pinned by the reference's own dispatch.
-* Refusals leave nothing behind: a staged group is all or nothing, and a task
-  domain batch refused with -ENOTSUP changes no later edge.
-"""
-import errno
+"""Segment (1-D) accesses seen by a BLAS domain on the GPU (through the C ABI):
+the golden fixture from the reference engine, random mixed streams against
+the oracle, and the refusals where the reference marks whole nodes."""
+import os
 
 import numpy as np
 import pytest
@@ -21,191 +11,52 @@ from oracle import segments as G
 from xkaapi_amd import resolver as R
 
 pytestmark = pytest.mark.gpu
-
-BASE = 0x7F12_3456_0000
-MODES = np.array([1, 2, 3, 4, 5, 6, 8, 9], np.uint8)
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def pairs(x):
-    return np.unique(np.asarray(x, np.int64).reshape(-1, 2), axis=0)
+def gpu_edges(st, ld, s, tpb, **kw):
+    acc = R.to_access_array(st["task"], st["mode"], st["addr"], st["m"], st["n"], kind=st["kind"])
+    with R.DependencyDomain(ld, s, **kw) as dom:
+        first = None
+        for a, b in R.batches_by_tasks(st["task"], tpb):
+            f = dom.submit(acc[a:b])
+            first = f if first is None else first
+        e = dom.edges().astype(np.int64) - (first or 0)
+    return np.unique(e, axis=0) if len(e) else e.reshape(0, 2)
 
 
-def same(x, y):
-    return np.array_equal(pairs(x), pairs(y))
+def test_golden_fixture():
+    z = np.load(os.path.join(GOLDEN, "segments_mixed.npz"))
+    st = {k: z[k] for k in ("task", "mode", "kind", "addr", "m", "n")}
+    for bx, by, tpb, kc in ((32, 4, 50, 2), (16, 2, 7, 2), (64, 8, 2000, 2)):
+        got = gpu_edges(st, int(z["ld"]), int(z["s"]), tpb, bin_x_bytes=bx, bin_y_cols=by, kernel=kc)
+        assert np.array_equal(got, z["edges"]), (bx, by, tpb)
 
 
-def seg_stream(seed, n, space, maxsz, per_task=3, base=BASE):
-    rng = np.random.default_rng(seed)
-    k = rng.integers(1, per_task + 1, n)
-    task = np.repeat(np.arange(n, dtype=np.uint32), k)[:n]
-    mode = rng.choice(MODES, n)
-    a = rng.integers(0, space, n).astype(np.uint64)
-    size = np.exp2(rng.random(n) * np.log2(maxsz)).astype(np.uint64) + 1
-    b = np.minimum(a + size, space)
-    return task, mode, np.stack([a, b], axis=1) + np.uint64(base)
+@pytest.mark.parametrize("seed", range(3))
+def test_random_mixed_matches_oracle(seed):
+    ld, cols, s = 16, 30, 2
+    st = G.mixed_stream(1500, ld, cols, s, 40 + seed, pseg=0.4, maxm=14, maxn=6)
+    rects, nrect, marks, nmark = G.stream_rects(st["kind"], st["addr"], st["m"], st["n"], ld, s)
+    want = np.unique(khp.resolve_marks(st["task"], st["mode"], rects, nrect, marks, nmark).astype(np.int64), axis=0)
+    for bx, by, tpb, kc in ((32, 8, 100, 1), (16, 4, 13, 2)):
+        assert np.array_equal(gpu_edges(st, ld, s, tpb, bin_x_bytes=bx, bin_y_cols=by, kernel=kc), want)
 
 
-def seg_acc(task, mode, seg):
-    return R.to_access_array(task, mode, seg[:, 0], seg[:, 1] - seg[:, 0], 0,
-                             kind=np.full(len(task), R.KIND_SEGMENT, np.uint8))
-
-
-@pytest.mark.parametrize("seed,col,bx,by,tasks,kernel", [
-    (0, 4096, 4096, 4, 500, 2),
-    (1, 4096, 1024, 8, 2000, 2),
-    (2, 1 << 16, 4096, 1, 100000, 1),
-    (3, 256, 256, 64, 300, 2),
-])
-def test_segment_domain_matches_oracle(seed, col, bx, by, tasks, kernel):
-    task, mode, seg = seg_stream(seed, 12000, 1 << 20, 1 << 14)
-    want = khp.resolve_intervals(task, mode, seg)
-    acc = seg_acc(task, mode, seg)
-    print(f"segment domain seed {seed} col {col} bins {bx}x{by} tasks/batch {tasks}", flush=True)
-    with R.SegmentDomain(col, bx, by, kernel=kernel) as d:
-        for a, b in R.batches_by_tasks(task, tasks):
-            d.submit(acc[a:b])
-        d.sync()
-        got = d.edges()
-        st = d.stats()
-    assert st["err"] == 0
-    assert len(want) > 5000
-    assert same(got, want)
-
-
-def test_segment_domain_refuses_tiles():
-    with R.SegmentDomain(4096, 4096, 4) as d:
-        tile = R.to_access_array([0], [2], [BASE], [4], [4])
-        with pytest.raises(R.ResolverError) as e:
-            d.submit(tile)
-        assert e.value.rc == -errno.EINVAL
-
-
-def test_pool_resolves_segment_and_blas_domains_together():
-    task, mode, seg = seg_stream(9, 4000, 1 << 18, 1 << 12, base=1 << 30)
-    want_seg = khp.resolve_intervals(task, mode, seg)
-    from xkaapi_amd import streams as S
-    ch = S.cholesky(1024, 128)
-    r, nr = ch.rects()
-    want_ch = khp.resolve(ch.task, ch.mode, r, nr)
-    print("pool: segment + cholesky domains", flush=True)
-    with R.Pool(kernel=1, max_rects=1 << 18) as p:
-        ds = p.segment_domain(4096, (0, 0), 1024, 1)
-        dc = p.domain(1024, 8, (0, 1024))
-        p.stage_group([ds, dc], [seg_acc(task, mode, seg), R.to_access_array(ch.task, ch.mode, ch.addr, ch.m, ch.n)])
-        p.launch()
-        p.sync()
-        e, dom = p.edges()
-        ids = ds.index, dc.index
-    assert same(e[dom == ids[0]], want_seg)
-    assert same(e[dom == ids[1]], want_ch)
-
-
-# 1 KiB bins for every domain of the 64 KiB programs
-TD = dict(kernel=2, seg_col_bytes=4096, seg_bin=(1024, 1), blas_bin=(128, 8))
-
-
-def program_args(p):
-    return [p[k] for k in ("task", "mode", "kind", "addr", "m", "n", "ld", "s")]
-
-
-def program_acc(p):
-    return R.to_task_access_array(p["task"], p["mode"], p["kind"], p["addr"], p["m"], p["n"], p["ld"], p["s"])
-
-
-@pytest.mark.parametrize("seed,tasks,base", [(0, 50, 1 << 20), (1, 400, 1 << 20), (2, 3000, BASE), (3, 7, BASE)])
-def test_task_domain_matches_oracle(seed, tasks, base):
-    p = G.program_stream(6000, seed, base=base, span=1 << 16)
-    want = khp.resolve_program(*program_args(p))
-    acc = program_acc(p)
-    print(f"task domain seed {seed} tasks/batch {tasks} base {base:#x}", flush=True)
-    with R.TaskDomain(**TD) as t:
-        for a, b in R.batches_by_tasks(p["task"], tasks):
-            first = t.submit(acc[a:b])
-            assert first == a
-        got = t.edges()
-        st = t.stats()
-    assert st["blas_domains"] == 3 and st["segments"] == int((p["kind"] == 1).sum())
-    assert len(want) > 3000
-    assert same(got, want)
-
-
-def test_task_domain_edges_in_pieces():
-    # edges copied out after every few batches: their union is the program's edges
-    p = G.program_stream(3000, 5)
-    want = khp.resolve_program(*program_args(p))
-    acc = program_acc(p)
-    got = []
-    with R.TaskDomain(**TD) as t:
-        for k, (a, b) in enumerate(R.batches_by_tasks(p["task"], 200)):
-            t.submit(acc[a:b])
-            if k % 3 == 2:
-                got.append(t.edges())
-        got.append(t.edges())
-        assert t.stats()["edges"] == sum(len(g) for g in got)
-    assert same(np.concatenate(got), want)
-
-
-def test_replay_refusals_and_wrapped_tile_guard():
-    # a BLAS domain that gets a long segment replayed refuses a later wrapped tile; a segment
-    # in the first ld * s bytes is refused and records nothing
-    with R.DependencyDomain(64, 8) as d:  # LDs = 512 B
-        seg = R.to_access_array([0], [2], [1 << 20], [3000], [0], kind=[R.KIND_SEGMENT])
-        low = R.to_access_array([0], [2], [100], [10], [0], kind=[R.KIND_SEGMENT])
-        with pytest.raises(R.ResolverError) as e:
-            d.replay_segments(low)
-        assert e.value.rc == -errno.ENOTSUP
-        # before the replay a wrapped tile at (1 MiB + 496 B) with 4 rows is accepted elsewhere;
-        # after it, the whole-column rect of the replayed segment spans its wrap gap
-        wrapped = R.to_access_array([1], [2], [(1 << 20) + 8 * 62], [4], [2])
-        d.replay_segments(seg)
-        with pytest.raises(R.ResolverError) as e:
-            d.submit(wrapped)
-        assert e.value.rc == -errno.ENOTSUP
-    with R.DependencyDomain(64, 8) as d:
-        d.submit(R.to_access_array([1], [2], [(1 << 20) + 8 * 62], [4], [2]))
-        d.sync()
-
-
-def test_stage_group_is_all_or_nothing():
-    # the second domain refuses its batch (a segment in its first column): the first domain's
-    # batch is not staged either, and its next batch gets the same first id
-    task, mode, seg = seg_stream(4, 500, 1 << 16, 1 << 10, base=1 << 24)
-    acc = seg_acc(task, mode, seg)
-    with R.Pool(kernel=1, max_rects=1 << 16) as p:
-        ds = p.segment_domain(4096, (0, 0), 1024, 1)
-        db = p.domain(64, 8, (0, 0))
-        bad = R.to_access_array([0], [2], [16], [8], [0], kind=[R.KIND_SEGMENT])
-        with pytest.raises(R.ResolverError) as e:
-            p.stage_group([ds, db], [acc[:200], bad])
-        assert e.value.rc == -errno.ENOTSUP
-        assert p.open()[0] == 0
-        first = p.stage_group([ds], [acc])
-        assert first == [0]
-        p.launch()
-        p.sync()
-        st = ds.stats()
-        e, dom = p.edges()
-        idx = ds.index
-    with pytest.raises(R.ResolverError):
-        ds.index  # the pool released its domains
-    assert st["err"] == 0
-    assert same(e[dom == idx], khp.resolve_intervals(task, mode, seg))
-
-
-def test_task_domain_refused_batch_changes_nothing():
-    p = G.program_stream(2000, 6)
-    want = khp.resolve_program(*program_args(p))
-    acc = program_acc(p)
-    batches = R.batches_by_tasks(p["task"], 100)
-    with R.TaskDomain(**TD) as t:
-        for k, (a, b) in enumerate(batches):
-            if k == len(batches) // 2:
-                # a task with a tile and a segment in the first bytes of memory: refused whole
-                bad = R.to_task_access_array([p["task"][a]] * 2, [2, 1], [0, 1], [1 << 20, 8], [1, 64], [1, 0],
-                                             [32, 0], [4, 0])
-                with pytest.raises(R.ResolverError) as e:
-                    t.submit(bad)
-                assert e.value.rc == -errno.ENOTSUP
-            t.submit(acc[a:b])
-        got = t.edges()
-    assert same(got, want)
+def test_segment_refused_once_column_zero_is_touched():
+    ld, s = 8, 1
+    tile0 = R.to_access_array(np.array([0], np.uint32), np.array([2], np.uint8), np.array([0], np.uint64),
+                              np.array([8], np.uint64), np.array([2], np.uint64))
+    seg = R.to_access_array(np.array([1], np.uint32), np.array([1], np.uint8), np.array([42], np.uint64),
+                            np.array([3], np.uint64), np.array([1], np.uint64), kind=np.array([1], np.uint8))
+    with R.DependencyDomain(ld, s, bin_x_bytes=8, bin_y_cols=4) as dom:
+        dom.submit(tile0)
+        with pytest.raises(R.ResolverError) as ei:
+            dom.submit(seg)
+        assert ei.value.rc == -95  # -ENOTSUP
+    low = R.to_access_array(np.array([0], np.uint32), np.array([1], np.uint8), np.array([3], np.uint64),
+                            np.array([4], np.uint64), np.array([1], np.uint64), kind=np.array([1], np.uint8))
+    with R.DependencyDomain(ld, s, bin_x_bytes=8, bin_y_cols=4) as dom:
+        with pytest.raises(R.ResolverError) as ei:
+            dom.submit(low)
+        assert ei.value.rc == -95
