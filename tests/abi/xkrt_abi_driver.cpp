@@ -8,11 +8,13 @@
 //      is refused with -EINVAL;
 //   2. a pool with two domains (Cholesky and LU, 1024^2 fp64, 128^2 tiles)
 //      staged together and resolved by one launch, edges with their domain;
-//   3. a coherency tree: who_owns and who_owns_static agree.
+//   3. a coherency tree: who_owns and who_owns_static agree;
+//   4. a task domain over a mixed program (program() below: tiles of two
+//      (ld, sizeof_type) pairs and byte segments), submitted in batches.
 //
 // Output lines: "chol <pred> <succ>", "pool <domain> <pred> <succ>",
-// "domain <name> <index>", "who_owns_equal <0|1>", "ok".  Exit status 0 on
-// success.
+// "domain <name> <index>", "who_owns_equal <0|1>", "prog <pred> <succ>",
+// "ok".  Exit status 0 on success.
 
 #include "xkrt_hip.h"
 
@@ -236,12 +238,93 @@ int run_coherency()
     return o1 == o2 ? 0 : 1;
 }
 
+// A mixed program of one task domain, generated the same way by the test:
+// 160 tasks over a 256 x 256 fp64 matrix at 1 MiB (32 x 32 tiles) viewed
+// with two leading dimensions (256 x 8 B and 512 x 4 B: two BLAS domains) and
+// byte segments over the same bytes.
+std::vector<xkrt_hip_task_access_t> program()
+{
+    std::vector<xkrt_hip_task_access_t> v;
+    const uint64_t base = 1u << 20;
+    for (uint32_t t = 0; t < 160; ++t) {
+        xkrt_hip_task_access_t a{};
+        a.task = t;
+        const uint32_t k = t % 64, i = k % 8, j = (k * 3 + t / 64) % 8;
+        if (t % 5 == 4) {  // a segment: [base + 4096 t mod 512 KiB, + 5000)
+            a.kind = XKRT_HIP_KIND_SEGMENT;
+            a.addr = base + (4096ull * t) % (512u << 10);
+            a.m = 5000;
+            a.mode = t % 2 ? XKRT_HIP_MODE_R : XKRT_HIP_MODE_W;
+            v.push_back(a);
+            continue;
+        }
+        a.kind = XKRT_HIP_KIND_TILE;
+        if (t % 3 == 0) {  // tile (i, j) of the 512 x 4 B view: 64 rows of 4 B
+            a.ld = 512;
+            a.sizeof_type = 4;
+            a.addr = base + (j * 32ull * 512 + i * 64) * 4;
+            a.m = 64;
+        } else {            // tile (i, j) of the 256 x 8 B view
+            a.ld = 256;
+            a.sizeof_type = 8;
+            a.addr = base + (j * 32ull * 256 + i * 32) * 8;
+            a.m = 32;
+        }
+        a.n = 32;
+        a.mode = XKRT_HIP_MODE_RW;
+        v.push_back(a);
+        if (t % 7 == 0) {  // and a segment read of its first column
+            xkrt_hip_task_access_t r = a;
+            r.kind = XKRT_HIP_KIND_SEGMENT;
+            r.m = 256;
+            r.mode = XKRT_HIP_MODE_R;
+            v.push_back(r);
+        }
+    }
+    return v;
+}
+
+int run_taskdom()
+{
+    xkrt_hip_taskdom_config_t c{};
+    c.kernel = 1;
+    c.seg_col_bytes = 4096;
+    c.seg_bin_x_bytes = 1024;
+    c.seg_bin_y_cols = 1;
+    c.blas_bin_x_bytes = 512;
+    c.blas_bin_y_cols = 16;
+    xkrt_hip_taskdom_t t = nullptr;
+    CHECK(xkrt_hip_taskdom_create(&c, nullptr, &t));
+    const std::vector<xkrt_hip_task_access_t> acc = program();
+    size_t a = 0;
+    while (a < acc.size()) {  // batches of 16 tasks
+        size_t b = a;
+        while (b < acc.size() && acc[b].task < acc[a].task + 16) ++b;
+        uint64_t first = 0;
+        CHECK(xkrt_hip_taskdom_submit(t, acc.data() + a, b - a, &first));
+        if (first != a) return 1;
+        a = b;
+    }
+    uint64_t n = 0;
+    CHECK(xkrt_hip_taskdom_sync(t, &n));
+    std::vector<uint64_t> e(2 * n + 2);
+    uint64_t got = 0;
+    CHECK(xkrt_hip_taskdom_edges_copy(t, e.data(), n, &got));
+    for (uint64_t i = 0; i < got; ++i)
+        std::printf("prog %llu %llu\n", (unsigned long long)e[2 * i], (unsigned long long)e[2 * i + 1]);
+    xkrt_hip_taskdom_stats_t st{};
+    CHECK(xkrt_hip_taskdom_stats(t, &st));
+    if (st.blas_domains != 2 || st.accesses != acc.size()) return 1;
+    CHECK(xkrt_hip_taskdom_destroy(t));
+    return 0;
+}
+
 } // namespace
 
 int main()
 {
     CHECK(xkrt_hip_init(0));
-    if (run_domain() || run_pool() || run_coherency()) return 1;
+    if (run_domain() || run_pool() || run_coherency() || run_taskdom()) return 1;
     std::printf("ok\n");
     return 0;
 }

@@ -12,6 +12,11 @@ Fixtures (npz, allow_pickle=False):
   partition_coherency.npz    memory-tree partition geometry of 500 coherency accesses (view 0)
   segments_mixed.npz         tiles and byte segments (at or above the first column) in one BLAS
                              domain: (pred, succ) access-index edges, unique sorted rows
+  intervals_k1.npz           byte segments at 64-bit addresses in one segment domain (the
+                             reference's KHPTree<1> with the interval-tree hooks): (pred, succ) rows
+  program_mixed.npz          one task domain over segments and tiles of three (ld, s) pairs (the
+                             reference's dispatch over its own trees): (pred, succ) rows
+Run with fixture names to write only those.
 Each resolver fixture holds the stream (task, slot, mode, addr, ld, m, n, s),
 the edges as (pred_task, pred_slot, succ_task, succ_slot) sorted rows, and
 the per-access fragment counts F_i.
@@ -56,9 +61,39 @@ def save_resolver(name, st):
     print(name, len(st), "accesses", len(e), "edges")
 
 
-def main():
+def intervals_stream(n=3000, space=1 << 20, maxsz=1 << 14, base=0x7F12_3456_0000, seed=23):
+    rng = np.random.default_rng(seed)
+    task = np.repeat(np.arange(n, dtype=np.uint32), rng.integers(1, 4, n))[:n]
+    mode = rng.choice(np.array([1, 2, 3, 4, 5, 6, 8, 9], np.uint8), n)
+    a = rng.integers(0, space, n).astype(np.uint64)
+    b = np.minimum(a + np.exp2(rng.random(n) * np.log2(maxsz)).astype(np.uint64) + np.uint64(1), np.uint64(space))
+    return task, mode, np.stack([a, b], axis=1) + np.uint64(base)
+
+
+def save_intervals():
+    task, mode, seg = intervals_stream()
+    e = refkhp.resolve_intervals(task, mode, seg)
+    np.savez_compressed(os.path.join(HERE, "intervals_k1.npz"), task=task, mode=mode, seg=seg,
+                        edges=np.unique(e.astype(np.int64), axis=0))
+    print("intervals_k1.npz", len(task), "accesses", len(e), "edges")
+
+
+def save_program():
+    p = G.program_stream(3000, 21)
+    e = refkhp.resolve_program(*[p[k] for k in ("task", "mode", "kind", "addr", "m", "n", "ld", "s")])
+    np.savez_compressed(os.path.join(HERE, "program_mixed.npz"), **p, edges=np.unique(e.astype(np.int64), axis=0))
+    print("program_mixed.npz", len(p["task"]), "accesses", len(e), "edges")
+
+
+def main(names=()):
     if not refkhp.available():
         raise SystemExit("oracle/_ref/libkhpref.so missing: run make -C oracle")
+    if names:
+        for nm in names:
+            {"intervals_k1.npz": save_intervals, "program_mixed.npz": save_program}[nm]()
+        return
+    save_intervals()
+    save_program()
     save_resolver("cholesky_t2_edges.npz", S.cholesky(512, 256))
     save_resolver("cholesky_4096_nb256.npz", S.cholesky(4096, 256))
     save_resolver("lu_1024_nb128.npz", S.lu(1024, 128))
@@ -83,4 +118,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

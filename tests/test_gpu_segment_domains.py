@@ -12,6 +12,7 @@
   domain batch refused with -ENOTSUP changes no later edge.
 """
 import errno
+import os
 
 import numpy as np
 import pytest
@@ -23,6 +24,7 @@ from xkaapi_amd import resolver as R
 pytestmark = pytest.mark.gpu
 
 BASE = 0x7F12_3456_0000
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 MODES = np.array([1, 2, 3, 4, 5, 6, 8, 9], np.uint8)
 
 
@@ -70,6 +72,18 @@ def test_segment_domain_matches_oracle(seed, col, bx, by, tasks, kernel):
     assert st["err"] == 0
     assert len(want) > 5000
     assert same(got, want)
+
+
+def test_segment_domain_matches_golden_fixture():
+    # the reference's KHPTree<1> edges (intervals_k1.npz, written by tests/golden/make_golden.py)
+    z = np.load(os.path.join(GOLDEN, "intervals_k1.npz"))
+    acc = seg_acc(z["task"], z["mode"], z["seg"])
+    for col, bx, by, tasks in ((4096, 1024, 1, 100), (1 << 16, 4096, 1, 5000)):
+        with R.SegmentDomain(col, bx, by, kernel=2) as d:
+            for a, b in R.batches_by_tasks(z["task"], tasks):
+                d.submit(acc[a:b])
+            got = d.edges()
+        assert same(got, z["edges"]), (col, bx, by, tasks)
 
 
 def test_segment_domain_refuses_tiles():
@@ -127,6 +141,19 @@ def test_task_domain_matches_oracle(seed, tasks, base):
     assert st["blas_domains"] == 3 and st["segments"] == int((p["kind"] == 1).sum())
     assert len(want) > 3000
     assert same(got, want)
+
+
+def test_task_domain_matches_golden_fixture():
+    # the reference's dispatch over its own trees (program_mixed.npz)
+    z = np.load(os.path.join(GOLDEN, "program_mixed.npz"))
+    p = {k: z[k] for k in ("task", "mode", "kind", "addr", "m", "n", "ld", "s")}
+    acc = program_acc(p)
+    for tasks in (30, 2000):
+        with R.TaskDomain(**TD) as t:
+            for a, b in R.batches_by_tasks(p["task"], tasks):
+                t.submit(acc[a:b])
+            got = t.edges()
+        assert same(got, z["edges"]), tasks
 
 
 def test_task_domain_edges_in_pieces():

@@ -4,6 +4,8 @@ per-point brute force on a tiny address space; and the segment -> BLAS replay
 (src/task/task.cc:177-198): segments put into a new BLAS tree without a LINK
 change no edge of the later tiles (their marks meet only nodes starting at
 column 0), checked on the reference engine."""
+import os
+
 import numpy as np
 import pytest
 
@@ -158,3 +160,19 @@ def test_program_composition_planted_mistakes_fail(mistake):
     want = refkhp.resolve_program(*args)
     assert same(khp.resolve_program(*args), want)
     assert not same(khp.resolve_program(*args, _mistake=mistake), want)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_k1_oracle_matches_golden_fixture():
+    # intervals_k1.npz: the reference's KHPTree<1> edges, written by tests/golden/make_golden.py
+    z = np.load(f"{GOLDEN}/intervals_k1.npz")
+    assert same(khp.resolve_intervals(z["task"], z["mode"], z["seg"]), z["edges"])
+
+
+def test_program_oracle_matches_golden_fixture():
+    # program_mixed.npz: the reference's task-domain dispatch over its own trees
+    z = np.load(f"{GOLDEN}/program_mixed.npz")
+    args = [z[k] for k in ("task", "mode", "kind", "addr", "m", "n", "ld", "s")]
+    assert same(khp.resolve_program(*args), z["edges"])
