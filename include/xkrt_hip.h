@@ -300,9 +300,14 @@ int xkrt_hip_pool_stage(xkrt_hip_pool_t p, xkrt_hip_dep_domain_t d, const xkrt_h
  * unit: either all of them are appended to the open launch (first ids in
  * first_id[k], nullable) or none is, and the first refusal's code is returned
  * (-ENOSPC: they do not fit the open launch together; launch it and stage
- * again).  A task whose accesses go to several domains is staged this way. */
+ * again).  A task whose accesses go to several domains is staged this way.
+ * ids (nullable; ids[k] nullable): the access ids the edges of batch k name,
+ * instead of the domain's consecutive ids -- strictly increasing within the
+ * domain across its batches (-EINVAL otherwise), e.g. program-wide ids shared
+ * by several domains. */
 int xkrt_hip_pool_stage_group(xkrt_hip_pool_t p, uint32_t nd, const xkrt_hip_dep_domain_t *d,
-                              const xkrt_hip_access_t *const *acc, const uint64_t *n, uint64_t *first_id);
+                              const xkrt_hip_access_t *const *acc, const uint32_t *const *ids, const uint64_t *n,
+                              uint64_t *first_id);
 
 /* Resolves every staged batch in one launch (asynchronous).  *ticket (may
  * be NULL) = the launch's sequence number, 1 for the pool's first launch. */
@@ -320,6 +325,13 @@ int xkrt_hip_pool_sync(xkrt_hip_pool_t p, uint64_t *nedges);
  * pairs, doms = the domain index of each.  Edges of failed domains are
  * dropped (*n counts the rest). */
 int xkrt_hip_pool_edges_copy(xkrt_hip_pool_t p, uint32_t *edges, uint32_t *doms, uint64_t cap, uint64_t *n);
+/* For consumers on the device: the edge buffer ((pred, succ) u32 pairs), the
+ * domain of each edge and the device word holding the number of unconsumed
+ * edges (read it on the pool's stream, after a launch). */
+int xkrt_hip_pool_edges_device(xkrt_hip_pool_t p, const uint32_t **edges, const uint32_t **doms,
+                               const uint64_t **count);
+/* Consumes every edge on the device, in stream order (no host copy). */
+int xkrt_hip_pool_edges_consume(xkrt_hip_pool_t p);
 int xkrt_hip_pool_stats(xkrt_hip_pool_t p, xkrt_hip_pool_stats_t *out);
 int xkrt_hip_pool_timing(xkrt_hip_pool_t p, int enable);
 /* Per timed launch, milliseconds from its first kernel's start to its last kernel's end. */
@@ -436,7 +448,8 @@ typedef struct xkrt_hip_task_access {
 
 typedef struct xkrt_hip_taskdom_config {
     uint32_t kernel;          /* LDS capacity class of every domain (see xkrt_hip_dep_config_t) */
-    uint32_t pad;
+    uint32_t release;         /* 1: edges are applied on the device and tasks released there (see
+                                 xkrt_hip_taskdom_ready); 0: edges are copied out */
     uint64_t max_rects;       /* rect records per launch (0 = 2^20) */
     uint64_t max_edges;       /* edge buffer capacity before edges must be copied out (0 = 2^22) */
     uint64_t budget_bytes;    /* device memory budget of all its domains; 0 = unlimited */
@@ -445,6 +458,9 @@ typedef struct xkrt_hip_taskdom_config {
     uint32_t seg_bin_y_cols;
     uint32_t blas_bin_x_bytes;/* BLAS domain bins (0 = 1024 B x 128 columns) */
     uint32_t blas_bin_y_cols;
+    uint64_t max_tasks;       /* release: tasks of the domain's lifetime (0 = 2^20) */
+    uint64_t max_accesses;    /* release: accesses (0 = 4 x max_tasks) */
+    uint64_t max_succ;        /* release: successor-list entries, one per edge applied (0 = 2 x max_accesses) */
 } xkrt_hip_taskdom_config_t;
 
 typedef struct xkrt_hip_taskdom_stats {
@@ -453,6 +469,9 @@ typedef struct xkrt_hip_taskdom_stats {
     uint64_t blas_domains;    /* BLAS domains created */
     uint64_t launches;
     uint64_t edges;           /* edges copied out so far */
+    uint64_t succ_nodes;      /* release: edges applied on the device (successor-list entries) */
+    uint64_t released;        /* release: tasks released */
+    uint64_t completed;       /* release: tasks completed */
 } xkrt_hip_taskdom_stats_t;
 
 int xkrt_hip_taskdom_create(const xkrt_hip_taskdom_config_t *cfg, void *stream, xkrt_hip_taskdom_t *out);
@@ -471,8 +490,27 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
 int xkrt_hip_taskdom_sync(xkrt_hip_taskdom_t t, uint64_t *nedges);
 /* Copies out and consumes every waiting edge: (pred, succ) program access ids,
  * 2 u64 each.  -ENOSPC, nothing consumed, *n = edges waiting, when cap is
- * smaller.  Call after xkrt_hip_taskdom_sync. */
+ * smaller.  Call after xkrt_hip_taskdom_sync.  -EPERM with release on. */
 int xkrt_hip_taskdom_edges_copy(xkrt_hip_taskdom_t t, uint64_t *edges, uint64_t cap, uint64_t *n);
+
+/* Release on the device (config release = 1; tasks numbered 0, 1, ... in
+ * submit order).  What the runtime does with an edge, a commit and a
+ * completion (__task_precedes, __task_commit, include/xkrt/task/task.hpp:
+ * 453-551; __task_complete, src/task.cc:160-247), done by kernels on the
+ * pool's stream: after each launch its edges are applied -- an edge from a
+ * task not yet completed adds one to the successor's wait counter and puts
+ * it on the predecessor access's list -- and its tasks' commit guards are
+ * dropped; a completion walks the task's lists (detached accesses, mode D,
+ * excepted) and decrements.  A counter reaching 0 releases its task into a
+ * queue in pinned host memory.
+ * xkrt_hip_taskdom_ready copies up to cap released tasks not read yet, in
+ * release order, without blocking (tasks whose launch has not been started
+ * are not there: xkrt_hip_taskdom_sync starts it).
+ * xkrt_hip_taskdom_complete enqueues the completion of released tasks; a task
+ * not released, or completed twice, fails the domain (-EINVAL at the next
+ * sync).  Capacities: max_tasks, max_accesses, max_succ (-ENOSPC). */
+int xkrt_hip_taskdom_ready(xkrt_hip_taskdom_t t, uint32_t *tasks, uint64_t cap, uint64_t *n);
+int xkrt_hip_taskdom_complete(xkrt_hip_taskdom_t t, const uint32_t *tasks, uint64_t n);
 int xkrt_hip_taskdom_stats(xkrt_hip_taskdom_t t, xkrt_hip_taskdom_stats_t *out);
 
 /* ------------------------------------------------------------------------

@@ -1,0 +1,92 @@
+"""ORACLE (test infrastructure): the runtime's successor release, restated
+from the reference for the device release engine's tests.
+
+  __task_precedes   include/xkrt/task/task.hpp:453-486: an edge pred -> succ
+                    whose predecessor task is not COMPLETED adds 1 to the
+                    successor task's wait counter and puts the successor
+                    access on the predecessor access's list
+  __task_commit     task.hpp:535-551: the counter starts at 1 (task.hpp:129),
+                    the commit takes that 1 away; 0 -> ready
+  __task_complete   src/task.cc:160-247: COMPLETED, then for every access not
+                    in mode D (199-201), every successor on its list loses 1;
+                    0 -> ready
+
+The edges given are already deduplicated per (pred access, succ task)
+(__access_precedes, task.hpp:494-512).  A launch applies the edges of the
+tasks it resolved, then commits those tasks; the link happens at the launch,
+so "completed" is tested there.
+
+Pinned by tests/test_oracle_release.py against a closed form over sets, with
+a planted mistake.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+MODE_D = 8
+
+
+class Release:
+    def __init__(self, acc_task, acc_mode):
+        self.acc_task = np.asarray(acc_task, np.int64)
+        self.acc_mode = np.asarray(acc_mode, np.int64)
+        ntask = int(self.acc_task.max()) + 1 if len(self.acc_task) else 0
+        self.wc = [1] * ntask           # the commit guard
+        self.state = ["alloc"] * ntask  # alloc -> ready -> done
+        self.succ = {}                  # pred access -> [succ task]
+
+    def _ready(self, t, out):
+        self.state[t] = "ready"
+        out.append(t)
+
+    def launch(self, edges, t0, t1, skip_completed=True):
+        """Apply a launch's edges, then commit its tasks [t0, t1); returns the released tasks.
+        skip_completed / honor_detached (complete) False plant mistakes for the tests."""
+        out = []
+        for p, s in np.asarray(edges, np.int64).reshape(-1, 2):
+            tp, ts = int(self.acc_task[p]), int(self.acc_task[s])
+            if skip_completed and self.state[tp] == "done":
+                continue
+            self.wc[ts] += 1
+            self.succ.setdefault(int(p), []).append(ts)
+        for t in range(t0, t1):
+            self.wc[t] -= 1
+            if self.wc[t] == 0:
+                self._ready(t, out)
+        return out
+
+    def complete(self, tasks, honor_detached=True):
+        """Complete released tasks; returns the tasks they release."""
+        out = []
+        for t in tasks:
+            t = int(t)
+            assert self.state[t] == "ready", t
+            self.state[t] = "done"
+            for a in np.flatnonzero(self.acc_task == t):
+                if honor_detached and self.acc_mode[a] & MODE_D:
+                    continue
+                for s in self.succ.get(int(a), []):
+                    self.wc[s] -= 1
+                    if self.wc[s] == 0:
+                        self._ready(s, out)
+        return out
+
+
+def released_closed_form(edges, acc_task, acc_mode, launched_at, done_at, now):
+    """The tasks released after event `now`, from sets: a task launched at or
+    before `now` is released unless some edge into it that counted -- its
+    predecessor task not completed before the successor's launch -- comes from
+    a detached access or from a task not completed by `now`.
+    launched_at[t] / done_at[t]: event index of t's launch / completion (inf
+    when none)."""
+    acc_task = np.asarray(acc_task, np.int64)
+    acc_mode = np.asarray(acc_mode, np.int64)
+    launched_at = np.asarray(launched_at, np.float64)
+    done_at = np.asarray(done_at, np.float64)
+    blocked = np.zeros(len(launched_at), bool)
+    for p, s in np.asarray(edges, np.int64).reshape(-1, 2):
+        tp, ts = acc_task[p], acc_task[s]
+        counted = not done_at[tp] < launched_at[ts]
+        if counted and ((acc_mode[p] & MODE_D) or not done_at[tp] <= now):
+            blocked[ts] = True
+    return set(np.flatnonzero((launched_at <= now) & ~blocked).tolist())

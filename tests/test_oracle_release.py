@@ -1,0 +1,89 @@
+"""The oracle's successor release (oracle/release.py: wait counters and
+successor lists, as the reference's __task_precedes / __task_commit /
+__task_complete) against a closed form over sets, on random schedules that
+interleave launches and out-of-order completions; planted mistakes (an edge
+from a completed task counted, a detached access fulfilled by completion)
+must fail."""
+import numpy as np
+import pytest
+
+from oracle import khp
+from oracle import release as REL
+from oracle import segments as G
+
+
+def program(seed, n=1500, pd=0.0):
+    modes = (1, 2, 3) if pd == 0 else (1, 2, 3, 9, 10)
+    p = G.program_stream(n, seed, modes=modes)
+    e = np.asarray(khp.resolve_program(*[p[k] for k in ("task", "mode", "kind", "addr", "m", "n", "ld", "s")]),
+                   np.int64)
+    return p, e
+
+
+def task_batches(task, k):
+    starts = np.flatnonzero(np.r_[True, np.diff(task.astype(np.int64)) != 0])
+    bounds = list(starts[::k]) + [len(task)]
+    return list(zip(bounds[:-1], bounds[1:]))
+
+
+def run(p, e, seed, batch_tasks=40, pcomp=0.7, mistake=None, check=True):
+    """Drive one schedule through the oracle; returns the release events (per event, the tasks)."""
+    rng = np.random.default_rng(seed)
+    task, mode = p["task"], p["mode"]
+    ntask = int(task.max()) + 1
+    rel = REL.Release(task, mode)
+    launched_at = np.full(ntask, np.inf)
+    done_at = np.full(ntask, np.inf)
+    cum, pool, events = set(), set(), []
+    ev = 0
+
+    def note(out):
+        nonlocal ev
+        cum.update(out)
+        pool.update(out)
+        events.append(sorted(out))
+        if check:
+            want = REL.released_closed_form(e, task, mode, launched_at, done_at, ev)
+            assert cum == want, ev
+        ev += 1
+
+    def completion():
+        pick = sorted(rng.choice(sorted(pool), min(len(pool), int(rng.integers(1, 8))), replace=False).tolist())
+        for t in pick:
+            pool.discard(t)
+            done_at[t] = ev
+        note(rel.complete(pick, honor_detached=mistake != "detached"))
+
+    for a, b in task_batches(task, batch_tasks):
+        t0, t1 = int(task[a]), int(task[b - 1]) + 1
+        launched_at[t0:t1] = ev
+        sel = (e[:, 1] >= a) & (e[:, 1] < b)
+        note(rel.launch(e[sel], t0, t1, skip_completed=mistake != "count_completed"))
+        while pool and rng.random() < pcomp:
+            completion()
+    while pool:
+        completion()
+    return events
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_release_matches_closed_form(seed):
+    p, e = program(seed)
+    events = run(p, e, seed)
+    released = sorted(t for ev in events for t in ev)
+    assert released == list(range(int(p["task"].max()) + 1))  # every task, once
+
+
+def test_detached_accesses_block_their_successors():
+    p, e = program(3, pd=0.3)
+    events = run(p, e, 3)
+    released = {t for ev in events for t in ev}
+    assert len(released) < int(p["task"].max()) + 1  # some tasks wait on a detached access for ever
+
+
+@pytest.mark.parametrize("mistake,pd", [("count_completed", 0.0), ("detached", 0.3)])
+def test_planted_mistakes_fail(mistake, pd):
+    with pytest.raises(AssertionError):
+        for seed in range(5):
+            p, e = program(seed, pd=pd)
+            run(p, e, seed, mistake=mistake)

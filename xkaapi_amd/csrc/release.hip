@@ -1,0 +1,289 @@
+// Successor release on the device: what the reference does on the host when
+// an edge is linked, when a task is committed and when it completes.
+//
+//   __task_precedes (include/xkrt/task/task.hpp:453-486): an edge whose
+//     predecessor task has not completed adds 1 to the successor task's wait
+//     counter and puts the successor on the predecessor access's list;
+//   __task_commit (task.hpp:535-551): the counter starts at 1, a guard that
+//     the commit drops once the task's accesses are linked -- here once the
+//     launch that resolved the task has had its edges applied;
+//   __task_complete (src/task.cc:160-247): the task turns COMPLETED, and every
+//     successor on the lists of its accesses -- except detached (D mode)
+//     accesses, whose task fulfils them itself -- loses 1; a counter reaching
+//     0 releases its task.
+//
+// Data layout in HBM (one engine per task domain, capacities fixed at
+// creation): per task a wait counter, a state word and its first access id;
+// per access its task (bit 31: detached) and the head of its successor list;
+// per applied edge one list node {successor task, next node}.  Released
+// tasks go to a queue in pinned host memory, one slot per task, written with
+// system-scope release stores: the host reads them as they land, with no
+// stream synchronisation.
+//
+// The kernels are integer and latency bound (a few dependent loads per edge
+// or successor); they run on the pool's stream after each launch, so edges
+// never leave the device.
+
+#include <hip/hip_runtime.h>
+
+#include <errno.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <vector>
+
+#include "release.h"
+
+namespace xkrt_release {
+
+namespace {
+
+constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+constexpr uint32_t ST_WAIT = 0, ST_READY = 1, ST_DONE = 2;
+constexpr uint32_t DETACHED = 1u << 31;
+constexpr uint8_t MODE_D = 8;  // ACCESS_MODE_D (include/xkrt/memory/access/mode.h:62)
+enum : uint32_t { RERR_NODES = 1, RERR_STATE = 2, RERR_RANGE = 4 };
+
+struct Dev {
+    uint32_t *wc;        // per task: wait counter (starts at 1: the commit guard)
+    uint32_t *state;     // per task: ST_*
+    uint32_t *first;     // per task + 1: first access id (first[t + 1] ends task t)
+    uint32_t *acc_task;  // per access: task | DETACHED
+    uint32_t *head;      // per access: first successor node, EMPTY when none
+    uint2 *node;         // per node: {successor task, next node}
+    uint32_t *ready;     // device address of the pinned release queue
+    unsigned int *ctr;   // [0] nodes used, [1] queue slots used, [2] error bits, [3] completions
+    uint32_t max_tasks, max_acc, max_nodes;
+};
+
+__device__ void release(const Dev &r, uint32_t t)
+{
+    r.state[t] = ST_READY;
+    const uint32_t slot = atomicAdd(&r.ctr[1], 1u);
+    if (slot < r.max_tasks)
+        __hip_atomic_store(&r.ready[slot], t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    else
+        atomicOr(&r.ctr[2], RERR_RANGE);
+}
+
+__global__ void k_fill(uint32_t *p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// one thread per unconsumed edge (grid-stride: the count is read on the device)
+__global__ void k_apply(Dev r, const uint2 *edges, const unsigned long long *count)
+{
+    const uint64_t n = *count;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint2 e = edges[i];
+        if (e.x >= r.max_acc || e.y >= r.max_acc) {
+            atomicOr(&r.ctr[2], RERR_RANGE);
+            continue;
+        }
+        const uint32_t tp = r.acc_task[e.x] & ~DETACHED, ts = r.acc_task[e.y] & ~DETACHED;
+        if (tp >= r.max_tasks || ts >= r.max_tasks) {
+            atomicOr(&r.ctr[2], RERR_RANGE);
+            continue;
+        }
+        if (r.state[tp] == ST_DONE) continue;  // a completed predecessor adds no edge
+        const uint32_t k = atomicAdd(&r.ctr[0], 1u);
+        if (k >= r.max_nodes) {
+            atomicOr(&r.ctr[2], RERR_NODES);
+            continue;
+        }
+        atomicAdd(&r.wc[ts], 1u);
+        r.node[k].x = ts;
+        r.node[k].y = atomicExch(&r.head[e.x], k);
+    }
+}
+
+// the commit guard of the launch's tasks
+__global__ void k_guard(Dev r, uint32_t t0, uint32_t t1)
+{
+    const uint32_t t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= t1) return;
+    if (atomicSub(&r.wc[t], 1u) == 1u) release(r, t);
+}
+
+// one thread per completed task: its accesses' successor lists
+__global__ void k_complete(Dev r, const uint32_t *tasks, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t t = tasks[i];
+    if (t >= r.max_tasks || atomicCAS(&r.state[t], ST_READY, ST_DONE) != ST_READY) {
+        atomicOr(&r.ctr[2], RERR_STATE);  // not released, or completed twice
+        return;
+    }
+    atomicAdd(&r.ctr[3], 1u);
+    for (uint32_t a = r.first[t]; a < r.first[t + 1]; ++a) {
+        if (r.acc_task[a] & DETACHED) continue;
+        for (uint32_t k = r.head[a]; k != EMPTY; k = r.node[k].y) {
+            const uint32_t s = r.node[k].x;
+            if (atomicSub(&r.wc[s], 1u) == 1u) release(r, s);
+        }
+    }
+}
+
+int hip_rc(hipError_t e) { return e == hipSuccess ? 0 : (e == hipErrorOutOfMemory ? -ENOMEM : -EIO); }
+
+#define RCHK(x)                        \
+    do {                               \
+        const int rc_ = hip_rc(x);     \
+        if (rc_) return rc_;           \
+    } while (0)
+
+} // namespace
+
+struct Engine {
+    hipStream_t stream = nullptr;
+    Dev d{};
+    uint32_t *h_acc_task = nullptr;  // pinned mirrors: append-only sources of the uploads
+    uint32_t *h_first = nullptr;
+    uint32_t *h_done = nullptr;      // completion lists, append-only
+    uint32_t *h_ready = nullptr;     // the release queue (host address)
+    unsigned int *h_ctr = nullptr;
+    uint64_t ntasks = 0, nacc = 0, done_used = 0, read = 0;
+};
+
+int create(void *stream, uint64_t max_tasks, uint64_t max_acc, uint64_t max_nodes, Engine **out)
+{
+    if (!out || !max_tasks || !max_acc || !max_nodes) return -EINVAL;
+    if (max_tasks >= EMPTY || max_acc >= EMPTY || max_nodes >= EMPTY || max_tasks >= DETACHED) return -ERANGE;
+    *out = nullptr;
+    Engine *e = new Engine();
+    e->stream = (hipStream_t)stream;
+    Dev &d = e->d;
+    d.max_tasks = (uint32_t)max_tasks;
+    d.max_acc = (uint32_t)max_acc;
+    d.max_nodes = (uint32_t)max_nodes;
+    auto fail = [&](int rc) {
+        destroy(e);
+        return rc;
+    };
+    int rc = 0;
+    if ((rc = hip_rc(hipMalloc(&d.wc, 4 * max_tasks))) || (rc = hip_rc(hipMalloc(&d.state, 4 * max_tasks))) ||
+        (rc = hip_rc(hipMalloc(&d.first, 4 * (max_tasks + 1)))) ||
+        (rc = hip_rc(hipMalloc(&d.acc_task, 4 * max_acc))) || (rc = hip_rc(hipMalloc(&d.head, 4 * max_acc))) ||
+        (rc = hip_rc(hipMalloc(&d.node, 8 * max_nodes))) || (rc = hip_rc(hipMalloc(&d.ctr, 64))))
+        return fail(rc);
+    if ((rc = hip_rc(hipHostMalloc(&e->h_acc_task, 4 * max_acc, hipHostMallocDefault))) ||
+        (rc = hip_rc(hipHostMalloc(&e->h_first, 4 * (max_tasks + 1), hipHostMallocDefault))) ||
+        (rc = hip_rc(hipHostMalloc(&e->h_done, 4 * max_tasks, hipHostMallocMapped))) ||
+        (rc = hip_rc(hipHostMalloc(&e->h_ready, 4 * max_tasks, hipHostMallocCoherent | hipHostMallocMapped))) ||
+        (rc = hip_rc(hipHostMalloc(&e->h_ctr, 64, hipHostMallocDefault))))
+        return fail(rc);
+    memset(e->h_ready, 0xFF, 4 * max_tasks);
+    if ((rc = hip_rc(hipHostGetDevicePointer((void **)&d.ready, e->h_ready, 0)))) return fail(rc);
+    const dim3 g(1024), b(256);
+    hipLaunchKernelGGL(k_fill, g, b, 0, e->stream, d.wc, max_tasks, 1u);
+    hipLaunchKernelGGL(k_fill, g, b, 0, e->stream, d.state, max_tasks, ST_WAIT);
+    hipLaunchKernelGGL(k_fill, g, b, 0, e->stream, d.head, max_acc, EMPTY);
+    hipLaunchKernelGGL(k_fill, g, b, 0, e->stream, (uint32_t *)d.ctr, 16, 0u);
+    if ((rc = hip_rc(hipGetLastError())) || (rc = hip_rc(hipStreamSynchronize(e->stream)))) return fail(rc);
+    *out = e;
+    return 0;
+}
+
+void destroy(Engine *e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    Dev &d = e->d;
+    for (void *p : {(void *)d.wc, (void *)d.state, (void *)d.first, (void *)d.acc_task, (void *)d.head, (void *)d.node,
+                    (void *)d.ctr})
+        if (p) (void)hipFree(p);
+    for (void *p : {(void *)e->h_acc_task, (void *)e->h_first, (void *)e->h_done, (void *)e->h_ready, (void *)e->h_ctr})
+        if (p) (void)hipHostFree(p);
+    delete e;
+}
+
+int add(Engine *e, uint64_t acc0, const uint32_t *task, const uint8_t *mode, uint64_t n)
+{
+    if (!e || (!n) || !task || !mode) return n ? -EINVAL : 0;
+    if (acc0 != e->nacc) return -EINVAL;
+    if (task[0] != e->ntasks) return -EINVAL;  // tasks continue the previous batch
+    for (uint64_t i = 1; i < n; ++i)
+        if (task[i] != task[i - 1] && task[i] != task[i - 1] + 1) return -EINVAL;
+    const uint64_t t_end = (uint64_t)task[n - 1] + 1;
+    if (t_end > e->d.max_tasks || acc0 + n > e->d.max_acc) return -ENOSPC;
+    for (uint64_t i = 0; i < n; ++i) {
+        e->h_acc_task[acc0 + i] = task[i] | ((mode[i] & MODE_D) ? DETACHED : 0u);
+        if (i == 0 || task[i] != task[i - 1]) e->h_first[task[i]] = (uint32_t)(acc0 + i);
+    }
+    e->h_first[t_end] = (uint32_t)(acc0 + n);
+    const uint64_t t0 = e->ntasks;
+    RCHK(hipMemcpyAsync(e->d.acc_task + acc0, e->h_acc_task + acc0, 4 * n, hipMemcpyHostToDevice, e->stream));
+    RCHK(hipMemcpyAsync(e->d.first + t0, e->h_first + t0, 4 * (t_end + 1 - t0), hipMemcpyHostToDevice, e->stream));
+    e->ntasks = t_end;
+    e->nacc = acc0 + n;
+    return 0;
+}
+
+int apply(Engine *e, const uint32_t *edges, const uint64_t *count, uint32_t t0, uint32_t t1)
+{
+    if (!e || !edges || !count || t0 > t1 || t1 > e->ntasks) return -EINVAL;
+    hipLaunchKernelGGL(k_apply, dim3(512), dim3(256), 0, e->stream, e->d, (const uint2 *)edges,
+                       (const unsigned long long *)count);
+    if (t1 > t0) hipLaunchKernelGGL(k_guard, dim3((t1 - t0 + 255) / 256), dim3(256), 0, e->stream, e->d, t0, t1);
+    return hip_rc(hipGetLastError());
+}
+
+int complete(Engine *e, const uint32_t *tasks, uint64_t n)
+{
+    if (!e || (n && !tasks)) return -EINVAL;
+    if (n == 0) return 0;
+    if (e->done_used + n > e->d.max_tasks) return -ENOSPC;  // each task completes once
+    uint32_t *src = e->h_done + e->done_used;
+    memcpy(src, tasks, 4 * n);
+    e->done_used += n;
+    // the kernel reads the list in place from pinned memory (never rewritten)
+    uint32_t *dsrc = nullptr;
+    RCHK(hipHostGetDevicePointer((void **)&dsrc, src, 0));
+    hipLaunchKernelGGL(k_complete, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, e->stream, e->d, dsrc, (uint32_t)n);
+    return hip_rc(hipGetLastError());
+}
+
+int ready(Engine *e, uint32_t *out, uint64_t cap, uint64_t *n)
+{
+    if (!e || !n || (cap && !out)) return -EINVAL;
+    uint64_t k = 0;
+    while (k < cap && e->read < e->d.max_tasks) {
+        const uint32_t t = __atomic_load_n(e->h_ready + e->read, __ATOMIC_ACQUIRE);
+        if (t == EMPTY) break;
+        out[k++] = t;
+        e->read++;
+    }
+    *n = k;
+    return 0;
+}
+
+int check(Engine *e)
+{
+    if (!e) return -EINVAL;
+    RCHK(hipMemcpyAsync(e->h_ctr, e->d.ctr, 64, hipMemcpyDeviceToHost, e->stream));
+    RCHK(hipStreamSynchronize(e->stream));
+    const uint32_t err = e->h_ctr[2];
+    if (err & RERR_STATE) return -EINVAL;
+    if (err & (RERR_NODES | RERR_RANGE)) return -ENOSPC;
+    return 0;
+}
+
+int counts(Engine *e, Counts *out)
+{
+    if (!e || !out) return -EINVAL;
+    const int rc = check(e);
+    out->tasks = e->ntasks;
+    out->accesses = e->nacc;
+    out->nodes = e->h_ctr[0];
+    out->released = e->h_ctr[1];
+    out->read = e->read;
+    out->completed = e->h_ctr[3];
+    return rc;
+}
+
+} // namespace xkrt_release

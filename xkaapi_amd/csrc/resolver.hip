@@ -10,8 +10,10 @@
 //     written area, each with its last writer (access id) -- and a log of the
 //     exact readers clipped to the bin (AoS, 5 u32 per fragment / entry:
 //     x0, x1, y0, y1, lw or acc; fcap fragments and rcap entries per bin);
-//   * rect records of a launch arrive as structure of arrays
-//     (x0, x1, y0, y1, acc, task<<8|mode: 24 B);
+//   * rect records of a launch are an array of 32 B structs, two 16-byte
+//     words each {x0, x1, y0, y1}, {acc, task<<8|mode, domain, 0}: one record
+//     is two dwordx4 loads by one lane (the record an incidence refers to is
+//     read whole, never a single field);
 //   * incidences (record x bin) are 32 B: {x0, x1, y0, y1}, {acc, tm, record, 0}.
 //
 // One launch resolves one batch of records in these kernels:
@@ -49,6 +51,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <atomic>
 #include <chrono>
 #include <deque>
@@ -128,8 +131,8 @@ struct KArgs {
     const DomDesc *dom;       // domain table
     const uint32_t *bin_dom;  // domain of each pool bin
     uint32_t *bin_nf, *bin_nrs;
-    uint32_t *frag;           // nbins * 5 * fcap (SoA x0, x1, y0, y1, lw)
-    uint32_t *rs;             // nbins * 5 * rcap reader log (SoA x0, x1, y0, y1, acc)
+    uint32_t *frag;           // nbins * fcap fragments, 5 words each (x0, x1, y0, y1, lw)
+    uint32_t *rs;             // nbins * rcap reader-log entries, 5 words each (x0, x1, y0, y1, acc)
     uint32_t fcap, rcap;
     uint32_t *bin_cnt;        // incidences per bin this launch (zeroed by k_resolve)
     uint32_t *touched;        // bins with records this launch, in first-touch order
@@ -978,7 +981,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
         if (lane == 0) s.tdom = dm << 24;
 
         // record ids, fragments and reader log of the bin, loads issued together
-        // (SoA in HBM: 5 x fcap fragment words, 5 x gcap log words)
+        // (AoS in HBM: 5 words per fragment and per log entry)
         BinState st;
         st.fb = 0; st.err = 0;
         st.slot = slot; st.ng0 = ng0; st.gload = 0; st.gdirty = ng0; st.fdirty = nf0 == 0 ? 1u : 0u;
@@ -1470,6 +1473,12 @@ const KClass TIERS[4] = {{64, 128, 64}, {64, 128, 256}, {256, 512, 512}, {1024, 
 template <int F, int R, int L>
 size_t lds_bytes() { return sizeof(BinLds<F, R, L, false>); }  // the same for both forms
 static_assert(sizeof(BinLds<1024, 2048, 1024, false>) <= 160u * 1024u, "largest tier exceeds the 160 KiB LDS of a CU");
+// the 64-bit LDS arrays sit at 8-byte offsets of the (16-byte aligned) dynamic LDS base
+template <class B>
+constexpr bool lds_aligned() { return offsetof(B, L) % 8 == 0 && offsetof(B, D) % 8 == 0; }
+static_assert(lds_aligned<BinLds<64, 128, 64, false>>() && lds_aligned<BinLds<64, 128, 256, true>>() &&
+              lds_aligned<BinLds<256, 512, 512, false>>() && lds_aligned<BinLds<1024, 2048, 1024, true>>(),
+              "64-bit LDS arrays must be 8-byte aligned");
 
 constexpr uint32_t NDRING = 4;
 
@@ -1598,6 +1607,8 @@ struct xkrt_hip_dep_domain {
     // a segment domain (interval/dependency-tree.hpp:187-363): segments only, each cut into
     // the rects of a frame of ld-byte columns and PUT exactly
     bool segments = false;
+    bool any_id = false;   // batches staged with caller access ids: the last one given
+    uint32_t last_id = 0;
     uint64_t n_acc = 0, n_rect = 0;
     uint32_t err = 0;   // device error bits of this domain seen by the host (sticky until reset)
 };
@@ -2114,15 +2125,23 @@ void destroy_pool(xkrt_hip_pool *p)
     delete p;
 }
 
-// dynamic LDS limit of one tier's kernels (both dedup forms)
+// dynamic LDS limit of one tier's kernels (both dedup forms).  The kernels
+// cast the dynamic LDS base to BinLds, whose 64-bit arrays need it 8-byte
+// aligned: that holds only while no static __shared__ variable sits before it,
+// so a kernel with static LDS is refused here rather than launched.
 template <int F, int R, int L>
 hipError_t set_lds()
 {
-    hipError_t e = hipFuncSetAttribute((const void *)k_resolve<F, R, L, false>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes<F, R, L>());
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void *)k_resolve<F, R, L, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds_bytes<F, R, L>());
+    const void *k[2] = {(const void *)k_resolve<F, R, L, false>, (const void *)k_resolve<F, R, L, true>};
+    for (const void *f : k) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes<F, R, L>());
+        if (e != hipSuccess) return e;
+        hipFuncAttributes at{};
+        e = hipFuncGetAttributes(&at, f);
+        if (e != hipSuccess) return e;
+        if (at.sharedSizeBytes != 0) return hipErrorInvalidKernelFile;
+    }
+    return hipSuccess;
 }
 
 int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool **out)
@@ -2259,8 +2278,13 @@ void remove_domain(xkrt_hip_dep_domain *d)
 // frame, access ids from first_id, launch-local task numbers from task0;
 // *ntask = tasks used
 int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t n, uint32_t first_id, uint32_t task0,
-                  uint64_t cap, xkrt_hip_rect_t *out, uint64_t *nrect, uint32_t *ylo, uint32_t *yhi, uint32_t *ntask)
+                  uint64_t cap, xkrt_hip_rect_t *out, uint64_t *nrect, uint32_t *ylo, uint32_t *yhi, uint32_t *ntask,
+                  const uint32_t *ids = nullptr)
 {
+    // caller ids: increasing across the domain's batches (the lowest id of a task = its lowest slot)
+    if (ids)
+        for (uint64_t i = 0; i < n; ++i)
+            if ((i ? ids[i] <= ids[i - 1] : (d->any_id && ids[0] <= d->last_id))) return -EINVAL;
     const uint64_t LDs = d->LDs, s = d->s;
     // frame: fixed by the first access ever seen (or given at creation)
     if (!d->framed && n) {
@@ -2332,7 +2356,7 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
                 xkrt_hip_rect_t &o = out[k];
                 o.x0 = (uint32_t)r[j][0]; o.x1 = (uint32_t)r[j][1];
                 o.y0 = (uint32_t)(r[j][2] - d->ybase); o.y1 = (uint32_t)(r[j][3] - d->ybase);
-                o.acc = first_id + (uint32_t)i;
+                o.acc = ids ? ids[i] : first_id + (uint32_t)i;
                 o.tm = (tl << 8) | q.mode | flag;
                 o.dom = d->idx;
                 o.pad = 0;
@@ -2344,6 +2368,10 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
     }
     d->max_xlen = maxx;
     d->col0 = col0;
+    if (ids && n) {
+        d->any_id = true;
+        d->last_id = ids[n - 1];
+    }
     *nrect = k;
     if (ylo) *ylo = n ? (uint32_t)lo : 0;
     if (yhi) *yhi = n ? (uint32_t)hi : 0;
@@ -2352,7 +2380,8 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
 }
 
 // append a batch to the pool's open launch (host staging)
-int stage(xkrt_hip_pool *p, xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t n, uint64_t *first_id)
+int stage(xkrt_hip_pool *p, xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t n, uint64_t *first_id,
+          const uint32_t *ids = nullptr)
 {
     if (p->poisoned) return -EINVAL;
     if (d->err) return -ECANCELED;  // a failed domain takes no batch until it is reset
@@ -2373,15 +2402,18 @@ int stage(xkrt_hip_pool *p, xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc
     uint64_t nrect = 0;
     uint32_t ylo = 0, yhi = 0, ntask = 0;
     const uint64_t saved_max = d->max_xlen, room = p->rec_cap - p->open_nrec;
-    const bool was_framed = d->framed, saved_col0 = d->col0;
+    const bool was_framed = d->framed, saved_col0 = d->col0, saved_any = d->any_id;
+    const uint32_t saved_last = d->last_id;
     int rc = build_records(d, acc, n, (uint32_t)d->next_id, p->open_tasks, room,
-                           (xkrt_hip_rect_t *)p->rec_host + p->open_nrec, &nrect, &ylo, &yhi, &ntask);
+                           (xkrt_hip_rect_t *)p->rec_host + p->open_nrec, &nrect, &ylo, &yhi, &ntask, ids);
     // the batch's rects (1 to 3 per access) must fit the open launch whole
     if (!rc && nrect > room) rc = p->open_nrec ? -ENOSPC : -EINVAL;
     if (!rc) rc = ensure_grid(d, ylo, yhi);
     if (rc) {
         d->max_xlen = saved_max;
         d->col0 = saved_col0;
+        d->any_id = saved_any;
+        d->last_id = saved_last;
         if (!was_framed) d->framed = false;
         return rc;
     }
@@ -2580,7 +2612,8 @@ int xkrt_hip_pool_stage(xkrt_hip_pool_t p, xkrt_hip_dep_domain_t d, const xkrt_h
 }
 
 int xkrt_hip_pool_stage_group(xkrt_hip_pool_t p, uint32_t nd, const xkrt_hip_dep_domain_t *d,
-                              const xkrt_hip_access_t *const *acc, const uint64_t *n, uint64_t *first_id)
+                              const xkrt_hip_access_t *const *acc, const uint32_t *const *ids, const uint64_t *n,
+                              uint64_t *first_id)
 {
     if (!p || (nd && (!d || !acc || !n))) return -EINVAL;
     if (p->server) return -EBUSY;
@@ -2589,23 +2622,25 @@ int xkrt_hip_pool_stage_group(xkrt_hip_pool_t p, uint32_t nd, const xkrt_hip_dep
     // what staging changes, restored when a later batch of the group is refused
     struct Saved {
         uint64_t next_id, n_acc, n_rect, max_xlen, ybase;
-        bool col0, framed;
+        bool col0, framed, any_id;
+        uint32_t last_id;
     };
     std::vector<Saved> saved(nd);
     for (uint32_t k = 0; k < nd; ++k) {
         const xkrt_hip_dep_domain *q = d[k];
-        saved[k] = {q->next_id, q->n_acc, q->n_rect, q->max_xlen, q->ybase, q->col0, q->framed};
+        saved[k] = {q->next_id, q->n_acc, q->n_rect, q->max_xlen, q->ybase, q->col0, q->framed, q->any_id, q->last_id};
     }
     const uint32_t o_nrec = p->open_nrec, o_tasks = p->open_tasks;
     const uint64_t o_acc = p->open_acc, o_batches = p->open_batches;
     for (uint32_t k = 0; k < nd; ++k) {
-        const int rc = stage(p, d[k], acc[k], n[k], first_id ? first_id + k : nullptr);
+        const int rc = stage(p, d[k], acc[k], n[k], first_id ? first_id + k : nullptr, ids ? ids[k] : nullptr);
         if (rc) {
             for (uint32_t j = nd; j-- > 0;) {  // newest first: a domain named twice gets its oldest state back
                 xkrt_hip_dep_domain *q = d[j];
                 const Saved &v = saved[j];
                 q->next_id = v.next_id; q->n_acc = v.n_acc; q->n_rect = v.n_rect;
                 q->max_xlen = v.max_xlen; q->col0 = v.col0;
+                q->any_id = v.any_id; q->last_id = v.last_id;
                 // a grid allocated meanwhile holds no fragment: it stays, with the frame it was sized in
                 if (!q->nby) { q->framed = v.framed; q->ybase = v.ybase; }
             }
@@ -2694,6 +2729,24 @@ int xkrt_hip_pool_edges_copy(xkrt_hip_pool_t p, uint32_t *edges, uint32_t *doms,
             *n = k + (ne - m);
         }
     }
+    hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, p->stream, p->ctr);
+    return hip_err(hipGetLastError());
+}
+
+int xkrt_hip_pool_edges_device(xkrt_hip_pool_t p, const uint32_t **edges, const uint32_t **doms,
+                               const uint64_t **count)
+{
+    if (!p || !edges || !doms || !count) return -EINVAL;
+    *edges = p->edges;
+    *doms = p->edom;
+    *count = (const uint64_t *)&p->ctr->edge_total;
+    return 0;
+}
+
+int xkrt_hip_pool_edges_consume(xkrt_hip_pool_t p)
+{
+    if (!p) return -EINVAL;
+    if (p->server) return -EBUSY;
     hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, p->stream, p->ctr);
     return hip_err(hipGetLastError());
 }

@@ -12,11 +12,17 @@
 //     exists when it is linked (task.cc:240-259): in program order, the BLAS
 //     domains created by earlier accesses;
 //   * a batch's per-domain parts are staged as one unit and resolved by one
-//     launch; each domain's access ids are mapped back to program ids.
+//     launch; every domain's records carry the program-wide access ids, so
+//     the edges of all domains name program accesses;
+//   * with release on, the edges of each launch are applied on the device
+//     (release.hip): wait counters, successor lists and a ready queue, with
+//     the commit guard of the launch's tasks dropped once its edges are in.
 //
-// Host code only: everything on the device goes through the pool's C ABI.
+// Host code: the device work goes through the pool's C ABI and the release
+// engine.
 
 #include "../../include/xkrt_hip.h"
+#include "release.h"
 
 #include <algorithm>
 #include <cerrno>
@@ -33,9 +39,7 @@ struct xkrt_hip_taskdom {
         xkrt_hip_dep_domain_t d;
     };
     std::vector<Blas> blas;
-    // program id of each domain access, by pool domain index then domain access id
-    std::vector<std::vector<uint64_t>> ids;
-    // (addr, size) of every segment enqueued: what a new BLAS domain gets replayed
+    // every segment enqueued: what a new BLAS domain gets replayed
     std::vector<xkrt_hip_access_t> segs;
     uint64_t next_id = 0, nseg = 0, launches = 0, edges_out = 0;
     bool any_task = false;
@@ -43,8 +47,14 @@ struct xkrt_hip_taskdom {
     int err = 0;  // a domain's device error: sticky
     // per-submit work lists, by pool domain index
     std::vector<std::vector<xkrt_hip_access_t>> part;
-    std::vector<std::vector<uint64_t>> part_ids;
+    std::vector<std::vector<uint32_t>> part_ids;
     std::vector<uint32_t> e32, edom;
+    // release on the device
+    xkrt_release::Engine *rel = nullptr;
+    uint64_t next_task = 0;              // tasks are numbered 0, 1, ... in submit order
+    uint64_t open_t0 = 0, open_t1 = 0;   // tasks staged into the open launch
+    std::vector<uint32_t> btask;
+    std::vector<uint8_t> bmode;
 };
 
 namespace {
@@ -72,12 +82,29 @@ xkrt_hip_access_t plain(const xkrt_hip_task_access_t &q)
 void admit(xkrt_hip_taskdom *t, xkrt_hip_dep_domain_t d)
 {
     const size_t i = (size_t)dom_index(d);
-    if (t->ids.size() <= i) {
-        t->ids.resize(i + 1);
+    if (t->part.size() <= i) {
         t->part.resize(i + 1);
         t->part_ids.resize(i + 1);
     }
-    t->ids[i].clear();
+}
+
+// launches the open launch; with release on, its edges are applied and
+// consumed on the device, then its tasks' commit guards dropped
+int launch(xkrt_hip_taskdom *t)
+{
+    uint64_t ticket = 0;
+    int rc = xkrt_hip_pool_launch(t->pool, &ticket);
+    if (rc) return rc;
+    t->launches++;
+    if (t->rel) {
+        const uint32_t *edges = nullptr, *doms = nullptr;
+        const uint64_t *count = nullptr;
+        rc = xkrt_hip_pool_edges_device(t->pool, &edges, &doms, &count);
+        if (!rc) rc = xkrt_release::apply(t->rel, edges, count, (uint32_t)t->open_t0, (uint32_t)t->open_t1);
+        if (!rc) rc = xkrt_hip_pool_edges_consume(t->pool);
+    }
+    t->open_t0 = t->open_t1;
+    return rc;
 }
 
 } // namespace
@@ -104,12 +131,25 @@ int xkrt_hip_taskdom_create(const xkrt_hip_taskdom_config_t *cfg, void *stream, 
     pc.max_rects = c.max_rects;
     pc.max_edges = c.max_edges;
     pc.budget_bytes = c.budget_bytes;
+    if (c.release) {
+        if (!c.max_tasks) c.max_tasks = 1ull << 20;
+        if (!c.max_accesses) c.max_accesses = 4 * c.max_tasks;
+        if (!c.max_succ) c.max_succ = 2 * c.max_accesses;
+    }
     xkrt_hip_pool_t p = nullptr;
-    const int rc = xkrt_hip_pool_create(&pc, stream, &p);
+    int rc = xkrt_hip_pool_create(&pc, stream, &p);
     if (rc) return rc;
     xkrt_hip_taskdom *t = new xkrt_hip_taskdom();
     t->pool = p;
     t->cfg = c;
+    if (c.release) {
+        rc = xkrt_release::create(xkrt_hip_pool_stream(p), c.max_tasks, c.max_accesses, c.max_succ, &t->rel);
+        if (rc) {
+            xkrt_hip_pool_destroy(p);
+            delete t;
+            return rc;
+        }
+    }
     *out = t;
     return 0;
 }
@@ -117,6 +157,7 @@ int xkrt_hip_taskdom_create(const xkrt_hip_taskdom_config_t *cfg, void *stream, 
 int xkrt_hip_taskdom_destroy(xkrt_hip_taskdom_t t)
 {
     if (!t) return -EINVAL;
+    xkrt_release::destroy(t->rel);
     const int rc = xkrt_hip_pool_destroy(t->pool);  // releases its domains
     delete t;
     return rc;
@@ -129,14 +170,19 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
     if (t->err) return -ECANCELED;
     if (n == 0) return 0;
     // validation first: a refused batch leaves nothing behind
+    if (t->next_id + n > 0xFFFFFFFFull) return -ERANGE;  // program access ids are 32-bit
     for (uint64_t i = 0; i < n; ++i) {
         const xkrt_hip_task_access_t &q = acc[i];
         const uint32_t prev = i ? acc[i - 1].task : t->last_task;
         if ((i || t->any_task) && q.task < prev) return -EINVAL;
+        // released on the device, tasks are numbered 0, 1, ... in submit order
+        if (t->rel && (i ? q.task - prev > 1 : q.task != t->next_task)) return -EINVAL;
         if (q.kind != XKRT_HIP_KIND_TILE && q.kind != XKRT_HIP_KIND_SEGMENT) return -EINVAL;
         if (q.m == 0) return -EINVAL;
         if (q.kind == XKRT_HIP_KIND_TILE && (q.n == 0 || q.ld == 0 || q.sizeof_type == 0)) return -EINVAL;
     }
+    if (t->rel && ((uint64_t)acc[n - 1].task + 1 > t->cfg.max_tasks || t->next_id + n > t->cfg.max_accesses))
+        return -ENOSPC;
     // domains this batch creates (destroyed again if the batch is refused)
     std::vector<xkrt_hip_dep_domain_t> created;
     const size_t nblas0 = t->blas.size();
@@ -156,7 +202,7 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
     for (auto &v : t->part_ids) v.clear();
     for (uint64_t i = 0; i < n; ++i) {
         const xkrt_hip_task_access_t &q = acc[i];
-        const uint64_t id = t->next_id + i;
+        const uint32_t id = (uint32_t)(t->next_id + i);
         const xkrt_hip_access_t a = plain(q);
         if (q.kind == XKRT_HIP_KIND_SEGMENT) {
             if (!t->seg) {
@@ -200,6 +246,7 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
     // stage every domain's part as one unit; launch the open launch first if it is full
     std::vector<xkrt_hip_dep_domain_t> doms;
     std::vector<const xkrt_hip_access_t *> ptr;
+    std::vector<const uint32_t *> idp;
     std::vector<uint64_t> cnt, first;
     auto add = [&](xkrt_hip_dep_domain_t d) {
         if (!d) return;
@@ -207,29 +254,37 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
         if (t->part[i].empty()) return;
         doms.push_back(d);
         ptr.push_back(t->part[i].data());
+        idp.push_back(t->part_ids[i].data());
         cnt.push_back(t->part[i].size());
     };
     add(t->seg);
     for (auto &b : t->blas) add(b.d);
     first.resize(doms.size());
-    int rc = xkrt_hip_pool_stage_group(t->pool, (uint32_t)doms.size(), doms.data(), ptr.data(), cnt.data(),
+    int rc = xkrt_hip_pool_stage_group(t->pool, (uint32_t)doms.size(), doms.data(), ptr.data(), idp.data(), cnt.data(),
                                        first.data());
     if (rc == -ENOSPC) {
-        uint64_t ticket = 0;
-        rc = xkrt_hip_pool_launch(t->pool, &ticket);
+        rc = launch(t);
         if (!rc) {
-            t->launches++;
-            rc = xkrt_hip_pool_stage_group(t->pool, (uint32_t)doms.size(), doms.data(), ptr.data(), cnt.data(),
-                                           first.data());
+            rc = xkrt_hip_pool_stage_group(t->pool, (uint32_t)doms.size(), doms.data(), ptr.data(), idp.data(),
+                                           cnt.data(), first.data());
             if (rc == -ENOSPC) rc = -EINVAL;  // larger than one launch
         }
     }
     if (rc) return undo(rc);
-    for (size_t k = 0; k < doms.size(); ++k) {
-        std::vector<uint64_t> &m = t->ids[dom_index(doms[k])];
-        const std::vector<uint64_t> &src = t->part_ids[dom_index(doms[k])];
-        if (m.size() != first[k]) return -EIO;  // domain ids and the map must agree
-        m.insert(m.end(), src.begin(), src.end());
+    if (t->rel) {
+        t->btask.resize(n);
+        t->bmode.resize(n);
+        for (uint64_t i = 0; i < n; ++i) {
+            t->btask[i] = acc[i].task;
+            t->bmode[i] = acc[i].mode;
+        }
+        rc = xkrt_release::add(t->rel, t->next_id, t->btask.data(), t->bmode.data(), n);
+        if (rc) {
+            t->err = rc;  // staged but not registered: nothing of the task domain can be trusted
+            return rc;
+        }
+        t->next_task = (uint64_t)acc[n - 1].task + 1;
+        t->open_t1 = t->next_task;
     }
     for (uint64_t i = 0; i < n; ++i) t->nseg += acc[i].kind == XKRT_HIP_KIND_SEGMENT;
     t->next_id += n;
@@ -243,11 +298,7 @@ int xkrt_hip_taskdom_sync(xkrt_hip_taskdom_t t, uint64_t *nedges)
     if (!t) return -EINVAL;
     uint64_t nr = 0, na = 0, nb = 0;
     int rc = xkrt_hip_pool_open(t->pool, &nr, &na, &nb);
-    if (!rc && nb) {
-        uint64_t ticket = 0;
-        rc = xkrt_hip_pool_launch(t->pool, &ticket);
-        if (!rc) t->launches++;
-    }
+    if (!rc && nb) rc = launch(t);
     uint64_t ne = 0;
     if (!rc) rc = xkrt_hip_pool_sync(t->pool, &ne);
     if (nedges) *nedges = ne;
@@ -259,12 +310,14 @@ int xkrt_hip_taskdom_sync(xkrt_hip_taskdom_t t, uint64_t *nedges)
         const int e = xkrt_hip_dep_sync(d, nullptr);
         if (e && !t->err) t->err = e;
     }
+    if (t->rel && !t->err) t->err = xkrt_release::check(t->rel);
     return t->err;
 }
 
 int xkrt_hip_taskdom_edges_copy(xkrt_hip_taskdom_t t, uint64_t *edges, uint64_t cap, uint64_t *n)
 {
     if (!t || !n || (cap && !edges)) return -EINVAL;
+    if (t->rel) return -EPERM;  // applied and consumed on the device
     uint64_t ne = 0;
     int rc = xkrt_hip_pool_sync(t->pool, &ne);
     *n = ne;
@@ -275,18 +328,24 @@ int xkrt_hip_taskdom_edges_copy(xkrt_hip_taskdom_t t, uint64_t *edges, uint64_t 
     uint64_t got = 0;
     rc = xkrt_hip_pool_edges_copy(t->pool, t->e32.data(), t->edom.data(), ne, &got);
     if (rc) return rc;
-    for (uint64_t i = 0; i < got; ++i) {
-        const uint32_t di = t->edom[i];
-        if (di >= t->ids.size()) return -EIO;
-        const std::vector<uint64_t> &m = t->ids[di];
-        const uint32_t p = t->e32[2 * i], q = t->e32[2 * i + 1];
-        if (p >= m.size() || q >= m.size()) return -EIO;
-        edges[2 * i] = m[p];
-        edges[2 * i + 1] = m[q];
-    }
+    for (uint64_t i = 0; i < 2 * got; ++i) edges[i] = t->e32[i];  // program ids already
     *n = got;
     t->edges_out += got;
     return 0;
+}
+
+int xkrt_hip_taskdom_ready(xkrt_hip_taskdom_t t, uint32_t *tasks, uint64_t cap, uint64_t *n)
+{
+    if (!t || !n) return -EINVAL;
+    if (!t->rel) return -EPERM;
+    return xkrt_release::ready(t->rel, tasks, cap, n);
+}
+
+int xkrt_hip_taskdom_complete(xkrt_hip_taskdom_t t, const uint32_t *tasks, uint64_t n)
+{
+    if (!t) return -EINVAL;
+    if (!t->rel) return -EPERM;
+    return xkrt_release::complete(t->rel, tasks, n);
 }
 
 int xkrt_hip_taskdom_stats(xkrt_hip_taskdom_t t, xkrt_hip_taskdom_stats_t *out)
@@ -298,6 +357,14 @@ int xkrt_hip_taskdom_stats(xkrt_hip_taskdom_t t, xkrt_hip_taskdom_stats_t *out)
     out->blas_domains = t->blas.size();
     out->launches = t->launches;
     out->edges = t->edges_out;
+    if (t->rel) {
+        xkrt_release::Counts c{};
+        const int rc = xkrt_release::counts(t->rel, &c);
+        out->succ_nodes = c.nodes;
+        out->released = c.released;
+        out->completed = c.completed;
+        return rc;
+    }
     return 0;
 }
 

@@ -82,13 +82,16 @@ assert TASK_ACCESS_DTYPE.itemsize == C.sizeof(TaskAccessT) == 48
 
 
 class TaskDomConfigT(C.Structure):
-    _fields_ = [("kernel", C.c_uint32), ("pad", C.c_uint32), ("max_rects", C.c_uint64), ("max_edges", C.c_uint64),
-                ("budget_bytes", C.c_uint64), ("seg_col_bytes", C.c_uint64), ("seg_bin_x_bytes", C.c_uint32),
-                ("seg_bin_y_cols", C.c_uint32), ("blas_bin_x_bytes", C.c_uint32), ("blas_bin_y_cols", C.c_uint32)]
+    _fields_ = [("kernel", C.c_uint32), ("release", C.c_uint32), ("max_rects", C.c_uint64),
+                ("max_edges", C.c_uint64), ("budget_bytes", C.c_uint64), ("seg_col_bytes", C.c_uint64),
+                ("seg_bin_x_bytes", C.c_uint32), ("seg_bin_y_cols", C.c_uint32), ("blas_bin_x_bytes", C.c_uint32),
+                ("blas_bin_y_cols", C.c_uint32), ("max_tasks", C.c_uint64), ("max_accesses", C.c_uint64),
+                ("max_succ", C.c_uint64)]
 
 
 class TaskDomStatsT(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in ("accesses", "segments", "blas_domains", "launches", "edges")]
+    _fields_ = [(n, C.c_uint64) for n in ("accesses", "segments", "blas_domains", "launches", "edges", "succ_nodes",
+                                          "released", "completed")]
 
 
 # device error bits: 2, 4, 8 and 128 belong to one domain (the pool's other domains go on);
@@ -172,7 +175,11 @@ def hip_lib():
     L.xkrt_hip_server_times.argtypes = [_p, _u64, _u64, _p, _p, _p]
     L.xkrt_hip_server_edges.argtypes = [_p, _u32, _p, _u64, C.POINTER(_u64)]
     L.xkrt_hip_server_stats.argtypes = [_p, C.POINTER(ServerStatsT)]
-    L.xkrt_hip_pool_stage_group.argtypes = [_p, _u32, _p, _p, _p, _p]
+    L.xkrt_hip_pool_stage_group.argtypes = [_p, _u32, _p, _p, _p, _p, _p]
+    L.xkrt_hip_pool_edges_device.argtypes = [_p, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p)]
+    L.xkrt_hip_pool_edges_consume.argtypes = [_p]
+    L.xkrt_hip_taskdom_ready.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_taskdom_complete.argtypes = [_p, _p, _u64]
     L.xkrt_hip_taskdom_create.argtypes = [C.POINTER(TaskDomConfigT), _p, C.POINTER(_p)]
     L.xkrt_hip_taskdom_destroy.argtypes = [_p]
     L.xkrt_hip_taskdom_submit.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
@@ -438,7 +445,7 @@ class Pool:
         ap = (_p * max(nd, 1))(*[a.ctypes.data for a in accs])
         cn = (_u64 * max(nd, 1))(*[len(a) for a in accs])
         first = (_u64 * max(nd, 1))()
-        _chk(self.L.xkrt_hip_pool_stage_group(self.h, nd, dh, ap, cn, first), "xkrt_hip_pool_stage_group")
+        _chk(self.L.xkrt_hip_pool_stage_group(self.h, nd, dh, ap, None, cn, first), "xkrt_hip_pool_stage_group")
         return [int(first[k]) for k in range(nd)]
 
     def stage(self, dom: PoolDomain, acc: np.ndarray) -> int:
@@ -604,11 +611,12 @@ class TaskDomain:
 
     def __init__(self, kernel: int = 1, max_rects: int = 1 << 20, max_edges: int = 1 << 22, budget_bytes: int = 0,
                  seg_col_bytes: int = 1 << 16, seg_bin: tuple[int, int] = (0, 0), blas_bin: tuple[int, int] = (1024, 128),
+                 release: bool = False, max_tasks: int = 0, max_accesses: int = 0, max_succ: int = 0,
                  stream: int | None = None, device: int = 0):
         self.L = hip_lib()
         _chk(self.L.xkrt_hip_init(device), "xkrt_hip_init")
-        cfg = TaskDomConfigT(kernel, 0, max_rects, max_edges, budget_bytes, seg_col_bytes, seg_bin[0], seg_bin[1],
-                             blas_bin[0], blas_bin[1])
+        cfg = TaskDomConfigT(kernel, 1 if release else 0, max_rects, max_edges, budget_bytes, seg_col_bytes,
+                             seg_bin[0], seg_bin[1], blas_bin[0], blas_bin[1], max_tasks, max_accesses, max_succ)
         h = _p()
         _chk(self.L.xkrt_hip_taskdom_create(C.byref(cfg), stream, C.byref(h)), "xkrt_hip_taskdom_create")
         self.h = h
@@ -650,6 +658,17 @@ class TaskDomain:
         _chk(self.L.xkrt_hip_taskdom_edges_copy(self.h, out.ctypes.data, n, C.byref(got)),
              "xkrt_hip_taskdom_edges_copy")
         return out[:int(got.value)]
+
+    def ready(self, cap: int = 1 << 16) -> np.ndarray:
+        """Released tasks not read yet (non-blocking), in release order."""
+        out = np.empty(max(cap, 1), np.uint32)
+        n = _u64()
+        _chk(self.L.xkrt_hip_taskdom_ready(self.h, out.ctypes.data, cap, C.byref(n)), "xkrt_hip_taskdom_ready")
+        return out[:int(n.value)].copy()
+
+    def complete(self, tasks) -> None:
+        t = np.ascontiguousarray(tasks, np.uint32)
+        _chk(self.L.xkrt_hip_taskdom_complete(self.h, t.ctypes.data, len(t)), "xkrt_hip_taskdom_complete")
 
     def stats(self) -> dict:
         st = TaskDomStatsT()
