@@ -514,6 +514,60 @@ int xkrt_hip_taskdom_complete(xkrt_hip_taskdom_t t, const uint32_t *tasks, uint6
 int xkrt_hip_taskdom_stats(xkrt_hip_taskdom_t t, xkrt_hip_taskdom_stats_t *out);
 
 /* ------------------------------------------------------------------------
+ * Handle dependency domains: OpenMP-style dependencies on opaque handles
+ * (pointers), the seam of the task domain's DependencyMap
+ * (include/xkrt/memory/access/handle/dependency-map.hpp:52-330; created at
+ * src/task/task.cc:222-234).  Per handle, the map keeps the concurrent writes
+ * since the last sequential write, the sequential reads since, and that
+ * write; a concurrent write after reads, or a read after concurrent writes,
+ * goes through an empty sequential write the map inserts (a task of its own,
+ * insert_empty_write, 94-135).  Edges are as the reference makes them when no
+ * task has completed (every edge materialised), deduplicated as
+ * __access_precedes does (include/xkrt/task/task.hpp:494-512).
+ * Each submit is one launch; handles are independent and resolved in
+ * parallel, each handle's accesses in spawn order.
+ * ------------------------------------------------------------------------ */
+
+typedef struct xkrt_hip_handle_domain *xkrt_hip_handle_domain_t;
+
+#define XKRT_HIP_CONC_SEQUENTIAL  1u  /* ACCESS_CONCURRENCY_* (include/xkrt/memory/access/concurrency.h:41-46) */
+#define XKRT_HIP_CONC_COMMUTATIVE 2u
+#define XKRT_HIP_CONC_CONCURRENT  4u
+#define XKRT_HIP_VIRTUAL_ID 0x80000000u  /* id bit of an inserted empty write */
+
+typedef struct xkrt_hip_handle_access {  /* 16 B */
+    uint64_t handle;       /* the handle (any value but UINT64_MAX) */
+    uint32_t task;         /* spawn-order task number, < 2^31; a task's accesses are contiguous */
+    uint8_t  mode;         /* XKRT_HIP_MODE_* bits */
+    uint8_t  concurrency;  /* XKRT_HIP_CONC_* */
+    uint8_t  pad[2];
+} xkrt_hip_handle_access_t;
+
+typedef struct xkrt_hip_handle_config {
+    uint64_t max_handles;   /* distinct handles of the domain's life (0 = 2^16) */
+    uint64_t max_accesses;  /* accesses of its life, < 2^31 (0 = 2^22) */
+    uint64_t max_virtual;   /* empty writes it may insert (0 = 2^20) */
+    uint64_t max_batch;     /* accesses per submit (0 = 2^18) */
+    uint64_t max_edges;     /* edge buffer before a copy is needed (0 = 2^22) */
+} xkrt_hip_handle_config_t;
+
+int xkrt_hip_handle_domain_create(const xkrt_hip_handle_config_t *cfg, void *stream, xkrt_hip_handle_domain_t *out);
+int xkrt_hip_handle_domain_destroy(xkrt_hip_handle_domain_t h);
+/* Resolves a batch of whole tasks (asynchronous); ids consecutive from
+ * *first_id (out, nullable).  -EINVAL: bad access or task order; -ENOSPC:
+ * max_accesses reached. */
+int xkrt_hip_handle_submit(xkrt_hip_handle_domain_t h, const xkrt_hip_handle_access_t *acc, uint64_t n,
+                           uint64_t *first_id);
+/* Waits for the domain's launches: *nedges waiting edges, *nvirtual empty
+ * writes inserted so far (ids XKRT_HIP_VIRTUAL_ID | k, k < *nvirtual; each
+ * one is an empty task with one VW access).  -ENOSPC when a capacity
+ * overflowed (the domain's edges are then incomplete). */
+int xkrt_hip_handle_sync(xkrt_hip_handle_domain_t h, uint64_t *nedges, uint64_t *nvirtual);
+/* Copies out and consumes the waiting edges, (pred id, succ id) u32 pairs;
+ * -ENOSPC and nothing consumed when cap is smaller. */
+int xkrt_hip_handle_edges_copy(xkrt_hip_handle_domain_t h, uint32_t *edges, uint64_t cap, uint64_t *n);
+
+/* ------------------------------------------------------------------------
  * Coherency fetch planner: one memory tree per (ld, sizeof_type), a host
  * (device 0) and one GPU (device 1), no eviction.  By default completion is
  * synchronous (the fetches of an access complete before the next access);

@@ -80,7 +80,8 @@ def test_ctypes_layouts_match_compiled_header(tmp_path):
         "xkrt_hip_pool_config_t": R.PoolConfigT, "xkrt_hip_pool_stats_t": R.PoolStatsT,
         "xkrt_hip_server_config_t": R.ServerConfigT, "xkrt_hip_server_stats_t": R.ServerStatsT,
         "xkrt_hip_task_access_t": R.TaskAccessT, "xkrt_hip_taskdom_config_t": R.TaskDomConfigT,
-        "xkrt_hip_taskdom_stats_t": R.TaskDomStatsT, "xkrt_hip_mcc_config_t": K.MccConfigT,
+        "xkrt_hip_taskdom_stats_t": R.TaskDomStatsT, "xkrt_hip_handle_access_t": R.HandleAccessT,
+        "xkrt_hip_handle_config_t": R.HandleConfigT, "xkrt_hip_mcc_config_t": K.MccConfigT,
         "xkrt_hip_mcc_access_t": K.MccAccessT, "xkrt_hip_mcc_stats_t": K.MccStatsT,
     }
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "xkrt_hip.h"', "int main(void) {"]

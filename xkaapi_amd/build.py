@@ -16,7 +16,7 @@ ARCH = os.environ.get("XKRT_HIP_ARCH", "gfx950")
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 
-HIP_SOURCES = ["resolver.hip", "coherency.hip", "release.hip", "taskdom.cpp"]
+HIP_SOURCES = ["resolver.hip", "coherency.hip", "release.hip", "handles.hip", "taskdom.cpp"]
 
 
 def _run(cmd):

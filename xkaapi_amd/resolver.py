@@ -94,6 +94,22 @@ class TaskDomStatsT(C.Structure):
                                           "released", "completed")]
 
 
+class HandleAccessT(C.Structure):
+    _fields_ = [("handle", C.c_uint64), ("task", C.c_uint32), ("mode", C.c_uint8), ("concurrency", C.c_uint8),
+                ("pad", C.c_uint8 * 2)]
+
+
+HANDLE_ACCESS_DTYPE = np.dtype([("handle", "<u8"), ("task", "<u4"), ("mode", "u1"), ("concurrency", "u1"),
+                                ("pad", "u1", (2,))])
+assert HANDLE_ACCESS_DTYPE.itemsize == C.sizeof(HandleAccessT) == 16
+CONC_SEQUENTIAL, CONC_COMMUTATIVE, CONC_CONCURRENT = 1, 2, 4
+VIRTUAL_ID = 1 << 31
+
+
+class HandleConfigT(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("max_handles", "max_accesses", "max_virtual", "max_batch", "max_edges")]
+
+
 # device error bits: 2, 4, 8 and 128 belong to one domain (the pool's other domains go on);
 # 16, 32 and 64 to the pool (shared launch scratch overflowed)
 ERR_BITS = {2: "bin record list", 4: "bin fragments", 8: "bin reader log", 128: "record outside its frame",
@@ -179,6 +195,11 @@ def hip_lib():
     L.xkrt_hip_pool_edges_device.argtypes = [_p, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p)]
     L.xkrt_hip_pool_edges_consume.argtypes = [_p]
     L.xkrt_hip_taskdom_ready.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_handle_domain_create.argtypes = [C.POINTER(HandleConfigT), _p, C.POINTER(_p)]
+    L.xkrt_hip_handle_domain_destroy.argtypes = [_p]
+    L.xkrt_hip_handle_submit.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_handle_sync.argtypes = [_p, C.POINTER(_u64), C.POINTER(_u64)]
+    L.xkrt_hip_handle_edges_copy.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
     L.xkrt_hip_taskdom_complete.argtypes = [_p, _p, _u64]
     L.xkrt_hip_taskdom_create.argtypes = [C.POINTER(TaskDomConfigT), _p, C.POINTER(_p)]
     L.xkrt_hip_taskdom_destroy.argtypes = [_p]
@@ -674,6 +695,62 @@ class TaskDomain:
         st = TaskDomStatsT()
         _chk(self.L.xkrt_hip_taskdom_stats(self.h, C.byref(st)), "xkrt_hip_taskdom_stats")
         return {f: int(getattr(st, f)) for f, _ in TaskDomStatsT._fields_}
+
+
+def to_handle_access_array(task, handle, mode, concurrency) -> np.ndarray:
+    a = np.zeros(len(task), HANDLE_ACCESS_DTYPE)
+    a["task"], a["handle"], a["mode"], a["concurrency"] = task, handle, mode, concurrency
+    return a
+
+
+class HandleDomain:
+    """A GPU handle dependency domain (the reference's DependencyMap,
+    include/xkrt/memory/access/handle/dependency-map.hpp:52-330): ids
+    consecutive per submit; inserted empty writes get ids VIRTUAL_ID | k."""
+
+    def __init__(self, max_handles: int = 1 << 16, max_accesses: int = 1 << 22, max_virtual: int = 1 << 20,
+                 max_batch: int = 1 << 18, max_edges: int = 1 << 22, stream: int | None = None, device: int = 0):
+        self.L = hip_lib()
+        _chk(self.L.xkrt_hip_init(device), "xkrt_hip_init")
+        cfg = HandleConfigT(max_handles, max_accesses, max_virtual, max_batch, max_edges)
+        h = _p()
+        _chk(self.L.xkrt_hip_handle_domain_create(C.byref(cfg), stream, C.byref(h)), "xkrt_hip_handle_domain_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.xkrt_hip_handle_domain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, acc: np.ndarray) -> int:
+        acc = np.ascontiguousarray(acc, HANDLE_ACCESS_DTYPE)
+        first = _u64()
+        _chk(self.L.xkrt_hip_handle_submit(self.h, acc.ctypes.data, len(acc), C.byref(first)), "xkrt_hip_handle_submit")
+        return int(first.value)
+
+    def sync(self):
+        ne, nv = _u64(), _u64()
+        _chk(self.L.xkrt_hip_handle_sync(self.h, C.byref(ne), C.byref(nv)), "xkrt_hip_handle_sync")
+        return int(ne.value), int(nv.value)
+
+    def edges(self) -> np.ndarray:
+        n, _ = self.sync()
+        out = np.empty((max(n, 1), 2), np.uint32)
+        got = _u64()
+        _chk(self.L.xkrt_hip_handle_edges_copy(self.h, out.ctypes.data, n, C.byref(got)), "xkrt_hip_handle_edges_copy")
+        return out[:int(got.value)]
 
 
 def batches_by_tasks(task: np.ndarray, tasks_per_batch) -> list[tuple[int, int]]:
