@@ -471,6 +471,203 @@ def serving_leg(fracs=(0.1, 0.3, 0.5, 0.7, 0.9), cpu=True):
     return out
 
 
+PCIE_PEAK_GBS = 64.0  # PCIe Gen5 x16, one direction (MI355X host link)
+
+
+def segments_leg(reps=3, nchunks=4096, steps=50):
+    """Segment domain (the reference's interval tree): a 1-D blocked stencil
+    over 4096 chunks of 64 KiB (256 MiB), 50 sweeps; each task reads its
+    neighbours' 256-byte halos and updates its chunk (3 segment accesses).
+    One sweep per launch, submitted from host memory."""
+    from xkaapi_amd import resolver as R
+    from xkaapi_amd import streams as S
+    st = S.stencil1d(nchunks, 1 << 16, steps, 256)
+    n = len(st["task"])
+    acc = R.to_access_array(st["task"], st["mode"], st["addr"], st["size"], 0,
+                            kind=np.full(n, R.KIND_SEGMENT, np.uint8))
+    batches = R.batches_by_tasks(st["task"], nchunks)
+    best, dev_ms, edges = None, None, 0
+    for _ in range(reps):
+        with R.SegmentDomain(1 << 16, 1 << 16, 4, kernel=1, max_rects=1 << 16) as d:
+            d.set_timing(True)
+            t0 = time.perf_counter()
+            for a, b in batches:
+                d.submit(acc[a:b])
+            edges = d.sync()
+            dt = time.perf_counter() - t0
+            lt = d.launch_times()
+        if best is None or dt < best:
+            best, dev_ms = dt, float(np.sum(lt))
+    out = {"accesses": n, "tasks": int(st["task"][-1]) + 1, "edges": edges, "launches": len(batches),
+           "accesses_per_s": round(n / best, 1), "accesses_per_s_device": round(n / (dev_ms * 1e-3), 1),
+           "ms": round(best * 1e3, 3), "device_ms": round(dev_ms, 3),
+           "workload": f"1-D stencil over segments: {nchunks} chunks of 64 KiB, {steps} sweeps, one sweep per "
+                       "launch (segment domain, 64 KiB columns, 64 KiB x 4-column bins)"}
+    # the reference's KHPTree<1> engine on the first sweeps, one core
+    k = max(1, min(steps, 5))
+    m = int(np.searchsorted(st["task"], k * nchunks))
+    seg = np.stack([st["addr"][:m], st["addr"][:m] + st["size"][:m]], axis=1)
+    try:
+        from oracle import refkhp
+        kind, fn = ("reference", refkhp.resolve_intervals) if refkhp.available() else (None, None)
+    except Exception:
+        kind, fn = None, None
+    if fn is None:
+        from oracle import khp
+        kind, fn = "port", khp.resolve_intervals
+    t0 = time.perf_counter()
+    fn(st["task"][:m], st["mode"][:m], seg)
+    dt = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "accesses/s", "cores": 1, "kind": kind,
+                           "sample": f"first {k} sweeps ({m} accesses) through "
+                                     + ("the reference's KHPTree<1> with the interval-tree hooks (oracle/_ref)"
+                                        if kind == "reference" else "the oracle's restatement")}
+    return out
+
+
+def release_leg():
+    """Successor release on the device: the Cholesky DAG 8192^2 fp64 in
+    128^2 tiles (45,760 tasks), empty tasks.  The host submits every task in
+    4096-task batches into a task domain with release on, then loops: read the
+    released tasks, complete them, until all ran."""
+    from xkaapi_amd import resolver as R
+    from xkaapi_amd import streams as S
+    from oracle import khp
+    st = S.cholesky(8192, 128)
+    ntask = int(st.task[-1]) + 1
+    acc = R.to_task_access_array(st.task, st.mode, np.zeros(len(st), np.uint8), st.addr, st.m, st.n, st.ld, st.s)
+    batches = R.batches_by_tasks(st.task, 4096)
+    best = None
+    for _ in range(3):
+        with R.TaskDomain(kernel=1, release=True, max_tasks=1 << 16, max_accesses=1 << 18, max_succ=1 << 20) as t:
+            t0 = time.perf_counter()
+            for a, b in batches:
+                t.submit(acc[a:b])
+            t.sync()
+            t1 = time.perf_counter()
+            done, waves, deadline = 0, 0, t1 + 60.0
+            while done < ntask:
+                r = t.ready()
+                if len(r):
+                    t.complete(r)
+                    done += len(r)
+                    waves += 1
+                elif time.perf_counter() > deadline:
+                    raise RuntimeError(f"release: {done} of {ntask} tasks ran")
+            t.sync()
+            t2 = time.perf_counter()
+            stt = t.stats()
+        if best is None or t2 - t0 < best["total_s"]:
+            best = {"total_s": t2 - t0, "resolve_s": t1 - t0, "run_s": t2 - t1, "waves": waves, "stats": stt}
+    out = {"tasks": ntask, "accesses": len(st), "edges_applied": best["stats"]["succ_nodes"],
+           "tasks_per_s": round(ntask / best["total_s"], 1), "ms": round(best["total_s"] * 1e3, 3),
+           "resolve_and_apply_ms": round(best["resolve_s"] * 1e3, 3), "run_ms": round(best["run_s"] * 1e3, 3),
+           "waves": best["waves"],
+           "workload": "cholesky 8192^2 nb128 DAG, empty tasks: submit all (4096-task batches, edges applied "
+                       "on the device), then read released tasks and complete them until all ran"}
+    # the reference's algorithm on one core: apply every edge, commit, run FIFO
+    r, nr = st.rects()
+    e = np.asarray(khp.resolve(st.task, st.mode, r, nr), np.uint32)
+    t0 = time.perf_counter()
+    ran, _ = khp.release_run(st.task, st.mode, e, ntask)
+    dt = time.perf_counter() - t0
+    assert ran == ntask
+    out["cpu_baseline"] = {"value": round(ntask / dt, 1), "unit": "tasks/s", "cores": 1, "kind": "port",
+                           "sample": f"the same DAG ({len(e)} edges given), oracle/khp_oracle.c xko_release_run: "
+                                     "edges applied, tasks committed and run FIFO as the reference's "
+                                     "__task_precedes / __task_commit / __task_complete, 1 thread"}
+    return out
+
+
+def handles_leg(n=1 << 20, nh=4096):
+    """Handle dependency domain: 2^20 accesses on 4096 handles, tasks of 1..3
+    accesses, 30% of writes concurrent (inoutset); 16384-task batches."""
+    from xkaapi_amd import resolver as R
+    from xkaapi_amd import streams as S
+    from oracle import handles as H
+    st = S.handle_stream(n, nh)
+    acc = R.to_handle_access_array(st["task"], st["handle"], st["mode"], st["concurrency"])
+    batches = R.batches_by_tasks(st["task"], 16384)
+    best = None
+    for _ in range(3):
+        with R.HandleDomain(max_handles=nh * 4, max_accesses=n + 16, max_virtual=n, max_batch=1 << 16,
+                            max_edges=1 << 23) as d:
+            t0 = time.perf_counter()
+            for a, b in batches:
+                d.submit(acc[a:b])
+            ne, nv = d.sync()
+            dt = time.perf_counter() - t0
+        if best is None or dt < best[0]:
+            best = (dt, ne, nv)
+    out = {"accesses": n, "handles": nh, "edges": best[1], "empty_writes": best[2],
+           "accesses_per_s": round(n / best[0], 1), "ms": round(best[0] * 1e3, 3), "launches": len(batches),
+           "workload": f"{n} handle accesses on {nh} handles, 1..3 per task, 30% of writes concurrent, "
+                       "16384-task batches from host memory"}
+    m = 100_000
+    t0 = time.perf_counter()
+    H.resolve(st["task"][:m], st["handle"][:m], st["mode"][:m], st["concurrency"][:m])
+    dt = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "accesses/s", "cores": 1, "kind": "port",
+                           "sample": f"first {m} accesses through oracle/handles.py (the dependency map restated "
+                                     "in Python), 1 thread"}
+    return out
+
+
+def fetch_exec_leg(torch, tile=1024):
+    """Fetch-plan execution: GPU reads of every 1024^2 tile of an 8192^2 fp64
+    matrix in pinned host memory (512 MiB), planned in one call on an
+    asynchronous coherency tree; the H2D fetch rows executed by one
+    xkrt_hip_mcc_exec kernel into the chunk arena.  Beside it: one DMA copy
+    of the same bytes (what per-fetch hipMemcpy2DAsync calls are bound by)."""
+    from xkaapi_amd import coherency as K
+    N, s = 8192, 8
+    host = torch.empty(N * N * s, dtype=torch.uint8, pin_memory=True)
+    arena = torch.empty(N * N * s + (64 << 20), dtype=torch.uint8, device="cuda")
+    nt = N // tile
+    ii, jj = np.meshgrid(np.arange(nt), np.arange(nt), indexing="ij")
+    addr = ((jj.ravel() * tile * N + ii.ravel() * tile) * s).astype(np.uint64)
+    k = len(addr)
+    acc = K.to_mcc_array(np.ones(k, np.uint8), np.ones(k, np.uint8), addr, np.full(k, tile, np.uint64),
+                         np.full(k, tile, np.uint64))
+    stream = torch.cuda.current_stream()
+    best = None
+    for _ in range(3):
+        with K.MemoryTree(N, s, (0, N), async_completion=True) as t:
+            t.exec_bind(host.data_ptr(), arena.data_ptr(), arena.numel())
+            plan = t.plan(acc)
+            base = t.stats()["last_fetch_base"]
+            ids = np.arange(base, base + len(plan.fetches), dtype=np.uint64)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            t.exec(ids, stream.cuda_stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1)
+            t.complete(ids)
+        best = ms if best is None else min(best, ms)
+    nbytes = N * N * s
+    dma = None
+    dev = arena[:nbytes]
+    for _ in range(3):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        dev.copy_(host, non_blocking=True)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        dma = ms if dma is None else min(dma, ms)
+    gbs = nbytes / (best * 1e-3) / 1e9
+    out = {"fetches": int(len(plan.fetches)), "bytes": nbytes, "ms": round(best, 3), "GB_per_s": round(gbs, 2),
+           "roofline": {"bound": "pcie", "achieved": round(gbs, 2), "peak": PCIE_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PCIE_PEAK_GBS, 4)},
+           "dma_copy_same_bytes": {"ms": round(dma, 3), "GB_per_s": round(nbytes / (dma * 1e-3) / 1e9, 2)},
+           "workload": f"{k} GPU reads of {tile}^2 fp64 tiles of an {N}^2 matrix in pinned host memory, one plan "
+                       "call, every H2D fetch row executed by one kernel"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -487,6 +684,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--heat-steps", type=int, default=200)
+    ap.add_argument("--no-rows", action="store_true", help="skip the segment / release / handle / fetch legs")
     ap.add_argument("--calibrate-mib", type=int, default=0,
                     help="also run one streaming copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -641,6 +839,13 @@ def main():
     serving = None if args.no_serving else serving_leg(cpu=not args.no_cpu)
     if serving:
         log(f"serving: {serving}")
+    rows = None
+    if not args.no_rows:
+        rows = {}
+        for name, fn in (("segments", segments_leg), ("release", release_leg), ("handles", handles_leg),
+                         ("fetch_exec", lambda: fetch_exec_leg(torch))):
+            rows[name] = fn()
+            log(f"{name}: {rows[name]}")
     out = {
         "metric": "resolved task accesses/sec + p99 submit-to-ready µs, batched vs plain GPU",
         "value": round(res["plain"]["rate"], 1),
@@ -671,6 +876,7 @@ def main():
         "coherency": coh,
         "heat": heat,
         "serving": serving,
+        "rows": rows,
     }
     print(json.dumps(out), flush=True)
 

@@ -179,6 +179,25 @@ def resolve_intervals(task, mode, seg):
     return resolve(task, mode, rects, np.ones(len(seg), np.uint8))
 
 
+def release_run(acc_task, acc_mode, edges, ntask=None):
+    """Runs the task graph on one host core (empty tasks, FIFO): the reference's
+    edge application, commit and completion.  Returns (tasks run, completion order)."""
+    L = lib()
+    if not getattr(L, "_rel_typed", False):
+        L.xko_release_run.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                      C.c_void_p]
+        L.xko_release_run.restype = C.c_int64
+        L._rel_typed = True
+    acc_task = _c(acc_task, np.uint32)
+    acc_mode = _c(acc_mode, np.uint8)
+    edges = _c(np.asarray(edges).reshape(-1, 2), np.uint32)
+    ntask = int(acc_task.max()) + 1 if ntask is None else ntask
+    order = np.empty(max(ntask, 1), np.uint32)
+    ran = L.xko_release_run(ntask, len(acc_task), acc_task.ctypes.data, acc_mode.ctypes.data, len(edges),
+                            edges.ctypes.data, order.ctypes.data)
+    return int(ran), order[:max(int(ran), 0)]
+
+
 def resolve_program(task, mode, kind, addr, m, n, ld, s, _mistake=None):
     """A task domain over a mixed program (src/task/task.cc:153-299): segments
     resolved by the segment domain (resolve_intervals); each (ld, s) pair's

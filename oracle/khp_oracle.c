@@ -1189,3 +1189,55 @@ done:
     (void)rc;  /* an abort is reported through *fail_at; the log holds what came before it */
     return (int64_t)lg.n;
 }
+
+/* ------------------------------------------------------------------------
+ * The runtime's successor release on one host core, the CPU side of the
+ * device release engine's measurement: every edge applied as
+ * __task_precedes does (include/xkrt/task/task.hpp:453-486: the successor's
+ * wait counter + 1, the successor on the predecessor access's list), every
+ * task committed (task.hpp:535-551: the guard of 1 dropped), then the ready
+ * tasks run in FIFO order as empty tasks, each completion walking its
+ * accesses' lists (src/task.cc:160-247; detached accesses, mode 8, skipped).
+ * acc_task: task of each access (tasks 0 .. ntask - 1); edges: (pred, succ)
+ * access pairs.  order (nullable): completion order.  Returns the number of
+ * tasks that ran.
+ * ------------------------------------------------------------------------ */
+int64_t xko_release_run(uint64_t ntask, uint64_t nacc, const uint32_t *acc_task, const uint8_t *acc_mode,
+                        uint64_t ne, const uint32_t *edges, uint32_t *order)
+{
+    uint32_t *wc = (uint32_t *)malloc(4 * (ntask + 1));
+    uint64_t *first = (uint64_t *)calloc(ntask + 2, 8);  /* accesses of task t: [first[t], first[t + 1]) */
+    uint64_t *head = (uint64_t *)malloc(8 * (nacc + 1));
+    uint64_t *next = (uint64_t *)malloc(8 * (ne + 1));
+    uint32_t *succ = (uint32_t *)malloc(4 * (ne + 1));
+    uint32_t *queue = (uint32_t *)malloc(4 * (ntask + 1));
+    if (!wc || !first || !head || !next || !succ || !queue) return -1;
+    for (uint64_t t = 0; t < ntask; ++t) wc[t] = 1;
+    for (uint64_t a = 0; a < nacc; ++a) {
+        head[a] = UINT64_MAX;
+        first[acc_task[a] + 1] = a + 1;
+    }
+    for (uint64_t t = 1; t <= ntask; ++t)
+        if (first[t] < first[t - 1]) first[t] = first[t - 1];
+    for (uint64_t e = 0; e < ne; ++e) {
+        const uint32_t p = edges[2 * e], s = edges[2 * e + 1];
+        wc[acc_task[s]]++;
+        succ[e] = acc_task[s];
+        next[e] = head[p];
+        head[p] = e;
+    }
+    uint64_t qh = 0, qt = 0;
+    for (uint64_t t = 0; t < ntask; ++t)
+        if (--wc[t] == 0) queue[qt++] = (uint32_t)t;
+    while (qh < qt) {
+        const uint32_t t = queue[qh++];
+        if (order) order[qh - 1] = t;
+        for (uint64_t a = first[t]; a < first[t + 1]; ++a) {
+            if (acc_mode[a] & 8) continue;
+            for (uint64_t e = head[a]; e != UINT64_MAX; e = next[e])
+                if (--wc[succ[e]] == 0) queue[qt++] = succ[e];
+        }
+    }
+    free(wc); free(first); free(head); free(next); free(succ); free(queue);
+    return (int64_t)qh;
+}

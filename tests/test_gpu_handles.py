@@ -72,3 +72,17 @@ def test_bad_accesses_are_refused():
             with pytest.raises(R.ResolverError) as ei:
                 d.submit(bad)
             assert ei.value.rc == -errno.EINVAL
+
+
+def test_measured_handle_workload_matches_oracle():
+    # the measured handle stream (a prefix), in its measured batching
+    from xkaapi_amd import streams as S
+    st = S.handle_stream(40000, 512)
+    want, nv = H.resolve(st["task"], st["handle"], st["mode"], st["concurrency"])
+    acc = R.to_handle_access_array(st["task"], st["handle"], st["mode"], st["concurrency"])
+    with R.HandleDomain(max_batch=1 << 16) as d:
+        for a, b in R.batches_by_tasks(st["task"], 16384):
+            d.submit(acc[a:b])
+        got = d.edges()
+        _, gv = d.sync()
+    assert gv == nv and canonical(got) == canonical(want)

@@ -101,3 +101,35 @@ def test_misuse_is_refused():
         with pytest.raises(R.ResolverError) as ei:
             t.sync()
         assert ei.value.rc == -errno.EINVAL
+
+
+def test_cholesky_dag_runs_to_completion():
+    # the measured loop (submit everything, then complete what is released) on a smaller Cholesky:
+    # every task runs once, each after all its predecessors, as the oracle's FIFO run
+    from xkaapi_amd import streams as S
+    st = S.cholesky(2048, 128)
+    ntask = int(st.task[-1]) + 1
+    r, nr = st.rects()
+    e = np.asarray(khp.resolve(st.task, st.mode, r, nr), np.int64)
+    acc = R.to_task_access_array(st.task, st.mode, np.zeros(len(st), np.uint8), st.addr, st.m, st.n, st.ld, st.s)
+    order = []
+    with R.TaskDomain(kernel=1, release=True, max_tasks=1 << 14, max_accesses=1 << 16, max_succ=1 << 18) as t:
+        for a, b in R.batches_by_tasks(st.task, 256):
+            t.submit(acc[a:b])
+        t.sync()
+        for _ in range(100000):
+            rdy = t.ready()
+            if len(rdy):
+                order += rdy.tolist()
+                t.complete(rdy)
+            else:
+                t.sync()
+                if len(order) == ntask and not len(t.ready()):
+                    break
+        assert t.stats()["completed"] == ntask
+    assert sorted(order) == list(range(ntask))
+    pos = {tk: i for i, tk in enumerate(order)}
+    for a, b in e:
+        assert pos[int(st.task[a])] < pos[int(st.task[b])]
+    ran, _ = khp.release_run(st.task, st.mode, e, ntask)
+    assert ran == ntask

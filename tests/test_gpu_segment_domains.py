@@ -236,3 +236,18 @@ def test_task_domain_refused_batch_changes_nothing():
             t.submit(acc[a:b])
         got = t.edges()
     assert same(got, want)
+
+
+def test_stencil_workload_matches_oracle():
+    # the measured 1-D stencil (fewer chunks and sweeps), in its measured domain configuration
+    from xkaapi_amd import streams as S
+    st = S.stencil1d(512, 1 << 16, 6, 256)
+    n = len(st["task"])
+    seg = np.stack([st["addr"], st["addr"] + st["size"]], axis=1)
+    want = khp.resolve_intervals(st["task"], st["mode"], seg)
+    acc = R.to_access_array(st["task"], st["mode"], st["addr"], st["size"], 0, kind=np.full(n, R.KIND_SEGMENT, np.uint8))
+    with R.SegmentDomain(1 << 16, 1 << 16, 4, kernel=1, max_rects=1 << 16) as d:
+        for a, b in R.batches_by_tasks(st["task"], 512):
+            d.submit(acc[a:b])
+        got = d.edges()
+    assert len(want) > 5000 and same(got, want)

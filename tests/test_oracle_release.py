@@ -87,3 +87,28 @@ def test_planted_mistakes_fail(mistake, pd):
         for seed in range(5):
             p, e = program(seed, pd=pd)
             run(p, e, seed, mistake=mistake)
+
+
+@pytest.mark.parametrize("seed,pd", [(0, 0.0), (1, 0.0), (2, 0.3)])
+def test_c_release_run_matches_the_model(seed, pd):
+    # oracle/khp_oracle.c's FIFO run (the CPU baseline of the device release): every task whose
+    # predecessors all ran runs, in a topological order; the same tasks as the Python model
+    # completing everything it releases
+    p, e = program(seed, pd=pd)
+    task, mode = p["task"], p["mode"]
+    ran, order = khp.release_run(task, mode, e)
+    rel = REL.Release(task, mode)
+    out = rel.launch(e, 0, int(task.max()) + 1)
+    done, pending = [], list(out)
+    while pending:
+        t = pending.pop(0)
+        done.append(t)
+        pending += rel.complete([t])
+    assert ran == len(done) and set(order.tolist()) == set(done)
+    pos = {int(t): i for i, t in enumerate(order.tolist())}
+    for a, b in e:
+        ta, tb = int(task[a]), int(task[b])
+        if tb in pos:
+            assert ta in pos and pos[ta] < pos[tb]
+    if pd == 0:
+        assert ran == int(task.max()) + 1

@@ -197,3 +197,54 @@ def concat(streams, renumber: bool = True) -> Stream:
     return Stream(np.concatenate([p[0] for p in parts]),
                   *[np.concatenate([getattr(p[1], f) for p in parts]) for f in
                     ("slot", "mode", "dev", "addr", "ld", "m", "n", "s")])
+
+
+def stencil1d(nchunks: int = 4096, chunk: int = 1 << 16, steps: int = 50, halo: int = 256,
+              base: int = 1 << 40) -> dict:
+    """A 1-D blocked stencil over byte segments: nchunks chunks of `chunk`
+    bytes from `base`; per sweep, task i reads the last `halo` bytes of chunk
+    i - 1 and the first `halo` bytes of chunk i + 1 (when they exist), then
+    updates chunk i (R, R, RW segment accesses, in that slot order).
+    Returns task, mode, addr, size (numpy arrays)."""
+    task, mode, addr, size = [], [], [], []
+    i = np.arange(nchunks, dtype=np.uint64)
+    left = np.flatnonzero(i > 0)
+    right = np.flatnonzero(i < nchunks - 1)
+    for s in range(steps):
+        t0 = s * nchunks
+        parts = [(left, base + left.astype(np.uint64) * chunk - halo, halo, MODE_R, 0),
+                 (right, base + (right.astype(np.uint64) + 1) * chunk, halo, MODE_R, 1),
+                 (np.arange(nchunks), base + i * chunk, chunk, MODE_RW, 2)]
+        tk, md, ad, sz, sl = [], [], [], [], []
+        for idx, a, z, m, slot in parts:
+            tk.append(t0 + np.asarray(idx, np.int64))
+            ad.append(np.asarray(a, np.uint64))
+            sz.append(np.full(len(idx), z, np.uint64))
+            md.append(np.full(len(idx), m, np.uint8))
+            sl.append(np.full(len(idx), slot, np.int64))
+        tk, ad, sz, md, sl = (np.concatenate(x) for x in (tk, ad, sz, md, sl))
+        order = np.lexsort((sl, tk))
+        task.append(tk[order]); mode.append(md[order]); addr.append(ad[order]); size.append(sz[order])
+    return dict(task=np.concatenate(task).astype(np.uint32), mode=np.concatenate(mode),
+                addr=np.concatenate(addr), size=np.concatenate(size))
+
+
+def handle_stream(n: int = 1 << 20, nhandles: int = 4096, pcnc: float = 0.3, per_task: int = 3,
+                  seed: int = 0x4A) -> dict:
+    """OpenMP-style dependencies on handles: tasks of 1..per_task accesses on
+    handles drawn uniformly from nhandles (a task's later accesses shifted to
+    other handles, so they rarely share one); modes R / W / RW, writes
+    concurrent (inoutset) with probability pcnc, else sequential or
+    commutative.  Returns task, handle, mode, concurrency."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(1, per_task + 1, n)
+    task = np.repeat(np.arange(n, dtype=np.uint32), k)[:n]
+    handle = (rng.integers(0, nhandles, n).astype(np.uint64) * np.uint64(64) + np.uint64(0x7F00_0000_0000))
+    # distinct handles inside a task: shift repeats by the slot
+    start = np.r_[0, np.flatnonzero(np.diff(task.astype(np.int64)) != 0) + 1]
+    slot = np.arange(n) - np.repeat(start, np.diff(np.r_[start, n]))
+    handle = handle + (slot.astype(np.uint64) * np.uint64(64 * 7919)) % np.uint64(64 * nhandles)
+    mode = rng.choice(np.array([MODE_R, MODE_W, MODE_RW], np.uint8), n)
+    conc = np.where(rng.random(n) < pcnc, 4, rng.choice(np.array([1, 2], np.uint8), n)).astype(np.uint8)
+    conc = np.where(mode & MODE_W, conc, 1).astype(np.uint8)
+    return dict(task=task, handle=handle, mode=mode, concurrency=conc)
