@@ -48,7 +48,8 @@ enum {
 /* access kinds */
 enum {
     XKRT_HIP_KIND_TILE    = 0,  /* column-major tile: addr, m rows, n columns */
-    XKRT_HIP_KIND_SEGMENT = 1   /* byte segment [addr, addr + m) seen by a BLAS domain */
+    XKRT_HIP_KIND_SEGMENT = 1,  /* byte segment [addr, addr + m) seen by a BLAS domain */
+    XKRT_HIP_KIND_HANDLE  = 2   /* task domains only: an opaque handle (addr), see xkrt_hip_taskdom_submit */
 };
 
 /* One access.  A tile is the region of the reference's BLAS access
@@ -427,8 +428,14 @@ int xkrt_hip_server_stats(xkrt_hip_server_t s, xkrt_hip_server_stats_t *out);
  * edges come from one domain (a segment's from the segment domain, a tile's
  * from its pair's), so the domains' edges, mapped to program ids, are the
  * reference's edges with no cross-domain duplicate.
- * Host memory: 8 bytes per access per domain it enters (id maps), 16 bytes
- * per segment (the replay).  Handle accesses are not taken (-EINVAL).
+ * Handle accesses (XKRT_HIP_KIND_HANDLE) go to a handle domain created at the
+ * first one (the DependencyMap of task.cc:222-234, see xkrt_hip_handle_*):
+ * their edges join the others' in program ids, an empty write the map inserts
+ * as XKRT_HIP_TASKDOM_VIRTUAL | k.  Not with release on (-ENOTSUP).  Limits:
+ * 2^18 handle accesses per batch (-EINVAL), 2^22 over the domain's life
+ * (-ENOSPC).
+ * Host memory: 32 bytes per segment (the replay), 4 bytes per handle access
+ * (its program id).
  * ------------------------------------------------------------------------ */
 
 typedef struct xkrt_hip_taskdom *xkrt_hip_taskdom_t;
@@ -442,9 +449,15 @@ typedef struct xkrt_hip_task_access {
     uint32_t sizeof_type;  /* tile: element size (bytes); segment: ignored */
     uint32_t task;         /* spawn-order task number; a task's accesses are contiguous */
     uint8_t  mode;         /* XKRT_HIP_MODE_* bits */
-    uint8_t  kind;         /* XKRT_HIP_KIND_TILE or XKRT_HIP_KIND_SEGMENT */
-    uint8_t  pad[6];
+    uint8_t  kind;         /* XKRT_HIP_KIND_TILE, _SEGMENT or _HANDLE */
+    uint8_t  concurrency;  /* handle: XKRT_HIP_CONC_* (0 = sequential); otherwise ignored */
+    uint8_t  pad[5];
 } xkrt_hip_task_access_t;
+
+/* Program id bit of an empty write inserted by the task domain's handle map
+ * (its k-th one: XKRT_HIP_TASKDOM_VIRTUAL | k; an empty task with one VW
+ * access, see xkrt_hip_handle_sync). */
+#define XKRT_HIP_TASKDOM_VIRTUAL (1ull << 63)
 
 typedef struct xkrt_hip_taskdom_config {
     uint32_t kernel;          /* LDS capacity class of every domain (see xkrt_hip_dep_config_t) */
@@ -489,7 +502,7 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
  * to be copied out.  A domain's device error is returned here (and later). */
 int xkrt_hip_taskdom_sync(xkrt_hip_taskdom_t t, uint64_t *nedges);
 /* Copies out and consumes every waiting edge: (pred, succ) program access ids,
- * 2 u64 each.  -ENOSPC, nothing consumed, *n = edges waiting, when cap is
+ * 2 u64 each (an inserted empty write: XKRT_HIP_TASKDOM_VIRTUAL | k).  -ENOSPC, nothing consumed, *n = edges waiting, when cap is
  * smaller.  Call after xkrt_hip_taskdom_sync.  -EPERM with release on. */
 int xkrt_hip_taskdom_edges_copy(xkrt_hip_taskdom_t t, uint64_t *edges, uint64_t cap, uint64_t *n);
 

@@ -198,7 +198,7 @@ def release_run(acc_task, acc_mode, edges, ntask=None):
     return int(ran), order[:max(int(ran), 0)]
 
 
-def resolve_program(task, mode, kind, addr, m, n, ld, s, _mistake=None):
+def resolve_program(task, mode, kind, addr, m, n, ld, s, _mistake=None, conc=None):
     """A task domain over a mixed program (src/task/task.cc:153-299): segments
     resolved by the segment domain (resolve_intervals); each (ld, s) pair's
     BLAS domain, created at its first tile, resolves its tiles and the segments
@@ -222,6 +222,15 @@ def resolve_program(task, mode, kind, addr, m, n, ld, s, _mistake=None):
     s = np.asarray(s, np.uint64)
     mode = np.asarray(mode, np.uint8)
     out = []
+    hnd = np.flatnonzero(kind == 2)
+    if len(hnd):
+        # handle accesses: the dependency map (oracle/handles.py) over program ids; an inserted
+        # empty write keeps the map's id (handles.VIRT | k)
+        from . import handles as HM
+        cc = np.ones(len(kind), np.uint8) if conc is None else np.asarray(conc, np.uint8)
+        cc = np.where(cc == 0, 1, cc)
+        hm = HM.HandleMap()
+        out.append(np.asarray(hm.resolve(hnd, task[hnd], addr[hnd], mode[hnd], cc[hnd]), np.int64).reshape(-1, 2))
     seg = np.flatnonzero(kind == 1)
     if len(seg) and _mistake != "no_segment_domain":
         e = resolve_intervals(task[seg], mode[seg], np.stack([addr[seg], addr[seg] + m[seg]], axis=1))
@@ -233,7 +242,7 @@ def resolve_program(task, mode, kind, addr, m, n, ld, s, _mistake=None):
         if key not in [p[0] for p in pairs]:
             pairs.append((key, int(i)))
     for (pld, ps), first in pairs:
-        if (addr < np.uint64(pld * ps)).any():
+        if ((addr < np.uint64(pld * ps)) & (kind != 2)).any():
             raise ValueError("an access in the first ld * s bytes: the replayed segments would mark nodes")
         late_seg = (kind == 1) & (np.arange(len(kind)) > first) & (_mistake != "segments_skip_blas")
         mine = np.flatnonzero(((kind == 0) & (ld == pld) & (s == ps)) | late_seg)
@@ -245,7 +254,8 @@ def resolve_program(task, mode, kind, addr, m, n, ld, s, _mistake=None):
     if not out:
         return np.zeros((0, 2), np.int64)
     e = np.concatenate(out).reshape(-1, 2)
-    key = np.stack([e[:, 0], task[e[:, 1]].astype(np.int64), e[:, 1]], axis=1)
+    tk = lambda x: np.where(x >= (1 << 31), x, task[np.minimum(x, len(task) - 1)].astype(np.int64))
+    key = np.stack([e[:, 0], tk(e[:, 1]), e[:, 1]], axis=1)
     key = key[np.lexsort((key[:, 2], key[:, 1], key[:, 0]))]
     first = np.r_[True, (np.diff(key[:, 0]) != 0) | (np.diff(key[:, 1]) != 0)]
     return key[first][:, [0, 2]]

@@ -251,3 +251,45 @@ def test_stencil_workload_matches_oracle():
             d.submit(acc[a:b])
         got = d.edges()
     assert len(want) > 5000 and same(got, want)
+
+
+def canonical_virtual(e, virt):
+    """Edges with each inserted empty write (id >= virt) renamed by its first program successor."""
+    e = [(int(p), int(q)) for p, q in np.asarray(e).reshape(-1, 2).tolist()]
+    first = {}
+    for p, q in e:
+        if p >= virt and q < virt:
+            first[p] = min(first.get(p, q), q)
+    name = lambda x: ("X", first[x]) if x >= virt else x
+    return {(name(p), name(q)) for p, q in e}
+
+
+@pytest.mark.parametrize("seed,tasks", [(0, 40), (1, 500)])
+def test_task_domain_with_handles_matches_oracle(seed, tasks):
+    # segments, tiles of three pairs and handle accesses (sequential, commutative and concurrent
+    # writes) in one program: the handle map's edges join the others'
+    p = G.program_stream(5000, 20 + seed)
+    rng = np.random.default_rng(seed)
+    hsel = rng.random(len(p["task"])) < 0.3
+    kind = np.where(hsel, R.KIND_HANDLE, p["kind"]).astype(np.uint8)
+    addr = np.where(hsel, rng.integers(0, 40, len(kind)).astype(np.uint64) * 8 + 8, p["addr"]).astype(np.uint64)
+    conc = np.where(hsel & (p["mode"] & 2 > 0), rng.choice(np.array([1, 2, 4], np.uint8), len(kind)), 1)
+    conc = conc.astype(np.uint8)
+    args = [p["task"], p["mode"], kind, addr, p["m"], p["n"], p["ld"], p["s"]]
+    want = khp.resolve_program(*args, conc=conc)
+    acc = R.to_task_access_array(*args, concurrency=conc)
+    with R.TaskDomain(**{k: v for k, v in TD.items()}) as t:
+        for a, b in R.batches_by_tasks(p["task"], tasks):
+            assert t.submit(acc[a:b]) == a
+        got = t.edges()
+    g = canonical_virtual(got.astype(object), R.TASKDOM_VIRTUAL)
+    w = canonical_virtual(np.asarray(want, np.int64), 1 << 31)
+    assert len(got) == len(want) and g == w
+
+
+def test_task_domain_handles_refused_with_release():
+    acc = R.to_task_access_array([0], [2], [R.KIND_HANDLE], [64], [1], [0], [0], [0])
+    with R.TaskDomain(**dict(TD, release=True, max_tasks=1 << 10, max_accesses=1 << 12, max_succ=1 << 12)) as t:
+        with pytest.raises(R.ResolverError) as ei:
+            t.submit(acc)
+        assert ei.value.rc == -errno.ENOTSUP

@@ -176,3 +176,25 @@ def test_program_oracle_matches_golden_fixture():
     z = np.load(f"{GOLDEN}/program_mixed.npz")
     args = [z[k] for k in ("task", "mode", "kind", "addr", "m", "n", "ld", "s")]
     assert same(khp.resolve_program(*args), z["edges"])
+
+
+def test_program_with_handles_is_the_union_of_its_domains():
+    # handle accesses join the program: the segment / BLAS edges are unchanged and the handle map's
+    # edges (between handle accesses only) are added
+    p = G.program_stream(2000, 12)
+    rng = np.random.default_rng(12)
+    hsel = rng.random(len(p["task"])) < 0.3
+    kind = np.where(hsel, 2, p["kind"]).astype(np.uint8)
+    addr = np.where(hsel, rng.integers(0, 20, len(kind)).astype(np.uint64) * 8 + 8, p["addr"]).astype(np.uint64)
+    conc = np.where(hsel & (p["mode"] & 2 > 0) & (rng.random(len(kind)) < 0.3), 4, 1).astype(np.uint8)
+    args = [p["task"], p["mode"], kind, addr, p["m"], p["n"], p["ld"], p["s"]]
+    got = pairs(khp.resolve_program(*args, conc=conc))
+    keep = ~hsel
+    sub = [np.asarray(a)[keep] for a in args]
+    ids = np.flatnonzero(keep)
+    base = ids[pairs(khp.resolve_program(*sub))]
+    from oracle import handles as HM
+    h = np.flatnonzero(hsel)
+    he = pairs(HM.HandleMap().resolve(h, p["task"][h], addr[h], p["mode"][h], conc[h]))
+    assert np.array_equal(got, np.unique(np.concatenate([base, he]), axis=0))
+    assert len(he) > 100

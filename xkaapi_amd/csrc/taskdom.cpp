@@ -49,6 +49,11 @@ struct xkrt_hip_taskdom {
     std::vector<std::vector<xkrt_hip_access_t>> part;
     std::vector<std::vector<uint32_t>> part_ids;
     std::vector<uint32_t> e32, edom;
+    // handle accesses: their domain, and the program id of each of its accesses
+    xkrt_hip_handle_domain_t hdom = nullptr;
+    std::vector<uint32_t> hids;
+    std::vector<xkrt_hip_handle_access_t> hpart;
+    std::vector<uint32_t> he32;
     // release on the device
     xkrt_release::Engine *rel = nullptr;
     uint64_t next_task = 0;              // tasks are numbered 0, 1, ... in submit order
@@ -58,6 +63,11 @@ struct xkrt_hip_taskdom {
 };
 
 namespace {
+
+// lifetime capacity of a task domain's handle map (accesses, and inserted empty writes), and
+// the handle accesses one batch may hold
+constexpr uint64_t HANDLE_MAX_ACCESSES = 1ull << 22;
+constexpr uint64_t HANDLE_MAX_BATCH = 1ull << 18;
 
 int dom_index(xkrt_hip_dep_domain_t d)
 {
@@ -158,6 +168,7 @@ int xkrt_hip_taskdom_destroy(xkrt_hip_taskdom_t t)
 {
     if (!t) return -EINVAL;
     xkrt_release::destroy(t->rel);
+    if (t->hdom) xkrt_hip_handle_domain_destroy(t->hdom);
     const int rc = xkrt_hip_pool_destroy(t->pool);  // releases its domains
     delete t;
     return rc;
@@ -171,16 +182,26 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
     if (n == 0) return 0;
     // validation first: a refused batch leaves nothing behind
     if (t->next_id + n > 0xFFFFFFFFull) return -ERANGE;  // program access ids are 32-bit
+    uint64_t nh = 0;  // handle accesses
     for (uint64_t i = 0; i < n; ++i) {
         const xkrt_hip_task_access_t &q = acc[i];
         const uint32_t prev = i ? acc[i - 1].task : t->last_task;
         if ((i || t->any_task) && q.task < prev) return -EINVAL;
         // released on the device, tasks are numbered 0, 1, ... in submit order
         if (t->rel && (i ? q.task - prev > 1 : q.task != t->next_task)) return -EINVAL;
+        if (q.kind == XKRT_HIP_KIND_HANDLE) {
+            if (t->rel) return -ENOTSUP;  // inserted empty writes are tasks the release engine does not hold
+            if (q.addr == UINT64_MAX || q.task >= 0x80000000u) return -EINVAL;
+            if (q.concurrency > XKRT_HIP_CONC_CONCURRENT || q.concurrency == 3) return -EINVAL;
+            ++nh;
+            continue;
+        }
         if (q.kind != XKRT_HIP_KIND_TILE && q.kind != XKRT_HIP_KIND_SEGMENT) return -EINVAL;
         if (q.m == 0) return -EINVAL;
         if (q.kind == XKRT_HIP_KIND_TILE && (q.n == 0 || q.ld == 0 || q.sizeof_type == 0)) return -EINVAL;
     }
+    if (nh > HANDLE_MAX_BATCH) return -EINVAL;
+    if (nh && t->hids.size() + nh > HANDLE_MAX_ACCESSES) return -ENOSPC;
     if (t->rel && ((uint64_t)acc[n - 1].task + 1 > t->cfg.max_tasks || t->next_id + n > t->cfg.max_accesses))
         return -ENOSPC;
     // domains this batch creates (destroyed again if the batch is refused)
@@ -204,6 +225,7 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
         const xkrt_hip_task_access_t &q = acc[i];
         const uint32_t id = (uint32_t)(t->next_id + i);
         const xkrt_hip_access_t a = plain(q);
+        if (q.kind == XKRT_HIP_KIND_HANDLE) continue;  // submitted below, once the rest is staged
         if (q.kind == XKRT_HIP_KIND_SEGMENT) {
             if (!t->seg) {
                 int rc = xkrt_hip_pool_seg_domain_create(t->pool, t->cfg.seg_col_bytes, t->cfg.seg_bin_x_bytes,
@@ -271,6 +293,35 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
         }
     }
     if (rc) return undo(rc);
+    if (nh) {
+        if (!t->hdom) {
+            xkrt_hip_handle_config_t hc{};
+            hc.max_accesses = HANDLE_MAX_ACCESSES;
+            hc.max_virtual = HANDLE_MAX_ACCESSES;
+            hc.max_batch = HANDLE_MAX_BATCH;
+            rc = xkrt_hip_handle_domain_create(&hc, nullptr, &t->hdom);
+            if (rc) {
+                t->err = rc;  // the rest of the batch is staged already
+                return rc;
+            }
+        }
+        t->hpart.clear();
+        for (uint64_t i = 0; i < n; ++i) {
+            if (acc[i].kind != XKRT_HIP_KIND_HANDLE) continue;
+            xkrt_hip_handle_access_t h{};
+            h.handle = acc[i].addr;
+            h.task = acc[i].task;
+            h.mode = acc[i].mode;
+            h.concurrency = acc[i].concurrency ? acc[i].concurrency : (uint8_t)XKRT_HIP_CONC_SEQUENTIAL;
+            t->hpart.push_back(h);
+            t->hids.push_back((uint32_t)(t->next_id + i));
+        }
+        rc = xkrt_hip_handle_submit(t->hdom, t->hpart.data(), t->hpart.size(), nullptr);
+        if (rc) {
+            t->err = rc;
+            return rc;
+        }
+    }
     if (t->rel) {
         t->btask.resize(n);
         t->bmode.resize(n);
@@ -311,6 +362,12 @@ int xkrt_hip_taskdom_sync(xkrt_hip_taskdom_t t, uint64_t *nedges)
         if (e && !t->err) t->err = e;
     }
     if (t->rel && !t->err) t->err = xkrt_release::check(t->rel);
+    if (t->hdom) {
+        uint64_t hn = 0;
+        const int e = xkrt_hip_handle_sync(t->hdom, &hn, nullptr);
+        if (e && !t->err) t->err = e;
+        if (nedges) *nedges += hn;
+    }
     return t->err;
 }
 
@@ -318,19 +375,39 @@ int xkrt_hip_taskdom_edges_copy(xkrt_hip_taskdom_t t, uint64_t *edges, uint64_t 
 {
     if (!t || !n || (cap && !edges)) return -EINVAL;
     if (t->rel) return -EPERM;  // applied and consumed on the device
-    uint64_t ne = 0;
+    uint64_t ne = 0, hn = 0;
     int rc = xkrt_hip_pool_sync(t->pool, &ne);
-    *n = ne;
+    if (!rc && t->hdom) rc = xkrt_hip_handle_sync(t->hdom, &hn, nullptr);
+    *n = ne + hn;
     if (rc) return rc;
-    if (ne > cap) return -ENOSPC;
+    if (ne + hn > cap) return -ENOSPC;
+    if (hn) {
+        t->he32.resize(2 * hn);
+        uint64_t got = 0;
+        rc = xkrt_hip_handle_edges_copy(t->hdom, t->he32.data(), hn, &got);
+        if (rc) return rc;
+        hn = got;
+        // handle-domain ids -> program ids; an inserted empty write keeps its index
+        auto prog = [&](uint32_t id) -> uint64_t {
+            return id & XKRT_HIP_VIRTUAL_ID ? (XKRT_HIP_TASKDOM_VIRTUAL | (id & ~XKRT_HIP_VIRTUAL_ID)) : t->hids[id];
+        };
+        for (uint64_t i = 0; i < hn; ++i) {
+            if (!(t->he32[2 * i] & XKRT_HIP_VIRTUAL_ID) && t->he32[2 * i] >= t->hids.size()) return -EIO;
+            if (!(t->he32[2 * i + 1] & XKRT_HIP_VIRTUAL_ID) && t->he32[2 * i + 1] >= t->hids.size()) return -EIO;
+            edges[2 * (ne + i)] = prog(t->he32[2 * i]);
+            edges[2 * (ne + i) + 1] = prog(t->he32[2 * i + 1]);
+        }
+    }
     t->e32.resize(2 * ne + 2);
     t->edom.resize(ne + 1);
     uint64_t got = 0;
     rc = xkrt_hip_pool_edges_copy(t->pool, t->e32.data(), t->edom.data(), ne, &got);
     if (rc) return rc;
     for (uint64_t i = 0; i < 2 * got; ++i) edges[i] = t->e32[i];  // program ids already
-    *n = got;
-    t->edges_out += got;
+    if (got < ne && hn)  // edges of failed domains were dropped: close the gap
+        memmove(edges + 2 * got, edges + 2 * ne, 16 * hn);
+    *n = got + hn;
+    t->edges_out += got + hn;
     return 0;
 }
 

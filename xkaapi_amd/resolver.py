@@ -73,11 +73,14 @@ class ServerStatsT(C.Structure):
 class TaskAccessT(C.Structure):
     _fields_ = [("addr", C.c_uint64), ("m", C.c_uint64), ("n", C.c_uint64), ("ld", C.c_uint64),
                 ("sizeof_type", C.c_uint32), ("task", C.c_uint32), ("mode", C.c_uint8), ("kind", C.c_uint8),
-                ("pad", C.c_uint8 * 6)]
+                ("concurrency", C.c_uint8), ("pad", C.c_uint8 * 5)]
 
 
+KIND_HANDLE = 2
+TASKDOM_VIRTUAL = 1 << 63
 TASK_ACCESS_DTYPE = np.dtype([("addr", "<u8"), ("m", "<u8"), ("n", "<u8"), ("ld", "<u8"), ("sizeof_type", "<u4"),
-                              ("task", "<u4"), ("mode", "u1"), ("kind", "u1"), ("pad", "u1", (6,))])
+                              ("task", "<u4"), ("mode", "u1"), ("kind", "u1"), ("concurrency", "u1"),
+                              ("pad", "u1", (5,))])
 assert TASK_ACCESS_DTYPE.itemsize == C.sizeof(TaskAccessT) == 48
 
 
@@ -614,13 +617,16 @@ class Server:
         return {f: int(getattr(st, f)) for f, _ in ServerStatsT._fields_}
 
 
-def to_task_access_array(task, mode, kind, addr, m, n, ld, sizeof_type) -> np.ndarray:
-    """Task-domain accesses: tiles (kind 0: addr, m rows, n columns, ld, sizeof_type)
-    and segments (kind 1: [addr, addr + m))."""
+def to_task_access_array(task, mode, kind, addr, m, n, ld, sizeof_type, concurrency=None) -> np.ndarray:
+    """Task-domain accesses: tiles (kind 0: addr, m rows, n columns, ld, sizeof_type),
+    segments (kind 1: [addr, addr + m)) and handles (kind 2: addr is the handle,
+    with a concurrency)."""
     a = np.zeros(len(task), TASK_ACCESS_DTYPE)
     a["task"], a["mode"], a["kind"] = task, mode, kind
     a["addr"], a["m"], a["n"] = addr, m, n
     a["ld"], a["sizeof_type"] = ld, sizeof_type
+    if concurrency is not None:
+        a["concurrency"] = concurrency
     return a
 
 
