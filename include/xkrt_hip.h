@@ -473,7 +473,7 @@ typedef struct xkrt_hip_taskdom_config {
     uint32_t blas_bin_y_cols;
     uint64_t max_tasks;       /* release: tasks of the domain's lifetime (0 = 2^20) */
     uint64_t max_accesses;    /* release: accesses (0 = 4 x max_tasks) */
-    uint64_t max_succ;        /* release: successor-list entries, one per edge applied (0 = 2 x max_accesses) */
+    uint64_t max_succ;        /* release: successor-list entries, one per edge resolved (0 = 2 x max_accesses) */
 } xkrt_hip_taskdom_config_t;
 
 typedef struct xkrt_hip_taskdom_stats {
@@ -482,7 +482,7 @@ typedef struct xkrt_hip_taskdom_stats {
     uint64_t blas_domains;    /* BLAS domains created */
     uint64_t launches;
     uint64_t edges;           /* edges copied out so far */
-    uint64_t succ_nodes;      /* release: edges applied on the device (successor-list entries) */
+    uint64_t succ_nodes;      /* release: successor-list entries taken, one per edge resolved */
     uint64_t released;        /* release: tasks released */
     uint64_t completed;       /* release: tasks completed */
 } xkrt_hip_taskdom_stats_t;
@@ -561,7 +561,8 @@ typedef struct xkrt_hip_handle_config {
     uint64_t max_accesses;  /* accesses of its life, < 2^31 (0 = 2^22) */
     uint64_t max_virtual;   /* empty writes it may insert (0 = 2^20) */
     uint64_t max_batch;     /* accesses per submit (0 = 2^18) */
-    uint64_t max_edges;     /* edge buffer before a copy is needed (0 = 2^22) */
+    uint64_t max_edges;     /* edge buffer before a copy is needed (0 = 2^22), in 64 stripes of max_edges / 64:
+                               a handle's edges go to stripe (its table slot mod 64) */
 } xkrt_hip_handle_config_t;
 
 int xkrt_hip_handle_domain_create(const xkrt_hip_handle_config_t *cfg, void *stream, xkrt_hip_handle_domain_t *out);

@@ -8,8 +8,9 @@
 //     node per slot: the last sequential write, the heads of the sequential
 //     reads and concurrent writes lists, and whether an access was put there
 //     (a handle nothing was put on yet links to nothing);
-//   * list entries {access, next} from a pool: an access enters at most one
-//     list once, so the pool needs one entry per access of the domain's life;
+//   * list entries {access, next}, indexed by access id: an access enters at
+//     most one list once (an inserted empty write none), so entry a is
+//     access a's and no counter is shared;
 //   * per access (program and empty writes) the task of its last successor,
 //     for __access_precedes' dedup (include/xkrt/task/task.hpp:494-512);
 //   * the edge buffer, (pred id, succ id) pairs; an empty write inserted by
@@ -24,8 +25,9 @@
 //   k_h_replay one thread per handle present (the first record of its run):
 //              the node state in registers, the handle's records replayed --
 //              LINK of every access of a task, then PUT (src/task/task.cc:
-//              285-299) -- and written back.
-// Handles are independent, so the work is one thread per handle; a handle's
+//              285-299) -- and written back; its edges go to one of 64
+//              stripes of the edge buffer.
+// Handles are independent, so the work is one chain per handle; a handle's
 // own chain is sequential, as it is in the reference.  The kernels are
 // latency bound (a handful of dependent loads per record).
 
@@ -35,6 +37,8 @@
 #include <errno.h>
 #include <stdint.h>
 #include <string.h>
+
+#include <algorithm>
 
 #include "../../include/xkrt_hip.h"
 
@@ -46,6 +50,9 @@ constexpr unsigned long long EMPTY_KEY = ~0ull;
 constexpr uint32_t MODE_R = 1, MODE_W = 2, MODE_VW = 6;
 constexpr uint32_t CNC = 4;
 enum : uint32_t { HERR_TABLE = 1, HERR_ENTRIES = 2, HERR_VIRTUAL = 4, HERR_EDGES = 8 };
+// the edge buffer is cut into stripes, a handle's edges going to stripe (slot mod ESTRIPES): the
+// runs of a launch append concurrently, and one shared count serialised them
+constexpr uint32_t ESTRIPES = 64;
 
 struct HRec {
     unsigned long long handle;
@@ -65,10 +72,10 @@ struct HDev {
     uint32_t *last_prog;       // per program access: task of its last successor
     uint32_t *last_virt;       // per empty write
     uint2 *edges;
-    unsigned int *ctr;         // [0] entries used, [1] empty writes, [2] errors, [3] edges (u64 at [4])
-    unsigned long long *nedges;
+    unsigned int *ctr;         // [1] empty writes, [2] errors
+    unsigned long long *scount;  // edges per stripe
     uint32_t mask, max_entries, max_virtual, max_prog;
-    unsigned long long max_edges;
+    unsigned long long stripe_cap;  // edges per stripe: max_edges / ESTRIPES
 };
 
 __device__ uint32_t hash64(unsigned long long h)
@@ -107,6 +114,7 @@ struct Replay {
     const HDev &d;
     Node st;
     uint32_t emitted;
+    uint32_t stripe;  // the edge stripe this handle appends to
 
     __device__ uint32_t &last(uint32_t id) { return id & VIRT ? d.last_virt[id & ~VIRT] : d.last_prog[id]; }
 
@@ -117,9 +125,9 @@ struct Replay {
         if (l == stask) return;
         l = stask;
         emitted++;
-        const unsigned long long k = atomicAdd(d.nedges, 1ull);
-        if (k < d.max_edges)
-            d.edges[k] = make_uint2(p, s);
+        const unsigned long long k = atomicAdd(&d.scount[stripe], 1ull);
+        if (k < d.stripe_cap)
+            d.edges[stripe * d.stripe_cap + k] = make_uint2(p, s);
         else
             atomicOr(&d.ctr[2], HERR_EDGES);
     }
@@ -129,15 +137,15 @@ struct Replay {
         for (uint32_t e = head; e != NONE; e = d.entry[e].y) precedes(d.entry[e].x, s, stask);
     }
 
+    // a program access enters at most one list, once: its entry is its own id (no shared counter)
     __device__ void push(uint32_t &head, uint32_t a)
     {
-        const uint32_t e = atomicAdd(&d.ctr[0], 1u);
-        if (e >= d.max_entries) {
+        if (a >= d.max_entries) {
             atomicOr(&d.ctr[2], HERR_ENTRIES);
             return;
         }
-        d.entry[e] = make_uint2(a, head);
-        head = e;
+        d.entry[a] = make_uint2(a, head);
+        head = a;
     }
 
     __device__ void put(uint32_t a, uint32_t mode, uint32_t conc)
@@ -206,7 +214,9 @@ struct Replay {
     }
 };
 
-// one thread per handle run of the sorted records
+// one thread per sorted record; the first record of each handle's run replays the run.  The
+// runs are chains of dependent accesses, so a launch lasts as long as its longest run: all of
+// them start together.
 __global__ void k_h_replay(HDev d, const HRec *rec, const unsigned long long *keys, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -215,7 +225,7 @@ __global__ void k_h_replay(HDev d, const HRec *rec, const unsigned long long *ke
     const uint32_t slot = (uint32_t)(k0 >> 32);
     if (k0 == ~0ull) return;
     if (i > 0 && (uint32_t)(keys[i - 1] >> 32) == slot) return;
-    Replay r{d, d.node[slot], 0};
+    Replay r{d, d.node[slot], 0, slot & (ESTRIPES - 1)};
     uint32_t j = i;
     while (j < n && (uint32_t)(keys[j] >> 32) == slot && keys[j] != ~0ull) {
         const uint32_t task = rec[(uint32_t)keys[j]].task;
@@ -270,6 +280,7 @@ struct xkrt_hip_handle_domain {
     hipEvent_t staged = nullptr;
     bool staged_pending = false;
     unsigned int *ctr_host = nullptr;  // pinned copy of the counters
+    unsigned long long *scount_host = nullptr;
     uint64_t next_id = 0, launches = 0;
     uint32_t last_task = 0;
     bool any_task = false;
@@ -310,7 +321,7 @@ int xkrt_hip_handle_domain_create(const xkrt_hip_handle_config_t *cfg, void *str
     d.max_entries = (uint32_t)c.max_accesses;
     d.max_virtual = (uint32_t)c.max_virtual;
     d.max_prog = (uint32_t)c.max_accesses;
-    d.max_edges = c.max_edges;
+    d.stripe_cap = c.max_edges / ESTRIPES;
     int bits = 1;
     while ((1ull << bits) < cap) ++bits;
     h->sort_bits = 32 + bits;
@@ -319,12 +330,13 @@ int xkrt_hip_handle_domain_create(const xkrt_hip_handle_config_t *cfg, void *str
         (rc = hrc(hipMalloc(&d.last_prog, 4ull * c.max_accesses))) ||
         (rc = hrc(hipMalloc(&d.last_virt, 4ull * c.max_virtual))) ||
         (rc = hrc(hipMalloc(&d.edges, 8ull * c.max_edges))) || (rc = hrc(hipMalloc(&d.ctr, 64))) ||
+        (rc = hrc(hipMalloc(&d.scount, 8ull * ESTRIPES))) ||
+        (rc = hrc(hipHostMalloc(&h->scount_host, 8ull * ESTRIPES, hipHostMallocDefault))) ||
         (rc = hrc(hipMalloc(&h->rec, sizeof(HRec) * c.max_batch))) ||
         (rc = hrc(hipMalloc(&h->keys, 8ull * c.max_batch))) || (rc = hrc(hipMalloc(&h->keys_sorted, 8ull * c.max_batch))) ||
         (rc = hrc(hipHostMalloc(&h->rec_host, sizeof(HRec) * c.max_batch, hipHostMallocDefault))) ||
         (rc = hrc(hipHostMalloc(&h->ctr_host, 64, hipHostMallocDefault))) || (rc = hrc(hipEventCreate(&h->staged))))
         return fail(rc);
-    d.nedges = (unsigned long long *)(d.ctr + 4);
     if ((rc = hrc(hipcub::DeviceRadixSort::SortKeys(nullptr, h->sort_tmp_bytes, h->keys, h->keys_sorted,
                                                      (int)c.max_batch, 0, h->sort_bits, h->stream))))
         return fail(rc);
@@ -335,6 +347,7 @@ int xkrt_hip_handle_domain_create(const xkrt_hip_handle_config_t *cfg, void *str
     hipLaunchKernelGGL(k_h_fill_u32, g, b, 0, h->stream, d.last_prog, (uint64_t)c.max_accesses, NONE);
     hipLaunchKernelGGL(k_h_fill_u32, g, b, 0, h->stream, d.last_virt, (uint64_t)c.max_virtual, NONE);
     hipLaunchKernelGGL(k_h_fill_u32, g, b, 0, h->stream, (uint32_t *)d.ctr, 16, 0u);
+    hipLaunchKernelGGL(k_h_fill_u32, g, b, 0, h->stream, (uint32_t *)d.scount, 2 * ESTRIPES, 0u);
     if ((rc = hrc(hipGetLastError())) || (rc = hrc(hipStreamSynchronize(h->stream)))) return fail(rc);
     *out = h;
     return 0;
@@ -346,11 +359,12 @@ int xkrt_hip_handle_domain_destroy(xkrt_hip_handle_domain_t h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     HDev &d = h->d;
     for (void *p : {(void *)d.keys, (void *)d.node, (void *)d.entry, (void *)d.last_prog, (void *)d.last_virt,
-                    (void *)d.edges, (void *)d.ctr, (void *)h->rec, (void *)h->keys, (void *)h->keys_sorted,
+                    (void *)d.edges, (void *)d.ctr, (void *)d.scount, (void *)h->rec, (void *)h->keys, (void *)h->keys_sorted,
                     h->sort_tmp})
         if (p) (void)hipFree(p);
     if (h->rec_host) (void)hipHostFree(h->rec_host);
     if (h->ctr_host) (void)hipHostFree(h->ctr_host);
+    if (h->scount_host) (void)hipHostFree(h->scount_host);
     if (h->staged) (void)hipEventDestroy(h->staged);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -409,9 +423,11 @@ int xkrt_hip_handle_sync(xkrt_hip_handle_domain_t h, uint64_t *nedges, uint64_t 
 {
     if (!h) return -EINVAL;
     int rc = hrc(hipMemcpyAsync(h->ctr_host, h->d.ctr, 64, hipMemcpyDeviceToHost, h->stream));
+    if (!rc) rc = hrc(hipMemcpyAsync(h->scount_host, h->d.scount, 8 * ESTRIPES, hipMemcpyDeviceToHost, h->stream));
     if (!rc) rc = hrc(hipStreamSynchronize(h->stream));
     if (rc) return rc;
-    const unsigned long long ne = *(unsigned long long *)(h->ctr_host + 4);
+    unsigned long long ne = 0;
+    for (uint32_t k = 0; k < ESTRIPES; ++k) ne += std::min<unsigned long long>(h->scount_host[k], h->d.stripe_cap);
     if (nedges) *nedges = ne;
     if (nvirtual) *nvirtual = h->ctr_host[1];
     const uint32_t err = h->ctr_host[2];
@@ -427,11 +443,16 @@ int xkrt_hip_handle_edges_copy(xkrt_hip_handle_domain_t h, uint32_t *edges, uint
     *n = ne;
     if (rc) return rc;
     if (ne > cap) return -ENOSPC;
-    if (ne) {
-        rc = hrc(hipMemcpyAsync(edges, h->d.edges, 8 * ne, hipMemcpyDeviceToHost, h->stream));
+    uint64_t off = 0;
+    for (uint32_t k = 0; k < ESTRIPES; ++k) {  // the stripes, one after the other
+        const uint64_t c = std::min<unsigned long long>(h->scount_host[k], h->d.stripe_cap);
+        if (!c) continue;
+        rc = hrc(hipMemcpyAsync(edges + 2 * off, h->d.edges + k * h->d.stripe_cap, 8 * c, hipMemcpyDeviceToHost,
+                                h->stream));
         if (rc) return rc;
+        off += c;
     }
-    hipLaunchKernelGGL(k_h_fill_u32, dim3(1), dim3(64), 0, h->stream, (uint32_t *)(h->d.ctr + 4), 2, 0u);
+    hipLaunchKernelGGL(k_h_fill_u32, dim3(1), dim3(64), 0, h->stream, (uint32_t *)h->d.scount, 2 * ESTRIPES, 0u);
     return hrc(hipStreamSynchronize(h->stream));
 }
 

@@ -15,7 +15,7 @@
 // Data layout in HBM (one engine per task domain, capacities fixed at
 // creation): per task a wait counter, a state word and its first access id;
 // per access its task (bit 31: detached) and the head of its successor list;
-// per applied edge one list node {successor task, next node}.  Released
+// per resolved edge one list node {successor task, next node}.  Released
 // tasks go to a queue in pinned host memory, one slot per task, written with
 // system-scope release stores: the host reads them as they land, with no
 // stream synchronisation.
@@ -74,10 +74,17 @@ __global__ void k_fill(uint32_t *p, uint64_t n, uint32_t v)
         p[i] = v;
 }
 
-// one thread per unconsumed edge (grid-stride: the count is read on the device)
+// one thread per unconsumed edge (grid-stride: the count is read on the device).  Edge i takes
+// list node base + i (base = nodes used before the launch; k_guard advances it), so no thread
+// contends for a shared node counter; an edge from a completed task leaves its node unused.
 __global__ void k_apply(Dev r, const uint2 *edges, const unsigned long long *count)
 {
     const uint64_t n = *count;
+    const uint64_t base = r.ctr[0];
+    if (base + n > r.max_nodes) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&r.ctr[2], RERR_NODES);
+        return;
+    }
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint2 e = edges[i];
         if (e.x >= r.max_acc || e.y >= r.max_acc) {
@@ -90,20 +97,21 @@ __global__ void k_apply(Dev r, const uint2 *edges, const unsigned long long *cou
             continue;
         }
         if (r.state[tp] == ST_DONE) continue;  // a completed predecessor adds no edge
-        const uint32_t k = atomicAdd(&r.ctr[0], 1u);
-        if (k >= r.max_nodes) {
-            atomicOr(&r.ctr[2], RERR_NODES);
-            continue;
-        }
+        const uint32_t k = (uint32_t)(base + i);
         atomicAdd(&r.wc[ts], 1u);
         r.node[k].x = ts;
         r.node[k].y = atomicExch(&r.head[e.x], k);
     }
 }
 
-// the commit guard of the launch's tasks
-__global__ void k_guard(Dev r, uint32_t t0, uint32_t t1)
+// the commit guard of the launch's tasks; the first thread moves the node base past the
+// launch's edges (k_apply has finished: stream order)
+__global__ void k_guard(Dev r, uint32_t t0, uint32_t t1, const unsigned long long *count)
 {
+    // edges not applied (no room): the launch's tasks keep their guard -- none runs before its
+    // predecessors; the domain reports -ENOSPC
+    if (r.ctr[2] & RERR_NODES) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) r.ctr[0] += (unsigned int)*count;
     const uint32_t t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= t1) return;
     if (atomicSub(&r.wc[t], 1u) == 1u) release(r, t);
@@ -229,7 +237,8 @@ int apply(Engine *e, const uint32_t *edges, const uint64_t *count, uint32_t t0, 
     if (!e || !edges || !count || t0 > t1 || t1 > e->ntasks) return -EINVAL;
     hipLaunchKernelGGL(k_apply, dim3(512), dim3(256), 0, e->stream, e->d, (const uint2 *)edges,
                        (const unsigned long long *)count);
-    if (t1 > t0) hipLaunchKernelGGL(k_guard, dim3((t1 - t0 + 255) / 256), dim3(256), 0, e->stream, e->d, t0, t1);
+    hipLaunchKernelGGL(k_guard, dim3(t1 > t0 ? (t1 - t0 + 255) / 256 : 1), dim3(256), 0, e->stream, e->d, t0, t1,
+                       (const unsigned long long *)count);
     return hip_rc(hipGetLastError());
 }
 
