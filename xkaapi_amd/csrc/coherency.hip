@@ -208,9 +208,19 @@ __global__ void c_count(CArgs a)
                     m &= m - 1;
                     const uint32_t *row = a.TY + (size_t)us * a.ny1;
                     uint32_t np = 0;
-                    for (uint32_t yc = y0; yc < y1; yc += 64) {
-                        const uint32_t y = yc + lane;
-                        np += (uint32_t)__popcll(__ballot(y < y1 && (y == y0 || row[y] <= aid)));
+                    // four 64-row chunks per step: their loads are in flight together
+                    for (uint32_t yc = y0; yc < y1; yc += 256) {
+                        uint32_t v[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const uint32_t y = yc + 64 * k + lane;
+                            v[k] = (y < y1 && y != y0) ? row[y] : 0u;
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const uint32_t y = yc + 64 * k + lane;
+                            np += (uint32_t)__popcll(__ballot(y < y1 && (y == y0 || v[k] <= aid)));
+                        }
                     }
                     if (lane == 0) a.xcnt[xo + (us - x0)] = np;
                     total += np;
@@ -228,9 +238,19 @@ __device__ void slab_rows(const CArgs &a, uint32_t i, uint32_t aid, uint32_t us,
     const uint32_t lane = lane_id();
     const uint32_t *row = a.TY + (size_t)us * a.ny1;
     uint32_t pend_y = NONE, pend_r = 0;
+    uint32_t v[4];  // four 64-row chunks loaded ahead, so their loads are in flight together
     for (uint32_t yc = y0; yc < y1; yc += 64) {
+        const uint32_t k = ((yc - y0) >> 6) & 3u;
+        if (k == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t yq = yc + 64 * q + lane;
+                v[q] = (yq < y1 && yq != y0) ? row[yq] : 0u;
+            }
+        }
         const uint32_t y = yc + lane;
-        const bool ps = y < y1 && (y == y0 || row[y] <= aid);
+        const uint32_t vk = k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
+        const bool ps = y < y1 && (y == y0 || vk <= aid);
         const uint64_t m = __ballot(ps);
         if (!m) continue;
         if (pend_y != NONE && lane == 0) {
