@@ -431,7 +431,8 @@ int xkrt_hip_server_stats(xkrt_hip_server_t s, xkrt_hip_server_stats_t *out);
  * Handle accesses (XKRT_HIP_KIND_HANDLE) go to a handle domain created at the
  * first one (the DependencyMap of task.cc:222-234, see xkrt_hip_handle_*):
  * their edges join the others' in program ids, an empty write the map inserts
- * as XKRT_HIP_TASKDOM_VIRTUAL | k.  Not with release on (-ENOTSUP).  Limits:
+ * as XKRT_HIP_TASKDOM_VIRTUAL | k; with release on, the device runs them itself
+ * (an empty task, completed as soon as it is released).  Limits:
  * 2^18 handle accesses per batch (-EINVAL), 2^22 over the domain's life
  * (-ENOSPC).
  * Host memory: 32 bytes per segment (the replay), 4 bytes per handle access
@@ -474,6 +475,7 @@ typedef struct xkrt_hip_taskdom_config {
     uint64_t max_tasks;       /* release: tasks of the domain's lifetime (0 = 2^20) */
     uint64_t max_accesses;    /* release: accesses (0 = 4 x max_tasks) */
     uint64_t max_succ;        /* release: successor-list entries, one per edge resolved (0 = 2 x max_accesses) */
+    uint64_t max_virtual;     /* release: empty writes the handle map may insert (0 = max_tasks) */
 } xkrt_hip_taskdom_config_t;
 
 typedef struct xkrt_hip_taskdom_stats {
@@ -572,6 +574,18 @@ int xkrt_hip_handle_domain_destroy(xkrt_hip_handle_domain_t h);
  * max_accesses reached. */
 int xkrt_hip_handle_submit(xkrt_hip_handle_domain_t h, const xkrt_hip_handle_access_t *acc, uint64_t n,
                            uint64_t *first_id);
+/* Same, with the access ids the edges name given (ids[i] of acc[i], strictly
+ * increasing across the domain's submits, below max_accesses): e.g. a task
+ * domain's program ids. */
+int xkrt_hip_handle_submit_ids(xkrt_hip_handle_domain_t h, const xkrt_hip_handle_access_t *acc, const uint32_t *ids,
+                               uint64_t n);
+/* For consumers on the device (the domain's stream order): the edge buffer,
+ * *nstripes stripes of *stripe_cap (pred, succ) u32 pairs, the u64 count of
+ * each stripe, and the u32 device word counting the inserted empty writes.
+ * xkrt_hip_handle_edges_consume clears the counts in stream order. */
+int xkrt_hip_handle_edges_device(xkrt_hip_handle_domain_t h, const uint32_t **edges, const uint64_t **stripe_counts,
+                                 uint32_t *nstripes, uint64_t *stripe_cap, const uint32_t **nvirtual);
+int xkrt_hip_handle_edges_consume(xkrt_hip_handle_domain_t h);
 /* Waits for the domain's launches: *nedges waiting edges, *nvirtual empty
  * writes inserted so far (ids XKRT_HIP_VIRTUAL_ID | k, k < *nvirtual; each
  * one is an empty task with one VW access).  -ENOSPC when a capacity

@@ -133,3 +133,47 @@ def test_cholesky_dag_runs_to_completion():
         assert pos[int(st.task[a])] < pos[int(st.task[b])]
     ran, _ = khp.release_run(st.task, st.mode, e, ntask)
     assert ran == ntask
+
+
+def test_release_with_handles_matches_oracle():
+    # handle accesses in a released task domain: the empty writes the map inserts run on the device
+    # (they never reach the queue); the released program tasks after every step equal the oracle's
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_oracle_release import handle_program
+    for seed in range(2):
+        p, e, trig = handle_program(10 + seed, n=2500)
+        task, mode = p["task"], p["mode"]
+        nv = len(trig)
+        kk = np.array([trig[k] for k in range(nv)])
+        acc = R.to_task_access_array(task, mode, p["kind"], p["addr"], p["m"], p["n"], p["ld"], p["s"],
+                                     concurrency=p["conc"])
+        rel = REL.Release(task, mode, nvirt=nv)
+        rng = np.random.default_rng(seed)
+        pool, steps = set(), 0
+        with R.TaskDomain(**TD) as t:
+            def step(want):
+                nonlocal steps
+                t.sync()
+                got = t.ready()
+                assert sorted(got.tolist()) == sorted(want), (seed, steps)
+                pool.update(want)
+                steps += 1
+
+            for a, b in task_batches(task, 30):
+                assert t.submit(acc[a:b]) == a
+                vs = np.flatnonzero((kk >= a) & (kk < b))
+                v = (int(vs.min()), int(vs.max()) + 1) if len(vs) else (0, 0)
+                sel = ((e[:, 1] >= a) & (e[:, 1] < b)) | ((e[:, 1] >= REL.VIRT) & np.isin(e[:, 1] - REL.VIRT, vs))
+                step(rel.launch(e[sel], int(task[a]), int(task[b - 1]) + 1, virt=v))
+                while pool and rng.random() < 0.7:
+                    pick = sorted(rng.choice(sorted(pool), min(len(pool), 5), replace=False).tolist())
+                    pool.difference_update(pick)
+                    t.complete(pick)
+                    step(rel.complete(pick))
+            while pool:
+                pick = sorted(pool)
+                pool.clear()
+                t.complete(pick)
+                step(rel.complete(pick))
+            assert t.stats()["completed"] == int(task.max()) + 1

@@ -287,9 +287,16 @@ def test_task_domain_with_handles_matches_oracle(seed, tasks):
     assert len(got) == len(want) and g == w
 
 
-def test_task_domain_handles_refused_with_release():
-    acc = R.to_task_access_array([0], [2], [R.KIND_HANDLE], [64], [1], [0], [0], [0])
-    with R.TaskDomain(**dict(TD, release=True, max_tasks=1 << 10, max_accesses=1 << 12, max_succ=1 << 12)) as t:
+def test_task_domain_handle_limits():
+    # more handle accesses in one batch than a handle launch takes are refused whole
+    n = (1 << 18) + 1
+    acc = R.to_task_access_array(np.arange(n, dtype=np.uint32), np.full(n, 2, np.uint8),
+                                 np.full(n, R.KIND_HANDLE, np.uint8), np.arange(n, dtype=np.uint64) * 8 + 8,
+                                 np.ones(n, np.uint64), np.zeros(n, np.uint64), np.zeros(n, np.uint64),
+                                 np.zeros(n, np.uint64))
+    with R.TaskDomain(**TD) as t:
         with pytest.raises(R.ResolverError) as ei:
             t.submit(acc)
-        assert ei.value.rc == -errno.ENOTSUP
+        assert ei.value.rc == -errno.EINVAL
+        assert t.submit(acc[:1000]) == 0
+        assert len(t.edges()) == 0

@@ -112,3 +112,81 @@ def test_c_release_run_matches_the_model(seed, pd):
             assert ta in pos and pos[ta] < pos[tb]
     if pd == 0:
         assert ran == int(task.max()) + 1
+
+
+def handle_program(seed, n=1500):
+    """A task-domain program with handle accesses (concurrent writes too): its
+    edges (program ids, empty writes as VIRT | k) and, per empty write, the access
+    it was inserted for (its first program successor)."""
+    p = G.program_stream(n, seed, modes=(1, 2, 3))
+    rng = np.random.default_rng(seed)
+    hsel = rng.random(len(p["task"])) < 0.4
+    kind = np.where(hsel, 2, p["kind"]).astype(np.uint8)
+    addr = np.where(hsel, rng.integers(0, 12, len(kind)).astype(np.uint64) * 8 + 8, p["addr"]).astype(np.uint64)
+    conc = np.where(hsel & (p["mode"] & 2 > 0), rng.choice(np.array([1, 4], np.uint8), len(kind)), 1).astype(np.uint8)
+    args = [p["task"], p["mode"], kind, addr, p["m"], p["n"], p["ld"], p["s"]]
+    e = np.asarray(khp.resolve_program(*args, conc=conc), np.int64)
+    trig = {}
+    for a, b in e:
+        if a >= REL.VIRT and b < REL.VIRT:
+            trig[int(a - REL.VIRT)] = min(trig.get(int(a - REL.VIRT), b), b)
+    return dict(p, kind=kind, addr=addr, conc=conc), e, trig
+
+
+def test_release_with_empty_writes_matches_closed_form():
+    # inserted empty writes are tasks of their own, completed as soon as they are released
+    for seed in range(3):
+        p, e, trig = handle_program(seed)
+        task, mode = p["task"], p["mode"]
+        nv = len(trig)
+        assert nv > 10 and sorted(trig) == list(range(nv))
+        rng = np.random.default_rng(seed)
+        rel = REL.Release(task, mode, nvirt=nv)
+        ntask = int(task.max()) + 1
+        la, da = np.full(ntask, np.inf), np.full(ntask, np.inf)
+        vla = np.full(nv, np.inf)
+        cum, pool, ev = set(), set(), 0
+        kk = np.array([trig[k] for k in range(nv)])
+        for a, b in task_batches(task, 30):
+            t0, t1 = int(task[a]), int(task[b - 1]) + 1
+            la[t0:t1] = ev
+            vs = np.flatnonzero((kk >= a) & (kk < b))
+            v = (int(vs.min()), int(vs.max()) + 1) if len(vs) else (0, 0)
+            vla[v[0]:v[1]] = ev
+            sel = ((e[:, 1] >= a) & (e[:, 1] < b)) | ((e[:, 1] >= REL.VIRT) & np.isin(e[:, 1] - REL.VIRT, vs))
+            out = rel.launch(e[sel], t0, t1, virt=v)
+            cum |= set(out); pool |= set(out)
+            assert cum == REL.released_closed_form_virtual(e, task, mode, la, da, ev, vla), ev
+            ev += 1
+            while pool and rng.random() < 0.7:
+                pick = sorted(rng.choice(sorted(pool), min(len(pool), 5), replace=False).tolist())
+                pool.difference_update(pick)
+                da[pick] = ev
+                out = rel.complete(pick)
+                cum |= set(out); pool |= set(out)
+                assert cum == REL.released_closed_form_virtual(e, task, mode, la, da, ev, vla), ev
+                ev += 1
+        while pool:
+            pick = sorted(pool)
+            pool.clear()
+            da[pick] = ev
+            out = rel.complete(pick)
+            cum |= set(out); pool |= set(out)
+            ev += 1
+        assert cum == set(range(ntask))
+
+
+def test_empty_writes_left_guarded_block_their_successors():
+    # a model that never drops the empty writes' guards releases fewer tasks than the closed form
+    p, e, trig = handle_program(1)
+    task, mode = p["task"], p["mode"]
+    rel = REL.Release(task, mode, nvirt=len(trig))
+    out = rel.launch(e, 0, int(task.max()) + 1, virt=(0, 0))
+    done = set(out)
+    pending = list(out)
+    while pending:
+        t = pending.pop()
+        more = rel.complete([t])
+        done |= set(more)
+        pending += more
+    assert len(done) < int(task.max()) + 1

@@ -284,6 +284,8 @@ struct xkrt_hip_handle_domain {
     uint64_t next_id = 0, launches = 0;
     uint32_t last_task = 0;
     bool any_task = false;
+    bool any_id = false;  // caller ids given: the last one
+    uint32_t last_id = 0;
 };
 
 extern "C" {
@@ -371,14 +373,24 @@ int xkrt_hip_handle_domain_destroy(xkrt_hip_handle_domain_t h)
     return 0;
 }
 
-int xkrt_hip_handle_submit(xkrt_hip_handle_domain_t h, const xkrt_hip_handle_access_t *acc, uint64_t n,
-                           uint64_t *first_id)
+} // extern "C"
+
+namespace {
+
+int submit(xkrt_hip_handle_domain_t h, const xkrt_hip_handle_access_t *acc, const uint32_t *ids, uint64_t n,
+           uint64_t *first_id)
 {
     if (!h || (!acc && n)) return -EINVAL;
     if (first_id) *first_id = h->next_id;
     if (n == 0) return 0;
     if (n > h->cfg.max_batch) return -EINVAL;
-    if (h->next_id + n > h->cfg.max_accesses) return -ENOSPC;
+    if (ids) {  // caller ids: strictly increasing across submits, below max_accesses
+        for (uint64_t i = 0; i < n; ++i)
+            if ((i ? ids[i] <= ids[i - 1] : (h->any_id && ids[0] <= h->last_id)) || ids[i] >= h->cfg.max_accesses)
+                return ids[i] >= h->cfg.max_accesses ? -ENOSPC : -EINVAL;
+    } else if (h->next_id + n > h->cfg.max_accesses) {
+        return -ENOSPC;
+    }
     for (uint64_t i = 0; i < n; ++i) {
         const xkrt_hip_handle_access_t &q = acc[i];
         const uint32_t prev = i ? acc[i - 1].task : h->last_task;
@@ -395,7 +407,7 @@ int xkrt_hip_handle_submit(xkrt_hip_handle_domain_t h, const xkrt_hip_handle_acc
     for (uint64_t i = 0; i < n; ++i) {
         HRec &r = h->rec_host[i];
         r.handle = acc[i].handle;
-        r.acc = (uint32_t)(h->next_id + i);
+        r.acc = ids ? ids[i] : (uint32_t)(h->next_id + i);
         r.task = acc[i].task;
         r.mc = acc[i].mode | (uint32_t)acc[i].concurrency << 8;
         r.pad = 0;
@@ -415,8 +427,48 @@ int xkrt_hip_handle_submit(xkrt_hip_handle_domain_t h, const xkrt_hip_handle_acc
     h->next_id += n;
     h->any_task = true;
     h->last_task = acc[n - 1].task;
+    if (ids) {
+        h->any_id = true;
+        h->last_id = ids[n - 1];
+    }
     h->launches++;
     return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int xkrt_hip_handle_submit(xkrt_hip_handle_domain_t h, const xkrt_hip_handle_access_t *acc, uint64_t n,
+                           uint64_t *first_id)
+{
+    return submit(h, acc, nullptr, n, first_id);
+}
+
+int xkrt_hip_handle_submit_ids(xkrt_hip_handle_domain_t h, const xkrt_hip_handle_access_t *acc, const uint32_t *ids,
+                               uint64_t n)
+{
+    if (!ids && n) return -EINVAL;
+    return submit(h, acc, ids, n, nullptr);
+}
+
+int xkrt_hip_handle_edges_device(xkrt_hip_handle_domain_t h, const uint32_t **edges, const uint64_t **stripe_counts,
+                                 uint32_t *nstripes, uint64_t *stripe_cap, const uint32_t **nvirtual)
+{
+    if (!h || !edges || !stripe_counts || !nstripes || !stripe_cap || !nvirtual) return -EINVAL;
+    *edges = (const uint32_t *)h->d.edges;
+    *stripe_counts = (const uint64_t *)h->d.scount;
+    *nstripes = ESTRIPES;
+    *stripe_cap = h->d.stripe_cap;
+    *nvirtual = (const uint32_t *)&h->d.ctr[1];
+    return 0;
+}
+
+int xkrt_hip_handle_edges_consume(xkrt_hip_handle_domain_t h)
+{
+    if (!h) return -EINVAL;
+    hipLaunchKernelGGL(k_h_fill_u32, dim3(1), dim3(64), 0, h->stream, (uint32_t *)h->d.scount, 2 * ESTRIPES, 0u);
+    return hrc(hipGetLastError());
 }
 
 int xkrt_hip_handle_sync(xkrt_hip_handle_domain_t h, uint64_t *nedges, uint64_t *nvirtual)

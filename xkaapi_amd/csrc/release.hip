@@ -43,6 +43,7 @@ namespace {
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 constexpr uint32_t ST_WAIT = 0, ST_READY = 1, ST_DONE = 2;
 constexpr uint32_t DETACHED = 1u << 31;
+constexpr uint32_t VIRT = 0x80000000u;  // edge end: the k-th empty write a handle map inserted
 constexpr uint8_t MODE_D = 8;  // ACCESS_MODE_D (include/xkrt/memory/access/mode.h:62)
 enum : uint32_t { RERR_NODES = 1, RERR_STATE = 2, RERR_RANGE = 4 };
 
@@ -54,11 +55,15 @@ struct Dev {
     uint32_t *head;      // per access: first successor node, EMPTY when none
     uint2 *node;         // per node: {successor task, next node}
     uint32_t *ready;     // device address of the pinned release queue
-    unsigned int *ctr;   // [0] nodes used, [1] queue slots used, [2] error bits, [3] completions
-    uint32_t max_tasks, max_acc, max_nodes;
+    unsigned int *ctr;   // [0] nodes used, [1] queue slots used, [2] error bits, [3] completions,
+                         // [4] empty writes whose guard was dropped
+    uint32_t max_tasks, max_acc, max_nodes, max_virtual;
+    // tasks [max_tasks, max_tasks + max_virtual) and accesses [max_acc, max_acc + max_virtual) are
+    // the empty writes handle maps insert (an empty task with one access each)
 };
 
-__device__ void release(const Dev &r, uint32_t t)
+// a program task to the ready queue
+__device__ void release_task(const Dev &r, uint32_t t)
 {
     r.state[t] = ST_READY;
     const uint32_t slot = atomicAdd(&r.ctr[1], 1u);
@@ -66,6 +71,64 @@ __device__ void release(const Dev &r, uint32_t t)
         __hip_atomic_store(&r.ready[slot], t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     else
         atomicOr(&r.ctr[2], RERR_RANGE);
+}
+
+// a released task: a program task goes to the queue; an inserted empty write has nothing to run
+// and completes at once (its successors are program tasks: an empty write is only ever linked to
+// reads, concurrent writes or program accesses, never to another empty write)
+__device__ void release(const Dev &r, uint32_t t)
+{
+    if (t < r.max_tasks) {
+        release_task(r, t);
+        return;
+    }
+    r.state[t] = ST_DONE;
+    for (uint32_t k = r.head[r.max_acc + (t - r.max_tasks)]; k != EMPTY; k = r.node[k].y) {
+        const uint32_t s = r.node[k].x;
+        if (s >= r.max_tasks) {
+            atomicOr(&r.ctr[2], RERR_RANGE);
+            continue;
+        }
+        if (atomicSub(&r.wc[s], 1u) == 1u) release_task(r, s);
+    }
+}
+
+// an edge end: its access (list head index) and task
+__device__ bool edge_end(const Dev &r, uint32_t id, uint32_t &acc, uint32_t &task)
+{
+    if (id & VIRT) {
+        const uint32_t k = id & ~VIRT;
+        if (k >= r.max_virtual) return false;
+        acc = r.max_acc + k;
+        task = r.max_tasks + k;
+        return true;
+    }
+    if (id >= r.max_acc) return false;
+    acc = id;
+    task = r.acc_task[id] & ~DETACHED;
+    return task < r.max_tasks;
+}
+
+// __task_precedes for one edge, its list node given
+__device__ void apply_edge(const Dev &r, uint2 e, uint32_t node)
+{
+    uint32_t pa, tp, sa, ts;
+    if (!edge_end(r, e.x, pa, tp) || !edge_end(r, e.y, sa, ts)) {
+        atomicOr(&r.ctr[2], RERR_RANGE);
+        return;
+    }
+    if (r.state[tp] == ST_DONE) return;  // a completed predecessor adds no edge
+    atomicAdd(&r.wc[ts], 1u);
+    r.node[node].x = ts;
+    r.node[node].y = atomicExch(&r.head[pa], node);
+}
+
+// edges of the stripes before stripe s (each clamped to the stripe's room)
+__device__ uint64_t stripe_prefix(const unsigned long long *scount, uint32_t s, uint64_t cap)
+{
+    uint64_t p = 0;
+    for (uint32_t j = 0; j < s; ++j) p += min((uint64_t)scount[j], cap);
+    return p;
 }
 
 __global__ void k_fill(uint32_t *p, uint64_t n, uint32_t v)
@@ -77,44 +140,59 @@ __global__ void k_fill(uint32_t *p, uint64_t n, uint32_t v)
 // one thread per unconsumed edge (grid-stride: the count is read on the device).  Edge i takes
 // list node base + i (base = nodes used before the launch; k_guard advances it), so no thread
 // contends for a shared node counter; an edge from a completed task leaves its node unused.
-__global__ void k_apply(Dev r, const uint2 *edges, const unsigned long long *count)
+__global__ void k_apply(Dev r, const uint2 *edges, const unsigned long long *count,
+                        const unsigned long long *hcount, uint32_t hstripes, uint64_t hcap)
 {
     const uint64_t n = *count;
     const uint64_t base = r.ctr[0];
-    if (base + n > r.max_nodes) {
+    const uint64_t nh = hcount ? stripe_prefix(hcount, hstripes, hcap) : 0;
+    if (base + n + nh > r.max_nodes) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&r.ctr[2], RERR_NODES);
         return;
     }
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint2 e = edges[i];
-        if (e.x >= r.max_acc || e.y >= r.max_acc) {
-            atomicOr(&r.ctr[2], RERR_RANGE);
-            continue;
-        }
-        const uint32_t tp = r.acc_task[e.x] & ~DETACHED, ts = r.acc_task[e.y] & ~DETACHED;
-        if (tp >= r.max_tasks || ts >= r.max_tasks) {
-            atomicOr(&r.ctr[2], RERR_RANGE);
-            continue;
-        }
-        if (r.state[tp] == ST_DONE) continue;  // a completed predecessor adds no edge
-        const uint32_t k = (uint32_t)(base + i);
-        atomicAdd(&r.wc[ts], 1u);
-        r.node[k].x = ts;
-        r.node[k].y = atomicExch(&r.head[e.x], k);
-    }
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        apply_edge(r, edges[i], (uint32_t)(base + i));
+}
+
+// the handle map's edges of the launch, in stripes (stripe s = blockIdx.y): they take the nodes
+// after the pool's edges
+__global__ void k_apply_striped(Dev r, const unsigned long long *count, const uint2 *hedges,
+                                const unsigned long long *hcount, uint32_t hstripes, uint64_t hcap)
+{
+    const uint64_t n = *count;
+    const uint64_t base = r.ctr[0];
+    if (base + n + stripe_prefix(hcount, hstripes, hcap) > r.max_nodes) return;  // k_apply flagged it
+    const uint32_t st = blockIdx.y;
+    const uint64_t m = min((uint64_t)hcount[st], hcap);
+    const uint64_t off = base + n + stripe_prefix(hcount, st, hcap);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+        apply_edge(r, hedges[st * hcap + i], (uint32_t)(off + i));
 }
 
 // the commit guard of the launch's tasks; the first thread moves the node base past the
-// launch's edges (k_apply has finished: stream order)
-__global__ void k_guard(Dev r, uint32_t t0, uint32_t t1, const unsigned long long *count)
+// launch's edges (the apply kernels have finished: stream order)
+__global__ void k_guard(Dev r, uint32_t t0, uint32_t t1, const unsigned long long *count,
+                        const unsigned long long *hcount, uint32_t hstripes, uint64_t hcap)
 {
     // edges not applied (no room): the launch's tasks keep their guard -- none runs before its
     // predecessors; the domain reports -ENOSPC
     if (r.ctr[2] & RERR_NODES) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) r.ctr[0] += (unsigned int)*count;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        r.ctr[0] += (unsigned int)(*count + (hcount ? stripe_prefix(hcount, hstripes, hcap) : 0));
     const uint32_t t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= t1) return;
     if (atomicSub(&r.wc[t], 1u) == 1u) release(r, t);
+}
+
+// the guards of the empty writes inserted since the last call (their count on the device)
+__global__ void k_guard_virtual(Dev r, const unsigned int *nvirtual)
+{
+    if (r.ctr[2] & RERR_NODES) return;
+    const uint32_t v0 = r.ctr[4], v1 = min(*nvirtual, r.max_virtual);
+    for (uint32_t k = v0 + threadIdx.x; k < v1; k += blockDim.x)
+        if (atomicSub(&r.wc[r.max_tasks + k], 1u) == 1u) release(r, r.max_tasks + k);
+    __syncthreads();
+    if (threadIdx.x == 0 && v1 > v0) r.ctr[4] = v1;
 }
 
 // one thread per completed task: its accesses' successor lists
@@ -153,15 +231,17 @@ struct Engine {
     uint32_t *h_acc_task = nullptr;  // pinned mirrors: append-only sources of the uploads
     uint32_t *h_first = nullptr;
     uint32_t *h_done = nullptr;      // completion lists, append-only
+    uint32_t *d_done = nullptr;      // their device address
     uint32_t *h_ready = nullptr;     // the release queue (host address)
     unsigned int *h_ctr = nullptr;
     uint64_t ntasks = 0, nacc = 0, done_used = 0, read = 0;
 };
 
-int create(void *stream, uint64_t max_tasks, uint64_t max_acc, uint64_t max_nodes, Engine **out)
+int create(void *stream, uint64_t max_tasks, uint64_t max_acc, uint64_t max_nodes, uint64_t max_virtual,
+           Engine **out)
 {
     if (!out || !max_tasks || !max_acc || !max_nodes) return -EINVAL;
-    if (max_tasks >= EMPTY || max_acc >= EMPTY || max_nodes >= EMPTY || max_tasks >= DETACHED) return -ERANGE;
+    if (max_tasks + max_virtual >= DETACHED || max_acc + max_virtual >= VIRT || max_nodes >= EMPTY) return -ERANGE;
     *out = nullptr;
     Engine *e = new Engine();
     e->stream = (hipStream_t)stream;
@@ -169,14 +249,16 @@ int create(void *stream, uint64_t max_tasks, uint64_t max_acc, uint64_t max_node
     d.max_tasks = (uint32_t)max_tasks;
     d.max_acc = (uint32_t)max_acc;
     d.max_nodes = (uint32_t)max_nodes;
+    d.max_virtual = (uint32_t)max_virtual;
+    const uint64_t nt = max_tasks + max_virtual, na = max_acc + max_virtual;
     auto fail = [&](int rc) {
         destroy(e);
         return rc;
     };
     int rc = 0;
-    if ((rc = hip_rc(hipMalloc(&d.wc, 4 * max_tasks))) || (rc = hip_rc(hipMalloc(&d.state, 4 * max_tasks))) ||
+    if ((rc = hip_rc(hipMalloc(&d.wc, 4 * nt))) || (rc = hip_rc(hipMalloc(&d.state, 4 * nt))) ||
         (rc = hip_rc(hipMalloc(&d.first, 4 * (max_tasks + 1)))) ||
-        (rc = hip_rc(hipMalloc(&d.acc_task, 4 * max_acc))) || (rc = hip_rc(hipMalloc(&d.head, 4 * max_acc))) ||
+        (rc = hip_rc(hipMalloc(&d.acc_task, 4 * max_acc))) || (rc = hip_rc(hipMalloc(&d.head, 4 * na))) ||
         (rc = hip_rc(hipMalloc(&d.node, 8 * max_nodes))) || (rc = hip_rc(hipMalloc(&d.ctr, 64))))
         return fail(rc);
     if ((rc = hip_rc(hipHostMalloc(&e->h_acc_task, 4 * max_acc, hipHostMallocDefault))) ||
@@ -187,10 +269,11 @@ int create(void *stream, uint64_t max_tasks, uint64_t max_acc, uint64_t max_node
         return fail(rc);
     memset(e->h_ready, 0xFF, 4 * max_tasks);
     if ((rc = hip_rc(hipHostGetDevicePointer((void **)&d.ready, e->h_ready, 0)))) return fail(rc);
+    if ((rc = hip_rc(hipHostGetDevicePointer((void **)&e->d_done, e->h_done, 0)))) return fail(rc);
     const dim3 g(1024), b(256);
-    hipLaunchKernelGGL(k_fill, g, b, 0, e->stream, d.wc, max_tasks, 1u);
-    hipLaunchKernelGGL(k_fill, g, b, 0, e->stream, d.state, max_tasks, ST_WAIT);
-    hipLaunchKernelGGL(k_fill, g, b, 0, e->stream, d.head, max_acc, EMPTY);
+    hipLaunchKernelGGL(k_fill, g, b, 0, e->stream, d.wc, nt, 1u);
+    hipLaunchKernelGGL(k_fill, g, b, 0, e->stream, d.state, nt, ST_WAIT);
+    hipLaunchKernelGGL(k_fill, g, b, 0, e->stream, d.head, na, EMPTY);
     hipLaunchKernelGGL(k_fill, g, b, 0, e->stream, (uint32_t *)d.ctr, 16, 0u);
     if ((rc = hip_rc(hipGetLastError())) || (rc = hip_rc(hipStreamSynchronize(e->stream)))) return fail(rc);
     *out = e;
@@ -232,13 +315,22 @@ int add(Engine *e, uint64_t acc0, const uint32_t *task, const uint8_t *mode, uin
     return 0;
 }
 
-int apply(Engine *e, const uint32_t *edges, const uint64_t *count, uint32_t t0, uint32_t t1)
+int apply(Engine *e, const uint32_t *edges, const uint64_t *count, const HandleEdges *h, uint32_t t0, uint32_t t1)
 {
     if (!e || !edges || !count || t0 > t1 || t1 > e->ntasks) return -EINVAL;
-    hipLaunchKernelGGL(k_apply, dim3(512), dim3(256), 0, e->stream, e->d, (const uint2 *)edges,
-                       (const unsigned long long *)count);
+    const unsigned long long *hc = h ? (const unsigned long long *)h->count : nullptr;
+    const uint32_t hs = h ? h->stripes : 0;
+    const uint64_t hcap = h ? h->stripe_cap : 0;
+    const unsigned long long *cnt = (const unsigned long long *)count;
+    hipLaunchKernelGGL(k_apply, dim3(512), dim3(256), 0, e->stream, e->d, (const uint2 *)edges, cnt, hc, hs, hcap);
+    if (h) {
+        hipLaunchKernelGGL(k_apply_striped, dim3(8, hs), dim3(256), 0, e->stream, e->d, cnt,
+                           (const uint2 *)h->edges, hc, hs, hcap);
+        hipLaunchKernelGGL(k_guard_virtual, dim3(1), dim3(256), 0, e->stream, e->d,
+                           (const unsigned int *)h->nvirtual);
+    }
     hipLaunchKernelGGL(k_guard, dim3(t1 > t0 ? (t1 - t0 + 255) / 256 : 1), dim3(256), 0, e->stream, e->d, t0, t1,
-                       (const unsigned long long *)count);
+                       cnt, hc, hs, hcap);
     return hip_rc(hipGetLastError());
 }
 
@@ -249,10 +341,9 @@ int complete(Engine *e, const uint32_t *tasks, uint64_t n)
     if (e->done_used + n > e->d.max_tasks) return -ENOSPC;  // each task completes once
     uint32_t *src = e->h_done + e->done_used;
     memcpy(src, tasks, 4 * n);
-    e->done_used += n;
     // the kernel reads the list in place from pinned memory (never rewritten)
-    uint32_t *dsrc = nullptr;
-    RCHK(hipHostGetDevicePointer((void **)&dsrc, src, 0));
+    uint32_t *dsrc = e->d_done + e->done_used;
+    e->done_used += n;
     hipLaunchKernelGGL(k_complete, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, e->stream, e->d, dsrc, (uint32_t)n);
     return hip_rc(hipGetLastError());
 }

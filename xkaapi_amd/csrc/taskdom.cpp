@@ -51,7 +51,8 @@ struct xkrt_hip_taskdom {
     std::vector<uint32_t> e32, edom;
     // handle accesses: their domain, and the program id of each of its accesses
     xkrt_hip_handle_domain_t hdom = nullptr;
-    std::vector<uint32_t> hids;
+    std::vector<uint32_t> hids;    // release off: program id of each handle-domain access
+    std::vector<uint32_t> hbatch;  // this batch's handle accesses' program ids
     std::vector<xkrt_hip_handle_access_t> hpart;
     std::vector<uint32_t> he32;
     // release on the device
@@ -110,8 +111,15 @@ int launch(xkrt_hip_taskdom *t)
         const uint32_t *edges = nullptr, *doms = nullptr;
         const uint64_t *count = nullptr;
         rc = xkrt_hip_pool_edges_device(t->pool, &edges, &doms, &count);
-        if (!rc) rc = xkrt_release::apply(t->rel, edges, count, (uint32_t)t->open_t0, (uint32_t)t->open_t1);
+        xkrt_release::HandleEdges he{};
+        if (!rc && t->hdom)
+            rc = xkrt_hip_handle_edges_device(t->hdom, &he.edges, &he.count, &he.stripes, &he.stripe_cap,
+                                              &he.nvirtual);
+        if (!rc)
+            rc = xkrt_release::apply(t->rel, edges, count, t->hdom ? &he : nullptr, (uint32_t)t->open_t0,
+                                     (uint32_t)t->open_t1);
         if (!rc) rc = xkrt_hip_pool_edges_consume(t->pool);
+        if (!rc && t->hdom) rc = xkrt_hip_handle_edges_consume(t->hdom);
     }
     t->open_t0 = t->open_t1;
     return rc;
@@ -145,6 +153,7 @@ int xkrt_hip_taskdom_create(const xkrt_hip_taskdom_config_t *cfg, void *stream, 
         if (!c.max_tasks) c.max_tasks = 1ull << 20;
         if (!c.max_accesses) c.max_accesses = 4 * c.max_tasks;
         if (!c.max_succ) c.max_succ = 2 * c.max_accesses;
+        if (!c.max_virtual) c.max_virtual = c.max_tasks;
     }
     xkrt_hip_pool_t p = nullptr;
     int rc = xkrt_hip_pool_create(&pc, stream, &p);
@@ -153,7 +162,8 @@ int xkrt_hip_taskdom_create(const xkrt_hip_taskdom_config_t *cfg, void *stream, 
     t->pool = p;
     t->cfg = c;
     if (c.release) {
-        rc = xkrt_release::create(xkrt_hip_pool_stream(p), c.max_tasks, c.max_accesses, c.max_succ, &t->rel);
+        rc = xkrt_release::create(xkrt_hip_pool_stream(p), c.max_tasks, c.max_accesses, c.max_succ, c.max_virtual,
+                                  &t->rel);
         if (rc) {
             xkrt_hip_pool_destroy(p);
             delete t;
@@ -190,7 +200,6 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
         // released on the device, tasks are numbered 0, 1, ... in submit order
         if (t->rel && (i ? q.task - prev > 1 : q.task != t->next_task)) return -EINVAL;
         if (q.kind == XKRT_HIP_KIND_HANDLE) {
-            if (t->rel) return -ENOTSUP;  // inserted empty writes are tasks the release engine does not hold
             if (q.addr == UINT64_MAX || q.task >= 0x80000000u) return -EINVAL;
             if (q.concurrency > XKRT_HIP_CONC_CONCURRENT || q.concurrency == 3) return -EINVAL;
             ++nh;
@@ -201,7 +210,7 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
         if (q.kind == XKRT_HIP_KIND_TILE && (q.n == 0 || q.ld == 0 || q.sizeof_type == 0)) return -EINVAL;
     }
     if (nh > HANDLE_MAX_BATCH) return -EINVAL;
-    if (nh && t->hids.size() + nh > HANDLE_MAX_ACCESSES) return -ENOSPC;
+    if (nh && !t->rel && t->hids.size() + nh > HANDLE_MAX_ACCESSES) return -ENOSPC;
     if (t->rel && ((uint64_t)acc[n - 1].task + 1 > t->cfg.max_tasks || t->next_id + n > t->cfg.max_accesses))
         return -ENOSPC;
     // domains this batch creates (destroyed again if the batch is refused)
@@ -296,16 +305,26 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
     if (nh) {
         if (!t->hdom) {
             xkrt_hip_handle_config_t hc{};
-            hc.max_accesses = HANDLE_MAX_ACCESSES;
-            hc.max_virtual = HANDLE_MAX_ACCESSES;
             hc.max_batch = HANDLE_MAX_BATCH;
-            rc = xkrt_hip_handle_domain_create(&hc, nullptr, &t->hdom);
+            if (t->rel) {
+                // program ids, and its edges applied on the device: on the pool's stream, so that
+                // a launch's apply follows the handle launches of its batches
+                hc.max_accesses = t->cfg.max_accesses;
+                hc.max_virtual = t->cfg.max_virtual;
+                hc.max_edges = std::max<uint64_t>(1ull << 22, 2 * t->cfg.max_rects);
+                rc = xkrt_hip_handle_domain_create(&hc, xkrt_hip_pool_stream(t->pool), &t->hdom);
+            } else {
+                hc.max_accesses = HANDLE_MAX_ACCESSES;
+                hc.max_virtual = HANDLE_MAX_ACCESSES;
+                rc = xkrt_hip_handle_domain_create(&hc, nullptr, &t->hdom);
+            }
             if (rc) {
                 t->err = rc;  // the rest of the batch is staged already
                 return rc;
             }
         }
         t->hpart.clear();
+        t->hbatch.clear();
         for (uint64_t i = 0; i < n; ++i) {
             if (acc[i].kind != XKRT_HIP_KIND_HANDLE) continue;
             xkrt_hip_handle_access_t h{};
@@ -314,9 +333,14 @@ int xkrt_hip_taskdom_submit(xkrt_hip_taskdom_t t, const xkrt_hip_task_access_t *
             h.mode = acc[i].mode;
             h.concurrency = acc[i].concurrency ? acc[i].concurrency : (uint8_t)XKRT_HIP_CONC_SEQUENTIAL;
             t->hpart.push_back(h);
-            t->hids.push_back((uint32_t)(t->next_id + i));
+            t->hbatch.push_back((uint32_t)(t->next_id + i));
         }
-        rc = xkrt_hip_handle_submit(t->hdom, t->hpart.data(), t->hpart.size(), nullptr);
+        if (t->rel) {
+            rc = xkrt_hip_handle_submit_ids(t->hdom, t->hpart.data(), t->hbatch.data(), t->hpart.size());
+        } else {
+            t->hids.insert(t->hids.end(), t->hbatch.begin(), t->hbatch.end());
+            rc = xkrt_hip_handle_submit(t->hdom, t->hpart.data(), t->hpart.size(), nullptr);
+        }
         if (rc) {
             t->err = rc;
             return rc;

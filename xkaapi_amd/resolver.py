@@ -89,7 +89,7 @@ class TaskDomConfigT(C.Structure):
                 ("max_edges", C.c_uint64), ("budget_bytes", C.c_uint64), ("seg_col_bytes", C.c_uint64),
                 ("seg_bin_x_bytes", C.c_uint32), ("seg_bin_y_cols", C.c_uint32), ("blas_bin_x_bytes", C.c_uint32),
                 ("blas_bin_y_cols", C.c_uint32), ("max_tasks", C.c_uint64), ("max_accesses", C.c_uint64),
-                ("max_succ", C.c_uint64)]
+                ("max_succ", C.c_uint64), ("max_virtual", C.c_uint64)]
 
 
 class TaskDomStatsT(C.Structure):
@@ -203,6 +203,10 @@ def hip_lib():
     L.xkrt_hip_handle_submit.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
     L.xkrt_hip_handle_sync.argtypes = [_p, C.POINTER(_u64), C.POINTER(_u64)]
     L.xkrt_hip_handle_edges_copy.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
+    L.xkrt_hip_handle_submit_ids.argtypes = [_p, _p, _p, _u64]
+    L.xkrt_hip_handle_edges_device.argtypes = [_p, C.POINTER(_p), C.POINTER(_p), C.POINTER(_u32), C.POINTER(_u64),
+                                               C.POINTER(_p)]
+    L.xkrt_hip_handle_edges_consume.argtypes = [_p]
     L.xkrt_hip_taskdom_complete.argtypes = [_p, _p, _u64]
     L.xkrt_hip_taskdom_create.argtypes = [C.POINTER(TaskDomConfigT), _p, C.POINTER(_p)]
     L.xkrt_hip_taskdom_destroy.argtypes = [_p]
@@ -639,11 +643,12 @@ class TaskDomain:
     def __init__(self, kernel: int = 1, max_rects: int = 1 << 20, max_edges: int = 1 << 22, budget_bytes: int = 0,
                  seg_col_bytes: int = 1 << 16, seg_bin: tuple[int, int] = (0, 0), blas_bin: tuple[int, int] = (1024, 128),
                  release: bool = False, max_tasks: int = 0, max_accesses: int = 0, max_succ: int = 0,
-                 stream: int | None = None, device: int = 0):
+                 max_virtual: int = 0, stream: int | None = None, device: int = 0):
         self.L = hip_lib()
         _chk(self.L.xkrt_hip_init(device), "xkrt_hip_init")
         cfg = TaskDomConfigT(kernel, 1 if release else 0, max_rects, max_edges, budget_bytes, seg_col_bytes,
-                             seg_bin[0], seg_bin[1], blas_bin[0], blas_bin[1], max_tasks, max_accesses, max_succ)
+                             seg_bin[0], seg_bin[1], blas_bin[0], blas_bin[1], max_tasks, max_accesses, max_succ,
+                             max_virtual)
         h = _p()
         _chk(self.L.xkrt_hip_taskdom_create(C.byref(cfg), stream, C.byref(h)), "xkrt_hip_taskdom_create")
         self.h = h
