@@ -693,10 +693,12 @@ class TaskDomain:
 
     def ready(self, cap: int = 1 << 16) -> np.ndarray:
         """Released tasks not read yet (non-blocking), in release order."""
-        out = np.empty(max(cap, 1), np.uint32)
-        n = _u64()
-        _chk(self.L.xkrt_hip_taskdom_ready(self.h, out.ctypes.data, cap, C.byref(n)), "xkrt_hip_taskdom_ready")
-        return out[:int(n.value)].copy()
+        buf = getattr(self, "_rbuf", None)
+        if buf is None or len(buf) < cap:
+            buf = self._rbuf = np.empty(max(cap, 1), np.uint32)
+            self._rn = _u64()
+        _chk(self.L.xkrt_hip_taskdom_ready(self.h, buf.ctypes.data, cap, C.byref(self._rn)), "xkrt_hip_taskdom_ready")
+        return buf[:int(self._rn.value)].copy()
 
     def complete(self, tasks) -> None:
         t = np.ascontiguousarray(tasks, np.uint32)
