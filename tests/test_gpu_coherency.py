@@ -61,6 +61,15 @@ def test_random_tiny_matches_oracle(seed):
 
 
 @pytest.mark.parametrize("seed", range(3))
+def test_tall_accesses_span_row_blocks(seed):
+    # columns of fragments taller than one work item (1024 rows): pieces, and
+    # empty row blocks, on both sides of item boundaries
+    st = tiny(70 + seed, n=80, ld=12, cols=3000, maxn=3000)
+    assert int(st.n.max()) > 2048
+    assert_same_plan(K.plan_stream(st, (0, 3000)), oracle_plan(st), len(st))
+
+
+@pytest.mark.parametrize("seed", range(3))
 def test_batches_match_one_call(seed):
     st = tiny(50 + seed, n=400, ld=40, cols=30, maxm=20, maxn=12)
     want = oracle_plan(st)

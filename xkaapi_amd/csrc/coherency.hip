@@ -13,7 +13,8 @@
 //     that introduced breakpoint u (x unit = sizeof_type bytes) and y (at
 //     x unit u), so access a sees exactly the breakpoints with a stamp <= a,
 //     and the fragments of every access of a batch are enumerated in
-//     parallel (c_count, c_rows);
+//     parallel: one work item per (slab, block of rows), so a large access
+//     spreads over many waves (c_slabs, c_icount, c_irows);
 //   * allocation views: a GPU access a reuses the chunk of the earliest
 //     earlier GPU access whose rect contains a's rect (that access allocated
 //     it and appended it to every point of its rect; the reference takes the
@@ -59,6 +60,7 @@ constexpr uint32_t TP = TW * TH;
 constexpr uint32_t TTHREADS = 256;
 constexpr uint32_t PPT = TP / TTHREADS;      // points per thread
 constexpr uint32_t TLIST = 4096;             // accesses of one tile per plan call (LDS list)
+constexpr uint32_t ITEM_ROWS = 1024;          // rows of a fragment work item (at least)
 constexpr uint32_t SCAN_BLOCK = 1024, SCAN_PER = 4, SCAN_CHUNK = SCAN_BLOCK * SCAN_PER;
 
 enum : uint32_t { F_GPU = 1u, F_INS = 2u, F_R = 4u, F_W = 8u };
@@ -73,6 +75,7 @@ struct CCounters {
     unsigned int fetches;   // fetch rows this call
     unsigned int viewerr;   // lowest access id whose chunk was missing from a point (NONE = none)
     unsigned int regs;      // registration rows this call (fetches in flight)
+    unsigned int items;     // fragment work items this call (slab x row block)
 };
 
 struct CArgs {
@@ -83,7 +86,7 @@ struct CArgs {
     uint32_t *rrpos;         // exclusive scan of rreg
     uint32_t *reg;           // registration rows, 8 u32 each (x0, x1, y0, y1, kind, chunk, dst chunk, access)
     const uint32_t *bx0, *bx1, *by0, *by1, *bflags, *bhist, *bxoff;
-    uint32_t *pcount, *rowbase;
+    uint32_t *rowbase;
     // tree
     uint32_t nxu, ny, ny1, s;
     uint32_t *TX, *TY;
@@ -94,8 +97,14 @@ struct CArgs {
     const unsigned long long *hbytes;
     uint32_t *hbeta, *hchunk;
     unsigned long long *chunk_size;
-    // per-slab fragment index
-    uint32_t *xcnt, *xrow;
+    // per unit of the batch's accesses (xtot, slab starts only where noted):
+    // fragment rows of the slab (start, count), its items (all units), their
+    // offset, the slab's end unit and access
+    uint32_t *xcnt, *xrow, *xitems, *xioff, *xend, *xacc;
+    uint32_t xtot, irows;    // units; rows per item
+    // work items: slab start unit, fragment count, first row
+    uint32_t *imap, *icnt, *irow;
+    uint32_t nitems, nrows;
     // fragment rows
     uint32_t *rx0, *rx1, *ry0, *ry1, *rpay, *rsrc, *racc;
     uint32_t *rflag, *rfpos;
@@ -190,121 +199,149 @@ __global__ void c_breaks(CArgs a)
     }
 }
 
-// fragments of each access: slabs x pieces, counted per slab
-__global__ void c_count(CArgs a)
-{
-    const uint32_t lane = lane_id();
-    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t i = wid; i < a.n; i += nw) {
-        uint32_t total = 0;
-        if (a.bflags[i] & F_INS) {
-            const uint32_t aid = a.gid0 + i;
-            const uint32_t x0 = a.bx0[i], x1 = a.bx1[i], y0 = a.by0[i], y1 = a.by1[i], xo = a.bxoff[i];
-            for (uint32_t uc = x0; uc < x1; uc += 64) {
-                const uint32_t u = uc + lane;
-                uint64_t m = __ballot(u < x1 && (u == x0 || a.TX[u] <= aid));
-                while (m) {
-                    const uint32_t us = uc + (uint32_t)__builtin_ctzll(m);
-                    m &= m - 1;
-                    const uint32_t *row = a.TY + (size_t)us * a.ny1;
-                    uint32_t np = 0;
-                    // four 64-row chunks per step: their loads are in flight together
-                    for (uint32_t yc = y0; yc < y1; yc += 256) {
-                        uint32_t v[4];
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const uint32_t y = yc + 64 * k + lane;
-                            v[k] = (y < y1 && y != y0) ? row[y] : 0u;
-                        }
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const uint32_t y = yc + 64 * k + lane;
-                            np += (uint32_t)__popcll(__ballot(y < y1 && (y == y0 || v[k] <= aid)));
-                        }
-                    }
-                    if (lane == 0) a.xcnt[xo + (us - x0)] = np;
-                    total += np;
-                }
-            }
-        }
-        if (lane == 0) a.pcount[i] = total;
-    }
-}
-
-// the fragment rows of one slab [us, ue) of access i, from row index r
-__device__ void slab_rows(const CArgs &a, uint32_t i, uint32_t aid, uint32_t us, uint32_t ue, uint32_t y0,
-                          uint32_t y1, uint32_t r)
-{
-    const uint32_t lane = lane_id();
-    const uint32_t *row = a.TY + (size_t)us * a.ny1;
-    uint32_t pend_y = NONE, pend_r = 0;
-    uint32_t v[4];  // four 64-row chunks loaded ahead, so their loads are in flight together
-    for (uint32_t yc = y0; yc < y1; yc += 64) {
-        const uint32_t k = ((yc - y0) >> 6) & 3u;
-        if (k == 0) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t yq = yc + 64 * q + lane;
-                v[q] = (yq < y1 && yq != y0) ? row[yq] : 0u;
-            }
-        }
-        const uint32_t y = yc + lane;
-        const uint32_t vk = k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
-        const bool ps = y < y1 && (y == y0 || vk <= aid);
-        const uint64_t m = __ballot(ps);
-        if (!m) continue;
-        if (pend_y != NONE && lane == 0) {
-            a.rx0[pend_r] = us * a.s; a.rx1[pend_r] = ue * a.s;
-            a.ry0[pend_r] = pend_y; a.ry1[pend_r] = yc + (uint32_t)__builtin_ctzll(m);
-            a.racc[pend_r] = i;
-        }
-        const uint64_t above = lane == 63 ? 0ull : (m & (~0ull << (lane + 1)));
-        const uint32_t idx = r + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (ps && above) {
-            a.rx0[idx] = us * a.s; a.rx1[idx] = ue * a.s;
-            a.ry0[idx] = y; a.ry1[idx] = yc + (uint32_t)__builtin_ctzll(above);
-            a.racc[idx] = i;
-        }
-        const uint32_t last = 63u - (uint32_t)__builtin_clzll(m);
-        r += (uint32_t)__popcll(m);
-        pend_y = yc + last;
-        pend_r = r - 1;
-    }
-    if (pend_y != NONE && lane == 0) {
-        a.rx0[pend_r] = us * a.s; a.rx1[pend_r] = ue * a.s;
-        a.ry0[pend_r] = pend_y; a.ry1[pend_r] = y1;
-        a.racc[pend_r] = i;
-    }
-}
-
-__global__ void c_rows(CArgs a)
+// slab starts of each access (the x units where a column of its fragments
+// begins: x0 and every dim-0 breakpoint seen by the access) with their ends;
+// per unit of the access, the row blocks of a.irows rows to enumerate there
+// (0 off a slab start)
+__global__ void c_slabs(CArgs a)
 {
     const uint32_t lane = lane_id();
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t i = wid; i < a.n; i += nw) {
         if (!(a.bflags[i] & F_INS)) continue;
         const uint32_t aid = a.gid0 + i;
-        const uint32_t x0 = a.bx0[i], x1 = a.bx1[i], y0 = a.by0[i], y1 = a.by1[i], xo = a.bxoff[i];
-        uint32_t r = a.rowbase[i];
-        uint32_t pend = NONE;
+        const uint32_t x0 = a.bx0[i], x1 = a.bx1[i], xo = a.bxoff[i];
+        const uint32_t nyb = (a.by1[i] - a.by0[i] + a.irows - 1) / a.irows;
+        uint32_t pend = NONE;  // the last slab start seen, its end not known yet
         for (uint32_t uc = x0; uc < x1; uc += 64) {
             const uint32_t u = uc + lane;
-            uint64_t m = __ballot(u < x1 && (u == x0 || a.TX[u] <= aid));
-            while (m) {
-                const uint32_t us = uc + (uint32_t)__builtin_ctzll(m);
-                m &= m - 1;
-                if (pend != NONE) {
-                    if (lane == 0) a.xrow[xo + (pend - x0)] = r;
-                    slab_rows(a, i, aid, pend, us, y0, y1, r);
-                    r += a.xcnt[xo + (pend - x0)];
-                }
-                pend = us;
+            const bool st = u < x1 && (u == x0 || a.TX[u] <= aid);
+            const uint64_t m = __ballot(st);
+            if (u < x1) a.xitems[xo + (u - x0)] = st ? nyb : 0u;
+            if (!m) continue;
+            if (pend != NONE && lane == 0) a.xend[xo + (pend - x0)] = uc + (uint32_t)__builtin_ctzll(m);
+            const uint64_t above = lane == 63 ? 0ull : (m & (~0ull << (lane + 1)));
+            if (st) {
+                a.xacc[xo + (u - x0)] = i;
+                if (above) a.xend[xo + (u - x0)] = uc + (uint32_t)__builtin_ctzll(above);
+            }
+            pend = uc + 63u - (uint32_t)__builtin_clzll(m);
+        }
+        if (pend != NONE && lane == 0) a.xend[xo + (pend - x0)] = x1;
+    }
+}
+
+// work items: item k is row block k - xioff[g] of the slab starting at unit g
+__global__ void c_imap(CArgs a)
+{
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < a.xtot; g += gridDim.x * blockDim.x) {
+        const uint32_t c = a.xitems[g];
+        if (!c) continue;
+        const uint32_t b = a.xioff[g];
+        for (uint32_t j = 0; j < c; ++j) a.imap[b + j] = g;
+    }
+}
+
+struct Item {
+    uint32_t g, i, aid, us, y0, ya, yb;
+};
+
+__device__ __forceinline__ Item item_of(const CArgs &a, uint32_t k)
+{
+    Item t;
+    t.g = a.imap[k];
+    const uint32_t j = k - a.xioff[t.g];
+    t.i = a.xacc[t.g];
+    t.aid = a.gid0 + t.i;
+    t.us = a.bx0[t.i] + (t.g - a.bxoff[t.i]);
+    t.y0 = a.by0[t.i];
+    t.ya = t.y0 + j * a.irows;
+    t.yb = min(a.by1[t.i], t.ya + a.irows);
+    return t;
+}
+
+// fragments of each item: the rows y of its block where a fragment starts
+// (y0, or a dim-1 breakpoint of the slab seen by the access)
+__global__ void c_icount(CArgs a)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t k = wid; k < a.nitems; k += nw) {
+        const Item t = item_of(a, k);
+        const uint32_t *row = a.TY + (size_t)t.us * a.ny1;
+        uint32_t np = 0;
+        // four 64-row chunks per step: their loads are in flight together
+        for (uint32_t yc = t.ya; yc < t.yb; yc += 256) {
+            uint32_t v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t y = yc + 64 * q + lane;
+                v[q] = (y < t.yb && y != t.y0) ? row[y] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t y = yc + 64 * q + lane;
+                np += (uint32_t)__popcll(__ballot(y < t.yb && (y == t.y0 || v[q] <= t.aid)));
             }
         }
-        if (pend != NONE) {
-            if (lane == 0) a.xrow[xo + (pend - x0)] = r;
-            slab_rows(a, i, aid, pend, x1, y0, y1, r);
+        if (lane == 0) a.icnt[k] = np;
+    }
+}
+
+// fragment rows of each item from row irow[k] (their dim-1 ends: c_yfix); a
+// slab's first item records the slab's rows; the row base of every access
+__global__ void c_irows(CArgs a)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t k = wid; k < a.nitems; k += nw) {
+        const Item t = item_of(a, k);
+        const uint32_t ue = a.xend[t.g];
+        const uint32_t *row = a.TY + (size_t)t.us * a.ny1;
+        uint32_t r = a.irow[k];
+        if (t.ya == t.y0 && lane == 0) {
+            const uint32_t e = k + a.xitems[t.g];
+            a.xrow[t.g] = r;
+            a.xcnt[t.g] = (e < a.nitems ? a.irow[e] : a.nrows) - r;
         }
+        for (uint32_t yc = t.ya; yc < t.yb; yc += 256) {
+            uint32_t v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t y = yc + 64 * q + lane;
+                v[q] = (y < t.yb && y != t.y0) ? row[y] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t y = yc + 64 * q + lane;
+                const bool ps = y < t.yb && (y == t.y0 || v[q] <= t.aid);
+                const uint64_t m = __ballot(ps);
+                if (ps) {
+                    const uint32_t idx = r + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                    a.rx0[idx] = t.us * a.s;
+                    a.rx1[idx] = ue * a.s;
+                    a.ry0[idx] = y;
+                    a.racc[idx] = t.i;
+                }
+                r += (uint32_t)__popcll(m);
+            }
+        }
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        const uint32_t xo = a.bxoff[i];
+        const uint32_t it = xo < a.xtot ? a.xioff[xo] : a.nitems;  // items of the accesses before
+        a.rowbase[i] = it < a.nitems ? a.irow[it] : a.nrows;
+    }
+}
+
+// dim-1 end of each row: the next row's start in the same slab, else the
+// access's y1 (the rows of a slab are consecutive, by ascending y)
+__global__ void c_yfix(CArgs a)
+{
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < a.nrows; r += gridDim.x * blockDim.x) {
+        const uint32_t i = a.racc[r];
+        const bool same = r + 1 < a.nrows && a.racc[r + 1] == i && a.rx0[r + 1] == a.rx0[r];
+        a.ry1[r] = same ? a.ry0[r + 1] : a.by1[i];
     }
 }
 
@@ -896,10 +933,13 @@ struct xkrt_hip_mcc {
     uint64_t chunk_cap = 0;
     // batch
     uint32_t *bx0 = nullptr, *bx1 = nullptr, *by0 = nullptr, *by1 = nullptr, *bflags = nullptr, *bhist = nullptr,
-             *bxoff = nullptr, *pcount = nullptr, *rowbase = nullptr, *foff = nullptr;
-    uint64_t bcap[10] = {};
-    uint32_t *xcnt = nullptr, *xrow = nullptr;
-    uint64_t xcap[2] = {};
+             *bxoff = nullptr, *rowbase = nullptr, *foff = nullptr;
+    uint64_t bcap[9] = {};
+    uint32_t *xcnt = nullptr, *xrow = nullptr, *xitems = nullptr, *xioff = nullptr, *xend = nullptr, *xacc = nullptr;
+    uint64_t xcap[6] = {};
+    uint32_t *imap = nullptr, *icnt = nullptr, *irow = nullptr;
+    uint64_t icap[3] = {};
+    uint32_t b_xtot = 0, b_irows = 1, b_items = 0, b_rows = 0;  // the plan call in progress
     uint32_t *tinc = nullptr;
     uint64_t tinc_cap = 0;
     uint32_t *bsum = nullptr;
@@ -963,12 +1003,15 @@ CArgs make_cargs(xkrt_hip_mcc *h, uint32_t n)
     a.n = n;
     a.gid0 = (uint32_t)h->naccess;
     a.bx0 = h->bx0; a.bx1 = h->bx1; a.by0 = h->by0; a.by1 = h->by1; a.bflags = h->bflags; a.bhist = h->bhist;
-    a.bxoff = h->bxoff; a.pcount = h->pcount; a.rowbase = h->rowbase;
+    a.bxoff = h->bxoff; a.rowbase = h->rowbase;
     a.nxu = h->nxu; a.ny = h->ny; a.ny1 = h->ny + 1; a.s = (uint32_t)h->s;
     a.TX = h->TX; a.TY = h->TY; a.ntx = h->ntx; a.nty = h->nty; a.pst = h->pst; a.pviews = h->pviews;
     a.hx0 = h->hx0; a.hx1 = h->hx1; a.hy0 = h->hy0; a.hy1 = h->hy1; a.hbytes = h->hbytes;
     a.hbeta = h->hbeta; a.hchunk = h->hchunk; a.chunk_size = h->chunk_size;
-    a.xcnt = h->xcnt; a.xrow = h->xrow;
+    a.xcnt = h->xcnt; a.xrow = h->xrow; a.xitems = h->xitems; a.xioff = h->xioff; a.xend = h->xend;
+    a.xacc = h->xacc;
+    a.xtot = h->b_xtot; a.irows = h->b_irows; a.nitems = h->b_items; a.nrows = h->b_rows;
+    a.imap = h->imap; a.icnt = h->icnt; a.irow = h->irow;
     a.rx0 = h->rx0; a.rx1 = h->rx1; a.ry0 = h->ry0; a.ry1 = h->ry1; a.rpay = h->rpay; a.rsrc = h->rsrc;
     a.racc = h->racc; a.rflag = h->rflag; a.rfpos = h->rfpos;
     a.fx0 = h->fx0; a.fx1 = h->fx1; a.fy0 = h->fy0; a.fy1 = h->fy1; a.fsd = h->fsd; a.fsc = h->fsc;
@@ -984,8 +1027,8 @@ void free_all(xkrt_hip_mcc *h)
 {
     void *ptrs[] = {h->TX, h->TY, h->pst, h->pviews, h->tcnt, h->tcur, h->tstart, h->tact, h->tbsum, h->ctr,
                     h->hx0, h->hx1, h->hy0, h->hy1, h->hbeta, h->hchunk, h->hbytes, h->chunk_size,
-                    h->bx0, h->bx1, h->by0, h->by1, h->bflags, h->bhist, h->bxoff, h->pcount, h->rowbase, h->foff,
-                    h->xcnt, h->xrow, h->tinc, h->bsum, h->rx0, h->rx1, h->ry0, h->ry1, h->rpay, h->rsrc, h->racc,
+                    h->bx0, h->bx1, h->by0, h->by1, h->bflags, h->bhist, h->bxoff, h->rowbase, h->foff,
+                    h->xcnt, h->xrow, h->xitems, h->xioff, h->xend, h->xacc, h->imap, h->icnt, h->irow, h->tinc, h->bsum, h->rx0, h->rx1, h->ry0, h->ry1, h->rpay, h->rsrc, h->racc,
                     h->rflag, h->rfpos, h->fx0, h->fx1, h->fy0, h->fy1, h->fsd, h->fsc, h->fdd, h->fdc,
                     h->rreg, h->rrpos, h->reg, h->cq};
     for (void *p : ptrs)
@@ -1176,14 +1219,24 @@ int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64
         }
     }
     if (xtot >= NONE || ttot >= NONE) return -ERANGE;
+    // rows per work item: ITEM_ROWS, doubled while the items could overflow
+    // their u32 count (one item per slab at most 1 << 24 rows: xtot items)
+    uint32_t irows = ITEM_ROWS;
+    for (;;) {
+        uint64_t bound = 0;
+        for (uint64_t i = 0; i < n; ++i)
+            if (fl[i] & F_INS) bound += (uint64_t)(x1[i] - x0[i]) * ((y1[i] - y0[i] + irows - 1) / irows);
+        if (bound < NONE || irows >= (1u << 24)) break;
+        irows <<= 1;
+    }
     const uint32_t ng = (uint32_t)gx0.size();
     hipStream_t s = h->stream;
     int rc = 0;
     // capacities
     uint64_t hneed = (uint64_t)h->nhist + ng;
-    uint32_t **bufs[] = {&h->bx0, &h->bx1, &h->by0, &h->by1, &h->bflags, &h->bhist, &h->bxoff, &h->pcount,
-                         &h->rowbase, &h->foff};
-    for (int k = 0; k < 10 && !rc; ++k) rc = grow(bufs[k], &h->bcap[k], n, false, s);
+    uint32_t **bufs[] = {&h->bx0, &h->bx1, &h->by0, &h->by1, &h->bflags, &h->bhist, &h->bxoff, &h->rowbase,
+                         &h->foff};
+    for (int k = 0; k < 9 && !rc; ++k) rc = grow(bufs[k], &h->bcap[k], n, false, s);
     if (!rc) rc = grow(&h->hx0, &h->hcap, hneed, true, s);
     if (!rc) rc = grow(&h->hx1, &h->hcap1, hneed, true, s);
     if (!rc) rc = grow(&h->hy0, &h->hcap2, hneed, true, s);
@@ -1192,8 +1245,8 @@ int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64
     if (!rc) rc = grow(&h->hchunk, &h->hcap5, hneed, true, s);
     if (!rc) rc = grow(&h->hbytes, &h->hcap6, hneed, true, s);
     if (!rc) rc = grow(&h->chunk_size, &h->chunk_cap, hneed, true, s);
-    if (!rc) rc = grow(&h->xcnt, &h->xcap[0], std::max<uint64_t>(xtot, 1), false, s);
-    if (!rc) rc = grow(&h->xrow, &h->xcap[1], std::max<uint64_t>(xtot, 1), false, s);
+    uint32_t **xb[] = {&h->xcnt, &h->xrow, &h->xitems, &h->xioff, &h->xend, &h->xacc};
+    for (int k = 0; k < 6 && !rc; ++k) rc = grow(xb[k], &h->xcap[k], std::max<uint64_t>(xtot, 1), false, s);
     if (!rc) rc = grow(&h->tinc, &h->tinc_cap, std::max<uint64_t>(ttot, 1), false, s);
     if (!rc) rc = grow(&h->tbsum, &h->tbsum_cap, 1, false, s);
     if (rc) return rc;
@@ -1216,13 +1269,32 @@ int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64
         hipLaunchKernelGGL(c_contain, dim3(waves_grid(ng)), dim3(256), 0, s, a, h->nhist, ng);
         hipLaunchKernelGGL(c_chunks, dim3(1), dim3(1024), 0, s, a, h->nhist, ng);
     }
+    h->b_xtot = (uint32_t)xtot;
+    h->b_irows = irows;
+    h->b_items = h->b_rows = 0;
+    a = make_cargs(h, (uint32_t)n);
     hipLaunchKernelGGL(c_breaks, dim3(gw), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(c_count, dim3(gw), dim3(256), 0, s, a);
-    rc = scan(h, h->pcount, (uint32_t)n, h->rowbase, &h->ctr->rows);
+    // fragments: slab starts, then work items of (slab, row block), counted
+    // and scanned into row offsets
+    hipLaunchKernelGGL(c_slabs, dim3(gw), dim3(256), 0, s, a);
+    rc = scan(h, h->xitems, (uint32_t)xtot, h->xioff, &h->ctr->items);
+    if (rc) return rc;
+    CCHK(hipMemcpyAsync(h->ctr_host, h->ctr, sizeof(CCounters), hipMemcpyDeviceToHost, s));
+    CCHK(hipStreamSynchronize(s));
+    const uint32_t nitems = xtot ? h->ctr_host->items : 0;
+    uint32_t **ib[] = {&h->imap, &h->icnt, &h->irow};
+    for (int k = 0; k < 3 && !rc; ++k) rc = grow(ib[k], &h->icap[k], std::max<uint32_t>(nitems, 1), false, s);
+    if (rc) return rc;
+    h->b_items = nitems;
+    a = make_cargs(h, (uint32_t)n);
+    if (xtot) hipLaunchKernelGGL(c_imap, dim3(std::min<uint64_t>((xtot + 255) / 256, 8192)), dim3(256), 0, s, a);
+    if (nitems) hipLaunchKernelGGL(c_icount, dim3(waves_grid(nitems)), dim3(256), 0, s, a);
+    rc = scan(h, h->icnt, nitems, h->irow, &h->ctr->rows);
     if (rc) return rc;
     CCHK(hipMemcpyAsync(h->ctr_host, h->ctr, sizeof(CCounters), hipMemcpyDeviceToHost, s));
     CCHK(hipStreamSynchronize(s));
     const uint32_t nrows = h->ctr_host->rows;
+    h->b_rows = nrows;
     uint32_t **rb[] = {&h->rx0, &h->rx1, &h->ry0, &h->ry1, &h->rpay, &h->rsrc, &h->racc, &h->rflag, &h->rfpos};
     for (int k = 0; k < 9 && !rc; ++k) rc = grow(rb[k], &h->rcap[k], std::max<uint32_t>(nrows, 1), false, s);
     if (rc) return rc;
@@ -1234,7 +1306,8 @@ int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64
     }
     a = make_cargs(h, (uint32_t)n);
     if (nrows) CCHK(hipMemsetAsync(h->rflag, 0, 4ull * nrows, s));
-    hipLaunchKernelGGL(c_rows, dim3(gw), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(c_irows, dim3(std::max(waves_grid(nitems), gw)), dim3(256), 0, s, a);
+    if (nrows) hipLaunchKernelGGL(c_yfix, dim3(std::min<uint32_t>((nrows + 255) / 256, 8192)), dim3(256), 0, s, a);
     // tiles
     const uint32_t ntiles = h->ntx * h->nty;
     hipLaunchKernelGGL(c_tcount, dim3(gw), dim3(256), 0, s, a);
