@@ -400,12 +400,59 @@ __global__ void c_tact(CArgs a, uint32_t ntiles)
 // Per-point state word: bits 0-1 coherency (1 host, 2 GPU), 2-9 coherent
 // views, 10-13 number of views, 14-15 fetching devices, 16-23 fetching views.
 constexpr uint32_t ST_DF = 14, ST_VF = 16;
+constexpr uint32_t DMAX = 256;  // access descriptors of a tile held in LDS at a time
+constexpr uint32_t LCAP = 512;  // fragment-row lookups of a tile deferred to its end
+constexpr uint32_t F_ALLOC = 1u << 31;  // descriptor flag: the access allocates its chunk
+
+// an access as c_state reads it (LDS)
+struct TDesc {
+    uint32_t x0, x1, y0, y1, fl, chunk, i, xo;
+};
+
+__device__ void load_descs(const CArgs &a, const uint32_t *L, TDesc *D, uint32_t q0, uint32_t m)
+{
+    for (uint32_t q = threadIdx.x; q < m; q += TTHREADS) {
+        const uint32_t i = L[q0 + q];
+        TDesc d;
+        d.x0 = a.bx0[i]; d.x1 = a.bx1[i]; d.y0 = a.by0[i]; d.y1 = a.by1[i];
+        d.fl = a.bflags[i];
+        d.chunk = NONE;
+        if (d.fl & F_GPU) {
+            const uint32_t hh = a.bhist[i];
+            d.chunk = a.hchunk[hh];
+            if (a.hbeta[hh] == NONE) d.fl |= F_ALLOC;
+        }
+        d.i = i;
+        d.xo = a.bxoff[i];
+        D[q] = d;
+    }
+}
+
+// the fragment row starting at (u, y) of the slab at unit xo: its payload
+__device__ void fill_row(const CArgs &a, uint32_t xo, uint32_t y, uint32_t pay, uint32_t schunk)
+{
+    uint32_t lo = a.xrow[xo], hi = lo + a.xcnt[xo];
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.ry0[mid] <= y) lo = mid; else hi = mid;
+    }
+    a.rpay[lo] = pay;
+    a.rsrc[lo] = schunk;
+    a.rflag[lo] = (pay >> 8) & 1u;
+    if (a.async) a.rreg[lo] = ((pay >> 12) & 3u) ? 1u : 0u;
+}
 
 // Replay the tile's accesses in order on its 32 x 32 points (4 per thread,
-// views in registers through constant-index selects).
+// views in registers through constant-index selects).  Only the points some
+// access of the tile covers are read and written (their state, the views
+// they hold, the breakpoint stamps at the point); the fragment rows whose
+// first point is in the tile are looked up once the replay is done.
 __global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
 {
     __shared__ uint32_t L[TLIST];
+    __shared__ TDesc D[DMAX];
+    __shared__ uint4 LK[LCAP];  // deferred lookups: slab unit, y, payload, source chunk
+    __shared__ uint32_t nlk;
     const uint32_t nact = a.ctr->nact;
     for (uint32_t b = blockIdx.x; b < nact; b += gridDim.x) {
         const uint32_t tile = a.tact[b];
@@ -418,6 +465,7 @@ __global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
         uint32_t n2 = 1;
         while (n2 < cnt) n2 <<= 1;
         for (uint32_t q = threadIdx.x; q < n2; q += TTHREADS) L[q] = q < cnt ? a.tinc[start + q] : NONE;
+        if (threadIdx.x == 0) nlk = 0;
         __syncthreads();
         for (uint32_t k = 2; k <= n2; k <<= 1)
             for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -430,135 +478,166 @@ __global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
                 }
                 __syncthreads();
             }
+        const uint32_t nch = (cnt + DMAX - 1) / DMAX;
+        load_descs(a, L, D, 0, min(cnt, DMAX));
+        __syncthreads();
 
         const uint32_t tx0 = (tile % a.ntx) * TW, ty0 = (tile / a.ntx) * TH;
-        uint32_t st[PPT], vw[PPT][VMAX], px[PPT], py[PPT];
-        bool valid[PPT];
+        uint32_t st[PPT], vw[PPT][VMAX], px[PPT], py[PPT], tx[PPT], ty[PPT], nv0[PPT];
+        bool cov[PPT];
 #pragma unroll
         for (uint32_t k = 0; k < PPT; ++k) {
             const uint32_t p = threadIdx.x + TTHREADS * k;
             px[k] = tx0 + p % TW;
             py[k] = ty0 + p / TW;
-            valid[k] = px[k] < a.nxu && py[k] < a.ny;
-            st[k] = a.pst[(size_t)tile * TP + p];
-#pragma unroll
-            for (uint32_t v = 0; v < VMAX; ++v) vw[k][v] = a.pviews[((size_t)tile * VMAX + v) * TP + p];
-        }
-
-        for (uint32_t q = 0; q < cnt; ++q) {
-            const uint32_t i = L[q];
-            const uint32_t aid = a.gid0 + i;
-            const uint32_t x0 = a.bx0[i], x1 = a.bx1[i], y0 = a.by0[i], y1 = a.by1[i], fl = a.bflags[i];
-            const uint32_t chunk = (fl & F_GPU) ? a.hchunk[a.bhist[i]] : NONE;
-            const bool alloc = (fl & F_GPU) && a.hbeta[a.bhist[i]] == NONE;
-#pragma unroll
-            for (uint32_t k = 0; k < PPT; ++k) {
-                if (!valid[k] || px[k] < x0 || px[k] >= x1 || py[k] < y0 || py[k] >= y1) continue;
-                uint32_t coh = st[k] & 3u, vc = (st[k] >> 2) & 0xffu, nv = (st[k] >> 10) & 0xfu;
-                uint32_t df = (st[k] >> ST_DF) & 3u, vf = (st[k] >> ST_VF) & 0xffu;
-                const uint32_t before = coh | (df << 2);
-                bool need = false;
-                uint32_t act = 0;  // 2: awaits a fetch in flight, 3: registers a forward
-                uint32_t sdev = 0, schunk = NONE;
-                if (fl & F_GPU) {
-                    uint32_t v = NONE;
-                    if (alloc) {
-                        if (nv >= VMAX) {
-                            atomicMin(&a.ctr->fail, aid);
-                            continue;
-                        }
-#pragma unroll
-                        for (uint32_t t = 0; t < VMAX; ++t)
-                            if (t == nv) vw[k][t] = chunk;
-                        v = nv++;
-                    } else {
-#pragma unroll
-                        for (uint32_t t = 0; t < VMAX; ++t)
-                            if (t < nv && vw[k][t] == chunk && v == NONE) v = t;
-                        if (v == NONE) {  // only after an abort has left this point's views short
-                            atomicMin(&a.ctr->viewerr, aid);
-                            continue;
-                        }
-                    }
-                    const uint32_t bit = 1u << v;
-                    if ((fl & F_R) && !(vc & bit)) {
-                        // fetch_access_setup_copies (1471-1631)
-                        if (vf & bit) {
-                            act = 2;  // this view is being fetched: await it (1516-1527)
-                            schunk = chunk;
-                        } else {
-                            if (coh & 2u) {  // from the first coherent view of the GPU
-                                need = true;
-                                const uint32_t sv = (uint32_t)__builtin_ctz(vc);
-                                sdev = 1;
-#pragma unroll
-                                for (uint32_t t = 0; t < VMAX; ++t)
-                                    if (t == sv) schunk = vw[k][t];
-                            } else if (df & 2u) {  // forward from the first view being fetched (1567-1600)
-                                act = 3;
-                                const uint32_t sv = (uint32_t)__builtin_ctz(vf);
-#pragma unroll
-                                for (uint32_t t = 0; t < VMAX; ++t)
-                                    if (t == sv) schunk = vw[k][t];
-                            } else {  // from host memory
-                                need = true;
-                            }
-                            vf |= bit;
-                            df |= 2u;
-                        }
-                    }
-                    if (fl & F_W) {  // fetch_access_set_coherent (1633-1667)
-                        coh = 2u;
-                        vc = bit;
-                    }
-                    if (need && !a.async) {  // synchronous completion (2067-2127)
-                        vc |= bit;
-                        coh |= 2u;
-                        vf &= ~bit;
-                        if (!vf) df &= ~2u;
-                    }
-                } else if (coh != 0u && !(coh & 1u) && !(df & 1u)) {
-                    // fetch_list_to_host_setup_partition (1110-1181): from the GPU view of the largest chunk
-                    need = true;
-                    sdev = 1;
-                    unsigned long long best = 0;
-#pragma unroll
-                    for (uint32_t t = 0; t < VMAX; ++t) {
-                        if (t < nv) {
-                            const unsigned long long sz = a.chunk_size[vw[k][t]];
-                            if (t == 0 || best < sz) {
-                                best = sz;
-                                schunk = vw[k][t];
-                            }
-                        }
-                    }
-                    if (a.async) df |= 1u;  // fetching to the host (1126-1130)
-                    else coh |= 1u;          // synchronous completion
-                }
-                st[k] = coh | (vc << 2) | (nv << 10) | (df << ST_DF) | (vf << ST_VF);
-                // the first point of one of the access's fragments: fill that row
-                const uint32_t u = px[k], y = py[k];
-                if ((u == x0 || a.TX[u] <= aid) && (y == y0 || a.TY[(size_t)u * a.ny1 + y] <= aid)) {
-                    const uint32_t xo = a.bxoff[i] + (u - x0);
-                    uint32_t lo = a.xrow[xo], hi = lo + a.xcnt[xo];
-                    while (hi - lo > 1) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (a.ry0[mid] <= y) lo = mid; else hi = mid;
-                    }
-                    a.rpay[lo] = before | ((need ? 1u : 0u) << 8) | (sdev << 9) | (act << 12);
-                    a.rsrc[lo] = schunk;
-                    a.rflag[lo] = need ? 1u : 0u;
-                    if (a.async) a.rreg[lo] = act ? 1u : 0u;
-                }
+            cov[k] = px[k] < a.nxu && py[k] < a.ny;
+            if (nch == 1 && cov[k]) {  // covered by an access of the tile (all valid points past one chunk)
+                bool in = false;
+                for (uint32_t q = 0; q < cnt && !in; ++q)
+                    in = D[q].x0 <= px[k] && px[k] < D[q].x1 && D[q].y0 <= py[k] && py[k] < D[q].y1;
+                cov[k] = in;
             }
         }
-
 #pragma unroll
         for (uint32_t k = 0; k < PPT; ++k) {
             const uint32_t p = threadIdx.x + TTHREADS * k;
-            a.pst[(size_t)tile * TP + p] = st[k];
+            st[k] = 0u; nv0[k] = 0u; tx[k] = NONE; ty[k] = NONE;
 #pragma unroll
-            for (uint32_t v = 0; v < VMAX; ++v) a.pviews[((size_t)tile * VMAX + v) * TP + p] = vw[k][v];
+            for (uint32_t v = 0; v < VMAX; ++v) vw[k][v] = NONE;
+            if (cov[k]) {
+                st[k] = a.pst[(size_t)tile * TP + p];
+                tx[k] = a.TX[px[k]];
+                ty[k] = a.TY[(size_t)px[k] * a.ny1 + py[k]];
+                nv0[k] = (st[k] >> 10) & 0xfu;
+#pragma unroll
+                for (uint32_t v = 0; v < VMAX; ++v)
+                    if (v < nv0[k]) vw[k][v] = a.pviews[((size_t)tile * VMAX + v) * TP + p];
+            }
+        }
+
+        for (uint32_t c = 0; c < nch; ++c) {
+            if (c > 0) {
+                __syncthreads();
+                load_descs(a, L, D, c * DMAX, min(cnt - c * DMAX, DMAX));
+                __syncthreads();
+            }
+            const uint32_t m = min(cnt - c * DMAX, DMAX);
+            for (uint32_t q = 0; q < m; ++q) {
+                const TDesc d = D[q];
+                const uint32_t aid = a.gid0 + d.i, fl = d.fl, chunk = d.chunk;
+                const bool alloc = (fl & F_ALLOC) != 0u;
+#pragma unroll
+                for (uint32_t k = 0; k < PPT; ++k) {
+                    if (!cov[k] || px[k] < d.x0 || px[k] >= d.x1 || py[k] < d.y0 || py[k] >= d.y1) continue;
+                    uint32_t coh = st[k] & 3u, vc = (st[k] >> 2) & 0xffu, nv = (st[k] >> 10) & 0xfu;
+                    uint32_t df = (st[k] >> ST_DF) & 3u, vf = (st[k] >> ST_VF) & 0xffu;
+                    const uint32_t before = coh | (df << 2);
+                    bool need = false;
+                    uint32_t act = 0;  // 2: awaits a fetch in flight, 3: registers a forward
+                    uint32_t sdev = 0, schunk = NONE;
+                    if (fl & F_GPU) {
+                        uint32_t v = NONE;
+                        if (alloc) {
+                            if (nv >= VMAX) {
+                                atomicMin(&a.ctr->fail, aid);
+                                continue;
+                            }
+#pragma unroll
+                            for (uint32_t t = 0; t < VMAX; ++t)
+                                if (t == nv) vw[k][t] = chunk;
+                            v = nv++;
+                        } else {
+#pragma unroll
+                            for (uint32_t t = 0; t < VMAX; ++t)
+                                if (t < nv && vw[k][t] == chunk && v == NONE) v = t;
+                            if (v == NONE) {  // only after an abort has left this point's views short
+                                atomicMin(&a.ctr->viewerr, aid);
+                                continue;
+                            }
+                        }
+                        const uint32_t bit = 1u << v;
+                        if ((fl & F_R) && !(vc & bit)) {
+                            // fetch_access_setup_copies (1471-1631)
+                            if (vf & bit) {
+                                act = 2;  // this view is being fetched: await it (1516-1527)
+                                schunk = chunk;
+                            } else {
+                                if (coh & 2u) {  // from the first coherent view of the GPU
+                                    need = true;
+                                    const uint32_t sv = (uint32_t)__builtin_ctz(vc);
+                                    sdev = 1;
+#pragma unroll
+                                    for (uint32_t t = 0; t < VMAX; ++t)
+                                        if (t == sv) schunk = vw[k][t];
+                                } else if (df & 2u) {  // forward from the first view being fetched (1567-1600)
+                                    act = 3;
+                                    const uint32_t sv = (uint32_t)__builtin_ctz(vf);
+#pragma unroll
+                                    for (uint32_t t = 0; t < VMAX; ++t)
+                                        if (t == sv) schunk = vw[k][t];
+                                } else {  // from host memory
+                                    need = true;
+                                }
+                                vf |= bit;
+                                df |= 2u;
+                            }
+                        }
+                        if (fl & F_W) {  // fetch_access_set_coherent (1633-1667)
+                            coh = 2u;
+                            vc = bit;
+                        }
+                        if (need && !a.async) {  // synchronous completion (2067-2127)
+                            vc |= bit;
+                            coh |= 2u;
+                            vf &= ~bit;
+                            if (!vf) df &= ~2u;
+                        }
+                    } else if (coh != 0u && !(coh & 1u) && !(df & 1u)) {
+                        // fetch_list_to_host_setup_partition (1110-1181): from the GPU view of the largest chunk
+                        need = true;
+                        sdev = 1;
+                        unsigned long long best = 0;
+#pragma unroll
+                        for (uint32_t t = 0; t < VMAX; ++t) {
+                            if (t < nv) {
+                                const unsigned long long sz = a.chunk_size[vw[k][t]];
+                                if (t == 0 || best < sz) {
+                                    best = sz;
+                                    schunk = vw[k][t];
+                                }
+                            }
+                        }
+                        if (a.async) df |= 1u;  // fetching to the host (1126-1130)
+                        else coh |= 1u;          // synchronous completion
+                    }
+                    st[k] = coh | (vc << 2) | (nv << 10) | (df << ST_DF) | (vf << ST_VF);
+                    // the first point of one of the access's fragments: its row gets the payload
+                    const uint32_t u = px[k], y = py[k];
+                    if ((u == d.x0 || tx[k] <= aid) && (y == d.y0 || ty[k] <= aid)) {
+                        const uint32_t pay = before | ((need ? 1u : 0u) << 8) | (sdev << 9) | (act << 12);
+                        const uint32_t xo = d.xo + (u - d.x0);
+                        const uint32_t slot = atomicAdd(&nlk, 1u);
+                        if (slot < LCAP) LK[slot] = make_uint4(xo, y, pay, schunk);
+                        else fill_row(a, xo, y, pay, schunk);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t nl = min(nlk, LCAP);
+        for (uint32_t e = threadIdx.x; e < nl; e += TTHREADS) {
+            const uint4 l = LK[e];
+            fill_row(a, l.x, l.y, l.z, l.w);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < PPT; ++k) {
+            if (!cov[k]) continue;
+            const uint32_t p = threadIdx.x + TTHREADS * k;
+            a.pst[(size_t)tile * TP + p] = st[k];
+            const uint32_t nv = (st[k] >> 10) & 0xfu;  // views only append: store the new ones
+#pragma unroll
+            for (uint32_t v = 0; v < VMAX; ++v)
+                if (v >= nv0[k] && v < nv) a.pviews[((size_t)tile * VMAX + v) * TP + p] = vw[k][v];
         }
         if (threadIdx.x == 0) {
             a.tcnt[tile] = 0;
@@ -940,6 +1019,7 @@ struct xkrt_hip_mcc {
     uint32_t *imap = nullptr, *icnt = nullptr, *irow = nullptr;
     uint64_t icap[3] = {};
     uint32_t b_xtot = 0, b_irows = 1, b_items = 0, b_rows = 0;  // the plan call in progress
+    uint32_t state_grid = 2048;  // c_state workgroups: as many as the device holds at once
     uint32_t *tinc = nullptr;
     uint64_t tinc_cap = 0;
     uint32_t *bsum = nullptr;
@@ -1111,6 +1191,13 @@ int xkrt_hip_mcc_create(uint64_t ld, uint64_t sizeof_type, const xkrt_hip_mcc_co
         if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return rc;
+    }
+    {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, c_state, TTHREADS, 0) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess &&
+            per_cu > 0 && cus > 0)
+            h->state_grid = (uint32_t)(per_cu * cus);
     }
     h->device_bytes = tbytes;
     *out = h;
@@ -1316,7 +1403,7 @@ int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64
     CCHK(hipMemsetAsync(&h->ctr->nact, 0, 4, s));
     hipLaunchKernelGGL(c_tfill, dim3(gw), dim3(256), 0, s, a);
     hipLaunchKernelGGL(c_tact, dim3((ntiles + 255) / 256), dim3(256), 0, s, a, ntiles);
-    hipLaunchKernelGGL(c_state, dim3(std::min<uint32_t>(ntiles, 2048)), dim3(TTHREADS), 0, s, a);
+    hipLaunchKernelGGL(c_state, dim3(std::min<uint32_t>(ntiles, h->state_grid)), dim3(TTHREADS), 0, s, a);
     // fetch rows
     rc = scan(h, h->rflag, std::max<uint32_t>(nrows, 1), h->rfpos, &h->ctr->fetches);
     if (rc) return rc;
