@@ -78,6 +78,14 @@ struct CCounters {
     unsigned int items;     // fragment work items this call (slab x row block)
 };
 
+constexpr uint32_t F_ALLOC = 1u << 31;  // descriptor flag: the access allocates its chunk
+
+// an access as the state replay reads it: rect, flags, chunk, index, unit offset
+struct __align__(16) TDesc {
+    uint32_t x0, x1, y0, y1, fl, chunk, i, xo;
+};
+static_assert(sizeof(TDesc) == 32, "descriptors are two 16-byte words");
+
 struct CArgs {
     // batch (index i in [0, n))
     uint32_t n, gid0;
@@ -113,6 +121,7 @@ struct CArgs {
     uint32_t *foff;
     // tiles
     uint32_t *tcnt, *tcur, *tstart, *tinc, *tact;
+    TDesc *tdesc;            // per access of the batch (inserted ones)
     CCounters *ctr;
 };
 
@@ -362,6 +371,20 @@ __global__ void c_tcount(CArgs a)
         if (!(a.bflags[i] & F_INS)) continue;
         uint32_t tx0, tx1, ty0, ty1;
         tile_range(a, i, tx0, tx1, ty0, ty1);
+        if (lane == 0) {
+            TDesc d;
+            d.x0 = a.bx0[i]; d.x1 = a.bx1[i]; d.y0 = a.by0[i]; d.y1 = a.by1[i];
+            d.fl = a.bflags[i];
+            d.chunk = NONE;
+            if (d.fl & F_GPU) {
+                const uint32_t hh = a.bhist[i];
+                d.chunk = a.hchunk[hh];
+                if (a.hbeta[hh] == NONE) d.fl |= F_ALLOC;
+            }
+            d.i = i;
+            d.xo = a.bxoff[i];
+            a.tdesc[i] = d;
+        }
         const uint32_t w = tx1 - tx0 + 1, nt = w * (ty1 - ty0 + 1);
         for (uint32_t k = lane; k < nt; k += 64) atomicAdd(&a.tcnt[(ty0 + k / w) * a.ntx + tx0 + k % w], 1u);
     }
@@ -402,39 +425,31 @@ __global__ void c_tact(CArgs a, uint32_t ntiles)
 constexpr uint32_t ST_DF = 14, ST_VF = 16;
 constexpr uint32_t DMAX = 256;  // access descriptors of a tile held in LDS at a time
 constexpr uint32_t LCAP = 512;  // fragment-row lookups of a tile deferred to its end
-constexpr uint32_t F_ALLOC = 1u << 31;  // descriptor flag: the access allocates its chunk
-
-// an access as c_state reads it (LDS)
-struct TDesc {
-    uint32_t x0, x1, y0, y1, fl, chunk, i, xo;
-};
 
 __device__ void load_descs(const CArgs &a, const uint32_t *L, TDesc *D, uint32_t q0, uint32_t m)
 {
-    for (uint32_t q = threadIdx.x; q < m; q += TTHREADS) {
-        const uint32_t i = L[q0 + q];
-        TDesc d;
-        d.x0 = a.bx0[i]; d.x1 = a.bx1[i]; d.y0 = a.by0[i]; d.y1 = a.by1[i];
-        d.fl = a.bflags[i];
-        d.chunk = NONE;
-        if (d.fl & F_GPU) {
-            const uint32_t hh = a.bhist[i];
-            d.chunk = a.hchunk[hh];
-            if (a.hbeta[hh] == NONE) d.fl |= F_ALLOC;
-        }
-        d.i = i;
-        d.xo = a.bxoff[i];
-        D[q] = d;
-    }
+    for (uint32_t q = threadIdx.x; q < m; q += TTHREADS) D[q] = a.tdesc[L[q0 + q]];
 }
 
 // the fragment row starting at (u, y) of the slab at unit xo: its payload
 __device__ void fill_row(const CArgs &a, uint32_t xo, uint32_t y, uint32_t pay, uint32_t schunk)
 {
+    // the slab's last row starting at or before y; seven probes in flight per step
     uint32_t lo = a.xrow[xo], hi = lo + a.xcnt[xo];
     while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a.ry0[mid] <= y) lo = mid; else hi = mid;
+        const uint32_t step = (hi - lo + 7) / 8;
+        uint32_t v[7];
+#pragma unroll
+        for (uint32_t j = 0; j < 7; ++j) {
+            const uint32_t idx = lo + step * (j + 1);
+            v[j] = idx < hi ? a.ry0[idx] : NONE;
+        }
+        uint32_t nlo = lo;
+#pragma unroll
+        for (uint32_t j = 0; j < 7; ++j)
+            if (lo + step * (j + 1) < hi && v[j] <= y) nlo = lo + step * (j + 1);
+        hi = min(hi, nlo + step);
+        lo = nlo;
     }
     a.rpay[lo] = pay;
     a.rsrc[lo] = schunk;
@@ -447,7 +462,7 @@ __device__ void fill_row(const CArgs &a, uint32_t xo, uint32_t y, uint32_t pay, 
 // access of the tile covers are read and written (their state, the views
 // they hold, the breakpoint stamps at the point); the fragment rows whose
 // first point is in the tile are looked up once the replay is done.
-__global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
+__global__ void __launch_bounds__(TTHREADS) __attribute__((amdgpu_waves_per_eu(4))) c_state(CArgs a)
 {
     __shared__ uint32_t L[TLIST];
     __shared__ TDesc D[DMAX];
@@ -462,26 +477,7 @@ __global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
             if (threadIdx.x == 0) atomicOr(&a.ctr->err, CERR_TILE);
             continue;
         }
-        uint32_t n2 = 1;
-        while (n2 < cnt) n2 <<= 1;
-        for (uint32_t q = threadIdx.x; q < n2; q += TTHREADS) L[q] = q < cnt ? a.tinc[start + q] : NONE;
-        if (threadIdx.x == 0) nlk = 0;
-        __syncthreads();
-        for (uint32_t k = 2; k <= n2; k <<= 1)
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t q = threadIdx.x; q < n2; q += TTHREADS) {
-                    const uint32_t p = q ^ j;
-                    if (p > q) {
-                        const uint32_t x = L[q], y = L[p];
-                        if ((x > y) == ((q & k) == 0)) { L[q] = y; L[p] = x; }
-                    }
-                }
-                __syncthreads();
-            }
-        const uint32_t nch = (cnt + DMAX - 1) / DMAX;
-        load_descs(a, L, D, 0, min(cnt, DMAX));
-        __syncthreads();
-
+        // the points' state words and breakpoint stamps: loads in flight while the list is sorted
         const uint32_t tx0 = (tile % a.ntx) * TW, ty0 = (tile / a.ntx) * TH;
         uint32_t st[PPT], vw[PPT][VMAX], px[PPT], py[PPT], tx[PPT], ty[PPT], nv0[PPT];
         bool cov[PPT];
@@ -491,28 +487,66 @@ __global__ void __launch_bounds__(TTHREADS) c_state(CArgs a)
             px[k] = tx0 + p % TW;
             py[k] = ty0 + p / TW;
             cov[k] = px[k] < a.nxu && py[k] < a.ny;
+            st[k] = 0u; tx[k] = NONE; ty[k] = NONE;
+            if (cov[k]) {
+                st[k] = a.pst[(size_t)tile * TP + p];
+                tx[k] = a.TX[px[k]];
+                ty[k] = a.TY[(size_t)px[k] * a.ny1 + py[k]];
+            }
+        }
+        if (threadIdx.x == 0) nlk = 0;
+        const uint32_t nch = (cnt + DMAX - 1) / DMAX;
+        if (cnt <= TTHREADS) {
+            // rank sort: access q's descriptor goes to its rank among the tile's access indices
+            const uint32_t key = threadIdx.x < cnt ? a.tinc[start + threadIdx.x] : NONE;
+            uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0;
+            if (threadIdx.x < cnt) {
+                const uint4 *g = reinterpret_cast<const uint4 *>(a.tdesc + key);
+                d0 = g[0];
+                d1 = g[1];
+                L[threadIdx.x] = key;
+            }
+            __syncthreads();
+            if (threadIdx.x < cnt) {
+                uint32_t r = 0;
+                for (uint32_t q = 0; q < cnt; ++q) r += L[q] < key ? 1u : 0u;
+                uint4 *o = reinterpret_cast<uint4 *>(D + r);
+                o[0] = d0;
+                o[1] = d1;
+            }
+            __syncthreads();
+        } else {
+            uint32_t n2 = 1;
+            while (n2 < cnt) n2 <<= 1;
+            for (uint32_t q = threadIdx.x; q < n2; q += TTHREADS) L[q] = q < cnt ? a.tinc[start + q] : NONE;
+            __syncthreads();
+            for (uint32_t k = 2; k <= n2; k <<= 1)
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                    for (uint32_t q = threadIdx.x; q < n2; q += TTHREADS) {
+                        const uint32_t p = q ^ j;
+                        if (p > q) {
+                            const uint32_t x = L[q], y = L[p];
+                            if ((x > y) == ((q & k) == 0)) { L[q] = y; L[p] = x; }
+                        }
+                    }
+                    __syncthreads();
+                }
+            load_descs(a, L, D, 0, DMAX);
+            __syncthreads();
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < PPT; ++k) {
             if (nch == 1 && cov[k]) {  // covered by an access of the tile (all valid points past one chunk)
                 bool in = false;
                 for (uint32_t q = 0; q < cnt && !in; ++q)
                     in = D[q].x0 <= px[k] && px[k] < D[q].x1 && D[q].y0 <= py[k] && py[k] < D[q].y1;
                 cov[k] = in;
             }
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < PPT; ++k) {
             const uint32_t p = threadIdx.x + TTHREADS * k;
-            st[k] = 0u; nv0[k] = 0u; tx[k] = NONE; ty[k] = NONE;
+            nv0[k] = cov[k] ? (st[k] >> 10) & 0xfu : 0u;
 #pragma unroll
-            for (uint32_t v = 0; v < VMAX; ++v) vw[k][v] = NONE;
-            if (cov[k]) {
-                st[k] = a.pst[(size_t)tile * TP + p];
-                tx[k] = a.TX[px[k]];
-                ty[k] = a.TY[(size_t)px[k] * a.ny1 + py[k]];
-                nv0[k] = (st[k] >> 10) & 0xfu;
-#pragma unroll
-                for (uint32_t v = 0; v < VMAX; ++v)
-                    if (v < nv0[k]) vw[k][v] = a.pviews[((size_t)tile * VMAX + v) * TP + p];
-            }
+            for (uint32_t v = 0; v < VMAX; ++v)
+                vw[k][v] = v < nv0[k] ? a.pviews[((size_t)tile * VMAX + v) * TP + p] : NONE;
         }
 
         for (uint32_t c = 0; c < nch; ++c) {
@@ -1022,6 +1056,8 @@ struct xkrt_hip_mcc {
     uint32_t state_grid = 2048;  // c_state workgroups: as many as the device holds at once
     uint32_t *tinc = nullptr;
     uint64_t tinc_cap = 0;
+    TDesc *tdesc = nullptr;
+    uint64_t tdesc_cap = 0;
     uint32_t *bsum = nullptr;
     uint64_t bsum_cap = 0;
     // rows
@@ -1097,6 +1133,7 @@ CArgs make_cargs(xkrt_hip_mcc *h, uint32_t n)
     a.fx0 = h->fx0; a.fx1 = h->fx1; a.fy0 = h->fy0; a.fy1 = h->fy1; a.fsd = h->fsd; a.fsc = h->fsc;
     a.fdd = h->fdd; a.fdc = h->fdc; a.foff = h->foff;
     a.tcnt = h->tcnt; a.tcur = h->tcur; a.tstart = h->tstart; a.tinc = h->tinc; a.tact = h->tact;
+    a.tdesc = h->tdesc;
     a.ctr = h->ctr;
     a.async = h->async ? 1u : 0u;
     a.rreg = h->rreg; a.rrpos = h->rrpos; a.reg = h->reg;
@@ -1108,7 +1145,7 @@ void free_all(xkrt_hip_mcc *h)
     void *ptrs[] = {h->TX, h->TY, h->pst, h->pviews, h->tcnt, h->tcur, h->tstart, h->tact, h->tbsum, h->ctr,
                     h->hx0, h->hx1, h->hy0, h->hy1, h->hbeta, h->hchunk, h->hbytes, h->chunk_size,
                     h->bx0, h->bx1, h->by0, h->by1, h->bflags, h->bhist, h->bxoff, h->rowbase, h->foff,
-                    h->xcnt, h->xrow, h->xitems, h->xioff, h->xend, h->xacc, h->imap, h->icnt, h->irow, h->tinc, h->bsum, h->rx0, h->rx1, h->ry0, h->ry1, h->rpay, h->rsrc, h->racc,
+                    h->xcnt, h->xrow, h->xitems, h->xioff, h->xend, h->xacc, h->imap, h->icnt, h->irow, h->tinc, h->tdesc, h->bsum, h->rx0, h->rx1, h->ry0, h->ry1, h->rpay, h->rsrc, h->racc,
                     h->rflag, h->rfpos, h->fx0, h->fx1, h->fy0, h->fy1, h->fsd, h->fsc, h->fdd, h->fdc,
                     h->rreg, h->rrpos, h->reg, h->cq};
     for (void *p : ptrs)
@@ -1335,6 +1372,7 @@ int xkrt_hip_mcc_plan(xkrt_hip_mcc_t h, const xkrt_hip_mcc_access_t *acc, uint64
     uint32_t **xb[] = {&h->xcnt, &h->xrow, &h->xitems, &h->xioff, &h->xend, &h->xacc};
     for (int k = 0; k < 6 && !rc; ++k) rc = grow(xb[k], &h->xcap[k], std::max<uint64_t>(xtot, 1), false, s);
     if (!rc) rc = grow(&h->tinc, &h->tinc_cap, std::max<uint64_t>(ttot, 1), false, s);
+    if (!rc) rc = grow(&h->tdesc, &h->tdesc_cap, n, false, s);
     if (!rc) rc = grow(&h->tbsum, &h->tbsum_cap, 1, false, s);
     if (rc) return rc;
     // uploads
