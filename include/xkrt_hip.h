@@ -213,6 +213,26 @@ void *xkrt_hip_dep_stream(xkrt_hip_dep_domain_t d);
  * domain in runtime_t::reset (src/memory/invalidate.cc:68-131). */
 int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d);
 
+/* Retires the accesses with id < horizon (ids as first_id and the edges
+ * report them): the caller states that their tasks have completed.  The
+ * reference links a completed predecessor to nothing (__task_precedes,
+ * include/xkrt/task/task.hpp:452-487), and every access before horizon is
+ * completed, so each later edge from one of them would be dropped there:
+ * the domain forgets them instead (a point last written by one becomes
+ * unwritten, their reader entries go) and the remaining ids drop by
+ * horizon -- later edges are exactly the reference's.  This is how a domain
+ * outlives 2^32 accesses: ids stay 32-bit, and the id an edge or first_id
+ * reports plus xkrt_hip_dep_id_base is the access's id over the domain's
+ * life.  After it, first_id continues from the old value - horizon, and
+ * caller ids (pools) must be given minus horizon.  -EBUSY while records
+ * are staged or edges unconsumed (they carry the old ids) or a serving
+ * engine drives the pool; -EINVAL past the ids handed out; -ECANCELED on a
+ * failed domain.  Synchronises the stream, then enqueues one kernel. */
+int xkrt_hip_dep_retire(xkrt_hip_dep_domain_t d, uint64_t horizon);
+
+/* Sum of the horizons retired so far (0 for a new domain; reset keeps it). */
+int xkrt_hip_dep_id_base(xkrt_hip_dep_domain_t d, uint64_t *id_base);
+
 /* Per-kernel HIP-event timing of every later launch (count, resolve over all
  * tiers, dedup, final). */
 int xkrt_hip_dep_timing(xkrt_hip_dep_domain_t d, int enable);

@@ -31,10 +31,13 @@ def _points(rect_row, nrect):
     return out
 
 
-def resolve(task, mode, rects, nrect, xdim, ydim, put=None):
+def resolve(task, mode, rects, nrect, xdim, ydim, put=None, done=None):
     """Edges as a set of (pred, succ) access indices.  put[a] False: the
     access links but its PUT changes nothing (a segment seen by a BLAS domain
-    whose own rects meet no node of it)."""
+    whose own rects meet no node of it).  done[a]: the accesses p < done[a]
+    belong to completed tasks when a is LINKed; the state keeps them, but
+    __task_precedes (include/xkrt/task/task.hpp:452-487) makes no edge from
+    a completed predecessor and __access_link records no successor."""
     n = len(task)
     LW = np.full((xdim, ydim), -1, np.int64)
     RS = [[[] for _ in range(ydim)] for _ in range(xdim)]
@@ -58,6 +61,8 @@ def resolve(task, mode, rects, nrect, xdim, ydim, put=None):
                         elif LW[x, y] >= 0:
                             preds.add(int(LW[x, y]))
             for p in sorted(preds):
+                if done is not None and p < done[a]:
+                    continue
                 if linked.get(p) != T:
                     linked[p] = T
                     edges.add((p, a))

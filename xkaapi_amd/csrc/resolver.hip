@@ -1440,6 +1440,45 @@ __global__ void k_consume(DevCounters *c)
     if (threadIdx.x == 0 && blockIdx.x == 0) c->edge_total = 0;
 }
 
+// retire the accesses with id < h of the bins [b0, b0 + n): a fragment last
+// written by one becomes unwritten, its reader-log entries go (the log is
+// compacted in order), and the remaining ids drop by h.  One wave per bin.
+__global__ void k_retire(uint32_t *nf, uint32_t *nrs, uint32_t *fr, uint32_t *rs, uint32_t fcap, uint32_t rcap,
+                         uint32_t b0, uint32_t n, uint32_t h)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t b = wid; b < n; b += nw) {
+        const uint32_t slot = b0 + b;
+        const uint32_t nfr = nf[slot];
+        uint32_t *f = fr + (size_t)slot * fcap * 5;
+        for (uint32_t k = lane; k < nfr; k += 64) {
+            const uint32_t lw = f[5 * k + 4];
+            f[5 * k + 4] = (lw == NONE || lw < h) ? NONE : lw - h;
+        }
+        const uint32_t ng = nrs[slot];
+        uint32_t *g = rs + (size_t)slot * rcap * 5;
+        uint32_t o = 0;
+        for (uint32_t c = 0; c < ng; c += 64) {
+            const uint32_t k = c + lane;
+            uint32_t e[5] = {0, 0, 0, 0, 0};
+            if (k < ng)
+                for (int w = 0; w < 5; ++w) e[w] = g[5 * k + w];
+            const bool keep = k < ng && e[4] >= h;
+            const uint64_t m = __ballot(keep);
+            // every lane holds its entry before any store: the stores go to
+            // positions at or below the loads of this and later chunks
+            if (keep) {
+                const uint32_t d = o + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                e[4] -= h;
+                for (int w = 0; w < 5; ++w) g[5 * d + w] = e[w];
+            }
+            o += (uint32_t)__popcll(m);
+        }
+        if (lane == 0) nrs[slot] = o;
+    }
+}
+
 // move a domain's grid to another range of the pool: old bin (bx, by) ->
 // new bin (bx, by + dy); the ranges are disjoint
 __global__ void k_regrid(uint32_t *nf, uint32_t *nrs, uint32_t *fr, uint32_t *rs, uint32_t fcap, uint32_t rcap,
@@ -1602,6 +1641,7 @@ struct xkrt_hip_dep_domain {
     bool framed = false;
     uint64_t ybase = 0;
     uint64_t next_id = 0;
+    uint64_t id_base = 0;  // ids retired so far: reported ids + id_base = ids over the domain's life
     uint64_t max_xlen = 0;
     bool col0 = false;  // some rect started at absolute column 0 (addresses below ld * sizeof_type)
     // a segment domain (interval/dependency-tree.hpp:187-363): segments only, each cut into
@@ -3010,6 +3050,44 @@ int xkrt_hip_dep_stats(xkrt_hip_dep_domain_t d, xkrt_hip_dep_stats_t *out)
 }
 
 void *xkrt_hip_dep_stream(xkrt_hip_dep_domain_t d) { return d ? (void *)d->pool->stream : nullptr; }
+
+int xkrt_hip_dep_retire(xkrt_hip_dep_domain_t d, uint64_t horizon)
+{
+    if (!d) return -EINVAL;
+    xkrt_hip_pool *p = d->pool;
+    if (p->server) return -EBUSY;
+    if (p->poisoned) return -EINVAL;
+    if (d->err) return -ECANCELED;
+    const uint64_t top = std::max<uint64_t>(d->next_id, d->any_id ? (uint64_t)d->last_id + 1 : 0);
+    if (horizon > top) return -EINVAL;
+    if (horizon == 0) return 0;
+    if (p->open_nrec) return -EBUSY;  // staged records carry the current ids
+    int rc = read_counters(p);        // the stream is idle after this
+    if (rc) return rc;
+    if (p->ctr_host->edge_total) return -EBUSY;  // edges in the buffer carry the current ids
+    const uint32_t nb = d->nbx * d->nby;
+    if (nb) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)nb + 3) / 4, 65536);
+        hipLaunchKernelGGL(k_retire, dim3(grid), dim3(256), 0, p->stream, p->bin_nf, p->bin_nrs, p->frag, p->rs,
+                           p->fcap, p->rcap, d->bin0, nb, (uint32_t)horizon);
+        rc = hip_err(hipGetLastError());
+        if (rc) return rc;
+    }
+    d->next_id = d->next_id > horizon ? d->next_id - horizon : 0;
+    if (d->any_id) {
+        if (d->last_id >= horizon) d->last_id -= (uint32_t)horizon;
+        else d->any_id = false;  // every caller id retired: any id may come next
+    }
+    d->id_base += horizon;
+    return 0;
+}
+
+int xkrt_hip_dep_id_base(xkrt_hip_dep_domain_t d, uint64_t *id_base)
+{
+    if (!d || !id_base) return -EINVAL;
+    *id_base = d->id_base;
+    return 0;
+}
 
 int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d)
 {

@@ -181,6 +181,8 @@ def hip_lib():
     L.xkrt_hip_dep_stream.argtypes = [_p]
     L.xkrt_hip_dep_stream.restype = _p
     L.xkrt_hip_dep_reset.argtypes = [_p]
+    L.xkrt_hip_dep_retire.argtypes = [_p, _u64]
+    L.xkrt_hip_dep_id_base.argtypes = [_p, C.POINTER(_u64)]
     L.xkrt_hip_dep_timing.argtypes = [_p, C.c_int]
     L.xkrt_hip_dep_kernel_times.argtypes = [_p, C.POINTER(C.c_double), C.POINTER(_u64), C.c_int]
     L.xkrt_hip_dep_launch_times.argtypes = [_p, _p, _u64, C.POINTER(_u64), C.c_int]
@@ -348,6 +350,18 @@ class DependencyDomain:
     def reset(self):
         """Forget every access (asynchronous); the grid and frame stay."""
         _chk(self.L.xkrt_hip_dep_reset(self.h), "xkrt_hip_dep_reset")
+
+    def retire(self, horizon: int):
+        """The accesses with id < horizon completed: forget them and rebase the ids
+        (ids reported later + id_base = ids over the domain's life).  The edges
+        must have been read first."""
+        _chk(self.L.xkrt_hip_dep_retire(self.h, horizon), "xkrt_hip_dep_retire")
+
+    @property
+    def id_base(self) -> int:
+        b = _u64()
+        _chk(self.L.xkrt_hip_dep_id_base(self.h, C.byref(b)), "xkrt_hip_dep_id_base")
+        return int(b.value)
 
     def set_timing(self, on: bool = True):
         _chk(self.L.xkrt_hip_dep_timing(self.h, 1 if on else 0), "xkrt_hip_dep_timing")
