@@ -143,6 +143,7 @@ struct KArgs {
     uint32_t max_tier;                              // tiers 0..max_tier; storage sized for max_tier
     uint32_t min_tier;                              // first tier a bin may take (1 in small launches)
     uint32_t skip1;                                 // 1: tier 0 passes bins to tier 2 (no tier-1 kernel)
+    uint32_t reg_first;                             // 1: k_resolve_reg took the touched list first
     uint32_t *cand;           // staged form: CSTRIPES stripes of cand_cap candidates (3 u32 each)
     uint32_t cand_cap;        // per stripe
     uint32_t *cstr;           // CSTRIPES candidate counters, 16 u32 apart
@@ -446,8 +447,8 @@ __device__ __forceinline__ void edge_at(const KArgs &a, unsigned long long pos, 
 
 // the staged candidates of this block into the candidate buffer (staged
 // form) or the dedup table (direct form: one edge counter update per 64)
-template <int FCAP, int GCAP, int LCAP, bool DIR>
-__device__ void flush_cands(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s)
+template <bool DIR, class S>
+__device__ void flush_cands_to(const KArgs &a, S &s)
 {
     __syncthreads();
     const uint32_t n = s.ccount;
@@ -484,6 +485,12 @@ __device__ void flush_cands(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s)
     __syncthreads();
     if (lane == 0) s.ccount = 0;
     __syncthreads();
+}
+
+template <int FCAP, int GCAP, int LCAP, bool DIR>
+__device__ __forceinline__ void flush_cands(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DIR> &s)
+{
+    flush_cands_to<DIR>(a, s);
 }
 
 // room for m more candidates in the staging buffer.  It is flushed only when
@@ -880,15 +887,20 @@ __device__ __forceinline__ void escalate(const KArgs &a, uint32_t tier, uint4 e,
 struct BinPre {
     uint4 q0, q1;
     uint32_t f0, f1, f2, f3, f4;
+    uint32_t g0, g1, g2, g3, g4;
 };
 
-__device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p)
+// With `log`, the reader log of a bin of at most 64 entries comes too (the
+// register replay below takes it whole); otherwise the log is loaded on first
+// use (ensure_log).
+__device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p, bool log)
 {
     const uint32_t i = lane_id();
     const uint32_t slot = e.x, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
     p.q0 = make_uint4(0, 0, 0, 0);
     p.q1 = make_uint4(NONE, 0, NONE, 0);
     p.f0 = p.f1 = p.f2 = p.f3 = p.f4 = 0;
+    p.g0 = p.g1 = p.g2 = p.g3 = p.g4 = 0;
     if (i < cnt && i < a.lmax) {
         const uint4 *ir = a.binrec + 2ull * ((size_t)slot * a.lmax + i);
         p.q0 = ir[0];
@@ -898,7 +910,497 @@ __device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p)
         const uint32_t *q = a.frag + (size_t)slot * a.fcap * 5 + 5 * i;
         p.f0 = q[0]; p.f1 = q[1]; p.f2 = q[2]; p.f3 = q[3]; p.f4 = q[4];
     }
-    (void)ng0;  // the reader log is loaded on first use (ensure_log)
+    if (log && i < ng0 && ng0 <= 64u && ng0 <= a.rcap) {
+        const uint32_t *q = a.rs + (size_t)slot * a.rcap * 5 + 5 * i;
+        p.g0 = q[0]; p.g1 = q[1]; p.g2 = q[2]; p.g3 = q[3]; p.g4 = q[4];
+    }
+}
+
+// ---------------------------------------------------------------- register replay
+//
+// k_resolve_reg, the first resolve kernel of a launch, replays every touched
+// bin of at most 64 records, 64 fragments and 64 log entries with the bin's
+// whole state in registers: fragment f in lane f, log entry g in lane g,
+// record r (spawn order) in lane r.  Rects reach the other lanes by readlane,
+// and ballots take the place of the LDS scans, counters and barriers of the
+// general replay (do_link, do_run, do_paint, do_log), whose semantics it
+// keeps.  The kernel is bound by instruction issue, not memory, so each step
+// loops over whichever side is shorter: a run of readers tests its records
+// (one per lane) against the fragments in turn, or each record against all
+// fragments; a writer tests its overlapping fragments in turn against all log
+// entries, or the log entries in turn against all fragments.  LDS only moves
+// values between lanes -- the record sort, a paint's new map, a log append or
+// compaction, the corner candidates of the coverage test -- and stages the
+// candidates.  A bin that does not fit, or would outgrow 64 fragments or log
+// entries (or meet more than 63 readers in one coverage test), goes to the
+// list of the first LDS tier with nothing written back; the candidates it
+// staged are a subset of what that tier emits for it, and the cross-bin dedup
+// absorbs the repeats.
+
+__device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ uint32_t popc(uint64_t m) { return (uint32_t)__popcll(m); }
+__device__ __forceinline__ uint32_t ctz(uint64_t m) { return (uint32_t)__builtin_ctzll(m); }
+// set bits of m below this lane
+__device__ __forceinline__ uint32_t popc_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ bool lane_in(uint64_t m) { return (m >> lane_id()) & 1ull; }
+__device__ __forceinline__ uint64_t lanes_lt(uint32_t n) { return n >= 64u ? ~0ull : (1ull << n) - 1ull; }
+
+struct RegLds {
+    static constexpr uint32_t CCAP = 256;
+    uint32_t X[6][64];         // lane exchange: records (6 words), a new map (5), log entries (5)
+    uint32_t cx[64], cy[64];   // corner candidates of a coverage test
+    uint32_t C[3 * CCAP];      // staged candidates (pred, succ access, domain << 24 | succ task)
+    uint32_t ccount;
+};
+
+// a bin's state, one fragment and one log entry per lane
+struct RegBin {
+    uint32_t fx0, fx1, fy0, fy1, flw;   // fragment (lane < nf)
+    uint32_t gx0, gx1, gy0, gy1, gacc;  // log entry (lane < ng)
+};
+
+// stages candidate (p, acc, task) of every lane of m; cc = candidates staged
+template <bool DIR>
+__device__ __forceinline__ void reg_append(const KArgs &a, RegLds &s, uint32_t &cc, uint64_t m, uint32_t p,
+                                           uint32_t acc, uint32_t task)
+{
+    if (!m) return;
+    const uint32_t n = popc(m);
+    if (cc + n > RegLds::CCAP) {
+        if (lane_id() == 0) s.ccount = cc;
+        flush_cands_to<DIR>(a, s);
+        cc = 0;
+    }
+    if (lane_in(m)) {
+        const uint32_t pos = 3 * (cc + popc_below(m));
+        s.C[pos] = p;
+        s.C[pos + 1] = acc;
+        s.C[pos + 2] = task;
+    }
+    cc += n;
+}
+
+// the lanes of m holding the first occurrence of each distinct value of v
+__device__ __forceinline__ uint64_t distinct_first(uint64_t m, uint32_t v)
+{
+    uint64_t first = 0;
+    while (m) {
+        const uint32_t l = ctz(m);
+        const uint32_t x = rl(v, l);
+        first |= 1ull << l;
+        m &= ~__ballot(v == x);
+    }
+    return first;
+}
+
+// True when Q minus the log entries of lanes K (clipped to Q) holds a point.
+// If the clipped areas sum below Q's area it does.  Otherwise: if it does, it
+// holds one (cx, cy) with cx in {Qx0} U {entry x1 < Qx1} and cy in {Qy0} U
+// {entry y1 < Qy1} -- from any uncovered point, move left to the largest such
+// x at or below it, then down likewise; no entry covers the points passed.  x
+// candidate j sits in lane j; the y candidates are taken in turn.  |K| <= 63.
+__device__ __forceinline__ bool reg_uncovered(RegLds &s, const RegBin &b, uint64_t K, uint32_t Qx0, uint32_t Qx1,
+                                              uint32_t Qy0, uint32_t Qy1)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t kx0 = max(b.gx0, Qx0), kx1 = min(b.gx1, Qx1), ky0 = max(b.gy0, Qy0), ky1 = min(b.gy1, Qy1);
+    {
+        const uint64_t ar = lane_in(K) ? (uint64_t)(kx1 - kx0) * (ky1 - ky0) : 0ull;
+        uint64_t sum = 0;
+        for (uint64_t kk = K; kk; kk &= kk - 1) {
+            const uint32_t l = ctz(kk);
+            sum += ((uint64_t)rl((uint32_t)(ar >> 32), l) << 32) | rl((uint32_t)ar, l);
+        }
+        if (sum < (uint64_t)(Qx1 - Qx0) * (Qy1 - Qy0)) return true;
+    }
+    if (lane == 0) { s.cx[0] = Qx0; s.cy[0] = Qy0; }
+    if (lane_in(K)) {
+        const uint32_t r = 1 + popc_below(K);
+        s.cx[r] = kx1 < Qx1 ? kx1 : Qx0;
+        s.cy[r] = ky1 < Qy1 ? ky1 : Qy0;
+    }
+    __syncthreads();
+    const uint32_t nc = popc(K) + 1;
+    const uint32_t px = s.cx[lane < nc ? lane : 0u], cyv = s.cy[lane < nc ? lane : 0u];
+    __syncthreads();  // read before the next test writes them
+    bool unc = false;
+    for (uint32_t jy = 0; jy < nc && !unc; ++jy) {
+        const uint32_t py = rl(cyv, jy);
+        bool cov = lane >= nc;
+        for (uint64_t kk = K; kk; kk &= kk - 1) {
+            const uint32_t l = ctz(kk);
+            cov = cov || (rl(kx0, l) <= px && px < rl(kx1, l) && rl(ky0, l) <= py && py < rl(ky1, l));
+        }
+        unc = __ballot(!cov) != 0;
+    }
+    return unc;
+}
+
+// drops the logged readers that no point still sees (every fragment they
+// touch has a newer writer) and compacts the log; gdirty drops to 0 when
+// entries move
+__device__ __forceinline__ void reg_prune(RegLds &s, RegBin &b, uint32_t nf, uint32_t &ng, uint32_t &gdirty)
+{
+    const uint32_t lane = lane_id();
+    uint64_t alive = 0;
+    if (nf <= ng) {  // fragments in turn, log entries in lanes
+        bool sees = false;
+        for (uint32_t f = 0; f < nf; ++f) {
+            const uint32_t lw = rl(b.flw, f);
+            sees = sees || ((lw == NONE || lw < b.gacc) &&
+                            overlaps(rl(b.fx0, f), rl(b.fx1, f), rl(b.fy0, f), rl(b.fy1, f), b.gx0, b.gx1, b.gy0, b.gy1));
+        }
+        alive = __ballot(lane < ng && sees);
+    } else {
+        for (uint32_t g = 0; g < ng; ++g) {
+            const uint32_t ga = rl(b.gacc, g);
+            const bool sees = lane < nf && (b.flw == NONE || b.flw < ga) &&
+                              overlaps(b.fx0, b.fx1, b.fy0, b.fy1, rl(b.gx0, g), rl(b.gx1, g), rl(b.gy0, g), rl(b.gy1, g));
+            if (__ballot(sees)) alive |= 1ull << g;
+        }
+    }
+    if (alive == lanes_lt(ng)) return;
+    if (lane_in(alive)) {
+        const uint32_t o = popc_below(alive);
+        s.X[0][o] = b.gx0; s.X[1][o] = b.gx1; s.X[2][o] = b.gy0; s.X[3][o] = b.gy1; s.X[4][o] = b.gacc;
+    }
+    __syncthreads();
+    ng = popc(alive);
+    if (lane < ng) {
+        b.gx0 = s.X[0][lane]; b.gx1 = s.X[1][lane]; b.gy0 = s.X[2][lane]; b.gy1 = s.X[3][lane]; b.gacc = s.X[4][lane];
+    }
+    __syncthreads();
+    gdirty = 0;
+}
+
+// PUT of a writer (do_paint): up to four guillotine pieces of every fragment
+// under the rect survive with their writer, the rect becomes one fragment.
+// False when the map would exceed 64 fragments.
+__device__ __forceinline__ bool reg_paint(RegLds &s, RegBin &b, uint32_t &nf, uint32_t X0, uint32_t X1, uint32_t Y0,
+                                          uint32_t Y1, uint32_t acc, bool whole)
+{
+    const uint32_t lane = lane_id();
+    if (whole) {  // the rect is the whole bin: one fragment
+        if (lane == 0) { b.fx0 = X0; b.fx1 = X1; b.fy0 = Y0; b.fy1 = Y1; b.flw = acc; }
+        nf = 1;
+        return true;
+    }
+    const bool live = lane < nf;
+    const bool ov = live && overlaps(b.fx0, b.fx1, b.fy0, b.fy1, X0, X1, Y0, Y1);
+    // pieces of f minus the rect: top, bottom, then left and right of the middle band
+    const bool pt = ov && b.fx0 < X0, pbm = ov && X1 < b.fx1, pl = ov && b.fy0 < Y0, pr = ov && Y1 < b.fy1;
+    const uint32_t npc = live ? (ov ? (uint32_t)pt + pbm + pl + pr : 1u) : 0u;
+    const uint64_t c1 = __ballot(npc >= 1), c2 = __ballot(npc >= 2), c3 = __ballot(npc >= 3), c4 = __ballot(npc >= 4);
+    const uint32_t tot = popc(c1) + popc(c2) + popc(c3) + popc(c4);
+    if (tot + 1 > 64u) return false;
+    uint32_t o = popc_below(c1) + popc_below(c2) + popc_below(c3) + popc_below(c4);
+    if (live && !ov) {
+        s.X[0][o] = b.fx0; s.X[1][o] = b.fx1; s.X[2][o] = b.fy0; s.X[3][o] = b.fy1; s.X[4][o] = b.flw;
+    } else if (ov) {
+        const uint32_t mx0 = max(b.fx0, X0), mx1 = min(b.fx1, X1);
+        if (pt) { s.X[0][o] = b.fx0; s.X[1][o] = X0; s.X[2][o] = b.fy0; s.X[3][o] = b.fy1; s.X[4][o] = b.flw; ++o; }
+        if (pbm) { s.X[0][o] = X1; s.X[1][o] = b.fx1; s.X[2][o] = b.fy0; s.X[3][o] = b.fy1; s.X[4][o] = b.flw; ++o; }
+        if (pl) { s.X[0][o] = mx0; s.X[1][o] = mx1; s.X[2][o] = b.fy0; s.X[3][o] = Y0; s.X[4][o] = b.flw; ++o; }
+        if (pr) { s.X[0][o] = mx0; s.X[1][o] = mx1; s.X[2][o] = Y1; s.X[3][o] = b.fy1; s.X[4][o] = b.flw; }
+    }
+    if (lane == 0) { s.X[0][tot] = X0; s.X[1][tot] = X1; s.X[2][tot] = Y0; s.X[3][tot] = Y1; s.X[4][tot] = acc; }
+    __syncthreads();
+    nf = tot + 1;
+    if (lane < nf) {
+        b.fx0 = s.X[0][lane]; b.fx1 = s.X[1][lane]; b.fy0 = s.X[2][lane]; b.fy1 = s.X[3][lane]; b.flw = s.X[4][lane];
+    }
+    __syncthreads();
+    return true;
+}
+
+// LINK of one record with the reader rule: the last writers of its points
+template <bool DIR>
+__device__ __forceinline__ void reg_link_reader(const KArgs &a, RegLds &s, uint32_t &cc, const RegBin &b, uint32_t nf,
+                                                uint32_t X0, uint32_t X1, uint32_t Y0, uint32_t Y1, uint32_t acc,
+                                                uint32_t task)
+{
+    const bool ov = lane_id() < nf && b.flw != NONE && overlaps(b.fx0, b.fx1, b.fy0, b.fy1, X0, X1, Y0, Y1);
+    reg_append<DIR>(a, s, cc, distinct_first(__ballot(ov), b.flw), b.flw, acc, task);
+}
+
+// LINK of one record with the writer rule.  Per overlapping fragment f and
+// Q = f n rect: every logged reader newer than lw(f) touching Q is a
+// predecessor, and lw(f) is one when some point of Q has no such reader.
+// False when a coverage test would meet more than 63 readers.
+template <bool DIR>
+__device__ __forceinline__ bool reg_link_writer(const KArgs &a, RegLds &s, uint32_t &cc, const RegBin &b, uint32_t nf,
+                                                uint32_t ng, uint32_t X0, uint32_t X1, uint32_t Y0, uint32_t Y1,
+                                                uint32_t acc, uint32_t task)
+{
+    const uint32_t lane = lane_id();
+    const bool ov = lane < nf && overlaps(b.fx0, b.fx1, b.fy0, b.fy1, X0, X1, Y0, Y1);
+    const uint32_t qx0 = max(b.fx0, X0), qx1 = min(b.fx1, X1), qy0 = max(b.fy0, Y0), qy1 = min(b.fy1, Y1);
+    const uint64_t OV = __ballot(ov);
+    uint64_t PG = 0;   // log lanes that are predecessors
+    uint64_t elw = 0;  // fragment lanes whose writer is one
+    uint64_t part = 0; // fragment lanes that readers cover in part: exact test
+    if (popc(OV) <= ng) {  // overlapping fragments in turn, log entries in lanes
+        for (uint64_t om = OV; om; om &= om - 1) {
+            const uint32_t l = ctz(om);
+            const uint32_t Qx0 = rl(qx0, l), Qx1 = rl(qx1, l), Qy0 = rl(qy0, l), Qy1 = rl(qy1, l);
+            const uint32_t lw = rl(b.flw, l);
+            const bool t = lane < ng && (lw == NONE || b.gacc > lw) &&
+                           overlaps(b.gx0, b.gx1, b.gy0, b.gy1, Qx0, Qx1, Qy0, Qy1);
+            const uint64_t K = __ballot(t);
+            PG |= K;
+            if (lw == NONE) continue;
+            if (!K) {
+                elw |= 1ull << l;
+                continue;
+            }
+            if (!__ballot(t && b.gx0 <= Qx0 && Qx1 <= b.gx1 && b.gy0 <= Qy0 && Qy1 <= b.gy1)) part |= 1ull << l;
+        }
+    } else {  // log entries in turn, fragments in lanes
+        uint32_t nk = 0;
+        bool full = false;
+        for (uint32_t g = 0; g < ng; ++g) {
+            const uint32_t ga = rl(b.gacc, g);
+            const uint32_t gx0 = rl(b.gx0, g), gx1 = rl(b.gx1, g), gy0 = rl(b.gy0, g), gy1 = rl(b.gy1, g);
+            const bool t = ov && (b.flw == NONE || ga > b.flw) && overlaps(gx0, gx1, gy0, gy1, qx0, qx1, qy0, qy1);
+            if (__ballot(t)) PG |= 1ull << g;
+            nk += t ? 1u : 0u;
+            full = full || (t && gx0 <= qx0 && qx1 <= gx1 && gy0 <= qy0 && qy1 <= gy1);
+        }
+        elw = __ballot(ov && b.flw != NONE && nk == 0);
+        part = __ballot(ov && b.flw != NONE && nk != 0 && !full);
+    }
+    for (; part; part &= part - 1) {
+        const uint32_t l = ctz(part);
+        const uint32_t Qx0 = rl(qx0, l), Qx1 = rl(qx1, l), Qy0 = rl(qy0, l), Qy1 = rl(qy1, l);
+        const uint32_t lw = rl(b.flw, l);
+        const uint64_t K = __ballot(lane < ng && b.gacc > lw && overlaps(b.gx0, b.gx1, b.gy0, b.gy1, Qx0, Qx1, Qy0, Qy1));
+        if (popc(K) > 63u) return false;
+        if (reg_uncovered(s, b, K, Qx0, Qx1, Qy0, Qy1)) elw |= 1ull << l;
+    }
+    reg_append<DIR>(a, s, cc, PG, b.gacc, acc, task);
+    reg_append<DIR>(a, s, cc, distinct_first(elw, b.flw), b.flw, acc, task);
+    return true;
+}
+
+template <bool DIR>
+__global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    RegLds &s = *reinterpret_cast<RegLds *>(lds_raw);
+    const uint32_t lane = lane_id();
+    if (a.ctr->err) return;  // a poisoned pool touches no state
+    const uint32_t nlist = min(a.ctr->ntouched, a.nbins);
+    const uint32_t next = a.min_tier;  // the tier that takes the bins not replayed here
+    unsigned long long visits = 0;
+    uint32_t maxf = 0, maxg = 0;
+    uint32_t cc = 0;  // candidates staged in s.C
+    // The same software pipeline over the block's bins as k_resolve: list entries 64 at
+    // a time, one per lane; the next bin's records, fragments and log loaded before this
+    // bin's write-back.
+    const uint32_t G = gridDim.x;
+    auto entry = [&](uint32_t x) {
+        if (x >= nlist) return make_uint4(0, 0, 0, 0);
+        const uint32_t slot = a.touched[x];
+        return make_uint4(slot, 0u, a.bin_cnt[slot], a.bin_nf[slot] | (a.bin_nrs[slot] << 11) | (a.bin_dom[slot] << 24));
+    };
+    uint32_t wbase = 0;
+    uint4 ew = entry(blockIdx.x + lane * G);
+    auto entry_k = [&](uint32_t k) {
+        if (k >= wbase + 64) {
+            wbase += 64;
+            ew = entry(blockIdx.x + (wbase + lane) * G);
+        }
+        const uint32_t l = k - wbase;
+        return make_uint4(rl(ew.x, l), rl(ew.y, l), rl(ew.z, l), rl(ew.w, l));
+    };
+    uint4 e_cur = entry_k(0), e_nxt = entry_k(1);
+    BinPre pre;
+    bin_pre_load(a, e_cur, pre, true);
+    for (uint32_t bi = blockIdx.x, k = 0; bi < nlist; bi += G, ++k) {
+        const uint4 e = e_cur;
+        const BinPre cur = pre;
+        e_cur = e_nxt;
+        const uint32_t slot = e.x, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu, dm = e.w >> 24;
+        if (a.dom_err[dm]) {  // a failed domain touches no state
+            bin_pre_load(a, e_cur, pre, true);
+            e_nxt = entry_k(k + 2);
+            if (lane == 0) a.bin_cnt[slot] = 0;
+            continue;
+        }
+        bool ok = cnt <= 64u && nf0 <= 64u && ng0 <= 64u;
+        const DomDesc g = a.dom[dm];
+        const uint32_t local = slot - g.bin0;
+        const uint32_t bxi = local % g.nbx, byi = local / g.nbx;
+        const uint32_t BX0 = bxi << g.bxs, BX1 = min((bxi + 1) << g.bxs, g.xmax);
+        const uint32_t BY0 = g.ybin0 + (byi << g.bys), BY1 = BY0 + (1u << g.bys);
+        const uint32_t tdom = dm << 24;
+
+        RegBin b;
+        uint32_t nf = nf0, ng = ng0, gdirty = ng0, fdirty = 0;
+        b.fx0 = cur.f0; b.fx1 = cur.f1; b.fy0 = cur.f2; b.fy1 = cur.f3; b.flw = cur.f4;
+        b.gx0 = cur.g0; b.gx1 = cur.g1; b.gy0 = cur.g2; b.gy1 = cur.g3; b.gacc = cur.g4;
+        if (nf0 == 0) {  // a bin never written: one fragment, no writer
+            b.fx0 = BX0; b.fx1 = BX1; b.fy0 = BY0; b.fy1 = BY1; b.flw = NONE;
+            nf = 1;
+            fdirty = 1;
+        }
+        // the records, clipped to the bin, in spawn order (rank by record index, distinct
+        // within a bin; k_count's buckets are often in order already)
+        uint32_t rx0 = max(cur.q0.x, BX0), rx1 = min(cur.q0.y, BX1), ry0 = max(cur.q0.z, BY0), ry1 = min(cur.q0.w, BY1);
+        uint32_t racc = cur.q1.x, rtm = cur.q1.y;
+        if (ok) {
+            const uint32_t key = lane < cnt ? cur.q1.z : NONE;
+            const uint32_t pkey = __shfl_up(key, 1, 64);
+            if (__ballot(lane > 0 && lane < cnt && pkey > key)) {
+                uint32_t rank = 0;
+                for (uint32_t j = 0; j < cnt; ++j) rank += rl(key, j) < key ? 1u : 0u;
+                if (lane < cnt) {
+                    s.X[0][rank] = rx0; s.X[1][rank] = rx1; s.X[2][rank] = ry0; s.X[3][rank] = ry1;
+                    s.X[4][rank] = racc; s.X[5][rank] = rtm;
+                }
+                __syncthreads();
+                if (lane < cnt) {
+                    rx0 = s.X[0][lane]; rx1 = s.X[1][lane]; ry0 = s.X[2][lane]; ry1 = s.X[3][lane];
+                    racc = s.X[4][lane]; rtm = s.X[5][lane];
+                }
+                __syncthreads();
+            }
+        }
+        const uint32_t rtask = rtm >> 8, rmode = rtm & 0xffu;
+        const uint32_t ptask = __shfl_up(rtask, 1, 64);
+        const uint64_t Wm = __ballot(lane < cnt && (rmode & 2u));                   // writer-rule LINKs
+        const uint64_t GS = __ballot(lane < cnt && (lane == 0 || ptask != rtask));  // task group starts
+        const uint64_t RD = __ballot(lane < cnt && rmode == 1u);                    // exact readers (logged)
+        uint32_t i = 0;
+        while (ok && i < cnt) {
+            const uint64_t wr = Wm & (~0ull << i);
+            uint32_t end = cnt;
+            if (wr) {
+                const uint32_t w = ctz(wr);
+                end = 63u - (uint32_t)__builtin_clzll(GS & lanes_lt(w + 1));  // the writer's group start
+            }
+            if (end > i) {
+                // Whole task groups with no writer: LINK reads only the writer map, PUT only
+                // appends readers, so every record links and the exact readers are then logged.
+                const uint64_t run = lanes_lt(end) & ~lanes_lt(i);
+                if (nf <= end - i) {  // fragments in turn, records in lanes
+                    const bool in = lane_in(run);
+                    uint32_t last = NONE;  // the writer this record emitted last
+                    for (uint32_t f = 0; f < nf; ++f) {
+                        const uint32_t lw = rl(b.flw, f);
+                        if (lw == NONE) continue;
+                        const bool ov = in && lw != last &&
+                                        overlaps(rl(b.fx0, f), rl(b.fx1, f), rl(b.fy0, f), rl(b.fy1, f), rx0, rx1, ry0, ry1);
+                        reg_append<DIR>(a, s, cc, __ballot(ov), lw, racc, rtask | tdom);
+                        last = ov ? lw : last;
+                    }
+                } else {
+                    for (uint32_t q = i; q < end; ++q)
+                        reg_link_reader<DIR>(a, s, cc, b, nf, rl(rx0, q), rl(rx1, q), rl(ry0, q), rl(ry1, q), rl(racc, q),
+                                             (rl(rtm, q) >> 8) | tdom);
+                }
+                const uint64_t rd = RD & run;
+                const uint32_t nr = popc(rd);
+                if (ng + nr > 64u) {
+                    reg_prune(s, b, nf, ng, gdirty);
+                    if (ng + nr > 64u) {
+                        ok = false;
+                        break;
+                    }
+                }
+                if (nr) {
+                    if (lane_in(rd)) {
+                        const uint32_t o = ng + popc_below(rd);
+                        s.X[0][o] = rx0; s.X[1][o] = rx1; s.X[2][o] = ry0; s.X[3][o] = ry1; s.X[4][o] = racc;
+                    }
+                    __syncthreads();
+                    if (lane >= ng && lane < ng + nr) {
+                        b.gx0 = s.X[0][lane]; b.gx1 = s.X[1][lane]; b.gy0 = s.X[2][lane]; b.gy1 = s.X[3][lane];
+                        b.gacc = s.X[4][lane];
+                    }
+                    __syncthreads();
+                    ng += nr;
+                }
+                visits += (unsigned long long)nf * (end - i);
+                i = end;
+                continue;
+            }
+            // the task group at i writes: LINK each record, then PUT each, in slot order
+            const uint64_t nx = GS & ~lanes_lt(i + 1);
+            end = nx ? min(ctz(nx), cnt) : cnt;
+            for (uint32_t q = i; ok && q < end; ++q) {
+                const uint32_t X0 = rl(rx0, q), X1 = rl(rx1, q), Y0 = rl(ry0, q), Y1 = rl(ry1, q);
+                const uint32_t acc = rl(racc, q), tm = rl(rtm, q), task = (tm >> 8) | tdom;
+                if (tm & 2u)
+                    ok = reg_link_writer<DIR>(a, s, cc, b, nf, ng, X0, X1, Y0, Y1, acc, task);
+                else
+                    reg_link_reader<DIR>(a, s, cc, b, nf, X0, X1, Y0, Y1, acc, task);
+            }
+            for (uint32_t q = i; ok && q < end; ++q) {
+                const uint32_t mode = rl(rtm, q) & 0xffu;
+                if (mode & 0x80u) continue;  // a segment seen by this domain: LINK only
+                const uint32_t X0 = rl(rx0, q), X1 = rl(rx1, q), Y0 = rl(ry0, q), Y1 = rl(ry1, q), acc = rl(racc, q);
+                if (mode & 2u) {
+                    ok = reg_paint(s, b, nf, X0, X1, Y0, Y1, acc, X0 == BX0 && X1 == BX1 && Y0 == BY0 && Y1 == BY1);
+                    fdirty = 1;
+                } else if (mode == 1u) {
+                    if (ng >= 64u) {
+                        reg_prune(s, b, nf, ng, gdirty);
+                        if (ng >= 64u) {
+                            ok = false;
+                            break;
+                        }
+                    }
+                    if (lane == ng) { b.gx0 = X0; b.gx1 = X1; b.gy0 = Y0; b.gy1 = Y1; b.gacc = acc; }
+                    ++ng;
+                }
+            }
+            visits += (unsigned long long)nf * (end - i);
+            i = end;
+        }
+        // a log past half the registers is pruned before it is stored
+        if (ok && ng > 32u) reg_prune(s, b, nf, ng, gdirty);
+        bin_pre_load(a, e_cur, pre, true);  // before the write-back stores
+        e_nxt = entry_k(k + 2);
+        if (!ok) {  // the first LDS tier takes the bin from its HBM state
+            if (lane == 0) {
+                const uint32_t pos = atomicAdd(&a.ctr->ntier[next], 1u);
+                a.active[(size_t)next * a.nbins + pos] = e;
+            }
+            continue;
+        }
+        uint32_t *fw = a.frag + (size_t)slot * a.fcap * 5;
+        uint32_t *gw = a.rs + (size_t)slot * a.rcap * 5;
+        if (fdirty && lane < nf) {
+            uint32_t *q = fw + 5 * lane;
+            q[0] = b.fx0; q[1] = b.fx1; q[2] = b.fy0; q[3] = b.fy1; q[4] = b.flw;
+        }
+        if (lane >= gdirty && lane < ng) {
+            uint32_t *q = gw + 5 * lane;
+            q[0] = b.gx0; q[1] = b.gx1; q[2] = b.gy0; q[3] = b.gy1; q[4] = b.gacc;
+        }
+        if (lane == 0) {
+            a.bin_nf[slot] = nf;
+            a.bin_nrs[slot] = ng;
+            a.bin_cnt[slot] = 0;
+        }
+        maxf = max(maxf, nf);
+        maxg = max(maxg, ng);
+    }
+    if (lane == 0) s.ccount = cc;
+    flush_cands_to<DIR>(a, s);
+    if (lane == 0 && visits) {  // statistics, in the first tier's slots
+        uint4 *o = &a.blk[next * BLKSLOTS + blockIdx.x % BLKSLOTS];
+        atomicMax(&o->x, maxf);
+        atomicMax(&o->y, maxg);
+        atomicAdd(reinterpret_cast<unsigned long long *>(&o->z), visits);
+    }
 }
 
 template <int FCAP, int GCAP, int LCAP, bool DIR>
@@ -910,7 +1412,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     if (a.ctr->err) return;  // a poisoned pool touches no state
     // a launch's first tier takes every touched bin straight from the touched list
     // (no k_alloc); later tiers take their active list (bins escalated from below)
-    const bool touched = tier == a.min_tier;
+    const bool touched = tier == a.min_tier && !a.reg_first;
     const uint32_t nlist = touched ? min(a.ctr->ntouched, a.nbins) : min(a.ctr->ntier[tier], a.nbins);
     const uint4 *list = a.active + (size_t)tier * a.nbins;
     unsigned long long visits = 0;
@@ -947,13 +1449,13 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     };
     uint4 e_cur = entry_k(0), e_nxt = entry_k(1);
     BinPre pre;
-    bin_pre_load(a, e_cur, pre);
+    bin_pre_load(a, e_cur, pre, false);
     for (uint32_t b = blockIdx.x, k = 0; b < nlist; b += G, ++k) {
         const uint4 e = e_cur;
         const BinPre cur = pre;
         auto advance = [&]() {
             e_cur = e_nxt;
-            bin_pre_load(a, e_cur, pre);
+            bin_pre_load(a, e_cur, pre, false);
             e_nxt = entry_k(k + 2);
         };
         const uint32_t slot = e.x, cnt = e.z, nf0 = e.w & 0x7ffu, ng0 = (e.w >> 11) & 0xfffu;
@@ -964,6 +1466,12 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
             if (lane == 0) a.bin_cnt[slot] = 0;
             continue;
         }
+        const uint32_t dm = e.w >> 24;
+        const DomDesc g = a.dom[dm];
+        const uint32_t local = slot - g.bin0;
+        const uint32_t bxi = local % g.nbx, byi = local / g.nbx;
+        const uint32_t BX0 = bxi << g.bxs, BX1 = min((bxi + 1) << g.bxs, g.xmax);
+        const uint32_t BY0 = g.ybin0 + (byi << g.bys), BY1 = BY0 + (1u << g.bys);
         // a bin goes on to the next tier when it does not fit this one, or when its map or
         // log fill more than half of this tier's room (they grow during the replay)
         if (cnt > (uint32_t)LCAP || nf0 > (uint32_t)FCAP || ng0 > (uint32_t)GCAP ||
@@ -972,12 +1480,6 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
             escalate<FCAP, GCAP, LCAP, DIR>(a, tier, e, ERR_LIST);
             continue;
         }
-        const uint32_t dm = e.w >> 24;
-        const DomDesc g = a.dom[dm];
-        const uint32_t local = slot - g.bin0;
-        const uint32_t bxi = local % g.nbx, byi = local / g.nbx;
-        const uint32_t BX0 = bxi << g.bxs, BX1 = min((bxi + 1) << g.bxs, g.xmax);
-        const uint32_t BY0 = g.ybin0 + (byi << g.bys), BY1 = BY0 + (1u << g.bys);
         if (lane == 0) s.tdom = dm << 24;
 
         // record ids, fragments and reader log of the bin, loads issued together
@@ -1875,6 +2377,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.max_tier = p->kclass + 1;  // tier t + 1 is class t
     a.min_tier = 0;
     a.skip1 = 0;
+    a.reg_first = 0;
     a.cand = p->cand; a.cand_cap = p->cand_cap; a.cstr = p->cstr; a.blk = p->blk;
     a.hkeys = p->hkeys; a.hvals = p->hvals; a.hmask = p->hmask;
     a.eslot = p->eslot; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
@@ -1928,6 +2431,19 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
         launch_on(p, st, spans, 3, k_resolve<F, R, L, false>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
 }
 
+// k_resolve_reg: blocks per CU (its VGPRs allow 7 waves per SIMD; its LDS, 2.8 KB, many more)
+constexpr uint32_t REG_PER_CU = 28;
+
+void launch_reg(xkrt_hip_pool *p, const KArgs &a, std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans,
+                uint32_t over)
+{
+    const uint32_t blocks = std::min<uint32_t>(p->cap_bins, 256u * REG_PER_CU * over);
+    if (p->cfg.direct_dedup)
+        launch(p, spans, 3, k_resolve_reg<true>, dim3(blocks), dim3(64), (uint32_t)sizeof(RegLds), a);
+    else
+        launch(p, spans, 3, k_resolve_reg<false>, dim3(blocks), dim3(64), (uint32_t)sizeof(RegLds), a);
+}
+
 // the kernels of one launch (timing slots: count, -, -, resolve, dedup, final)
 // k_count blocks (4 waves of KRPW records each) for n records; it strides over more
 uint32_t count_blocks(uint64_t n)
@@ -1956,8 +2472,14 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint64_t nrec,
     // bin) gain up to 8 blocks per slot; short ones do best with 3 (random 4096-task launches:
     // 30.9 ms per pass with 2, 30.4 with 3) and lose to the extra block starts beyond that.
     const uint32_t over0 = nrec >= (1u << 16) ? 8u : 3u;
+    // the register replay first: it takes the touched bins and passes those it cannot hold
+    // to the first LDS tier, which then walks only them
+    if (!small) {
+        ak.reg_first = 1;
+        launch_reg(p, ak, spans, over0);
+    }
     // tier 0 first; bins that outgrow a tier are retried by the next one
-    if (!small) launch_resolve<64, 128, 64>(p, ak, 0, spans, nullptr, over0);
+    if (!small) launch_resolve<64, 128, 64>(p, ak, 0, spans, nullptr, 1);
     if (!ak.skip1) launch_resolve<64, 128, 256>(p, ak, 1, spans);
     if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, ak, 2, spans);
     if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, ak, 3, spans);
@@ -2262,6 +2784,13 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     if (!rc) chk(hipMemsetAsync(p->dom_err, 0, 4ull * MAX_DOMS, p->stream));
     if (!rc) chk(hipMemsetAsync(p->cstr, 0, 64ull * CSTRIPES, p->stream));
     if (!rc) chk(hipMemsetAsync(p->blk, 0, 16ull * 4 * BLKSLOTS, p->stream));
+    {
+        hipFuncAttributes at{};
+        for (const void *f : {(const void *)k_resolve_reg<false>, (const void *)k_resolve_reg<true>}) {
+            chk(hipFuncGetAttributes(&at, f));
+            if (!rc && at.sharedSizeBytes != 0) rc = -ENOEXEC;  // dynamic LDS must start at offset 0
+        }
+    }
     chk(set_lds<64, 128, 64>());
     chk(set_lds<64, 128, 256>());
     chk(set_lds<256, 512, 512>());
