@@ -14,7 +14,8 @@
 //     words each {x0, x1, y0, y1}, {acc, task<<8|mode, domain, 0}: one record
 //     is two dwordx4 loads by one lane (the record an incidence refers to is
 //     read whole, never a single field);
-//   * incidences (record x bin) are 32 B: {x0, x1, y0, y1}, {acc, tm, record, 0}.
+//   * incidences (record x bin) are 32 B: {x0, x1, y0, y1}, {acc, tm, record, one}
+//     (one: the record's access is this record alone and lies in this bin).
 //
 // One launch resolves one batch of records in these kernels:
 //   k_count   records -> per-bin record counts; each incidence goes into its
@@ -66,6 +67,9 @@
 
 #define NONE 0xFFFFFFFFu
 #define CSTRIPES 16u   // candidate buffer stripes, each with its own counter (64 B apart)
+#define CSTRIPES_C 8u  // stripes [0, 8) hold candidates for the dedup table, [8, 16) edges known
+                       // unique where they were found (k_resolve_reg), which bypass the table
+#define DIRECT_SLOT 0x80000000u  // eslot entry of such an edge: this bit | its candidate index
 #define BLKSLOTS 4096u // per-block statistics slots per tier (4 tiers)
 #define EMPTY64 0xFFFFFFFFFFFFFFFFull
 
@@ -131,12 +135,14 @@ struct KArgs {
     const DomDesc *dom;       // domain table
     const uint32_t *bin_dom;  // domain of each pool bin
     uint32_t *bin_nf, *bin_nrs;
+    uint4 *bin_one;           // per bin, 64-bit masks {fragments lo, hi, log entries lo, hi}: the
+                              // access of fragment / entry i lies in this bin alone (0: unknown)
     uint32_t *frag;           // nbins * fcap fragments, 5 words each (x0, x1, y0, y1, lw)
     uint32_t *rs;             // nbins * rcap reader-log entries, 5 words each (x0, x1, y0, y1, acc)
     uint32_t fcap, rcap;
     uint32_t *bin_cnt;        // incidences per bin this launch (zeroed by k_resolve)
     uint32_t *touched;        // bins with records this launch, in first-touch order
-    uint4 *binrec;  // per bin a bucket of lmax incidences, 2 x uint4 each: {x0, x1, y0, y1}, {acc, tm, record, 0}
+    uint4 *binrec;  // per bin a bucket of lmax incidences, 2 x uint4 each: {x0, x1, y0, y1}, {acc, tm, record, one}
     uint4 *active;  // per tier, nbins entries {slot, incidence start, record count, nf | ng << 16}
     uint32_t lmax;            // bucket capacity: records per bin and launch of the highest tier
     uint32_t nbins;
@@ -152,6 +158,7 @@ struct KArgs {
     uint32_t *hvals;
     uint32_t hmask;
     uint32_t *eslot;
+    uint32_t *dcnt;           // staged form: new keys per k_dedup block this launch
     uint32_t *edges;          // (pred id, succ id) per edge
     uint32_t *edom;           // domain of each edge
     unsigned long long edge_cap;
@@ -225,14 +232,16 @@ constexpr uint32_t KRPW = 16;  // records per wave in k_count
 
 struct WaveRecs {
     uint32_t r, valid, bx0, w, by0, n, incl, bin0, nbx;
+    uint32_t one;  // the record's access is this one record, in one bin
 };
 
-__device__ __forceinline__ WaveRecs wave_recs(const KArgs &a, uint32_t r, uint32_t nrec)
+// records [0, nall) of the launch; this wave's are those below nrec
+__device__ __forceinline__ WaveRecs wave_recs(const KArgs &a, uint32_t r, uint32_t nrec, uint32_t nall)
 {
     WaveRecs q;
     q.r = r;
     q.valid = r < nrec;
-    q.bx0 = q.w = q.by0 = q.n = q.bin0 = q.nbx = 0;
+    q.bx0 = q.w = q.by0 = q.n = q.bin0 = q.nbx = q.one = 0;
     if (q.valid) {
         uint4 q0, q1;
         DomDesc g;
@@ -240,6 +249,10 @@ __device__ __forceinline__ WaveRecs wave_recs(const KArgs &a, uint32_t r, uint32
         if (rec_bins(a, r, q0, q1, g, bx0, bx1, by0, by1)) {
             q.bx0 = bx0; q.w = bx1 - bx0 + 1; q.by0 = by0; q.n = q.w * (by1 - by0 + 1);
             q.bin0 = g.bin0; q.nbx = g.nbx;
+            // an access's records are adjacent (a wrapped tile has two, a segment up to three)
+            const uint32_t pa = r > 0 ? a.rec[2ull * (r - 1) + 1].x : NONE;
+            const uint32_t na = r + 1 < nall ? a.rec[2ull * (r + 1) + 1].x : NONE;
+            q.one = q.n == 1 && pa != q1.x && na != q1.x ? 1u : 0u;
         }
     }
     q.incl = wave_incl_scan(q.n);
@@ -304,12 +317,13 @@ __global__ void __launch_bounds__(256) k_count(KArgs a)
     // spread over the wave's 64 lanes, so more waves share a launch's records
     const uint32_t wave = tid >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t r0 = wave * KRPW; r0 < nrec; r0 += nwaves * KRPW) {
-        const WaveRecs q = wave_recs(a, r0 + lane, lane < KRPW ? min(nrec, r0 + KRPW) : 0u);
+        const WaveRecs q = wave_recs(a, r0 + lane, lane < KRPW ? min(nrec, r0 + KRPW) : 0u, nrec);
         const uint32_t total = __shfl(q.incl, 63, 64);
         for (uint32_t k0 = 0; k0 < total; k0 += 64) {
             const uint32_t k = k0 + lane;
             uint32_t owner;
             const uint32_t bin = wave_rec_bin(q, min(k, total - 1), owner);
+            const uint32_t one = __shfl(q.one, (int)owner, 64);
             bool first = false;
             if (k < total) {
                 // the incidence goes straight into its bin's bucket; a bin with more than lmax
@@ -321,7 +335,7 @@ __global__ void __launch_bounds__(256) k_count(KArgs a)
                 if (pos < a.lmax) {
                     uint4 *o = a.binrec + 2ull * ((size_t)bin * a.lmax + pos);
                     o[0] = q0;
-                    o[1] = make_uint4(q1.x, q1.y, r, 0u);
+                    o[1] = make_uint4(q1.x, q1.y, r, one);
                 }
             }
             // the bins this launch touches, listed once each
@@ -453,7 +467,7 @@ __device__ void flush_cands_to(const KArgs &a, S &s)
     __syncthreads();
     const uint32_t n = s.ccount;
     const uint32_t lane = lane_id();
-    const uint32_t stripe = blockIdx.x % CSTRIPES;
+    const uint32_t stripe = blockIdx.x % CSTRIPES_C;
     uint32_t base = 0;
     if (n && lane == 0) base = atomicAdd(&a.cstr[16 * stripe], n);
     if (!DIR && n) {  // staged form: the candidates go to the buffer for k_dedup
@@ -526,7 +540,7 @@ __device__ __forceinline__ void emit(const KArgs &a, BinLds<FCAP, GCAP, LCAP, DI
         h = (h + 1) & (DSZ - 1);
     }
     // the bin's set is crowded: this candidate goes on alone
-    const uint32_t stripe = blockIdx.x % CSTRIPES;
+    const uint32_t stripe = blockIdx.x % CSTRIPES_C;
     const uint32_t pos = atomicAdd(&a.cstr[16 * stripe], 1u);
     if (DIR) {
         const uint32_t slot = hash_insert(a, p, acc, task);
@@ -888,6 +902,7 @@ struct BinPre {
     uint4 q0, q1;
     uint32_t f0, f1, f2, f3, f4;
     uint32_t g0, g1, g2, g3, g4;
+    uint4 mk;  // the bin's single-bin marks (with the log)
 };
 
 // With `log`, the reader log of a bin of at most 64 entries comes too (the
@@ -901,6 +916,8 @@ __device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p,
     p.q1 = make_uint4(NONE, 0, NONE, 0);
     p.f0 = p.f1 = p.f2 = p.f3 = p.f4 = 0;
     p.g0 = p.g1 = p.g2 = p.g3 = p.g4 = 0;
+    p.mk = make_uint4(0, 0, 0, 0);
+    if (log && (cnt || nf0)) p.mk = a.bin_one[slot];
     if (i < cnt && i < a.lmax) {
         const uint4 *ir = a.binrec + 2ull * ((size_t)slot * a.lmax + i);
         p.q0 = ir[0];
@@ -952,38 +969,78 @@ __device__ __forceinline__ bool lane_in(uint64_t m) { return (m >> lane_id()) & 
 __device__ __forceinline__ uint64_t lanes_lt(uint32_t n) { return n >= 64u ? ~0ull : (1ull << n) - 1ull; }
 
 struct RegLds {
-    static constexpr uint32_t CCAP = 256;
-    uint32_t X[6][64];         // lane exchange: records (6 words), a new map (5), log entries (5)
+    static constexpr uint32_t CCAP = 128;
+    uint32_t X[7][64];         // lane exchange: records (7 words), a new map (6), log entries (6)
     uint32_t cx[64], cy[64];   // corner candidates of a coverage test
     uint32_t C[3 * CCAP];      // staged candidates (pred, succ access, domain << 24 | succ task)
+    uint32_t E[3 * CCAP];      // staged edges known unique, the same three words
     uint32_t ccount;
 };
 
-// a bin's state, one fragment and one log entry per lane
+// a bin's state, one fragment and one log entry per lane; fo / go: the access
+// of the fragment's writer / of the logged reader lies in this bin alone
 struct RegBin {
-    uint32_t fx0, fx1, fy0, fy1, flw;   // fragment (lane < nf)
-    uint32_t gx0, gx1, gy0, gy1, gacc;  // log entry (lane < ng)
+    uint32_t fx0, fx1, fy0, fy1, flw, fo;   // fragment (lane < nf)
+    uint32_t gx0, gx1, gy0, gy1, gacc, go;  // log entry (lane < ng)
 };
 
-// stages candidate (p, acc, task) of every lane of m; cc = candidates staged
-template <bool DIR>
-__device__ __forceinline__ void reg_append(const KArgs &a, RegLds &s, uint32_t &cc, uint64_t m, uint32_t p,
-                                           uint32_t acc, uint32_t task)
+// The staged edges known unique into the edge stripes: k_dedup gives them
+// positions without the table.
+__device__ __forceinline__ void flush_direct(const KArgs &a, RegLds &s, uint32_t n)
 {
-    if (!m) return;
-    const uint32_t n = popc(m);
-    if (cc + n > RegLds::CCAP) {
-        if (lane_id() == 0) s.ccount = cc;
-        flush_cands_to<DIR>(a, s);
-        cc = 0;
+    __syncthreads();  // the staged entries are in LDS
+    const uint32_t lane = lane_id();
+    const uint32_t stripe = CSTRIPES_C + blockIdx.x % (CSTRIPES - CSTRIPES_C);
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&a.cstr[16 * stripe], n);
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    if ((uint64_t)base + n > a.cand_cap) {
+        if (lane == 0) atomicOr(&a.ctr->err, ERR_CAND);
+    } else {
+        uint32_t *dst = a.cand + 3ull * ((size_t)stripe * a.cand_cap + base);
+        for (uint32_t i = lane; i < 3 * n; i += 64) dst[i] = s.E[i];
     }
-    if (lane_in(m)) {
-        const uint32_t pos = 3 * (cc + popc_below(m));
-        s.C[pos] = p;
-        s.C[pos + 1] = acc;
-        s.C[pos + 2] = task;
+    __syncthreads();  // E is refilled after this
+}
+
+// Stages (p, acc, task) of every lane of m: the lanes of md (a subset) as edges
+// known unique -- the predecessor's access lies in this bin alone and the
+// successor's task has one record here, so no other bin and no other record
+// can find this (pred, task) key -- the others as candidates for the dedup
+// table.  cc / ec: candidates / edges staged.  Staged edges leave LDS only
+// between bins (a bin that fails drops its own, see k_resolve_reg); when the
+// buffer is full they go on as candidates, which the table dedups just as well.
+template <bool DIR>
+__device__ __forceinline__ void reg_append(const KArgs &a, RegLds &s, uint32_t &cc, uint32_t &ec, uint64_t m,
+                                           uint64_t md, uint32_t p, uint32_t acc, uint32_t task)
+{
+    if (DIR || ec + popc(md) > RegLds::CCAP) md = 0;  // the direct form inserts everything into the table
+    const uint64_t mc = m & ~md;
+    if (mc) {
+        const uint32_t n = popc(mc);
+        if (cc + n > RegLds::CCAP) {
+            if (lane_id() == 0) s.ccount = cc;
+            flush_cands_to<DIR>(a, s);
+            cc = 0;
+        }
+        if (lane_in(mc)) {
+            const uint32_t pos = 3 * (cc + popc_below(mc));
+            s.C[pos] = p;
+            s.C[pos + 1] = acc;
+            s.C[pos + 2] = task;
+        }
+        cc += n;
     }
-    cc += n;
+    if (md) {
+        const uint32_t n = popc(md);
+        if (lane_in(md)) {
+            const uint32_t pos = 3 * (ec + popc_below(md));
+            s.E[pos] = p;
+            s.E[pos + 1] = acc;
+            s.E[pos + 2] = task;
+        }
+        ec += n;
+    }
 }
 
 // the lanes of m holding the first occurrence of each distinct value of v
@@ -1069,11 +1126,13 @@ __device__ __forceinline__ void reg_prune(RegLds &s, RegBin &b, uint32_t nf, uin
     if (lane_in(alive)) {
         const uint32_t o = popc_below(alive);
         s.X[0][o] = b.gx0; s.X[1][o] = b.gx1; s.X[2][o] = b.gy0; s.X[3][o] = b.gy1; s.X[4][o] = b.gacc;
+        s.X[5][o] = b.go;
     }
     __syncthreads();
     ng = popc(alive);
     if (lane < ng) {
         b.gx0 = s.X[0][lane]; b.gx1 = s.X[1][lane]; b.gy0 = s.X[2][lane]; b.gy1 = s.X[3][lane]; b.gacc = s.X[4][lane];
+        b.go = s.X[5][lane];
     }
     __syncthreads();
     gdirty = 0;
@@ -1083,11 +1142,11 @@ __device__ __forceinline__ void reg_prune(RegLds &s, RegBin &b, uint32_t nf, uin
 // under the rect survive with their writer, the rect becomes one fragment.
 // False when the map would exceed 64 fragments.
 __device__ __forceinline__ bool reg_paint(RegLds &s, RegBin &b, uint32_t &nf, uint32_t X0, uint32_t X1, uint32_t Y0,
-                                          uint32_t Y1, uint32_t acc, bool whole)
+                                          uint32_t Y1, uint32_t acc, uint32_t one, bool whole)
 {
     const uint32_t lane = lane_id();
     if (whole) {  // the rect is the whole bin: one fragment
-        if (lane == 0) { b.fx0 = X0; b.fx1 = X1; b.fy0 = Y0; b.fy1 = Y1; b.flw = acc; }
+        if (lane == 0) { b.fx0 = X0; b.fx1 = X1; b.fy0 = Y0; b.fy1 = Y1; b.flw = acc; b.fo = one; }
         nf = 1;
         return true;
     }
@@ -1100,33 +1159,42 @@ __device__ __forceinline__ bool reg_paint(RegLds &s, RegBin &b, uint32_t &nf, ui
     const uint32_t tot = popc(c1) + popc(c2) + popc(c3) + popc(c4);
     if (tot + 1 > 64u) return false;
     uint32_t o = popc_below(c1) + popc_below(c2) + popc_below(c3) + popc_below(c4);
+    auto put = [&](uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1) {
+        s.X[0][o] = x0; s.X[1][o] = x1; s.X[2][o] = y0; s.X[3][o] = y1; s.X[4][o] = b.flw; s.X[5][o] = b.fo;
+        ++o;
+    };
     if (live && !ov) {
-        s.X[0][o] = b.fx0; s.X[1][o] = b.fx1; s.X[2][o] = b.fy0; s.X[3][o] = b.fy1; s.X[4][o] = b.flw;
+        put(b.fx0, b.fx1, b.fy0, b.fy1);
     } else if (ov) {
         const uint32_t mx0 = max(b.fx0, X0), mx1 = min(b.fx1, X1);
-        if (pt) { s.X[0][o] = b.fx0; s.X[1][o] = X0; s.X[2][o] = b.fy0; s.X[3][o] = b.fy1; s.X[4][o] = b.flw; ++o; }
-        if (pbm) { s.X[0][o] = X1; s.X[1][o] = b.fx1; s.X[2][o] = b.fy0; s.X[3][o] = b.fy1; s.X[4][o] = b.flw; ++o; }
-        if (pl) { s.X[0][o] = mx0; s.X[1][o] = mx1; s.X[2][o] = b.fy0; s.X[3][o] = Y0; s.X[4][o] = b.flw; ++o; }
-        if (pr) { s.X[0][o] = mx0; s.X[1][o] = mx1; s.X[2][o] = Y1; s.X[3][o] = b.fy1; s.X[4][o] = b.flw; }
+        if (pt) put(b.fx0, X0, b.fy0, b.fy1);
+        if (pbm) put(X1, b.fx1, b.fy0, b.fy1);
+        if (pl) put(mx0, mx1, b.fy0, Y0);
+        if (pr) put(mx0, mx1, Y1, b.fy1);
     }
-    if (lane == 0) { s.X[0][tot] = X0; s.X[1][tot] = X1; s.X[2][tot] = Y0; s.X[3][tot] = Y1; s.X[4][tot] = acc; }
+    if (lane == 0) {
+        s.X[0][tot] = X0; s.X[1][tot] = X1; s.X[2][tot] = Y0; s.X[3][tot] = Y1; s.X[4][tot] = acc; s.X[5][tot] = one;
+    }
     __syncthreads();
     nf = tot + 1;
     if (lane < nf) {
         b.fx0 = s.X[0][lane]; b.fx1 = s.X[1][lane]; b.fy0 = s.X[2][lane]; b.fy1 = s.X[3][lane]; b.flw = s.X[4][lane];
+        b.fo = s.X[5][lane];
     }
     __syncthreads();
     return true;
 }
 
 // LINK of one record with the reader rule: the last writers of its points
+// (single: the record's task has one record in this bin)
 template <bool DIR>
-__device__ __forceinline__ void reg_link_reader(const KArgs &a, RegLds &s, uint32_t &cc, const RegBin &b, uint32_t nf,
-                                                uint32_t X0, uint32_t X1, uint32_t Y0, uint32_t Y1, uint32_t acc,
-                                                uint32_t task)
+__device__ __forceinline__ void reg_link_reader(const KArgs &a, RegLds &s, uint32_t &cc, uint32_t &ec, const RegBin &b,
+                                                uint32_t nf, uint32_t X0, uint32_t X1, uint32_t Y0, uint32_t Y1,
+                                                uint32_t acc, uint32_t task, bool single)
 {
     const bool ov = lane_id() < nf && b.flw != NONE && overlaps(b.fx0, b.fx1, b.fy0, b.fy1, X0, X1, Y0, Y1);
-    reg_append<DIR>(a, s, cc, distinct_first(__ballot(ov), b.flw), b.flw, acc, task);
+    const uint64_t first = distinct_first(__ballot(ov), b.flw);
+    reg_append<DIR>(a, s, cc, ec, first, single ? first & __ballot(b.fo != 0) : 0ull, b.flw, acc, task);
 }
 
 // LINK of one record with the writer rule.  Per overlapping fragment f and
@@ -1134,9 +1202,9 @@ __device__ __forceinline__ void reg_link_reader(const KArgs &a, RegLds &s, uint3
 // predecessor, and lw(f) is one when some point of Q has no such reader.
 // False when a coverage test would meet more than 63 readers.
 template <bool DIR>
-__device__ __forceinline__ bool reg_link_writer(const KArgs &a, RegLds &s, uint32_t &cc, const RegBin &b, uint32_t nf,
-                                                uint32_t ng, uint32_t X0, uint32_t X1, uint32_t Y0, uint32_t Y1,
-                                                uint32_t acc, uint32_t task)
+__device__ __forceinline__ bool reg_link_writer(const KArgs &a, RegLds &s, uint32_t &cc, uint32_t &ec, const RegBin &b,
+                                                uint32_t nf, uint32_t ng, uint32_t X0, uint32_t X1, uint32_t Y0,
+                                                uint32_t Y1, uint32_t acc, uint32_t task, bool single)
 {
     const uint32_t lane = lane_id();
     const bool ov = lane < nf && overlaps(b.fx0, b.fx1, b.fy0, b.fy1, X0, X1, Y0, Y1);
@@ -1183,8 +1251,11 @@ __device__ __forceinline__ bool reg_link_writer(const KArgs &a, RegLds &s, uint3
         if (popc(K) > 63u) return false;
         if (reg_uncovered(s, b, K, Qx0, Qx1, Qy0, Qy1)) elw |= 1ull << l;
     }
-    reg_append<DIR>(a, s, cc, PG, b.gacc, acc, task);
-    reg_append<DIR>(a, s, cc, distinct_first(elw, b.flw), b.flw, acc, task);
+    // log entries with one access id are the records of one access (a wrapped tile): never
+    // single-bin, so the edges known unique hold each reader once
+    reg_append<DIR>(a, s, cc, ec, PG, single ? PG & __ballot(b.go != 0) : 0ull, b.gacc, acc, task);
+    const uint64_t first = distinct_first(elw, b.flw);
+    reg_append<DIR>(a, s, cc, ec, first, single ? first & __ballot(b.fo != 0) : 0ull, b.flw, acc, task);
     return true;
 }
 
@@ -1199,7 +1270,7 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
     const uint32_t next = a.min_tier;  // the tier that takes the bins not replayed here
     unsigned long long visits = 0;
     uint32_t maxf = 0, maxg = 0;
-    uint32_t cc = 0;  // candidates staged in s.C
+    uint32_t cc = 0, ec = 0;  // candidates staged in s.C, edges known unique in s.E
     // The same software pipeline over the block's bins as k_resolve: list entries 64 at
     // a time, one per lane; the next bin's records, fragments and log loaded before this
     // bin's write-back.
@@ -1234,6 +1305,11 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
             continue;
         }
         bool ok = cnt <= 64u && nf0 <= 64u && ng0 <= 64u;
+        if (ec > RegLds::CCAP / 2) {  // room for this bin's edges known unique
+            flush_direct(a, s, ec);
+            ec = 0;
+        }
+        const uint32_t ec0 = ec;  // a bin that fails drops the edges it staged
         const DomDesc g = a.dom[dm];
         const uint32_t local = slot - g.bin0;
         const uint32_t bxi = local % g.nbx, byi = local / g.nbx;
@@ -1245,15 +1321,17 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
         uint32_t nf = nf0, ng = ng0, gdirty = ng0, fdirty = 0;
         b.fx0 = cur.f0; b.fx1 = cur.f1; b.fy0 = cur.f2; b.fy1 = cur.f3; b.flw = cur.f4;
         b.gx0 = cur.g0; b.gx1 = cur.g1; b.gy0 = cur.g2; b.gy1 = cur.g3; b.gacc = cur.g4;
+        b.fo = lane < nf0 ? ((lane < 32 ? cur.mk.x >> lane : cur.mk.y >> (lane - 32)) & 1u) : 0u;
+        b.go = lane < ng0 ? ((lane < 32 ? cur.mk.z >> lane : cur.mk.w >> (lane - 32)) & 1u) : 0u;
         if (nf0 == 0) {  // a bin never written: one fragment, no writer
-            b.fx0 = BX0; b.fx1 = BX1; b.fy0 = BY0; b.fy1 = BY1; b.flw = NONE;
+            b.fx0 = BX0; b.fx1 = BX1; b.fy0 = BY0; b.fy1 = BY1; b.flw = NONE; b.fo = 0;
             nf = 1;
             fdirty = 1;
         }
         // the records, clipped to the bin, in spawn order (rank by record index, distinct
         // within a bin; k_count's buckets are often in order already)
         uint32_t rx0 = max(cur.q0.x, BX0), rx1 = min(cur.q0.y, BX1), ry0 = max(cur.q0.z, BY0), ry1 = min(cur.q0.w, BY1);
-        uint32_t racc = cur.q1.x, rtm = cur.q1.y;
+        uint32_t racc = cur.q1.x, rtm = cur.q1.y, rone = cur.q1.w;
         if (ok) {
             const uint32_t key = lane < cnt ? cur.q1.z : NONE;
             const uint32_t pkey = __shfl_up(key, 1, 64);
@@ -1262,12 +1340,12 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
                 for (uint32_t j = 0; j < cnt; ++j) rank += rl(key, j) < key ? 1u : 0u;
                 if (lane < cnt) {
                     s.X[0][rank] = rx0; s.X[1][rank] = rx1; s.X[2][rank] = ry0; s.X[3][rank] = ry1;
-                    s.X[4][rank] = racc; s.X[5][rank] = rtm;
+                    s.X[4][rank] = racc; s.X[5][rank] = rtm; s.X[6][rank] = rone;
                 }
                 __syncthreads();
                 if (lane < cnt) {
                     rx0 = s.X[0][lane]; rx1 = s.X[1][lane]; ry0 = s.X[2][lane]; ry1 = s.X[3][lane];
-                    racc = s.X[4][lane]; rtm = s.X[5][lane];
+                    racc = s.X[4][lane]; rtm = s.X[5][lane]; rone = s.X[6][lane];
                 }
                 __syncthreads();
             }
@@ -1277,6 +1355,8 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
         const uint64_t Wm = __ballot(lane < cnt && (rmode & 2u));                   // writer-rule LINKs
         const uint64_t GS = __ballot(lane < cnt && (lane == 0 || ptask != rtask));  // task group starts
         const uint64_t RD = __ballot(lane < cnt && rmode == 1u);                    // exact readers (logged)
+        // records whose task has no other record in this bin
+        const uint64_t SG = cnt ? GS & ((GS >> 1) | (1ull << (cnt - 1))) : 0ull;
         uint32_t i = 0;
         while (ok && i < cnt) {
             const uint64_t wr = Wm & (~0ull << i);
@@ -1289,7 +1369,10 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
                 // Whole task groups with no writer: LINK reads only the writer map, PUT only
                 // appends readers, so every record links and the exact readers are then logged.
                 const uint64_t run = lanes_lt(end) & ~lanes_lt(i);
-                if (nf <= end - i) {  // fragments in turn, records in lanes
+                if (nf <= end - i) {
+                    // fragments in turn, records in lanes.  A writer's fragments are adjacent in
+                    // the map (a paint keeps the order of the pieces and appends the new writer,
+                    // whose id is the largest), so `last` emits each writer once per record.
                     const bool in = lane_in(run);
                     uint32_t last = NONE;  // the writer this record emitted last
                     for (uint32_t f = 0; f < nf; ++f) {
@@ -1297,13 +1380,14 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
                         if (lw == NONE) continue;
                         const bool ov = in && lw != last &&
                                         overlaps(rl(b.fx0, f), rl(b.fx1, f), rl(b.fy0, f), rl(b.fy1, f), rx0, rx1, ry0, ry1);
-                        reg_append<DIR>(a, s, cc, __ballot(ov), lw, racc, rtask | tdom);
+                        const uint64_t m = __ballot(ov);
+                        reg_append<DIR>(a, s, cc, ec, m, rl(b.fo, f) ? m & SG : 0ull, lw, racc, rtask | tdom);
                         last = ov ? lw : last;
                     }
                 } else {
                     for (uint32_t q = i; q < end; ++q)
-                        reg_link_reader<DIR>(a, s, cc, b, nf, rl(rx0, q), rl(rx1, q), rl(ry0, q), rl(ry1, q), rl(racc, q),
-                                             (rl(rtm, q) >> 8) | tdom);
+                        reg_link_reader<DIR>(a, s, cc, ec, b, nf, rl(rx0, q), rl(rx1, q), rl(ry0, q), rl(ry1, q),
+                                             rl(racc, q), (rl(rtm, q) >> 8) | tdom, (SG >> q) & 1ull);
                 }
                 const uint64_t rd = RD & run;
                 const uint32_t nr = popc(rd);
@@ -1318,11 +1402,12 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
                     if (lane_in(rd)) {
                         const uint32_t o = ng + popc_below(rd);
                         s.X[0][o] = rx0; s.X[1][o] = rx1; s.X[2][o] = ry0; s.X[3][o] = ry1; s.X[4][o] = racc;
+                        s.X[5][o] = rone;
                     }
                     __syncthreads();
                     if (lane >= ng && lane < ng + nr) {
                         b.gx0 = s.X[0][lane]; b.gx1 = s.X[1][lane]; b.gy0 = s.X[2][lane]; b.gy1 = s.X[3][lane];
-                        b.gacc = s.X[4][lane];
+                        b.gacc = s.X[4][lane]; b.go = s.X[5][lane];
                     }
                     __syncthreads();
                     ng += nr;
@@ -1334,20 +1419,22 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
             // the task group at i writes: LINK each record, then PUT each, in slot order
             const uint64_t nx = GS & ~lanes_lt(i + 1);
             end = nx ? min(ctz(nx), cnt) : cnt;
+            const bool single = end == i + 1;  // the group's only record in this bin
             for (uint32_t q = i; ok && q < end; ++q) {
                 const uint32_t X0 = rl(rx0, q), X1 = rl(rx1, q), Y0 = rl(ry0, q), Y1 = rl(ry1, q);
                 const uint32_t acc = rl(racc, q), tm = rl(rtm, q), task = (tm >> 8) | tdom;
                 if (tm & 2u)
-                    ok = reg_link_writer<DIR>(a, s, cc, b, nf, ng, X0, X1, Y0, Y1, acc, task);
+                    ok = reg_link_writer<DIR>(a, s, cc, ec, b, nf, ng, X0, X1, Y0, Y1, acc, task, single);
                 else
-                    reg_link_reader<DIR>(a, s, cc, b, nf, X0, X1, Y0, Y1, acc, task);
+                    reg_link_reader<DIR>(a, s, cc, ec, b, nf, X0, X1, Y0, Y1, acc, task, single);
             }
             for (uint32_t q = i; ok && q < end; ++q) {
                 const uint32_t mode = rl(rtm, q) & 0xffu;
                 if (mode & 0x80u) continue;  // a segment seen by this domain: LINK only
                 const uint32_t X0 = rl(rx0, q), X1 = rl(rx1, q), Y0 = rl(ry0, q), Y1 = rl(ry1, q), acc = rl(racc, q);
+                const uint32_t one = rl(rone, q);
                 if (mode & 2u) {
-                    ok = reg_paint(s, b, nf, X0, X1, Y0, Y1, acc, X0 == BX0 && X1 == BX1 && Y0 == BY0 && Y1 == BY1);
+                    ok = reg_paint(s, b, nf, X0, X1, Y0, Y1, acc, one, X0 == BX0 && X1 == BX1 && Y0 == BY0 && Y1 == BY1);
                     fdirty = 1;
                 } else if (mode == 1u) {
                     if (ng >= 64u) {
@@ -1357,7 +1444,7 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
                             break;
                         }
                     }
-                    if (lane == ng) { b.gx0 = X0; b.gx1 = X1; b.gy0 = Y0; b.gy1 = Y1; b.gacc = acc; }
+                    if (lane == ng) { b.gx0 = X0; b.gx1 = X1; b.gy0 = Y0; b.gy1 = Y1; b.gacc = acc; b.go = one; }
                     ++ng;
                 }
             }
@@ -1369,6 +1456,7 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
         bin_pre_load(a, e_cur, pre, true);  // before the write-back stores
         e_nxt = entry_k(k + 2);
         if (!ok) {  // the first LDS tier takes the bin from its HBM state
+            ec = ec0;
             if (lane == 0) {
                 const uint32_t pos = atomicAdd(&a.ctr->ntier[next], 1u);
                 a.active[(size_t)next * a.nbins + pos] = e;
@@ -1385,16 +1473,19 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
             uint32_t *q = gw + 5 * lane;
             q[0] = b.gx0; q[1] = b.gx1; q[2] = b.gy0; q[3] = b.gy1; q[4] = b.gacc;
         }
+        const uint64_t fm = __ballot(lane < nf && b.fo), gm = __ballot(lane < ng && b.go);
         if (lane == 0) {
             a.bin_nf[slot] = nf;
             a.bin_nrs[slot] = ng;
             a.bin_cnt[slot] = 0;
+            a.bin_one[slot] = make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)gm, (uint32_t)(gm >> 32));
         }
         maxf = max(maxf, nf);
         maxg = max(maxg, ng);
     }
     if (lane == 0) s.ccount = cc;
     flush_cands_to<DIR>(a, s);
+    if (ec) flush_direct(a, s, ec);
     if (lane == 0 && visits) {  // statistics, in the first tier's slots
         uint4 *o = &a.blk[next * BLKSLOTS + blockIdx.x % BLKSLOTS];
         atomicMax(&o->x, maxf);
@@ -1681,6 +1772,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
                 a.bin_nf[slot] = st.nf;
                 a.bin_nrs[slot] = st.ng;
                 a.bin_cnt[slot] = 0;
+                a.bin_one[slot] = make_uint4(0, 0, 0, 0);  // this replay keeps no single-bin marks
             }
             maxf = max(maxf, st.nf);
             maxg = max(maxg, st.ng);
@@ -1699,23 +1791,29 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
 // ---------------------------------------------------------------- dedup
 
 // k_dedup's LDS: where each candidate stripe starts in the launch's candidate
-// numbering, and the block's new-edge counts per wave for one edge-counter update
+// numbering, and the block's new-edge counts per wave
 constexpr uint32_t DEDUP_PER_THREAD = 4;  // candidates per thread and pass
+constexpr uint32_t DEDUP_COMPACT_MIN = 1u << 16;  // candidates from which a launch hashes into 2n slots
+constexpr uint32_t DEDUP_BLOCKS = 1024;   // k_dedup's and k_final's grids (staged form)
 struct DedupLds {
     uint32_t pre[CSTRIPES + 1];
     uint32_t wave[4];
-    unsigned long long base;
 };
 
-// Inserts the staged candidates into the dedup table.  New keys take edge
-// positions from the launch's edge counter, one update per block and pass of
-// 256 x DEDUP_PER_THREAD candidates: that counter is a single address, and one
-// update per wave serialised the kernel on it.
+// Inserts the staged candidates into the dedup table.  Block b lists the table
+// slots of the new keys it made in its own part of eslot (entries
+// [b * R, b * R + dcnt[b]), R = eslot size / DEDUP_BLOCKS) and k_final gives
+// them edge positions by a prefix over dcnt: no counter is shared (one edge
+// counter update per block and pass had every block of a large launch queue
+// on that one address).
 __global__ void __launch_bounds__(256) k_dedup(KArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     DedupLds &s = *reinterpret_cast<DedupLds *>(lds_raw);
-    if (a.ctr->err & ERR_CAND) return;  // candidates are incomplete
+    if (a.ctr->err & ERR_CAND) {  // candidates are incomplete (k_final sees the error)
+        if (threadIdx.x == 0) a.dcnt[blockIdx.x] = 0;
+        return;
+    }
     // candidates: stripe q holds cstr[q] of them from index q * cand_cap
     if (threadIdx.x == 0) {
         uint32_t t = 0;
@@ -1727,9 +1825,20 @@ __global__ void __launch_bounds__(256) k_dedup(KArgs a)
     }
     __syncthreads();
     const uint32_t n = s.pre[CSTRIPES];
-    const uint32_t mask = a.hmask;  // the whole table: spread keys keep concurrent atomics on distinct lines
-    const unsigned long long ebase = a.ctr->edge_base;
+    // A launch of many candidates hashes into the first 2^k >= 2n slots: the table is sized
+    // for the pool's largest launch (up to GBs), and a launch spread over all of it takes
+    // an HBM line per atomic; at 2n slots the table stays in the last-level cache.  Small
+    // launches keep the whole table (spread keys keep their few atomics on distinct lines).
+    uint32_t mask = a.hmask;
+    if (n >= DEDUP_COMPACT_MIN) {
+        uint32_t m = 1;
+        while (m < 2u * n) m <<= 1;
+        mask = min(mask, m - 1);
+    }
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t region = (a.hmask + 1) / DEDUP_BLOCKS;  // this block's part of eslot
+    uint32_t *mine = a.eslot + (size_t)blockIdx.x * region;
+    uint32_t bcount = 0;  // new keys this block made (block-uniform)
     // several candidates per thread only when the grid has more than one each: a
     // thread's inserts run one after the other
     const uint32_t nthreads = gridDim.x * blockDim.x;
@@ -1739,13 +1848,14 @@ __global__ void __launch_bounds__(256) k_dedup(KArgs a)
     for (uint32_t i0 = blockIdx.x * per_pass; i0 < n; i0 += gridDim.x * per_pass) {
         uint32_t h[DEDUP_PER_THREAD], acc[DEDUP_PER_THREAD];
         unsigned long long key[DEDUP_PER_THREAD], old[DEDUP_PER_THREAD];
-        bool live[DEDUP_PER_THREAD];
+        bool live[DEDUP_PER_THREAD], direct[DEDUP_PER_THREAD];
         // the first probe of each of the thread's candidates is issued before any
         // result is needed, so their atomics are in flight together
 #pragma unroll
         for (uint32_t k = 0; k < DEDUP_PER_THREAD; ++k) {
             const uint32_t i = i0 + k * blockDim.x + threadIdx.x;
             live[k] = k < dpt && i < n;
+            direct[k] = false;
             h[k] = 0;
             acc[k] = 0;
             key[k] = EMPTY64;
@@ -1753,18 +1863,26 @@ __global__ void __launch_bounds__(256) k_dedup(KArgs a)
             if (live[k]) {
                 uint32_t q = 0;
                 while (i >= s.pre[q + 1]) ++q;
-                const uint32_t *c = a.cand + 3ull * ((size_t)q * a.cand_cap + (i - s.pre[q]));
-                key[k] = ((unsigned long long)c[0] << 32) | c[2];
-                acc[k] = c[1];
-                h[k] = (uint32_t)mix64(key[k]) & mask;
-                old[k] = atomicCAS(&a.hkeys[h[k]], EMPTY64, key[k]);
+                const size_t ci = (size_t)q * a.cand_cap + (i - s.pre[q]);
+                if (q >= CSTRIPES_C) {
+                    // an edge known unique: its own position, no table entry
+                    h[k] = DIRECT_SLOT | (uint32_t)ci;
+                    live[k] = false;
+                    direct[k] = true;
+                } else {
+                    const uint32_t *c = a.cand + 3 * ci;
+                    key[k] = ((unsigned long long)c[0] << 32) | c[2];
+                    acc[k] = c[1];
+                    h[k] = (uint32_t)mix64(key[k]) & mask;
+                    old[k] = atomicCAS(&a.hkeys[h[k]], EMPTY64, key[k]);
+                }
             }
         }
         uint64_t nm[DEDUP_PER_THREAD];
         uint32_t wc = 0;
 #pragma unroll
         for (uint32_t k = 0; k < DEDUP_PER_THREAD; ++k) {
-            bool isnew = false;
+            bool isnew = direct[k];
             if (live[k]) {
                 // linear probing from the first slot on
                 uint32_t hh = h[k], probe = 0;
@@ -1788,30 +1906,67 @@ __global__ void __launch_bounds__(256) k_dedup(KArgs a)
         }
         if (lane == 0) s.wave[w] = wc;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t total = 0;
-            for (uint32_t u = 0; u < (blockDim.x >> 6); ++u) total += s.wave[u];
-            s.base = total ? atomicAdd(&a.ctr->edge_total, (unsigned long long)total) : 0ull;
+        uint32_t pos = bcount, total = 0;
+        for (uint32_t u = 0; u < (blockDim.x >> 6); ++u) {
+            pos += u < w ? s.wave[u] : 0u;
+            total += s.wave[u];
         }
-        __syncthreads();
-        unsigned long long pos = s.base;
-        for (uint32_t u = 0; u < w; ++u) pos += s.wave[u];
 #pragma unroll
         for (uint32_t k = 0; k < DEDUP_PER_THREAD; ++k) {
             if ((nm[k] >> lane) & 1ull) {
-                const unsigned long long e = pos + (unsigned long long)__popcll(nm[k] & ((1ull << lane) - 1ull));
-                if (e >= a.edge_cap || e - ebase > a.hmask) atomicOr(&a.ctr->err, ERR_EDGES);
-                else a.eslot[e - ebase] = h[k];
+                const uint32_t e = pos + (uint32_t)__popcll(nm[k] & ((1ull << lane) - 1ull));
+                if (e >= region) atomicOr(&a.ctr->err, ERR_EDGES);
+                else mine[e] = h[k];
             }
-            pos += (unsigned long long)__popcll(nm[k]);
+            pos += (uint32_t)__popcll(nm[k]);
         }
-        __syncthreads();  // s.wave and s.base are rewritten by the next pass
+        bcount += total;
+        __syncthreads();  // s.wave is rewritten by the next pass
     }
+    if (threadIdx.x == 0) a.dcnt[blockIdx.x] = bcount;
 }
 
-__global__ void k_final(KArgs a)
+// The staged form's edge count of the launch: the new keys of the k_dedup
+// blocks, and where block b's start (an exclusive prefix over dcnt, in LDS)
+struct FinalLds {
+    uint32_t part[4];
+    uint32_t start;      // this block's first edge, relative to the launch's first
+    uint32_t total;      // the launch's new edges
+};
+
+__global__ void __launch_bounds__(256) k_final(KArgs a)
 {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    FinalLds &s = *reinterpret_cast<FinalLds *>(lds_raw);
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool staged = a.cand_cap != 0;
+    const unsigned long long ebase = a.ctr->edge_base;
+    if (staged) {
+        // prefix over the k_dedup blocks' counts: 4 per thread, then over the 256 threads
+        uint32_t c[4], t = 0;
+        for (uint32_t k = 0; k < 4; ++k) {
+            c[k] = a.dcnt[4 * threadIdx.x + k];
+            t += c[k];
+        }
+        const uint32_t incl = wave_incl_scan(t);
+        if (lane_id() == 63) s.part[threadIdx.x >> 6] = incl;  // the four waves' totals
+        __syncthreads();
+        if ((blockIdx.x >> 2) == threadIdx.x) {
+            uint32_t st = incl - t;
+            for (uint32_t u = 0; u < (threadIdx.x >> 6); ++u) st += s.part[u];
+            for (uint32_t k = 0; k < (blockIdx.x & 3u); ++k) st += c[k];
+            s.start = st;
+        }
+        if (threadIdx.x == 0) s.total = s.part[0] + s.part[1] + s.part[2] + s.part[3];
+        __syncthreads();
+        // the launch's edges are [edge_base, edge_base + total): block 0's thread 0 stores the
+        // count below, before it writes the launch outcome (no fence: the same thread reads it)
+        if (tid == 0) {
+            const unsigned long long et = ebase + s.total;
+            if (et > a.edge_cap) atomicOr(&a.ctr->err, ERR_EDGES);
+            a.ctr->edge_total = et;
+        }
+    }
     if (blockIdx.x == 0) {
         // the launch's outcome to pinned host memory (seen once the launch's completion is):
         // the per-domain error bits first, four domain slots per word, then the counters
@@ -1831,17 +1986,63 @@ __global__ void k_final(KArgs a)
             __hip_atomic_store(&o->launch, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
-    if (a.ctr->err & (ERR_EDGES | ERR_HASH)) return;  // eslot is incomplete
-    const unsigned long long ebase = a.ctr->edge_base;
-    unsigned long long et = a.ctr->edge_total;
-    if (et > a.edge_cap) et = a.edge_cap;
-    unsigned long long n = et > ebase ? et - ebase : 0ull;
-    if (n > (unsigned long long)a.hmask + 1) n = (unsigned long long)a.hmask + 1;
     if (tid == 0) {
         unsigned long long nc = 0;
         for (uint32_t q = 0; q < CSTRIPES; ++q) nc += a.cstr[16 * q];
         a.ctr->cand_total += nc;
     }
+    if (staged) {
+        if (a.ctr->err & (ERR_EDGES | ERR_HASH | ERR_CAND)) return;  // eslot is incomplete
+        // this block writes the edges of k_dedup block blockIdx.x: its slots, at its start
+        const uint32_t region = (a.hmask + 1) / DEDUP_BLOCKS;
+        const uint32_t *mine = a.eslot + (size_t)blockIdx.x * region;
+        const uint32_t n = a.dcnt[blockIdx.x];
+        const unsigned long long e0 = ebase + s.start;
+        for (uint32_t i0 = threadIdx.x; i0 < n; i0 += 4 * blockDim.x) {
+            uint32_t h[4], v[4];
+            unsigned long long key[4];
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t i = i0 + k * blockDim.x;
+                h[k] = i < n ? mine[i] : 0u;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t i = i0 + k * blockDim.x;
+                key[k] = 0ull;
+                v[k] = 0u;
+                if (i < n && (h[k] & DIRECT_SLOT)) {  // an edge known unique: its staged words
+                    const uint32_t *c = a.cand + 3ull * (h[k] & ~DIRECT_SLOT);
+                    key[k] = ((unsigned long long)c[0] << 32) | c[2];
+                    v[k] = c[1];
+                } else if (i < n) {
+                    key[k] = a.hkeys[h[k]];
+                    v[k] = a.hvals[h[k]];
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t i = i0 + k * blockDim.x;
+                if (i < n) {
+                    const unsigned long long e = e0 + i;
+                    a.edges[2 * e] = (uint32_t)(key[k] >> 32);
+                    a.edges[2 * e + 1] = v[k];
+                    a.edom[e] = (uint32_t)(key[k] >> 24) & 0xffu;
+                    if (!(h[k] & DIRECT_SLOT)) {
+                        a.hkeys[h[k]] = EMPTY64;
+                        a.hvals[h[k]] = NONE;
+                    }
+                }
+            }
+        }
+        return;
+    }
+    // direct form: the resolve kernels gave each new key its position (eslot[e - edge_base])
+    if (a.ctr->err & (ERR_EDGES | ERR_HASH)) return;  // eslot is incomplete
+    unsigned long long et = a.ctr->edge_total;
+    if (et > a.edge_cap) et = a.edge_cap;
+    unsigned long long n = et > ebase ? et - ebase : 0ull;
+    if (n > (unsigned long long)a.hmask + 1) n = (unsigned long long)a.hmask + 1;
     // four edges per thread and pass, their table reads issued before any store (each
     // edge is a chain: position -> table slot -> key and successor)
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
@@ -1876,14 +2077,15 @@ __global__ void k_final(KArgs a)
 // ---------------------------------------------------------------- maintenance
 
 // forget the state of pool bins [b0, b0 + n); with c, also the pool's counters
-__global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint32_t b0, uint32_t n, DevCounters *c,
-                        uint4 *blk)
+__global__ void k_reset(uint32_t *nf, uint32_t *nrs, uint32_t *cnt, uint4 *one, uint32_t b0, uint32_t n,
+                        DevCounters *c, uint4 *blk)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t i = tid; i < n; i += gridDim.x * blockDim.x) {
         nf[b0 + i] = 0;
         nrs[b0 + i] = 0;
         cnt[b0 + i] = 0;
+        one[b0 + i] = make_uint4(0, 0, 0, 0);
     }
     if (c && blk)
         for (uint32_t i = tid; i < 4 * BLKSLOTS; i += gridDim.x * blockDim.x) blk[i] = make_uint4(0, 0, 0, 0);
@@ -1945,8 +2147,8 @@ __global__ void k_consume(DevCounters *c)
 // retire the accesses with id < h of the bins [b0, b0 + n): a fragment last
 // written by one becomes unwritten, its reader-log entries go (the log is
 // compacted in order), and the remaining ids drop by h.  One wave per bin.
-__global__ void k_retire(uint32_t *nf, uint32_t *nrs, uint32_t *fr, uint32_t *rs, uint32_t fcap, uint32_t rcap,
-                         uint32_t b0, uint32_t n, uint32_t h)
+__global__ void k_retire(uint32_t *nf, uint32_t *nrs, uint4 *one, uint32_t *fr, uint32_t *rs, uint32_t fcap,
+                         uint32_t rcap, uint32_t b0, uint32_t n, uint32_t h)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
@@ -1977,14 +2179,18 @@ __global__ void k_retire(uint32_t *nf, uint32_t *nrs, uint32_t *fr, uint32_t *rs
             }
             o += (uint32_t)__popcll(m);
         }
-        if (lane == 0) nrs[slot] = o;
+        if (lane == 0) {
+            nrs[slot] = o;
+            one[slot] = make_uint4(0, 0, 0, 0);  // entries moved: their single-bin marks are dropped
+        }
     }
 }
 
 // move a domain's grid to another range of the pool: old bin (bx, by) ->
 // new bin (bx, by + dy); the ranges are disjoint
-__global__ void k_regrid(uint32_t *nf, uint32_t *nrs, uint32_t *fr, uint32_t *rs, uint32_t fcap, uint32_t rcap,
-                         uint32_t ob0, uint32_t onbx, uint32_t onby, uint32_t nb0, uint32_t nnbx, uint32_t dy)
+__global__ void k_regrid(uint32_t *nf, uint32_t *nrs, uint4 *one, uint32_t *fr, uint32_t *rs, uint32_t fcap,
+                         uint32_t rcap, uint32_t ob0, uint32_t onbx, uint32_t onby, uint32_t nb0, uint32_t nnbx,
+                         uint32_t dy)
 {
     const uint32_t ob = blockIdx.x;
     if (ob >= onbx * onby) return;
@@ -1996,6 +2202,7 @@ __global__ void k_regrid(uint32_t *nf, uint32_t *nrs, uint32_t *fr, uint32_t *rs
     if (threadIdx.x == 0) {
         nf[n] = f;
         nrs[n] = r;
+        one[n] = one[o];
     }
 }
 
@@ -2053,6 +2260,7 @@ struct xkrt_hip_pool {
     uint32_t cap_bins = 0;
     std::vector<std::pair<uint32_t, uint32_t>> free_ranges;
     uint32_t *bin_nf = nullptr, *bin_nrs = nullptr, *frag = nullptr, *rs = nullptr;
+    uint4 *bin_one = nullptr;  // per bin: single-bin marks of fragments and log entries
     uint32_t *bin_cnt = nullptr, *bin_dom = nullptr, *touched = nullptr;
     uint4 *active = nullptr;
     // domains
@@ -2072,6 +2280,7 @@ struct xkrt_hip_pool {
     uint32_t *hvals = nullptr;
     uint32_t hmask = 0;
     uint32_t *eslot = nullptr;
+    uint32_t *dcnt = nullptr;  // new keys per k_dedup block
     uint32_t *edges = nullptr, *edom = nullptr;  // device addresses (of host memory with host_edges)
     uint32_t *edges_h = nullptr, *edom_h = nullptr;  // host_edges: the same buffers' host addresses
     uint64_t edge_cap = 0;
@@ -2159,7 +2368,7 @@ namespace {
 
 uint64_t bin_bytes(const xkrt_hip_pool *p)
 {
-    return 4ull * 5 + 64ull + 20ull * p->fcap + 20ull * p->rcap + 32ull * p->lmax;
+    return 4ull * 5 + 64ull + 16ull + 20ull * p->fcap + 20ull * p->rcap + 32ull * p->lmax;
 }
 
 hipEvent_t take_event(xkrt_hip_pool *p)
@@ -2231,8 +2440,9 @@ int grow_bins(xkrt_hip_pool *p, uint32_t need)
     }
     HIPCHK(hipStreamSynchronize(p->stream));
     uint32_t *nf, *nrs, *fr, *rs, *cnt, *dom, *tch;
-    uint4 *act, *brec;
+    uint4 *act, *brec, *one;
     HIPCHK(hipMalloc(&nf, nc * 4));
+    HIPCHK(hipMalloc(&one, nc * 16));
     HIPCHK(hipMalloc(&nrs, nc * 4));
     HIPCHK(hipMalloc(&fr, nc * p->fcap * 20ull));
     HIPCHK(hipMalloc(&rs, nc * p->rcap * 20ull));
@@ -2243,22 +2453,25 @@ int grow_bins(xkrt_hip_pool *p, uint32_t need)
     HIPCHK(hipMalloc(&act, nc * 16 * 4));
     HIPCHK(hipMemsetAsync(nf, 0, nc * 4, p->stream));
     HIPCHK(hipMemsetAsync(nrs, 0, nc * 4, p->stream));
+    HIPCHK(hipMemsetAsync(one, 0, nc * 16, p->stream));
     HIPCHK(hipMemsetAsync(cnt, 0, nc * 4, p->stream));
     HIPCHK(hipMemsetAsync(dom, 0, nc * 4, p->stream));
     const uint64_t oc = p->cap_bins;
     if (oc) {
         HIPCHK(hipMemcpyAsync(nf, p->bin_nf, oc * 4, hipMemcpyDeviceToDevice, p->stream));
         HIPCHK(hipMemcpyAsync(nrs, p->bin_nrs, oc * 4, hipMemcpyDeviceToDevice, p->stream));
+        HIPCHK(hipMemcpyAsync(one, p->bin_one, oc * 16, hipMemcpyDeviceToDevice, p->stream));
         HIPCHK(hipMemcpyAsync(fr, p->frag, oc * p->fcap * 20ull, hipMemcpyDeviceToDevice, p->stream));
         HIPCHK(hipMemcpyAsync(rs, p->rs, oc * p->rcap * 20ull, hipMemcpyDeviceToDevice, p->stream));
         HIPCHK(hipMemcpyAsync(cnt, p->bin_cnt, oc * 4, hipMemcpyDeviceToDevice, p->stream));
         HIPCHK(hipMemcpyAsync(dom, p->bin_dom, oc * 4, hipMemcpyDeviceToDevice, p->stream));
         HIPCHK(hipStreamSynchronize(p->stream));
-        (void)hipFree(p->bin_nf); (void)hipFree(p->bin_nrs); (void)hipFree(p->frag); (void)hipFree(p->rs);
+        (void)hipFree(p->bin_nf); (void)hipFree(p->bin_nrs); (void)hipFree(p->bin_one); (void)hipFree(p->frag);
+        (void)hipFree(p->rs);
         (void)hipFree(p->bin_cnt); (void)hipFree(p->bin_dom); (void)hipFree(p->active);
         (void)hipFree(p->touched); (void)hipFree(p->binrec);
     }
-    p->bin_nf = nf; p->bin_nrs = nrs; p->frag = fr; p->rs = rs;
+    p->bin_nf = nf; p->bin_nrs = nrs; p->bin_one = one; p->frag = fr; p->rs = rs;
     p->bin_cnt = cnt; p->bin_dom = dom; p->active = act; p->touched = tch; p->binrec = brec;
     // the new tail is free
     if (!p->free_ranges.empty() && p->free_ranges.back().first + p->free_ranges.back().second == oc)
@@ -2282,7 +2495,8 @@ int alloc_bins(xkrt_hip_pool *p, uint32_t n, uint32_t dom, uint32_t *b0)
             r.second -= n;
             if (r.second == 0) p->free_ranges.erase(p->free_ranges.begin() + (long)i);
             hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>((n + 255) / 256, 4096u)), dim3(256), 0, p->stream,
-                               p->bin_nf, p->bin_nrs, p->bin_cnt, *b0, n, (DevCounters *)nullptr, (uint4 *)nullptr);
+                               p->bin_nf, p->bin_nrs, p->bin_cnt, p->bin_one, *b0, n, (DevCounters *)nullptr,
+                               (uint4 *)nullptr);
             hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>((n + 255) / 256, 4096u)), dim3(256), 0,
                                p->stream, p->bin_dom + *b0, n, dom);
             return hip_err(hipGetLastError());
@@ -2348,8 +2562,8 @@ int ensure_grid(xkrt_hip_dep_domain *d, uint32_t ylo, uint32_t yhi)
     if (rc) return rc;
     if (d->nby) {
         const uint32_t dy = (uint32_t)((olo - nlo) / BY);
-        hipLaunchKernelGGL(k_regrid, dim3(d->nbx * d->nby), dim3(256), 0, p->stream, p->bin_nf, p->bin_nrs, p->frag,
-                           p->rs, p->fcap, p->rcap, d->bin0, d->nbx, d->nby, nb0, d->nbx, dy);
+        hipLaunchKernelGGL(k_regrid, dim3(d->nbx * d->nby), dim3(256), 0, p->stream, p->bin_nf, p->bin_nrs,
+                           p->bin_one, p->frag, p->rs, p->fcap, p->rcap, d->bin0, d->nbx, d->nby, nb0, d->nbx, dy);
         HIPCHK(hipGetLastError());
         free_bins(p, d->bin0, d->nbx * d->nby);
     }
@@ -2368,7 +2582,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.nrec_ptr = (const uint32_t *)p->rec_dev_hdr;
     a.dom = p->d_dom;
     a.bin_dom = p->bin_dom;
-    a.bin_nf = p->bin_nf; a.bin_nrs = p->bin_nrs; a.frag = p->frag; a.rs = p->rs;
+    a.bin_nf = p->bin_nf; a.bin_nrs = p->bin_nrs; a.bin_one = p->bin_one; a.frag = p->frag; a.rs = p->rs;
     a.fcap = p->fcap; a.rcap = p->rcap;
     a.bin_cnt = p->bin_cnt; a.active = p->active; a.binrec = p->binrec;
     a.touched = p->touched;
@@ -2380,7 +2594,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.reg_first = 0;
     a.cand = p->cand; a.cand_cap = p->cand_cap; a.cstr = p->cstr; a.blk = p->blk;
     a.hkeys = p->hkeys; a.hvals = p->hvals; a.hmask = p->hmask;
-    a.eslot = p->eslot; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
+    a.eslot = p->eslot; a.dcnt = p->dcnt; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
     a.ctr = p->ctr;
     a.dom_err = p->dom_err;
     a.snap = p->snap;
@@ -2432,7 +2646,7 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
 }
 
 // k_resolve_reg: blocks per CU (its VGPRs allow 7 waves per SIMD; its LDS, 2.8 KB, many more)
-constexpr uint32_t REG_PER_CU = 28;
+constexpr uint32_t REG_PER_CU = 20;
 
 void launch_reg(xkrt_hip_pool *p, const KArgs &a, std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans,
                 uint32_t over)
@@ -2483,8 +2697,9 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint64_t nrec,
     if (!ak.skip1) launch_resolve<64, 128, 256>(p, ak, 1, spans);
     if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, ak, 2, spans);
     if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, ak, 3, spans);
-    if (!p->cfg.direct_dedup) launch(p, spans, 4, k_dedup, dim3(1024), dim3(256), (uint32_t)sizeof(DedupLds), a);
-    launch(p, spans, 5, k_final, dim3(1024), dim3(256), 0, a);
+    if (!p->cfg.direct_dedup)
+        launch(p, spans, 4, k_dedup, dim3(DEDUP_BLOCKS), dim3(256), (uint32_t)sizeof(DedupLds), a);
+    launch(p, spans, 5, k_final, dim3(DEDUP_BLOCKS), dim3(256), (uint32_t)sizeof(FinalLds), a);
 }
 
 int push_domains(xkrt_hip_pool *p)
@@ -2659,9 +2874,9 @@ void destroy_pool(xkrt_hip_pool *p)
             (void)hipEventDestroy(b);
         }
     for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
-    void *dev[] = {p->bin_nf, p->bin_nrs, p->frag, p->rs, p->bin_cnt, p->bin_dom, p->active, p->touched,
+    void *dev[] = {p->bin_nf, p->bin_nrs, p->bin_one, p->frag, p->rs, p->bin_cnt, p->bin_dom, p->active, p->touched,
                    p->d_dom, p->dom_err,
-                   p->binrec, p->cand, p->cstr, p->blk, p->hkeys, p->hvals, p->eslot, p->ctr,
+                   p->binrec, p->cand, p->cstr, p->blk, p->hkeys, p->hvals, p->eslot, p->dcnt, p->ctr,
                    p->gate_count,
                    p->rec_dev_hdr};
     for (void *q : dev)
@@ -2743,6 +2958,7 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     chk(hipMalloc(&p->hkeys, 8ull * hsz));
     chk(hipMalloc(&p->hvals, 4ull * hsz));
     chk(hipMalloc(&p->eslot, 4ull * hsz));
+    chk(hipMalloc(&p->dcnt, 4ull * DEDUP_BLOCKS));
     if (p->cfg.host_edges) {
         // k_final's edge stores land in pinned host memory: a launch's edges are on the host
         // when the launch completes, with no copy
@@ -3597,8 +3813,8 @@ int xkrt_hip_dep_retire(xkrt_hip_dep_domain_t d, uint64_t horizon)
     const uint32_t nb = d->nbx * d->nby;
     if (nb) {
         const uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)nb + 3) / 4, 65536);
-        hipLaunchKernelGGL(k_retire, dim3(grid), dim3(256), 0, p->stream, p->bin_nf, p->bin_nrs, p->frag, p->rs,
-                           p->fcap, p->rcap, d->bin0, nb, (uint32_t)horizon);
+        hipLaunchKernelGGL(k_retire, dim3(grid), dim3(256), 0, p->stream, p->bin_nf, p->bin_nrs, p->bin_one, p->frag,
+                           p->rs, p->fcap, p->rcap, d->bin0, nb, (uint32_t)horizon);
         rc = hip_err(hipGetLastError());
         if (rc) return rc;
     }
@@ -3626,10 +3842,10 @@ int xkrt_hip_dep_reset(xkrt_hip_dep_domain_t d)
     DevCounters *c = d->private_pool ? p->ctr : nullptr;
     if (nb) {
         hipLaunchKernelGGL(k_reset, dim3(std::min<uint32_t>((nb + 255) / 256, 4096u)), dim3(256), 0, p->stream,
-                           p->bin_nf, p->bin_nrs, p->bin_cnt, d->bin0, nb, c, p->blk);
+                           p->bin_nf, p->bin_nrs, p->bin_cnt, p->bin_one, d->bin0, nb, c, p->blk);
     } else if (c) {
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, p->stream, p->bin_nf, p->bin_nrs, p->bin_cnt, 0u, 0u, c,
-                           p->blk);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, p->stream, p->bin_nf, p->bin_nrs, p->bin_cnt, p->bin_one,
+                           0u, 0u, c, p->blk);
     }
     HIPCHK(hipGetLastError());
     if (d->private_pool) p->poisoned = false;
