@@ -902,7 +902,7 @@ struct BinPre {
     uint4 q0, q1;
     uint32_t f0, f1, f2, f3, f4;
     uint32_t g0, g1, g2, g3, g4;
-    uint4 mk;  // the bin's single-bin marks (with the log)
+    uint32_t mf, mg;  // this lane's word of the bin's single-bin marks (with the log)
 };
 
 // With `log`, the reader log of a bin of at most 64 entries comes too (the
@@ -916,8 +916,12 @@ __device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p,
     p.q1 = make_uint4(NONE, 0, NONE, 0);
     p.f0 = p.f1 = p.f2 = p.f3 = p.f4 = 0;
     p.g0 = p.g1 = p.g2 = p.g3 = p.g4 = 0;
-    p.mk = make_uint4(0, 0, 0, 0);
-    if (log && (cnt || nf0)) p.mk = a.bin_one[slot];
+    p.mf = p.mg = 0;
+    if (log && (cnt || nf0)) {
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(a.bin_one + slot);
+        p.mf = w[i >> 5];
+        p.mg = w[2 + (i >> 5)];
+    }
     if (i < cnt && i < a.lmax) {
         const uint4 *ir = a.binrec + 2ull * ((size_t)slot * a.lmax + i);
         p.q0 = ir[0];
@@ -1260,7 +1264,7 @@ __device__ __forceinline__ bool reg_link_writer(const KArgs &a, RegLds &s, uint3
 }
 
 template <bool DIR>
-__global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
+__global__ void __launch_bounds__(64, 7) k_resolve_reg(KArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     RegLds &s = *reinterpret_cast<RegLds *>(lds_raw);
@@ -1321,8 +1325,8 @@ __global__ void __launch_bounds__(64) k_resolve_reg(KArgs a)
         uint32_t nf = nf0, ng = ng0, gdirty = ng0, fdirty = 0;
         b.fx0 = cur.f0; b.fx1 = cur.f1; b.fy0 = cur.f2; b.fy1 = cur.f3; b.flw = cur.f4;
         b.gx0 = cur.g0; b.gx1 = cur.g1; b.gy0 = cur.g2; b.gy1 = cur.g3; b.gacc = cur.g4;
-        b.fo = lane < nf0 ? ((lane < 32 ? cur.mk.x >> lane : cur.mk.y >> (lane - 32)) & 1u) : 0u;
-        b.go = lane < ng0 ? ((lane < 32 ? cur.mk.z >> lane : cur.mk.w >> (lane - 32)) & 1u) : 0u;
+        b.fo = lane < nf0 ? (cur.mf >> (lane & 31u)) & 1u : 0u;
+        b.go = lane < ng0 ? (cur.mg >> (lane & 31u)) & 1u : 0u;
         if (nf0 == 0) {  // a bin never written: one fragment, no writer
             b.fx0 = BX0; b.fx1 = BX1; b.fy0 = BY0; b.fy1 = BY1; b.flw = NONE; b.fo = 0;
             nf = 1;
@@ -2646,12 +2650,15 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
 }
 
 // k_resolve_reg: blocks per CU (its VGPRs allow 7 waves per SIMD; its LDS, 2.8 KB, many more)
-constexpr uint32_t REG_PER_CU = 20;
+constexpr uint32_t REG_PER_CU = 28;
 
+// k_resolve_reg's blocks stride over the launch's touched list: a small launch gets at most one
+// block per record
 void launch_reg(xkrt_hip_pool *p, const KArgs &a, std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans,
-                uint32_t over)
+                uint32_t over, uint64_t nrec)
 {
-    const uint32_t blocks = std::min<uint32_t>(p->cap_bins, 256u * REG_PER_CU * over);
+    const uint32_t blocks = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>({(uint64_t)p->cap_bins, 256ull * REG_PER_CU * over, std::max<uint64_t>(nrec, 256)}));
     if (p->cfg.direct_dedup)
         launch(p, spans, 3, k_resolve_reg<true>, dim3(blocks), dim3(64), (uint32_t)sizeof(RegLds), a);
     else
@@ -2688,10 +2695,8 @@ void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint64_t nrec,
     const uint32_t over0 = nrec >= (1u << 16) ? 8u : 3u;
     // the register replay first: it takes the touched bins and passes those it cannot hold
     // to the first LDS tier, which then walks only them
-    if (!small) {
-        ak.reg_first = 1;
-        launch_reg(p, ak, spans, over0);
-    }
+    ak.reg_first = 1;
+    launch_reg(p, ak, spans, small ? 1u : over0, nrec);
     // tier 0 first; bins that outgrow a tier are retried by the next one
     if (!small) launch_resolve<64, 128, 64>(p, ak, 0, spans, nullptr, 1);
     if (!ak.skip1) launch_resolve<64, 128, 256>(p, ak, 1, spans);
