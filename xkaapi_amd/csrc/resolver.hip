@@ -9,7 +9,9 @@
 //   * each bin holds a writer painter map -- fragments tiling the bin's
 //     written area, each with its last writer (access id) -- and a log of the
 //     exact readers clipped to the bin (AoS, 5 u32 per fragment / entry:
-//     x0, x1, y0, y1, lw or acc; fcap fragments and rcap entries per bin);
+//     x0, x1, y0, y1, lw or acc; fcap fragments and rcap entries per bin),
+//     and two 64-bit single-bin masks (fragment / entry i: its access lies
+//     in this bin alone; kept by the register replay, cleared elsewhere);
 //   * rect records of a launch are an array of 32 B structs, two 16-byte
 //     words each {x0, x1, y0, y1}, {acc, task<<8|mode, domain, 0}: one record
 //     is two dwordx4 loads by one lane (the record an incidence refers to is
@@ -20,22 +22,31 @@
 // One launch resolves one batch of records in these kernels:
 //   k_count   records -> per-bin record counts; each incidence goes into its
 //             bin's bucket, each bin the launch touches into the touched list
-//   k_resolve one kernel per LDS tier, one wave64 per bin: the first tier takes
-//             the touched list, a bin that does not fit a tier (with headroom)
-//             goes on to the next one's list.  The bin's fragments, reader log and
-//             incidences move to LDS, its records are put in spawn order
-//             (rank sort up to 64, bitonic on 64-bit keys above) and replayed
-//             -- LINK of every access of a task, then PUT -- emitting
-//             (pred, succ access, succ task) candidates; W paints over the
+//   k_resolve_reg  one wave64 per touched bin of at most 64 records, fragments
+//             and log entries: the bin's whole state in registers (one
+//             fragment, log entry and record per lane), replayed -- LINK of
+//             every access of a task, then PUT -- with readlane and ballots.
+//             Candidates (pred, succ access, succ task) are staged; an edge
+//             whose predecessor's access lies in this bin alone, found for a
+//             task with one record here, is unique and staged as an edge.  A
+//             bin that does not fit goes on to the first LDS tier's list.
+//   k_resolve one kernel per LDS tier, one wave64 per bin, on the bins passed
+//             on: a bin that does not fit a tier (with headroom) goes on to the
+//             next one's list.  The bin's fragments, reader log and incidences
+//             move to LDS, its records are put in spawn order (rank sort up to
+//             64, bitonic on 64-bit keys above) and replayed; W paints over the
 //             writer map, an exact R appends to the reader log; the bin is
-//             written back compacted.  Candidates are deduplicated per bin
-//             in LDS, then either staged in a candidate buffer (throughput
-//             form) or inserted into the launch's table at once (direct
-//             form: one kernel less per launch, for small launches)
+//             written back compacted.  Candidates are deduplicated per bin in
+//             LDS, then either staged in a candidate buffer (throughput form)
+//             or inserted into the launch's table at once (direct form: one
+//             kernel less per launch)
 //   k_dedup   (staged form) candidates -> open-addressing table keyed
 //             (pred, succ task), keeping the lowest succ access id
-//             (= lowest slot, since LINK runs in slot order)
-//   k_final   table entries -> (pred id, succ id) edges; table entries reset;
+//             (= lowest slot, since LINK runs in slot order); edges known
+//             unique skip the table.  Each block lists its new keys and edges
+//             in its own part of eslot
+//   k_final   edge positions by a prefix over the blocks' counts; table entries
+//             and unique edges -> (pred id, succ id) edges; table entries reset;
 //             the launch's edge count and error bits to pinned host memory
 //
 // Because every fragment of a bin carries a uniform state, the edges found
