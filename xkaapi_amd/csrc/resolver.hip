@@ -974,6 +974,7 @@ __device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t l)
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
 }
 __device__ __forceinline__ uint32_t popc(uint64_t m) { return (uint32_t)__popcll(m); }
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 __device__ __forceinline__ uint32_t ctz(uint64_t m) { return (uint32_t)__builtin_ctzll(m); }
 // set bits of m below this lane
 __device__ __forceinline__ uint32_t popc_below(uint64_t m)
@@ -983,6 +984,8 @@ __device__ __forceinline__ uint32_t popc_below(uint64_t m)
 __device__ __forceinline__ bool lane_in(uint64_t m) { return (m >> lane_id()) & 1ull; }
 __device__ __forceinline__ uint64_t lanes_lt(uint32_t n) { return n >= 64u ? ~0ull : (1ull << n) - 1ull; }
 
+constexpr uint32_t REG_RECS = 256;  // records of a bin the register replay takes (in chunks of 64)
+
 struct RegLds {
     static constexpr uint32_t CCAP = 128;
     uint32_t X[7][64];         // lane exchange: records (7 words), a new map (6), log entries (6)
@@ -990,6 +993,8 @@ struct RegLds {
     uint32_t C[3 * CCAP];      // staged candidates (pred, succ access, domain << 24 | succ task)
     uint32_t E[3 * CCAP];      // staged edges known unique, the same three words
     uint32_t ccount;
+    uint8_t ord[REG_RECS];     // a bin of more than 64 records: bucket positions in spawn order
+    uint4 win[64];             // list entries of the block's next 64 bins
 };
 
 // a bin's state, one fragment and one log entry per lane; fo / go: the access
@@ -1275,7 +1280,7 @@ __device__ __forceinline__ bool reg_link_writer(const KArgs &a, RegLds &s, uint3
 }
 
 template <bool DIR>
-__global__ void __launch_bounds__(64, 7) k_resolve_reg(KArgs a)
+__global__ void __launch_bounds__(64, 6) k_resolve_reg(KArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     RegLds &s = *reinterpret_cast<RegLds *>(lds_raw);
@@ -1295,15 +1300,20 @@ __global__ void __launch_bounds__(64, 7) k_resolve_reg(KArgs a)
         const uint32_t slot = a.touched[x];
         return make_uint4(slot, 0u, a.bin_cnt[slot], a.bin_nf[slot] | (a.bin_nrs[slot] << 11) | (a.bin_dom[slot] << 24));
     };
+    // the entries of bins k in [wbase, wbase + 64), one per lane, kept in LDS (s.win)
     uint32_t wbase = 0;
-    uint4 ew = entry(blockIdx.x + lane * G);
+    s.win[lane] = entry(blockIdx.x + lane * G);
+    __syncthreads();
     auto entry_k = [&](uint32_t k) {
         if (k >= wbase + 64) {
             wbase += 64;
-            ew = entry(blockIdx.x + (wbase + lane) * G);
+            const uint4 e = entry(blockIdx.x + (wbase + lane) * G);
+            __syncthreads();
+            s.win[lane] = e;
+            __syncthreads();
         }
-        const uint32_t l = k - wbase;
-        return make_uint4(rl(ew.x, l), rl(ew.y, l), rl(ew.z, l), rl(ew.w, l));
+        const uint4 e = s.win[k - wbase];
+        return make_uint4(uni(e.x), uni(e.y), uni(e.z), uni(e.w));
     };
     uint4 e_cur = entry_k(0), e_nxt = entry_k(1);
     BinPre pre;
@@ -1319,7 +1329,7 @@ __global__ void __launch_bounds__(64, 7) k_resolve_reg(KArgs a)
             if (lane == 0) a.bin_cnt[slot] = 0;
             continue;
         }
-        bool ok = cnt <= 64u && nf0 <= 64u && ng0 <= 64u;
+        bool ok = cnt <= REG_RECS && cnt <= a.lmax && nf0 <= 64u && ng0 <= 64u;
         if (ec > RegLds::CCAP / 2) {  // room for this bin's edges known unique
             flush_direct(a, s, ec);
             ec = 0;
@@ -1333,7 +1343,7 @@ __global__ void __launch_bounds__(64, 7) k_resolve_reg(KArgs a)
         const uint32_t tdom = dm << 24;
 
         RegBin b;
-        uint32_t nf = nf0, ng = ng0, gdirty = ng0, fdirty = 0;
+        uint32_t nf = nf0, ng = ng0, gdirty = ng0, fdirty = 0, painted = 0;
         b.fx0 = cur.f0; b.fx1 = cur.f1; b.fy0 = cur.f2; b.fy1 = cur.f3; b.flw = cur.f4;
         b.gx0 = cur.g0; b.gx1 = cur.g1; b.gy0 = cur.g2; b.gy1 = cur.g3; b.gacc = cur.g4;
         b.fo = lane < nf0 ? (cur.mf >> (lane & 31u)) & 1u : 0u;
@@ -1343,11 +1353,53 @@ __global__ void __launch_bounds__(64, 7) k_resolve_reg(KArgs a)
             nf = 1;
             fdirty = 1;
         }
+        // A bin of more than 64 records is put in spawn order in LDS (s.ord: bucket positions by
+        // rank of record index) and replayed in chunks of up to 64 records that end at task
+        // group boundaries; the fragments and the log stay in registers across chunks.
+        const bool big = cnt > 64u;
+        const uint4 *bucket = a.binrec + 2ull * (size_t)slot * a.lmax;
+        if (ok && big) {
+            // the record indices in LDS (position p at X[p / 64][p % 64]), then each position's
+            // rank among them, one column of 64 positions at a time
+            const uint32_t ncol = (cnt + 63) / 64;
+            for (uint32_t j = 0; j < ncol; ++j) {
+                const uint32_t p = lane + 64 * j;
+                s.X[j][lane] = p < cnt ? bucket[2 * p + 1].z : NONE;
+            }
+            __syncthreads();
+            bool sorted = true;
+            for (uint32_t j = 0; j < ncol; ++j) {
+                const uint32_t p = lane + 64 * j;
+                sorted = sorted && (p == 0 || p >= cnt || s.X[(p - 1) >> 6][(p - 1) & 63u] < s.X[j][lane]);
+            }
+            for (uint32_t j = 0; j < ncol; ++j) {
+                const uint32_t p = lane + 64 * j, key = s.X[j][lane];
+                uint32_t rank = p;
+                if (__ballot(!sorted)) {
+                    rank = 0;
+                    for (uint32_t q = 0; q < cnt; ++q) rank += s.X[q >> 6][q & 63u] < key ? 1u : 0u;
+                }
+                if (p < cnt) s.ord[rank] = (uint8_t)p;
+            }
+            __syncthreads();
+        }
+        for (uint32_t c0 = 0; ok && c0 < cnt;) {
+        const uint32_t m = min(64u, cnt - c0);  // records of this chunk before its cut
         // the records, clipped to the bin, in spawn order (rank by record index, distinct
         // within a bin; k_count's buckets are often in order already)
-        uint32_t rx0 = max(cur.q0.x, BX0), rx1 = min(cur.q0.y, BX1), ry0 = max(cur.q0.z, BY0), ry1 = min(cur.q0.w, BY1);
-        uint32_t racc = cur.q1.x, rtm = cur.q1.y, rone = cur.q1.w;
-        if (ok) {
+        uint4 q0 = cur.q0, q1 = cur.q1;
+        if (big) {
+            q0 = make_uint4(0, 0, 0, 0);
+            q1 = make_uint4(NONE, 0, NONE, 0);
+            if (lane < m) {
+                const uint32_t p = s.ord[c0 + lane];
+                q0 = bucket[2 * p];
+                q1 = bucket[2 * p + 1];
+            }
+        }
+        uint32_t rx0 = max(q0.x, BX0), rx1 = min(q0.y, BX1), ry0 = max(q0.z, BY0), ry1 = min(q0.w, BY1);
+        uint32_t racc = q1.x, rtm = q1.y, rone = q1.w;
+        if (!big) {
             const uint32_t key = lane < cnt ? cur.q1.z : NONE;
             const uint32_t pkey = __shfl_up(key, 1, 64);
             if (__ballot(lane > 0 && lane < cnt && pkey > key)) {
@@ -1367,15 +1419,28 @@ __global__ void __launch_bounds__(64, 7) k_resolve_reg(KArgs a)
         }
         const uint32_t rtask = rtm >> 8, rmode = rtm & 0xffu;
         const uint32_t ptask = __shfl_up(rtask, 1, 64);
-        const uint64_t Wm = __ballot(lane < cnt && (rmode & 2u));                   // writer-rule LINKs
-        const uint64_t GS = __ballot(lane < cnt && (lane == 0 || ptask != rtask));  // task group starts
-        const uint64_t RD = __ballot(lane < cnt && rmode == 1u);                    // exact readers (logged)
+        uint32_t nc = m;  // records of this chunk: it ends at a task group boundary
+        if (c0 + m < cnt) {
+            const uint32_t ntask = bucket[2 * s.ord[c0 + m] + 1].y >> 8;  // the next chunk's first task
+            if (rl(rtask, m - 1) == ntask) {
+                // cut before the group of the chunk's last record; a group of 64 records or
+                // more does not fit
+                nc = 63u - (uint32_t)__builtin_clzll(__ballot(lane < m && (lane == 0 || ptask != rtask)));
+                if (nc == 0) {
+                    ok = false;
+                    break;
+                }
+            }
+        }
+        const uint64_t Wm = __ballot(lane < nc && (rmode & 2u));                   // writer-rule LINKs
+        const uint64_t GS = __ballot(lane < nc && (lane == 0 || ptask != rtask));  // task group starts
+        const uint64_t RD = __ballot(lane < nc && rmode == 1u);                    // exact readers (logged)
         // records whose task has no other record in this bin
-        const uint64_t SG = cnt ? GS & ((GS >> 1) | (1ull << (cnt - 1))) : 0ull;
+        const uint64_t SG = GS & ((GS >> 1) | (1ull << (nc - 1)));
         uint32_t i = 0;
-        while (ok && i < cnt) {
+        while (ok && i < nc) {
             const uint64_t wr = Wm & (~0ull << i);
-            uint32_t end = cnt;
+            uint32_t end = nc;
             if (wr) {
                 const uint32_t w = ctz(wr);
                 end = 63u - (uint32_t)__builtin_clzll(GS & lanes_lt(w + 1));  // the writer's group start
@@ -1433,7 +1498,7 @@ __global__ void __launch_bounds__(64, 7) k_resolve_reg(KArgs a)
             }
             // the task group at i writes: LINK each record, then PUT each, in slot order
             const uint64_t nx = GS & ~lanes_lt(i + 1);
-            end = nx ? min(ctz(nx), cnt) : cnt;
+            end = nx ? min(ctz(nx), nc) : nc;
             const bool single = end == i + 1;  // the group's only record in this bin
             for (uint32_t q = i; ok && q < end; ++q) {
                 const uint32_t X0 = rl(rx0, q), X1 = rl(rx1, q), Y0 = rl(ry0, q), Y1 = rl(ry1, q);
@@ -1450,7 +1515,7 @@ __global__ void __launch_bounds__(64, 7) k_resolve_reg(KArgs a)
                 const uint32_t one = rl(rone, q);
                 if (mode & 2u) {
                     ok = reg_paint(s, b, nf, X0, X1, Y0, Y1, acc, one, X0 == BX0 && X1 == BX1 && Y0 == BY0 && Y1 == BY1);
-                    fdirty = 1;
+                    fdirty = painted = 1;
                 } else if (mode == 1u) {
                     if (ng >= 64u) {
                         reg_prune(s, b, nf, ng, gdirty);
@@ -1466,8 +1531,11 @@ __global__ void __launch_bounds__(64, 7) k_resolve_reg(KArgs a)
             visits += (unsigned long long)nf * (end - i);
             i = end;
         }
-        // a log past half the registers is pruned before it is stored
-        if (ok && ng > 32u) reg_prune(s, b, nf, ng, gdirty);
+        c0 += nc;
+        }
+        // a log past half the registers is pruned before it is stored, when a paint may have
+        // left readers that no point sees
+        if (ok && painted && ng > 32u) reg_prune(s, b, nf, ng, gdirty);
         bin_pre_load(a, e_cur, pre, true);  // before the write-back stores
         e_nxt = entry_k(k + 2);
         if (!ok) {  // the first LDS tier takes the bin from its HBM state
@@ -2660,8 +2728,8 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
         launch_on(p, st, spans, 3, k_resolve<F, R, L, false>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
 }
 
-// k_resolve_reg: blocks per CU (its VGPRs allow 7 waves per SIMD; its LDS, 2.8 KB, many more)
-constexpr uint32_t REG_PER_CU = 28;
+// k_resolve_reg: blocks per CU (its 80 VGPRs allow 6 waves per SIMD; its LDS, 6.6 KB, 24 blocks)
+constexpr uint32_t REG_PER_CU = 24;
 
 // k_resolve_reg's blocks stride over the launch's touched list: a small launch gets at most one
 // block per record
