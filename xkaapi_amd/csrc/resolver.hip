@@ -158,9 +158,7 @@ struct KArgs {
     uint32_t lmax;            // bucket capacity: records per bin and launch of the highest tier
     uint32_t nbins;
     uint32_t max_tier;                              // tiers 0..max_tier; storage sized for max_tier
-    uint32_t min_tier;                              // first tier a bin may take (1 in small launches)
-    uint32_t skip1;                                 // 1: tier 0 passes bins to tier 2 (no tier-1 kernel)
-    uint32_t reg_first;                             // 1: k_resolve_reg took the touched list first
+    uint32_t first_tier;                            // the LDS tier of the bins k_resolve_reg passes on
     uint32_t *cand;           // staged form: CSTRIPES stripes of cand_cap candidates (3 u32 each)
     uint32_t cand_cap;        // per stripe
     uint32_t *cstr;           // CSTRIPES candidate counters, 16 u32 apart
@@ -898,7 +896,7 @@ __device__ __forceinline__ void escalate(const KArgs &a, uint32_t tier, uint4 e,
     if (tier < a.max_tier) {
         // tier 1 has tier 0's fragment and log room and only more record room; in a launch
         // without a tier-1 kernel tier 0 passes its bins straight to tier 2
-        const uint32_t next = tier == 0 && a.skip1 ? 2u : tier + 1;
+        const uint32_t next = tier + 1;
         const uint32_t pos = atomicAdd(&a.ctr->ntier[next], 1u);
         a.active[(size_t)next * a.nbins + pos] = e;
     } else {
@@ -1287,7 +1285,7 @@ __global__ void __launch_bounds__(64, 6) k_resolve_reg(KArgs a)
     const uint32_t lane = lane_id();
     if (a.ctr->err) return;  // a poisoned pool touches no state
     const uint32_t nlist = min(a.ctr->ntouched, a.nbins);
-    const uint32_t next = a.min_tier;  // the tier that takes the bins not replayed here
+    const uint32_t next = a.first_tier;  // the LDS tier that takes the bins not replayed here
     unsigned long long visits = 0;
     uint32_t maxf = 0, maxg = 0;
     uint32_t cc = 0, ec = 0;  // candidates staged in s.C, edges known unique in s.E
@@ -1584,10 +1582,8 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     BinLds<FCAP, GCAP, LCAP, DIR> &s = *reinterpret_cast<BinLds<FCAP, GCAP, LCAP, DIR> *>(lds_raw);
     const uint32_t lane = lane_id();
     if (a.ctr->err) return;  // a poisoned pool touches no state
-    // a launch's first tier takes every touched bin straight from the touched list
-    // (no k_alloc); later tiers take their active list (bins escalated from below)
-    const bool touched = tier == a.min_tier && !a.reg_first;
-    const uint32_t nlist = touched ? min(a.ctr->ntouched, a.nbins) : min(a.ctr->ntier[tier], a.nbins);
+    // each tier walks its list: the bins the register replay or the tier below passed on
+    const uint32_t nlist = min(a.ctr->ntier[tier], a.nbins);
     const uint4 *list = a.active + (size_t)tier * a.nbins;
     unsigned long long visits = 0;
     uint32_t maxf = 0, maxg = 0;
@@ -1600,15 +1596,7 @@ __global__ void __launch_bounds__(64) k_resolve(KArgs a, uint32_t tier)
     // bin, each entry sat on that bin's path behind the block's earlier loads
     // and stores.
     const uint32_t G = gridDim.x;
-    auto entry = [&](uint32_t x) {
-        if (x >= nlist) return make_uint4(0, 0, 0, 0);
-        if (touched) {
-            const uint32_t slot = a.touched[x];
-            return make_uint4(slot, 0u, a.bin_cnt[slot],
-                              a.bin_nf[slot] | (a.bin_nrs[slot] << 11) | (a.bin_dom[slot] << 24));
-        }
-        return list[x];
-    };
+    auto entry = [&](uint32_t x) { return x < nlist ? list[x] : make_uint4(0, 0, 0, 0); };
     uint32_t wbase = 0;  // the window holds the entries of bins k in [wbase, wbase + 64)
     uint4 ew = entry(blockIdx.x + lane * G);
     // the entry of the block's k-th bin; k grows by one from call to call
@@ -2298,8 +2286,9 @@ struct KClass {
 };
 // fragments, reader-log entries, records per bin and launch
 const KClass KCLASSES[3] = {{64, 128, 256}, {256, 512, 512}, {1024, 2048, 1024}};
-// LDS tiers: a small tier for the many bins with few records, then the three classes
-const KClass TIERS[4] = {{64, 128, 64}, {64, 128, 256}, {256, 512, 512}, {1024, 2048, 1024}};
+// Tiers by (fragments, log entries, records): tier 0 is the register replay's room
+// (k_resolve_reg), tiers 1-3 the LDS kernels; tier t + 1 is class t's room
+const KClass TIERS[4] = {{64, 64, 256}, {64, 128, 256}, {256, 512, 512}, {1024, 2048, 1024}};
 
 template <int F, int R, int L>
 size_t lds_bytes() { return sizeof(BinLds<F, R, L, false>); }  // the same for both forms
@@ -2672,9 +2661,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.lmax = p->lmax;
     a.nbins = p->cap_bins;
     a.max_tier = p->kclass + 1;  // tier t + 1 is class t
-    a.min_tier = 0;
-    a.skip1 = 0;
-    a.reg_first = 0;
+    a.first_tier = p->kclass >= 1 ? 2u : 1u;
     a.cand = p->cand; a.cand_cap = p->cand_cap; a.cstr = p->cstr; a.blk = p->blk;
     a.hkeys = p->hkeys; a.hvals = p->hvals; a.hmask = p->hmask;
     a.eslot = p->eslot; a.dcnt = p->dcnt; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
@@ -2751,36 +2738,24 @@ uint32_t count_blocks(uint64_t n)
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 4 * KRPW - 1) / (4 * KRPW), 8192));
 }
 
-void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint64_t nrec,
+void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint64_t nrec, bool small,
                     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
     const uint32_t rblocks = count_blocks(nrec);
-    // a small launch starts at tier 1 (one resolve kernel less: occupancy does not matter
-    // there); every launch's first tier takes the touched bins itself (no k_alloc)
-    const bool small = a.min_tier > 0;
-    // A large launch of under 128K records in a class with a tier 2 has no tier-1 kernel:
-    // tier 1 only adds record room (bins of more than 64 records), such launches hardly ever
-    // fill a bin that far, and the empty kernel still costs a launch slot.  Launches of more
-    // records keep it: many of their bins exceed 64 records, and tier 2 fits a third of
-    // tier 1's blocks per CU.
-    KArgs ak = a;
-    ak.skip1 = !small && p->kclass >= 1 && nrec < (1u << 17) ? 1u : 0u;
-    launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), COUNT_LDS, ak);
-    // Tier 0 has several blocks per resident slot: each walks fewer bins, and a waiting block
-    // starts wherever one finished, so the bins that take several times the mean no longer pile
-    // up on some blocks while the others idle.  Launches of many records (tens of records per
-    // bin) gain up to 8 blocks per slot; short ones do best with 3 (random 4096-task launches:
-    // 30.9 ms per pass with 2, 30.4 with 3) and lose to the extra block starts beyond that.
-    const uint32_t over0 = nrec >= (1u << 16) ? 8u : 3u;
-    // the register replay first: it takes the touched bins and passes those it cannot hold
-    // to the first LDS tier, which then walks only them
-    ak.reg_first = 1;
-    launch_reg(p, ak, spans, small ? 1u : over0, nrec);
-    // tier 0 first; bins that outgrow a tier are retried by the next one
-    if (!small) launch_resolve<64, 128, 64>(p, ak, 0, spans, nullptr, 1);
-    if (!ak.skip1) launch_resolve<64, 128, 256>(p, ak, 1, spans);
-    if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, ak, 2, spans);
-    if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, ak, 3, spans);
+    launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), COUNT_LDS, a);
+    // The register replay takes the touched bins.  Several blocks per resident slot: each
+    // walks fewer bins, and a waiting block starts wherever one finished, so the bins that take
+    // several times the mean do not pile up on some blocks while the others idle.  Launches of
+    // many records (tens of records per bin) gain up to 8 blocks per slot; short ones do best
+    // with 3 and lose to the extra block starts beyond that; a small launch takes one.
+    const uint32_t over = small ? 1u : (nrec >= (1u << 16) ? 8u : 3u);
+    launch_reg(p, a, spans, over, nrec);
+    // The bins it passes on hold more than 64 records, fragments or log entries: tier 0 (64
+    // fragments and records) cannot take them, so they go to the first tier with more room --
+    // tier 1 (128 log entries, 256 records) in class 0, else tier 2 -- and up from there.
+    if (a.first_tier <= 1) launch_resolve<64, 128, 256>(p, a, 1, spans);
+    if (p->kclass >= 1) launch_resolve<256, 512, 512>(p, a, 2, spans);
+    if (p->kclass >= 2) launch_resolve<1024, 2048, 1024>(p, a, 3, spans);
     if (!p->cfg.direct_dedup)
         launch(p, spans, 4, k_dedup, dim3(DEDUP_BLOCKS), dim3(256), (uint32_t)sizeof(DedupLds), a);
     launch(p, spans, 5, k_final, dim3(DEDUP_BLOCKS), dim3(256), (uint32_t)sizeof(FinalLds), a);
@@ -2808,16 +2783,15 @@ int push_domains(xkrt_hip_pool *p)
 int ticket(xkrt_hip_pool *p);
 
 // a small launch: records read in place from the pinned host buffer at hdr + 32 B
-// (header = record count), no graph, resolve from tier 1
+// (header = record count), no graph, the register replay on one block per resident slot
 int enqueue_small(xkrt_hip_pool *p, const uint4 *hdr, uint32_t nrec)
 {
     int rc = push_domains(p);
     if (rc) return rc;
     KArgs a = make_args(p, hdr + 2, NONE);
     a.nrec_ptr = (const uint32_t *)hdr;
-    a.min_tier = 1;
     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
-    launch_kernels(p, a, nrec, p->timing ? &spans : nullptr);
+    launch_kernels(p, a, nrec, true, p->timing ? &spans : nullptr);
     if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
     return ticket(p);
 }
@@ -2837,7 +2811,7 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
             p->graph[gk] = nullptr;
             const KArgs a = make_args(p, p->rec_dev, NONE);
             HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-            launch_kernels(p, a, cap_nrec, nullptr);
+            launch_kernels(p, a, cap_nrec, false, nullptr);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(p->stream, &g);
             if (e != hipSuccess) return hip_err(e);
@@ -2861,7 +2835,7 @@ int enqueue(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec, bool counted)
     } else {
         const KArgs a = make_args(p, rec, counted ? NONE : nrec);
         std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> spans;
-        launch_kernels(p, a, nrec, p->timing ? &spans : nullptr);
+        launch_kernels(p, a, nrec, false, p->timing ? &spans : nullptr);
         if (!spans.empty()) p->ev_pending.push_back(std::move(spans));
     }
     return ticket(p);
@@ -3091,7 +3065,6 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
             if (!rc && at.sharedSizeBytes != 0) rc = -ENOEXEC;  // dynamic LDS must start at offset 0
         }
     }
-    chk(set_lds<64, 128, 64>());
     chk(set_lds<64, 128, 256>());
     chk(set_lds<256, 512, 512>());
     chk(set_lds<1024, 2048, 1024>());
@@ -3351,9 +3324,8 @@ int gate_arm(xkrt_hip_pool *p)
                        p->gate_count);
     KArgs a = make_args(p, p->rec_host_dev[hb] + 2, NONE);
     a.nrec_ptr = p->gate_count;
-    a.min_tier = 1;
     // grids for any count: k_count strides over the records
-    launch_kernels(p, a, p->cfg.small_launch_rects, nullptr);
+    launch_kernels(p, a, p->cfg.small_launch_rects, true, nullptr);
     rc = ticket(p);
     if (rc) return rc;
     p->gate_armed = true;
