@@ -12,8 +12,11 @@ pytestmark = pytest.mark.gpu
 
 
 def unique_rows(e):
+    # sorted rows; a row twice would break one edge per (pred access, succ task)
     e = np.asarray(e, np.int64).reshape(-1, 2)
-    return np.unique(e, axis=0) if len(e) else e
+    u = np.unique(e, axis=0) if len(e) else e
+    assert len(u) == len(e), "duplicate edges"
+    return u
 
 
 def oracle_edges(st):

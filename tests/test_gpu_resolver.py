@@ -27,7 +27,10 @@ def oracle_edges(st):
 
 
 def same(a, b):
-    ua = np.unique(np.asarray(a, np.int64).reshape(-1, 2), axis=0)
+    # one edge per (pred access, succ task) (include/xkrt/task/task.hpp:494-512): no row twice
+    a = np.asarray(a, np.int64).reshape(-1, 2)
+    ua = np.unique(a, axis=0)
+    assert len(ua) == len(a), "duplicate edges"
     ub = np.unique(np.asarray(b, np.int64).reshape(-1, 2), axis=0)
     return ua.shape == ub.shape and np.array_equal(ua, ub)
 
@@ -134,3 +137,51 @@ def test_wrapped_tile_exact_when_no_overshoot():
     assert (nr == 2).any()
     got = R.resolve_stream(st, tasks_per_batch=50, bin_x_bytes=16, bin_y_cols=4, kernel=1)
     assert same(got, oracle_edges(st))
+
+
+def _stream(rows_cols_mn_mode, ld, s):
+    """One access per task: (row, col, m, n, mode) in elements of an (ld, s) matrix."""
+    a = np.asarray(rows_cols_mn_mode, np.int64)
+    n = len(a)
+    return S.Stream(np.arange(n, dtype=np.uint32), np.zeros(n, np.uint16), a[:, 4].astype(np.uint8),
+                    np.ones(n, np.uint8), ((a[:, 1] * ld + a[:, 0]) * s).astype(np.uint64),
+                    np.full(n, ld, np.uint64), a[:, 2].astype(np.uint64), a[:, 3].astype(np.uint64),
+                    np.full(n, s, np.uint64))
+
+
+def test_register_replay_overflow_drops_its_unique_edges():
+    # One bin (256 x 128 elements of 4 B = 1024 B x 128 columns), one launch: a writer, readers
+    # inside it (single-bin predecessor, one record per task: their edges are known unique and
+    # staged as such), then 80 disjoint small writers that push the map past 64 fragments, so
+    # the register replay gives the bin up and LDS tier 2 replays it; then more readers.  The
+    # edges staged before the overflow must be dropped, or they would be found twice.
+    acc = [(0, 0, 64, 32, 2)]
+    acc += [(4 * k, k, 8, 8, 1) for k in range(10)]
+    acc += [(16 * (k % 16), 40 + 8 * (k // 16), 4, 4, 2) for k in range(80)]
+    acc += [(16 * (k % 16), 40 + 8 * (k // 16), 8, 6, 1) for k in range(0, 80, 3)]
+    acc += [(0, 0, 128, 128, 3)]
+    st = _stream(acc, 256, 4)
+    want = oracle_edges(st)
+    got = R.resolve_stream(st, tasks_per_batch=len(acc), bin_x_bytes=1024, bin_y_cols=128, kernel=1)
+    assert same(got, want)
+
+
+def test_unique_edges_mixed_with_shared_keys():
+    # multi-record tasks (two accesses of one task in one bin) and rects spanning several bins
+    # beside single-bin ones: edges known unique and table candidates in one launch
+    # (launches of at most 400 tasks, capacity class 2: a bin's records fit its 1024-record bucket)
+    rng = np.random.default_rng(21)
+    n, ld, s = 2000, 512, 4
+    task = np.repeat(np.arange(n // 2, dtype=np.uint32), 2)
+    mode = rng.choice(np.array([1, 1, 2, 3], np.uint8), n)
+    big = rng.random(n) < 0.1
+    m = np.where(big, rng.integers(100, 300, n), rng.integers(1, 40, n)).astype(np.uint64)
+    nn = np.where(big, rng.integers(50, 200, n), rng.integers(1, 40, n)).astype(np.uint64)
+    row = rng.integers(0, ld - 300, n).astype(np.uint64)
+    col = rng.integers(0, 400, n).astype(np.uint64)
+    st = S.Stream(task, np.tile(np.array([0, 1], np.uint16), n // 2), mode, np.ones(n, np.uint8),
+                  (col * ld + row) * s, np.full(n, ld, np.uint64), m, nn, np.full(n, s, np.uint64))
+    want = oracle_edges(st)
+    for tpb in (64, 400):
+        got = R.resolve_stream(st, tasks_per_batch=tpb, bin_x_bytes=512, bin_y_cols=64, kernel=2)
+        assert same(got, want), tpb

@@ -26,7 +26,11 @@ def filtered(edges, done):
 
 
 def uniq(e):
-    return np.unique(np.asarray(e, np.int64).reshape(-1, 2), axis=0)
+    # sorted rows; a row twice would break one edge per (pred access, succ task)
+    e = np.asarray(e, np.int64).reshape(-1, 2)
+    u = np.unique(e, axis=0)
+    assert len(u) == len(e), "duplicate edges"
+    return u
 
 
 def schedule(st, seed, nb):

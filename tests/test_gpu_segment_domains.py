@@ -29,7 +29,11 @@ MODES = np.array([1, 2, 3, 4, 5, 6, 8, 9], np.uint8)
 
 
 def pairs(x):
-    return np.unique(np.asarray(x, np.int64).reshape(-1, 2), axis=0)
+    # sorted rows; a row twice would break one edge per (pred access, succ task)
+    x = np.asarray(x, np.int64).reshape(-1, 2)
+    u = np.unique(x, axis=0)
+    assert len(u) == len(x), "duplicate edges"
+    return u
 
 
 def same(x, y):
