@@ -159,6 +159,7 @@ struct KArgs {
     uint32_t nbins;
     uint32_t max_tier;                              // tiers 0..max_tier; storage sized for max_tier
     uint32_t first_tier;                            // the LDS tier of the bins k_resolve_reg passes on
+    uint32_t krpw;                                  // k_count: records per wave (16 or 64)
     uint32_t *cand;           // staged form: CSTRIPES stripes of cand_cap candidates (3 u32 each)
     uint32_t cand_cap;        // per stripe
     uint32_t *cstr;           // CSTRIPES candidate counters, 16 u32 apart
@@ -237,7 +238,10 @@ __device__ __forceinline__ bool rec_bins(const KArgs &a, uint32_t r, uint4 &q0, 
 // incl_j; incidence k belongs to the lane j with incl_{j-1} <= k < incl_j and
 // is bin (bx0_j + off % w_j, by0_j + off / w_j), off = k - incl_{j-1}.  So a
 // record covering many bins does not stall its wave.
-constexpr uint32_t KRPW = 16;  // records per wave in k_count
+// k_count's records per wave (KArgs::krpw): 16 spreads a small launch over more waves; a large
+// launch takes 64, a lane's worth, since 16 leave most lanes idle when records cover one bin
+constexpr uint32_t KRPW_SMALL = 16, KRPW_LARGE = 64;
+constexpr uint64_t KRPW_LARGE_FROM = 1u << 17;  // records of a launch from which it takes 64
 
 struct WaveRecs {
     uint32_t r, valid, bx0, w, by0, n, incl, bin0, nbx;
@@ -322,11 +326,11 @@ __global__ void __launch_bounds__(256) k_count(KArgs a)
     if (tid == 0) a.ctr->nrec = nrec;  // for the later kernels
     if (a.ctr->err) return;  // a poisoned pool touches no state
     const uint32_t lane = lane_id();
-    // KRPW records per wave: their incidences (several per record in general) are
-    // spread over the wave's 64 lanes, so more waves share a launch's records
-    const uint32_t wave = tid >> 6, nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t r0 = wave * KRPW; r0 < nrec; r0 += nwaves * KRPW) {
-        const WaveRecs q = wave_recs(a, r0 + lane, lane < KRPW ? min(nrec, r0 + KRPW) : 0u, nrec);
+    // krpw records per wave: their incidences (several per record in general) are
+    // spread over the wave's 64 lanes
+    const uint32_t wave = tid >> 6, nwaves = (gridDim.x * blockDim.x) >> 6, krpw = a.krpw;
+    for (uint32_t r0 = wave * krpw; r0 < nrec; r0 += nwaves * krpw) {
+        const WaveRecs q = wave_recs(a, r0 + lane, lane < krpw ? min(nrec, r0 + krpw) : 0u, nrec);
         const uint32_t total = __shfl(q.incl, 63, 64);
         for (uint32_t k0 = 0; k0 < total; k0 += 64) {
             const uint32_t k = k0 + lane;
@@ -2662,6 +2666,7 @@ KArgs make_args(xkrt_hip_pool *p, const uint4 *rec, uint32_t nrec_arg)
     a.nbins = p->cap_bins;
     a.max_tier = p->kclass + 1;  // tier t + 1 is class t
     a.first_tier = p->kclass >= 1 ? 2u : 1u;
+    a.krpw = KRPW_SMALL;
     a.cand = p->cand; a.cand_cap = p->cand_cap; a.cstr = p->cstr; a.blk = p->blk;
     a.hkeys = p->hkeys; a.hvals = p->hvals; a.hmask = p->hmask;
     a.eslot = p->eslot; a.dcnt = p->dcnt; a.edges = p->edges; a.edom = p->edom; a.edge_cap = p->edge_cap;
@@ -2732,17 +2737,18 @@ void launch_reg(xkrt_hip_pool *p, const KArgs &a, std::vector<std::tuple<int, hi
 }
 
 // the kernels of one launch (timing slots: count, -, -, resolve, dedup, final)
-// k_count blocks (4 waves of KRPW records each) for n records; it strides over more
-uint32_t count_blocks(uint64_t n)
+// k_count blocks (4 waves of krpw records each) for n records; it strides over more
+uint32_t count_blocks(uint64_t n, uint32_t krpw)
 {
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 4 * KRPW - 1) / (4 * KRPW), 8192));
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 4 * krpw - 1) / (4 * krpw), 8192));
 }
 
 void launch_kernels(xkrt_hip_pool *p, const KArgs &a, uint64_t nrec, bool small,
                     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans)
 {
-    const uint32_t rblocks = count_blocks(nrec);
-    launch(p, spans, 0, k_count, dim3(rblocks), dim3(256), COUNT_LDS, a);
+    KArgs ak = a;
+    ak.krpw = nrec >= KRPW_LARGE_FROM ? KRPW_LARGE : KRPW_SMALL;
+    launch(p, spans, 0, k_count, dim3(count_blocks(nrec, ak.krpw)), dim3(256), COUNT_LDS, ak);
     // The register replay takes the touched bins.  Several blocks per resident slot: each
     // walks fewer bins, and a waiting block starts wherever one finished, so the bins that take
     // several times the mean do not pile up on some blocks while the others idle.  Launches of
