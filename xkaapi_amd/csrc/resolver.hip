@@ -952,24 +952,27 @@ __device__ __forceinline__ void bin_pre_load(const KArgs &a, uint4 e, BinPre &p,
 
 // ---------------------------------------------------------------- register replay
 //
-// k_resolve_reg, the first resolve kernel of a launch, replays every touched
-// bin of at most 64 records, 64 fragments and 64 log entries with the bin's
-// whole state in registers: fragment f in lane f, log entry g in lane g,
-// record r (spawn order) in lane r.  Rects reach the other lanes by readlane,
-// and ballots take the place of the LDS scans, counters and barriers of the
-// general replay (do_link, do_run, do_paint, do_log), whose semantics it
-// keeps.  The kernel is bound by instruction issue, not memory, so each step
-// loops over whichever side is shorter: a run of readers tests its records
-// (one per lane) against the fragments in turn, or each record against all
-// fragments; a writer tests its overlapping fragments in turn against all log
-// entries, or the log entries in turn against all fragments.  LDS only moves
-// values between lanes -- the record sort, a paint's new map, a log append or
+// k_resolve_reg<DIR, 1>, the first resolve kernel of a launch, replays every
+// touched bin of at most 256 records, 64 fragments and 64 log entries with the
+// bin's whole state in registers: fragment f in lane f, log entry g in lane
+// g, record r (spawn order) in lane r.  k_resolve_reg<DIR, 2> then takes the
+// bins whose map outgrew 64 fragments, with fragment f in lane f % 64 of half
+// f / 64 (up to 128).  Rects reach the other lanes by readlane, and ballots
+// take the place of the LDS scans, counters and barriers of the general
+// replay (do_link, do_run, do_paint, do_log), whose semantics it keeps.  The
+// kernel is bound by instruction issue, not memory, so each step loops over
+// whichever side is shorter: a run of readers tests its records (one per
+// lane) against the fragments in turn, or each record against all fragments;
+// a writer tests its overlapping fragments in turn against all log entries,
+// or the log entries in turn against all fragments.  LDS only moves values
+// between lanes -- the record sort, a paint's new map, a log append or
 // compaction, the corner candidates of the coverage test -- and stages the
-// candidates.  A bin that does not fit, or would outgrow 64 fragments or log
-// entries (or meet more than 63 readers in one coverage test), goes to the
-// list of the first LDS tier with nothing written back; the candidates it
-// staged are a subset of what that tier emits for it, and the cross-bin dedup
-// absorbs the repeats.
+// candidates.  A bin that does not fit, or would outgrow the fragments or
+// the 64 log entries (or meet more than 63 readers in one coverage test),
+// goes on -- from the first kernel to the second when its map is what grew,
+// else to the first LDS tier -- with nothing written back; the candidates it
+// staged are a subset of what the next replay emits for it, and the
+// cross-bin dedup absorbs the repeats.
 
 __device__ __forceinline__ uint32_t rl(uint32_t v, uint32_t l)
 {
@@ -988,9 +991,10 @@ __device__ __forceinline__ uint64_t lanes_lt(uint32_t n) { return n >= 64u ? ~0u
 
 constexpr uint32_t REG_RECS = 256;  // records of a bin the register replay takes (in chunks of 64)
 
+template <int KF>
 struct RegLds {
     static constexpr uint32_t CCAP = 128;
-    uint32_t X[7][64];         // lane exchange: records (7 words), a new map (6), log entries (6)
+    uint32_t X[7][64 * KF];    // lane exchange: records (7 words), a new map (6), log entries (6)
     uint32_t cx[64], cy[64];   // corner candidates of a coverage test
     uint32_t C[3 * CCAP];      // staged candidates (pred, succ access, domain << 24 | succ task)
     uint32_t E[3 * CCAP];      // staged edges known unique, the same three words
@@ -999,16 +1003,27 @@ struct RegLds {
     uint4 win[64];             // list entries of the block's next 64 bins
 };
 
-// a bin's state, one fragment and one log entry per lane; fo / go: the access
-// of the fragment's writer / of the logged reader lies in this bin alone
+// a bin's state: fragment lane + 64 k in half k, one log entry per lane; fo /
+// go: the access of the fragment's writer / of the logged reader lies in this
+// bin alone (kept for the first 64 fragments)
+template <int KF>
 struct RegBin {
-    uint32_t fx0, fx1, fy0, fy1, flw, fo;   // fragment (lane < nf)
-    uint32_t gx0, gx1, gy0, gy1, gacc, go;  // log entry (lane < ng)
+    uint32_t fx0[KF], fx1[KF], fy0[KF], fy1[KF], flw[KF], fo[KF];  // fragment (lane + 64 k < nf)
+    uint32_t gx0, gx1, gy0, gy1, gacc, go;                          // log entry (lane < ng)
 };
+
+// word v of fragment f (uniform f)
+template <int KF>
+__device__ __forceinline__ uint32_t rlf(const uint32_t (&v)[KF], uint32_t f)
+{
+    if (KF == 1 || f < 64) return rl(v[0], f & 63u);
+    return rl(v[KF - 1], f - 64);
+}
 
 // The staged edges known unique into the edge stripes: k_dedup gives them
 // positions without the table.
-__device__ __forceinline__ void flush_direct(const KArgs &a, RegLds &s, uint32_t n)
+template <class S>
+__device__ __forceinline__ void flush_direct(const KArgs &a, S &s, uint32_t n)
 {
     __syncthreads();  // the staged entries are in LDS
     const uint32_t lane = lane_id();
@@ -1032,15 +1047,15 @@ __device__ __forceinline__ void flush_direct(const KArgs &a, RegLds &s, uint32_t
 // table.  cc / ec: candidates / edges staged.  Staged edges leave LDS only
 // between bins (a bin that fails drops its own, see k_resolve_reg); when the
 // buffer is full they go on as candidates, which the table dedups just as well.
-template <bool DIR>
-__device__ __forceinline__ void reg_append(const KArgs &a, RegLds &s, uint32_t &cc, uint32_t &ec, uint64_t m,
+template <bool DIR, class S>
+__device__ __forceinline__ void reg_append(const KArgs &a, S &s, uint32_t &cc, uint32_t &ec, uint64_t m,
                                            uint64_t md, uint32_t p, uint32_t acc, uint32_t task)
 {
-    if (DIR || ec + popc(md) > RegLds::CCAP) md = 0;  // the direct form inserts everything into the table
+    if (DIR || ec + popc(md) > S::CCAP) md = 0;  // the direct form inserts everything into the table
     const uint64_t mc = m & ~md;
     if (mc) {
         const uint32_t n = popc(mc);
-        if (cc + n > RegLds::CCAP) {
+        if (cc + n > S::CCAP) {
             if (lane_id() == 0) s.ccount = cc;
             flush_cands_to<DIR>(a, s);
             cc = 0;
@@ -1078,14 +1093,38 @@ __device__ __forceinline__ uint64_t distinct_first(uint64_t m, uint32_t v)
     return first;
 }
 
+// the same over the fragment halves: first[k] holds, per distinct value of v
+// among the fragments of m, the fragment of lowest index
+template <int KF>
+__device__ __forceinline__ void distinct_first_f(const uint64_t (&m)[KF], const uint32_t (&v)[KF],
+                                                 uint64_t (&first)[KF])
+{
+    uint64_t mm[KF];
+#pragma unroll
+    for (int k = 0; k < KF; ++k) {
+        mm[k] = m[k];
+        first[k] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < KF; ++k)
+        while (mm[k]) {
+            const uint32_t l = ctz(mm[k]);
+            const uint32_t x = rl(v[k], l);
+            first[k] |= 1ull << l;
+#pragma unroll
+            for (int k2 = k; k2 < KF; ++k2) mm[k2] &= ~__ballot(v[k2] == x);
+        }
+}
+
 // True when Q minus the log entries of lanes K (clipped to Q) holds a point.
 // If the clipped areas sum below Q's area it does.  Otherwise: if it does, it
 // holds one (cx, cy) with cx in {Qx0} U {entry x1 < Qx1} and cy in {Qy0} U
 // {entry y1 < Qy1} -- from any uncovered point, move left to the largest such
 // x at or below it, then down likewise; no entry covers the points passed.  x
 // candidate j sits in lane j; the y candidates are taken in turn.  |K| <= 63.
-__device__ __forceinline__ bool reg_uncovered(RegLds &s, const RegBin &b, uint64_t K, uint32_t Qx0, uint32_t Qx1,
-                                              uint32_t Qy0, uint32_t Qy1)
+template <class S, class B>
+__device__ __forceinline__ bool reg_uncovered(S &s, const B &b, uint64_t K, uint32_t Qx0, uint32_t Qx1, uint32_t Qy0,
+                                              uint32_t Qy1)
 {
     const uint32_t lane = lane_id();
     const uint32_t kx0 = max(b.gx0, Qx0), kx1 = min(b.gx1, Qx1), ky0 = max(b.gy0, Qy0), ky1 = min(b.gy1, Qy1);
@@ -1124,23 +1163,29 @@ __device__ __forceinline__ bool reg_uncovered(RegLds &s, const RegBin &b, uint64
 // drops the logged readers that no point still sees (every fragment they
 // touch has a newer writer) and compacts the log; gdirty drops to 0 when
 // entries move
-__device__ __forceinline__ void reg_prune(RegLds &s, RegBin &b, uint32_t nf, uint32_t &ng, uint32_t &gdirty)
+template <int KF>
+__device__ __forceinline__ void reg_prune(RegLds<KF> &s, RegBin<KF> &b, uint32_t nf, uint32_t &ng, uint32_t &gdirty)
 {
     const uint32_t lane = lane_id();
     uint64_t alive = 0;
     if (nf <= ng) {  // fragments in turn, log entries in lanes
         bool sees = false;
         for (uint32_t f = 0; f < nf; ++f) {
-            const uint32_t lw = rl(b.flw, f);
+            const uint32_t lw = rlf<KF>(b.flw, f);
             sees = sees || ((lw == NONE || lw < b.gacc) &&
-                            overlaps(rl(b.fx0, f), rl(b.fx1, f), rl(b.fy0, f), rl(b.fy1, f), b.gx0, b.gx1, b.gy0, b.gy1));
+                            overlaps(rlf<KF>(b.fx0, f), rlf<KF>(b.fx1, f), rlf<KF>(b.fy0, f), rlf<KF>(b.fy1, f),
+                                     b.gx0, b.gx1, b.gy0, b.gy1));
         }
         alive = __ballot(lane < ng && sees);
     } else {
         for (uint32_t g = 0; g < ng; ++g) {
             const uint32_t ga = rl(b.gacc, g);
-            const bool sees = lane < nf && (b.flw == NONE || b.flw < ga) &&
-                              overlaps(b.fx0, b.fx1, b.fy0, b.fy1, rl(b.gx0, g), rl(b.gx1, g), rl(b.gy0, g), rl(b.gy1, g));
+            const uint32_t x0 = rl(b.gx0, g), x1 = rl(b.gx1, g), y0 = rl(b.gy0, g), y1 = rl(b.gy1, g);
+            bool sees = false;
+#pragma unroll
+            for (int k = 0; k < KF; ++k)
+                sees = sees || (lane + 64 * k < nf && (b.flw[k] == NONE || b.flw[k] < ga) &&
+                                overlaps(b.fx0[k], b.fx1[k], b.fy0[k], b.fy1[k], x0, x1, y0, y1));
             if (__ballot(sees)) alive |= 1ull << g;
         }
     }
@@ -1161,47 +1206,67 @@ __device__ __forceinline__ void reg_prune(RegLds &s, RegBin &b, uint32_t nf, uin
 }
 
 // PUT of a writer (do_paint): up to four guillotine pieces of every fragment
-// under the rect survive with their writer, the rect becomes one fragment.
-// False when the map would exceed 64 fragments.
-__device__ __forceinline__ bool reg_paint(RegLds &s, RegBin &b, uint32_t &nf, uint32_t X0, uint32_t X1, uint32_t Y0,
-                                          uint32_t Y1, uint32_t acc, uint32_t one, bool whole)
+// under the rect survive with their writer, in fragment order, and the rect
+// becomes the last fragment.  False when the map would exceed 64 KF fragments.
+template <int KF>
+__device__ __forceinline__ bool reg_paint(RegLds<KF> &s, RegBin<KF> &b, uint32_t &nf, uint32_t X0, uint32_t X1,
+                                          uint32_t Y0, uint32_t Y1, uint32_t acc, uint32_t one, bool whole)
 {
     const uint32_t lane = lane_id();
     if (whole) {  // the rect is the whole bin: one fragment
-        if (lane == 0) { b.fx0 = X0; b.fx1 = X1; b.fy0 = Y0; b.fy1 = Y1; b.flw = acc; b.fo = one; }
+        if (lane == 0) { b.fx0[0] = X0; b.fx1[0] = X1; b.fy0[0] = Y0; b.fy1[0] = Y1; b.flw[0] = acc; b.fo[0] = one; }
         nf = 1;
         return true;
     }
-    const bool live = lane < nf;
-    const bool ov = live && overlaps(b.fx0, b.fx1, b.fy0, b.fy1, X0, X1, Y0, Y1);
-    // pieces of f minus the rect: top, bottom, then left and right of the middle band
-    const bool pt = ov && b.fx0 < X0, pbm = ov && X1 < b.fx1, pl = ov && b.fy0 < Y0, pr = ov && Y1 < b.fy1;
-    const uint32_t npc = live ? (ov ? (uint32_t)pt + pbm + pl + pr : 1u) : 0u;
-    const uint64_t c1 = __ballot(npc >= 1), c2 = __ballot(npc >= 2), c3 = __ballot(npc >= 3), c4 = __ballot(npc >= 4);
-    const uint32_t tot = popc(c1) + popc(c2) + popc(c3) + popc(c4);
-    if (tot + 1 > 64u) return false;
-    uint32_t o = popc_below(c1) + popc_below(c2) + popc_below(c3) + popc_below(c4);
-    auto put = [&](uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1) {
-        s.X[0][o] = x0; s.X[1][o] = x1; s.X[2][o] = y0; s.X[3][o] = y1; s.X[4][o] = b.flw; s.X[5][o] = b.fo;
-        ++o;
-    };
-    if (live && !ov) {
-        put(b.fx0, b.fx1, b.fy0, b.fy1);
-    } else if (ov) {
-        const uint32_t mx0 = max(b.fx0, X0), mx1 = min(b.fx1, X1);
-        if (pt) put(b.fx0, X0, b.fy0, b.fy1);
-        if (pbm) put(X1, b.fx1, b.fy0, b.fy1);
-        if (pl) put(mx0, mx1, b.fy0, Y0);
-        if (pr) put(mx0, mx1, Y1, b.fy1);
+    uint32_t base = 0;
+    bool live[KF], ov[KF], pt[KF], pbm[KF], pl[KF], pr[KF];
+    uint32_t o[KF];
+#pragma unroll
+    for (int k = 0; k < KF; ++k) {
+        live[k] = lane + 64 * k < nf;
+        ov[k] = live[k] && overlaps(b.fx0[k], b.fx1[k], b.fy0[k], b.fy1[k], X0, X1, Y0, Y1);
+        // pieces of f minus the rect: top, bottom, then left and right of the middle band
+        pt[k] = ov[k] && b.fx0[k] < X0;
+        pbm[k] = ov[k] && X1 < b.fx1[k];
+        pl[k] = ov[k] && b.fy0[k] < Y0;
+        pr[k] = ov[k] && Y1 < b.fy1[k];
+        const uint32_t npc = live[k] ? (ov[k] ? (uint32_t)pt[k] + pbm[k] + pl[k] + pr[k] : 1u) : 0u;
+        const uint64_t c1 = __ballot(npc >= 1), c2 = __ballot(npc >= 2), c3 = __ballot(npc >= 3),
+                       c4 = __ballot(npc >= 4);
+        o[k] = base + popc_below(c1) + popc_below(c2) + popc_below(c3) + popc_below(c4);
+        base += popc(c1) + popc(c2) + popc(c3) + popc(c4);
+    }
+    const uint32_t tot = base;
+    if (tot + 1 > 64u * KF) return false;
+#pragma unroll
+    for (int k = 0; k < KF; ++k) {
+        uint32_t q = o[k];
+        auto put = [&](uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1) {
+            s.X[0][q] = x0; s.X[1][q] = x1; s.X[2][q] = y0; s.X[3][q] = y1; s.X[4][q] = b.flw[k]; s.X[5][q] = b.fo[k];
+            ++q;
+        };
+        if (live[k] && !ov[k]) {
+            put(b.fx0[k], b.fx1[k], b.fy0[k], b.fy1[k]);
+        } else if (ov[k]) {
+            const uint32_t mx0 = max(b.fx0[k], X0), mx1 = min(b.fx1[k], X1);
+            if (pt[k]) put(b.fx0[k], X0, b.fy0[k], b.fy1[k]);
+            if (pbm[k]) put(X1, b.fx1[k], b.fy0[k], b.fy1[k]);
+            if (pl[k]) put(mx0, mx1, b.fy0[k], Y0);
+            if (pr[k]) put(mx0, mx1, Y1, b.fy1[k]);
+        }
     }
     if (lane == 0) {
         s.X[0][tot] = X0; s.X[1][tot] = X1; s.X[2][tot] = Y0; s.X[3][tot] = Y1; s.X[4][tot] = acc; s.X[5][tot] = one;
     }
     __syncthreads();
     nf = tot + 1;
-    if (lane < nf) {
-        b.fx0 = s.X[0][lane]; b.fx1 = s.X[1][lane]; b.fy0 = s.X[2][lane]; b.fy1 = s.X[3][lane]; b.flw = s.X[4][lane];
-        b.fo = s.X[5][lane];
+#pragma unroll
+    for (int k = 0; k < KF; ++k) {
+        const uint32_t f = lane + 64 * k;
+        if (f < nf) {
+            b.fx0[k] = s.X[0][f]; b.fx1[k] = s.X[1][f]; b.fy0[k] = s.X[2][f]; b.fy1[k] = s.X[3][f];
+            b.flw[k] = s.X[4][f]; b.fo[k] = s.X[5][f];
+        }
     }
     __syncthreads();
     return true;
@@ -1209,87 +1274,129 @@ __device__ __forceinline__ bool reg_paint(RegLds &s, RegBin &b, uint32_t &nf, ui
 
 // LINK of one record with the reader rule: the last writers of its points
 // (single: the record's task has one record in this bin)
-template <bool DIR>
-__device__ __forceinline__ void reg_link_reader(const KArgs &a, RegLds &s, uint32_t &cc, uint32_t &ec, const RegBin &b,
-                                                uint32_t nf, uint32_t X0, uint32_t X1, uint32_t Y0, uint32_t Y1,
-                                                uint32_t acc, uint32_t task, bool single)
+template <bool DIR, int KF>
+__device__ __forceinline__ void reg_link_reader(const KArgs &a, RegLds<KF> &s, uint32_t &cc, uint32_t &ec,
+                                                const RegBin<KF> &b, uint32_t nf, uint32_t X0, uint32_t X1, uint32_t Y0,
+                                                uint32_t Y1, uint32_t acc, uint32_t task, bool single)
 {
-    const bool ov = lane_id() < nf && b.flw != NONE && overlaps(b.fx0, b.fx1, b.fy0, b.fy1, X0, X1, Y0, Y1);
-    const uint64_t first = distinct_first(__ballot(ov), b.flw);
-    reg_append<DIR>(a, s, cc, ec, first, single ? first & __ballot(b.fo != 0) : 0ull, b.flw, acc, task);
+    uint64_t m[KF], first[KF];
+#pragma unroll
+    for (int k = 0; k < KF; ++k)
+        m[k] = __ballot(lane_id() + 64 * k < nf && b.flw[k] != NONE &&
+                        overlaps(b.fx0[k], b.fx1[k], b.fy0[k], b.fy1[k], X0, X1, Y0, Y1));
+    distinct_first_f<KF>(m, b.flw, first);
+#pragma unroll
+    for (int k = 0; k < KF; ++k)
+        reg_append<DIR>(a, s, cc, ec, first[k], single ? first[k] & __ballot(b.fo[k] != 0) : 0ull, b.flw[k], acc,
+                        task);
 }
 
 // LINK of one record with the writer rule.  Per overlapping fragment f and
 // Q = f n rect: every logged reader newer than lw(f) touching Q is a
 // predecessor, and lw(f) is one when some point of Q has no such reader.
 // False when a coverage test would meet more than 63 readers.
-template <bool DIR>
-__device__ __forceinline__ bool reg_link_writer(const KArgs &a, RegLds &s, uint32_t &cc, uint32_t &ec, const RegBin &b,
-                                                uint32_t nf, uint32_t ng, uint32_t X0, uint32_t X1, uint32_t Y0,
-                                                uint32_t Y1, uint32_t acc, uint32_t task, bool single)
+template <bool DIR, int KF>
+__device__ __forceinline__ bool reg_link_writer(const KArgs &a, RegLds<KF> &s, uint32_t &cc, uint32_t &ec,
+                                                const RegBin<KF> &b, uint32_t nf, uint32_t ng, uint32_t X0, uint32_t X1,
+                                                uint32_t Y0, uint32_t Y1, uint32_t acc, uint32_t task, bool single)
 {
     const uint32_t lane = lane_id();
-    const bool ov = lane < nf && overlaps(b.fx0, b.fx1, b.fy0, b.fy1, X0, X1, Y0, Y1);
-    const uint32_t qx0 = max(b.fx0, X0), qx1 = min(b.fx1, X1), qy0 = max(b.fy0, Y0), qy1 = min(b.fy1, Y1);
-    const uint64_t OV = __ballot(ov);
-    uint64_t PG = 0;   // log lanes that are predecessors
-    uint64_t elw = 0;  // fragment lanes whose writer is one
-    uint64_t part = 0; // fragment lanes that readers cover in part: exact test
-    if (popc(OV) <= ng) {  // overlapping fragments in turn, log entries in lanes
-        for (uint64_t om = OV; om; om &= om - 1) {
-            const uint32_t l = ctz(om);
-            const uint32_t Qx0 = rl(qx0, l), Qx1 = rl(qx1, l), Qy0 = rl(qy0, l), Qy1 = rl(qy1, l);
-            const uint32_t lw = rl(b.flw, l);
-            const bool t = lane < ng && (lw == NONE || b.gacc > lw) &&
-                           overlaps(b.gx0, b.gx1, b.gy0, b.gy1, Qx0, Qx1, Qy0, Qy1);
-            const uint64_t K = __ballot(t);
-            PG |= K;
-            if (lw == NONE) continue;
-            if (!K) {
-                elw |= 1ull << l;
-                continue;
+    bool ov[KF];
+    uint32_t qx0[KF], qx1[KF], qy0[KF], qy1[KF];
+    uint64_t OV[KF], elw[KF], part[KF];  // per half: overlapping, writer a predecessor, readers cover in part
+    uint32_t nov = 0;
+#pragma unroll
+    for (int k = 0; k < KF; ++k) {
+        ov[k] = lane + 64 * k < nf && overlaps(b.fx0[k], b.fx1[k], b.fy0[k], b.fy1[k], X0, X1, Y0, Y1);
+        qx0[k] = max(b.fx0[k], X0); qx1[k] = min(b.fx1[k], X1); qy0[k] = max(b.fy0[k], Y0); qy1[k] = min(b.fy1[k], Y1);
+        OV[k] = __ballot(ov[k]);
+        elw[k] = part[k] = 0;
+        nov += popc(OV[k]);
+    }
+    uint64_t PG = 0;  // log lanes that are predecessors
+    if (nov <= ng) {  // overlapping fragments in turn, log entries in lanes
+#pragma unroll
+        for (int k = 0; k < KF; ++k)
+            for (uint64_t om = OV[k]; om; om &= om - 1) {
+                const uint32_t l = ctz(om);
+                const uint32_t Qx0 = rl(qx0[k], l), Qx1 = rl(qx1[k], l), Qy0 = rl(qy0[k], l), Qy1 = rl(qy1[k], l);
+                const uint32_t lw = rl(b.flw[k], l);
+                const bool t = lane < ng && (lw == NONE || b.gacc > lw) &&
+                               overlaps(b.gx0, b.gx1, b.gy0, b.gy1, Qx0, Qx1, Qy0, Qy1);
+                const uint64_t K = __ballot(t);
+                PG |= K;
+                if (lw == NONE) continue;
+                if (!K) {
+                    elw[k] |= 1ull << l;
+                    continue;
+                }
+                if (!__ballot(t && b.gx0 <= Qx0 && Qx1 <= b.gx1 && b.gy0 <= Qy0 && Qy1 <= b.gy1)) part[k] |= 1ull << l;
             }
-            if (!__ballot(t && b.gx0 <= Qx0 && Qx1 <= b.gx1 && b.gy0 <= Qy0 && Qy1 <= b.gy1)) part |= 1ull << l;
-        }
     } else {  // log entries in turn, fragments in lanes
-        uint32_t nk = 0;
-        bool full = false;
+        uint32_t nk[KF];
+        bool full[KF];
+#pragma unroll
+        for (int k = 0; k < KF; ++k) {
+            nk[k] = 0;
+            full[k] = false;
+        }
         for (uint32_t g = 0; g < ng; ++g) {
             const uint32_t ga = rl(b.gacc, g);
             const uint32_t gx0 = rl(b.gx0, g), gx1 = rl(b.gx1, g), gy0 = rl(b.gy0, g), gy1 = rl(b.gy1, g);
-            const bool t = ov && (b.flw == NONE || ga > b.flw) && overlaps(gx0, gx1, gy0, gy1, qx0, qx1, qy0, qy1);
-            if (__ballot(t)) PG |= 1ull << g;
-            nk += t ? 1u : 0u;
-            full = full || (t && gx0 <= qx0 && qx1 <= gx1 && gy0 <= qy0 && qy1 <= gy1);
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < KF; ++k) {
+                const bool t = ov[k] && (b.flw[k] == NONE || ga > b.flw[k]) &&
+                               overlaps(gx0, gx1, gy0, gy1, qx0[k], qx1[k], qy0[k], qy1[k]);
+                nk[k] += t ? 1u : 0u;
+                full[k] = full[k] || (t && gx0 <= qx0[k] && qx1[k] <= gx1 && gy0 <= qy0[k] && qy1[k] <= gy1);
+                any = any || t;
+            }
+            if (__ballot(any)) PG |= 1ull << g;
         }
-        elw = __ballot(ov && b.flw != NONE && nk == 0);
-        part = __ballot(ov && b.flw != NONE && nk != 0 && !full);
+#pragma unroll
+        for (int k = 0; k < KF; ++k) {
+            elw[k] = __ballot(ov[k] && b.flw[k] != NONE && nk[k] == 0);
+            part[k] = __ballot(ov[k] && b.flw[k] != NONE && nk[k] != 0 && !full[k]);
+        }
     }
-    for (; part; part &= part - 1) {
-        const uint32_t l = ctz(part);
-        const uint32_t Qx0 = rl(qx0, l), Qx1 = rl(qx1, l), Qy0 = rl(qy0, l), Qy1 = rl(qy1, l);
-        const uint32_t lw = rl(b.flw, l);
-        const uint64_t K = __ballot(lane < ng && b.gacc > lw && overlaps(b.gx0, b.gx1, b.gy0, b.gy1, Qx0, Qx1, Qy0, Qy1));
-        if (popc(K) > 63u) return false;
-        if (reg_uncovered(s, b, K, Qx0, Qx1, Qy0, Qy1)) elw |= 1ull << l;
-    }
+#pragma unroll
+    for (int k = 0; k < KF; ++k)
+        for (; part[k]; part[k] &= part[k] - 1) {
+            const uint32_t l = ctz(part[k]);
+            const uint32_t Qx0 = rl(qx0[k], l), Qx1 = rl(qx1[k], l), Qy0 = rl(qy0[k], l), Qy1 = rl(qy1[k], l);
+            const uint32_t lw = rl(b.flw[k], l);
+            const uint64_t K =
+                __ballot(lane < ng && b.gacc > lw && overlaps(b.gx0, b.gx1, b.gy0, b.gy1, Qx0, Qx1, Qy0, Qy1));
+            if (popc(K) > 63u) return false;
+            if (reg_uncovered(s, b, K, Qx0, Qx1, Qy0, Qy1)) elw[k] |= 1ull << l;
+        }
     // log entries with one access id are the records of one access (a wrapped tile): never
     // single-bin, so the edges known unique hold each reader once
     reg_append<DIR>(a, s, cc, ec, PG, single ? PG & __ballot(b.go != 0) : 0ull, b.gacc, acc, task);
-    const uint64_t first = distinct_first(elw, b.flw);
-    reg_append<DIR>(a, s, cc, ec, first, single ? first & __ballot(b.fo != 0) : 0ull, b.flw, acc, task);
+    uint64_t first[KF];
+    distinct_first_f<KF>(elw, b.flw, first);
+#pragma unroll
+    for (int k = 0; k < KF; ++k)
+        reg_append<DIR>(a, s, cc, ec, first[k], single ? first[k] & __ballot(b.fo[k] != 0) : 0ull, b.flw[k], acc,
+                        task);
     return true;
 }
 
-template <bool DIR>
-__global__ void __launch_bounds__(64, 6) k_resolve_reg(KArgs a)
+// KF = 1: the touched list; bins whose map outgrows 64 fragments go on to the
+// KF = 2 kernel (list 0) when the pool has one (class >= 1), the others to the
+// first LDS tier.  KF = 2: list 0; what it cannot hold goes to the first LDS
+// tier.
+template <bool DIR, int KF>
+__global__ void __launch_bounds__(64, (KF == 1 ? 6 : 4)) k_resolve_reg(KArgs a)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    RegLds &s = *reinterpret_cast<RegLds *>(lds_raw);
+    RegLds<KF> &s = *reinterpret_cast<RegLds<KF> *>(lds_raw);
     const uint32_t lane = lane_id();
     if (a.ctr->err) return;  // a poisoned pool touches no state
-    const uint32_t nlist = min(a.ctr->ntouched, a.nbins);
+    const uint32_t nlist = KF == 1 ? min(a.ctr->ntouched, a.nbins) : min(a.ctr->ntier[0], a.nbins);
     const uint32_t next = a.first_tier;  // the LDS tier that takes the bins not replayed here
+    const bool wide = KF == 1 && a.first_tier >= 2;  // a KF = 2 kernel follows
     unsigned long long visits = 0;
     uint32_t maxf = 0, maxg = 0;
     uint32_t cc = 0, ec = 0;  // candidates staged in s.C, edges known unique in s.E
@@ -1299,6 +1406,7 @@ __global__ void __launch_bounds__(64, 6) k_resolve_reg(KArgs a)
     const uint32_t G = gridDim.x;
     auto entry = [&](uint32_t x) {
         if (x >= nlist) return make_uint4(0, 0, 0, 0);
+        if (KF != 1) return a.active[x];
         const uint32_t slot = a.touched[x];
         return make_uint4(slot, 0u, a.bin_cnt[slot], a.bin_nf[slot] | (a.bin_nrs[slot] << 11) | (a.bin_dom[slot] << 24));
     };
@@ -1331,8 +1439,10 @@ __global__ void __launch_bounds__(64, 6) k_resolve_reg(KArgs a)
             if (lane == 0) a.bin_cnt[slot] = 0;
             continue;
         }
-        bool ok = cnt <= REG_RECS && cnt <= a.lmax && nf0 <= 64u && ng0 <= 64u;
-        if (ec > RegLds::CCAP / 2) {  // room for this bin's edges known unique
+        const bool fits = cnt <= REG_RECS && cnt <= a.lmax && ng0 <= 64u;
+        bool ok = fits && nf0 <= 64u * KF;
+        bool grew = fits && nf0 > 64u * KF;  // the map is what does not fit
+        if (ec > RegLds<KF>::CCAP / 2) {  // room for this bin's edges known unique
             flush_direct(a, s, ec);
             ec = 0;
         }
@@ -1344,14 +1454,25 @@ __global__ void __launch_bounds__(64, 6) k_resolve_reg(KArgs a)
         const uint32_t BY0 = g.ybin0 + (byi << g.bys), BY1 = BY0 + (1u << g.bys);
         const uint32_t tdom = dm << 24;
 
-        RegBin b;
+        RegBin<KF> b;
         uint32_t nf = nf0, ng = ng0, gdirty = ng0, fdirty = 0, painted = 0;
-        b.fx0 = cur.f0; b.fx1 = cur.f1; b.fy0 = cur.f2; b.fy1 = cur.f3; b.flw = cur.f4;
+        b.fx0[0] = cur.f0; b.fx1[0] = cur.f1; b.fy0[0] = cur.f2; b.fy1[0] = cur.f3; b.flw[0] = cur.f4;
+        b.fo[0] = lane < nf0 ? (cur.mf >> (lane & 31u)) & 1u : 0u;
+#pragma unroll
+        for (int h = 1; h < KF; ++h) {  // fragments past the first 64 (no single-bin marks)
+            const uint32_t f = lane + 64 * h;
+            b.fx0[h] = b.fx1[h] = b.fy0[h] = b.fy1[h] = 0;
+            b.flw[h] = NONE;
+            b.fo[h] = 0;
+            if (ok && f < nf0) {
+                const uint32_t *q = a.frag + (size_t)slot * a.fcap * 5 + 5 * f;
+                b.fx0[h] = q[0]; b.fx1[h] = q[1]; b.fy0[h] = q[2]; b.fy1[h] = q[3]; b.flw[h] = q[4];
+            }
+        }
         b.gx0 = cur.g0; b.gx1 = cur.g1; b.gy0 = cur.g2; b.gy1 = cur.g3; b.gacc = cur.g4;
-        b.fo = lane < nf0 ? (cur.mf >> (lane & 31u)) & 1u : 0u;
         b.go = lane < ng0 ? (cur.mg >> (lane & 31u)) & 1u : 0u;
         if (nf0 == 0) {  // a bin never written: one fragment, no writer
-            b.fx0 = BX0; b.fx1 = BX1; b.fy0 = BY0; b.fy1 = BY1; b.flw = NONE; b.fo = 0;
+            b.fx0[0] = BX0; b.fx1[0] = BX1; b.fy0[0] = BY0; b.fy1[0] = BY1; b.flw[0] = NONE; b.fo[0] = 0;
             nf = 1;
             fdirty = 1;
         }
@@ -1458,23 +1579,24 @@ __global__ void __launch_bounds__(64, 6) k_resolve_reg(KArgs a)
                     const bool in = lane_in(run);
                     uint32_t last = NONE;  // the writer this record emitted last
                     for (uint32_t f = 0; f < nf; ++f) {
-                        const uint32_t lw = rl(b.flw, f);
+                        const uint32_t lw = rlf<KF>(b.flw, f);
                         if (lw == NONE) continue;
                         const bool ov = in && lw != last &&
-                                        overlaps(rl(b.fx0, f), rl(b.fx1, f), rl(b.fy0, f), rl(b.fy1, f), rx0, rx1, ry0, ry1);
-                        const uint64_t m = __ballot(ov);
-                        reg_append<DIR>(a, s, cc, ec, m, rl(b.fo, f) ? m & SG : 0ull, lw, racc, rtask | tdom);
+                                        overlaps(rlf<KF>(b.fx0, f), rlf<KF>(b.fx1, f), rlf<KF>(b.fy0, f),
+                                                 rlf<KF>(b.fy1, f), rx0, rx1, ry0, ry1);
+                        const uint64_t mm = __ballot(ov);
+                        reg_append<DIR>(a, s, cc, ec, mm, rlf<KF>(b.fo, f) ? mm & SG : 0ull, lw, racc, rtask | tdom);
                         last = ov ? lw : last;
                     }
                 } else {
                     for (uint32_t q = i; q < end; ++q)
-                        reg_link_reader<DIR>(a, s, cc, ec, b, nf, rl(rx0, q), rl(rx1, q), rl(ry0, q), rl(ry1, q),
-                                             rl(racc, q), (rl(rtm, q) >> 8) | tdom, (SG >> q) & 1ull);
+                        reg_link_reader<DIR, KF>(a, s, cc, ec, b, nf, rl(rx0, q), rl(rx1, q), rl(ry0, q), rl(ry1, q),
+                                                 rl(racc, q), (rl(rtm, q) >> 8) | tdom, (SG >> q) & 1ull);
                 }
                 const uint64_t rd = RD & run;
                 const uint32_t nr = popc(rd);
                 if (ng + nr > 64u) {
-                    reg_prune(s, b, nf, ng, gdirty);
+                    reg_prune<KF>(s, b, nf, ng, gdirty);
                     if (ng + nr > 64u) {
                         ok = false;
                         break;
@@ -1506,9 +1628,9 @@ __global__ void __launch_bounds__(64, 6) k_resolve_reg(KArgs a)
                 const uint32_t X0 = rl(rx0, q), X1 = rl(rx1, q), Y0 = rl(ry0, q), Y1 = rl(ry1, q);
                 const uint32_t acc = rl(racc, q), tm = rl(rtm, q), task = (tm >> 8) | tdom;
                 if (tm & 2u)
-                    ok = reg_link_writer<DIR>(a, s, cc, ec, b, nf, ng, X0, X1, Y0, Y1, acc, task, single);
+                    ok = reg_link_writer<DIR, KF>(a, s, cc, ec, b, nf, ng, X0, X1, Y0, Y1, acc, task, single);
                 else
-                    reg_link_reader<DIR>(a, s, cc, ec, b, nf, X0, X1, Y0, Y1, acc, task, single);
+                    reg_link_reader<DIR, KF>(a, s, cc, ec, b, nf, X0, X1, Y0, Y1, acc, task, single);
             }
             for (uint32_t q = i; ok && q < end; ++q) {
                 const uint32_t mode = rl(rtm, q) & 0xffu;
@@ -1516,11 +1638,13 @@ __global__ void __launch_bounds__(64, 6) k_resolve_reg(KArgs a)
                 const uint32_t X0 = rl(rx0, q), X1 = rl(rx1, q), Y0 = rl(ry0, q), Y1 = rl(ry1, q), acc = rl(racc, q);
                 const uint32_t one = rl(rone, q);
                 if (mode & 2u) {
-                    ok = reg_paint(s, b, nf, X0, X1, Y0, Y1, acc, one, X0 == BX0 && X1 == BX1 && Y0 == BY0 && Y1 == BY1);
+                    ok = reg_paint<KF>(s, b, nf, X0, X1, Y0, Y1, acc, one,
+                                       X0 == BX0 && X1 == BX1 && Y0 == BY0 && Y1 == BY1);
+                    grew = !ok;
                     fdirty = painted = 1;
                 } else if (mode == 1u) {
                     if (ng >= 64u) {
-                        reg_prune(s, b, nf, ng, gdirty);
+                        reg_prune<KF>(s, b, nf, ng, gdirty);
                         if (ng >= 64u) {
                             ok = false;
                             break;
@@ -1537,28 +1661,33 @@ __global__ void __launch_bounds__(64, 6) k_resolve_reg(KArgs a)
         }
         // a log past half the registers is pruned before it is stored, when a paint may have
         // left readers that no point sees
-        if (ok && painted && ng > 32u) reg_prune(s, b, nf, ng, gdirty);
+        if (ok && painted && ng > 32u) reg_prune<KF>(s, b, nf, ng, gdirty);
         bin_pre_load(a, e_cur, pre, true);  // before the write-back stores
         e_nxt = entry_k(k + 2);
-        if (!ok) {  // the first LDS tier takes the bin from its HBM state
+        if (!ok) {  // the next replay takes the bin from its HBM state
             ec = ec0;
             if (lane == 0) {
-                const uint32_t pos = atomicAdd(&a.ctr->ntier[next], 1u);
-                a.active[(size_t)next * a.nbins + pos] = e;
+                const uint32_t to = wide && grew ? 0u : next;
+                const uint32_t pos = atomicAdd(&a.ctr->ntier[to], 1u);
+                a.active[(size_t)to * a.nbins + pos] = e;
             }
             continue;
         }
         uint32_t *fw = a.frag + (size_t)slot * a.fcap * 5;
         uint32_t *gw = a.rs + (size_t)slot * a.rcap * 5;
-        if (fdirty && lane < nf) {
-            uint32_t *q = fw + 5 * lane;
-            q[0] = b.fx0; q[1] = b.fx1; q[2] = b.fy0; q[3] = b.fy1; q[4] = b.flw;
+#pragma unroll
+        for (int h = 0; h < KF; ++h) {
+            const uint32_t f = lane + 64 * h;
+            if (fdirty && f < nf) {
+                uint32_t *q = fw + 5 * f;
+                q[0] = b.fx0[h]; q[1] = b.fx1[h]; q[2] = b.fy0[h]; q[3] = b.fy1[h]; q[4] = b.flw[h];
+            }
         }
         if (lane >= gdirty && lane < ng) {
             uint32_t *q = gw + 5 * lane;
             q[0] = b.gx0; q[1] = b.gx1; q[2] = b.gy0; q[3] = b.gy1; q[4] = b.gacc;
         }
-        const uint64_t fm = __ballot(lane < nf && b.fo), gm = __ballot(lane < ng && b.go);
+        const uint64_t fm = __ballot(lane < nf && b.fo[0]), gm = __ballot(lane < ng && b.go);
         if (lane == 0) {
             a.bin_nf[slot] = nf;
             a.bin_nrs[slot] = ng;
@@ -2720,20 +2849,31 @@ void launch_resolve(xkrt_hip_pool *p, const KArgs &a, uint32_t tier,
         launch_on(p, st, spans, 3, k_resolve<F, R, L, false>, dim3(blocks), dim3(64), (uint32_t)lds, a, tier);
 }
 
-// k_resolve_reg: blocks per CU (its 80 VGPRs allow 6 waves per SIMD; its LDS, 6.6 KB, 24 blocks)
-constexpr uint32_t REG_PER_CU = 24;
+// k_resolve_reg blocks per CU: 64 fragments per bin, 6 waves per SIMD by its launch bound (its
+// LDS, 6.6 KB, allows 24 blocks); 128 fragments, 4 waves per SIMD (8.3 KB)
+constexpr uint32_t REG_PER_CU = 24, REG2_PER_CU = 16;
 
-// k_resolve_reg's blocks stride over the launch's touched list: a small launch gets at most one
-// block per record
+template <int KF>
+void launch_reg_kf(xkrt_hip_pool *p, const KArgs &a, std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans,
+                   uint32_t blocks)
+{
+    if (p->cfg.direct_dedup)
+        launch(p, spans, 3, k_resolve_reg<true, KF>, dim3(blocks), dim3(64), (uint32_t)sizeof(RegLds<KF>), a);
+    else
+        launch(p, spans, 3, k_resolve_reg<false, KF>, dim3(blocks), dim3(64), (uint32_t)sizeof(RegLds<KF>), a);
+}
+
+// The register replay: 64 fragments per bin on the touched list (its blocks stride over it: a
+// small launch gets at most one block per record), then, in classes with more than 64 fragments
+// per bin, 128 on the bins whose map outgrew 64.
 void launch_reg(xkrt_hip_pool *p, const KArgs &a, std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> *spans,
                 uint32_t over, uint64_t nrec)
 {
-    const uint32_t blocks = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>({(uint64_t)p->cap_bins, 256ull * REG_PER_CU * over, std::max<uint64_t>(nrec, 256)}));
-    if (p->cfg.direct_dedup)
-        launch(p, spans, 3, k_resolve_reg<true>, dim3(blocks), dim3(64), (uint32_t)sizeof(RegLds), a);
-    else
-        launch(p, spans, 3, k_resolve_reg<false>, dim3(blocks), dim3(64), (uint32_t)sizeof(RegLds), a);
+    const uint64_t cap = std::max<uint64_t>(nrec, 256);
+    launch_reg_kf<1>(p, a, spans,
+                     (uint32_t)std::min<uint64_t>({(uint64_t)p->cap_bins, 256ull * REG_PER_CU * over, cap}));
+    if (a.first_tier >= 2)
+        launch_reg_kf<2>(p, a, spans, (uint32_t)std::min<uint64_t>({(uint64_t)p->cap_bins, 256ull * REG2_PER_CU, cap}));
 }
 
 // the kernels of one launch (timing slots: count, -, -, resolve, dedup, final)
@@ -3066,7 +3206,8 @@ int create_pool(const xkrt_hip_pool_config_t *cfg, void *stream, xkrt_hip_pool *
     if (!rc) chk(hipMemsetAsync(p->blk, 0, 16ull * 4 * BLKSLOTS, p->stream));
     {
         hipFuncAttributes at{};
-        for (const void *f : {(const void *)k_resolve_reg<false>, (const void *)k_resolve_reg<true>}) {
+        for (const void *f : {(const void *)k_resolve_reg<false, 1>, (const void *)k_resolve_reg<true, 1>,
+                              (const void *)k_resolve_reg<false, 2>, (const void *)k_resolve_reg<true, 2>}) {
             chk(hipFuncGetAttributes(&at, f));
             if (!rc && at.sharedSizeBytes != 0) rc = -ENOEXEC;  // dynamic LDS must start at offset 0
         }
