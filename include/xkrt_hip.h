@@ -387,7 +387,9 @@ typedef struct xkrt_hip_server_config {
     uint32_t prearm;         /* 1: the next launch is queued on the device while the current one
                                 runs, held by a gate kernel that the engine opens by writing pinned
                                 memory once the batches are staged (no launch calls on the critical
-                                path); needs depth = 1 and a pool with small_launch_rects > 0 */
+                                path); needs depth <= 2 and a pool with small_launch_rects > 0 (with
+                                depth 2 the gate of the launch after next is armed once the
+                                oldest launch has freed its staging buffer) */
     uint32_t pad;
 } xkrt_hip_server_config_t;
 

@@ -128,3 +128,17 @@ def test_prearmed_launches_mix(clients, mode):
                   small_launch_rects=4096)
     check(res, clients)
     assert res.stats["gated"] > 0 and res.stats["error"] == 0
+
+
+@pytest.mark.parametrize("load", [0.3, 0.8])
+def test_prearmed_launches_two_in_flight(clients, load):
+    # two launches in flight: the gate of the launch after next is armed once the oldest launch
+    # has freed its staging buffer, so a buffer can be armed again before the engine retires
+    # the launch that used it
+    sat = V.saturation(clients, "batched", depth=2, max_rects=1 << 16, prearm=True, small_launch_rects=4096)
+    check(sat, clients)
+    rate = load * sum(len(c.batches) for c in clients) / sat.makespan_s
+    res = V.serve(clients, V.poisson_schedule(clients, rate, seed=9), "batched", depth=2, max_rects=1 << 16,
+                  prearm=True, small_launch_rects=4096)
+    check(res, clients)
+    assert res.stats["gated"] > 0 and res.stats["error"] == 0

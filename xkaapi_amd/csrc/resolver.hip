@@ -2515,6 +2515,10 @@ struct xkrt_hip_pool {
     // launch gate: a small launch enqueued ahead of its records (serving engine)
     bool gate_armed = false;
     uint32_t gate_hb = 0, gate_seq = 0;
+    // per staging buffer: the gate result of its previous launch, kept when the buffer is armed
+    // again (a second launch in flight may re-arm it before the engine retires that launch)
+    uint64_t gate_prev_ticket[2] = {0, 0};
+    uint32_t gate_prev_result[2] = {0, 0};
     uint64_t gate_ticket = 0;
     uint32_t *gate_count = nullptr;      // device word: the gated launch's record count
     unsigned long long gate_limit = 0;   // wall-clock ticks a gate waits (1 s)
@@ -3462,6 +3466,8 @@ int gate_arm(xkrt_hip_pool *p)
     int rc = push_domains(p);
     if (rc) return rc;
     volatile uint32_t *h = (volatile uint32_t *)p->rec_host_hdr[hb];
+    p->gate_prev_ticket[hb] = p->buf_ticket[hb];  // that launch completed: its gate result is final
+    p->gate_prev_result[hb] = h[2];
     const uint32_t seq = ++p->gate_seq;
     h[0] = 0;
     h[1] = seq - 1;
@@ -3479,6 +3485,34 @@ int gate_arm(xkrt_hip_pool *p)
     p->gate_hb = hb;
     p->gate_ticket = p->n_launch;
     return 0;
+}
+
+// arms the gate when the staging buffer it would take is free, without
+// waiting: with two launches in flight that buffer may still be read by the
+// older one; the engine tries again later
+int gate_try_arm(xkrt_hip_pool *p)
+{
+    if (p->gate_armed || p->open_nrec) return 0;
+    const uint32_t hb = p->cur_host;
+    if (p->staged_pending[hb]) {
+        const hipError_t q = hipEventQuery(p->staged[hb]);
+        if (q == hipErrorNotReady) return 0;
+        if (q != hipSuccess) return hip_err(q);
+        p->staged_pending[hb] = false;
+    }
+    if (p->buf_ticket[hb] > p->done) {
+        const int rc = retire(p, 0);
+        if (rc) return rc;
+        if (p->buf_ticket[hb] > p->done) return 0;
+    }
+    return gate_arm(p);
+}
+
+// the gate result (GATE_*) of completed launch `ticket`, gated on buffer hb
+uint32_t gate_result(const xkrt_hip_pool *p, uint32_t hb, uint64_t ticket)
+{
+    if (p->gate_prev_ticket[hb] == ticket) return p->gate_prev_result[hb];
+    return ((volatile uint32_t *)p->rec_host_hdr[hb])[2];
 }
 
 // opens the armed gate on the staged records (count 0: an empty launch)
@@ -4226,7 +4260,7 @@ int srv_retire(xkrt_hip_server *s, bool &work)
     while (!s->flights.empty() && s->flights.front().ticket <= p->done) {
         SrvFlight &f = s->flights.front();
         // a gate that gave up before the engine opened it: the batches were not resolved
-        if (f.gate_hb >= 0 && ((volatile uint32_t *)p->rec_host_hdr[f.gate_hb])[2] != GATE_OPEN) return -EIO;
+        if (f.gate_hb >= 0 && gate_result(p, (uint32_t)f.gate_hb, f.ticket) != GATE_OPEN) return -EIO;
         // the launch's k_final wrote its outcome before the launch completed
         LaunchSnap *o = &p->snap[f.ticket % NSNAP];
         if (__atomic_load_n(&o->launch, __ATOMIC_ACQUIRE) != f.ticket) return -EIO;
@@ -4379,9 +4413,10 @@ int srv_launch(xkrt_hip_server *s, bool &work)
     s->flights.push_back({tk, std::move(batch), ghb});
     s->count(xkrt_hip_server::S_LAUNCHES);
     work = true;
-    // queue the next launch behind this one while the device works
+    // queue the next launch behind this one while the device works (with two launches in
+    // flight, once the staging buffer it takes is free)
     if (s->cfg.prearm) {
-        const int rc = gate_arm(p);
+        const int rc = s->cfg.depth > 1 ? gate_try_arm(p) : gate_arm(p);
         if (rc) return rc;
     }
     return 0;
@@ -4452,6 +4487,11 @@ int srv_housekeep(xkrt_hip_server *s, bool &work, double idle_us)
         s->hold = false;
         s->count(xkrt_hip_server::S_CONSUMES);
         work = true;
+    }
+    // with two launches in flight the next gate is armed once a staging buffer is free
+    if (s->cfg.prearm && s->cfg.depth > 1 && !p->gate_armed && !s->hold) {
+        const int rc = gate_try_arm(p);
+        if (rc) return rc;
     }
     // a gate armed for long without work runs its launch empty (it would give up after 1 s)
     if (p->gate_armed && s->flights.empty() && s->pending.empty() && idle_us > 200000.0) {
@@ -4563,7 +4603,7 @@ int xkrt_hip_server_create(xkrt_hip_pool_t p, const xkrt_hip_server_config_t *cf
     if (c.depth == 0) c.depth = 2;
     if (c.depth > 16) return -EINVAL;
     // a gated launch waits in the stream: only with one launch in flight, over small launches
-    if (c.prearm && (c.depth != 1 || p->cfg.small_launch_rects == 0)) return -EINVAL;
+    if (c.prearm && (c.depth > 2 || p->cfg.small_launch_rects == 0)) return -EINVAL;
     int rc = read_counters(p);  // edges already in the buffer are not the engine's
     if (rc) return rc;
     if (c.reserve_bytes) {

@@ -151,7 +151,7 @@ def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30
         rb = bins_needed(clients, bin_x_bytes, bin_y_cols) * pool.stats()["bytes_per_bin"] if reserve else 0
         srv = R.Server(pool, depth=depth, max_batches=1 if mode == "plain" else 0, max_rects=launch_rects,
                        reserve_bytes=rb,
-                       prearm=prearm and depth == 1 and small_launch_rects > 0)
+                       prearm=prearm and depth <= 2 and small_launch_rects > 0)
         ids = {c.cid: srv.client(int(c.stream.ld[0]), int(c.stream.s[0]), col_range(c.stream), bin_x_bytes,
                                  bin_y_cols) for c in clients}
         nxt = {c.cid: 0 for c in clients}
