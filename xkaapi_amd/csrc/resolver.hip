@@ -204,28 +204,27 @@ __device__ __forceinline__ uint64_t mix64(uint64_t k)
 
 // ---------------------------------------------------------------- binning
 
-// The bins of record r: [bx0, bx1] x [by0, by1] of its domain's grid.  False
-// when the record names no domain slot, is empty, or leaves its domain's grid
-// (records may come from the caller's device memory): its domain is marked
+// The bins of record (q0, q1): [bx0, bx1] x [by0, by1] of its domain's grid.
+// False when the record names no domain slot, is empty, or leaves its domain's
+// grid (records may come from the caller's device memory): its domain is marked
 // ERR_RANGE and the record is dropped, so its domain touches no state in this
 // launch (k_resolve skips the bins of failed domains).
-__device__ __forceinline__ bool rec_bins(const KArgs &a, uint32_t r, uint4 &q0, uint4 &q1, DomDesc &g,
+__device__ __forceinline__ bool rec_bins(const KArgs &a, const uint4 &q0, const uint4 &q1, DomDesc &g,
                                          uint32_t &bx0, uint32_t &bx1, uint32_t &by0, uint32_t &by1)
 {
-    q0 = a.rec[2ull * r];
-    q1 = a.rec[2ull * r + 1];
     if (q1.z >= MAX_DOMS) {
         atomicOr(&a.ctr->err, ERR_RANGE);  // no domain to charge: the pool's launch is unsound
         return false;
     }
     g = a.dom[q1.z];
+    const uint32_t derr = a.dom_err[q1.z];  // read beside the descriptor: one round trip
     const bool ok = g.nbx != 0 && q0.x < q0.y && q0.y <= g.xmax && q0.z < q0.w && q0.z >= g.ybin0 &&
                     ((q0.w - 1 - g.ybin0) >> g.bys) < g.nby;
     if (!ok) {
         atomicOr(&a.dom_err[q1.z], ERR_RANGE);
         return false;
     }
-    if (a.dom_err[q1.z]) return false;  // a failed domain: its records are not counted
+    if (derr) return false;  // a failed domain: its records are not counted
     bx0 = q0.x >> g.bxs;
     bx1 = (q0.y - 1) >> g.bxs;
     by0 = (q0.z - g.ybin0) >> g.bys;
@@ -246,25 +245,43 @@ constexpr uint64_t KRPW_LARGE_FROM = 1u << 17;  // records of a launch from whic
 struct WaveRecs {
     uint32_t r, valid, bx0, w, by0, n, incl, bin0, nbx;
     uint32_t one;  // the record's access is this one record, in one bin
+    uint4 q0;      // the record, read once (a small launch reads it from host memory)
+    uint32_t acc, tm;
 };
 
-// records [0, nall) of the launch; this wave's are those below nrec
+// Lane j takes record r = (wave's first) + j; this wave's records are those below
+// nrec, the launch's below nall.  Every record is read once: its neighbours' access
+// ids (an access's records are adjacent: a wrapped tile has two, a segment up to
+// three) come from the adjacent lanes, and only the wave's two outer neighbours are
+// extra loads, issued beside the records' own.
 __device__ __forceinline__ WaveRecs wave_recs(const KArgs &a, uint32_t r, uint32_t nrec, uint32_t nall)
 {
     WaveRecs q;
     q.r = r;
     q.valid = r < nrec;
     q.bx0 = q.w = q.by0 = q.n = q.bin0 = q.nbx = q.one = 0;
+    uint4 q0 = make_uint4(0, 0, 0, 0), q1 = make_uint4(0, 0, 0, 0);
+    uint32_t pa = NONE, na = NONE;
+    const bool head = q.valid && lane_id() == 0, tail = q.valid && r + 1 == nrec;
     if (q.valid) {
-        uint4 q0, q1;
+        q0 = a.rec[2ull * r];
+        q1 = a.rec[2ull * r + 1];
+    }
+    if (head && r > 0) pa = a.rec[2ull * (r - 1) + 1].x;
+    if (tail && r + 1 < nall) na = a.rec[2ull * (r + 1) + 1].x;
+    const uint32_t up = __shfl(q1.x, (int)((lane_id() + 63) & 63), 64);
+    const uint32_t dn = __shfl(q1.x, (int)((lane_id() + 1) & 63), 64);
+    if (!head) pa = up;
+    if (!tail) na = dn;
+    q.q0 = q0;
+    q.acc = q1.x;
+    q.tm = q1.y;
+    if (q.valid) {
         DomDesc g;
         uint32_t bx0, bx1, by0, by1;
-        if (rec_bins(a, r, q0, q1, g, bx0, bx1, by0, by1)) {
+        if (rec_bins(a, q0, q1, g, bx0, bx1, by0, by1)) {
             q.bx0 = bx0; q.w = bx1 - bx0 + 1; q.by0 = by0; q.n = q.w * (by1 - by0 + 1);
             q.bin0 = g.bin0; q.nbx = g.nbx;
-            // an access's records are adjacent (a wrapped tile has two, a segment up to three)
-            const uint32_t pa = r > 0 ? a.rec[2ull * (r - 1) + 1].x : NONE;
-            const uint32_t na = r + 1 < nall ? a.rec[2ull * (r + 1) + 1].x : NONE;
             q.one = q.n == 1 && pa != q1.x && na != q1.x ? 1u : 0u;
         }
     }
@@ -337,18 +354,21 @@ __global__ void __launch_bounds__(256) k_count(KArgs a)
             uint32_t owner;
             const uint32_t bin = wave_rec_bin(q, min(k, total - 1), owner);
             const uint32_t one = __shfl(q.one, (int)owner, 64);
+            // the owner's record, from its lane
+            const uint4 q0 = make_uint4(__shfl(q.q0.x, (int)owner, 64), __shfl(q.q0.y, (int)owner, 64),
+                                        __shfl(q.q0.z, (int)owner, 64), __shfl(q.q0.w, (int)owner, 64));
+            const uint32_t acc = __shfl(q.acc, (int)owner, 64), tm = __shfl(q.tm, (int)owner, 64);
             bool first = false;
             if (k < total) {
                 // the incidence goes straight into its bin's bucket; a bin with more than lmax
                 // incidences fits no tier and fails in k_resolve
                 const uint32_t r = r0 + owner;
-                const uint4 q0 = a.rec[2ull * r], q1 = a.rec[2ull * r + 1];
                 const uint32_t pos = atomicAdd(&a.bin_cnt[bin], 1u);
                 first = pos == 0;
                 if (pos < a.lmax) {
                     uint4 *o = a.binrec + 2ull * ((size_t)bin * a.lmax + pos);
                     o[0] = q0;
-                    o[1] = make_uint4(q1.x, q1.y, r, one);
+                    o[1] = make_uint4(acc, tm, r, one);
                 }
             }
             // the bins this launch touches, listed once each
@@ -4209,6 +4229,7 @@ struct xkrt_hip_server {
     uint64_t edges_seen = 0;  // pool edge count after the last retired launch
     uint64_t harvested = 0;   // edges copied out
     bool hold = false;        // no launches until the pipeline drains (edge buffer reset)
+    bool arm_due = false;     // a launch went out: queue the next one behind it (prearm)
     hipStream_t hs = nullptr;
     uint32_t *he = nullptr, *hd = nullptr;  // pinned harvest buffers
     uint64_t hcap = 0;
@@ -4413,13 +4434,20 @@ int srv_launch(xkrt_hip_server *s, bool &work)
     s->flights.push_back({tk, std::move(batch), ghb});
     s->count(xkrt_hip_server::S_LAUNCHES);
     work = true;
-    // queue the next launch behind this one while the device works (with two launches in
-    // flight, once the staging buffer it takes is free)
-    if (s->cfg.prearm) {
-        const int rc = s->cfg.depth > 1 ? gate_try_arm(p) : gate_arm(p);
-        if (rc) return rc;
-    }
+    s->arm_due = s->cfg.prearm;
     return 0;
+}
+
+// Queues the next launch behind the one just started, while the device works (with two
+// launches in flight, once the staging buffer it takes is free).  Arming enqueues the whole
+// kernel chain, tens of microseconds of host time: the engine does it after the retired
+// batches are stamped ready, so that they do not wait for it.
+int srv_arm(xkrt_hip_server *s)
+{
+    if (!s->arm_due) return 0;
+    s->arm_due = false;
+    if (s->hold) return 0;  // the edge buffer is reset first
+    return s->cfg.depth > 1 ? gate_try_arm(s->p) : gate_arm(s->p);
 }
 
 // The edges of retired launches into their clients' host buffers.  Retired
@@ -4546,6 +4574,7 @@ void srv_loop(xkrt_hip_server *s)
         int rc = srv_retire(s, work);
         if (!rc) rc = srv_launch(s, work);
         if (!rc) rc = srv_complete(s);
+        if (!rc) rc = srv_arm(s);
         const double now = srv_us(s->t0);
         if (!rc) rc = srv_housekeep(s, work, now - last_work);
         if (rc) {
