@@ -545,9 +545,16 @@ int xkrt_hip_taskdom_edges_copy(xkrt_hip_taskdom_t t, uint64_t *edges, uint64_t 
  * are not there: xkrt_hip_taskdom_sync starts it).
  * xkrt_hip_taskdom_complete enqueues the completion of released tasks; a task
  * not released, or completed twice, fails the domain (-EINVAL at the next
- * sync).  Capacities: max_tasks, max_accesses, max_succ (-ENOSPC). */
+ * sync).  Capacities: max_tasks, max_accesses, max_succ (-ENOSPC).
+ * xkrt_hip_taskdom_complete_wait is the synchronous form, as __task_complete
+ * is on the host: it returns once the device has released the successors of
+ * these tasks (and of every completion enqueued before), so the next
+ * xkrt_hip_taskdom_ready sees them; it polls a word in pinned memory that the
+ * completion kernel writes, without synchronising the stream (-ETIMEDOUT
+ * after 10 s). */
 int xkrt_hip_taskdom_ready(xkrt_hip_taskdom_t t, uint32_t *tasks, uint64_t cap, uint64_t *n);
 int xkrt_hip_taskdom_complete(xkrt_hip_taskdom_t t, const uint32_t *tasks, uint64_t n);
+int xkrt_hip_taskdom_complete_wait(xkrt_hip_taskdom_t t, const uint32_t *tasks, uint64_t n);
 int xkrt_hip_taskdom_stats(xkrt_hip_taskdom_t t, xkrt_hip_taskdom_stats_t *out);
 
 /* ------------------------------------------------------------------------

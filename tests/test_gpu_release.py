@@ -33,16 +33,17 @@ def task_batches(task, k):
     return list(zip(bounds[:-1], bounds[1:]))
 
 
-def drive(p, e, seed, batch_tasks=40, pcomp=0.7):
+def drive(p, e, seed, batch_tasks=40, pcomp=0.7, wait=False):
     rng = np.random.default_rng(seed)
     task, mode = p["task"], p["mode"]
     acc = R.to_task_access_array(task, mode, p["kind"], p["addr"], p["m"], p["n"], p["ld"], p["s"])
     rel = REL.Release(task, mode)
     pool, steps = set(), 0
     with R.TaskDomain(**TD) as t:
-        def step(want):
+        def step(want, sync=True):
             nonlocal steps
-            t.sync()
+            if sync:
+                t.sync()
             got = t.ready()
             assert sorted(got.tolist()) == sorted(want), (seed, steps)
             pool.update(want)
@@ -51,8 +52,9 @@ def drive(p, e, seed, batch_tasks=40, pcomp=0.7):
         def completion():
             pick = sorted(rng.choice(sorted(pool), min(len(pool), int(rng.integers(1, 8))), replace=False).tolist())
             pool.difference_update(pick)
-            t.complete(pick)
-            step(rel.complete(pick))
+            t.complete(pick, wait=wait)
+            # a completion that waited has released its successors: no sync before reading them
+            step(rel.complete(pick), sync=not wait)
 
         for a, b in task_batches(task, batch_tasks):
             assert t.submit(acc[a:b]) == a
@@ -74,6 +76,15 @@ def test_release_matches_oracle(seed):
     ntask = int(p["task"].max()) + 1
     assert st["released"] == ntask and st["completed"] == ntask
     assert st["succ_nodes"] <= len(e) and steps > 100
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_waiting_completions_release_before_returning(seed):
+    p, e = program(10 + seed)
+    print(f"waiting completions seed {seed}: {int(p['task'].max()) + 1} tasks, {len(e)} edges", flush=True)
+    st, steps = drive(p, e, seed, wait=True)
+    ntask = int(p["task"].max()) + 1
+    assert st["released"] == ntask and st["completed"] == ntask and steps > 100
 
 
 def test_detached_accesses_hold_their_successors():
@@ -117,11 +128,11 @@ def test_cholesky_dag_runs_to_completion():
         for a, b in R.batches_by_tasks(st.task, 256):
             t.submit(acc[a:b])
         t.sync()
-        for _ in range(100000):
+        for k in range(100000):
             rdy = t.ready()
             if len(rdy):
                 order += rdy.tolist()
-                t.complete(rdy)
+                t.complete(rdy, wait=k % 2 == 1)  # both forms, interleaved
             else:
                 t.sync()
                 if len(order) == ntask and not len(t.ready()):

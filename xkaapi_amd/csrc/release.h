@@ -40,6 +40,10 @@ int apply(Engine *e, const uint32_t *edges, const uint64_t *count, const HandleE
 // completion of ready tasks (stream order)
 int complete(Engine *e, const uint32_t *tasks, uint64_t n);
 
+// the same, then waits until the device has released their successors (and
+// those of every completion enqueued before): they are in the queue on return
+int complete_wait(Engine *e, const uint32_t *tasks, uint64_t n);
+
 // released tasks not read yet, in release order (non-blocking)
 int ready(Engine *e, uint32_t *out, uint64_t cap, uint64_t *n);
 

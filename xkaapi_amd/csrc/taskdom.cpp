@@ -449,6 +449,13 @@ int xkrt_hip_taskdom_complete(xkrt_hip_taskdom_t t, const uint32_t *tasks, uint6
     return xkrt_release::complete(t->rel, tasks, n);
 }
 
+int xkrt_hip_taskdom_complete_wait(xkrt_hip_taskdom_t t, const uint32_t *tasks, uint64_t n)
+{
+    if (!t) return -EINVAL;
+    if (!t->rel) return -EPERM;
+    return xkrt_release::complete_wait(t->rel, tasks, n);
+}
+
 int xkrt_hip_taskdom_stats(xkrt_hip_taskdom_t t, xkrt_hip_taskdom_stats_t *out)
 {
     if (!t || !out) return -EINVAL;

@@ -210,6 +210,7 @@ def hip_lib():
                                                C.POINTER(_p)]
     L.xkrt_hip_handle_edges_consume.argtypes = [_p]
     L.xkrt_hip_taskdom_complete.argtypes = [_p, _p, _u64]
+    L.xkrt_hip_taskdom_complete_wait.argtypes = [_p, _p, _u64]
     L.xkrt_hip_taskdom_create.argtypes = [C.POINTER(TaskDomConfigT), _p, C.POINTER(_p)]
     L.xkrt_hip_taskdom_destroy.argtypes = [_p]
     L.xkrt_hip_taskdom_submit.argtypes = [_p, _p, _u64, C.POINTER(_u64)]
@@ -714,9 +715,16 @@ class TaskDomain:
         _chk(self.L.xkrt_hip_taskdom_ready(self.h, buf.ctypes.data, cap, C.byref(self._rn)), "xkrt_hip_taskdom_ready")
         return buf[:int(self._rn.value)].copy()
 
-    def complete(self, tasks) -> None:
+    def complete(self, tasks, wait: bool = False) -> None:
+        """Completes released tasks.  wait: return once their successors are
+        released (as __task_complete, src/task.cc:160-247), so the next ready()
+        sees them; otherwise the completion is only enqueued."""
         t = np.ascontiguousarray(tasks, np.uint32)
-        _chk(self.L.xkrt_hip_taskdom_complete(self.h, t.ctypes.data, len(t)), "xkrt_hip_taskdom_complete")
+        if wait:
+            _chk(self.L.xkrt_hip_taskdom_complete_wait(self.h, t.ctypes.data, len(t)),
+                 "xkrt_hip_taskdom_complete_wait")
+        else:
+            _chk(self.L.xkrt_hip_taskdom_complete(self.h, t.ctypes.data, len(t)), "xkrt_hip_taskdom_complete")
 
     def stats(self) -> dict:
         st = TaskDomStatsT()
