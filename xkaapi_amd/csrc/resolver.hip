@@ -66,7 +66,9 @@
 #include <cstddef>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -3287,38 +3289,119 @@ void remove_domain(xkrt_hip_dep_domain *d)
     delete d;
 }
 
-// host records (AoS) of n accesses into out[0 .. cap): rects in the domain
-// frame, access ids from first_id, launch-local task numbers from task0;
-// *ntask = tasks used
-int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t n, uint32_t first_id, uint32_t task0,
-                  uint64_t cap, xkrt_hip_rect_t *out, uint64_t *nrect, uint32_t *ylo, uint32_t *yhi, uint32_t *ntask,
-                  const uint32_t *ids = nullptr)
-{
-    // caller ids: increasing across the domain's batches (the lowest id of a task = its lowest slot)
-    if (ids)
-        for (uint64_t i = 0; i < n; ++i)
-            if ((i ? ids[i] <= ids[i - 1] : (d->any_id && ids[0] <= d->last_id))) return -EINVAL;
-    const uint64_t LDs = d->LDs, s = d->s;
-    // frame: fixed by the first access ever seen (or given at creation)
-    if (!d->framed && n) {
-        uint64_t ymin = UINT64_MAX;
-        for (uint64_t i = 0; i < n; ++i) ymin = std::min(ymin, acc[i].addr / LDs);
-        d->ybase = ymin > (1ull << 30) ? ymin - (1ull << 30) : 0;
-        d->framed = true;
+// A fixed set of host threads that build the records of large batches in
+// chunks (the calling thread takes chunks too).  Calls from several threads
+// take turns.  A worker reads the job only inside a generation it joined
+// under the lock, and a call sets up its job only once no worker is still
+// inside the previous one.
+class HostWorkers {
+  public:
+    static HostWorkers &get()
+    {
+        static HostWorkers w;
+        return w;
     }
-    uint64_t k = 0, lo = UINT64_MAX, hi = 0;
-    uint32_t tl = task0;
-    uint64_t maxx = d->max_xlen;
-    bool col0 = d->col0;
-    for (uint64_t i = 0; i < n; ++i) {
+    uint32_t threads() const { return (uint32_t)th_.size() + 1; }
+    // f(c) for c in [0, n): returns once every call has returned
+    template <typename F>
+    void run(uint32_t n, F &&f)
+    {
+        std::lock_guard<std::mutex> turn(call_mu_);
+        std::function<void(uint32_t)> job(std::forward<F>(f));
+        std::unique_lock<std::mutex> g(mu_);
+        done_.wait(g, [&] { return busy_ == 0; });
+        job_ = &job;
+        n_ = n;
+        next_.store(0, std::memory_order_relaxed);
+        left_ = n;
+        ++gen_;
+        g.unlock();
+        cv_.notify_all();
+        work();
+        g.lock();
+        done_.wait(g, [&] { return left_ == 0 && busy_ == 0; });
+        job_ = nullptr;
+    }
+
+  private:
+    HostWorkers()
+    {
+        uint32_t t = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        if (const char *e = getenv("XKRT_HIP_HOST_THREADS")) t = (uint32_t)std::max(1, std::min(64, atoi(e)));
+        for (uint32_t i = 1; i < t; ++i) th_.emplace_back([this] { loop(); });
+    }
+    ~HostWorkers()
+    {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void work()
+    {
+        uint32_t c;
+        while ((c = next_.fetch_add(1, std::memory_order_relaxed)) < n_) {
+            (*job_)(c);
+            std::lock_guard<std::mutex> g(mu_);
+            if (--left_ == 0) done_.notify_all();
+        }
+    }
+    void loop()
+    {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> g(mu_);
+        while (true) {
+            cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            ++busy_;
+            g.unlock();
+            work();
+            g.lock();
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_;
+    std::function<void(uint32_t)> *job_ = nullptr;
+    uint32_t n_ = 0, left_ = 0, busy_ = 0;
+    std::atomic<uint32_t> next_{0};
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+// What building the records of accesses [0, i) leaves for access i: records so far, the task
+// number, the widest rect in dim 0 and whether a rect started at column 0 (the two feed the
+// refusals below), and the column range.
+struct BuildState {
+    uint64_t k = 0;
+    uint32_t tl = 0;
+    uint64_t maxx = 0;
+    bool col0 = false;
+    uint64_t lo = UINT64_MAX, hi = 0;
+};
+
+// the records of accesses [i0, i1) of a batch from state st (out == nullptr: count only)
+int build_range(const xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t i0, uint64_t i1,
+                uint32_t first_id, uint64_t cap, xkrt_hip_rect_t *out, const uint32_t *ids, BuildState &st)
+{
+    const uint64_t LDs = d->LDs, s = d->s;
+    uint64_t k = st.k, lo = st.lo, hi = st.hi, maxx = st.maxx;
+    uint32_t tl = st.tl;
+    bool col0 = st.col0;
+    int rc = 0;
+    for (uint64_t i = i0; i < i1; ++i) {
         const xkrt_hip_access_t &q = acc[i];
         if (i && q.task != acc[i - 1].task) {
-            if (q.task < acc[i - 1].task) return -EINVAL;
+            if (q.task < acc[i - 1].task) { rc = -EINVAL; break; }
             tl++;
         }
-        if (q.m == 0 || (q.n == 0 && q.kind == XKRT_HIP_KIND_TILE) || q.kind > XKRT_HIP_KIND_SEGMENT) return -EINVAL;
-        if (d->segments && q.kind != XKRT_HIP_KIND_SEGMENT) return -EINVAL;
-        if (tl > 0xFFFFFFu) return -ERANGE;
+        if (q.m == 0 || (q.n == 0 && q.kind == XKRT_HIP_KIND_TILE) || q.kind > XKRT_HIP_KIND_SEGMENT) { rc = -EINVAL; break; }
+        if (d->segments && q.kind != XKRT_HIP_KIND_SEGMENT) { rc = -EINVAL; break; }
+        if (tl > 0xFFFFFFu) { rc = -ERANGE; break; }
         uint64_t r[3][4];
         int nr;
         uint32_t flag = 0;
@@ -3337,7 +3420,7 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
             // start at column 0 -- which exist only once some access touched the
             // first ld * sizeof_type bytes of memory.  Before that the PUT marks
             // nothing and the segment only links; after it, it is refused.
-            if (!d->segments && (q.addr < LDs || col0)) return -ENOTSUP;
+            if (!d->segments && (q.addr < LDs || col0)) { rc = -ENOTSUP; break; }
             const uint64_t size = q.m, x1 = std::min(X0 + size, LDs), dx10 = x1 - X0;
             const uint64_t x3 = (size - dx10) % LDs;
             const uint64_t y1 = Y0 + 1, y3 = Y0 + (size - dx10 - x3) / LDs + 2, y2 = y3 - 1;
@@ -3353,19 +3436,19 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
             r[0][0] = X0; r[0][1] = X0 + ms; r[0][2] = Y0; r[0][3] = Y0 + q.n;
             nr = 1;
         } else {
-            if (ms > LDs) return -EINVAL;
+            if (ms > LDs) { rc = -EINVAL; break; }
             r[0][0] = X0; r[0][1] = LDs; r[0][2] = Y0; r[0][3] = Y0 + q.n;
             r[1][0] = 0; r[1][1] = ms - (LDs - X0); r[1][2] = Y0 + 1; r[1][3] = Y0 + q.n + 1;
             nr = 2;
             // a prior rect spanning [x3 - 1, x0] in dim 0 makes the reference
             // mark split fragments beyond the tile
-            if (maxx > X0 - r[1][1]) return -ENOTSUP;
+            if (maxx > X0 - r[1][1]) { rc = -ENOTSUP; break; }
         }
-        for (int j = 0; j < nr; ++j) {
-            if (r[j][2] < d->ybase || r[j][3] - d->ybase > 0xFFFFFFFFull) return -ERANGE;
+        for (int j = 0; j < nr && !rc; ++j) {
+            if (r[j][2] < d->ybase || r[j][3] - d->ybase > 0xFFFFFFFFull) { rc = -ERANGE; break; }
             maxx = std::max(maxx, r[j][1] - r[j][0]);
             col0 = col0 || r[j][2] == 0;
-            if (k < cap) {
+            if (out && k < cap) {
                 xkrt_hip_rect_t &o = out[k];
                 o.x0 = (uint32_t)r[j][0]; o.x1 = (uint32_t)r[j][1];
                 o.y0 = (uint32_t)(r[j][2] - d->ybase); o.y1 = (uint32_t)(r[j][3] - d->ybase);
@@ -3378,17 +3461,89 @@ int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t
             hi = std::max(hi, r[j][3] - d->ybase);
             ++k;
         }
+        if (rc) break;
     }
-    d->max_xlen = maxx;
-    d->col0 = col0;
+    st.k = k; st.tl = tl; st.maxx = maxx; st.col0 = col0; st.lo = lo; st.hi = hi;
+    return rc;
+}
+
+// batches of at least this many accesses are built by the host workers, in chunks of at least
+// BUILD_CHUNK accesses
+constexpr uint64_t BUILD_PARALLEL_MIN = 1u << 16, BUILD_CHUNK = 1u << 14;
+
+// host records (AoS) of n accesses into out[0 .. cap): rects in the domain
+// frame, access ids from first_id, launch-local task numbers from task0;
+// *ntask = tasks used.  A large batch is built in chunks on several threads:
+// a first pass counts each chunk's records and task changes and takes its
+// widest rect and column-0 flag (all independent of what came before), then
+// each chunk is built from the state its predecessors leave.  A chunk that
+// fails in the first pass fails in the second as well (the refusals only get
+// likelier with more state behind them), and the batch's error is the one of
+// its earliest failing access, as a single pass would find.
+int build_records(xkrt_hip_dep_domain *d, const xkrt_hip_access_t *acc, uint64_t n, uint32_t first_id, uint32_t task0,
+                  uint64_t cap, xkrt_hip_rect_t *out, uint64_t *nrect, uint32_t *ylo, uint32_t *yhi, uint32_t *ntask,
+                  const uint32_t *ids = nullptr)
+{
+    // caller ids: increasing across the domain's batches (the lowest id of a task = its lowest slot)
+    if (ids)
+        for (uint64_t i = 0; i < n; ++i)
+            if ((i ? ids[i] <= ids[i - 1] : (d->any_id && ids[0] <= d->last_id))) return -EINVAL;
+    const uint64_t LDs = d->LDs;
+    // frame: fixed by the first access ever seen (or given at creation)
+    if (!d->framed && n) {
+        uint64_t ymin = UINT64_MAX;
+        for (uint64_t i = 0; i < n; ++i) ymin = std::min(ymin, acc[i].addr / LDs);
+        d->ybase = ymin > (1ull << 30) ? ymin - (1ull << 30) : 0;
+        d->framed = true;
+    }
+    BuildState st;
+    st.tl = task0;
+    st.maxx = d->max_xlen;
+    st.col0 = d->col0;
+    HostWorkers &hw = HostWorkers::get();
+    const uint64_t nch = n >= BUILD_PARALLEL_MIN && hw.threads() > 1
+                             ? std::min<uint64_t>(4ull * hw.threads(), n / BUILD_CHUNK) : 1;
+    if (nch <= 1) {
+        const int rc = build_range(d, acc, 0, n, first_id, cap, out, ids, st);
+        if (rc) return rc;
+    } else {
+        const uint64_t per = (n + nch - 1) / nch;
+        std::vector<BuildState> part(nch), fin(nch);
+        std::vector<int> rc1(nch, 0), rc2(nch, 0);
+        hw.run((uint32_t)nch, [&](uint32_t c) {
+            const uint64_t a = c * per, b = std::min(n, a + per);
+            rc1[c] = build_range(d, acc, a, b, first_id, 0, nullptr, ids, part[c]);
+        });
+        BuildState at = st;
+        for (uint64_t c = 0; c < nch; ++c) {
+            fin[c] = at;
+            at.k += part[c].k;
+            at.tl += part[c].tl;
+            at.maxx = std::max(at.maxx, part[c].maxx);
+            at.col0 = at.col0 || part[c].col0;
+        }
+        hw.run((uint32_t)nch, [&](uint32_t c) {
+            const uint64_t a = c * per, b = std::min(n, a + per);
+            rc2[c] = build_range(d, acc, a, b, first_id, cap, out, ids, fin[c]);
+        });
+        for (uint64_t c = 0; c < nch; ++c)
+            if (rc2[c]) return rc2[c];
+        st = fin[nch - 1];
+        for (uint64_t c = 0; c + 1 < nch; ++c) {
+            st.lo = std::min(st.lo, fin[c].lo);
+            st.hi = std::max(st.hi, fin[c].hi);
+        }
+    }
+    d->max_xlen = st.maxx;
+    d->col0 = st.col0;
     if (ids && n) {
         d->any_id = true;
         d->last_id = ids[n - 1];
     }
-    *nrect = k;
-    if (ylo) *ylo = n ? (uint32_t)lo : 0;
-    if (yhi) *yhi = n ? (uint32_t)hi : 0;
-    if (ntask) *ntask = n ? tl - task0 + 1 : 0;
+    *nrect = st.k;
+    if (ylo) *ylo = n ? (uint32_t)st.lo : 0;
+    if (yhi) *yhi = n ? (uint32_t)st.hi : 0;
+    if (ntask) *ntask = n ? st.tl - task0 + 1 : 0;
     return 0;
 }
 

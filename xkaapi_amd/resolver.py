@@ -332,10 +332,16 @@ class DependencyDomain:
             raise ResolverError(rc, "xkrt_hip_dep_sync", int(st.err))
         return int(n.value)
 
-    def edges(self) -> np.ndarray:
-        """Copy and consume every unconsumed edge: (E, 2) uint32 (pred id, succ id)."""
+    def edges(self, out: np.ndarray | None = None) -> np.ndarray:
+        """Copy and consume every unconsumed edge: (E, 2) uint32 (pred id, succ id).
+        out: a C-contiguous (cap, 2) uint32 buffer to copy into (pinned host memory
+        takes the copy at DMA speed); the result is its first E rows."""
         n = self.sync()
-        out = np.empty((max(n, 1), 2), np.uint32)
+        if out is None:
+            out = np.empty((max(n, 1), 2), np.uint32)
+        elif out.dtype != np.uint32 or out.ndim != 2 or out.shape[1] != 2 or not out.flags.c_contiguous \
+                or out.shape[0] < n:
+            raise ValueError(f"edges: out must be a C-contiguous (>= {n}, 2) uint32 array")
         got = _u64()
         _chk(self.L.xkrt_hip_dep_edges_copy(self.h, out.ctypes.data, n, C.byref(got)), "xkrt_hip_dep_edges_copy")
         return out[:n]
