@@ -137,7 +137,9 @@ __device__ uint64_t stripe_prefix(const unsigned long long *scount, uint32_t s, 
 }
 
 // The edges of a launch: blockIdx.y 0 takes the pool's buffer (count on the device), y = 1 + s
-// the handle map's stripe s.  f(edge) for each, grid-stride.
+// the handle map's stripe s.  f(edge, valid) for each wave-wide group of 64, grid-stride: every
+// lane of a wave calls f the same number of times (a lane past the end with valid false), so f
+// may use the wave's ballots.
 struct EdgeSrc {
     const uint2 *edges;
     const unsigned long long *count;
@@ -160,8 +162,11 @@ __device__ void for_each_edge(const EdgeSrc &src, F f)
         e = src.hedges + st * src.hcap;
         n = min((uint64_t)src.hcount[st], src.hcap);
     }
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        f(e[i]);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;  // a multiple of 64
+    for (uint64_t w0 = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~63u); w0 < n; w0 += stride) {
+        const uint64_t i = w0 + (threadIdx.x & 63u);
+        f(i < n ? e[i] : make_uint2(0, 0), i < n);
+    }
 }
 
 // the ends of an edge that links (its predecessor task has not completed); false otherwise
@@ -175,9 +180,14 @@ __device__ bool linking(const Dev &r, uint2 e, uint32_t &pa, uint32_t &ts)
     return r.state[tp] != ST_DONE;  // a completed predecessor adds no edge
 }
 
+__device__ __forceinline__ uint32_t lane_below(uint64_t m)
+{
+    return (uint32_t)__popcll(m & ((1ull << (threadIdx.x & 63u)) - 1ull));
+}
+
 // __task_precedes, pass 1: the successor's wait counter, and the successors per predecessor
-// access (the first edge of an access lists it).  Nodes for every edge must fit, or nothing of
-// the launch is applied.
+// access (the first edge of an access lists it; one list-counter update per wave).  Nodes for
+// every edge must fit, or nothing of the launch is applied.
 __global__ void k_apply_count(Dev r, EdgeSrc src)
 {
     const uint64_t n = *src.count + (src.hcount ? stripe_prefix(src.hcount, src.hstripes, src.hcap) : 0);
@@ -186,29 +196,54 @@ __global__ void k_apply_count(Dev r, EdgeSrc src)
         if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) atomicOr(&r.ctr[2], RERR_NODES);
         return;
     }
-    for_each_edge(src, [&](uint2 e) {
-        uint32_t pa, ts;
-        if (!linking(r, e, pa, ts)) return;
-        atomicAdd(&r.wc[ts], 1u);
-        if (atomicAdd(&r.deg[pa], 1u) == 0u) r.tl[atomicAdd(&r.ctr[7], 1u)] = pa;
+    for_each_edge(src, [&](uint2 e, bool valid) {
+        uint32_t pa = 0, ts;
+        bool first = false;
+        if (valid && linking(r, e, pa, ts)) {
+            atomicAdd(&r.wc[ts], 1u);
+            first = atomicAdd(&r.deg[pa], 1u) == 0u;
+        }
+        const uint64_t m = __ballot(first);
+        if (!m) return;
+        uint32_t b = 0;
+        if ((threadIdx.x & 63u) == 0) b = atomicAdd(&r.ctr[7], (uint32_t)__popcll(m));
+        b = __shfl(b, 0, 64);
+        if (first) r.tl[b + lane_below(m)] = pa;
     });
 }
 
-// pass 2: each listed access gets a segment of deg nodes at the head of its list
+// pass 2: each listed access gets a segment of deg nodes at the head of its list (the nodes and
+// segments of a wave's accesses taken with one update of each counter)
 __global__ void k_apply_link(Dev r)
 {
     if (r.ctr[2] & RERR_NODES) return;
-    const uint32_t m = r.ctr[7];
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x) {
-        const uint32_t pa = r.tl[j];
-        const uint32_t d = r.deg[pa];
-        const uint32_t base = atomicAdd(&r.ctr[0], d);
-        const uint32_t g = atomicAdd(&r.ctr[6], 1u);  // segments <= nodes: within max_nodes
-        r.seg[g] = make_uint4(base, d, r.head[pa], 0u);
-        r.head[pa] = g;
-        r.sbase[pa] = base;
-        r.fill[pa] = 0;
-        r.deg[pa] = 0;  // ready for the next launch
+    const uint32_t m = r.ctr[7], lane = threadIdx.x & 63u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t w0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); w0 < m; w0 += stride) {
+        const uint32_t j = w0 + lane;
+        const uint32_t pa = j < m ? r.tl[j] : 0u;
+        const uint32_t d = j < m ? r.deg[pa] : 0u;
+        uint32_t incl = d;  // inclusive prefix of the wave's node counts
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        const uint32_t tot = __shfl(incl, 63, 64), cnt = min(64u, m - w0);
+        uint32_t nb = 0, gb = 0;
+        if (lane == 0) {
+            nb = atomicAdd(&r.ctr[0], tot);
+            gb = atomicAdd(&r.ctr[6], cnt);  // segments <= nodes: within max_nodes
+        }
+        nb = __shfl(nb, 0, 64);
+        gb = __shfl(gb, 0, 64);
+        if (j < m) {
+            const uint32_t base = nb + incl - d, g = gb + lane;
+            r.seg[g] = make_uint4(base, d, r.head[pa], 0u);
+            r.head[pa] = g;
+            r.sbase[pa] = base;
+            r.fill[pa] = 0;
+            r.deg[pa] = 0;  // ready for the next launch
+        }
     }
 }
 
@@ -217,9 +252,9 @@ __global__ void k_apply_fill(Dev r, EdgeSrc src)
 {
     if (r.ctr[2] & RERR_NODES) return;
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) r.ctr[7] = 0;  // read by pass 2 only
-    for_each_edge(src, [&](uint2 e) {
+    for_each_edge(src, [&](uint2 e, bool valid) {
         uint32_t pa, ts;
-        if (!linking(r, e, pa, ts)) return;
+        if (!valid || !linking(r, e, pa, ts)) return;
         r.node[r.sbase[pa] + atomicAdd(&r.fill[pa], 1u)] = ts;
     });
 }
