@@ -61,6 +61,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cstddef>
@@ -3293,19 +3294,25 @@ void remove_domain(xkrt_hip_dep_domain *d)
 // chunks (the calling thread takes chunks too).  Calls from several threads
 // take turns.  A worker reads the job only inside a generation it joined
 // under the lock, and a call sets up its job only once no worker is still
-// inside the previous one.
+// inside the previous one.  The set lives until the process ends (never
+// destroyed: no join at exit); a forked child, which has none of the
+// threads, runs every chunk itself.
 class HostWorkers {
   public:
     static HostWorkers &get()
     {
-        static HostWorkers w;
-        return w;
+        static HostWorkers *w = new HostWorkers();
+        return *w;
     }
-    uint32_t threads() const { return (uint32_t)th_.size() + 1; }
+    uint32_t threads() const { return getpid() == pid_ ? (uint32_t)th_.size() + 1 : 1u; }
     // f(c) for c in [0, n): returns once every call has returned
     template <typename F>
     void run(uint32_t n, F &&f)
     {
+        if (getpid() != pid_) {
+            for (uint32_t c = 0; c < n; ++c) f(c);
+            return;
+        }
         std::lock_guard<std::mutex> turn(call_mu_);
         std::function<void(uint32_t)> job(std::forward<F>(f));
         std::unique_lock<std::mutex> g(mu_);
@@ -3324,20 +3331,11 @@ class HostWorkers {
     }
 
   private:
-    HostWorkers()
+    HostWorkers() : pid_(getpid())
     {
         uint32_t t = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         if (const char *e = getenv("XKRT_HIP_HOST_THREADS")) t = (uint32_t)std::max(1, std::min(64, atoi(e)));
         for (uint32_t i = 1; i < t; ++i) th_.emplace_back([this] { loop(); });
-    }
-    ~HostWorkers()
-    {
-        {
-            std::lock_guard<std::mutex> g(mu_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto &t : th_) t.join();
     }
     void work()
     {
@@ -3353,8 +3351,7 @@ class HostWorkers {
         uint64_t seen = 0;
         std::unique_lock<std::mutex> g(mu_);
         while (true) {
-            cv_.wait(g, [&] { return stop_ || gen_ != seen; });
-            if (stop_) return;
+            cv_.wait(g, [&] { return gen_ != seen; });
             seen = gen_;
             ++busy_;
             g.unlock();
@@ -3370,7 +3367,7 @@ class HostWorkers {
     uint32_t n_ = 0, left_ = 0, busy_ = 0;
     std::atomic<uint32_t> next_{0};
     uint64_t gen_ = 0;
-    bool stop_ = false;
+    const pid_t pid_;
 };
 
 // What building the records of accesses [0, i) leaves for access i: records so far, the task
