@@ -4308,15 +4308,16 @@ int xkrt_hip_dep_launch_times(xkrt_hip_dep_domain_t d, float *ms, uint64_t cap, 
 // ---------------------------------------------------------------- serving engine
 //
 // One host thread drives a pool for many clients.  The API threads only
-// append to an inbox under a mutex; every pool and HIP call happens on the
-// engine thread, which loops over: take the inbox; retire completed launches
-// (stamp their batches ready; what each launch's k_final wrote to pinned host
-// memory gives its edge count and error bits without a stream sync); stage every
-// arrived batch of admitted clients into one launch while fewer than `depth`
-// launches are in flight, admitting a client's domain at its first batch;
-// copy the edges of retired launches out per client (on a second stream,
-// while the device is busy or the loop is idle); reset the edge buffer when
-// it is half full and the pipeline is empty; release closed clients.
+// append to an inbox under a mutex; every pool call on the pool's stream
+// happens on the engine thread, which loops over: take the inbox; retire
+// completed launches (what each launch's k_final wrote to pinned host memory
+// gives its edge count and error bits without a stream sync) and hand them to
+// the finisher thread, which copies their edges out per client and stamps
+// their batches ready; stage every arrived batch of admitted clients into one
+// launch while fewer than `depth` launches are in flight, admitting a client's
+// domain at its first batch; arm the next launch; reset the edge buffer when
+// it is half full, the pipeline is empty and every edge is out; release closed
+// clients.
 
 namespace {
 
@@ -4334,15 +4335,16 @@ struct SrvClient {
     uint64_t ld = 0, s = 0, ylo = 0, yhi = 0;
     uint32_t bx = 0, by = 0;
     xkrt_hip_dep_domain *dom = nullptr;
-    int status = 0;         // < 0: a batch was refused; the client's later batches are refused too
+    std::atomic<int> status{0};  // < 0: a batch was refused; the client's later batches are refused too
     bool closing = false;   // its close arrived (after all of its batches)
     bool closed_api = false;  // close was called (API side, under the engine mutex)
     bool left = false;
     uint64_t round = 0;     // staging round in which it may not stage further batches
     uint64_t first_id = UINT64_MAX;  // domain access id of its first access
-    uint64_t nrecv = 0, ndone = 0;
+    uint64_t nrecv = 0;
+    std::atomic<uint64_t> ndone{0};  // batches stamped (by the finisher, or refused by the engine)
     uint64_t need = 0;      // pool edge count to harvest before it may leave
-    std::vector<uint32_t> edges;  // (pred, succ) in its own access ids; guarded by the engine mutex
+    std::vector<uint32_t> edges;  // (pred, succ) in its own access ids; guarded by the server's emu
 };
 
 struct SrvFlight {
@@ -4374,12 +4376,21 @@ struct xkrt_hip_server {
     // engine thread only
     std::deque<SrvRequest *> pending;
     std::deque<SrvFlight> flights;
-    std::vector<SrvFlight> retired;  // completed launches whose batches are stamped ready after their edges are out
     std::vector<SrvClient *> live;   // received a batch or a close, not left
     SrvClient *owner[MAX_DOMS] = {};
     uint64_t round = 0;
     uint64_t edges_seen = 0;  // pool edge count after the last retired launch
-    uint64_t harvested = 0;   // edges copied out
+    std::atomic<uint64_t> harvested{0};  // edges copied out (by the finisher)
+    // The finisher thread: the retired launches, each with the pool's edge count after it, go to
+    // it in order; it copies their edges into the clients' buffers and stamps their batches ready,
+    // while the engine thread stages and arms the next launches.
+    std::thread fin_th;
+    std::mutex emu;  // the clients' edge buffers and the owner table (apart from mu: submits never wait for a copy)
+    std::mutex fin_mu;
+    std::condition_variable fin_cv;
+    std::deque<std::pair<SrvFlight, uint64_t>> fin_q;  // under fin_mu
+    bool fin_stop = false;                             // under fin_mu: finish the queue, then end
+    std::atomic<uint64_t> fin_pending{0};              // launches queued or being finished
     bool hold = false;        // no launches until the pipeline drains (edge buffer reset)
     bool arm_due = false;     // a launch went out: queue the next one behind it (prearm)
     hipStream_t hs = nullptr;
@@ -4417,12 +4428,15 @@ int srv_admit(xkrt_hip_server *s, SrvClient *c)
     }
     d->fixed_frame = true;
     c->dom = d;
-    s->owner[d->idx] = c;
+    {
+        std::lock_guard<std::mutex> g(s->emu);  // the finisher reads the table
+        s->owner[d->idx] = c;
+    }
     s->count(xkrt_hip_server::S_ADMITTED);
     return 0;
 }
 
-int srv_harvest(xkrt_hip_server *s);
+int srv_harvest(xkrt_hip_server *s, uint64_t upto);
 
 int srv_retire(xkrt_hip_server *s, bool &work)
 {
@@ -4464,30 +4478,64 @@ int srv_retire(xkrt_hip_server *s, bool &work)
                 c->dom->err |= bits;
                 c->status = err_to_errno(bits);
             }
-            r->outcome = bits ? c->status : 0;  // published by srv_complete
+            r->outcome = bits ? c->status.load() : 0;  // published by the finisher
         }
-        s->retired.push_back(std::move(f));
+        s->fin_pending.fetch_add(1, std::memory_order_acq_rel);
+        {
+            std::lock_guard<std::mutex> g(s->fin_mu);
+            s->fin_q.emplace_back(std::move(f), total);
+        }
+        s->fin_cv.notify_one();
         s->flights.pop_front();
         work = true;
     }
     return 0;
 }
 
-// A batch is ready once its edges are in its client's host buffer: the edges of
-// the retired launches are copied out (after the next launch was enqueued, so
-// the device works meanwhile), then their batches are stamped.
-int srv_complete(xkrt_hip_server *s)
+// A batch is ready once its edges are in its client's host buffer.  The finisher
+// thread takes the retired launches in order: their edges are copied out, then
+// their batches are stamped -- while the engine thread stages and arms the next
+// launch.  After an engine error it stamps what it still holds with the error.
+void srv_finisher(xkrt_hip_server *s)
 {
-    if (s->retired.empty()) return 0;
-    const int rc = srv_harvest(s);
-    if (rc) return rc;
-    const double t = srv_us(s->t0);
-    for (SrvFlight &f : s->retired) {
-        for (SrvRequest *r : f.reqs) srv_finish(s, r, r->outcome, t);
-        s->count(xkrt_hip_server::S_BATCHES, f.reqs.size());
+    (void)hipSetDevice(s->p->device);
+    while (true) {
+        // poll for a while before sleeping: a wake-up through the condition variable costs tens
+        // of microseconds, which a batch would wait on the way to ready
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t k = 0; s->fin_pending.load(std::memory_order_acquire) == 0; ++k) {
+            if ((k & 63u) == 63u && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+            std::this_thread::yield();
+        }
+        std::pair<SrvFlight, uint64_t> job;
+        {
+            std::unique_lock<std::mutex> g(s->fin_mu);
+            s->fin_cv.wait(g, [&] { return s->fin_stop || !s->fin_q.empty(); });
+            if (s->fin_q.empty()) return;
+            job = std::move(s->fin_q.front());
+            s->fin_q.pop_front();
+        }
+        int rc = s->err.load(std::memory_order_acquire);
+        if (!rc) {
+            rc = srv_harvest(s, job.second);
+            if (rc) s->err.store(rc, std::memory_order_release);
+        }
+        const double t = srv_us(s->t0);
+        for (SrvRequest *r : job.first.reqs) srv_finish(s, r, rc ? rc : r->outcome, t);
+        s->count(xkrt_hip_server::S_BATCHES, job.first.reqs.size());
+        s->fin_pending.fetch_sub(1, std::memory_order_acq_rel);
     }
-    s->retired.clear();
-    return 0;
+}
+
+// ends the finisher once it has stamped every launch handed to it
+void srv_finisher_join(xkrt_hip_server *s)
+{
+    {
+        std::lock_guard<std::mutex> g(s->fin_mu);
+        s->fin_stop = true;
+    }
+    s->fin_cv.notify_all();
+    if (s->fin_th.joinable()) s->fin_th.join();
 }
 
 int srv_launch(xkrt_hip_server *s, bool &work)
@@ -4592,8 +4640,8 @@ int srv_launch(xkrt_hip_server *s, bool &work)
 
 // Queues the next launch behind the one just started, while the device works (with two
 // launches in flight, once the staging buffer it takes is free).  Arming enqueues the whole
-// kernel chain, tens of microseconds of host time: the engine does it after the retired
-// batches are stamped ready, so that they do not wait for it.
+// kernel chain, tens of microseconds of host time, while the finisher stamps the retired
+// batches.
 int srv_arm(xkrt_hip_server *s)
 {
     if (!s->arm_due) return 0;
@@ -4602,14 +4650,15 @@ int srv_arm(xkrt_hip_server *s)
     return s->cfg.depth > 1 ? gate_try_arm(s->p) : gate_arm(s->p);
 }
 
-// The edges of retired launches into their clients' host buffers.  Retired
-// launches never change these edges again; later launches append after them.
-int srv_harvest(xkrt_hip_server *s)
+// The edges of retired launches, up to the pool's edge count `upto`, into their
+// clients' host buffers (finisher thread).  Retired launches never change these
+// edges again; later launches append after them.
+int srv_harvest(xkrt_hip_server *s, uint64_t upto)
 {
     xkrt_hip_pool *p = s->p;
-    while (s->harvested < s->edges_seen) {
-        const uint64_t a = s->harvested;
-        uint64_t n = s->edges_seen - a;
+    while (s->harvested.load(std::memory_order_acquire) < upto) {
+        const uint64_t a = s->harvested.load(std::memory_order_relaxed);
+        uint64_t n = upto - a;
         const uint32_t *ee, *ed;
         if (p->edges_h) {  // k_final wrote them to host memory before its launch completed
             ee = p->edges_h + 2 * a;
@@ -4625,7 +4674,7 @@ int srv_harvest(xkrt_hip_server *s)
             ed = s->hd;
         }
         {
-            std::lock_guard<std::mutex> g(s->mu);
+            std::lock_guard<std::mutex> g(s->emu);
             for (uint64_t i = 0; i < n; ++i) {
                 SrvClient *c = s->owner[ed[i] & (MAX_DOMS - 1)];
                 if (!c) return -EIO;
@@ -4635,7 +4684,7 @@ int srv_harvest(xkrt_hip_server *s)
                 c->edges.push_back(ee[2 * i + 1] - f);
             }
         }
-        s->harvested = a + n;
+        s->harvested.store(a + n, std::memory_order_release);
         s->count(xkrt_hip_server::S_HARVESTS);
     }
     return 0;
@@ -4647,22 +4696,16 @@ int srv_housekeep(xkrt_hip_server *s, bool &work, double idle_us)
     bool leaving = false;
     for (SrvClient *c : s->live)
         if (c->closing && c->ndone == c->nrecv) leaving = true;
-    const bool busy = s->flights.size() >= s->cfg.depth;
-    // idle for a while: arrivals are sparse, a copy now delays none
-    const bool idle = s->flights.empty() && s->pending.empty() && idle_us > 100.0;
-    if (s->edges_seen > s->harvested &&
-        (busy || idle || leaving || s->hold || s->edges_seen - s->harvested > p->edge_cap / 4)) {
-        const int rc = srv_harvest(s);
-        if (rc) return rc;
-        work = true;
-    }
-    // reset the half-used edge buffer once the pipeline has drained and every edge is out
-    if (s->hold && s->flights.empty() && s->harvested == s->edges_seen) {
+    // reset the half-used edge buffer once the pipeline has drained and the finisher has
+    // copied every edge out and is idle
+    if (s->hold && s->flights.empty() && s->fin_pending.load(std::memory_order_acquire) == 0 &&
+        s->harvested.load(std::memory_order_acquire) == s->edges_seen) {
         const int rc = gate_cancel(p);  // the reset must not wait behind a gated launch
         if (rc) return rc;
         hipLaunchKernelGGL(k_consume, dim3(1), dim3(64), 0, p->stream, p->ctr);
         HIPCHK(hipGetLastError());
-        s->edges_seen = s->harvested = 0;
+        s->edges_seen = 0;
+        s->harvested.store(0, std::memory_order_release);
         for (SrvClient *c : s->live) c->need = 0;
         s->hold = false;
         s->count(xkrt_hip_server::S_CONSUMES);
@@ -4684,7 +4727,10 @@ int srv_housekeep(xkrt_hip_server *s, bool &work, double idle_us)
         SrvClient *c = s->live[i];
         if (c->closing && c->ndone == c->nrecv && c->need <= s->harvested) {
             if (c->dom) {
-                s->owner[c->dom->idx] = nullptr;
+                {
+                    std::lock_guard<std::mutex> g(s->emu);  // the finisher reads the table
+                    s->owner[c->dom->idx] = nullptr;
+                }
                 remove_domain(c->dom);  // stream order keeps later launches off its old bins
                 c->dom = nullptr;
             }
@@ -4704,6 +4750,7 @@ int srv_housekeep(xkrt_hip_server *s, bool &work, double idle_us)
 void srv_loop(xkrt_hip_server *s)
 {
     (void)hipSetDevice(s->p->device);
+    s->fin_th = std::thread(srv_finisher, s);
     std::vector<std::pair<SrvRequest *, SrvClient *>> in;
     double last_work = srv_us(s->t0);
     while (true) {
@@ -4725,21 +4772,20 @@ void srv_loop(xkrt_hip_server *s)
         in.clear();
         int rc = srv_retire(s, work);
         if (!rc) rc = srv_launch(s, work);
-        if (!rc) rc = srv_complete(s);
         if (!rc) rc = srv_arm(s);
         const double now = srv_us(s->t0);
         if (!rc) rc = srv_housekeep(s, work, now - last_work);
+        if (!rc) rc = s->err.load(std::memory_order_acquire);  // the finisher's
         if (rc) {
             (void)gate_cancel(s->p);
             s->err.store(rc, std::memory_order_release);
-            // refuse everything not resolved, so that drains end
+            // refuse everything not resolved, so that drains end (the finisher stamps what it
+            // holds with the error)
+            srv_finisher_join(s);
             const double t = srv_us(s->t0);
             for (SrvRequest *r : s->pending) srv_finish(s, r, rc, t);
             for (auto &f : s->flights)
                 for (SrvRequest *r : f.reqs) srv_finish(s, r, rc, t);
-            for (auto &f : s->retired)
-                for (SrvRequest *r : f.reqs) srv_finish(s, r, rc, t);
-            s->retired.clear();
             s->pending.clear();
             s->flights.clear();
             break;
@@ -4751,6 +4797,7 @@ void srv_loop(xkrt_hip_server *s)
         if (s->stop.load(std::memory_order_acquire) && s->flights.empty()) {
             const int rc2 = gate_cancel(s->p);
             if (rc2) s->err.store(rc2, std::memory_order_release);
+            srv_finisher_join(s);
             break;
         }
         // poll: an arrival is seen within microseconds; after 2 ms without work, back off
@@ -4925,6 +4972,7 @@ int xkrt_hip_server_edges(xkrt_hip_server_t s, uint32_t client, uint32_t *edges,
     if (!s || !n || (cap && !edges)) return -EINVAL;
     std::lock_guard<std::mutex> g(s->mu);
     if (client >= s->clients.size()) return -EINVAL;
+    std::lock_guard<std::mutex> ge(s->emu);  // order: mu, then emu
     const auto &v = s->clients[client]->edges;
     *n = v.size() / 2;
     const uint64_t m = std::min<uint64_t>(*n, cap);
