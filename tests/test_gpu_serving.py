@@ -63,11 +63,14 @@ def test_launch_cap_mix(clients):
     assert capped.launches > 1
 
 
-def test_open_loop_mix(clients):
-    sat = V.saturation(clients, "batched", max_rects=1 << 16)
+@pytest.mark.parametrize("driver", ["native", "python"])
+def test_open_loop_mix(clients, driver):
+    # arrivals submitted on schedule from C (the default) or from Python
+    sat = V.saturation(clients, "batched", max_rects=1 << 16, driver=driver)
     rate = 0.5 * sum(len(c.batches) for c in clients) / sat.makespan_s
-    res = V.serve(clients, V.poisson_schedule(clients, rate, seed=3), "batched", max_rects=1 << 16)
+    res = V.serve(clients, V.poisson_schedule(clients, rate, seed=3), "batched", max_rects=1 << 16, driver=driver)
     check(res, clients)
+    assert (res.lag_us >= 0).all()
 
 
 def test_admission_limited_mix(clients):

@@ -16,9 +16,11 @@
  * With any output array NULL it only counts and returns the number of
  * accesses.  Otherwise it writes min(count, cap) records and returns count.
  */
+#define _POSIX_C_SOURCE 199309L  /* clock_gettime */
 #include <math.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <time.h>
 
 #define MODE_R  1u
 #define MODE_W  2u
@@ -315,4 +317,39 @@ void xks_matrix_to_rects(uint64_t count, const uint64_t *addr, const uint64_t *l
             nrects[i] = 2;
         }
     }
+}
+
+/*
+ * Open-loop submission for the serving workload: batch i is handed to
+ * submit(srv, client[i], acc[i], nacc[i], &req[i]) at t_s[i] seconds after the
+ * call starts (a spin on CLOCK_MONOTONIC), and close_client(srv, client[i])
+ * follows when last[i] is set.  lag_s[i] = how late the submit call started.
+ * The server's entry points come in as function pointers, so this file links
+ * against nothing.  Returns 0, or the first nonzero code of a call (the loop
+ * stops there).
+ */
+typedef int (*xks_submit_fn)(void *srv, uint32_t client, const void *acc, uint64_t n, uint64_t *request);
+typedef int (*xks_close_fn)(void *srv, uint32_t client);
+
+static double xks_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+int xks_open_loop(void *srv, xks_submit_fn submit, xks_close_fn close_client, uint64_t n, const double *t_s,
+                  const uint32_t *client, const void *const *acc, const uint64_t *nacc, const uint8_t *last,
+                  double *lag_s, uint64_t *req)
+{
+    const double t0 = xks_now();
+    for (uint64_t i = 0; i < n; ++i) {
+        double now = xks_now() - t0;
+        while (now < t_s[i]) now = xks_now() - t0;
+        lag_s[i] = now - t_s[i];
+        int rc = submit(srv, client[i], acc[i], nacc[i], &req[i]);
+        if (rc) return rc;
+        if (last[i] && (rc = close_client(srv, client[i])) != 0) return rc;
+    }
+    return 0;
 }

@@ -614,6 +614,30 @@ class Server:
     def close_client(self, client: int):
         _chk(self.L.xkrt_hip_server_close(self.h, client), "xkrt_hip_server_close")
 
+    def open_loop(self, t_s, clients, batches, last) -> np.ndarray:
+        """Submits batch i (an ACCESS_DTYPE array) of client clients[i] at t_s[i] seconds from
+        the call, and closes the client after it when last[i], from native code (no Python
+        between submissions); returns how late each submit started, in seconds."""
+        from .streams import _lib as streams_lib
+        n = len(t_s)
+        t = np.ascontiguousarray(t_s, np.float64)
+        cl = np.ascontiguousarray(clients, np.uint32)
+        bs = [np.ascontiguousarray(b, ACCESS_DTYPE) for b in batches]
+        ptrs = (C.c_void_p * max(n, 1))(*[b.ctypes.data for b in bs])
+        cnt = np.array([len(b) for b in bs], np.uint64)
+        lst = np.ascontiguousarray(last, np.uint8)
+        lag = np.zeros(max(n, 1), np.float64)
+        req = np.zeros(max(n, 1), np.uint64)
+        lib = streams_lib()
+        rc = lib.xks_open_loop(self.h, C.cast(self.L.xkrt_hip_server_submit, C.c_void_p),
+                               C.cast(self.L.xkrt_hip_server_close, C.c_void_p), n, t.ctypes.data,
+                               cl.ctypes.data, ptrs, cnt.ctypes.data, lst.ctypes.data, lag.ctypes.data,
+                               req.ctypes.data)
+        _chk(rc, "xks_open_loop")
+        if n:
+            self.nreq = int(req[n - 1]) + 1
+        return lag[:n]
+
     def drain(self, timeout_s: float = 0.0):
         _chk(self.L.xkrt_hip_server_drain(self.h, int(timeout_s * 1e6)), "xkrt_hip_server_drain")
 

@@ -126,7 +126,7 @@ def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30
           graph: bool = True, depth: int = 1, max_rects: int = 1 << 20, bin_x_bytes: int = 1024,
           bin_y_cols: int = 128, reserve: bool = True, max_edges: int = 0,
           direct_dedup: bool = False, small_launch_rects: int = 16384, prearm: bool = True,
-          launch_rects: int = 32768, host_edges: bool = True) -> RunResult:
+          launch_rects: int = 32768, host_edges: bool = True, driver: str = "native") -> RunResult:
     """Run one schedule through a fresh pool and serving engine; mode "batched" or "plain".
     Defaults (measured on the 64-client workload): the small LDS class (0), one
     launch in flight (a batch that arrives during a launch goes into the next
@@ -138,7 +138,10 @@ def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30
     reserve: allocate the bins of every client up front (within the budget), so
     admissions do not allocate device memory while serving.
     host_edges: the pool's edge buffer lives in pinned host memory (4M edges unless
-    max_edges says otherwise), so a launch's edges are on the host when it completes."""
+    max_edges says otherwise), so a launch's edges are on the host when it completes.
+    driver: "native" submits on schedule from C (libxkstreams' open loop, no Python between
+    submissions); "python" from this thread.  Either way a submission's lateness behind its
+    arrival time is added to its latency."""
     byid = {c.cid: c for c in clients}
     nacc = sum(len(c.acc) for c in clients)
     if host_edges and not max_edges:
@@ -156,20 +159,32 @@ def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30
                                  bin_y_cols) for c in clients}
         nxt = {c.cid: 0 for c in clients}
         meta, lag = [], np.zeros(len(schedule))
-        clock = time.perf_counter
-        t0 = clock()
-        for i, (t, cid) in enumerate(schedule):
-            while clock() - t0 < t:  # open loop: submit at the arrival time
-                pass
-            lag[i] = clock() - t0 - t
-            c = byid[cid]
-            b = nxt[cid]
-            nxt[cid] = b + 1
-            a, e = c.batches[b]
-            srv.submit(ids[cid], c.acc[a:e])
-            meta.append((cid, b))
-            if b + 1 == len(c.batches):
-                srv.close_client(ids[cid])
+        if driver == "native":
+            sub, last = [], []
+            for t, cid in schedule:
+                c = byid[cid]
+                b = nxt[cid]
+                nxt[cid] = b + 1
+                a, e = c.batches[b]
+                sub.append(c.acc[a:e])
+                meta.append((cid, b))
+                last.append(b + 1 == len(c.batches))
+            lag = srv.open_loop([t for t, _ in schedule], [ids[cid] for _, cid in schedule], sub, last)
+        else:
+            clock = time.perf_counter
+            t0 = clock()
+            for i, (t, cid) in enumerate(schedule):
+                while clock() - t0 < t:  # open loop: submit at the arrival time
+                    pass
+                lag[i] = clock() - t0 - t
+                c = byid[cid]
+                b = nxt[cid]
+                nxt[cid] = b + 1
+                a, e = c.batches[b]
+                srv.submit(ids[cid], c.acc[a:e])
+                meta.append((cid, b))
+                if b + 1 == len(c.batches):
+                    srv.close_client(ids[cid])
         srv.drain(timeout_s=600)
         ts, tr, st = srv.times()
         if (st != 0).any():

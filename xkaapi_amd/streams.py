@@ -40,6 +40,8 @@ def _lib():
         lib.xks_cholesky.restype = _u64
         lib.xks_lu.argtypes = [_u64, _u64, _u64, _u64, C.c_uint32] + sig
         lib.xks_lu.restype = _u64
+        lib.xks_open_loop.argtypes = [_p, _p, _p, _u64, _p, _p, _p, _p, _p, _p, _p]
+        lib.xks_open_loop.restype = C.c_int
         lib.xks_random2d.argtypes = [_u64, _u64, _u64, _u64, _u64, _u64, C.c_double, C.c_double, _u64] + sig
         lib.xks_random2d.restype = _u64
         lib.xks_heat.argtypes = [_u64, _u64, _u64, _u64, _u64, C.c_uint32] + sig
