@@ -124,11 +124,12 @@ def test_refused_client_fails_alone(clients):
 @pytest.mark.parametrize("mode", ["batched", "plain"])
 def test_prearmed_launches_mix(clients, mode):
     # the next launch waits on the device behind a gate the engine opens by writing pinned memory
-    sat = V.saturation(clients, mode, max_rects=1 << 16, prearm=True, small_launch_rects=4096)
+    # (one launch in flight; two below)
+    sat = V.saturation(clients, mode, depth=1, max_rects=1 << 16, prearm=True, small_launch_rects=4096)
     check(sat, clients)
     rate = 0.3 * sum(len(c.batches) for c in clients) / sat.makespan_s
-    res = V.serve(clients, V.poisson_schedule(clients, rate, seed=5), mode, max_rects=1 << 16, prearm=True,
-                  small_launch_rects=4096)
+    res = V.serve(clients, V.poisson_schedule(clients, rate, seed=5), mode, depth=1, max_rects=1 << 16,
+                  prearm=True, small_launch_rects=4096)
     check(res, clients)
     assert res.stats["gated"] > 0 and res.stats["error"] == 0
 
