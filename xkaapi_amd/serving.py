@@ -125,16 +125,18 @@ def col_range(st):
 def serve(clients, schedule, mode: str = "batched", budget_bytes: int = 16 << 30, kernel: int = 0,
           graph: bool = True, depth: int = 2, max_rects: int = 1 << 20, bin_x_bytes: int = 1024,
           bin_y_cols: int = 128, reserve: bool = True, max_edges: int = 0,
-          direct_dedup: bool = False, small_launch_rects: int = 16384, prearm: bool = True,
-          launch_rects: int = 32768, host_edges: bool = True, driver: str = "native") -> RunResult:
+          direct_dedup: bool = False, small_launch_rects: int = 32768, prearm: bool = True,
+          launch_rects: int = 65536, host_edges: bool = True, driver: str = "native") -> RunResult:
     """Run one schedule through a fresh pool and serving engine; mode "batched" or "plain".
     Defaults (measured on the 64-client workload): the small LDS class (0), two
     launches in flight (the gate of the launch after next armed once a staging
     buffer is free; at 90% load p99 ~700 us against 1.5-1.9 ms with one), staged dedup, launches
-    of up to 16384 rects read in place from pinned host memory, the next
-    launch queued on the device behind a gate (prearm), and at most 32768 rects
-    per launch (launch_rects; 0 = no cap).  With no cap a backlog went out in a
-    few huge launches, which resolve more slowly per access.
+    of up to 32768 rects read in place from pinned host memory, the next
+    launch queued on the device behind a gate (prearm), and at most 65536 rects
+    per launch (launch_rects; 0 = no cap).  At 67 M acc/s offered, gated launches
+    of up to 32768 rects gave p50 ~180 / p99 ~950 us against ~330 / ~1,300 with
+    16384 (a backlog beyond a gated launch cancels the gate and goes out as a
+    launch with an upload); 65536 or no cap on top measured the same.
     reserve: allocate the bins of every client up front (within the budget), so
     admissions do not allocate device memory while serving.
     host_edges: the pool's edge buffer lives in pinned host memory (4M edges unless
