@@ -368,9 +368,10 @@ int xkrt_hip_dep_index(xkrt_hip_dep_domain_t d, uint32_t *idx);
  * batch (clients wait while the budget is full), stages every arrived batch
  * of admitted clients into one launch whenever fewer than `depth` launches
  * are in flight (continuous batching; max_batches = 1 gives the plain
- * per-batch form), retires launches as they complete, copies each launch's
- * edges out per client, and releases a closed client's bins once its last
- * batch is resolved.  Submit-to-ready times are stamped by the engine.
+ * per-batch form), retires launches as they complete, and releases a closed
+ * client's bins once its last batch is resolved; a second host thread copies
+ * each retired launch's edges out per client and stamps its batches ready
+ * (submit times are stamped at submit).
  * While an engine runs, the pool's own stage / launch / edge calls return
  * -EBUSY.  No counterpart in the reference (its resolver runs inline in the
  * spawning thread, src/task/task.cc:285-299).
